@@ -1,0 +1,939 @@
+// api.cpp -- the C ABI of libdbie: handles, device residency, operator apply and
+// the GMRES / mixed-precision refinement drivers.
+//
+// Host code owns control flow only; every O(N) or larger step runs on the GPU.
+// Per Krylov iteration the host reads back the (j+2) x K Hessenberg column to
+// update the per-column Givens rotations (a few KB), as Krylov.jl does on the CPU.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace dbie {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+[[noreturn]] void fail(dbie_status code, const std::string &msg) { throw Error{code, msg}; }
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) fail(DBIE_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+std::once_flag g_init_flag;
+dbie_status g_init_status = DBIE_OK;
+std::string g_init_msg;
+
+void ensure_device() {
+    std::call_once(g_init_flag, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            g_init_status = DBIE_NO_DEVICE;
+            g_init_msg = "no HIP device visible";
+            return;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+            g_init_status = DBIE_NO_DEVICE;
+            g_init_msg = "hipGetDeviceProperties failed";
+            return;
+        }
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            g_init_status = DBIE_NO_DEVICE;
+            g_init_msg = std::string("libdbie is built for gfx950 only; device is ") + prop.gcnArchName;
+        }
+    });
+    if (g_init_status != DBIE_OK) fail(g_init_status, g_init_msg);
+}
+
+// Device buffer with RAII.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        release();
+        if (hipMalloc(&p, b) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            fail(DBIE_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed");
+        }
+        bytes = b;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <class T>
+void upload(DBuf &b, const T *src, size_t n, hipStream_t st) {
+    b.ensure(std::max<size_t>(n, 1) * sizeof(T));
+    if (n) DBIE_HIP(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+}  // namespace
+
+}  // namespace dbie
+
+using namespace dbie;
+
+struct dbie_op {
+    Geometry g;                  // host copy of the interface
+    std::vector<double> diag_h;  // t per node
+    std::vector<double> eps_in, eps_out;
+    hipStream_t stream = nullptr;
+    // resident device data
+    DBuf x, y, z, nx, ny, nz, w, xf, yf, zf, nxf, nyf, nzf, wf, plane, diag;
+    DBuf bsr_row, bsr_col, bsr_v64, bsr_v32;
+    DevGeom dg;
+    DevBSR bsr;
+    int adaptive_hits = 0;
+    // workspaces (grown on demand)
+    DBuf part, xin, yout, tmp;
+    ~dbie_op() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int64_t N() const { return g.N; }
+};
+
+namespace dbie {
+namespace {
+
+template <class F>
+dbie_status guard(F &&f) {
+    try {
+        f();
+        return DBIE_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return DBIE_OOM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return DBIE_BAD_ARG;
+    }
+}
+
+void check_op(const dbie_op *op) {
+    if (!op) fail(DBIE_BAD_ARG, "operator handle is NULL");
+}
+
+// Plane id per 256-node tile: nodes whose panels share one plane (normal and
+// n.corner offset equal) get the same id; tiles spanning two planes get -1.
+std::vector<int> tile_planes(const Geometry &g) {
+    const int npp = g.npp();
+    std::vector<int> pid(g.P);
+    std::vector<std::array<double, 4>> planes;
+    for (int64_t p = 0; p < g.P; ++p) {
+        const double *n = &g.normal[3 * p], *c = &g.corners[12 * p];
+        const std::array<double, 4> key{n[0], n[1], n[2], n[0] * c[0] + n[1] * c[1] + n[2] * c[2]};
+        int id = -1;
+        for (size_t q = 0; q < planes.size(); ++q)
+            if (planes[q] == key) { id = (int)q; break; }
+        if (id < 0) {
+            if (planes.size() >= 4096) { id = -1; }
+            else { planes.push_back(key); id = (int)planes.size() - 1; }
+        }
+        pid[p] = id;
+    }
+    const int64_t nt = (g.N + 255) / 256;
+    std::vector<int> tp(nt);
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t n0 = t * 256, n1 = std::min<int64_t>(g.N, n0 + 256);
+        int id = pid[n0 / npp];
+        for (int64_t n = n0; n < n1 && id >= 0; ++n)
+            if (pid[n / npp] != id) id = -1;
+        tp[t] = id;
+    }
+    return tp;
+}
+
+void compute_diag(dbie_op *op, const double *eps_in, const double *eps_out) {
+    const Geometry &g = op->g;
+    const int npp = g.npp();
+    op->eps_in.assign(eps_in, eps_in + g.P);
+    op->eps_out.assign(eps_out, eps_out + g.P);
+    op->diag_h.resize(g.N);
+    for (int64_t p = 0; p < g.P; ++p) {
+        const double t = 0.5 * (eps_out[p] + eps_in[p]) / (eps_out[p] - eps_in[p]);
+        for (int j = 0; j < npp; ++j) op->diag_h[p * npp + j] = t;
+    }
+    upload(op->diag, op->diag_h.data(), g.N, op->stream);
+}
+
+// Y = A X on the device (X, Y: N x K with leading dims).
+void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
+    const int64_t N = op->N();
+    const int64_t need = far_field_part_elems(N, K);
+    op->part.ensure(need * sizeof(double));
+    int S;
+    if ((prec == DBIE_BF16 || prec == DBIE_BF16X3 || prec == DBIE_FP64) && far_field_mfma_supported((int)prec, K))
+        S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need);
+    else if (prec == DBIE_FP64)
+        S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
+    else
+        S = far_field_f32(op->stream, op->dg, K, X, ldx, op->part.as<double>(), need);
+    near_combine(op->stream, op->bsr, prec != DBIE_FP64, S, op->part.as<double>(), N, K, X, ldx,
+                 op->diag.as<double>(), Y, ldy);
+    DBIE_HIP(hipGetLastError());
+}
+
+// Column 2-norms of an N x K block (contiguous), returned on the host.
+void col_norms(dbie_op *op, const double *V, int64_t N, int K, DBuf &partial, DBuf &h, std::vector<double> &out) {
+    partial.ensure(dots_partial_elems(N, K, 1) * sizeof(double));
+    h.ensure(sizeof(double) * K);
+    block_dots(op->stream, V, 1, V, N, K, partial.as<double>(), h.as<double>());
+    out.resize(K);
+    DBIE_HIP(hipMemcpyAsync(out.data(), h.p, sizeof(double) * K, hipMemcpyDeviceToHost, op->stream));
+    DBIE_HIP(hipStreamSynchronize(op->stream));
+    for (auto &v : out) v = std::sqrt(std::max(v, 0.0));
+}
+
+struct GmresOut {
+    int iters = 0;
+    bool solved = false;
+};
+
+// GMRES(m) for K independent columns sharing each block matvec.  Right
+// preconditioner: diagonal dinv (dielectric_diagonal_scaling).  X receives the
+// solution (initial guess zero).  tol_k = atol + rtol * ||B_k||.
+GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol, int itmax,
+               int restart, const double *dinv) {
+    const int64_t N = op->N();
+    const size_t NK = (size_t)N * K;
+    const int m = std::max(1, std::min(restart > 0 ? restart : 100, std::max(itmax, 1)));
+    DBuf V, W, Z, U, partial, hbuf, nbuf;
+    V.ensure(sizeof(double) * NK * (m + 1));
+    W.ensure(sizeof(double) * NK);
+    U.ensure(sizeof(double) * NK);
+    if (dinv) Z.ensure(sizeof(double) * NK);
+    partial.ensure(sizeof(double) * std::max(dots_partial_elems(N, K, m + 1), dots_partial_elems(N, K, 1)));
+    hbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
+    nbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
+    hipStream_t st = op->stream;
+    double *Vp = V.as<double>(), *Wp = W.as<double>();
+
+    DBIE_HIP(hipMemsetAsync(X, 0, sizeof(double) * NK, st));
+    // r0 = B (x0 = 0) lives in W
+    DBIE_HIP(hipMemcpyAsync(Wp, B, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
+    std::vector<double> beta, tol(K), H((size_t)(m + 1) * m * K), cs((size_t)m * K), sn((size_t)m * K),
+        gv((size_t)(m + 1) * K), hh((size_t)(m + 1) * K), yv((size_t)m * K);
+    std::vector<int> nconv(K);
+    std::vector<char> active(K), conv(K);
+    GmresOut res;
+    bool first = true;
+    for (;;) {
+        col_norms(op, Wp, N, K, partial, nbuf, beta);
+        bool any = false;
+        for (int k = 0; k < K; ++k) {
+            if (first) {
+                if (!std::isfinite(beta[k])) fail(DBIE_BAD_ARG, "right-hand side contains NaN or Inf");
+                tol[k] = atol + rtol * beta[k];
+            }
+            active[k] = beta[k] > tol[k] && beta[k] > 0.0;
+            conv[k] = !active[k];
+            nconv[k] = 0;
+            any = any || active[k];
+        }
+        first = false;
+        if (!any) { res.solved = true; break; }
+        if (res.iters >= itmax) break;
+        // V0 = r / beta
+        DBIE_HIP(hipMemcpyAsync(nbuf.p, beta.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+        col_scale(st, Wp, nbuf.as<double>(), true, Vp, N, K);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        for (int k = 0; k < K; ++k) gv[(size_t)k * (m + 1)] = active[k] ? beta[k] : 0.0;
+        int j = 0;
+        for (; j < m && res.iters < itmax; ++j) {
+            const double *Vj = Vp + (size_t)j * NK;
+            const double *Zin = Vj;
+            if (dinv) {
+                row_scale(st, dinv, Vj, Z.as<double>(), N, K);
+                Zin = Z.as<double>();
+            }
+            apply_dev(op, K, Zin, N, Wp, N, prec);
+            // classical Gram-Schmidt, twice (CGS2)
+            block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), hbuf.as<double>());
+            block_subtract(st, Vp, j + 1, hbuf.as<double>(), Wp, N, K);
+            block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), nbuf.as<double>());
+            block_subtract(st, Vp, j + 1, nbuf.as<double>(), Wp, N, K);
+            std::vector<double> h1((size_t)(j + 1) * K), h2((size_t)(j + 1) * K);
+            DBIE_HIP(hipMemcpyAsync(h1.data(), hbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
+            DBIE_HIP(hipMemcpyAsync(h2.data(), nbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
+            std::vector<double> hn;
+            col_norms(op, Wp, N, K, partial, nbuf, hn);  // synchronizes the stream
+            DBIE_HIP(hipMemcpyAsync(nbuf.p, hn.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+            col_scale(st, Wp, nbuf.as<double>(), true, Vp + (size_t)(j + 1) * NK, N, K);
+            ++res.iters;
+            bool all = true;
+            for (int k = 0; k < K; ++k) {
+                if (conv[k]) continue;
+                double *Hk = &H[(size_t)k * (m + 1) * m];
+                double *col = Hk + (size_t)j * (m + 1);
+                for (int i = 0; i <= j; ++i) col[i] = h1[(size_t)i * K + k] + h2[(size_t)i * K + k];
+                col[j + 1] = hn[k];
+                double *c = &cs[(size_t)k * m], *s = &sn[(size_t)k * m], *g = &gv[(size_t)k * (m + 1)];
+                for (int i = 0; i < j; ++i) {
+                    const double t = c[i] * col[i] + s[i] * col[i + 1];
+                    col[i + 1] = -s[i] * col[i] + c[i] * col[i + 1];
+                    col[i] = t;
+                }
+                const double d = std::hypot(col[j], col[j + 1]);
+                c[j] = d != 0.0 ? col[j] / d : 1.0;
+                s[j] = d != 0.0 ? col[j + 1] / d : 0.0;
+                col[j] = d;
+                col[j + 1] = 0.0;
+                g[j + 1] = -s[j] * g[j];
+                g[j] = c[j] * g[j];
+                nconv[k] = j + 1;
+                if (std::fabs(g[j + 1]) <= tol[k] || hn[k] == 0.0) conv[k] = 1;
+                else all = false;
+            }
+            if (all) { ++j; break; }
+        }
+        // y_k = H_k^{-1} g_k (upper triangular, size nconv[k])
+        std::fill(yv.begin(), yv.end(), 0.0);
+        for (int k = 0; k < K; ++k) {
+            if (!active[k]) continue;
+            const int nk = nconv[k];
+            const double *Hk = &H[(size_t)k * (m + 1) * m];
+            const double *g = &gv[(size_t)k * (m + 1)];
+            std::vector<double> y(nk);
+            for (int i = nk - 1; i >= 0; --i) {
+                double s = g[i];
+                for (int l = i + 1; l < nk; ++l) s -= Hk[(size_t)l * (m + 1) + i] * y[l];
+                y[i] = s / Hk[(size_t)i * (m + 1) + i];
+            }
+            for (int i = 0; i < nk; ++i) yv[(size_t)i * K + k] = y[i];
+        }
+        int nmax = 0;
+        for (int k = 0; k < K; ++k) nmax = std::max(nmax, nconv[k]);
+        DBIE_HIP(hipMemcpyAsync(hbuf.p, yv.data(), sizeof(double) * (size_t)std::max(nmax, 1) * K,
+                                hipMemcpyHostToDevice, st));
+        DBIE_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * NK, st));
+        block_combine(st, Vp, nmax, hbuf.as<double>(), U.as<double>(), N, K);
+        if (dinv) row_scale(st, dinv, U.as<double>(), U.as<double>(), N, K);
+        add_to(st, U.as<double>(), X, N, K);
+        bool all_conv = true;
+        for (int k = 0; k < K; ++k) all_conv = all_conv && conv[k];
+        if (all_conv) { res.solved = true; break; }
+        if (res.iters >= itmax) break;
+        // restart: r = B - A X
+        apply_dev(op, K, X, N, Wp, N, DBIE_FP64 == prec ? DBIE_FP64 : prec);
+        sub_from(st, B, Wp, N, K);
+    }
+    DBIE_HIP(hipStreamSynchronize(st));
+    return res;
+}
+
+}  // namespace
+}  // namespace dbie
+
+// ============================== C ABI ========================================
+extern "C" {
+
+const char *dbie_version(void) { return "dbie-mi355x 0.1 (gfx950)"; }
+
+dbie_status dbie_last_error(char *buf, size_t len) {
+    if (!buf || len == 0) return DBIE_BAD_ARG;
+    std::strncpy(buf, g_last_error.c_str(), len - 1);
+    buf[len - 1] = '\0';
+    return DBIE_OK;
+}
+
+dbie_status dbie_init(int device) {
+    return guard([&] {
+        ensure_device();
+        DBIE_HIP(hipSetDevice(device));
+    });
+}
+
+void dbie_near_opts_default(dbie_near_opts *o) {
+    if (!o) return;
+    o->up_tol = 1e-9;
+    o->max_order = 8;
+    o->correct_edges = 1;
+    o->adaptive_atol = 1e-9;
+    o->adaptive_rtol = 1.4901161193847656e-8;
+    o->adaptive_n_GL = 0;
+    o->adaptive_max_depth = 20;
+    o->build_corrections = 1;
+    o->n_threads = 0;
+}
+
+void dbie_solve_opts_default(dbie_solve_opts *o) {
+    if (!o) return;
+    o->rtol = 1e-10;
+    o->atol = 0.0;
+    o->itmax = 500;
+    o->restart = 0;
+    o->precondition = 1;
+    o->precision = DBIE_FP64;
+    o->inner_rtol = 1e-4;
+    o->max_refine = 10;
+    o->check_residual = 1;
+}
+
+dbie_status dbie_gauss_legendre(int n, double *x, double *w) {
+    return guard([&] {
+        if (!x || !w) fail(DBIE_BAD_ARG, "NULL output");
+        gauss_legendre(n, x, w);
+    });
+}
+
+dbie_status dbie_best_grid(double W, double H, double alpha, int *m, int *n) {
+    return guard([&] {
+        if (!m || !n) fail(DBIE_BAD_ARG, "NULL output");
+        if (!(W > 0 && H > 0)) fail(DBIE_BAD_ARG, "W and H must be positive");
+        if (alpha < 1) fail(DBIE_BAD_ARG, "alpha must be >= 1");
+        if (!best_grid(W, H, alpha, m, n)) fail(DBIE_BAD_ARG, "no (m,n) found within search limits");
+    });
+}
+
+dbie_status dbie_geometry_box(double Lx, double Ly, double Lz, int n_quad, double l_ec, double alpha,
+                              dbie_geom **out) {
+    return guard([&] {
+        if (!out) fail(DBIE_BAD_ARG, "NULL output");
+        std::unique_ptr<Geometry> g(make_box(Lx, Ly, Lz, n_quad, l_ec, alpha));
+        auto *h = new dbie_geom;
+        h->g = std::move(*g);
+        *out = h;
+    });
+}
+
+dbie_status dbie_geometry_from_panels(int64_t P, int n_quad, const double *corners, const double *normals,
+                                      const int *is_edge, dbie_geom **out) {
+    return guard([&] {
+        if (!out) fail(DBIE_BAD_ARG, "NULL output");
+        std::unique_ptr<Geometry> g(make_from_panels(P, n_quad, corners, normals, is_edge));
+        auto *h = new dbie_geom;
+        h->g = std::move(*g);
+        *out = h;
+    });
+}
+
+dbie_status dbie_geom_sizes(const dbie_geom *g, int64_t *P, int64_t *N, int *n_quad) {
+    return guard([&] {
+        if (!g) fail(DBIE_BAD_ARG, "geometry handle is NULL");
+        if (P) *P = g->g.P;
+        if (N) *N = g->g.N;
+        if (n_quad) *n_quad = g->g.nq;
+    });
+}
+
+dbie_status dbie_geom_nodes(const dbie_geom *gh, double *out) {
+    return guard([&] {
+        if (!gh || !out) fail(DBIE_BAD_ARG, "NULL argument");
+        const Geometry &g = gh->g;
+        const int64_t N = g.N;
+        const int npp = g.npp();
+        std::memcpy(out, g.x.data(), sizeof(double) * N);
+        std::memcpy(out + N, g.y.data(), sizeof(double) * N);
+        std::memcpy(out + 2 * N, g.z.data(), sizeof(double) * N);
+        for (int64_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) out[(3 + k) * N + i] = g.normal[3 * (i / npp) + k];
+        std::memcpy(out + 6 * N, g.w.data(), sizeof(double) * N);
+    });
+}
+
+dbie_status dbie_geom_panels(const dbie_geom *gh, double *corners, double *normals, int *is_edge) {
+    return guard([&] {
+        if (!gh) fail(DBIE_BAD_ARG, "NULL argument");
+        const Geometry &g = gh->g;
+        if (corners) std::memcpy(corners, g.corners.data(), sizeof(double) * 12 * g.P);
+        if (normals) std::memcpy(normals, g.normal.data(), sizeof(double) * 3 * g.P);
+        if (is_edge) std::memcpy(is_edge, g.is_edge.data(), sizeof(int) * g.P);
+    });
+}
+
+void dbie_geom_destroy(dbie_geom *g) { delete g; }
+
+dbie_status dbie_neighbor_list(const dbie_geom *gh, int max_order, double atol, int correct_edges, int64_t *n_up,
+                               int64_t *n_adp, int *up_src, int *up_trg, int *up_order, int *adp_src, int *adp_trg) {
+    return guard([&] {
+        if (!gh || !n_up || !n_adp) fail(DBIE_BAD_ARG, "NULL argument");
+        if (!(atol > 0 && atol < 1)) fail(DBIE_BAD_ARG, "atol must be in (0, 1)");
+        if (max_order < 1 || max_order > 16) fail(DBIE_BAD_ARG, "max_order must be in 1..16");
+        const PairList pl = build_pair_list(gh->g, max_order, atol, correct_edges != 0, 0);
+        *n_up = (int64_t)pl.up_src.size();
+        *n_adp = (int64_t)pl.adp_src.size();
+        if (up_src) std::copy(pl.up_src.begin(), pl.up_src.end(), up_src);
+        if (up_trg) std::copy(pl.up_trg.begin(), pl.up_trg.end(), up_trg);
+        if (up_order) std::copy(pl.up_order.begin(), pl.up_order.end(), up_order);
+        if (adp_src) std::copy(pl.adp_src.begin(), pl.adp_src.end(), adp_src);
+        if (adp_trg) std::copy(pl.adp_trg.begin(), pl.adp_trg.end(), adp_trg);
+    });
+}
+
+dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, const double *eps_out,
+                                 const dbie_near_opts *opts, dbie_op **out) {
+    return guard([&] {
+        if (!gh || !eps_in || !eps_out || !out) fail(DBIE_BAD_ARG, "NULL argument");
+        ensure_device();
+        dbie_near_opts o;
+        if (opts) o = *opts;
+        else dbie_near_opts_default(&o);
+        if (o.max_order < 1 || o.max_order > 16) fail(DBIE_BAD_ARG, "max_order must be in 1..16");
+        if (!(o.up_tol > 0 && o.up_tol < 1)) fail(DBIE_BAD_ARG, "up_tol must be in (0, 1)");
+        std::unique_ptr<dbie_op> op(new dbie_op);
+        op->g = gh->g;
+        const Geometry &g = op->g;
+        DBIE_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+        hipStream_t st = op->stream;
+        const int64_t N = g.N;
+        // node streams
+        std::vector<double> nrm[3];
+        for (int k = 0; k < 3; ++k) nrm[k].resize(N);
+        for (int64_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) nrm[k][i] = g.normal[3 * (i / g.npp()) + k];
+        upload(op->x, g.x.data(), N, st);
+        upload(op->y, g.y.data(), N, st);
+        upload(op->z, g.z.data(), N, st);
+        upload(op->nx, nrm[0].data(), N, st);
+        upload(op->ny, nrm[1].data(), N, st);
+        upload(op->nz, nrm[2].data(), N, st);
+        upload(op->w, g.w.data(), N, st);
+        auto f32 = [&](const std::vector<double> &v) {
+            std::vector<float> r(v.size());
+            for (size_t i = 0; i < v.size(); ++i) r[i] = (float)v[i];
+            return r;
+        };
+        const std::vector<float> fx = f32(g.x), fy = f32(g.y), fz = f32(g.z), fnx = f32(nrm[0]), fny = f32(nrm[1]),
+                                 fnz = f32(nrm[2]), fw = f32(g.w);
+        upload(op->xf, fx.data(), N, st);
+        upload(op->yf, fy.data(), N, st);
+        upload(op->zf, fz.data(), N, st);
+        upload(op->nxf, fnx.data(), N, st);
+        upload(op->nyf, fny.data(), N, st);
+        upload(op->nzf, fnz.data(), N, st);
+        upload(op->wf, fw.data(), N, st);
+        const std::vector<int> tp = tile_planes(g);
+        upload(op->plane, tp.data(), tp.size(), st);
+        // near-field corrections
+        BlockSparse bs;
+        bs.bd = g.npp();
+        if (o.build_corrections) {
+            const PairList pl = build_pair_list(g, o.max_order, o.up_tol, o.correct_edges != 0, o.n_threads);
+            NearOpts no;
+            no.up_tol = o.up_tol;
+            no.max_order = o.max_order;
+            no.correct_edges = o.correct_edges != 0;
+            no.adaptive_atol = o.adaptive_atol;
+            no.adaptive_rtol = o.adaptive_rtol;
+            no.adaptive_n_GL = o.adaptive_n_GL;
+            no.adaptive_max_depth = o.adaptive_max_depth;
+            no.n_threads = o.n_threads;
+            bs = build_corrections(g, pl, no);
+        } else {
+            bs.row_ptr.assign(g.P + 1, 0);
+        }
+        op->adaptive_hits = bs.adaptive_hits;
+        upload(op->bsr_row, bs.row_ptr.data(), bs.row_ptr.size(), st);
+        upload(op->bsr_col, bs.col.data(), bs.col.size(), st);
+        upload(op->bsr_v64, bs.val.data(), bs.val.size(), st);
+        {
+            const std::vector<float> v32 = f32(bs.val);
+            upload(op->bsr_v32, v32.data(), v32.size(), st);
+            DBIE_HIP(hipStreamSynchronize(st));
+        }
+        compute_diag(op.get(), eps_in, eps_out);
+        DevGeom &d = op->dg;
+        d.N = N;
+        d.x = op->x.as<double>(); d.y = op->y.as<double>(); d.z = op->z.as<double>();
+        d.nx = op->nx.as<double>(); d.ny = op->ny.as<double>(); d.nz = op->nz.as<double>();
+        d.w = op->w.as<double>();
+        d.xf = op->xf.as<float>(); d.yf = op->yf.as<float>(); d.zf = op->zf.as<float>();
+        d.nxf = op->nxf.as<float>(); d.nyf = op->nyf.as<float>(); d.nzf = op->nzf.as<float>();
+        d.wf = op->wf.as<float>();
+        d.tile_plane = op->plane.as<int>();
+        DevBSR &c = op->bsr;
+        c.P = (int)g.P;
+        c.bd = bs.bd;
+        c.nb = bs.nblocks();
+        c.row_ptr = op->bsr_row.as<int>();
+        c.col = op->bsr_col.as<int>();
+        c.val64 = op->bsr_v64.as<double>();
+        c.val32 = op->bsr_v32.as<float>();
+        DBIE_HIP(hipStreamSynchronize(st));
+        *out = op.release();
+    });
+}
+
+void dbie_operator_destroy(dbie_op *op) { delete op; }
+
+dbie_status dbie_op_sizes(const dbie_op *op, int64_t *N, int64_t *n_blocks, int *block_dim) {
+    return guard([&] {
+        check_op(op);
+        if (N) *N = op->N();
+        if (n_blocks) *n_blocks = op->bsr.nb;
+        if (block_dim) *block_dim = op->bsr.bd;
+    });
+}
+
+dbie_status dbie_op_set_eps(dbie_op *op, const double *eps_in, const double *eps_out) {
+    return guard([&] {
+        check_op(op);
+        if (!eps_in || !eps_out) fail(DBIE_BAD_ARG, "NULL eps");
+        compute_diag(op, eps_in, eps_out);
+        DBIE_HIP(hipStreamSynchronize(op->stream));
+    });
+}
+
+dbie_status dbie_op_corrections(const dbie_op *op, int *src, int *trg, double *blocks) {
+    return guard([&] {
+        check_op(op);
+        const DevBSR &c = op->bsr;
+        std::vector<int> rp(c.P + 1), col(c.nb);
+        DBIE_HIP(hipMemcpy(rp.data(), c.row_ptr, sizeof(int) * (c.P + 1), hipMemcpyDeviceToHost));
+        if (c.nb) DBIE_HIP(hipMemcpy(col.data(), c.col, sizeof(int) * c.nb, hipMemcpyDeviceToHost));
+        for (int p = 0; p < c.P; ++p)
+            for (int b = rp[p]; b < rp[p + 1]; ++b) {
+                if (src) src[b] = col[b];
+                if (trg) trg[b] = p;
+            }
+        if (blocks && c.nb)
+            DBIE_HIP(hipMemcpy(blocks, c.val64, sizeof(double) * c.nb * c.bd * c.bd, hipMemcpyDeviceToHost));
+    });
+}
+
+dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
+    return guard([&] {
+        check_op(op);
+        if (!bytes) fail(DBIE_BAD_ARG, "NULL output");
+        int64_t b = 0;
+        for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
+                              &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
+                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->part, &op->xin, &op->yout, &op->tmp})
+            b += (int64_t)d->bytes;
+        *bytes = b;
+    });
+}
+
+void *dbie_op_stream(dbie_op *op) { return op ? (void *)op->stream : nullptr; }
+
+dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
+                              dbie_precision precision) {
+    return guard([&] {
+        check_op(op);
+        if (K < 1 || !dX || !dY) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
+        if (ldx < op->N() || ldy < op->N()) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        apply_dev(op, K, dX, ldx, dY, ldy, precision);
+    });
+}
+
+dbie_status dbie_apply(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy,
+                       dbie_precision precision) {
+    return guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (K < 1 || !X || !Y) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
+        if (ldx < N || ldy < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        hipStream_t st = op->stream;
+        op->xin.ensure(sizeof(double) * N * K);
+        op->yout.ensure(sizeof(double) * N * K);
+        DBIE_HIP(hipMemcpy2DAsync(op->xin.p, sizeof(double) * N, X, sizeof(double) * ldx, sizeof(double) * N, K,
+                                  hipMemcpyHostToDevice, st));
+        apply_dev(op, K, op->xin.as<double>(), N, op->yout.as<double>(), N, precision);
+        DBIE_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldy, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
+                                  hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+    });
+}
+
+dbie_status dbie_apply_kernel(dbie_op *op, dbie_kernel kernel, int K, const double *X, int64_t ldx, double *Y,
+                              int64_t ldy) {
+    return guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (K < 1 || !X || !Y) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
+        if (ldx < N || ldy < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        if (kernel < DBIE_OP_DT || kernel > DBIE_OP_D) fail(DBIE_BAD_ARG, "unknown kernel");
+        hipStream_t st = op->stream;
+        op->xin.ensure(sizeof(double) * N * K);
+        op->yout.ensure(sizeof(double) * N * K);
+        const int64_t need = far_field_part_elems(N, K);
+        op->part.ensure(need * sizeof(double));
+        DBIE_HIP(hipMemcpy2DAsync(op->xin.p, sizeof(double) * N, X, sizeof(double) * ldx, sizeof(double) * N, K,
+                                  hipMemcpyHostToDevice, st));
+        // the plain operator: far-field partial sums reduced with no corrections and zero diag
+        const int S = far_field_f64(st, op->dg, (FarMode)kernel, K, op->xin.as<double>(), N, op->part.as<double>(), need);
+        op->tmp.ensure(sizeof(double) * N);
+        DBIE_HIP(hipMemsetAsync(op->tmp.p, 0, sizeof(double) * N, st));
+        DevBSR empty = op->bsr;
+        std::vector<int> zeros(op->g.P + 1, 0);
+        DBuf zrow;
+        upload(zrow, zeros.data(), zeros.size(), st);
+        empty.row_ptr = zrow.as<int>();
+        near_combine(st, empty, false, S, op->part.as<double>(), N, K, op->xin.as<double>(), N, op->tmp.as<double>(),
+                     op->yout.as<double>(), N);
+        DBIE_HIP(hipGetLastError());
+        DBIE_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldy, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
+                                  hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+    });
+}
+
+dbie_status dbie_lhs_dense(dbie_op *op, double *A, int64_t lda) {
+    return guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (!A) fail(DBIE_BAD_ARG, "NULL output");
+        if (lda < N) fail(DBIE_DIM_MISMATCH, "lda < N");
+        if (N > 65535) fail(DBIE_BAD_ARG, "dense assembly is limited to N <= 65535");
+        DBuf dA;
+        dA.ensure(sizeof(double) * N * N);
+        dense_lhs(op->stream, op->dg, op->diag.as<double>(), dA.as<double>(), N);
+        DBIE_HIP(hipGetLastError());
+        DBIE_HIP(hipMemcpy2DAsync(A, sizeof(double) * lda, dA.p, sizeof(double) * N, sizeof(double) * N, N,
+                                  hipMemcpyDeviceToHost, op->stream));
+        DBIE_HIP(hipStreamSynchronize(op->stream));
+    });
+}
+
+dbie_status dbie_solve_lu(int64_t n, const double *A, int64_t lda, int nrhs, double *B, int64_t ldb) {
+    return guard([&] {
+        ensure_device();
+        if (n < 1 || nrhs < 1 || !A || !B) fail(DBIE_BAD_ARG, "n, nrhs >= 1 and non-NULL buffers required");
+        if (lda < n || ldb < n) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than n");
+        if (n > 2147483647) fail(DBIE_BAD_ARG, "n too large");
+        rocblas_handle h;
+        if (rocblas_create_handle(&h) != rocblas_status_success) fail(DBIE_HIP_ERROR, "rocblas_create_handle failed");
+        std::unique_ptr<_rocblas_handle, decltype(&rocblas_destroy_handle)> hg(h, rocblas_destroy_handle);
+        DBuf dA, dB, dP, dI;
+        dA.ensure(sizeof(double) * n * n);
+        dB.ensure(sizeof(double) * n * nrhs);
+        dP.ensure(sizeof(rocblas_int) * n);
+        dI.ensure(sizeof(rocblas_int));
+        DBIE_HIP(hipMemcpy2D(dA.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                             hipMemcpyHostToDevice));
+        DBIE_HIP(hipMemcpy2D(dB.p, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                             hipMemcpyHostToDevice));
+        if (rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, dA.as<double>(), (rocblas_int)n, dP.as<rocblas_int>(),
+                             dI.as<rocblas_int>()) != rocblas_status_success)
+            fail(DBIE_HIP_ERROR, "rocsolver_dgetrf failed");
+        rocblas_int info = 0;
+        DBIE_HIP(hipMemcpy(&info, dI.p, sizeof(info), hipMemcpyDeviceToHost));
+        if (info != 0) fail(DBIE_BAD_ARG, "matrix is singular (zero pivot at " + std::to_string(info) + ")");
+        if (rocsolver_dgetrs(h, rocblas_operation_none, (rocblas_int)n, nrhs, dA.as<double>(), (rocblas_int)n,
+                             dP.as<rocblas_int>(), dB.as<double>(), (rocblas_int)n) != rocblas_status_success)
+            fail(DBIE_HIP_ERROR, "rocsolver_dgetrs failed");
+        DBIE_HIP(hipMemcpy2D(B, sizeof(double) * ldb, dB.p, sizeof(double) * n, sizeof(double) * n, nrhs,
+                             hipMemcpyDeviceToHost));
+    });
+}
+
+dbie_status dbie_rhs_points(dbie_op *op, int K, const double *src_xyz, const double *q, double eps_src, double *F,
+                            int64_t ldf) {
+    return guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (K < 1 || !src_xyz || !q || !F) fail(DBIE_BAD_ARG, "K >= 1 and non-NULL buffers required");
+        if (ldf < N) fail(DBIE_DIM_MISMATCH, "ldf < N");
+        if (eps_src == 0.0) fail(DBIE_BAD_ARG, "eps_src must be nonzero");
+        hipStream_t st = op->stream;
+        DBuf ds, dq;
+        upload(ds, src_xyz, 3 * (size_t)K, st);
+        upload(dq, q, K, st);
+        op->yout.ensure(sizeof(double) * N * K);
+        rhs_points(st, op->dg, K, ds.as<double>(), dq.as<double>(), eps_src, op->yout.as<double>(), N);
+        DBIE_HIP(hipGetLastError());
+        DBIE_HIP(hipMemcpy2DAsync(F, sizeof(double) * ldf, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
+                                  hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+    });
+}
+
+dbie_status dbie_rhs_volume(dbie_op *op, int K, const int64_t *col_ptr, const double *xyz, const double *w,
+                            const double *rho, int screening, double eps_src, double *F, int64_t ldf) {
+    return guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (K < 1 || !col_ptr || !F) fail(DBIE_BAD_ARG, "K >= 1, col_ptr and F required");
+        if (ldf < N) fail(DBIE_DIM_MISMATCH, "ldf < N");
+        const int64_t S = col_ptr[K];
+        if (col_ptr[0] != 0 || S < 0) fail(DBIE_BAD_ARG, "col_ptr must start at 0");
+        for (int k = 0; k < K; ++k)
+            if (col_ptr[k + 1] < col_ptr[k]) fail(DBIE_BAD_ARG, "col_ptr must be non-decreasing");
+        if (S > 0 && (!xyz || !w || !rho)) fail(DBIE_BAD_ARG, "source arrays required");
+        if (!screening && eps_src == 0.0) fail(DBIE_BAD_ARG, "eps_src must be nonzero");
+        hipStream_t st = op->stream;
+        DBuf dcp, dpos, dw, drho, dwr;
+        upload(dcp, col_ptr, (size_t)K + 1, st);
+        upload(dpos, xyz, 3 * (size_t)S, st);
+        upload(dw, w, (size_t)S, st);
+        upload(drho, rho, (size_t)S, st);
+        dwr.ensure(sizeof(double) * std::max<int64_t>(S, 1));
+        double scale = 1.0;
+        if (screening) {
+            const Geometry &g = op->g;
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int64_t p = 0; p < g.P; ++p)
+                for (int c = 0; c < 4; ++c)
+                    for (int k = 0; k < 3; ++k) {
+                        lo[k] = std::min(lo[k], g.corners[12 * p + 3 * c + k]);
+                        hi[k] = std::max(hi[k], g.corners[12 * p + 3 * c + k]);
+                    }
+            for (int64_t p = 1; p < g.P; ++p)
+                if (op->eps_in[p] != op->eps_in[0] || op->eps_out[p] != op->eps_out[0])
+                    fail(DBIE_BAD_ARG, "screened_volume_source requires a uniform eps across the interface");
+            const double tol = std::sqrt(2.220446049250313e-16) *
+                               std::max(std::max(std::fabs(hi[0] - lo[0]), std::fabs(hi[1] - lo[1])), std::fabs(hi[2] - lo[2]));
+            screen_sharp(st, S, dpos.as<double>(), drho.as<double>(), dw.as<double>(), lo, hi, tol, op->eps_in[0],
+                         op->eps_out[0], dwr.as<double>());
+        } else {
+            // charges w.*rho, 1/eps_src applied at the end
+            const double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
+            screen_sharp(st, S, dpos.as<double>(), drho.as<double>(), dw.as<double>(), lo, hi, 0.0, 1.0, 1.0,
+                         dwr.as<double>());
+            scale = 1.0 / eps_src;
+        }
+        op->yout.ensure(sizeof(double) * N * K);
+        rhs_volume(st, op->dg, K, dcp.as<int64_t>(), dpos.as<double>(), dwr.as<double>(), op->yout.as<double>(), N,
+                   scale);
+        DBIE_HIP(hipGetLastError());
+        DBIE_HIP(hipMemcpy2DAsync(F, sizeof(double) * ldf, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
+                                  hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+    });
+}
+
+dbie_status dbie_pottrg(dbie_op *op, int64_t M, const double *trg, int K, const double *Sigma, int64_t lds, double *U,
+                        int64_t ldu) {
+    return guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (M < 1 || K < 1 || !trg || !Sigma || !U) fail(DBIE_BAD_ARG, "M, K >= 1 and non-NULL buffers required");
+        if (lds < N || ldu < M) fail(DBIE_DIM_MISMATCH, "leading dimension too small");
+        hipStream_t st = op->stream;
+        DBuf dt, dS, dU;
+        upload(dt, trg, 3 * (size_t)M, st);
+        dS.ensure(sizeof(double) * N * K);
+        dU.ensure(sizeof(double) * M * K);
+        DBIE_HIP(hipMemcpy2DAsync(dS.p, sizeof(double) * N, Sigma, sizeof(double) * lds, sizeof(double) * N, K,
+                                  hipMemcpyHostToDevice, st));
+        pottrg(st, op->dg, M, dt.as<double>(), K, dS.as<double>(), N, dU.as<double>(), M);
+        DBIE_HIP(hipGetLastError());
+        DBIE_HIP(hipMemcpy2DAsync(U, sizeof(double) * ldu, dU.p, sizeof(double) * M, sizeof(double) * M, K,
+                                  hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+    });
+}
+
+dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, double *Sigma, int64_t lds,
+                             const dbie_solve_opts *opts, dbie_solve_stats *stats) {
+    return guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (K < 1 || !F || !Sigma) fail(DBIE_BAD_ARG, "K >= 1 and non-NULL buffers required");
+        if (ldf < N || lds < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        dbie_solve_opts o;
+        if (opts) o = *opts;
+        else dbie_solve_opts_default(&o);
+        if (o.itmax < 1) fail(DBIE_BAD_ARG, "itmax must be >= 1");
+        if (!(o.rtol >= 0) || !(o.atol >= 0)) fail(DBIE_BAD_ARG, "tolerances must be >= 0");
+        hipStream_t st = op->stream;
+        const size_t NK = (size_t)N * K;
+        DBuf dF, dX, dR, dD, dinv;
+        dF.ensure(sizeof(double) * NK);
+        dX.ensure(sizeof(double) * NK);
+        DBIE_HIP(hipMemcpy2DAsync(dF.p, sizeof(double) * N, F, sizeof(double) * ldf, sizeof(double) * N, K,
+                                  hipMemcpyHostToDevice, st));
+        const double *dinvp = nullptr;
+        if (o.precondition) {
+            std::vector<double> inv(N);
+            for (int64_t i = 0; i < N; ++i) inv[i] = 1.0 / op->diag_h[i];
+            upload(dinv, inv.data(), N, st);
+            dinvp = dinv.as<double>();
+        }
+        hipEvent_t e0, e1;
+        DBIE_HIP(hipEventCreate(&e0));
+        DBIE_HIP(hipEventCreate(&e1));
+        DBIE_HIP(hipEventRecord(e0, st));
+        dbie_solve_stats s{};
+        std::vector<double> bnorm, rn;
+        if (o.precision == DBIE_FP64) {
+            const GmresOut r = gmres(op, K, dF.as<double>(), dX.as<double>(), DBIE_FP64, o.rtol, o.atol, o.itmax,
+                                     o.restart, dinvp);
+            s.niter = r.iters;
+            s.solved = r.solved;
+            s.nrefine = 0;
+        } else {
+            // mixed precision: FP64 residuals around low-precision inner GMRES
+            dR.ensure(sizeof(double) * NK);
+            dD.ensure(sizeof(double) * NK);
+            DBuf partial, hb;
+            col_norms(op, dF.as<double>(), N, K, partial, hb, bnorm);
+            DBIE_HIP(hipMemsetAsync(dX.p, 0, sizeof(double) * NK, st));
+            DBIE_HIP(hipMemcpyAsync(dR.p, dF.p, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
+            rn = bnorm;
+            s.solved = 0;
+            for (int it = 0;; ++it) {
+                bool done = true;
+                std::vector<double> mask(K);
+                for (int k = 0; k < K; ++k) {
+                    const bool ok = rn[k] <= o.atol + o.rtol * bnorm[k];
+                    mask[k] = ok ? 0.0 : 1.0;
+                    done = done && ok;
+                }
+                if (done) { s.solved = 1; break; }
+                if (it >= o.max_refine || s.niter >= o.itmax) break;
+                // zero the columns that already meet the tolerance
+                hb.ensure(sizeof(double) * K);
+                DBIE_HIP(hipMemcpyAsync(hb.p, mask.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+                col_scale(st, dR.as<double>(), hb.as<double>(), false, dR.as<double>(), N, K);
+                const GmresOut r = gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
+                                         o.itmax - s.niter, o.restart, dinvp);
+                s.niter += r.iters;
+                add_to(st, dD.as<double>(), dX.as<double>(), N, K);
+                apply_dev(op, K, dX.as<double>(), N, dR.as<double>(), N, DBIE_FP64);
+                sub_from(st, dF.as<double>(), dR.as<double>(), N, K);
+                ++s.nrefine;
+                col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+            }
+        }
+        DBIE_HIP(hipEventRecord(e1, st));
+        DBIE_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        DBIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+        s.time_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (o.precision == DBIE_FP64 && o.check_residual) {
+            dR.ensure(sizeof(double) * NK);
+            DBuf partial, hb;
+            col_norms(op, dF.as<double>(), N, K, partial, hb, bnorm);
+            apply_dev(op, K, dX.as<double>(), N, dR.as<double>(), N, DBIE_FP64);
+            sub_from(st, dF.as<double>(), dR.as<double>(), N, K);
+            col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+        }
+        double mx = 0.0;
+        for (size_t k = 0; k < rn.size(); ++k) mx = std::max(mx, bnorm[k] > 0 ? rn[k] / bnorm[k] : rn[k]);
+        s.max_rel_residual = rn.empty() ? -1.0 : mx;
+        DBIE_HIP(hipMemcpy2DAsync(Sigma, sizeof(double) * lds, dX.p, sizeof(double) * N, sizeof(double) * N, K,
+                                  hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+        if (stats) *stats = s;
+        if (!s.solved) fail(DBIE_NOT_CONVERGED, "solve did not reach the tolerance within itmax / max_refine");
+    });
+}
+
+}  // extern "C"

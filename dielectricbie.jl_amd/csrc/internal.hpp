@@ -1,0 +1,81 @@
+// internal.hpp -- shared host-side types of libdbie (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dbie.h"
+
+namespace dbie {
+
+// ---- error handling -------------------------------------------------------
+void set_error(const std::string &msg);
+struct Error {
+    dbie_status code;
+    std::string msg;
+};
+[[noreturn]] void fail(dbie_status code, const std::string &msg);
+void hip_check(hipError_t e, const char *what);
+#define DBIE_HIP(call) ::dbie::hip_check((call), #call)
+
+// ---- geometry --------------------------------------------------------------
+constexpr double kFourPi = 4.0 * 3.14159265358979311599796346854;
+
+struct Geometry {
+    int64_t P = 0, N = 0;
+    int nq = 0;
+    std::vector<double> corners;  // P x 4 x 3 (a, b, c, d)
+    std::vector<double> normal;   // P x 3
+    std::vector<int> is_edge;     // P
+    double glx[16] = {}, glw[16] = {}, bary[16] = {};
+    // node streams (structure of arrays)
+    std::vector<double> x, y, z, w;
+    int npp() const { return nq * nq; }
+};
+
+void gauss_legendre(int n, double *x, double *w);  // throws on n < 1
+bool best_grid(double W, double H, double alpha, int *m, int *n);
+Geometry *make_box(double Lx, double Ly, double Lz, int nq, double l_ec, double alpha);
+Geometry *make_from_panels(int64_t P, int nq, const double *corners, const double *normals, const int *is_edge);
+void barycentric_row(int n, const double *x, const double *lam, double xq, double *r);
+
+// ---- near field --------------------------------------------------------------
+struct NearOpts {
+    double up_tol = 1e-9;
+    int max_order = 8;
+    bool correct_edges = true;
+    double adaptive_atol = 1e-9;
+    double adaptive_rtol = 1.4901161193847656e-8;
+    int adaptive_n_GL = 0;
+    int adaptive_max_depth = 20;
+    int n_threads = 0;
+};
+
+struct PairList {
+    // sorted by (src, trg); upsampled pairs carry order >= nq, touching pairs order = 0
+    std::vector<int> up_src, up_trg, up_order;
+    std::vector<int> adp_src, adp_trg;
+};
+
+PairList build_pair_list(const Geometry &g, int max_order, double atol, bool correct_edges, int n_threads);
+
+// Block-sparse (BSR) correction matrix: block rows = target panels.
+struct BlockSparse {
+    int bd = 0;                    // block dimension = nq^2
+    std::vector<int> row_ptr;      // P + 1
+    std::vector<int> col;          // source panel per block
+    std::vector<double> val;       // nb x bd x bd, element (t, c) at t*bd + c (row-major)
+    int64_t nblocks() const { return (int64_t)col.size(); }
+    int adaptive_hits = 0;         // touching pairs that reached max_depth
+};
+
+BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearOpts &o);
+
+}  // namespace dbie
+
+// Opaque handles of the C ABI.
+struct dbie_geom {
+    dbie::Geometry g;
+};

@@ -1,0 +1,86 @@
+// kernels.hpp -- launchers of the gfx950 kernels (implemented in *.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dbie {
+
+// Device-resident node streams of one interface (structure of arrays).
+struct DevGeom {
+    int64_t N = 0;
+    const double *x = nullptr, *y = nullptr, *z = nullptr;      // node coordinates
+    const double *nx = nullptr, *ny = nullptr, *nz = nullptr;   // node normals
+    const double *w = nullptr;                                  // quadrature weights
+    const float *xf = nullptr, *yf = nullptr, *zf = nullptr;    // FP32 copies (mixed path)
+    const float *nxf = nullptr, *nyf = nullptr, *nzf = nullptr;
+    const float *wf = nullptr;
+    // face structure for skipping same-plane tiles: plane id per 64-node tile (-1 = mixed)
+    const int *tile_plane = nullptr;
+};
+
+// Block-sparse near correction on the device.
+struct DevBSR {
+    int P = 0, bd = 0;
+    int64_t nb = 0;
+    const int *row_ptr = nullptr, *col = nullptr;
+    const double *val64 = nullptr;  // nb x bd x bd, row-major blocks
+    const float *val32 = nullptr;
+};
+
+enum FarMode { FAR_DT = 0, FAR_S = 1, FAR_D = 2 };
+
+// Far-field partial sums: part[s][k][i] = sum over sources in split s of G(i,j) w_j X[j,k]
+// (G of the chosen mode, self pair excluded, 1/(4 pi) applied).  Returns the split count S.
+int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const double *X, int64_t ldx,
+                  double *part, int64_t part_cap);
+int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int64_t ldx, double *part,
+                  int64_t part_cap);
+// MFMA contraction of the DT far field: tiles generated on the fly and contracted with the
+// RHS block.  prec: 0 = FP64 MFMA, 2 = BF16 MFMA, 3 = split BF16 (3 products).
+int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
+                   int64_t part_cap);
+bool far_field_mfma_supported(int prec, int K);
+int64_t far_field_part_elems(int64_t N, int K);  // workspace needed for any far-field launch
+
+// Y[:,k] = sum_s part[s][k] + dscale*diag.*X[:,k] + C X[:,k] (C in FP64 or FP32 values).
+void near_combine(hipStream_t st, const DevBSR &C, bool use_f32, int S, const double *part, int64_t N, int K,
+                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy);
+
+// Right-hand sides.
+void rhs_points(hipStream_t st, const DevGeom &g, int K, const double *src, const double *q, double eps_src,
+                double *F, int64_t ldf);
+void rhs_volume(hipStream_t st, const DevGeom &g, int K, const int64_t *col_ptr, const double *pos,
+                const double *wrho, double *F, int64_t ldf, double scale);
+void screen_sharp(hipStream_t st, int64_t S, const double *pos, const double *rho, const double *w,
+                  const double *lo, const double *hi, double tol, double eps_in, double eps_out, double *wrho);
+
+// Off-surface single-layer potential U[t,k] = sum_i w_i X[i,k]/(4 pi |t - x_i|).
+void pottrg(hipStream_t st, const DevGeom &g, int64_t M, const double *trg, int K, const double *X, int64_t ldx,
+            double *U, int64_t ldu);
+
+// Dense A = DT + diag(t) (column-major, lda).
+void dense_lhs(hipStream_t st, const DevGeom &g, const double *diag, double *A, int64_t lda);
+
+// ---- Krylov primitives on column blocks (N x K, column-major, ld = N) ----
+// h[i*K + k] = <V_i[:,k], W[:,k]> for i < nv   (V_i = V + i*N*K); partial buffer of
+// dots_partial_elems(N, K, nv) doubles.
+int64_t dots_partial_elems(int64_t N, int K, int nv);
+void block_dots(hipStream_t st, const double *V, int nv, const double *W, int64_t N, int K, double *partial,
+                double *h);
+// W[:,k] -= sum_i V_i[:,k] h[i*K+k]
+void block_subtract(hipStream_t st, const double *V, int nv, const double *h, double *W, int64_t N, int K);
+// out[:,k] = in[:,k] * s[k] (s may be inverted: out = in / s when invert)
+void col_scale(hipStream_t st, const double *in, const double *s, bool invert, double *out, int64_t N, int K);
+// out[:,k] = d .* in[:,k]  (d per row)
+void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K);
+// X[:,k] += sum_i V_i[:,k] y[i*K+k]
+void block_combine(hipStream_t st, const double *V, int nv, const double *y, double *X, int64_t N, int K);
+// R = F - R (residual from AX stored in R), column-wise
+void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K);
+// X += D
+void add_to(hipStream_t st, const double *D, double *X, int64_t N, int K);
+// copy with leading dimensions
+void copy_cols(hipStream_t st, const double *src, int64_t lds, double *dst, int64_t ldd, int64_t N, int K);
+
+}  // namespace dbie

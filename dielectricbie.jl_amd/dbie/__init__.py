@@ -1,0 +1,35 @@
+"""dbie -- MI355X (gfx950) boundary-operator engine with DielectricBIE.jl's API.
+
+Host-side mirror of the reference's operator/solver interface; all arithmetic
+runs in libdbie.so (HIP kernels for gfx950 behind a C ABI, include/dbie.h).
+"""
+from ._lib import DbieError, NotConverged, lib, LIB_PATH  # noqa: F401
+from .api import (  # noqa: F401
+    BatchedDielectricOperator,
+    DielectricInterface,
+    PointSource,
+    SharpScreening,
+    VolumeSource,
+    all_weights,
+    batched_lhs_dielectric_box3d_fmm3d_corrected,
+    best_grid,
+    best_grid_mn,
+    build_neighbor_list,
+    dielectric_diagonal_scaling,
+    gausslegendre,
+    interface_from_panels,
+    lhs_dielectric_box3d,
+    lhs_dielectric_box3d_fmm3d,
+    lhs_dielectric_box3d_fmm3d_corrected,
+    num_points,
+    rhs_dielectric_box3d,
+    rhs_dielectric_box3d_fmm3d,
+    rhs_point_charges,
+    screened_volume_source,
+    single_dielectric_box3d,
+    solve_dielectric_box3d_block,
+    solve_gmres,
+    solve_lu,
+)
+
+__version__ = "0.1.0"

@@ -1,0 +1,515 @@
+"""Host-side mirror of DielectricBIE.jl's boundary-operator API, backed by libdbie on the GPU.
+
+Names, argument meaning and error behaviour follow the reference (Julia) functions
+cited in each docstring; Julia's 1-based panel/node indices become 0-based here.
+Arrays are numpy float64.  Every compute call goes through the C ABI of
+libdbie.so; nothing here computes kernel sums on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)
+
+
+# --------------------------------------------------------------------------- #
+# Utilities
+# --------------------------------------------------------------------------- #
+def gausslegendre(n: int):
+    """FastGaussQuadrature.gausslegendre(n): ascending nodes and weights on [-1, 1]."""
+    x = np.zeros(n); w = np.zeros(n)
+    L.check(L.lib().dbie_gauss_legendre(int(n), L.dptr(x), L.dptr(w)))
+    return x, w
+
+
+def best_grid_mn(W: float, H: float, alpha: float):
+    """best_grid_mn (src/utils/best_grid.jl:123-126); raises ValueError (ArgumentError) on bad input."""
+    m = C.c_int(); n = C.c_int()
+    L.check(L.lib().dbie_best_grid(float(W), float(H), float(alpha), C.byref(m), C.byref(n)))
+    return m.value, n.value
+
+
+best_grid = best_grid_mn
+
+
+# --------------------------------------------------------------------------- #
+# Sources (src/core/sources.jl)
+# --------------------------------------------------------------------------- #
+@dataclass
+class PointSource:
+    """PointSource{T,3}(point, charge) (src/core/sources.jl:5-8)."""
+    point: tuple
+    charge: float
+
+
+@dataclass
+class VolumeSource:
+    """VolumeSource (src/core/sources.jl:20-32): positions 3 x S, weights S, density S."""
+    positions: np.ndarray
+    weights: np.ndarray
+    density: np.ndarray
+
+    def __post_init__(self):
+        self.positions = np.asarray(self.positions, np.float64)
+        if self.positions.ndim == 2 and self.positions.shape[0] != 3 and self.positions.shape[1] == 3:
+            self.positions = self.positions.T
+        self.positions = np.ascontiguousarray(self.positions.reshape(3, -1))
+        self.weights = np.ascontiguousarray(self.weights, np.float64).ravel()
+        self.density = np.ascontiguousarray(self.density, np.float64).ravel()
+        S = self.positions.shape[1]
+        if len(self.weights) != S or len(self.density) != S:
+            raise ValueError("points, weights and density must have the same length")
+
+    @classmethod
+    def from_points(cls, points, weights, density, tol: float = 0.0):
+        """VolumeSource(points::Vector{NTuple{3}}, weights, density; tol) (src/core/sources.jl:249-261)
+        with the |density| >= tol truncation of _truncate_volume_source (:100-112)."""
+        pos = np.asarray(points, np.float64).reshape(-1, 3).T
+        w = np.asarray(weights, np.float64); rho = np.asarray(density, np.float64)
+        if tol < 0:
+            raise ValueError("tol must be >= 0")
+        if tol > 0:
+            keep = np.abs(rho) >= tol
+            if not keep.any():
+                raise ValueError(f"all source points are truncated by tol={tol}")
+            pos, w, rho = pos[:, keep], w[keep], rho[keep]
+        return cls(pos, w, rho)
+
+    @classmethod
+    def point_charge(cls, point, charge: float):
+        """A point charge as a one-point VolumeSource (src/core/sources.jl:249)."""
+        return cls(np.asarray(point, np.float64).reshape(3, 1), np.ones(1), np.array([float(charge)]))
+
+
+class SharpScreening:
+    """SharpScreening (src/core/sources.jl:265): rho / eps_in inside the box, rho / eps_out outside."""
+
+
+# --------------------------------------------------------------------------- #
+# Geometry (src/core/panels.jl, src/shape/box3d_single.jl)
+# --------------------------------------------------------------------------- #
+class DielectricInterface:
+    """DielectricInterface of flat tensor-Gauss-Legendre panels (src/core/panels.jl:64-68).
+
+    Node order: panel order, then (i outer along b-a, j inner along d-a)."""
+
+    def __init__(self, handle, eps_in, eps_out):
+        self._h = C.c_void_p(handle)
+        P = C.c_int64(); N = C.c_int64(); nq = C.c_int()
+        L.check(L.lib().dbie_geom_sizes(self._h, C.byref(P), C.byref(N), C.byref(nq)))
+        self.n_panels, self.n_points, self.n_quad = P.value, N.value, nq.value
+        self.eps_in = np.ascontiguousarray(np.broadcast_to(np.asarray(eps_in, np.float64), (self.n_panels,)))
+        self.eps_out = np.ascontiguousarray(np.broadcast_to(np.asarray(eps_out, np.float64), (self.n_panels,)))
+        self._nodes = None
+        self._panels = None
+        self._ops = {}
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and L._lib is not None:
+                L._lib.dbie_geom_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def __repr__(self):
+        return f"DielectricInterface with {self.n_panels} panels in Float64 ({self.n_points} points)"
+
+    @property
+    def nodes(self) -> np.ndarray:
+        """7 x N SoA: x, y, z, nx, ny, nz, w."""
+        if self._nodes is None:
+            out = np.zeros(7 * self.n_points)
+            L.check(L.lib().dbie_geom_nodes(self._h, L.dptr(out)))
+            self._nodes = out.reshape(7, self.n_points)
+        return self._nodes
+
+    @property
+    def points(self) -> np.ndarray:
+        return self.nodes[0:3]
+
+    @property
+    def normals(self) -> np.ndarray:
+        return self.nodes[3:6]
+
+    @property
+    def weights(self) -> np.ndarray:
+        return self.nodes[6]
+
+    def panels(self):
+        """(corners P x 4 x 3, normals P x 3, is_edge P)."""
+        if self._panels is None:
+            c = np.zeros(12 * self.n_panels); n = np.zeros(3 * self.n_panels); e = np.zeros(self.n_panels, np.int32)
+            L.check(L.lib().dbie_geom_panels(self._h, L.dptr(c), L.dptr(n), L.iptr(e)))
+            self._panels = (c.reshape(-1, 4, 3), n.reshape(-1, 3), e.astype(bool))
+        return self._panels
+
+    def diag(self) -> np.ndarray:
+        t = 0.5 * (self.eps_out + self.eps_in) / (self.eps_out - self.eps_in)
+        return np.repeat(t, self.n_quad ** 2)
+
+
+def num_points(interface: DielectricInterface) -> int:
+    """num_points (src/core/panels.jl:117)."""
+    return interface.n_points
+
+
+def all_weights(interface: DielectricInterface) -> np.ndarray:
+    """all_weights (src/core/panels.jl:118)."""
+    return interface.weights.copy()
+
+
+def single_dielectric_box3d(Lx, Ly, Lz, n_quad: int, l_ec, eps_in, eps_out, T=np.float64, *, alpha=math.sqrt(2.0)):
+    """single_dielectric_box3d (src/shape/box3d_single.jl:49-70)."""
+    if T not in (np.float64, float):
+        raise TypeError("only Float64 geometry is supported")
+    h = C.c_void_p()
+    L.check(L.lib().dbie_geometry_box(float(Lx), float(Ly), float(Lz), int(n_quad), float(l_ec), float(alpha),
+                                      C.byref(h)))
+    return DielectricInterface(h.value, eps_in, eps_out)
+
+
+def interface_from_panels(corners, normals, n_quad: int, eps_in, eps_out, is_edge=None):
+    """DielectricInterface(panels, eps_in, eps_out) of rect_panel3d_discretize panels
+    (src/shape/box3d.jl:22-64); corners P x 4 x 3 ordered (a, b, c, d)."""
+    c = np.ascontiguousarray(corners, np.float64).reshape(-1, 12)
+    P = c.shape[0]
+    n = np.ascontiguousarray(normals, np.float64).reshape(P, 3)
+    e = np.zeros(P, np.int32) if is_edge is None else np.ascontiguousarray(is_edge, np.int32)
+    h = C.c_void_p()
+    L.check(L.lib().dbie_geometry_from_panels(P, int(n_quad), L.dptr(c.ravel()), L.dptr(n.ravel()), L.iptr(e),
+                                              C.byref(h)))
+    return DielectricInterface(h.value, eps_in, eps_out)
+
+
+def build_neighbor_list(interface: DielectricInterface, max_order: int, atol: float, *, correct_edges: bool = False):
+    """build_neighbor_list (src/kernel/laplace3d_near.jl:240-392) -> (upsample, adaptive) with 0-based
+    (source panel, target panel) keys; upsample maps to p_up."""
+    nu = C.c_int64(); na = C.c_int64()
+    lib = L.lib()
+    L.check(lib.dbie_neighbor_list(interface._h, int(max_order), float(atol), int(bool(correct_edges)),
+                                   C.byref(nu), C.byref(na), None, None, None, None, None))
+    us = np.zeros(nu.value, np.int32); ut = np.zeros(nu.value, np.int32); uo = np.zeros(nu.value, np.int32)
+    asrc = np.zeros(na.value, np.int32); atrg = np.zeros(na.value, np.int32)
+    L.check(lib.dbie_neighbor_list(interface._h, int(max_order), float(atol), int(bool(correct_edges)),
+                                   C.byref(nu), C.byref(na), L.iptr(us), L.iptr(ut), L.iptr(uo), L.iptr(asrc),
+                                   L.iptr(atrg)))
+    upsample = {(int(i), int(j)): int(p) for i, j, p in zip(us, ut, uo)}
+    adaptive = {(int(i), int(j)) for i, j in zip(asrc, atrg)}
+    return upsample, adaptive
+
+
+# --------------------------------------------------------------------------- #
+# Operators
+# --------------------------------------------------------------------------- #
+class BatchedDielectricOperator:
+    """BatchedDielectricOperator (src/solver/multi_rhs.jl:46-141), resident in HBM.
+
+    ``op @ X`` / ``mul(Y, X)`` apply A = Dᵀ + C + diag(t) to a vector or an N x K block.
+    The far field is the exact direct sum (the reference's FMM at fmm_tol approximates it)."""
+
+    def __init__(self, interface: DielectricInterface, *, corrected=True, up_tol=1e-9, max_order=8,
+                 correct_edges=True, adaptive_atol=None, adaptive_rtol=SQRT_EPS, adaptive_n_GL=0,
+                 adaptive_max_depth=20, n_threads=0):
+        o = L.NearOpts()
+        L.lib().dbie_near_opts_default(C.byref(o))
+        o.up_tol = float(up_tol)
+        o.max_order = int(max_order)
+        o.correct_edges = int(bool(correct_edges))
+        o.adaptive_atol = float(up_tol if adaptive_atol is None else adaptive_atol)
+        o.adaptive_rtol = float(adaptive_rtol)
+        o.adaptive_n_GL = int(adaptive_n_GL)
+        o.adaptive_max_depth = int(adaptive_max_depth)
+        o.build_corrections = int(bool(corrected))
+        o.n_threads = int(n_threads)
+        h = C.c_void_p()
+        L.check(L.lib().dbie_operator_create(interface._h, L.dptr(interface.eps_in), L.dptr(interface.eps_out),
+                                             C.byref(o), C.byref(h)))
+        self._h = C.c_void_p(h.value)
+        self.interface = interface
+        self.n = interface.n_points
+        N = C.c_int64(); nb = C.c_int64(); bd = C.c_int()
+        L.check(L.lib().dbie_op_sizes(self._h, C.byref(N), C.byref(nb), C.byref(bd)))
+        self.n_blocks, self.block_dim = nb.value, bd.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and L._lib is not None:
+                L._lib.dbie_operator_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.n, self.n)
+
+    def size(self, d=None):
+        return self.shape if d is None else (self.n if d <= 2 else 1)
+
+    dtype = np.float64
+
+    def set_eps(self, eps_in, eps_out):
+        ei = np.ascontiguousarray(np.broadcast_to(np.asarray(eps_in, np.float64), (self.interface.n_panels,)))
+        eo = np.ascontiguousarray(np.broadcast_to(np.asarray(eps_out, np.float64), (self.interface.n_panels,)))
+        L.check(L.lib().dbie_op_set_eps(self._h, L.dptr(ei), L.dptr(eo)))
+
+    def corrections(self):
+        """Correction blocks: (src panels, trg panels, blocks[b, t, c]) with C[trg*bd+t, src*bd+c] = blocks[b,t,c]."""
+        nb, bd = self.n_blocks, self.block_dim
+        src = np.zeros(nb, np.int32); trg = np.zeros(nb, np.int32); blk = np.zeros(max(nb, 1) * bd * bd)
+        L.check(L.lib().dbie_op_corrections(self._h, L.iptr(src), L.iptr(trg), L.dptr(blk)))
+        return src, trg, blk[: nb * bd * bd].reshape(nb, bd, bd)
+
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        L.check(L.lib().dbie_op_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def apply(self, X, precision="fp64"):
+        X = np.asarray(X, np.float64)
+        vec = X.ndim == 1
+        Xf = L.fortran(X, self.n)
+        K = Xf.shape[1]
+        Y = np.zeros((self.n, K), order="F")
+        L.check(L.lib().dbie_apply(self._h, K, L.dptr(Xf), self.n, L.dptr(Y), self.n,
+                                   L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)))
+        return Y[:, 0] if vec else Y
+
+    def apply_kernel(self, X, kernel="DT"):
+        """Plain laplace3d_DT / _S / _D matvec (no corrections, no diagonal)."""
+        X = np.asarray(X, np.float64)
+        vec = X.ndim == 1
+        Xf = L.fortran(X, self.n)
+        K = Xf.shape[1]
+        Y = np.zeros((self.n, K), order="F")
+        kid = {"DT": L.OP_DT, "S": L.OP_S, "D": L.OP_D}[kernel]
+        L.check(L.lib().dbie_apply_kernel(self._h, kid, K, L.dptr(Xf), self.n, L.dptr(Y), self.n))
+        return Y[:, 0] if vec else Y
+
+    def __matmul__(self, X):
+        return self.apply(X)
+
+    def __mul__(self, X):
+        return self.apply(X)
+
+    def mul(self, Y, X):
+        """mul!(Y, op, X) (src/solver/multi_rhs.jl:103-136)."""
+        Y[...] = self.apply(X).reshape(Y.shape)
+        return Y
+
+    def dense(self) -> np.ndarray:
+        """lhs_dielectric_box3d: dense Dᵀ + diag(t) assembled on the GPU (N <= 65535)."""
+        A = np.zeros((self.n, self.n), order="F")
+        L.check(L.lib().dbie_lhs_dense(self._h, L.dptr(A), self.n))
+        return A
+
+    def solve(self, F, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precondition=True, precision="fp64",
+              inner_rtol=1e-4, max_refine=10, check_residual=True, raise_on_fail=True):
+        """Block GMRES-type solve of A Σ = F on the GPU; returns (Σ, stats dict)."""
+        F = np.asarray(F, np.float64)
+        vec = F.ndim == 1
+        Ff = L.fortran(F, self.n)
+        K = Ff.shape[1]
+        S = np.zeros((self.n, K), order="F")
+        o = L.SolveOpts()
+        L.lib().dbie_solve_opts_default(C.byref(o))
+        o.rtol, o.atol, o.itmax, o.restart = float(rtol), float(atol), int(itmax), int(restart)
+        o.precondition = int(bool(precondition))
+        o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
+        st = L.SolveStats()
+        status = L.lib().dbie_solve_block(self._h, K, L.dptr(Ff), self.n, L.dptr(S), self.n, C.byref(o),
+                                          C.byref(st))
+        stats = st.as_dict()
+        if status == L.NOT_CONVERGED and not raise_on_fail:
+            pass
+        else:
+            L.check(status)
+        return (S[:, 0] if vec else S), stats
+
+    def pottrg(self, targets, Sigma):
+        """Induced potential u[σ](t) = Σ_i w_i σ_i /(4π|t - x_i|) at targets (3 x M or M x 3)."""
+        t = np.asarray(targets, np.float64)
+        if t.ndim == 2 and t.shape[0] == 3 and t.shape[1] != 3:
+            t = t.T
+        t = np.ascontiguousarray(t.reshape(-1, 3))
+        M = t.shape[0]
+        Sg = np.asarray(Sigma, np.float64)
+        vec = Sg.ndim == 1
+        Sf = L.fortran(Sg, self.n)
+        K = Sf.shape[1]
+        U = np.zeros((M, K), order="F")
+        L.check(L.lib().dbie_pottrg(self._h, M, L.dptr(t.ravel()), K, L.dptr(Sf), self.n, L.dptr(U), M))
+        return U[:, 0] if vec else U
+
+
+def _op_for(interface, key, **kw):
+    op = interface._ops.get(key)
+    if op is None:
+        op = BatchedDielectricOperator(interface, **kw)
+        interface._ops[key] = op
+    return op
+
+
+def lhs_dielectric_box3d(interface: DielectricInterface) -> np.ndarray:
+    """lhs_dielectric_box3d (src/solver/dielectric_box3d.jl:1-17): dense Dᵀ + diag(t)."""
+    return _op_for(interface, ("plain",), corrected=False).dense()
+
+
+def lhs_dielectric_box3d_fmm3d(interface: DielectricInterface, tol: float = 1e-6):
+    """lhs_dielectric_box3d_fmm3d (src/solver/dielectric_box3d.jl:20-44): matrix-free Dᵀ + diag(t).
+    ``tol`` is accepted for API compatibility; the GPU sum is exact."""
+    return _op_for(interface, ("plain",), corrected=False)
+
+
+def lhs_dielectric_box3d_fmm3d_corrected(interface: DielectricInterface, fmm_tol: float, up_tol: float,
+                                         max_order: int, *, correct_edges=True, adaptive_atol=None,
+                                         adaptive_rtol=SQRT_EPS, adaptive_n_GL=0, adaptive_max_depth=20):
+    """lhs_dielectric_box3d_fmm3d_corrected (src/solver/dielectric_box3d.jl:48-92)."""
+    kw = dict(up_tol=up_tol, max_order=max_order, correct_edges=correct_edges,
+              adaptive_atol=up_tol if adaptive_atol is None else adaptive_atol, adaptive_rtol=adaptive_rtol,
+              adaptive_n_GL=adaptive_n_GL, adaptive_max_depth=adaptive_max_depth)
+    return _op_for(interface, ("corr",) + tuple(sorted(kw.items())), **kw)
+
+
+def batched_lhs_dielectric_box3d_fmm3d_corrected(interface: DielectricInterface, fmm_tol: float, up_tol: float,
+                                                 max_order: int, **kw):
+    """batched_lhs_dielectric_box3d_fmm3d_corrected (src/solver/multi_rhs.jl:60-100)."""
+    return lhs_dielectric_box3d_fmm3d_corrected(interface, fmm_tol, up_tol, max_order, **kw)
+
+
+def dielectric_diagonal_scaling(interface: DielectricInterface) -> np.ndarray:
+    """dielectric_diagonal_scaling (src/solver/multi_rhs.jl:234-247): 1 / t per node."""
+    return 1.0 / interface.diag()
+
+
+def _rhs_op(interface):
+    ops = interface._ops
+    if ops:
+        return next(iter(ops.values()))
+    return _op_for(interface, ("plain",), corrected=False)
+
+
+def screened_volume_source(interface: DielectricInterface, vs: VolumeSource, mode=None) -> VolumeSource:
+    """screened_volume_source(interface, vs, SharpScreening()) (src/core/sources.jl:390-406).
+    The screening itself runs on the GPU inside the RHS kernels; this host form exists for
+    callers that need the screened density (interface bounds + sqrt(eps)*max side tolerance)."""
+    if mode is None:
+        raise TypeError("screened_volume_source requires a screening mode (MethodError in the reference)")
+    if not isinstance(mode, SharpScreening):
+        raise NotImplementedError("only SharpScreening is on the GPU path")
+    ei, eo = interface.eps_in, interface.eps_out
+    if not (np.all(ei == ei[0]) and np.all(eo == eo[0])):
+        raise ValueError("screened_volume_source requires a uniform eps across the interface")
+    corners = interface.panels()[0].reshape(-1, 3)
+    lo, hi = corners.min(0), corners.max(0)
+    tol = SQRT_EPS * np.max(np.abs(hi - lo))
+    p = vs.positions
+    inside = np.all((lo[:, None] - tol <= p) & (p <= hi[:, None] + tol), axis=0)
+    rho = vs.density / np.where(inside, ei[0], eo[0])
+    return VolumeSource(p.copy(), vs.weights.copy(), rho)
+
+
+def rhs_dielectric_box3d(interface: DielectricInterface, source, eps_src=None) -> np.ndarray:
+    """rhs_dielectric_box3d for a PointSource (src/solver/dielectric_box3d.jl:95-104), a VolumeSource
+    with eps_src (:106-120) or a VolumeSource screened by the interface (:122-124)."""
+    op = _rhs_op(interface)
+    N = interface.n_points
+    if isinstance(source, PointSource):
+        if eps_src is None:
+            raise TypeError("eps_src is required for a PointSource")
+        src = np.ascontiguousarray(source.point, np.float64).reshape(3)
+        q = np.array([float(source.charge)])
+        F = np.zeros((N, 1), order="F")
+        L.check(L.lib().dbie_rhs_points(op._h, 1, L.dptr(src), L.dptr(q), float(eps_src), L.dptr(F), N))
+        return F[:, 0]
+    if isinstance(source, VolumeSource):
+        return _rhs_volume(op, [source], screening=eps_src is None, eps_src=1.0 if eps_src is None else eps_src)[:, 0]
+    raise TypeError(f"unsupported source type {type(source).__name__}")
+
+
+def rhs_point_charges(interface: DielectricInterface, points, charges, eps_src: float) -> np.ndarray:
+    """N x K RHS with one column per point charge (the PointSource rule applied column-wise)."""
+    op = _rhs_op(interface)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(charges, np.float64).ravel()
+    K = pts.shape[0]
+    if len(q) != K:
+        raise ValueError("points and charges must have the same length")
+    F = np.zeros((interface.n_points, K), order="F")
+    L.check(L.lib().dbie_rhs_points(op._h, K, L.dptr(pts.ravel()), L.dptr(q), float(eps_src), L.dptr(F),
+                                    interface.n_points))
+    return F
+
+
+def _rhs_volume(op, vss, screening: bool, eps_src: float = 1.0) -> np.ndarray:
+    K = len(vss)
+    col_ptr = np.zeros(K + 1, np.int64)
+    col_ptr[1:] = np.cumsum([v.positions.shape[1] for v in vss])
+    xyz = np.ascontiguousarray(np.concatenate([v.positions.T for v in vss], axis=0)) if K else np.zeros((0, 3))
+    w = np.ascontiguousarray(np.concatenate([v.weights for v in vss])) if K else np.zeros(0)
+    rho = np.ascontiguousarray(np.concatenate([v.density for v in vss])) if K else np.zeros(0)
+    N = op.n
+    F = np.zeros((N, max(K, 1)), order="F")
+    if K == 0:
+        return np.zeros((N, 0))
+    if xyz.size == 0:
+        xyz = np.zeros((1, 3)); w = np.zeros(1); rho = np.zeros(1)
+    L.check(L.lib().dbie_rhs_volume(op._h, K, L.i64ptr(col_ptr), L.dptr(xyz.ravel()), L.dptr(w), L.dptr(rho),
+                                    int(bool(screening)), float(eps_src), L.dptr(F), N))
+    return F
+
+
+def rhs_dielectric_box3d_fmm3d(interface: DielectricInterface, sources, *args):
+    """rhs_dielectric_box3d_fmm3d: single VolumeSource (src/solver/dielectric_box3d.jl:127-159) or
+    Vector{VolumeSource} -> N x K (src/solver/multi_rhs.jl:168-211).  Forms:
+    (interface, vs, eps_src, thresh), (interface, vs, thresh), (interface, vss, thresh)."""
+    op = _rhs_op(interface)
+    if isinstance(sources, VolumeSource):
+        if len(args) == 2:
+            return _rhs_volume(op, [sources], screening=False, eps_src=float(args[0]))[:, 0]
+        return _rhs_volume(op, [sources], screening=True)[:, 0]
+    return _rhs_volume(op, list(sources), screening=True)
+
+
+# --------------------------------------------------------------------------- #
+# Solvers
+# --------------------------------------------------------------------------- #
+def solve_lu(A, b):
+    """solve_lu (src/utils/linear_algebra.jl:1): LU with partial pivoting, on the GPU (rocSOLVER)."""
+    A = L.fortran(A)
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise ValueError("DimensionMismatch: A must be square")
+    b = np.asarray(b, np.float64)
+    vec = b.ndim == 1
+    B = L.fortran(b, n).copy(order="F")
+    L.check(L.lib().dbie_solve_lu(n, L.dptr(A), n, B.shape[1], L.dptr(B), n))
+    return B[:, 0] if vec else B
+
+
+def solve_gmres(A, b, atol: float = 1e-6, rtol: float = 1e-6, itmax: int = 1000):
+    """solve_gmres (src/utils/linear_algebra.jl:3-6): unpreconditioned GMRES on the GPU."""
+    if not isinstance(A, BatchedDielectricOperator):
+        raise TypeError("solve_gmres needs a GPU operator (lhs_dielectric_box3d_fmm3d[_corrected])")
+    x, _ = A.solve(b, rtol=rtol, atol=atol, itmax=itmax, precondition=False, raise_on_fail=False)
+    return x
+
+
+def solve_dielectric_box3d_block(interface: DielectricInterface, vss, *, fmm_tol=1e-9, up_tol=1e-9, max_order=8,
+                                 rtol=1e-10, atol=0.0, itmax=500, precondition=True, correct_edges=True,
+                                 precision="fp64", inner_rtol=1e-4, max_refine=10):
+    """solve_dielectric_box3d_block (src/solver/multi_rhs.jl:270-286) -> (Σ, stats).
+    ``precision`` other than "fp64" selects the mixed-precision refinement solve."""
+    op = batched_lhs_dielectric_box3d_fmm3d_corrected(interface, fmm_tol, up_tol, max_order,
+                                                      correct_edges=correct_edges)
+    F = rhs_dielectric_box3d_fmm3d(interface, list(vss), fmm_tol)
+    return op.solve(F, rtol=rtol, atol=atol, itmax=itmax, precondition=precondition, precision=precision,
+                    inner_rtol=inner_rtol, max_refine=max_refine)

@@ -1,0 +1,198 @@
+/*
+ * dbie.h -- C ABI of libdbie, an MI355X (gfx950) implementation of
+ * DielectricBIE.jl's boundary-operator hot path.
+ *
+ * Conventions
+ *   - All functions return dbie_status (0 = OK).  On error a thread-local
+ *     message is available from dbie_last_error().
+ *   - Matrices are column-major FP64 with an explicit leading dimension,
+ *     exactly like Julia's Matrix{Float64} (the reference's caller type).
+ *   - Indices of panels/nodes are 0-based.  Node order is panel order, then
+ *     (i outer along b-a, j inner along d-a), matching
+ *     src/shape/box3d.jl:44-59 of the reference.
+ *   - Host-pointer entry points are synchronous: they copy in, run on the
+ *     library's HIP stream and copy out before returning.  The *_device
+ *     variants take device pointers and are stream-ordered on the handle's
+ *     stream (dbie_op_stream); they return before the GPU finishes.
+ *   - A handle is not thread-safe; distinct handles are independent.
+ *
+ * Each entry point names the reference interface (file:line) it replaces.
+ */
+#ifndef DBIE_H
+#define DBIE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int dbie_status;
+enum {
+    DBIE_OK = 0,
+    DBIE_BAD_ARG = 1,        /* ArgumentError in the reference */
+    DBIE_DIM_MISMATCH = 2,   /* DimensionMismatch (src/solver/multi_rhs.jl:105) */
+    DBIE_OOM = 3,            /* device or host allocation failed */
+    DBIE_HIP_ERROR = 4,      /* a HIP runtime call failed */
+    DBIE_NOT_CONVERGED = 5,  /* iterative solve hit itmax (Krylov stats.solved == false) */
+    DBIE_NO_DEVICE = 6       /* no usable gfx950 device */
+};
+
+/* Precision of the far-field kernel x RHS-block contraction. */
+typedef enum {
+    DBIE_FP64 = 0,   /* FP64 tile generation and FP64 accumulation (the plain path) */
+    DBIE_FP32 = 1,   /* FP32 tile generation, FP32 accumulation */
+    DBIE_BF16 = 2,   /* FP32 tile generation, BF16 MFMA contraction, FP32 accumulation */
+    DBIE_BF16X3 = 3  /* FP32 generation, split-BF16 (hi*hi+hi*lo+lo*hi) MFMA, FP32 accumulation */
+} dbie_precision;
+
+/* Which boundary operator a plain apply evaluates (src/kernel/laplace3d.jl:18-52). */
+typedef enum {
+    DBIE_OP_DT = 0,  /* adjoint double layer  DT[i,j] = w_j n_i.(x_i-y_j)/(4 pi r^3) */
+    DBIE_OP_S = 1,   /* single layer          S[i,j]  = w_j /(4 pi r)                  */
+    DBIE_OP_D = 2    /* double layer          D[i,j]  = w_j n_j.(x_j-x_i)/(4 pi r^3)  */
+} dbie_kernel;
+
+typedef struct dbie_geom dbie_geom;  /* a DielectricInterface of flat tensor-GL panels */
+typedef struct dbie_op dbie_op;      /* a BatchedDielectricOperator resident on the GPU */
+
+/* Near-field options: keyword arguments of
+   lhs_dielectric_box3d_fmm3d_corrected (src/solver/dielectric_box3d.jl:48-58) and
+   batched_lhs_dielectric_box3d_fmm3d_corrected (src/solver/multi_rhs.jl:60-66). */
+typedef struct {
+    double up_tol;           /* Bernstein threshold atol (default 1e-9) */
+    int max_order;           /* p_up ceiling (default 8) */
+    int correct_edges;       /* build touching-pair (adaptive) blocks (default 1) */
+    double adaptive_atol;    /* default = up_tol */
+    double adaptive_rtol;    /* default sqrt(eps) */
+    int adaptive_n_GL;       /* 0 = source panel's n_quad */
+    int adaptive_max_depth;  /* default 20 */
+    int build_corrections;   /* 0 = plain (uncorrected) operator, as lhs_dielectric_box3d_fmm3d */
+    int n_threads;           /* host threads for the near-field build (0 = all cores) */
+} dbie_near_opts;
+
+/* Solve options: keywords of solve_dielectric_box3d_block (src/solver/multi_rhs.jl:270-276)
+   and solve_gmres (src/utils/linear_algebra.jl:3). */
+typedef struct {
+    double rtol;             /* default 1e-10 */
+    double atol;             /* default 0 */
+    int itmax;               /* default 500 */
+    int restart;             /* Krylov basis size before restart (0 = min(itmax, 100)) */
+    int precondition;        /* right diagonal scaling 1/t (dielectric_diagonal_scaling) */
+    dbie_precision precision;/* DBIE_FP64 = plain GMRES; otherwise mixed-precision IR */
+    double inner_rtol;       /* mixed: inner-solve relative tolerance (default 1e-4) */
+    int max_refine;          /* mixed: maximum FP64 refinement steps (default 10) */
+    int check_residual;      /* FP64 path: evaluate the true FP64 residual after the solve
+                                (one extra matvec; fills max_rel_residual; default 1) */
+} dbie_solve_opts;
+
+typedef struct {
+    int niter;               /* total Krylov iterations (inner iterations for mixed) */
+    int nrefine;             /* FP64 residual evaluations (mixed) */
+    int solved;              /* all columns reached the tolerance */
+    double max_rel_residual; /* max_k ||F_k - A Sigma_k|| / ||F_k|| in FP64 */
+    double time_ms;          /* device time of the solve */
+} dbie_solve_stats;
+
+/* ---- library ---------------------------------------------------------- */
+const char *dbie_version(void);
+/* Copies the calling thread's last error message into buf (NUL-terminated). */
+dbie_status dbie_last_error(char *buf, size_t len);
+/* Initialise the HIP device (returns DBIE_NO_DEVICE if there is no GPU). */
+dbie_status dbie_init(int device);
+/* Default-initialise option structs. */
+void dbie_near_opts_default(dbie_near_opts *o);
+void dbie_solve_opts_default(dbie_solve_opts *o);
+
+/* ---- geometry (host) -------------------------------------------------- */
+/* gausslegendre(n) (FastGaussQuadrature, ascending nodes) */
+dbie_status dbie_gauss_legendre(int n, double *x, double *w);
+/* best_grid_mn (src/utils/best_grid.jl:8-126) */
+dbie_status dbie_best_grid(double W, double H, double alpha, int *m, int *n);
+/* single_dielectric_box3d (src/shape/box3d_single.jl:49-70) */
+dbie_status dbie_geometry_box(double Lx, double Ly, double Lz, int n_quad, double l_ec, double alpha,
+                              dbie_geom **out);
+/* DielectricInterface(panels, ...) of rect_panel3d_discretize panels
+   (src/shape/box3d.jl:22-64, src/core/panels.jl:64-68); corners P x 4 x 3
+   (lb, rb, rt, lt), normals P x 3, is_edge P (may be NULL). */
+dbie_status dbie_geometry_from_panels(int64_t P, int n_quad, const double *corners, const double *normals,
+                                      const int *is_edge, dbie_geom **out);
+dbie_status dbie_geom_sizes(const dbie_geom *g, int64_t *P, int64_t *N, int *n_quad);
+/* node SoA, 7 x N rows x,y,z,nx,ny,nz,w (eachpoint + all_weights, src/core/panels.jl:85-118) */
+dbie_status dbie_geom_nodes(const dbie_geom *g, double *xyz_nrm_w);
+dbie_status dbie_geom_panels(const dbie_geom *g, double *corners, double *normals, int *is_edge);
+void dbie_geom_destroy(dbie_geom *g);
+
+/* ---- near-field list (host, exposed for checking) --------------------- */
+/* build_neighbor_list (src/kernel/laplace3d_near.jl:240-392).  Call with NULL
+   arrays to get the counts, then again with arrays of those sizes. Pairs are
+   sorted by (source panel, target panel). */
+dbie_status dbie_neighbor_list(const dbie_geom *g, int max_order, double atol, int correct_edges,
+                               int64_t *n_up, int64_t *n_adp, int *up_src, int *up_trg, int *up_order,
+                               int *adp_src, int *adp_trg);
+
+/* ---- operator --------------------------------------------------------- */
+/* batched_lhs_dielectric_box3d_fmm3d_corrected (src/solver/multi_rhs.jl:60-100):
+   geometry SoA, correction blocks (laplace3d_DT_corrections,
+   src/kernel/laplace3d_near_upsampling.jl:272) and diag t are built and kept
+   resident in HBM.  eps_in/eps_out are per panel (length P). */
+dbie_status dbie_operator_create(const dbie_geom *g, const double *eps_in, const double *eps_out,
+                                 const dbie_near_opts *opts, dbie_op **out);
+void dbie_operator_destroy(dbie_op *op);
+dbie_status dbie_op_sizes(const dbie_op *op, int64_t *N, int64_t *n_blocks, int *block_dim);
+/* Update eps (diag only; geometry and corrections depend on geometry alone). */
+dbie_status dbie_op_set_eps(dbie_op *op, const double *eps_in, const double *eps_out);
+/* Export the correction blocks: pairs (src, trg) and blocks[b][t + c*bd] (FP64). */
+dbie_status dbie_op_corrections(const dbie_op *op, int *src, int *trg, double *blocks);
+/* HBM bytes the operator holds. */
+dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes);
+
+/* mul!(Y, op, X) (src/solver/multi_rhs.jl:103-136): Y = DT X + C X + t.*X.
+   precision selects the far-field contraction (DBIE_FP64 = plain path). */
+dbie_status dbie_apply(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy,
+                       dbie_precision precision);
+/* Plain boundary operators without corrections/diag: laplace3d_DT / _S / _D applied
+   matrix-free (src/kernel/laplace3d.jl:18-52 and the *_fmm3d maps :84-137). */
+dbie_status dbie_apply_kernel(dbie_op *op, dbie_kernel kernel, int K, const double *X, int64_t ldx,
+                              double *Y, int64_t ldy);
+/* Dense lhs_dielectric_box3d (src/solver/dielectric_box3d.jl:1-17): A = DT + diag(t), N x N. */
+dbie_status dbie_lhs_dense(dbie_op *op, double *A, int64_t lda);
+/* solve_lu (src/utils/linear_algebra.jl:1): LU with partial pivoting on the GPU. */
+dbie_status dbie_solve_lu(int64_t n, const double *A, int64_t lda, int nrhs, double *B, int64_t ldb);
+
+/* ---- right-hand sides --------------------------------------------------- */
+/* rhs_dielectric_box3d(interface, PointSource, eps_src) (src/solver/dielectric_box3d.jl:95-104),
+   one column per charge: F[i,k] = -q_k n_i.(x_i-s_k)/(4 pi r^3 eps_src). */
+dbie_status dbie_rhs_points(dbie_op *op, int K, const double *src_xyz, const double *q, double eps_src,
+                            double *F, int64_t ldf);
+/* rhs_dielectric_box3d(interface, VolumeSource[, eps_src]) (src/solver/dielectric_box3d.jl:106-124) and
+   the Vector{VolumeSource} form (src/solver/multi_rhs.jl:168-211).  Column k uses source points
+   col_ptr[k] .. col_ptr[k+1]-1 of xyz (3 x S), w, rho.  screening = 1 applies SharpScreening
+   through the interface bounds (src/core/sources.jl:390-406) and eps_src is then 1. */
+dbie_status dbie_rhs_volume(dbie_op *op, int K, const int64_t *col_ptr, const double *xyz, const double *w,
+                            const double *rho, int screening, double eps_src, double *F, int64_t ldf);
+
+/* ---- solves ------------------------------------------------------------ */
+/* solve_dielectric_box3d_block / Krylov.block_gmres (src/solver/multi_rhs.jl:270-286) and
+   solve_gmres (src/utils/linear_algebra.jl:3-6): A Sigma = F for K columns on the GPU.
+   precision = DBIE_FP64 runs FP64 GMRES; any other value runs mixed-precision iterative
+   refinement (FP64 residuals, low-precision inner GMRES). */
+dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, double *Sigma, int64_t lds,
+                             const dbie_solve_opts *opts, dbie_solve_stats *stats);
+
+/* ---- induced potential off the surface --------------------------------- */
+/* laplace3d_pottrg / laplace3d_pottrg_fmm3d (src/kernel/laplace3d.jl:69-82,169-191):
+   U[t,k] = sum_i w_i Sigma[i,k] / (4 pi |trg_t - x_i|) for M targets (3 x M). */
+dbie_status dbie_pottrg(dbie_op *op, int64_t M, const double *trg, int K, const double *Sigma, int64_t lds,
+                        double *U, int64_t ldu);
+
+/* ---- device-pointer variants (stream-ordered on the operator's stream) ---- */
+void *dbie_op_stream(dbie_op *op);
+dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
+                              dbie_precision precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DBIE_H */

@@ -1,0 +1,380 @@
+"""CPU FP64 oracle for the boundary-operator path (test infrastructure only).
+
+This module is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import it; the product package never does.
+
+It wraps ``oracle/liboracle.so`` (a C restatement of the reference, see
+``dbie_oracle.c`` for the per-function ``file:line`` citations) with numpy, and
+adds the pieces that are plain linear algebra:
+
+* ``solve_lu``  -- restates ``src/utils/linear_algebra.jl:1`` with scipy's LAPACK LU;
+* ``gmres``     -- restates ``Krylov.gmres`` as used at ``src/utils/linear_algebra.jl:3-6``
+  (full, unrestarted GMRES, stop when ``||r|| <= atol + rtol*||b||``).  Krylov.jl's
+  iterate trajectory is not pinned by any reference test; only the converged
+  solution is compared.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.orc_gauss_legendre.argtypes = [C.c_int, _dp, _dp]
+        L.orc_best_grid.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_iface_box.restype = vp
+        L.orc_iface_box.argtypes = [C.c_double] * 3 + [C.c_int] + [C.c_double] * 4
+        L.orc_iface_from_panels.restype = vp
+        L.orc_iface_from_panels.argtypes = [C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp]
+        L.orc_iface_free.argtypes = [vp]
+        L.orc_iface_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_iface_nodes.argtypes = [vp, _dp]
+        L.orc_iface_panels.argtypes = [vp, _dp, _dp, _ip]
+        L.orc_iface_set_eps.argtypes = [vp, _dp, _dp]
+        L.orc_diag.argtypes = [vp, _dp]
+        for n in ("orc_dense_DT", "orc_dense_S", "orc_dense_D", "orc_lhs_dense"):
+            getattr(L, n).argtypes = [vp, _dp]
+        L.orc_apply_direct.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        L.orc_apply_DT_rows.argtypes = [vp, C.c_int, _ip, _dp, _dp]
+        L.orc_pottrg.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.orc_rhs_points.argtypes = [vp, C.c_int, _dp, _dp, C.c_double, _dp]
+        L.orc_rhs_volume.argtypes = [vp, C.c_int, _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_screen_sharp.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+        L.orc_bernstein_rho_panel.restype = C.c_double
+        L.orc_bernstein_rho_panel.argtypes = [vp, C.c_int, _dp]
+        L.orc_neighbor_list.restype = vp
+        L.orc_neighbor_list.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        L.orc_nlist_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_nlist_get.argtypes = [vp, _ip, _ip, _ip, _ip, _ip]
+        L.orc_nlist_free.argtypes = [vp]
+        L.orc_corrections.restype = C.c_int
+        L.orc_corrections.argtypes = [vp, vp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
+        L.orc_corrections_apply.argtypes = [vp, vp, _dp, C.c_int, _dp, _dp]
+        L.orc_laplace3d_grad.restype = C.c_double
+        L.orc_laplace3d_grad.argtypes = [_dp, _dp, _dp]
+        L.orc_laplace3d_pot.restype = C.c_double
+        L.orc_laplace3d_pot.argtypes = [_dp, _dp]
+        L.orc_bary_row.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
+        L.orc_rho_from_pole.restype = C.c_double
+        L.orc_rho_from_pole.argtypes = [C.c_double, C.c_double]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def gauss_legendre(n: int):
+    x = np.zeros(n); w = np.zeros(n)
+    if lib().orc_gauss_legendre(n, x, w) != 0:
+        raise ValueError("n must be >= 1")
+    return x, w
+
+
+def best_grid(W: float, H: float, alpha: float):
+    m = C.c_int(); n = C.c_int()
+    rc = lib().orc_best_grid(W, H, alpha, C.byref(m), C.byref(n))
+    if rc != 0:
+        raise ValueError(f"best_grid: invalid arguments (code {rc})")
+    return m.value, n.value
+
+
+def laplace3d_grad(s, t, n):
+    return lib().orc_laplace3d_grad(_f64(s), _f64(t), _f64(n))
+
+
+def laplace3d_pot(s, t):
+    return lib().orc_laplace3d_pot(_f64(s), _f64(t))
+
+
+def bary_row(x, w, xq):
+    r = np.zeros(len(x))
+    lib().orc_bary_row(len(x), _f64(x), _f64(w), float(xq), r)
+    return r
+
+
+class Interface:
+    """A DielectricInterface restated on the CPU (uniform n_quad)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise ValueError("oracle interface construction failed")
+        self.h = C.c_void_p(handle)
+        P = C.c_int(); N = C.c_int(); nq = C.c_int()
+        lib().orc_iface_sizes(self.h, C.byref(P), C.byref(N), C.byref(nq))
+        self.P, self.N, self.nq = P.value, N.value, nq.value
+
+    @classmethod
+    def box(cls, Lx, Ly, Lz, n_quad, l_ec, eps_in, eps_out, alpha=float(np.sqrt(2.0))):
+        """single_dielectric_box3d (src/shape/box3d_single.jl:49-70)."""
+        return cls(lib().orc_iface_box(Lx, Ly, Lz, n_quad, l_ec, eps_in, eps_out, alpha))
+
+    @classmethod
+    def from_panels(cls, corners, normals, n_quad, eps_in, eps_out, is_edge=None):
+        corners = _f64(corners).reshape(-1, 4, 3)
+        P = corners.shape[0]
+        ie = np.zeros(P, np.int32) if is_edge is None else np.ascontiguousarray(is_edge, dtype=np.int32)
+        return cls(lib().orc_iface_from_panels(P, n_quad, corners.ravel(), _f64(normals).ravel(), ie,
+                                               _f64(np.broadcast_to(eps_in, P)), _f64(np.broadcast_to(eps_out, P))))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.orc_iface_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def nodes(self):
+        out = np.zeros(7 * self.N)
+        lib().orc_iface_nodes(self.h, out)
+        return out.reshape(7, self.N)
+
+    def panels(self):
+        c = np.zeros(12 * self.P); n = np.zeros(3 * self.P); e = np.zeros(self.P, np.int32)
+        lib().orc_iface_panels(self.h, c, n, e)
+        return c.reshape(self.P, 4, 3), n.reshape(self.P, 3), e
+
+    def set_eps(self, eps_in, eps_out):
+        lib().orc_iface_set_eps(self.h, _f64(np.broadcast_to(eps_in, self.P)), _f64(np.broadcast_to(eps_out, self.P)))
+
+    def diag(self):
+        t = np.zeros(self.N)
+        lib().orc_diag(self.h, t)
+        return t
+
+    def _dense(self, name):
+        M = np.zeros(self.N * self.N)
+        getattr(lib(), name)(self.h, M)
+        return M.reshape(self.N, self.N).T  # stored column-major
+
+    def dense_DT(self):
+        return self._dense("orc_dense_DT")
+
+    def dense_S(self):
+        return self._dense("orc_dense_S")
+
+    def dense_D(self):
+        return self._dense("orc_dense_D")
+
+    def lhs_dense(self):
+        return self._dense("orc_lhs_dense")
+
+    def apply_direct(self, X, mode=0):
+        X = np.asarray(X, np.float64)
+        vec = X.ndim == 1
+        Xm = X.reshape(self.N, -1)
+        K = Xm.shape[1]
+        Y = np.zeros(self.N * K)
+        lib().orc_apply_direct(self.h, mode, K, _f64(Xm.T).ravel(), Y)
+        Y = Y.reshape(K, self.N).T
+        return Y[:, 0] if vec else Y
+
+    def apply_DT_rows(self, rows, x):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        y = np.zeros(len(rows))
+        lib().orc_apply_DT_rows(self.h, len(rows), rows, _f64(x), y)
+        return y
+
+    def pottrg(self, targets, X):
+        targets = _f64(targets).reshape(-1, 3)
+        X = np.asarray(X, np.float64)
+        vec = X.ndim == 1
+        Xm = X.reshape(self.N, -1)
+        M, K = targets.shape[0], Xm.shape[1]
+        out = np.zeros(M * K)
+        lib().orc_pottrg(self.h, M, targets.ravel(), K, _f64(Xm.T).ravel(), out)
+        out = out.reshape(K, M).T
+        return out[:, 0] if vec else out
+
+    def rhs_points(self, src, q, eps_src):
+        src = _f64(src).reshape(-1, 3)
+        q = _f64(np.atleast_1d(q))
+        K = src.shape[0]
+        F = np.zeros(self.N * K)
+        lib().orc_rhs_points(self.h, K, src.ravel(), q, float(eps_src), F)
+        return F.reshape(K, self.N).T
+
+    def rhs_volume(self, pos, w, rho, eps_src):
+        pos = _f64(pos).reshape(-1, 3)
+        F = np.zeros(self.N)
+        lib().orc_rhs_volume(self.h, pos.shape[0], pos.ravel(), _f64(w), _f64(rho), float(eps_src), F)
+        return F
+
+    def screen_sharp(self, pos, rho):
+        pos = _f64(pos).reshape(-1, 3)
+        out = np.zeros(pos.shape[0])
+        if lib().orc_screen_sharp(self.h, pos.shape[0], pos.ravel(), _f64(rho), out) != 0:
+            raise ValueError("screened_volume_source requires a uniform eps across the interface")
+        return out
+
+    def bernstein_rho_panel(self, p, trg):
+        return lib().orc_bernstein_rho_panel(self.h, int(p), _f64(trg))
+
+    def neighbor_list(self, max_order=8, atol=1e-9, correct_edges=True):
+        return NeighborList(self, lib().orc_neighbor_list(self.h, max_order, atol, int(bool(correct_edges))))
+
+
+class NeighborList:
+    """build_neighbor_list output (src/kernel/laplace3d_near.jl:240-392), 0-based panels."""
+
+    def __init__(self, iface, handle):
+        self.iface = iface
+        self.h = C.c_void_p(handle)
+        nu = C.c_int(); na = C.c_int()
+        lib().orc_nlist_sizes(self.h, C.byref(nu), C.byref(na))
+        self.n_up, self.n_adp = nu.value, na.value
+        self.up_i = np.zeros(self.n_up, np.int32); self.up_j = np.zeros(self.n_up, np.int32)
+        self.up_p = np.zeros(self.n_up, np.int32)
+        self.adp_i = np.zeros(self.n_adp, np.int32); self.adp_j = np.zeros(self.n_adp, np.int32)
+        lib().orc_nlist_get(self.h, self.up_i, self.up_j, self.up_p, self.adp_i, self.adp_j)
+        self.blocks = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.orc_nlist_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def upsample(self):
+        return {(int(i), int(j)): int(p) for i, j, p in zip(self.up_i, self.up_j, self.up_p)}
+
+    @property
+    def adaptive(self):
+        return {(int(i), int(j)) for i, j in zip(self.adp_i, self.adp_j)}
+
+    def corrections(self, adp_atol=1e-9, adp_rtol=float(np.sqrt(np.finfo(float).eps)), adp_nGL=0, adp_max_depth=20):
+        npp = self.iface.nq ** 2
+        nb = self.n_up + self.n_adp
+        out = np.zeros(max(nb, 1) * npp * npp)
+        self.hits = lib().orc_corrections(self.iface.h, self.h, adp_atol, adp_rtol, adp_nGL, adp_max_depth, out)
+        self.blocks = out[: nb * npp * npp].reshape(nb, npp, npp).transpose(0, 2, 1)  # [b, t, c]
+        return self.blocks
+
+    def pairs(self):
+        """(src panel, trg panel) for each block, upsampled first then adaptive."""
+        return np.concatenate([self.up_i, self.adp_i]), np.concatenate([self.up_j, self.adp_j])
+
+    def apply(self, X):
+        X = np.asarray(X, np.float64)
+        vec = X.ndim == 1
+        Xm = X.reshape(self.iface.N, -1)
+        K = Xm.shape[1]
+        blk = _f64(self.blocks.transpose(0, 2, 1)).ravel()
+        Y = np.zeros(self.iface.N * K)
+        lib().orc_corrections_apply(self.iface.h, self.h, blk, K, _f64(Xm.T).ravel(), Y)
+        Y = Y.reshape(K, self.iface.N).T
+        return Y[:, 0] if vec else Y
+
+    def sparse(self):
+        """The correction matrix as scipy CSR (rows = targets, cols = sources)."""
+        import scipy.sparse as sp
+        npp = self.iface.nq ** 2
+        src, trg = self.pairs()
+        r = (trg[:, None, None] * npp + np.arange(npp)[None, :, None]) + 0 * np.arange(npp)[None, None, :]
+        c = (src[:, None, None] * npp + np.arange(npp)[None, None, :]) + 0 * np.arange(npp)[None, :, None]
+        return sp.csr_matrix((self.blocks.ravel(), (r.ravel(), c.ravel())), shape=(self.iface.N, self.iface.N))
+
+
+class CorrectedOperator:
+    """y = Dᵀ x (direct sum) + C x + t∘x : the FP64 operator of
+    lhs_dielectric_box3d_fmm3d_corrected / BatchedDielectricOperator
+    (src/solver/dielectric_box3d.jl:48-92, src/solver/multi_rhs.jl:60-130) with the
+    FMM replaced by its exact direct sum."""
+
+    def __init__(self, iface, max_order=8, up_tol=1e-9, correct_edges=True, adaptive_atol=None,
+                 adaptive_rtol=float(np.sqrt(np.finfo(float).eps)), adaptive_n_GL=0, adaptive_max_depth=20):
+        self.iface = iface
+        self.nl = iface.neighbor_list(max_order, up_tol, correct_edges)
+        self.nl.corrections(up_tol if adaptive_atol is None else adaptive_atol, adaptive_rtol, adaptive_n_GL,
+                            adaptive_max_depth)
+        self.Cs = self.nl.sparse()
+        self.t = iface.diag()
+        self.N = iface.N
+
+    def __matmul__(self, X):
+        X = np.asarray(X, np.float64)
+        Y = self.iface.apply_direct(X, 0)
+        Y = Y + self.Cs @ X + (self.t[:, None] * X if X.ndim == 2 else self.t * X)
+        return Y
+
+
+def solve_lu(A, b):
+    """solve_lu (src/utils/linear_algebra.jl:1): LAPACK LU with partial pivoting."""
+    import scipy.linalg as sl
+    return sl.lu_solve(sl.lu_factor(A), b)
+
+
+def gmres(apply, b, rtol=1e-10, atol=0.0, itmax=500, M=None):
+    """Full (unrestarted) GMRES with modified Gram-Schmidt and Givens rotations.
+    ``M`` is an optional right preconditioner (vector of diagonal entries)."""
+    b = np.asarray(b, np.float64)
+    n = b.shape[0]
+    beta = np.linalg.norm(b)
+    if beta == 0.0:
+        return np.zeros(n), 0, 0.0
+    tol = atol + rtol * beta
+    V = np.zeros((itmax + 1, n))
+    H = np.zeros((itmax + 1, itmax))
+    cs = np.zeros(itmax); sn = np.zeros(itmax)
+    g = np.zeros(itmax + 1); g[0] = beta
+    V[0] = b / beta
+    k = 0
+    res = beta
+    for k in range(itmax):
+        z = V[k] * M if M is not None else V[k]
+        w = apply(z)
+        for i in range(k + 1):
+            H[i, k] = V[i] @ w
+            w = w - H[i, k] * V[i]
+        H[k + 1, k] = np.linalg.norm(w)
+        if H[k + 1, k] != 0.0:
+            V[k + 1] = w / H[k + 1, k]
+        for i in range(k):
+            tmp = cs[i] * H[i, k] + sn[i] * H[i + 1, k]
+            H[i + 1, k] = -sn[i] * H[i, k] + cs[i] * H[i + 1, k]
+            H[i, k] = tmp
+        d = np.hypot(H[k, k], H[k + 1, k])
+        cs[k] = H[k, k] / d; sn[k] = H[k + 1, k] / d
+        H[k, k] = d; H[k + 1, k] = 0.0
+        g[k + 1] = -sn[k] * g[k]; g[k] = cs[k] * g[k]
+        res = abs(g[k + 1])
+        if res <= tol:
+            k += 1
+            break
+    else:
+        k = itmax
+    y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+    x = V[:k].T @ y
+    if M is not None:
+        x = x * M
+    return x, k, res / beta
