@@ -82,10 +82,18 @@ struct DBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Host-to-device copy that completes before returning (host sources may be temporaries).
 template <class T>
 void upload(DBuf &b, const T *src, size_t n, hipStream_t st) {
     b.ensure(std::max<size_t>(n, 1) * sizeof(T));
-    if (n) DBIE_HIP(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+    if (n) {
+        DBIE_HIP(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+    }
+}
+void h2d(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    DBIE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    DBIE_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -106,7 +114,7 @@ struct dbie_op {
     DevBSR bsr;
     int adaptive_hits = 0;
     // workspaces (grown on demand)
-    DBuf part, xin, yout, tmp;
+    DBuf part, xin, yout, tmp, mscratch;
     ~dbie_op() {
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -187,7 +195,10 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
     op->part.ensure(need * sizeof(double));
     int S;
     if ((prec == DBIE_BF16 || prec == DBIE_BF16X3 || prec == DBIE_FP64) && far_field_mfma_supported((int)prec, K))
-        S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need);
+    {
+        op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
+        S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p);
+    }
     else if (prec == DBIE_FP64)
         S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
     else
@@ -258,7 +269,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         if (!any) { res.solved = true; break; }
         if (res.iters >= itmax) break;
         // V0 = r / beta
-        DBIE_HIP(hipMemcpyAsync(nbuf.p, beta.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+        h2d(nbuf.p, beta.data(), sizeof(double) * K, st);
         col_scale(st, Wp, nbuf.as<double>(), true, Vp, N, K);
         std::fill(gv.begin(), gv.end(), 0.0);
         for (int k = 0; k < K; ++k) gv[(size_t)k * (m + 1)] = active[k] ? beta[k] : 0.0;
@@ -281,7 +292,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
             DBIE_HIP(hipMemcpyAsync(h2.data(), nbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
             std::vector<double> hn;
             col_norms(op, Wp, N, K, partial, nbuf, hn);  // synchronizes the stream
-            DBIE_HIP(hipMemcpyAsync(nbuf.p, hn.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+            h2d(nbuf.p, hn.data(), sizeof(double) * K, st);
             col_scale(st, Wp, nbuf.as<double>(), true, Vp + (size_t)(j + 1) * NK, N, K);
             ++res.iters;
             bool all = true;
@@ -327,8 +338,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         }
         int nmax = 0;
         for (int k = 0; k < K; ++k) nmax = std::max(nmax, nconv[k]);
-        DBIE_HIP(hipMemcpyAsync(hbuf.p, yv.data(), sizeof(double) * (size_t)std::max(nmax, 1) * K,
-                                hipMemcpyHostToDevice, st));
+        h2d(hbuf.p, yv.data(), sizeof(double) * (size_t)std::max(nmax, 1) * K, st);
         DBIE_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * NK, st));
         block_combine(st, Vp, nmax, hbuf.as<double>(), U.as<double>(), N, K);
         if (dinv) row_scale(st, dinv, U.as<double>(), U.as<double>(), N, K);
@@ -622,7 +632,8 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
         int64_t b = 0;
         for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
                               &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
-                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->part, &op->xin, &op->yout, &op->tmp})
+                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->part, &op->xin, &op->yout, &op->tmp,
+                              &op->mscratch})
             b += (int64_t)d->bytes;
         *bytes = b;
     });
@@ -898,7 +909,7 @@ dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, d
                 if (it >= o.max_refine || s.niter >= o.itmax) break;
                 // zero the columns that already meet the tolerance
                 hb.ensure(sizeof(double) * K);
-                DBIE_HIP(hipMemcpyAsync(hb.p, mask.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+                h2d(hb.p, mask.data(), sizeof(double) * K, st);
                 col_scale(st, dR.as<double>(), hb.as<double>(), false, dR.as<double>(), N, K);
                 const GmresOut r = gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
                                          o.itmax - s.niter, o.restart, dinvp);

@@ -38,8 +38,11 @@ int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int6
                   int64_t part_cap);
 // MFMA contraction of the DT far field: tiles generated on the fly and contracted with the
 // RHS block.  prec: 0 = FP64 MFMA, 2 = BF16 MFMA, 3 = split BF16 (3 products).
+// scratch: far_field_mfma_scratch_bytes(N, prec, K) bytes of device memory (the
+// pre-scaled RHS block w.*X in FP64, or its tile-major bf16 split).
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap);
+                   int64_t part_cap, void *scratch);
+size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K);
 bool far_field_mfma_supported(int prec, int K);
 int64_t far_field_part_elems(int64_t N, int K);  // workspace needed for any far-field launch
 

@@ -1,0 +1,180 @@
+"""GPU parity of libdbie against the CPU oracle (run with -m gpu on the MI355X box).
+
+Bars: 1e-12 relative for operator applies and blocks, 1e-10 for FP64 solutions,
+1e-8 for mixed-precision solutions against the FP64 solution."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CFG1 = (1.2, 0.8, 0.6, 4, 0.2)
+
+
+def say(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.fixture(scope="module")
+def cfg1(dbie, O):
+    say("build config-1 operators (N=2560)")
+    g = dbie.single_dielectric_box3d(*CFG1, 4.0, 1.0)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
+    ref_if = O.Interface.box(*CFG1, 4.0, 1.0)
+    ref = O.CorrectedOperator(ref_if)
+    return g, op, ref_if, ref
+
+
+def test_corrected_apply_parity(cfg1):
+    g, op, ref_if, ref = cfg1
+    say("corrected apply K=1,5 vs oracle")
+    rng = np.random.default_rng(0)
+    for K in (1, 5):
+        X = rng.standard_normal((g.n_points, K))
+        assert rel(op @ X, ref @ X) < 1e-12
+    x = np.linspace(0.1, 1.0, g.n_points)  # test/solver/multi_rhs.jl:47-57
+    assert rel(op @ x, ref @ x) < 1e-12
+
+
+def test_correction_blocks_parity(cfg1):
+    g, op, ref_if, ref = cfg1
+    say("correction blocks vs oracle")
+    src, trg, blk = op.corrections()
+    s2, t2 = ref.nl.pairs()
+    B2 = ref.nl.blocks
+    order = np.lexsort((s2, t2))  # product stores block rows by target, then source
+    assert np.array_equal(src, s2[order]) and np.array_equal(trg, t2[order])
+    scale = np.abs(B2).max()
+    assert np.abs(blk - B2[order]).max() <= 1e-13 * scale
+
+
+def test_plain_kernels_parity(dbie, O, cfg1):
+    g, op, ref_if, ref = cfg1
+    say("plain DT/S/D apply vs oracle dense")
+    X = np.random.default_rng(1).standard_normal((g.n_points, 3))
+    for name, M in (("DT", ref_if.dense_DT()), ("S", ref_if.dense_S()), ("D", ref_if.dense_D())):
+        assert rel(op.apply_kernel(X, name), M @ X) < 1e-12, name
+
+
+def test_dense_lhs_and_lu(dbie, O, cfg1):
+    g, op, ref_if, ref = cfg1
+    say("dense lhs + rocSOLVER LU (config 1, README quickstart)")
+    A = dbie.lhs_dielectric_box3d(g)
+    A_ref = ref_if.lhs_dense()
+    assert np.abs(A - A_ref).max() <= 1e-13 * np.abs(A_ref).max()
+    b = dbie.rhs_dielectric_box3d(g, dbie.PointSource((0.1, 0.1, 0.1), 1.0), 4.0)
+    b_ref = ref_if.rhs_points([0.1, 0.1, 0.1], [1.0], 4.0)[:, 0]
+    assert rel(b, b_ref) < 1e-13
+    x = dbie.solve_lu(A, b)
+    x_ref = O.solve_lu(A_ref, b_ref)
+    assert rel(x, x_ref) < 1e-10
+    assert np.linalg.norm(A_ref @ x - b_ref) < 1e-10
+    assert abs(np.dot(g.weights, x) + 0.25 - 1.0) < 0.1
+
+
+def test_fp64_solve_matches_oracle(dbie, O, cfg1):
+    g, op, ref_if, ref = cfg1
+    say("FP64 GMRES on the corrected operator vs oracle LU")
+    F = dbie.rhs_point_charges(g, [[0.1, 0.1, 0.1], [-0.2, 0.05, 0.1]], [1.0, -0.5], 4.0)
+    Sig, st = op.solve(F, rtol=1e-12)
+    assert st["solved"] and st["max_rel_residual"] < 1e-11
+    Ad = ref_if.dense_DT() + ref.Cs.toarray() + np.diag(ref.t)
+    S_ref = O.solve_lu(Ad, ref_if.rhs_points([[0.1, 0.1, 0.1], [-0.2, 0.05, 0.1]], [1.0, -0.5], 4.0))
+    for k in range(2):
+        assert rel(Sig[:, k], S_ref[:, k]) < 1e-10
+
+
+def test_block_equals_per_column(dbie, cfg1):
+    # test/solver/multi_rhs.jl:59-66 and multi_rhs_vector.jl:38-48
+    g, op, ref_if, ref = cfg1
+    say("block solve == per-column solves")
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-0.25, 0.25, (4, 3))
+    F = dbie.rhs_point_charges(g, pts, rng.uniform(-1, 1, 4), 4.0)
+    S, _ = op.solve(F, rtol=1e-12)
+    for k in range(4):
+        sk, _ = op.solve(F[:, k], rtol=1e-12)
+        assert rel(S[:, k], sk) < 1e-10
+
+
+def test_mfma_apply_parity(cfg1):
+    """K >= 16 routes the far field through the MFMA kernels (FP64: v_mfma_f64_16x16x4)."""
+    g, op, ref_if, ref = cfg1
+    rng = np.random.default_rng(11)
+    for K in (16, 40, 256):
+        say(f"MFMA FP64 apply K={K} vs oracle")
+        X = rng.standard_normal((g.n_points, K))
+        Y = op @ X
+        Yr = ref @ X
+        for k in (0, K // 2, K - 1):
+            assert rel(Y[:, k], Yr[:, k]) < 1e-12
+        assert rel(Y, Yr) < 1e-12
+    X = rng.standard_normal((g.n_points, 48))
+    Y64 = op @ X
+    for prec, bound in (("fp32", 1e-5), ("bf16", 2e-2), ("bf16x3", 2e-4)):
+        say(f"low-precision apply {prec} K=48")
+        Yp = op.apply(X, prec)
+        for k in range(48):
+            assert rel(Yp[:, k], Y64[:, k]) < bound, (prec, k)
+
+
+@pytest.mark.parametrize("precision,K", [("fp32", 8), ("bf16", 8), ("bf16", 16), ("bf16x3", 24), ("fp32", 20)])
+def test_mixed_precision_solve(dbie, cfg1, precision, K):
+    g, op, ref_if, ref = cfg1
+    say(f"mixed-precision IR ({precision}, K={K}) vs FP64 solve")
+    rng = np.random.default_rng(4)
+    F = dbie.rhs_point_charges(g, rng.uniform(-0.25, 0.25, (K, 3)), rng.uniform(-1, 1, K), 4.0)
+    S64, _ = op.solve(F, rtol=1e-12)
+    Sm, st = op.solve(F, rtol=1e-11, precision=precision)
+    assert st["solved"] and st["nrefine"] >= 1
+    for k in range(K):
+        assert rel(Sm[:, k], S64[:, k]) < 1e-8
+
+
+def test_rhs_volume_screened(dbie, O, cfg1):
+    g, op, ref_if, ref = cfg1
+    say("volume RHS with sharp screening vs oracle")
+    rng = np.random.default_rng(5)
+    vss, cols = [], []
+    for k in range(3):
+        pos = rng.uniform(-0.8, 0.8, (50 + 10 * k, 3))
+        w = rng.uniform(0.1, 1.0, len(pos)); rho = rng.standard_normal(len(pos))
+        vss.append(dbie.VolumeSource(pos.T, w, rho))
+        cols.append(ref_if.rhs_volume(pos, w, ref_if.screen_sharp(pos, rho), 1.0))
+    F = dbie.rhs_dielectric_box3d_fmm3d(g, vss, 1e-9)
+    for k in range(3):
+        assert rel(F[:, k], cols[k]) < 1e-12
+    # unscreened single source with eps_src
+    f1 = dbie.rhs_dielectric_box3d(g, vss[0], 2.0)
+    assert rel(f1, ref_if.rhs_volume(vss[0].positions.T, vss[0].weights, vss[0].density, 2.0)) < 1e-12
+
+
+def test_pottrg_parity(dbie, O, cfg1):
+    g, op, ref_if, ref = cfg1
+    say("pottrg vs oracle")
+    rng = np.random.default_rng(6)
+    trg = rng.uniform(-0.5, 0.5, (300, 3))
+    Sg = rng.standard_normal((g.n_points, 2))
+    assert rel(op.pottrg(trg, Sg), ref_if.pottrg(trg, Sg)) < 1e-12
+
+
+def test_config2_sampled_rows(dbie, O):
+    say("config-2 size (N=64384) corrected apply, sampled rows vs oracle")
+    box = (2.4, 1.6, 1.2, 4, 0.05)
+    g = dbie.single_dielectric_box3d(*box, 10.0, 1.0)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
+    ref_if = O.Interface.box(*box, 10.0, 1.0)
+    nl = ref_if.neighbor_list(8, 1e-9, True)
+    nl.corrections()
+    x = np.random.default_rng(7).standard_normal(g.n_points)
+    y = op @ x
+    rows = np.sort(np.random.default_rng(8).choice(g.n_points, 512, replace=False)).astype(np.int32)
+    y_ref = ref_if.apply_DT_rows(rows, x) + nl.apply(x)[rows] + ref_if.diag()[rows] * x[rows]
+    assert rel(y[rows], y_ref) < 1e-12
+    assert op.n_blocks == 196768 + 4080
