@@ -1,0 +1,177 @@
+"""Benchmark: batched many-RHS solves of the single dielectric box on one MI355X.
+
+Workload (the configuration the metric is quoted on): box 2.4 x 1.6 x 1.2, l_ec = 0.05,
+n_quad = 4 (N = 64,384 unknowns), eps_in = 4, eps_out = 1, K = 256 point charges placed
+uniformly at random inside the box at least 0.05 from every face, q ~ U(-1, 1) (seeded).
+Corrected operator with the block-solve defaults (up_tol = 1e-9, max_order = 8,
+correct_edges = true), right diagonal preconditioner, rtol = 1e-10.
+
+A step is one block solve of all K right-hand sides, inputs resident in HBM:
+  * mixed:  FP64 iterative refinement around GMRES on the split-BF16 MFMA operator;
+  * fp64:   the plain FP64 GMRES path (the comparator), timed beside it.
+value = K * steps / (time of the timed steps) for the mixed path, in RHS-solves/s.
+
+Prints ONE JSON line.  Extra fields: the FP64 comparator rate, the mixed/FP64 ratio,
+p50/p99 step latency, per-RHS max relative error of the mixed solution vs the FP64
+solution, the roofline of the dominant kernel (HIP events on the operator's stream) and
+the CPU baseline (the oracle on this host's cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dielectricbie.jl_amd"))
+
+METRIC = "RHS-solves/s at ≤1e-8 rel err vs FP64, mixed vs plain FP64 GPU; p50/p99 latency"
+BOX = (2.4, 1.6, 1.2)
+L_EC, NQ, EPS_IN, EPS_OUT = 0.05, 4, 4.0, 1.0
+PEAK = {"fp64": 78.6, "bf16": 2500.0, "hbm": 8000.0}  # TFLOP/s, TFLOP/s, GB/s
+
+
+def charges(K: int, seed: int = 0, margin: float = 0.05):
+    rng = np.random.default_rng(seed)
+    half = np.array(BOX) / 2 - margin
+    return rng.uniform(-half, half, (K, 3)), rng.uniform(-1.0, 1.0, K)
+
+
+def cross_plane_pairs(iface) -> int:
+    """Nonzero Dᵀ pairs: N^2 minus same-plane pairs (the self pair is same-plane)."""
+    nd = iface.nodes
+    key = np.round(np.vstack([nd[3:6], (nd[3:6] * nd[0:3]).sum(0)]).T, 12)
+    _, counts = np.unique(key, axis=0, return_counts=True)
+    N = iface.n_points
+    return int(N * N - np.sum(counts.astype(np.int64) ** 2))
+
+
+def cpu_baseline(iters_fp64: int, ksample: int = 8):
+    """Time the oracle's FP64 corrected matvec on all host cores for a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    f = O.Interface.box(*BOX, NQ, L_EC, EPS_IN, EPS_OUT)
+    A = O.CorrectedOperator(f)
+    X = np.random.default_rng(1).standard_normal((f.N, ksample))
+    t0 = time.perf_counter()
+    A @ X
+    dt = time.perf_counter() - t0
+    rate = ksample / (dt * max(iters_fp64, 1))
+    return {"value": rate, "unit": "RHS-solves/s", "cores": int(O.lib().orc_num_threads()), "kind": "port",
+            "sample": f"one FP64 corrected matvec (direct-sum far field + near corrections) of {ksample} columns "
+                      f"at N={f.N} took {dt:.2f} s; rate = {ksample} / (t x {iters_fp64} GMRES iterations of the "
+                      f"FP64 path)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--K", type=int, default=256)
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16", "bf16x3"])
+    ap.add_argument("--inner-rtol", type=float, default=1e-5)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X; the path shards by RHS "
+              "columns or requests across GPUs (no collectives) and is not launched at N > 1", file=sys.stderr)
+        sys.exit(2)
+
+    import dbie
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    iface = dbie.single_dielectric_box3d(*BOX, NQ, L_EC, EPS_IN, EPS_OUT)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(iface, 1e-9, 1e-9, 8)
+    t_setup = time.perf_counter() - t0
+    pts, q = charges(a.K, a.seed)
+    F = dbie.rhs_point_charges(iface, pts, q, EPS_IN)
+    N = iface.n_points
+    log(f"setup N={N} blocks={op.n_blocks} {t_setup:.2f}s")
+
+    # plain FP64 comparator (also the reference solution for the error)
+    fp64_times, S64 = [], None
+    for it in range(1 + a.fp64_steps):
+        S64, st64 = op.solve(F, rtol=1e-10, precision="fp64", check_residual=False)
+        if it > 0:
+            fp64_times.append(st64["time_ms"])
+        log(f"fp64 solve iters={st64['niter']} {st64['time_ms']:.1f} ms")
+    fp64_ms = float(np.median(fp64_times))
+    iters64 = int(st64["niter"])
+
+    solve_kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol)
+    for _ in range(a.warmup):
+        op.solve(F, **solve_kw)
+    op.profile(True)
+    op.profile_read()
+    lat, Sm, st = [], None, None
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        t1 = time.perf_counter()
+        Sm, st = op.solve(F, **solve_kw)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    wall = time.perf_counter() - t_start
+    prof = op.profile_read()
+    op.profile(False)
+    log(f"mixed {a.precision}: iters={st['niter']} refine={st['nrefine']} {np.median(lat):.1f} ms/step; prof={prof}")
+
+    err = np.linalg.norm(Sm - S64, axis=0) / np.linalg.norm(S64, axis=0)
+    value = a.K * a.steps / wall
+    fp64_value = a.K / (fp64_ms * 1e-3)
+
+    # roofline of the dominant kernel (largest device time in the timed region)
+    pairs = cross_plane_pairs(iface)
+    name, (launches, total_ms) = max(prof.items(), key=lambda kv: kv[1][1])
+    avg_ms = total_ms / max(launches, 1)
+    if name == "near_combine":
+        nb, bd = op.n_blocks, op.block_dim
+        bytes_ = nb * bd * bd * 4 + 8 * N * a.K * 3 + 4 * nb  # FP32 blocks, X read twice-ish, Y write
+        achieved = bytes_ / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm"], "unit": "GB/s",
+                "frac": achieved / PEAK["hbm"], "traffic": None, "kernel": name}
+    else:
+        flops = 2.0 * pairs * a.K
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        peak = PEAK["fp64"] if name == "far_fp64" else PEAK["bf16"]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": None, "kernel": ("far_mfma_f64" if name == "far_fp64" else f"far_mfma_{a.precision}"),
+                "avg_launch_ms": avg_ms, "launches": launches, "algorithmic_flops_per_launch": flops}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        tr = json.load(open(traffic_file)).get(roof.get("kernel"))
+        if tr:
+            roof["traffic"] = tr
+    roof["kernels_ms"] = {k: v[1] for k, v in prof.items()}
+
+    cpu = None if a.no_cpu else cpu_baseline(iters64)
+    out = {
+        "metric": METRIC, "value": value, "unit": "RHS-solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": float(np.mean(lat)), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": a.precision, "data": "synthetic",
+        "config": {"workload": "single box 2.4x1.6x1.2, l_ec=0.05, n_quad=4, eps_in=4, K=256 point charges, "
+                               "block solve to rtol 1e-10 (mixed-precision IR)",
+                   "N": N, "K": a.K, "near_blocks": op.n_blocks, "inner_rtol": a.inner_rtol,
+                   "parallelism": "single GPU"},
+        "fp64_value": fp64_value, "fp64_ms_per_step": fp64_ms, "fp64_iters": iters64,
+        "speedup_vs_fp64": value / fp64_value,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "max_rel_err_vs_fp64": float(err.max()), "median_rel_err_vs_fp64": float(np.median(err)),
+        "mixed_inner_iters": st["niter"], "mixed_refinements": st["nrefine"],
+        "mixed_max_rel_residual": st["max_rel_residual"], "setup_s": t_setup,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

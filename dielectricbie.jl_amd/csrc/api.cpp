@@ -115,7 +115,15 @@ struct dbie_op {
     int adaptive_hits = 0;
     // workspaces (grown on demand)
     DBuf part, xin, yout, tmp, mscratch;
+    // optional per-kernel timing: event pairs recorded around launches
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[3];
     ~dbie_op() {
+        for (auto &v : prof)
+            for (auto &e : v) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
         if (stream) (void)hipStreamDestroy(stream);
     }
     int64_t N() const { return g.N; }
@@ -188,23 +196,46 @@ void compute_diag(dbie_op *op, const double *eps_in, const double *eps_out) {
     upload(op->diag, op->diag_h.data(), g.N, op->stream);
 }
 
+// Event pair around a launch when profiling is on (kernel id: 0 far FP64, 1 far low precision, 2 combine).
+struct ProfScope {
+    dbie_op *op;
+    int id;
+    hipEvent_t e1 = nullptr;
+    ProfScope(dbie_op *o, int k) : op(o), id(k) {
+        if (!op->profiling) return;
+        hipEvent_t e0;
+        DBIE_HIP(hipEventCreate(&e0));
+        DBIE_HIP(hipEventCreate(&e1));
+        DBIE_HIP(hipEventRecord(e0, op->stream));
+        op->prof[id].push_back({e0, e1});
+    }
+    ~ProfScope() {
+        if (e1) (void)hipEventRecord(e1, op->stream);
+    }
+};
+
 // Y = A X on the device (X, Y: N x K with leading dims).
 void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
     const int64_t N = op->N();
     const int64_t need = far_field_part_elems(N, K);
     op->part.ensure(need * sizeof(double));
     int S;
-    if ((prec == DBIE_BF16 || prec == DBIE_BF16X3 || prec == DBIE_FP64) && far_field_mfma_supported((int)prec, K))
     {
-        op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
-        S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p);
+        ProfScope ps(op, prec == DBIE_FP64 ? 0 : 1);
+        if ((prec == DBIE_BF16 || prec == DBIE_BF16X3 || prec == DBIE_FP64) && far_field_mfma_supported((int)prec, K)) {
+            op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
+            S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p);
+        } else if (prec == DBIE_FP64) {
+            S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
+        } else {
+            S = far_field_f32(op->stream, op->dg, K, X, ldx, op->part.as<double>(), need);
+        }
     }
-    else if (prec == DBIE_FP64)
-        S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
-    else
-        S = far_field_f32(op->stream, op->dg, K, X, ldx, op->part.as<double>(), need);
-    near_combine(op->stream, op->bsr, prec != DBIE_FP64, S, op->part.as<double>(), N, K, X, ldx,
-                 op->diag.as<double>(), Y, ldy);
+    {
+        ProfScope ps(op, 2);
+        near_combine(op->stream, op->bsr, prec != DBIE_FP64, S, op->part.as<double>(), N, K, X, ldx,
+                     op->diag.as<double>(), Y, ldy);
+    }
     DBIE_HIP(hipGetLastError());
 }
 
@@ -557,6 +588,16 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
             bs.row_ptr.assign(g.P + 1, 0);
         }
         op->adaptive_hits = bs.adaptive_hits;
+        const bool mfma_order = bs.bd == 16;
+        if (mfma_order) {  // permute each block into MFMA operand order
+            std::vector<double> tmpb(256);
+            for (int64_t b = 0; b < bs.nblocks(); ++b) {
+                double *blk = &bs.val[(size_t)b * 256];
+                for (int t = 0; t < 16; ++t)
+                    for (int c = 0; c < 16; ++c) tmpb[bsr_storage_index(16, true, t, c)] = blk[t * 16 + c];
+                std::copy(tmpb.begin(), tmpb.end(), blk);
+            }
+        }
         upload(op->bsr_row, bs.row_ptr.data(), bs.row_ptr.size(), st);
         upload(op->bsr_col, bs.col.data(), bs.col.size(), st);
         upload(op->bsr_v64, bs.val.data(), bs.val.size(), st);
@@ -583,6 +624,7 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         c.col = op->bsr_col.as<int>();
         c.val64 = op->bsr_v64.as<double>();
         c.val32 = op->bsr_v32.as<float>();
+        c.mfma_order = mfma_order;
         DBIE_HIP(hipStreamSynchronize(st));
         *out = op.release();
     });
@@ -620,8 +662,15 @@ dbie_status dbie_op_corrections(const dbie_op *op, int *src, int *trg, double *b
                 if (src) src[b] = col[b];
                 if (trg) trg[b] = p;
             }
-        if (blocks && c.nb)
-            DBIE_HIP(hipMemcpy(blocks, c.val64, sizeof(double) * c.nb * c.bd * c.bd, hipMemcpyDeviceToHost));
+        if (blocks && c.nb) {
+            const size_t bb = (size_t)c.bd * c.bd;
+            std::vector<double> raw(c.nb * bb);
+            DBIE_HIP(hipMemcpy(raw.data(), c.val64, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+            for (int64_t b = 0; b < c.nb; ++b)
+                for (int t = 0; t < c.bd; ++t)
+                    for (int cc = 0; cc < c.bd; ++cc)
+                        blocks[b * bb + (size_t)t * c.bd + cc] = raw[b * bb + bsr_storage_index(c.bd, c.mfma_order, t, cc)];
+        }
     });
 }
 
@@ -640,6 +689,32 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
 }
 
 void *dbie_op_stream(dbie_op *op) { return op ? (void *)op->stream : nullptr; }
+
+dbie_status dbie_op_profile(dbie_op *op, int enable) {
+    return guard([&] {
+        check_op(op);
+        op->profiling = enable != 0;
+    });
+}
+
+dbie_status dbie_op_profile_read(dbie_op *op, int kernel, int64_t *launches, double *total_ms) {
+    return guard([&] {
+        check_op(op);
+        if (kernel < 0 || kernel > 2) fail(DBIE_BAD_ARG, "kernel id must be 0, 1 or 2");
+        DBIE_HIP(hipStreamSynchronize(op->stream));
+        double ms = 0.0;
+        for (auto &e : op->prof[kernel]) {
+            float t = 0.f;
+            DBIE_HIP(hipEventElapsedTime(&t, e.first, e.second));
+            ms += t;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        if (launches) *launches = (int64_t)op->prof[kernel].size();
+        if (total_ms) *total_ms = ms;
+        op->prof[kernel].clear();
+    });
+}
 
 dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
                               dbie_precision precision) {

@@ -1,22 +1,24 @@
 // far_mfma.hip -- far-field Dᵀ x RHS-block contraction on the CDNA4 matrix cores.
 //
 // Y[i,k] = sum_j G(i,j) Z[j,k],  G(i,j) = n_i.(x_i - y_j) / r^3 (self pair 0),  Z = w .* X.
-// The kernel matrix G is never stored: every wave generates its A-operand tile
-// on VALU, directly in the MFMA register layout, and contracts it with the
-// RHS block staged in LDS.  One lane owns one (target, source) entry per MFMA
-// k-slice, so generation needs no LDS round trip.
+// The kernel matrix G is never stored in HBM.  A workgroup of 8 waves owns 128 targets
+// (eight 16-target slices) and all columns of its column group.  Per source stage:
+//   1. wave w generates G for target slice w on VALU and writes it to a double-buffered
+//      LDS tile laid out exactly as the MFMA A operand (one 16-byte / 8-byte chunk per lane);
+//   2. one barrier;
+//   3. every wave contracts the target slices it owns with its own 16-column blocks; the
+//      B operand (w.*X) is read from a tile-major copy in L2, prefetched one stage ahead.
+// One barrier per stage; the slice written at stage s+1 goes to the other buffer.
 //
-//   FP64: v_mfma_f64_16x16x4_f64   (A: 16 targets x 4 sources, lane l -> (l%16, l/16))
-//   BF16: v_mfma_f32_16x16x32_bf16 (A: 16 targets x 32 sources, lane l -> (l%16, 8*(l/16)+0..7))
-//         with FP32 generation and FP32 accumulation; the BF16X3 form adds the
-//         hi*lo + lo*hi cross products of a two-term bf16 split of G and Z.
-// D (16 x 16) lands in lane l, register r at col l%16 and row 4*r + l/16 for the FP64
-// op but row 4*(l/16) + r for the BF16 op (both measured with a one-hot probe on gfx950).
+//   FP64: v_mfma_f64_16x16x4_f64, stage = 16 sources (4 k-slices of 4)
+//   BF16: v_mfma_f32_16x16x32_bf16, stage = 64 sources (2 k-slices of 32), FP32 generation,
+//         FP32 accumulation; the BF16X3 form adds hi*lo + lo*hi of a two-term bf16 split.
+// Operand layouts on gfx950 (measured with a one-hot probe): A lane l = (row l%16,
+// k-group l/16); B lane l = (k-group l/16, col l%16); D col = l%16, row = 4*r + l/16 for
+// the FP64 op and 4*(l/16) + r for the BF16 op.
 //
-// Workgroup: 4 waves x 32 targets = 128 targets; a 16-column block per MFMA, NCB blocks
-// per workgroup.  Source tiles whose nodes all share the target block's plane are
-// skipped (exact zeros).  Each source split writes its own partial sum (transposed
-// through LDS so that the stores are coalesced along targets).
+// Source tiles whose nodes all share the target block's plane are skipped (exact zeros).
+// Each source split (gridDim.z) writes its own FP64 partial sum.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,248 +33,308 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr double kInv4Pi = 1.0 / (4.0 * 3.14159265358979311599796346854);
-constexpr int kTargets = 128;  // targets per workgroup
-constexpr int kOutStride = kTargets + 4;
+constexpr int kWaves = 8;
+constexpr int kThreads = kWaves * 64;
+constexpr int kTargets = kWaves * 16;  // 128 targets per workgroup
+
+// Work items per stage are (target slice, 16-column block).  With NCB >= 8 blocks each
+// wave owns NCB/8 column blocks and all 8 slices; with NCB < 8 each column block is
+// shared by 8/NCB waves, each owning NCB of the slices.
+template <int NCB>
+struct Items {
+    static constexpr int kCbPerWave = NCB >= kWaves ? NCB / kWaves : 1;
+    static constexpr int kSlicesPerWave = NCB >= kWaves ? kWaves : NCB;
+    static constexpr int kN = kCbPerWave * kSlicesPerWave;
+    __device__ static int cb(int wave, int q) { return NCB >= kWaves ? wave * kCbPerWave + q : wave % NCB; }
+    __device__ static int slice(int wave, int s) { return NCB >= kWaves ? s : (wave / NCB) + s * (kWaves / NCB); }
+};
 
 // ---------------------------------------------------------------- FP64
-constexpr int kTS64 = 32;         // sources per LDS stage
-constexpr int kZS64 = kTS64 + 4;  // row stride (doubles) of the [col][source] tile
+constexpr int kS64 = 16;  // sources per stage
+
+// Tile-major w.*X: Zt[((j/16)*Kp + col)*16 + j%16]
+__global__ void scale_tiles_f64(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
+                                int K, int Kp, double *__restrict__ Zt) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    const int64_t ntile = (N + kS64 - 1) / kS64;
+    if (j >= ntile * kS64) return;
+    const double z = (j < N && k < K) ? w[j] * X[j + (int64_t)k * ldx] : 0.0;
+    Zt[((j / kS64) * Kp + k) * kS64 + (j % kS64)] = z;
+}
 
 template <int NCB>
-__global__ __launch_bounds__(256) void far_mfma_f64_kernel(DevGeom g, int K, const double *__restrict__ Z,
-                                                           double *__restrict__ part, int src_per_split) {
-    constexpr int CB = NCB * 16;
-    constexpr int kSmem = (CB * kZS64 > 16 * kOutStride) ? CB * kZS64 : 16 * kOutStride;
-    __shared__ double4 s_pos[kTS64];
-    __shared__ double s_mem[kSmem];
+__global__ __launch_bounds__(kThreads) void far_mfma_f64_kernel(DevGeom g, int K, int Kp, const double *__restrict__ Zt,
+                                                                double *__restrict__ part, int src_per_split) {
+    using I = Items<NCB>;
+    constexpr int KS = kS64 / 4;
+    __shared__ double s_g[2][kWaves][KS][64];  // [buf][slice][k-slice][lane]
+    __shared__ double4 s_pos[2][kS64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t N = g.N;
     const int64_t i0 = (int64_t)blockIdx.x * kTargets;
-    const int k0 = blockIdx.y * CB;
+    const int k0 = blockIdx.y * NCB * 16;
     const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
     const int64_t jend = min(N, jb0 + src_per_split);
 
-    double tx[2], ty[2], tz[2], tn0[2], tn1[2], tn2[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        int64_t t = i0 + wave * 32 + a * 16 + (lane & 15);
-        t = t < N ? t : N - 1;
-        tx[a] = g.x[t]; ty[a] = g.y[t]; tz[a] = g.z[t];
-        tn0[a] = g.nx[t]; tn1[a] = g.ny[t]; tn2[a] = g.nz[t];
-    }
-    v4d acc[2][NCB];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) acc[a][cb] = v4d{0.0, 0.0, 0.0, 0.0};
-    const int tplane = g.tile_plane ? g.tile_plane[i0 / 256] : -1;
+    // this wave's generation slice: target i0 + 16*wave + lane%16
+    int64_t tg = i0 + wave * 16 + (lane & 15);
+    tg = tg < N ? tg : N - 1;
+    const double tx = g.x[tg], ty = g.y[tg], tz = g.z[tg];
+    const double tn0 = g.nx[tg], tn1 = g.ny[tg], tn2 = g.nz[tg];
 
-    for (int64_t jb = jb0; jb < jend; jb += kTS64) {
-        if (tplane >= 0 && g.tile_plane[jb / 256] == tplane) continue;  // uniform per workgroup
-        __syncthreads();
-        if (tid < kTS64) {
+    v4d acc[I::kN];
+#pragma unroll
+    for (int q = 0; q < I::kN; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int tplane = g.tile_plane ? g.tile_plane[i0 / 256] : -1;
+    auto next_stage = [&](int64_t jb) {  // skip same-plane 256-node tiles
+        while (jb < jend && tplane >= 0 && g.tile_plane[jb / 256] == tplane) jb += 256 - (jb % 256);
+        return jb;
+    };
+    double bcur[I::kCbPerWave][KS], bnxt[I::kCbPerWave][KS];
+    auto load_b = [&](int64_t jb, double (&b)[I::kCbPerWave][KS]) {
+        const double *zt = Zt + ((jb / kS64) * Kp + k0) * kS64;
+#pragma unroll
+        for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+                b[q][ks] = zt[(I::cb(wave, q) * 16 + (lane & 15)) * kS64 + ks * 4 + (lane >> 4)];
+    };
+    auto load_pos = [&](int64_t jb, int buf) {
+        if (tid < kS64) {
             const int64_t j = jb + tid;
-            s_pos[tid] = j < jend ? make_double4(g.x[j], g.y[j], g.z[j], 1.0) : make_double4(0.0, 0.0, 0.0, 0.0);
+            s_pos[buf][tid] = j < jend ? make_double4(g.x[j], g.y[j], g.z[j], 1.0) : make_double4(0.0, 0.0, 0.0, 0.0);
         }
-        {
-            const int s = tid & (kTS64 - 1);
-            const int64_t j = jb + s;
-            for (int c = tid / kTS64; c < CB; c += 256 / kTS64) {
-                const int col = k0 + c;
-                s_mem[c * kZS64 + s] = (j < jend && col < K) ? Z[j + (int64_t)col * N] : 0.0;
-            }
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int ks = 0; ks < kTS64 / 4; ++ks) {
-            const int sidx = ks * 4 + (lane >> 4);
-            const double4 sp = s_pos[sidx];
-            double gv[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const double dx = tx[a] - sp.x, dy = ty[a] - sp.y, dz = tz[a] - sp.z;
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                const double rinv = (r2 > 0.0 && sp.w != 0.0) ? rsqrt(r2) : 0.0;
-                gv[a] = (tn0[a] * dx + tn1[a] * dy + tn2[a] * dz) * (rinv * rinv * rinv);
-            }
-#pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                const double b = s_mem[(cb * 16 + (lane & 15)) * kZS64 + sidx];
-                acc[0][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(gv[0], b, acc[0][cb], 0, 0, 0);
-                acc[1][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(gv[1], b, acc[1][cb], 0, 0, 0);
-            }
-        }
+    };
+
+    int64_t jb = next_stage(jb0);
+    int buf = 0;
+    if (jb < jend) {
+        load_pos(jb, 0);
+        load_b(jb, bcur);
     }
-    // transpose each 16-column block through LDS; coalesced stores along targets
+    __syncthreads();
+    while (jb < jend) {
+        const int64_t jn = next_stage(jb + kS64);
+        // 1. generate this wave's slice: k-slice ks holds sources ks*4 + lane/16
 #pragma unroll
-    for (int cb = 0; cb < NCB; ++cb) {
-        __syncthreads();
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                s_mem[(lane & 15) * kOutStride + wave * 32 + a * 16 + 4 * r + (lane >> 4)] = acc[a][cb][r];
-        __syncthreads();
-        const int t = tid & (kTargets - 1);
-        const int64_t i = i0 + t;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = (tid >> 7) + 2 * q;
-            const int col = k0 + cb * 16 + c;
-            if (col < K && i < N) part[((int64_t)blockIdx.z * K + col) * N + i] = s_mem[c * kOutStride + t] * kInv4Pi;
+        for (int ks = 0; ks < KS; ++ks) {
+            const double4 sp = s_pos[buf][ks * 4 + (lane >> 4)];
+            const double dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double valid = (r2 > 0.0 && sp.w != 0.0) ? 1.0 : 0.0;
+            const double rinv = rsqrt(r2 > 0.0 ? r2 : 1.0) * valid;
+            s_g[buf][wave][ks][lane] = (tn0 * dx + tn1 * dy + tn2 * dz) * (rinv * rinv * rinv);
         }
+        // 2. prefetch the next stage
+        if (jn < jend) {
+            load_pos(jn, buf ^ 1);
+            load_b(jn, bnxt);
+        }
+        __syncthreads();
+        // 3. contract
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int s = 0; s < I::kSlicesPerWave; ++s) {
+                const double a = s_g[buf][I::slice(wave, s)][ks][lane];
+#pragma unroll
+                for (int q = 0; q < I::kCbPerWave; ++q)
+                    acc[q * I::kSlicesPerWave + s] =
+                        __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[q][ks], acc[q * I::kSlicesPerWave + s], 0, 0, 0);
+            }
+#pragma unroll
+        for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) bcur[q][ks] = bnxt[q][ks];
+        buf ^= 1;
+        jb = jn;
     }
+    // D: col = lane%16, row = 4*r + lane/16
+#pragma unroll
+    for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+        for (int s = 0; s < I::kSlicesPerWave; ++s) {
+            const int col = k0 + I::cb(wave, q) * 16 + (lane & 15);
+            if (col >= K) continue;
+            double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t i = i0 + I::slice(wave, s) * 16 + 4 * r + (lane >> 4);
+                if (i < N) dst[i] = acc[q * I::kSlicesPerWave + s][r] * kInv4Pi;
+            }
+        }
 }
 
 // ---------------------------------------------------------------- BF16
-constexpr int kTSb = 64;         // sources per LDS stage (two k-slices of 32)
-constexpr int kZSb = kTSb + 8;   // row stride (bf16) of the [col][source] tile
+constexpr int kSb = 64;  // sources per stage (two k-slices of 32)
 
-// Z = w .* X split into bf16 hi + lo, stored tile-major: [j / 64][col][j % 64], Kp columns.
-__global__ void split_bf16_kernel(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
-                                  int K, int Kp, __bf16 *__restrict__ zh, __bf16 *__restrict__ zl) {
+// w.*X split into bf16 hi + lo, tile-major: [j/64][col][j%64], Kp columns.
+__global__ void split_tiles_bf16(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
+                                 int K, int Kp, __bf16 *__restrict__ zh, __bf16 *__restrict__ zl) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    const int64_t ntile = (N + kTSb - 1) / kTSb;
-    if (j >= ntile * kTSb) return;
-    float z = 0.0f, lo = 0.0f;
-    __bf16 h = (__bf16)0.0f;
+    const int64_t ntile = (N + kSb - 1) / kSb;
+    if (j >= ntile * kSb) return;
+    __bf16 h = (__bf16)0.0f, l = (__bf16)0.0f;
     if (j < N && k < K) {
         const double zd = w[j] * X[j + (int64_t)k * ldx];
-        z = (float)zd;
-        h = (__bf16)z;
-        lo = (float)(zd - (double)(float)h);
+        h = (__bf16)(float)zd;
+        l = (__bf16)(float)(zd - (double)(float)h);
     }
-    const int64_t idx = ((j / kTSb) * Kp + k) * kTSb + (j % kTSb);
+    const int64_t idx = ((j / kSb) * Kp + k) * kSb + (j % kSb);
     zh[idx] = h;
-    if (zl) zl[idx] = (__bf16)lo;
+    if (zl) zl[idx] = l;
 }
 
 template <int NCB, bool X3>
-__global__ __launch_bounds__(256) void far_mfma_bf16_kernel(DevGeom g, int K, int Kp, const __bf16 *__restrict__ zh,
-                                                            const __bf16 *__restrict__ zl, double *__restrict__ part,
-                                                            int src_per_split) {
-    constexpr int CB = NCB * 16;
-    constexpr int kZ = CB * kZSb;  // bf16 elements per LDS tile
-    constexpr int kOutF = 16 * kOutStride;
-    constexpr int kSmemBytes = ((X3 ? 2 : 1) * kZ * 2 > kOutF * 4) ? (X3 ? 2 : 1) * kZ * 2 : kOutF * 4;
-    __shared__ float4 s_pos[kTSb];
-    __shared__ __attribute__((aligned(16))) unsigned char s_raw[kSmemBytes];
-    __bf16 *s_zh = reinterpret_cast<__bf16 *>(s_raw);
-    __bf16 *s_zl = s_zh + kZ;
-    float *s_out = reinterpret_cast<float *>(s_raw);
-
+__global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int K, int Kp, const __bf16 *__restrict__ zh,
+                                                                 const __bf16 *__restrict__ zl, double *__restrict__ part,
+                                                                 int src_per_split) {
+    using I = Items<NCB>;
+    constexpr int KS = kSb / 32;
+    __shared__ v8bf s_gh[2][kWaves][KS][64];  // [buf][slice][k-slice][lane]
+    __shared__ v8bf s_gl[X3 ? 2 : 1][kWaves][KS][64];
+    __shared__ float4 s_pos[2][kSb];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t N = g.N;
     const int64_t i0 = (int64_t)blockIdx.x * kTargets;
-    const int k0 = blockIdx.y * CB;
+    const int k0 = blockIdx.y * NCB * 16;
     const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
     const int64_t jend = min(N, jb0 + src_per_split);
 
-    float tx[2], ty[2], tz[2], tn0[2], tn1[2], tn2[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        int64_t t = i0 + wave * 32 + a * 16 + (lane & 15);
-        t = t < N ? t : N - 1;
-        tx[a] = g.xf[t]; ty[a] = g.yf[t]; tz[a] = g.zf[t];
-        tn0[a] = g.nxf[t]; tn1[a] = g.nyf[t]; tn2[a] = g.nzf[t];
-    }
-    v4f acc[2][NCB];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) acc[a][cb] = v4f{0.f, 0.f, 0.f, 0.f};
-    const int tplane = g.tile_plane ? g.tile_plane[i0 / 256] : -1;
+    int64_t tg = i0 + wave * 16 + (lane & 15);
+    tg = tg < N ? tg : N - 1;
+    const float tx = g.xf[tg], ty = g.yf[tg], tz = g.zf[tg];
+    const float tn0 = g.nxf[tg], tn1 = g.nyf[tg], tn2 = g.nzf[tg];
 
-    for (int64_t jb = jb0; jb < jend; jb += kTSb) {
-        if (tplane >= 0 && g.tile_plane[jb / 256] == tplane) continue;  // uniform per workgroup
-        __syncthreads();
-        if (tid < kTSb) {
-            const int64_t j = jb + tid;
-            s_pos[tid] = j < jend ? make_float4(g.xf[j], g.yf[j], g.zf[j], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        {
-            // tile-major global layout: the (CB x 64) block of this tile is contiguous per column
-            const int64_t tile = jb / kTSb;
-            const uint4 *gh = reinterpret_cast<const uint4 *>(zh + (tile * Kp + k0) * kTSb);
-            const uint4 *gl = X3 ? reinterpret_cast<const uint4 *>(zl + (tile * Kp + k0) * kTSb) : nullptr;
-            for (int e = tid; e < CB * (kTSb / 8); e += 256) {
-                const int c = e / (kTSb / 8), q = e % (kTSb / 8);
-                *reinterpret_cast<uint4 *>(s_zh + c * kZSb + 8 * q) = gh[e];
-                if (X3) *reinterpret_cast<uint4 *>(s_zl + c * kZSb + 8 * q) = gl[e];
-            }
-        }
-        __syncthreads();
+    v4f acc[I::kN];
 #pragma unroll
-        for (int ks = 0; ks < kTSb / 32; ++ks) {
-            const int sbase = ks * 32 + 8 * (lane >> 4);
-            v8bf ah[2], al[2];
+    for (int q = 0; q < I::kN; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int tplane = g.tile_plane ? g.tile_plane[i0 / 256] : -1;
+    auto next_stage = [&](int64_t jb) {
+        while (jb < jend && tplane >= 0 && g.tile_plane[jb / 256] == tplane) jb += 256 - (jb % 256);
+        return jb;
+    };
+    constexpr int NL = X3 ? I::kCbPerWave : 1;
+    uint4 bh_cur[I::kCbPerWave][KS], bh_nxt[I::kCbPerWave][KS];
+    uint4 bl_cur[NL][KS], bl_nxt[NL][KS];
+    auto load_b = [&](int64_t jb, uint4 (&bh)[I::kCbPerWave][KS], uint4 (&bl)[NL][KS]) {
+        const int64_t base = ((jb / kSb) * Kp + k0) * kSb;
+#pragma unroll
+        for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int64_t off = base + (int64_t)(I::cb(wave, q) * 16 + (lane & 15)) * kSb + ks * 32 + 8 * (lane >> 4);
+                bh[q][ks] = *reinterpret_cast<const uint4 *>(zh + off);
+                if (X3) bl[q < NL ? q : 0][ks] = *reinterpret_cast<const uint4 *>(zl + off);
+            }
+    };
+    auto load_pos = [&](int64_t jb, int buf) {
+        if (tid < kSb) {
+            const int64_t j = jb + tid;
+            s_pos[buf][tid] = j < jend ? make_float4(g.xf[j], g.yf[j], g.zf[j], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+
+    int64_t jb = next_stage(jb0);
+    int buf = 0;
+    if (jb < jend) {
+        load_pos(jb, 0);
+        load_b(jb, bh_cur, bl_cur);
+    }
+    __syncthreads();
+    while (jb < jend) {
+        const int64_t jn = next_stage(jb + kSb);
+        // 1. generate this wave's slice (16 targets x 64 sources): 8 entries per lane per k-slice
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            v8bf gh, gl;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const float4 sp = s_pos[sbase + e];
+                const float4 sp = s_pos[buf][ks * 32 + 8 * (lane >> 4) + e];
+                const float dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float valid = (r2 > 0.0f && sp.w != 0.0f) ? 1.0f : 0.0f;
+                const float rinv = __frsqrt_rn(r2 > 0.0f ? r2 : 1.0f) * valid;
+                const float gk = (tn0 * dx + tn1 * dy + tn2 * dz) * (rinv * rinv * rinv);
+                const __bf16 h = (__bf16)gk;
+                gh[e] = h;
+                if (X3) gl[e] = (__bf16)(gk - (float)h);
+            }
+            s_gh[buf][wave][ks][lane] = gh;
+            if (X3) s_gl[buf][wave][ks][lane] = gl;
+        }
+        // 2. prefetch the next stage
+        if (jn < jend) {
+            load_pos(jn, buf ^ 1);
+            load_b(jn, bh_nxt, bl_nxt);
+        }
+        __syncthreads();
+        // 3. contract
 #pragma unroll
-                for (int a = 0; a < 2; ++a) {
-                    const float dx = tx[a] - sp.x, dy = ty[a] - sp.y, dz = tz[a] - sp.z;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    const float rinv = (r2 > 0.0f && sp.w != 0.0f) ? __frsqrt_rn(r2) : 0.0f;
-                    const float gk = (tn0[a] * dx + tn1[a] * dy + tn2[a] * dz) * (rinv * rinv * rinv);
-                    const __bf16 h = (__bf16)gk;
-                    ah[a][e] = h;
-                    if (X3) al[a][e] = (__bf16)(gk - (float)h);
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int s = 0; s < I::kSlicesPerWave; ++s) {
+                const int sl = I::slice(wave, s);
+                const v8bf ah = s_gh[buf][sl][ks][lane];
+                v8bf al = ah;
+                if (X3) al = s_gl[buf][sl][ks][lane];
+#pragma unroll
+                for (int q = 0; q < I::kCbPerWave; ++q) {
+                    v4f &c = acc[q * I::kSlicesPerWave + s];
+                    const v8bf bh = __builtin_bit_cast(v8bf, bh_cur[q][ks]);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+                    if (X3) {
+                        const v8bf bl = __builtin_bit_cast(v8bf, bl_cur[q < NL ? q : 0][ks]);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+                    }
                 }
             }
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                const int off = (cb * 16 + (lane & 15)) * kZSb + sbase;
-                const v8bf bh = *reinterpret_cast<const v8bf *>(s_zh + off);
-                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bh, acc[0][cb], 0, 0, 0);
-                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1], bh, acc[1][cb], 0, 0, 0);
-                if (X3) {
-                    const v8bf bl = *reinterpret_cast<const v8bf *>(s_zl + off);
-                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bl, acc[0][cb], 0, 0, 0);
-                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1], bl, acc[1][cb], 0, 0, 0);
-                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], bh, acc[0][cb], 0, 0, 0);
-                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[1], bh, acc[1][cb], 0, 0, 0);
-                }
+        for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                bh_cur[q][ks] = bh_nxt[q][ks];
+                if (X3) bl_cur[q < NL ? q : 0][ks] = bl_nxt[q < NL ? q : 0][ks];
+            }
+        buf ^= 1;
+        jb = jn;
+    }
+    // D: col = lane%16, row = 4*(lane/16) + r
+#pragma unroll
+    for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+        for (int s = 0; s < I::kSlicesPerWave; ++s) {
+            const int col = k0 + I::cb(wave, q) * 16 + (lane & 15);
+            if (col >= K) continue;
+            double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t i = i0 + I::slice(wave, s) * 16 + 4 * (lane >> 4) + r;
+                if (i < N) dst[i] = (double)acc[q * I::kSlicesPerWave + s][r] * kInv4Pi;
             }
         }
-    }
-#pragma unroll
-    for (int cb = 0; cb < NCB; ++cb) {
-        __syncthreads();
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                s_out[(lane & 15) * kOutStride + wave * 32 + a * 16 + 4 * (lane >> 4) + r] = acc[a][cb][r];
-        __syncthreads();
-        const int t = tid & (kTargets - 1);
-        const int64_t i = i0 + t;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = (tid >> 7) + 2 * q;
-            const int col = k0 + cb * 16 + c;
-            if (col < K && i < N) part[((int64_t)blockIdx.z * K + col) * N + i] = (double)s_out[c * kOutStride + t] * kInv4Pi;
-        }
-    }
 }
 
-// Source split count for a launch of `groups` workgroups (splits start on 256-node tiles).
+// 16-column blocks per workgroup for K columns (power of two, at most 16)
+int ncb_for(int K) {
+    const int nb = (K + 15) / 16;
+    int ncb = 1;
+    while (ncb < nb && ncb < 16) ncb *= 2;
+    return ncb;
+}
+
+// Source split count (splits start on 256-node tiles).
 int mfma_splits(int64_t N, int K, int64_t groups, int64_t part_cap, int64_t *per_out) {
-    int64_t s = std::max<int64_t>(1, (1536 + groups - 1) / groups);
+    int64_t s = std::max<int64_t>(1, (768 + groups - 1) / groups);
     s = std::min<int64_t>(s, std::max<int64_t>(1, N / 2048));
     while (s > 1 && s * K * N > part_cap) --s;
     int64_t per = (N + s - 1) / s;
     per = (per + 255) / 256 * 256;
     *per_out = per;
     return (int)((N + per - 1) / per);
-}
-
-__global__ void scale_rows_kernel(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
-                                  double *__restrict__ Z) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    if (j < N) Z[j + (int64_t)k * N] = w[j] * X[j + (int64_t)k * ldx];
 }
 }  // namespace
 
@@ -282,11 +344,10 @@ bool far_field_mfma_supported(int prec, int K) {
 }
 
 size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K) {
-    if (prec == 0) return sizeof(double) * (size_t)N * K;
-    const int ncb = std::min(16, (K + 15) / 16);
-    const int CB = ncb * 16;
-    const int Kp = ((K + CB - 1) / CB) * CB;
-    const size_t zelems = (size_t)((N + kTSb - 1) / kTSb) * Kp * kTSb;
+    const int ncb = ncb_for(K);
+    const int Kp = ((K + ncb * 16 - 1) / (ncb * 16)) * ncb * 16;
+    if (prec == 0) return sizeof(double) * (size_t)((N + kS64 - 1) / kS64) * kS64 * Kp;
+    const size_t zelems = (size_t)((N + kSb - 1) / kSb) * Kp * kSb;
     return sizeof(__bf16) * zelems * (prec == 3 ? 2 : 1);
 }
 
@@ -294,48 +355,42 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
                    int64_t part_cap, void *scratch) {
     const int64_t N = g.N;
     const int64_t bx = (N + kTargets - 1) / kTargets;
+    const int ncb = ncb_for(K);
+    const int CB = ncb * 16;
+    const int by = (K + CB - 1) / CB;
+    const int Kp = by * CB;
+    int64_t per;
+    const int S = mfma_splits(N, K, bx * by, part_cap, &per);
+    dim3 grid((unsigned)bx, (unsigned)by, (unsigned)S);
     if (prec == 0) {
-        const int ncb = std::min(8, (K + 15) / 16);
-        const int CB = ncb * 16;
-        const int by = (K + CB - 1) / CB;
-        int64_t per;
-        const int S = mfma_splits(N, K, bx * by, part_cap, &per);
-        double *Z = static_cast<double *>(scratch);
-        scale_rows_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)K), 256, 0, st>>>(X, ldx, g.w, N, Z);
-        dim3 grid((unsigned)bx, (unsigned)by, (unsigned)S);
+        double *Zt = static_cast<double *>(scratch);
+        const int64_t ntile = (N + kS64 - 1) / kS64;
+        scale_tiles_f64<<<dim3((unsigned)((ntile * kS64 + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K,
+                                                                                                     Kp, Zt);
         switch (ncb) {
-#define DBIE_F64(n) case n: far_mfma_f64_kernel<n><<<grid, 256, 0, st>>>(g, K, Z, part, (int)per); break;
-            DBIE_F64(1) DBIE_F64(2) DBIE_F64(3) DBIE_F64(4) DBIE_F64(5) DBIE_F64(6) DBIE_F64(7) DBIE_F64(8)
+#define DBIE_F64(n) case n: far_mfma_f64_kernel<n><<<grid, kThreads, 0, st>>>(g, K, Kp, Zt, part, (int)per); break;
+            DBIE_F64(1) DBIE_F64(2) DBIE_F64(4) DBIE_F64(8) DBIE_F64(16)
 #undef DBIE_F64
         }
         return S;
     }
     const bool x3 = prec == 3;
-    const int ncb = std::min(16, (K + 15) / 16);
-    const int CB = ncb * 16;
-    const int by = (K + CB - 1) / CB;
-    const int Kp = by * CB;
-    const int64_t ntile = (N + kTSb - 1) / kTSb;
-    const size_t zelems = (size_t)ntile * Kp * kTSb;
+    const int64_t ntile = (N + kSb - 1) / kSb;
+    const size_t zelems = (size_t)ntile * Kp * kSb;
     __bf16 *zh = static_cast<__bf16 *>(scratch);
     __bf16 *zl = x3 ? zh + zelems : nullptr;
-    split_bf16_kernel<<<dim3((unsigned)((ntile * kTSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
-                                                                                                   zh, zl);
-    int64_t per;
-    const int S = mfma_splits(N, K, bx * by, part_cap, &per);
-    dim3 grid((unsigned)bx, (unsigned)by, (unsigned)S);
+    split_tiles_bf16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
+                                                                                                 zh, zl);
     if (x3) {
         switch (ncb) {
-#define DBIE_B3(n) case n: far_mfma_bf16_kernel<n, true><<<grid, 256, 0, st>>>(g, K, Kp, zh, zl, part, (int)per); break;
-            DBIE_B3(1) DBIE_B3(2) DBIE_B3(3) DBIE_B3(4) DBIE_B3(5) DBIE_B3(6) DBIE_B3(7) DBIE_B3(8)
-            DBIE_B3(9) DBIE_B3(10) DBIE_B3(11) DBIE_B3(12) DBIE_B3(13) DBIE_B3(14) DBIE_B3(15) DBIE_B3(16)
+#define DBIE_B3(n) case n: far_mfma_bf16_kernel<n, true><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, zl, part, (int)per); break;
+            DBIE_B3(1) DBIE_B3(2) DBIE_B3(4) DBIE_B3(8) DBIE_B3(16)
 #undef DBIE_B3
         }
     } else {
         switch (ncb) {
-#define DBIE_B1(n) case n: far_mfma_bf16_kernel<n, false><<<grid, 256, 0, st>>>(g, K, Kp, zh, nullptr, part, (int)per); break;
-            DBIE_B1(1) DBIE_B1(2) DBIE_B1(3) DBIE_B1(4) DBIE_B1(5) DBIE_B1(6) DBIE_B1(7) DBIE_B1(8)
-            DBIE_B1(9) DBIE_B1(10) DBIE_B1(11) DBIE_B1(12) DBIE_B1(13) DBIE_B1(14) DBIE_B1(15) DBIE_B1(16)
+#define DBIE_B1(n) case n: far_mfma_bf16_kernel<n, false><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, nullptr, part, (int)per); break;
+            DBIE_B1(1) DBIE_B1(2) DBIE_B1(4) DBIE_B1(8) DBIE_B1(16)
 #undef DBIE_B1
         }
     }

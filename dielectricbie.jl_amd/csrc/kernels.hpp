@@ -24,9 +24,13 @@ struct DevBSR {
     int P = 0, bd = 0;
     int64_t nb = 0;
     const int *row_ptr = nullptr, *col = nullptr;
-    const double *val64 = nullptr;  // nb x bd x bd, row-major blocks
+    const double *val64 = nullptr;  // nb x bd x bd blocks, element (t, c) at bsr_storage_index(bd, mfma_order, t, c)
     const float *val32 = nullptr;
+    bool mfma_order = false;        // bd = 16: blocks stored in MFMA operand order
 };
+// Storage position of block element (t, c): MFMA operand order for bd = 16
+// (k-slice c/4, then lane (c%4)*16 + t), row-major t*bd + c otherwise.
+int64_t bsr_storage_index(int bd, bool mfma_order, int t, int c);
 
 enum FarMode { FAR_DT = 0, FAR_S = 1, FAR_D = 2 };
 

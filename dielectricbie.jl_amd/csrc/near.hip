@@ -2,8 +2,15 @@
 // reduction and the diagonal contrast term (mul! of BatchedDielectricOperator,
 // src/solver/multi_rhs.jl:103-130):
 //     Y[i,k] = sum_s part[s][k][i] + t_i X[i,k] + sum_b C_b[i - bd*p, :] . X[bd*src_b + :, k]
-// Block rows are target panels; each block is a dense bd x bd (bd = n_quad^2)
-// row-major tile.  Values are read once per launch (HBM-bound for K = 1).
+// Block rows are target panels; each block is a dense bd x bd (bd = n_quad^2) tile.
+//
+// Two kernels:
+//  * K >= 16 and bd = 16: FP64 MFMA (v_mfma_f64_16x16x4_f64).  One workgroup per block
+//    row and 256 columns; the A operand is the correction block, stored in MFMA operand
+//    order so that each k-slice is one contiguous 512-byte load; the B operand is the
+//    matching 16-row slice of X.  Block rows are remapped so that consecutive rows (which
+//    share most source panels) run on the same XCD and reuse its L2.
+//  * otherwise: one thread per (row, column), VALU.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,9 +19,19 @@
 
 namespace dbie {
 
+// Position of block element (t, c) in storage: MFMA order for bd = 16
+// (k-slice c/4, then lane (c%4)*16 + t), row-major otherwise.
+__host__ __device__ inline int blk_idx(int bd, bool mfma_order, int t, int c) {
+    return mfma_order ? (c >> 2) * 64 + (c & 3) * 16 + t : t * bd + c;
+}
+
+int64_t bsr_storage_index(int bd, bool mfma_order, int t, int c) { return blk_idx(bd, mfma_order, t, c); }
+
 namespace {
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 template <typename VT>
-__global__ __launch_bounds__(256) void near_combine_kernel(int P, int bd, const int *__restrict__ row_ptr,
+__global__ __launch_bounds__(256) void near_combine_kernel(int P, int bd, int mfma_order, const int *__restrict__ row_ptr,
                                                            const int *__restrict__ col, const VT *__restrict__ val,
                                                            int S, const double *__restrict__ part, int64_t N, int K,
                                                            int KTe, int RP, const double *__restrict__ X, int64_t ldx,
@@ -36,27 +53,93 @@ __global__ __launch_bounds__(256) void near_combine_kernel(int P, int bd, const 
     double corr = 0.0;
     const int b1 = row_ptr[p + 1];
     for (int b = row_ptr[p]; b < b1; ++b) {
-        const VT *v = val + ((int64_t)b * bd + row) * bd;
+        const VT *v = val + (int64_t)b * bd * bd;
         const double *xs = xk + (int64_t)col[b] * bd;
-        for (int c = 0; c < bd; ++c) corr += (double)v[c] * xs[c];
+        for (int c = 0; c < bd; ++c) corr += (double)v[blk_idx(bd, mfma_order != 0, row, c)] * xs[c];
     }
     Y[i + (int64_t)k * ldy] = acc + corr;
+}
+
+// XCD-aware block-row order: workgroups are dispatched round-robin over 8 XCDs, so
+// workgroup w runs on XCD w % 8; give each XCD a contiguous range of block rows.
+__device__ inline int xcd_row(int w, int P) {
+    const int per = (P + 7) / 8;
+    const int r = (w % 8) * per + w / 8;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__restrict__ row_ptr,
+                                                        const int *__restrict__ col, const double *__restrict__ val,
+                                                        int S, const double *__restrict__ part, int64_t N, int K,
+                                                        const double *__restrict__ X, int64_t ldx,
+                                                        const double *__restrict__ diag, double *__restrict__ Y,
+                                                        int64_t ldy) {
+    __shared__ double s_out[256][17];  // [column within the 256-column group][row], padded
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (P + 7) / 8;
+    const int p = xcd_row(blockIdx.x, P);
+    const int k0 = blockIdx.y * 256;
+    if (p >= P || blockIdx.x >= (unsigned)(8 * per)) return;  // uniform per workgroup
+    v4d acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
+    for (int b = b0; b < b1; ++b) {
+        const double *v = val + (int64_t)b * 256;
+        const int64_t src_row = (int64_t)col[b] * 16;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const double a = v[kk * 64 + lane];  // C_b[t = lane%16][c = kk*4 + lane/16]
+            const int64_t xr = src_row + kk * 4 + (lane >> 4);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = k0 + (wave + 4 * q) * 16 + (lane & 15);
+                const double xv = c < K ? X[xr + (int64_t)c * ldx] : 0.0;
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xv, acc[q], 0, 0, 0);
+            }
+        }
+    }
+    // D(row = 4r + lane/16, col = lane%16) -> LDS [col][row]
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s_out[(wave + 4 * q) * 16 + (lane & 15)][4 * r + (lane >> 4)] = acc[q][r];
+    __syncthreads();
+    // each thread finishes 16 outputs: row = tid % 16, columns tid/16 + 16*m
+    const int row = tid & 15;
+    const int64_t i = (int64_t)p * 16 + row;
+    const double t = diag[i];
+    for (int m = 0; m < 16; ++m) {
+        const int cl = (tid >> 4) + 16 * m;
+        const int c = k0 + cl;
+        if (c >= K) continue;
+        double y = s_out[cl][row] + t * X[i + (int64_t)c * ldx];
+        for (int s = 0; s < S; ++s) y += part[((int64_t)s * K + c) * N + i];
+        Y[i + (int64_t)c * ldy] = y;
+    }
 }
 }  // namespace
 
 void near_combine(hipStream_t st, const DevBSR &C, bool use_f32, int S, const double *part, int64_t N, int K,
                   const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy) {
     const int bd = C.bd;
+    if (C.mfma_order && bd == 16 && K >= 16) {
+        const int per = (C.P + 7) / 8;
+        dim3 grid((unsigned)(8 * per), (unsigned)((K + 255) / 256));
+        near_mfma_kernel<<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.val64, S, part, N, K, X, ldx, diag, Y, ldy);
+        return;
+    }
     const int KT = std::max(1, 256 / bd);
     const int KTe = std::min(K, KT);
     const int RP = std::max(1, KT / KTe);
     dim3 grid((unsigned)((C.P + RP - 1) / RP), (unsigned)((K + KTe - 1) / KTe));
+    const int mo = C.mfma_order ? 1 : 0;
     if (use_f32 && C.val32)
-        near_combine_kernel<float><<<grid, 256, 0, st>>>(C.P, bd, C.row_ptr, C.col, C.val32, S, part, N, K, KTe, RP, X,
-                                                          ldx, diag, Y, ldy);
+        near_combine_kernel<float><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, C.val32, S, part, N, K, KTe, RP,
+                                                          X, ldx, diag, Y, ldy);
     else
-        near_combine_kernel<double><<<grid, 256, 0, st>>>(C.P, bd, C.row_ptr, C.col, C.val64, S, part, N, K, KTe, RP,
-                                                           X, ldx, diag, Y, ldy);
+        near_combine_kernel<double><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, C.val64, S, part, N, K, KTe,
+                                                           RP, X, ldx, diag, Y, ldy);
 }
 
 }  // namespace dbie

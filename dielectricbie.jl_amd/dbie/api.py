@@ -272,6 +272,19 @@ class BatchedDielectricOperator:
         L.check(L.lib().dbie_op_device_bytes(self._h, C.byref(b)))
         return b.value
 
+    def profile(self, enable=True):
+        """Record HIP events around the far-field and combine launches (kernel timing)."""
+        L.check(L.lib().dbie_op_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        """{kernel name: (launches, total ms)} since the last read; clears the records."""
+        out = {}
+        for kid, name in ((0, "far_fp64"), (1, "far_lowprec"), (2, "near_combine")):
+            n = C.c_int64(); ms = C.c_double()
+            L.check(L.lib().dbie_op_profile_read(self._h, kid, C.byref(n), C.byref(ms)))
+            out[name] = (n.value, ms.value)
+        return out
+
     def apply(self, X, precision="fp64"):
         X = np.asarray(X, np.float64)
         vec = X.ndim == 1

@@ -187,6 +187,14 @@ dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, d
 dbie_status dbie_pottrg(dbie_op *op, int64_t M, const double *trg, int K, const double *Sigma, int64_t lds,
                         double *U, int64_t ldu);
 
+/* ---- kernel timing (HIP events on the operator's stream) ----------------- */
+/* No counterpart in the reference (it has no profiler hooks); used by the benchmark to
+   time the dominant kernels inside a timed region.  kernel ids: 0 = far field FP64,
+   1 = far field low precision (FP32/BF16/BF16X3), 2 = near-field combine.  Reading
+   synchronizes the stream and clears the records. */
+dbie_status dbie_op_profile(dbie_op *op, int enable);
+dbie_status dbie_op_profile_read(dbie_op *op, int kernel, int64_t *launches, double *total_ms);
+
 /* ---- device-pointer variants (stream-ordered on the operator's stream) ---- */
 void *dbie_op_stream(dbie_op *op);
 dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
