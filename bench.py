@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--mode", default="solve", choices=["solve", "apply"],
+                    help="apply: time only operator applies (one per precision) for kernel profiling")
+    ap.add_argument("--reps", type=int, default=3)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; the path shards by RHS "
@@ -98,6 +101,22 @@ def main():
     F = dbie.rhs_point_charges(iface, pts, q, EPS_IN)
     N = iface.n_points
     log(f"setup N={N} blocks={op.n_blocks} {t_setup:.2f}s")
+    if a.mode == "apply":
+        X = np.random.default_rng(2).standard_normal((N, a.K))
+        res = {}
+        for prec in ("fp64", a.precision):
+            op.apply(X, prec)
+            op.profile(True)
+            op.profile_read()
+            for _ in range(a.reps):
+                op.apply(X, prec)
+            prof = op.profile_read()
+            op.profile(False)
+            res[prec] = {k: v[1] / max(v[0], 1) for k, v in prof.items() if v[0]}
+            log(f"apply {prec}: per-launch ms {res[prec]}")
+        print(json.dumps({"mode": "apply", "N": N, "K": a.K, "per_launch_ms": res,
+                          "cross_plane_pairs": cross_plane_pairs(iface)}), flush=True)
+        return
 
     # plain FP64 comparator (also the reference solution for the error)
     fp64_times, S64 = [], None

@@ -3,12 +3,13 @@
 // Y[i,k] = sum_j G(i,j) Z[j,k],  G(i,j) = n_i.(x_i - y_j) / r^3 (self pair 0),  Z = w .* X.
 // The kernel matrix G is never stored in HBM.  A workgroup of 8 waves owns 128 targets
 // (eight 16-target slices) and all columns of its column group.  Per source stage:
-//   1. wave w generates G for target slice w on VALU and writes it to a double-buffered
-//      LDS tile laid out exactly as the MFMA A operand (one 16-byte / 8-byte chunk per lane);
-//   2. one barrier;
-//   3. every wave contracts the target slices it owns with its own 16-column blocks; the
-//      B operand (w.*X) is read from a tile-major copy in L2, prefetched one stage ahead.
-// One barrier per stage; the slice written at stage s+1 goes to the other buffer.
+//   * every wave contracts the target slices it owns with its own 16-column blocks, reading
+//     the generated tile of stage s from LDS (laid out exactly as the MFMA A operand) and the
+//     B operand (w.*X) from a tile-major copy in L2, prefetched one stage ahead;
+//   * interleaved with those MFMAs, wave w generates on VALU the entries of stage s+1 for
+//     target slice w and stores them into the next LDS tile;
+//   * one barrier per stage.  Tiles are triple-buffered so that a wave writing stage s+1
+//     never overwrites a tile a slower wave still reads.
 //
 //   FP64: v_mfma_f64_16x16x4_f64, stage = 16 sources (4 k-slices of 4)
 //   BF16: v_mfma_f32_16x16x32_bf16, stage = 64 sources (2 k-slices of 32), FP32 generation,
@@ -31,6 +32,8 @@ namespace {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 
 constexpr double kInv4Pi = 1.0 / (4.0 * 3.14159265358979311599796346854);
 constexpr int kWaves = 8;
@@ -63,21 +66,38 @@ __global__ void scale_tiles_f64(const double *__restrict__ X, int64_t ldx, const
     Zt[((j / kS64) * Kp + k) * kS64 + (j % kS64)] = z;
 }
 
+// Stage sequence of a source split: stages start every kS sources, same-plane 256-node
+// tiles are skipped (their entries are exact zeros).
+struct StageWalk {
+    int64_t jend;
+    int tplane;
+    const int *tile_plane;
+    __device__ int64_t next(int64_t jb) const {
+        while (jb < jend && tplane >= 0 && tile_plane[jb / 256] == tplane) jb += 256 - (jb % 256);
+        return jb;
+    }
+};
+
+// Padding sources (past the split) are parked far away: their kernel entries underflow
+// to zero, so no per-entry validity select is needed.  The self pair (r2 = 0) has
+// n.(x - y) = 0 and r2 is clamped before the reciprocal square root.
+constexpr double kFar64 = 1.0e150;
+constexpr float kFar32 = 1.0e18f;
+
 template <int NCB>
 __global__ __launch_bounds__(kThreads) void far_mfma_f64_kernel(DevGeom g, int K, int Kp, const double *__restrict__ Zt,
                                                                 double *__restrict__ part, int src_per_split) {
     using I = Items<NCB>;
     constexpr int KS = kS64 / 4;
-    __shared__ double s_g[2][kWaves][KS][64];  // [buf][slice][k-slice][lane]
-    __shared__ double4 s_pos[2][kS64];
+    __shared__ double s_g[3][kWaves][KS][64];  // [stage % 3][slice][k-slice][lane]
+    __shared__ double4 s_pos[2][kS64];         // [stage % 2][source]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t N = g.N;
     const int64_t i0 = (int64_t)blockIdx.x * kTargets;
     const int k0 = blockIdx.y * NCB * 16;
     const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
-    const int64_t jend = min(N, jb0 + src_per_split);
+    const StageWalk walk{min(N, jb0 + src_per_split), g.tile_plane ? g.tile_plane[i0 / 256] : -1, g.tile_plane};
 
-    // this wave's generation slice: target i0 + 16*wave + lane%16
     int64_t tg = i0 + wave * 16 + (lane & 15);
     tg = tg < N ? tg : N - 1;
     const double tx = g.x[tg], ty = g.y[tg], tz = g.z[tg];
@@ -86,11 +106,6 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f64_kernel(DevGeom g, int K
     v4d acc[I::kN];
 #pragma unroll
     for (int q = 0; q < I::kN; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
-    const int tplane = g.tile_plane ? g.tile_plane[i0 / 256] : -1;
-    auto next_stage = [&](int64_t jb) {  // skip same-plane 256-node tiles
-        while (jb < jend && tplane >= 0 && g.tile_plane[jb / 256] == tplane) jb += 256 - (jb % 256);
-        return jb;
-    };
     double bcur[I::kCbPerWave][KS], bnxt[I::kCbPerWave][KS];
     auto load_b = [&](int64_t jb, double (&b)[I::kCbPerWave][KS]) {
         const double *zt = Zt + ((jb / kS64) * Kp + k0) * kS64;
@@ -100,55 +115,69 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f64_kernel(DevGeom g, int K
             for (int ks = 0; ks < KS; ++ks)
                 b[q][ks] = zt[(I::cb(wave, q) * 16 + (lane & 15)) * kS64 + ks * 4 + (lane >> 4)];
     };
-    auto load_pos = [&](int64_t jb, int buf) {
+    auto load_pos = [&](int64_t jb, int pb) {
         if (tid < kS64) {
             const int64_t j = jb + tid;
-            s_pos[buf][tid] = j < jend ? make_double4(g.x[j], g.y[j], g.z[j], 1.0) : make_double4(0.0, 0.0, 0.0, 0.0);
+            s_pos[pb][tid] = j < walk.jend ? make_double4(g.x[j], g.y[j], g.z[j], 0.0)
+                                           : make_double4(kFar64, kFar64, kFar64, 0.0);
         }
     };
+    auto gen = [&](int pb, int ks) {  // entry (slice target, source ks*4 + lane/16)
+        const double4 sp = s_pos[pb][ks * 4 + (lane >> 4)];
+        const double dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
+        const double r2 = fmax(dx * dx + dy * dy + dz * dz, 1e-200);
+        const double rinv = rsqrt(r2);
+        return (tn0 * dx + tn1 * dy + tn2 * dz) * (rinv * rinv * rinv);
+    };
 
-    int64_t jb = next_stage(jb0);
-    int buf = 0;
-    if (jb < jend) {
-        load_pos(jb, 0);
-        load_b(jb, bcur);
+    // prologue: stage 0 generated, stage 1 positions staged, B(0) loaded (skipped, uniformly
+    // per workgroup, when the whole split is same-plane; the zero partial is still written)
+    int64_t jc = walk.next(jb0);
+    int64_t j1 = walk.jend;
+    if (jc < walk.jend) {
+        j1 = walk.next(jc + kS64);
+        load_pos(jc, 0);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) s_g[0][wave][ks][lane] = gen(0, ks);
+        if (j1 < walk.jend) load_pos(j1, 1);
+        load_b(jc, bcur);
+        __syncthreads();
     }
-    __syncthreads();
-    while (jb < jend) {
-        const int64_t jn = next_stage(jb + kS64);
-        // 1. generate this wave's slice: k-slice ks holds sources ks*4 + lane/16
+    int sg = 0, pb = 1;
+    while (jc < walk.jend) {
+        const bool more = j1 < walk.jend;
+        const int64_t j2 = more ? walk.next(j1 + kS64) : walk.jend;
+        if (more) load_b(j1, bnxt);
+        double gnext[KS];
+        // MFMAs of stage jc interleaved with generation of stage j1
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            const double4 sp = s_pos[buf][ks * 4 + (lane >> 4)];
-            const double dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double valid = (r2 > 0.0 && sp.w != 0.0) ? 1.0 : 0.0;
-            const double rinv = rsqrt(r2 > 0.0 ? r2 : 1.0) * valid;
-            s_g[buf][wave][ks][lane] = (tn0 * dx + tn1 * dy + tn2 * dz) * (rinv * rinv * rinv);
-        }
-        // 2. prefetch the next stage
-        if (jn < jend) {
-            load_pos(jn, buf ^ 1);
-            load_b(jn, bnxt);
-        }
-        __syncthreads();
-        // 3. contract
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
+            if (more) gnext[ks] = gen(pb, ks);
 #pragma unroll
             for (int s = 0; s < I::kSlicesPerWave; ++s) {
-                const double a = s_g[buf][I::slice(wave, s)][ks][lane];
+                const double a = s_g[sg][I::slice(wave, s)][ks][lane];
 #pragma unroll
                 for (int q = 0; q < I::kCbPerWave; ++q)
                     acc[q * I::kSlicesPerWave + s] =
                         __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[q][ks], acc[q * I::kSlicesPerWave + s], 0, 0, 0);
             }
+        }
+        const int sn = sg == 2 ? 0 : sg + 1;
+        if (more) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) s_g[sn][wave][ks][lane] = gnext[ks];
+            if (j2 < walk.jend) load_pos(j2, pb ^ 1);
+        }
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < I::kCbPerWave; ++q)
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) bcur[q][ks] = bnxt[q][ks];
-        buf ^= 1;
-        jb = jn;
+        sg = sn;
+        pb ^= 1;
+        jc = j1;
+        j1 = j2;
     }
     // D: col = lane%16, row = 4*r + lane/16
 #pragma unroll
@@ -193,15 +222,15 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
                                                                  int src_per_split) {
     using I = Items<NCB>;
     constexpr int KS = kSb / 32;
-    __shared__ v8bf s_gh[2][kWaves][KS][64];  // [buf][slice][k-slice][lane]
-    __shared__ v8bf s_gl[X3 ? 2 : 1][kWaves][KS][64];
+    constexpr int NG = X3 ? 2 : 1;                 // hi (and lo) planes of the generated tile
+    __shared__ uint4 s_g[3][NG][kWaves][KS][64];    // [stage % 3][plane][slice][k-slice][lane]
     __shared__ float4 s_pos[2][kSb];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t N = g.N;
     const int64_t i0 = (int64_t)blockIdx.x * kTargets;
     const int k0 = blockIdx.y * NCB * 16;
     const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
-    const int64_t jend = min(N, jb0 + src_per_split);
+    const StageWalk walk{min(N, jb0 + src_per_split), g.tile_plane ? g.tile_plane[i0 / 256] : -1, g.tile_plane};
 
     int64_t tg = i0 + wave * 16 + (lane & 15);
     tg = tg < N ? tg : N - 1;
@@ -211,11 +240,6 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
     v4f acc[I::kN];
 #pragma unroll
     for (int q = 0; q < I::kN; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
-    const int tplane = g.tile_plane ? g.tile_plane[i0 / 256] : -1;
-    auto next_stage = [&](int64_t jb) {
-        while (jb < jend && tplane >= 0 && g.tile_plane[jb / 256] == tplane) jb += 256 - (jb % 256);
-        return jb;
-    };
     constexpr int NL = X3 ? I::kCbPerWave : 1;
     uint4 bh_cur[I::kCbPerWave][KS], bh_nxt[I::kCbPerWave][KS];
     uint4 bl_cur[NL][KS], bl_nxt[NL][KS];
@@ -230,56 +254,72 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
                 if (X3) bl[q < NL ? q : 0][ks] = *reinterpret_cast<const uint4 *>(zl + off);
             }
     };
-    auto load_pos = [&](int64_t jb, int buf) {
+    auto load_pos = [&](int64_t jb, int pb) {
         if (tid < kSb) {
             const int64_t j = jb + tid;
-            s_pos[buf][tid] = j < jend ? make_float4(g.xf[j], g.yf[j], g.zf[j], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+            s_pos[pb][tid] = j < walk.jend ? make_float4(g.xf[j], g.yf[j], g.zf[j], 0.0f)
+                                           : make_float4(kFar32, kFar32, kFar32, 0.0f);
         }
     };
+    // one pair of entries (sources e, e+1 of k-slice ks) -> packed bf16 hi (and lo)
+    auto gen2 = [&](int pb, int ks, int e, unsigned &hi, unsigned &lo) {
+        float v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const float4 sp = s_pos[pb][ks * 32 + 8 * (lane >> 4) + e + u];
+            const float dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
+            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            v[u] = (tn0 * dx + tn1 * dy + tn2 * dz) * (rinv * rinv * rinv);
+        }
+        const v2bf h = __builtin_convertvector(v2f{v[0], v[1]}, v2bf);
+        hi = __builtin_bit_cast(unsigned, h);
+        if (X3) {
+            const v2f hf = __builtin_convertvector(h, v2f);
+            const v2bf l = __builtin_convertvector(v2f{v[0] - hf[0], v[1] - hf[1]}, v2bf);
+            lo = __builtin_bit_cast(unsigned, l);
+        }
+    };
+    auto gen_slice = [&](int pb, int ks, uint4 &h4, uint4 &l4) {
+        unsigned h[4], l[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) gen2(pb, ks, 2 * e, h[e], l[e]);
+        h4 = uint4{h[0], h[1], h[2], h[3]};
+        l4 = uint4{l[0], l[1], l[2], l[3]};
+    };
 
-    int64_t jb = next_stage(jb0);
-    int buf = 0;
-    if (jb < jend) {
-        load_pos(jb, 0);
-        load_b(jb, bh_cur, bl_cur);
-    }
-    __syncthreads();
-    while (jb < jend) {
-        const int64_t jn = next_stage(jb + kSb);
-        // 1. generate this wave's slice (16 targets x 64 sources): 8 entries per lane per k-slice
+    int64_t jc = walk.next(jb0);
+    int64_t j1 = walk.jend;
+    if (jc < walk.jend) {
+        j1 = walk.next(jc + kSb);
+        load_pos(jc, 0);
+        __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            v8bf gh, gl;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float4 sp = s_pos[buf][ks * 32 + 8 * (lane >> 4) + e];
-                const float dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const float valid = (r2 > 0.0f && sp.w != 0.0f) ? 1.0f : 0.0f;
-                const float rinv = __frsqrt_rn(r2 > 0.0f ? r2 : 1.0f) * valid;
-                const float gk = (tn0 * dx + tn1 * dy + tn2 * dz) * (rinv * rinv * rinv);
-                const __bf16 h = (__bf16)gk;
-                gh[e] = h;
-                if (X3) gl[e] = (__bf16)(gk - (float)h);
-            }
-            s_gh[buf][wave][ks][lane] = gh;
-            if (X3) s_gl[buf][wave][ks][lane] = gl;
+            uint4 h4, l4;
+            gen_slice(0, ks, h4, l4);
+            s_g[0][0][wave][ks][lane] = h4;
+            if (X3) s_g[0][NG - 1][wave][ks][lane] = l4;
         }
-        // 2. prefetch the next stage
-        if (jn < jend) {
-            load_pos(jn, buf ^ 1);
-            load_b(jn, bh_nxt, bl_nxt);
-        }
+        if (j1 < walk.jend) load_pos(j1, 1);
+        load_b(jc, bh_cur, bl_cur);
         __syncthreads();
-        // 3. contract
+    }
+    int sg = 0, pb = 1;
+    while (jc < walk.jend) {
+        const bool more = j1 < walk.jend;
+        const int64_t j2 = more ? walk.next(j1 + kSb) : walk.jend;
+        if (more) load_b(j1, bh_nxt, bl_nxt);
+        uint4 gh[KS], gl[KS];
+        // MFMAs of stage jc interleaved with generation of stage j1
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
+        for (int ks = 0; ks < KS; ++ks) {
+            if (more) gen_slice(pb, ks, gh[ks], gl[ks]);
 #pragma unroll
             for (int s = 0; s < I::kSlicesPerWave; ++s) {
                 const int sl = I::slice(wave, s);
-                const v8bf ah = s_gh[buf][sl][ks][lane];
-                v8bf al = ah;
-                if (X3) al = s_gl[buf][sl][ks][lane];
+                const v8bf ah = __builtin_bit_cast(v8bf, s_g[sg][0][sl][ks][lane]);
+                const v8bf al = __builtin_bit_cast(v8bf, s_g[sg][NG - 1][sl][ks][lane]);
 #pragma unroll
                 for (int q = 0; q < I::kCbPerWave; ++q) {
                     v4f &c = acc[q * I::kSlicesPerWave + s];
@@ -292,6 +332,17 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
                     }
                 }
             }
+        }
+        const int sn = sg == 2 ? 0 : sg + 1;
+        if (more) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                s_g[sn][0][wave][ks][lane] = gh[ks];
+                if (X3) s_g[sn][NG - 1][wave][ks][lane] = gl[ks];
+            }
+            if (j2 < walk.jend) load_pos(j2, pb ^ 1);
+        }
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < I::kCbPerWave; ++q)
 #pragma unroll
@@ -299,8 +350,10 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
                 bh_cur[q][ks] = bh_nxt[q][ks];
                 if (X3) bl_cur[q < NL ? q : 0][ks] = bl_nxt[q < NL ? q : 0][ks];
             }
-        buf ^= 1;
-        jb = jn;
+        sg = sn;
+        pb ^= 1;
+        jc = j1;
+        j1 = j2;
     }
     // D: col = lane%16, row = 4*(lane/16) + r
 #pragma unroll
