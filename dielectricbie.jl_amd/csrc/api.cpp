@@ -12,6 +12,8 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -114,7 +116,8 @@ struct dbie_op {
     DevBSR bsr;
     int adaptive_hits = 0;
     // workspaces (grown on demand)
-    DBuf part, xin, yout, tmp, mscratch;
+    DBuf part, xin, yout, tmp, mscratch, gmax;
+    float f16_gscale = 0.0f;  // power-of-two scale of the fp16 kernel entries (0 = not yet computed)
     // optional per-kernel timing: event pairs recorded around launches
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[3];
@@ -214,6 +217,10 @@ struct ProfScope {
     }
 };
 
+void check_precision(int p) {
+    if (p < DBIE_FP64 || p > DBIE_FP16X3) fail(DBIE_BAD_ARG, "unknown precision");
+}
+
 // Y = A X on the device (X, Y: N x K with leading dims).
 void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
     const int64_t N = op->N();
@@ -222,9 +229,15 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
     int S;
     {
         ProfScope ps(op, prec == DBIE_FP64 ? 0 : 1);
-        if ((prec == DBIE_BF16 || prec == DBIE_BF16X3 || prec == DBIE_FP64) && far_field_mfma_supported((int)prec, K)) {
+        if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
+            if (prec == DBIE_FP16X3 && op->f16_gscale == 0.0f) {
+                op->gmax.ensure(sizeof(unsigned));
+                op->f16_gscale = far_field_f16_scale(op->stream, op->dg, op->gmax.p);
+                if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
+            }
             op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
-            S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p);
+            S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p,
+                               op->f16_gscale);
         } else if (prec == DBIE_FP64) {
             S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
         } else {
@@ -384,6 +397,131 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     }
     DBIE_HIP(hipStreamSynchronize(st));
     return res;
+}
+
+// Solve A X = F for K columns, F and X on the device (N x K, ld = N).  Returns DBIE_OK or
+// DBIE_NOT_CONVERGED (X and stats filled either way); throws on argument or runtime errors.
+dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const dbie_solve_opts *opts,
+                      dbie_solve_stats *stats) {
+    const int64_t N = op->N();
+    dbie_solve_opts o;
+    if (opts) o = *opts;
+    else dbie_solve_opts_default(&o);
+    if (o.itmax < 1) fail(DBIE_BAD_ARG, "itmax must be >= 1");
+    if (!(o.rtol >= 0) || !(o.atol >= 0)) fail(DBIE_BAD_ARG, "tolerances must be >= 0");
+    check_precision(o.precision);
+    hipStream_t st = op->stream;
+    const size_t NK = (size_t)N * K;
+    DBuf dR, dD, dT, dinv, partial, hb;
+    const double *dinvp = nullptr;
+    if (o.precondition) {
+        std::vector<double> inv(N);
+        for (int64_t i = 0; i < N; ++i) inv[i] = 1.0 / op->diag_h[i];
+        upload(dinv, inv.data(), N, st);
+        dinvp = dinv.as<double>();
+    }
+    hipEvent_t e0, e1;
+    DBIE_HIP(hipEventCreate(&e0));
+    DBIE_HIP(hipEventCreate(&e1));
+    DBIE_HIP(hipEventRecord(e0, st));
+    dbie_solve_stats s{};
+    std::vector<double> bnorm, rn;
+    if (o.precision == DBIE_FP64) {
+        const GmresOut r = gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp);
+        s.niter = r.iters;
+        s.solved = r.solved;
+    } else {
+        // Mixed precision: FP64 iterative refinement around low-precision inner GMRES.
+        // While corrections are large the residual is recomputed with the FP64 operator;
+        // once they are small (||D_k|| <= 1e-3 ||X_k||) it is updated with the low-precision
+        // operator, R -= A_low D, whose error is ~delta*||A||*||D||, and a final FP64
+        // residual verifies convergence before the solve returns.
+        dR.ensure(sizeof(double) * NK);
+        dD.ensure(sizeof(double) * NK);
+        col_norms(op, dF, N, K, partial, hb, bnorm);
+        DBIE_HIP(hipMemsetAsync(dX, 0, sizeof(double) * NK, st));
+        DBIE_HIP(hipMemcpyAsync(dR.p, dF, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
+        rn = bnorm;
+        bool exact = true;
+        int n_exact = 0;
+        std::vector<double> dn, xn;
+        const bool trace = std::getenv("DBIE_TRACE") != nullptr;
+        auto worst = [&](const std::vector<double> &v) {
+            double m = 0.0;
+            for (int k = 0; k < K; ++k) m = std::max(m, bnorm[k] > 0 ? v[k] / bnorm[k] : v[k]);
+            return m;
+        };
+        for (int it = 0;; ++it) {
+            bool done = true;
+            std::vector<double> mask(K);
+            for (int k = 0; k < K; ++k) {
+                const bool ok = rn[k] <= o.atol + o.rtol * bnorm[k];
+                mask[k] = ok ? 0.0 : 1.0;
+                done = done && ok;
+            }
+            if (done && exact) { s.solved = 1; break; }
+            if (done) {  // verify the updated residual with the FP64 operator
+                if (trace) std::fprintf(stderr, "[dbie] verify: updated residual %.3e\n", worst(rn));
+                apply_dev(op, K, dX, N, dR.as<double>(), N, DBIE_FP64);
+                sub_from(st, dF, dR.as<double>(), N, K);
+                ++s.nrefine;
+                exact = true;
+                ++n_exact;
+                col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+                continue;
+            }
+            if (it >= o.max_refine || s.niter >= o.itmax) break;
+            hb.ensure(sizeof(double) * K);
+            h2d(hb.p, mask.data(), sizeof(double) * K, st);
+            col_scale(st, dR.as<double>(), hb.as<double>(), false, dR.as<double>(), N, K);
+            const GmresOut r = gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
+                                     o.itmax - s.niter, o.restart, dinvp);
+            s.niter += r.iters;
+            if (trace) std::fprintf(stderr, "[dbie] refine %d: inner iters %d, residual %.3e (%s)\n", it, r.iters,
+                                    worst(rn), exact ? "fp64" : "updated");
+            add_to(st, dD.as<double>(), dX, N, K);
+            bool small = n_exact >= 1;
+            if (small) {
+                col_norms(op, dD.as<double>(), N, K, partial, hb, dn);
+                col_norms(op, dX, N, K, partial, hb, xn);
+                for (int k = 0; k < K && small; ++k)
+                    if (mask[k] != 0.0 && dn[k] > 1e-3 * xn[k]) small = false;
+            }
+            if (small) {
+                dT.ensure(sizeof(double) * NK);
+                apply_dev(op, K, dD.as<double>(), N, dT.as<double>(), N, o.precision);
+                sub_to(st, dT.as<double>(), dR.as<double>(), N, K);
+                exact = false;
+            } else {
+                apply_dev(op, K, dX, N, dR.as<double>(), N, DBIE_FP64);
+                sub_from(st, dF, dR.as<double>(), N, K);
+                ++s.nrefine;
+                exact = true;
+                ++n_exact;
+            }
+            col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+        }
+    }
+    DBIE_HIP(hipEventRecord(e1, st));
+    DBIE_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DBIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    s.time_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (o.precision == DBIE_FP64 && o.check_residual) {
+        dR.ensure(sizeof(double) * NK);
+        col_norms(op, dF, N, K, partial, hb, bnorm);
+        apply_dev(op, K, dX, N, dR.as<double>(), N, DBIE_FP64);
+        sub_from(st, dF, dR.as<double>(), N, K);
+        col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+    }
+    double mx = 0.0;
+    for (size_t k = 0; k < rn.size(); ++k) mx = std::max(mx, bnorm[k] > 0 ? rn[k] / bnorm[k] : rn[k]);
+    s.max_rel_residual = rn.empty() ? -1.0 : mx;
+    DBIE_HIP(hipStreamSynchronize(st));
+    if (stats) *stats = s;
+    return s.solved ? DBIE_OK : DBIE_NOT_CONVERGED;
 }
 
 }  // namespace
@@ -722,6 +860,7 @@ dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx,
         check_op(op);
         if (K < 1 || !dX || !dY) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
         if (ldx < op->N() || ldy < op->N()) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        check_precision(precision);
         apply_dev(op, K, dX, ldx, dY, ldy, precision);
     });
 }
@@ -733,6 +872,7 @@ dbie_status dbie_apply(dbie_op *op, int K, const double *X, int64_t ldx, double 
         const int64_t N = op->N();
         if (K < 1 || !X || !Y) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
         if (ldx < N || ldy < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        check_precision(precision);
         hipStream_t st = op->stream;
         op->xin.ensure(sizeof(double) * N * K);
         op->yout.ensure(sizeof(double) * N * K);
@@ -926,99 +1066,87 @@ dbie_status dbie_pottrg(dbie_op *op, int64_t M, const double *trg, int K, const 
 
 dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, double *Sigma, int64_t lds,
                              const dbie_solve_opts *opts, dbie_solve_stats *stats) {
-    return guard([&] {
+    dbie_status rc = DBIE_OK;
+    const dbie_status g = guard([&] {
         check_op(op);
         const int64_t N = op->N();
         if (K < 1 || !F || !Sigma) fail(DBIE_BAD_ARG, "K >= 1 and non-NULL buffers required");
         if (ldf < N || lds < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
-        dbie_solve_opts o;
-        if (opts) o = *opts;
-        else dbie_solve_opts_default(&o);
-        if (o.itmax < 1) fail(DBIE_BAD_ARG, "itmax must be >= 1");
-        if (!(o.rtol >= 0) || !(o.atol >= 0)) fail(DBIE_BAD_ARG, "tolerances must be >= 0");
         hipStream_t st = op->stream;
-        const size_t NK = (size_t)N * K;
-        DBuf dF, dX, dR, dD, dinv;
-        dF.ensure(sizeof(double) * NK);
-        dX.ensure(sizeof(double) * NK);
+        DBuf dF, dX;
+        dF.ensure(sizeof(double) * N * K);
+        dX.ensure(sizeof(double) * N * K);
         DBIE_HIP(hipMemcpy2DAsync(dF.p, sizeof(double) * N, F, sizeof(double) * ldf, sizeof(double) * N, K,
                                   hipMemcpyHostToDevice, st));
-        const double *dinvp = nullptr;
-        if (o.precondition) {
-            std::vector<double> inv(N);
-            for (int64_t i = 0; i < N; ++i) inv[i] = 1.0 / op->diag_h[i];
-            upload(dinv, inv.data(), N, st);
-            dinvp = dinv.as<double>();
-        }
-        hipEvent_t e0, e1;
-        DBIE_HIP(hipEventCreate(&e0));
-        DBIE_HIP(hipEventCreate(&e1));
-        DBIE_HIP(hipEventRecord(e0, st));
-        dbie_solve_stats s{};
-        std::vector<double> bnorm, rn;
-        if (o.precision == DBIE_FP64) {
-            const GmresOut r = gmres(op, K, dF.as<double>(), dX.as<double>(), DBIE_FP64, o.rtol, o.atol, o.itmax,
-                                     o.restart, dinvp);
-            s.niter = r.iters;
-            s.solved = r.solved;
-            s.nrefine = 0;
-        } else {
-            // mixed precision: FP64 residuals around low-precision inner GMRES
-            dR.ensure(sizeof(double) * NK);
-            dD.ensure(sizeof(double) * NK);
-            DBuf partial, hb;
-            col_norms(op, dF.as<double>(), N, K, partial, hb, bnorm);
-            DBIE_HIP(hipMemsetAsync(dX.p, 0, sizeof(double) * NK, st));
-            DBIE_HIP(hipMemcpyAsync(dR.p, dF.p, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
-            rn = bnorm;
-            s.solved = 0;
-            for (int it = 0;; ++it) {
-                bool done = true;
-                std::vector<double> mask(K);
-                for (int k = 0; k < K; ++k) {
-                    const bool ok = rn[k] <= o.atol + o.rtol * bnorm[k];
-                    mask[k] = ok ? 0.0 : 1.0;
-                    done = done && ok;
-                }
-                if (done) { s.solved = 1; break; }
-                if (it >= o.max_refine || s.niter >= o.itmax) break;
-                // zero the columns that already meet the tolerance
-                hb.ensure(sizeof(double) * K);
-                h2d(hb.p, mask.data(), sizeof(double) * K, st);
-                col_scale(st, dR.as<double>(), hb.as<double>(), false, dR.as<double>(), N, K);
-                const GmresOut r = gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
-                                         o.itmax - s.niter, o.restart, dinvp);
-                s.niter += r.iters;
-                add_to(st, dD.as<double>(), dX.as<double>(), N, K);
-                apply_dev(op, K, dX.as<double>(), N, dR.as<double>(), N, DBIE_FP64);
-                sub_from(st, dF.as<double>(), dR.as<double>(), N, K);
-                ++s.nrefine;
-                col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
-            }
-        }
-        DBIE_HIP(hipEventRecord(e1, st));
-        DBIE_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        DBIE_HIP(hipEventElapsedTime(&ms, e0, e1));
-        s.time_ms = ms;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (o.precision == DBIE_FP64 && o.check_residual) {
-            dR.ensure(sizeof(double) * NK);
-            DBuf partial, hb;
-            col_norms(op, dF.as<double>(), N, K, partial, hb, bnorm);
-            apply_dev(op, K, dX.as<double>(), N, dR.as<double>(), N, DBIE_FP64);
-            sub_from(st, dF.as<double>(), dR.as<double>(), N, K);
-            col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
-        }
-        double mx = 0.0;
-        for (size_t k = 0; k < rn.size(); ++k) mx = std::max(mx, bnorm[k] > 0 ? rn[k] / bnorm[k] : rn[k]);
-        s.max_rel_residual = rn.empty() ? -1.0 : mx;
+        rc = solve_dev(op, K, dF.as<double>(), dX.as<double>(), opts, stats);
         DBIE_HIP(hipMemcpy2DAsync(Sigma, sizeof(double) * lds, dX.p, sizeof(double) * N, sizeof(double) * N, K,
                                   hipMemcpyDeviceToHost, st));
         DBIE_HIP(hipStreamSynchronize(st));
-        if (stats) *stats = s;
-        if (!s.solved) fail(DBIE_NOT_CONVERGED, "solve did not reach the tolerance within itmax / max_refine");
+    });
+    if (g != DBIE_OK) return g;
+    if (rc == DBIE_NOT_CONVERGED) set_error("solve did not reach the tolerance within itmax / max_refine");
+    return rc;
+}
+
+dbie_status dbie_solve_block_device(dbie_op *op, int K, const double *dF, int64_t ldf, double *dSigma, int64_t lds,
+                                    const dbie_solve_opts *opts, dbie_solve_stats *stats) {
+    dbie_status rc = DBIE_OK;
+    const dbie_status g = guard([&] {
+        check_op(op);
+        const int64_t N = op->N();
+        if (K < 1 || !dF || !dSigma) fail(DBIE_BAD_ARG, "K >= 1 and non-NULL buffers required");
+        if (ldf < N || lds < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        hipStream_t st = op->stream;
+        DBuf cF, cX;
+        const double *F = dF;
+        double *X = dSigma;
+        if (ldf != N) {
+            cF.ensure(sizeof(double) * N * K);
+            copy_cols(st, dF, ldf, cF.as<double>(), N, N, K);
+            F = cF.as<double>();
+        }
+        if (lds != N) {
+            cX.ensure(sizeof(double) * N * K);
+            X = cX.as<double>();
+        }
+        rc = solve_dev(op, K, F, X, opts, stats);
+        if (lds != N) copy_cols(st, X, N, dSigma, lds, N, K);
+        DBIE_HIP(hipStreamSynchronize(st));
+    });
+    if (g != DBIE_OK) return g;
+    if (rc == DBIE_NOT_CONVERGED) set_error("solve did not reach the tolerance within itmax / max_refine");
+    return rc;
+}
+
+dbie_status dbie_device_alloc(size_t bytes, void **dptr) {
+    return guard([&] {
+        ensure_device();
+        if (!dptr) fail(DBIE_BAD_ARG, "NULL output");
+        if (hipMalloc(dptr, std::max<size_t>(bytes, 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            *dptr = nullptr;
+            fail(DBIE_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+    });
+}
+
+dbie_status dbie_device_free(void *dptr) {
+    return guard([&] {
+        if (dptr) DBIE_HIP(hipFree(dptr));
+    });
+}
+
+dbie_status dbie_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    return guard([&] {
+        if (bytes && (!dst || !src)) fail(DBIE_BAD_ARG, "NULL pointer");
+        if (bytes) DBIE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+dbie_status dbie_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    return guard([&] {
+        if (bytes && (!dst || !src)) fail(DBIE_BAD_ARG, "NULL pointer");
+        if (bytes) DBIE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     });
 }
 

@@ -22,6 +22,9 @@
 // Each source split (gridDim.z) writes its own FP64 partial sum.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstring>
+
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -148,12 +151,14 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f64_kernel(DevGeom g, int K
     while (jc < walk.jend) {
         const bool more = j1 < walk.jend;
         const int64_t j2 = more ? walk.next(j1 + kS64) : walk.jend;
-        if (more) load_b(j1, bnxt);
+        // branch-free body: without a next stage the generation below writes a tile that is
+        // never read and the B prefetch re-reads the current (valid) stage
+        load_b(more ? j1 : jc, bnxt);
         double gnext[KS];
         // MFMAs of stage jc interleaved with generation of stage j1
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            if (more) gnext[ks] = gen(pb, ks);
+            gnext[ks] = gen(pb, ks);
 #pragma unroll
             for (int s = 0; s < I::kSlicesPerWave; ++s) {
                 const double a = s_g[sg][I::slice(wave, s)][ks][lane];
@@ -163,12 +168,16 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f64_kernel(DevGeom g, int K
                         __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[q][ks], acc[q * I::kSlicesPerWave + s], 0, 0, 0);
             }
         }
-        const int sn = sg == 2 ? 0 : sg + 1;
-        if (more) {
+        // ask the scheduler to co-issue: one MFMA, then two VALU, repeated
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) s_g[sn][wave][ks][lane] = gnext[ks];
-            if (j2 < walk.jend) load_pos(j2, pb ^ 1);
+        for (int m = 0; m < KS * I::kN; ++m) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
         }
+        const int sn = sg == 2 ? 0 : sg + 1;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) s_g[sn][wave][ks][lane] = gnext[ks];
+        if (j2 < walk.jend) load_pos(j2, pb ^ 1);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < I::kCbPerWave; ++q)
@@ -309,12 +318,14 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
     while (jc < walk.jend) {
         const bool more = j1 < walk.jend;
         const int64_t j2 = more ? walk.next(j1 + kSb) : walk.jend;
-        if (more) load_b(j1, bh_nxt, bl_nxt);
+        // branch-free body: without a next stage the generation below writes a tile that is
+        // never read and the B prefetch re-reads the current (valid) stage
+        load_b(more ? j1 : jc, bh_nxt, bl_nxt);
         uint4 gh[KS], gl[KS];
         // MFMAs of stage jc interleaved with generation of stage j1
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            if (more) gen_slice(pb, ks, gh[ks], gl[ks]);
+            gen_slice(pb, ks, gh[ks], gl[ks]);
 #pragma unroll
             for (int s = 0; s < I::kSlicesPerWave; ++s) {
                 const int sl = I::slice(wave, s);
@@ -333,15 +344,19 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
                 }
             }
         }
-        const int sn = sg == 2 ? 0 : sg + 1;
-        if (more) {
+        // ask the scheduler to co-issue: one MFMA, then a few VALU, repeated
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                s_g[sn][0][wave][ks][lane] = gh[ks];
-                if (X3) s_g[sn][NG - 1][wave][ks][lane] = gl[ks];
-            }
-            if (j2 < walk.jend) load_pos(j2, pb ^ 1);
+        for (int m = 0; m < KS * I::kN * (X3 ? 3 : 1); ++m) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, X3 ? 3 : 8, 0);
         }
+        const int sn = sg == 2 ? 0 : sg + 1;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            s_g[sn][0][wave][ks][lane] = gh[ks];
+            if (X3) s_g[sn][NG - 1][wave][ks][lane] = gl[ks];
+        }
+        if (j2 < walk.jend) load_pos(j2, pb ^ 1);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < I::kCbPerWave; ++q)
@@ -371,6 +386,209 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
         }
 }
 
+// ---------------------------------------------------------------- FP16 x3
+// Two-term fp16 split of both operands (11-bit mantissas, ~22 bits together) with
+// hi*hi + hi*lo + lo*hi products accumulated in FP32: about 2e-7 per entry.  Coordinate
+// differences are formed in FP64 before rounding so that close node pairs keep their
+// accuracy.  Range: entries are scaled by 2^-eG (|G| <= 1/r_min^2 <= 2^14 after scaling)
+// and each column of w.*X by 2^e_k (max |.| in [2^13, 2^14)); both are undone on output.
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+
+__global__ void zero_u32_kernel(unsigned *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// per-column max |w_j X_jk| as float bits (non-negative floats order like unsigned ints)
+__global__ void colmax_kernel(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
+                              unsigned *__restrict__ cmax) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    float m = 0.0f;
+    for (int64_t jj = j; jj < N; jj += (int64_t)gridDim.x * blockDim.x) m = fmaxf(m, fabsf((float)(w[jj] * X[jj + (int64_t)k * ldx])));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(cmax + k, __float_as_uint(m));
+}
+
+__device__ inline int col_exponent(unsigned bits) {
+    const float m = __uint_as_float(bits);
+    if (!(m > 0.0f)) return 0;
+    int e;
+    (void)frexpf(m, &e);  // m < 2^e
+    return 14 - e;
+}
+
+__global__ void split_tiles_f16(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
+                                int K, int Kp, const unsigned *__restrict__ cmax, _Float16 *__restrict__ zh,
+                                _Float16 *__restrict__ zl) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    const int64_t ntile = (N + kSb - 1) / kSb;
+    if (j >= ntile * kSb) return;
+    _Float16 h = (_Float16)0.0f, l = (_Float16)0.0f;
+    if (j < N && k < K) {
+        const double zd = ldexp(w[j] * X[j + (int64_t)k * ldx], col_exponent(cmax[k]));
+        h = (_Float16)(float)zd;
+        l = (_Float16)(float)(zd - (double)(float)h);
+    }
+    const int64_t idx = ((j / kSb) * Kp + k) * kSb + (j % kSb);
+    zh[idx] = h;
+    zl[idx] = l;
+}
+
+template <int NCB>
+__global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int K, int Kp, const _Float16 *__restrict__ zh,
+                                                                  const _Float16 *__restrict__ zl,
+                                                                  const unsigned *__restrict__ cmax, float gscale,
+                                                                  double *__restrict__ part, int src_per_split) {
+    using I = Items<NCB>;
+    constexpr int KS = kSb / 32;
+    __shared__ uint4 s_g[3][2][kWaves][KS][64];  // [stage % 3][hi/lo][slice][k-slice][lane]
+    __shared__ double4 s_pos[2][kSb];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t N = g.N;
+    const int64_t i0 = (int64_t)blockIdx.x * kTargets;
+    const int k0 = blockIdx.y * NCB * 16;
+    const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
+    const StageWalk walk{min(N, jb0 + src_per_split), g.tile_plane ? g.tile_plane[i0 / 256] : -1, g.tile_plane};
+
+    int64_t tg = i0 + wave * 16 + (lane & 15);
+    tg = tg < N ? tg : N - 1;
+    const double tx = g.x[tg], ty = g.y[tg], tz = g.z[tg];
+    const float tn0 = g.nxf[tg], tn1 = g.nyf[tg], tn2 = g.nzf[tg];
+
+    v4f acc[I::kN];
+#pragma unroll
+    for (int q = 0; q < I::kN; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
+    uint4 bh_cur[I::kCbPerWave][KS], bh_nxt[I::kCbPerWave][KS];
+    uint4 bl_cur[I::kCbPerWave][KS], bl_nxt[I::kCbPerWave][KS];
+    auto load_b = [&](int64_t jb, uint4 (&bh)[I::kCbPerWave][KS], uint4 (&bl)[I::kCbPerWave][KS]) {
+        const int64_t base = ((jb / kSb) * Kp + k0) * kSb;
+#pragma unroll
+        for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int64_t off = base + (int64_t)(I::cb(wave, q) * 16 + (lane & 15)) * kSb + ks * 32 + 8 * (lane >> 4);
+                bh[q][ks] = *reinterpret_cast<const uint4 *>(zh + off);
+                bl[q][ks] = *reinterpret_cast<const uint4 *>(zl + off);
+            }
+    };
+    auto load_pos = [&](int64_t jb, int pb) {
+        if (tid < kSb) {
+            const int64_t j = jb + tid;
+            s_pos[pb][tid] = j < walk.jend ? make_double4(g.x[j], g.y[j], g.z[j], 0.0)
+                                           : make_double4(1.0e18, 1.0e18, 1.0e18, 0.0);
+        }
+    };
+    auto gen2 = [&](int pb, int ks, int e, unsigned &hi, unsigned &lo) {
+        float v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const double4 sp = s_pos[pb][ks * 32 + 8 * (lane >> 4) + e + u];
+            const float dx = (float)(tx - sp.x), dy = (float)(ty - sp.y), dz = (float)(tz - sp.z);
+            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            v[u] = ((tn0 * dx + tn1 * dy + tn2 * dz) * gscale) * (rinv * rinv * rinv);
+        }
+        const v2h h = __builtin_convertvector(v2f{v[0], v[1]}, v2h);
+        const v2f hf = __builtin_convertvector(h, v2f);
+        const v2h l = __builtin_convertvector(v2f{v[0] - hf[0], v[1] - hf[1]}, v2h);
+        hi = __builtin_bit_cast(unsigned, h);
+        lo = __builtin_bit_cast(unsigned, l);
+    };
+    auto gen_slice = [&](int pb, int ks, uint4 &h4, uint4 &l4) {
+        unsigned h[4], l[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) gen2(pb, ks, 2 * e, h[e], l[e]);
+        h4 = uint4{h[0], h[1], h[2], h[3]};
+        l4 = uint4{l[0], l[1], l[2], l[3]};
+    };
+
+    int64_t jc = walk.next(jb0);
+    int64_t j1 = walk.jend;
+    if (jc < walk.jend) {
+        j1 = walk.next(jc + kSb);
+        load_pos(jc, 0);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            uint4 h4, l4;
+            gen_slice(0, ks, h4, l4);
+            s_g[0][0][wave][ks][lane] = h4;
+            s_g[0][1][wave][ks][lane] = l4;
+        }
+        if (j1 < walk.jend) load_pos(j1, 1);
+        load_b(jc, bh_cur, bl_cur);
+        __syncthreads();
+    }
+    int sg = 0, pb = 1;
+    while (jc < walk.jend) {
+        const bool more = j1 < walk.jend;
+        const int64_t j2 = more ? walk.next(j1 + kSb) : walk.jend;
+        load_b(more ? j1 : jc, bh_nxt, bl_nxt);
+        uint4 gh[KS], gl[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            gen_slice(pb, ks, gh[ks], gl[ks]);
+#pragma unroll
+            for (int s = 0; s < I::kSlicesPerWave; ++s) {
+                const int sl = I::slice(wave, s);
+                const v8h ah = __builtin_bit_cast(v8h, s_g[sg][0][sl][ks][lane]);
+                const v8h al = __builtin_bit_cast(v8h, s_g[sg][1][sl][ks][lane]);
+#pragma unroll
+                for (int q = 0; q < I::kCbPerWave; ++q) {
+                    v4f &c = acc[q * I::kSlicesPerWave + s];
+                    const v8h bh = __builtin_bit_cast(v8h, bh_cur[q][ks]);
+                    const v8h bl = __builtin_bit_cast(v8h, bl_cur[q][ks]);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < KS * I::kN * 3; ++m) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+        }
+        const int sn = sg == 2 ? 0 : sg + 1;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            s_g[sn][0][wave][ks][lane] = gh[ks];
+            s_g[sn][1][wave][ks][lane] = gl[ks];
+        }
+        if (j2 < walk.jend) load_pos(j2, pb ^ 1);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                bh_cur[q][ks] = bh_nxt[q][ks];
+                bl_cur[q][ks] = bl_nxt[q][ks];
+            }
+        sg = sn;
+        pb ^= 1;
+        jc = j1;
+        j1 = j2;
+    }
+    const double out_scale = kInv4Pi / (double)gscale;
+#pragma unroll
+    for (int q = 0; q < I::kCbPerWave; ++q)
+#pragma unroll
+        for (int s = 0; s < I::kSlicesPerWave; ++s) {
+            const int col = k0 + I::cb(wave, q) * 16 + (lane & 15);
+            if (col >= K) continue;
+            const double cs = ldexp(out_scale, -col_exponent(cmax[col]));
+            double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t i = i0 + I::slice(wave, s) * 16 + 4 * (lane >> 4) + r;
+                if (i < N) dst[i] = (double)acc[q * I::kSlicesPerWave + s][r] * cs;
+            }
+        }
+}
+
 // 16-column blocks per workgroup for K columns (power of two, at most 16)
 int ncb_for(int K) {
     const int nb = (K + 15) / 16;
@@ -389,11 +607,57 @@ int mfma_splits(int64_t N, int K, int64_t groups, int64_t part_cap, int64_t *per
     *per_out = per;
     return (int)((N + per - 1) / per);
 }
+// max over all node pairs of |n_i . (x_i - y_j)| / r^3, in the arithmetic of the fp16 tile
+// generator (FP64 differences rounded to FP32), as float bits.  grid: (target tiles, source splits)
+__global__ __launch_bounds__(256) void gmax_kernel(DevGeom g, unsigned *__restrict__ out) {
+    __shared__ double4 s_pos[256];
+    const int64_t N = g.N;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t ic = i < N ? i : N - 1;
+    const double tx = g.x[ic], ty = g.y[ic], tz = g.z[ic];
+    const float n0 = g.nxf[ic], n1 = g.nyf[ic], n2 = g.nzf[ic];
+    const int64_t per = ((N + gridDim.y - 1) / gridDim.y + 255) / 256 * 256;
+    const int64_t j0 = (int64_t)blockIdx.y * per, j1 = min(N, j0 + per);
+    float m = 0.0f;
+    for (int64_t jb = j0; jb < j1; jb += 256) {
+        __syncthreads();
+        const int64_t j = jb + threadIdx.x;
+        s_pos[threadIdx.x] = j < j1 ? make_double4(g.x[j], g.y[j], g.z[j], 0.0) : make_double4(tx, ty, tz, 0.0);
+        __syncthreads();
+        for (int u = 0; u < 256; ++u) {
+            const double4 sp = s_pos[u];
+            const float dx = (float)(tx - sp.x), dy = (float)(ty - sp.y), dz = (float)(tz - sp.z);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > 0.0f) {
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                m = fmaxf(m, fabsf(n0 * dx + n1 * dy + n2 * dz) * (rinv * rinv * rinv));
+            }
+        }
+    }
+    if (i >= N) m = 0.0f;
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
 }  // namespace
+
+float far_field_f16_scale(hipStream_t st, const DevGeom &g, void *tmp4) {
+    unsigned *d = static_cast<unsigned *>(tmp4);
+    unsigned h = 0;
+    zero_u32_kernel<<<1, 64, 0, st>>>(d, 1);
+    gmax_kernel<<<dim3((unsigned)((g.N + 255) / 256), 8), 256, 0, st>>>(g, d);
+    if (hipMemcpyAsync(&h, d, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) return 0.0f;
+    if (hipStreamSynchronize(st) != hipSuccess) return 0.0f;
+    float gm;
+    std::memcpy(&gm, &h, sizeof gm);
+    if (!(gm > 0.0f) || !std::isfinite(gm)) return 1.0f;
+    int e;
+    (void)std::frexp(gm, &e);  // gm < 2^e
+    return std::ldexp(1.0f, 14 - e);  // scaled entries stay below 2^14
+}
 
 bool far_field_mfma_supported(int prec, int K) {
     if (K < 16) return false;
-    return prec == 0 || prec == 2 || prec == 3;
+    return prec == 0 || prec == 2 || prec == 3 || prec == 4;
 }
 
 size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K) {
@@ -401,11 +665,12 @@ size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K) {
     const int Kp = ((K + ncb * 16 - 1) / (ncb * 16)) * ncb * 16;
     if (prec == 0) return sizeof(double) * (size_t)((N + kS64 - 1) / kS64) * kS64 * Kp;
     const size_t zelems = (size_t)((N + kSb - 1) / kSb) * Kp * kSb;
+    if (prec == 4) return sizeof(_Float16) * zelems * 2 + sizeof(unsigned) * (size_t)Kp;
     return sizeof(__bf16) * zelems * (prec == 3 ? 2 : 1);
 }
 
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap, void *scratch) {
+                   int64_t part_cap, void *scratch, float gscale) {
     const int64_t N = g.N;
     const int64_t bx = (N + kTargets - 1) / kTargets;
     const int ncb = ncb_for(K);
@@ -427,9 +692,24 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         }
         return S;
     }
-    const bool x3 = prec == 3;
     const int64_t ntile = (N + kSb - 1) / kSb;
     const size_t zelems = (size_t)ntile * Kp * kSb;
+    if (prec == 4) {
+        _Float16 *zh = static_cast<_Float16 *>(scratch);
+        _Float16 *zl = zh + zelems;
+        unsigned *cmax = reinterpret_cast<unsigned *>(zl + zelems);
+        zero_u32_kernel<<<(Kp + 255) / 256, 256, 0, st>>>(cmax, Kp);
+        colmax_kernel<<<dim3(64, (unsigned)K), 256, 0, st>>>(X, ldx, g.w, N, cmax);
+        split_tiles_f16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
+                                                                                                    cmax, zh, zl);
+        switch (ncb) {
+#define DBIE_H3(n) case n: far_mfma_f16x3_kernel<n><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per); break;
+            DBIE_H3(1) DBIE_H3(2) DBIE_H3(4) DBIE_H3(8) DBIE_H3(16)
+#undef DBIE_H3
+        }
+        return S;
+    }
+    const bool x3 = prec == 3;
     __bf16 *zh = static_cast<__bf16 *>(scratch);
     __bf16 *zl = x3 ? zh + zelems : nullptr;
     split_tiles_bf16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,

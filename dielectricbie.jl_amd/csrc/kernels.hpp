@@ -41,12 +41,16 @@ int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const d
 int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int64_t ldx, double *part,
                   int64_t part_cap);
 // MFMA contraction of the DT far field: tiles generated on the fly and contracted with the
-// RHS block.  prec: 0 = FP64 MFMA, 2 = BF16 MFMA, 3 = split BF16 (3 products).
+// RHS block.  prec: 0 = FP64 MFMA, 2 = BF16 MFMA, 3 = split BF16 (3 products), 4 = split FP16.
 // scratch: far_field_mfma_scratch_bytes(N, prec, K) bytes of device memory (the
 // pre-scaled RHS block w.*X in FP64, or its tile-major bf16 split).
+// gscale: power-of-two scale of the kernel entries for the fp16 form (prec 4)
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap, void *scratch);
+                   int64_t part_cap, void *scratch, float gscale);
 size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K);
+// Power-of-two scale for the fp16 kernel entries: 2^(14 - e) with max |G_ij| < 2^e over all
+// node pairs.  tmp4: 4 bytes of device memory.  Synchronises the stream; 0 on a HIP error.
+float far_field_f16_scale(hipStream_t st, const DevGeom &g, void *tmp4);
 bool far_field_mfma_supported(int prec, int K);
 int64_t far_field_part_elems(int64_t N, int K);  // workspace needed for any far-field launch
 
@@ -87,6 +91,8 @@ void block_combine(hipStream_t st, const double *V, int nv, const double *y, dou
 void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K);
 // X += D
 void add_to(hipStream_t st, const double *D, double *X, int64_t N, int K);
+// X -= D
+void sub_to(hipStream_t st, const double *D, double *X, int64_t N, int K);
 // copy with leading dimensions
 void copy_cols(hipStream_t st, const double *src, int64_t lds, double *dst, int64_t ldd, int64_t N, int K);
 

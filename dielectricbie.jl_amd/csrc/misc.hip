@@ -219,6 +219,12 @@ __global__ void add_to_kernel(const double *__restrict__ D, double *__restrict__
     if (n < N) X[(int64_t)k * N + n] += D[(int64_t)k * N + n];
 }
 
+__global__ void sub_to_kernel(const double *__restrict__ D, double *__restrict__ X, int64_t N, int K) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (n < N) X[(int64_t)k * N + n] -= D[(int64_t)k * N + n];
+}
+
 __global__ void copy_cols_kernel(const double *__restrict__ src, int64_t lds, double *__restrict__ dst, int64_t ldd,
                                  int64_t N) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -291,6 +297,10 @@ void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K) {
 
 void add_to(hipStream_t st, const double *D, double *X, int64_t N, int K) {
     add_to_kernel<<<grid2(N, K), 256, 0, st>>>(D, X, N, K);
+}
+
+void sub_to(hipStream_t st, const double *D, double *X, int64_t N, int K) {
+    sub_to_kernel<<<grid2(N, K), 256, 0, st>>>(D, X, N, K);
 }
 
 void copy_cols(hipStream_t st, const double *src, int64_t lds, double *dst, int64_t ldd, int64_t N, int K) {

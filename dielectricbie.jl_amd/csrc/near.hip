@@ -76,34 +76,54 @@ __global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__rest
                                                         int64_t ldy) {
     __shared__ double s_out[256][17];  // [column within the 256-column group][row], padded
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int per = (P + 7) / 8;
     const int p = xcd_row(blockIdx.x, P);
     const int k0 = blockIdx.y * 256;
-    if (p >= P || blockIdx.x >= (unsigned)(8 * per)) return;  // uniform per workgroup
+    if (p >= P) return;  // uniform per workgroup
     v4d acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    // column of this lane for each of its 4 column blocks (clamped; masked by a zero multiplier)
+    int64_t coff[4];
+    double cmask[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = k0 + (wave + 4 * q) * 16 + (lane & 15);
+        cmask[q] = c < K ? 1.0 : 0.0;
+        coff[q] = (int64_t)(c < K ? c : K - 1) * ldx;
+    }
     const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
-    for (int b = b0; b < b1; ++b) {
+    // operands of one block: A (4 k-slices) and X (4 k-slices x 4 column blocks), prefetched one block ahead
+    double an[4], xn[4][4];
+    auto load = [&](int b, double (&a)[4], double (&xv)[4][4]) {
         const double *v = val + (int64_t)b * 256;
-        const int64_t src_row = (int64_t)col[b] * 16;
+        const int64_t src_row = (int64_t)col[b] * 16 + (lane >> 4);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            const double a = v[kk * 64 + lane];  // C_b[t = lane%16][c = kk*4 + lane/16]
-            const int64_t xr = src_row + kk * 4 + (lane >> 4);
+            a[kk] = v[kk * 64 + lane];  // C_b[t = lane%16][c = kk*4 + lane/16]
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int c = k0 + (wave + 4 * q) * 16 + (lane & 15);
-                const double xv = c < K ? X[xr + (int64_t)c * ldx] : 0.0;
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xv, acc[q], 0, 0, 0);
-            }
+            for (int q = 0; q < 4; ++q) xv[kk][q] = X[src_row + kk * 4 + coff[q]];
         }
+    };
+    if (b0 < b1) load(b0, an, xn);
+    for (int b = b0; b < b1; ++b) {
+        double a[4], xv[4][4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            a[kk] = an[kk];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[kk][q] = xn[kk][q];
+        }
+        if (b + 1 < b1) load(b + 1, an, xn);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], xv[kk][q], acc[q], 0, 0, 0);
     }
     // D(row = 4r + lane/16, col = lane%16) -> LDS [col][row]
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) s_out[(wave + 4 * q) * 16 + (lane & 15)][4 * r + (lane >> 4)] = acc[q][r];
+        for (int r = 0; r < 4; ++r) s_out[(wave + 4 * q) * 16 + (lane & 15)][4 * r + (lane >> 4)] = acc[q][r] * cmask[q];
     __syncthreads();
     // each thread finishes 16 outputs: row = tid % 16, columns tid/16 + 16*m
     const int row = tid & 15;

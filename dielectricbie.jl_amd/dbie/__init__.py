@@ -6,6 +6,7 @@ runs in libdbie.so (HIP kernels for gfx950 behind a C ABI, include/dbie.h).
 from ._lib import DbieError, NotConverged, lib, LIB_PATH  # noqa: F401
 from .api import (  # noqa: F401
     BatchedDielectricOperator,
+    DeviceMatrix,
     DielectricInterface,
     PointSource,
     SharpScreening,

@@ -14,9 +14,9 @@ import numpy as np
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdbie.so")
 
 OK, BAD_ARG, DIM_MISMATCH, OOM, HIP_ERROR, NOT_CONVERGED, NO_DEVICE = range(7)
-FP64, FP32, BF16, BF16X3 = 0, 1, 2, 3
+FP64, FP32, BF16, BF16X3, FP16X3 = 0, 1, 2, 3, 4
 OP_DT, OP_S, OP_D = 0, 1, 2
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16x3": BF16X3}
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16x3": BF16X3, "fp16x3": FP16X3}
 
 
 class DbieError(RuntimeError):
@@ -88,6 +88,12 @@ SIGNATURES = {
     "dbie_op_profile": [_vp, C.c_int],
     "dbie_op_profile_read": [_vp, C.c_int, _i64p, _dp],
     "dbie_apply_device": [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, C.c_int],
+    "dbie_solve_block_device": [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, C.POINTER(SolveOpts),
+                                C.POINTER(SolveStats)],
+    "dbie_device_alloc": [C.c_size_t, C.POINTER(_vp)],
+    "dbie_device_free": [_vp],
+    "dbie_memcpy_h2d": [_vp, _vp, C.c_size_t],
+    "dbie_memcpy_d2h": [_vp, _vp, C.c_size_t],
     "dbie_version": [],
 }
 _RESTYPES = {"dbie_version": C.c_char_p, "dbie_op_stream": _vp, "dbie_geom_destroy": None,

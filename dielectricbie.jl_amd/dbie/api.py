@@ -347,6 +347,25 @@ class BatchedDielectricOperator:
             L.check(status)
         return (S[:, 0] if vec else S), stats
 
+    def solve_device(self, dF: "DeviceMatrix", dSigma: "DeviceMatrix", *, rtol=1e-10, atol=0.0, itmax=500,
+                     restart=0, precondition=True, precision="fp64", inner_rtol=1e-4, max_refine=10,
+                     check_residual=True, raise_on_fail=True):
+        """Solve with F and Σ resident in device memory (dbie_solve_block_device); returns stats."""
+        if dF.shape[0] != self.n or dSigma.shape != dF.shape:
+            raise ValueError("DimensionMismatch: device blocks must be N x K and equal in shape")
+        o = L.SolveOpts()
+        L.lib().dbie_solve_opts_default(C.byref(o))
+        o.rtol, o.atol, o.itmax, o.restart = float(rtol), float(atol), int(itmax), int(restart)
+        o.precondition = int(bool(precondition))
+        o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
+        st = L.SolveStats()
+        status = L.lib().dbie_solve_block_device(self._h, dF.shape[1], dF.ptr, self.n, dSigma.ptr, self.n,
+                                                 C.byref(o), C.byref(st))
+        if not (status == L.NOT_CONVERGED and not raise_on_fail):
+            L.check(status)
+        return st.as_dict()
+
     def pottrg(self, targets, Sigma):
         """Induced potential u[σ](t) = Σ_i w_i σ_i /(4π|t - x_i|) at targets (3 x M or M x 3)."""
         t = np.asarray(targets, np.float64)
@@ -361,6 +380,37 @@ class BatchedDielectricOperator:
         U = np.zeros((M, K), order="F")
         L.check(L.lib().dbie_pottrg(self._h, M, L.dptr(t.ravel()), K, L.dptr(Sf), self.n, L.dptr(U), M))
         return U[:, 0] if vec else U
+
+
+class DeviceMatrix:
+    """An N x K column-major FP64 block in device memory (HBM), owned by this object."""
+
+    def __init__(self, rows: int, cols: int):
+        self.shape = (int(rows), int(cols))
+        self.nbytes = 8 * self.shape[0] * self.shape[1]
+        p = C.c_void_p()
+        L.check(L.lib().dbie_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, A):
+        A = L.fortran(A)
+        d = cls(*A.shape)
+        L.check(L.lib().dbie_memcpy_h2d(d.ptr, A.ctypes.data_as(C.c_void_p), d.nbytes))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.zeros(self.shape, order="F")
+        L.check(L.lib().dbie_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and L._lib is not None:
+                L._lib.dbie_device_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
 
 
 def _op_for(interface, key, **kw):

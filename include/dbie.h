@@ -44,7 +44,9 @@ typedef enum {
     DBIE_FP64 = 0,   /* FP64 tile generation and FP64 accumulation (the plain path) */
     DBIE_FP32 = 1,   /* FP32 tile generation, FP32 accumulation */
     DBIE_BF16 = 2,   /* FP32 tile generation, BF16 MFMA contraction, FP32 accumulation */
-    DBIE_BF16X3 = 3  /* FP32 generation, split-BF16 (hi*hi+hi*lo+lo*hi) MFMA, FP32 accumulation */
+    DBIE_BF16X3 = 3, /* FP32 generation, split-BF16 (hi*hi+hi*lo+lo*hi) MFMA, FP32 accumulation */
+    DBIE_FP16X3 = 4  /* FP64 differences, FP32 generation, scaled split-FP16 (3 products) MFMA,
+                        FP32 accumulation: entry error about 2e-7 */
 } dbie_precision;
 
 /* Which boundary operator a plain apply evaluates (src/kernel/laplace3d.jl:18-52). */
@@ -199,6 +201,16 @@ dbie_status dbie_op_profile_read(dbie_op *op, int kernel, int64_t *launches, dou
 void *dbie_op_stream(dbie_op *op);
 dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
                               dbie_precision precision);
+/* dbie_solve_block with F and Sigma resident in device memory (solve_dielectric_box3d_block,
+   src/solver/multi_rhs.jl:270-286); returns after the solve completes. */
+dbie_status dbie_solve_block_device(dbie_op *op, int K, const double *dF, int64_t ldf, double *dSigma, int64_t lds,
+                                    const dbie_solve_opts *opts, dbie_solve_stats *stats);
+/* Device memory helpers for callers without a GPU array library (no counterpart in the
+   reference, whose arrays live in host memory). */
+dbie_status dbie_device_alloc(size_t bytes, void **dptr);
+dbie_status dbie_device_free(void *dptr);
+dbie_status dbie_memcpy_h2d(void *dst, const void *src, size_t bytes);
+dbie_status dbie_memcpy_d2h(void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -117,14 +117,32 @@ def test_mfma_apply_parity(cfg1):
         assert rel(Y, Yr) < 1e-12
     X = rng.standard_normal((g.n_points, 48))
     Y64 = op @ X
-    for prec, bound in (("fp32", 1e-5), ("bf16", 2e-2), ("bf16x3", 2e-4)):
+    for prec, bound in (("fp32", 1e-5), ("bf16", 2e-2), ("bf16x3", 2e-4), ("fp16x3", 2e-6)):
         say(f"low-precision apply {prec} K=48")
         Yp = op.apply(X, prec)
+        errs = [rel(Yp[:, k], Y64[:, k]) for k in range(48)]
+        say(f"  max column error {max(errs):.3e}")
         for k in range(48):
-            assert rel(Yp[:, k], Y64[:, k]) < bound, (prec, k)
+            assert errs[k] < bound, (prec, k)
 
 
-@pytest.mark.parametrize("precision,K", [("fp32", 8), ("bf16", 8), ("bf16", 16), ("bf16x3", 24), ("fp32", 20)])
+def test_fp16x3_column_scaling(cfg1):
+    """Columns spanning 40 decades keep their relative accuracy (per-column power-of-two scaling)."""
+    g, op, ref_if, ref = cfg1
+    rng = np.random.default_rng(12)
+    K = 32
+    X = rng.standard_normal((g.n_points, K)) * (10.0 ** np.linspace(-20, 20, K))[None, :]
+    X[:, 5] = 0.0  # an all-zero column stays exactly zero
+    Y64 = op @ X
+    Yp = op.apply(X, "fp16x3")
+    assert np.all(Yp[:, 5] == 0.0)
+    for k in range(K):
+        if k != 5:
+            assert rel(Yp[:, k], Y64[:, k]) < 2e-6, k
+
+
+@pytest.mark.parametrize("precision,K", [("fp32", 8), ("bf16", 8), ("bf16", 16), ("bf16x3", 24), ("fp32", 20),
+                                               ("fp16x3", 16), ("fp16x3", 40)])
 def test_mixed_precision_solve(dbie, cfg1, precision, K):
     g, op, ref_if, ref = cfg1
     say(f"mixed-precision IR ({precision}, K={K}) vs FP64 solve")
@@ -178,3 +196,20 @@ def test_config2_sampled_rows(dbie, O):
     y_ref = ref_if.apply_DT_rows(rows, x) + nl.apply(x)[rows] + ref_if.diag()[rows] * x[rows]
     assert rel(y[rows], y_ref) < 1e-12
     assert op.n_blocks == 196768 + 4080
+
+
+def test_device_resident_solve(dbie, cfg1):
+    g, op, ref_if, ref = cfg1
+    say("device-resident solve == host solve (FP64 and mixed)")
+    rng = np.random.default_rng(9)
+    F = dbie.rhs_point_charges(g, rng.uniform(-0.25, 0.25, (32, 3)), rng.uniform(-1, 1, 32), 4.0)
+    dF = dbie.DeviceMatrix.from_host(F)
+    dS = dbie.DeviceMatrix(g.n_points, 32)
+    S_host, _ = op.solve(F, rtol=1e-12)
+    st = op.solve_device(dF, dS, rtol=1e-12)
+    assert st["solved"] and rel(dS.to_host(), S_host) < 1e-12
+    st = op.solve_device(dF, dS, rtol=1e-11, precision="bf16x3", inner_rtol=1e-5)
+    assert st["solved"]
+    Sm = dS.to_host()
+    for k in range(32):
+        assert rel(Sm[:, k], S_host[:, k]) < 1e-8
