@@ -20,6 +20,7 @@
 
 #include "internal.hpp"
 #include "kernels.hpp"
+#include "runtime.hpp"
 
 namespace dbie {
 
@@ -31,10 +32,11 @@ void hip_check(hipError_t e, const char *what) {
 }
 
 namespace {
-
 std::once_flag g_init_flag;
 dbie_status g_init_status = DBIE_OK;
 std::string g_init_msg;
+
+}  // namespace
 
 void ensure_device() {
     std::call_once(g_init_flag, [] {
@@ -57,48 +59,6 @@ void ensure_device() {
     });
     if (g_init_status != DBIE_OK) fail(g_init_status, g_init_msg);
 }
-
-// Device buffer with RAII.
-struct DBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void ensure(size_t b) {
-        if (b <= bytes) return;
-        release();
-        if (hipMalloc(&p, b) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            fail(DBIE_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed");
-        }
-        bytes = b;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-// Host-to-device copy that completes before returning (host sources may be temporaries).
-template <class T>
-void upload(DBuf &b, const T *src, size_t n, hipStream_t st) {
-    b.ensure(std::max<size_t>(n, 1) * sizeof(T));
-    if (n) {
-        DBIE_HIP(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, st));
-        DBIE_HIP(hipStreamSynchronize(st));
-    }
-}
-void h2d(void *dst, const void *src, size_t bytes, hipStream_t st) {
-    DBIE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
-    DBIE_HIP(hipStreamSynchronize(st));
-}
-
-}  // namespace
 
 }  // namespace dbie
 
@@ -134,23 +94,6 @@ struct dbie_op {
 
 namespace dbie {
 namespace {
-
-template <class F>
-dbie_status guard(F &&f) {
-    try {
-        f();
-        return DBIE_OK;
-    } catch (const Error &e) {
-        set_error(e.msg);
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        set_error("host allocation failed");
-        return DBIE_OOM;
-    } catch (const std::exception &e) {
-        set_error(e.what());
-        return DBIE_BAD_ARG;
-    }
-}
 
 void check_op(const dbie_op *op) {
     if (!op) fail(DBIE_BAD_ARG, "operator handle is NULL");

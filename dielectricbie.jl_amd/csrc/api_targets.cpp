@@ -1,0 +1,306 @@
+// api_targets.cpp -- C ABI of the off-surface target map and the reaction matrix.
+//
+//   dbie_tmap_create      laplace3d_pottrg_fmm3d_corrected_hcubature (src/kernel/laplace3d_near_hcubature.jl:337-375)
+//   dbie_tmap_apply       the returned LinearMap applied to densities: U = u[Sigma] at the targets
+//   dbie_reaction_matrix  four_index_matrix (src/solver/multi_rhs.jl:299-321), induced term
+//
+// dbie_tmap_create builds the refined interface, the prolongation and the near-target
+// corrections on the host (targets.cpp, no device needed); the first evaluation uploads them and
+// the device evaluates U or R.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstring>
+#include <memory>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+using namespace dbie;
+
+struct dbie_tmap {
+    std::unique_ptr<TargetMap> h;
+    int64_t N = 0;  // coarse nodes
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    DBuf x, y, z, w, trg;                      // refined nodes, targets
+    DBuf p_ptr, p_col, p_val;                  // prolongation (N_ref rows)
+    DBuf pt_ptr, pt_col, pt_val;               // transposed prolongation (N rows)
+    DBuf c_ptr, c_col, c_val;                  // corrections (M rows)
+    DBuf xr, xin, uout;                        // workspaces
+    DevGeom dg;
+    ~dbie_tmap() {
+        if (blas) (void)rocblas_destroy_handle(blas);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void check_tm(const dbie_tmap *tm) {
+    if (!tm) fail(DBIE_BAD_ARG, "target map handle is NULL");
+}
+
+void blas_check(rocblas_status s, const char *what) {
+    if (s != rocblas_status_success) fail(DBIE_HIP_ERROR, std::string(what) + ": rocBLAS status " + std::to_string((int)s));
+}
+
+// Uploads the refined nodes, targets and sparse matrices on first device use.
+void make_resident(dbie_tmap *tm) {
+    if (tm->stream) return;
+    ensure_device();
+    const TargetMap &h = *tm->h;
+    DBIE_HIP(hipStreamCreateWithFlags(&tm->stream, hipStreamNonBlocking));
+    hipStream_t st = tm->stream;
+    const Geometry &R = h.ref;
+    upload(tm->x, R.x.data(), R.N, st);
+    upload(tm->y, R.y.data(), R.N, st);
+    upload(tm->z, R.z.data(), R.N, st);
+    upload(tm->w, R.w.data(), R.N, st);
+    upload(tm->trg, h.trg.data(), h.trg.size(), st);
+    upload(tm->p_ptr, h.p_ptr.data(), h.p_ptr.size(), st);
+    upload(tm->p_col, h.p_col.data(), h.p_col.size(), st);
+    upload(tm->p_val, h.p_val.data(), h.p_val.size(), st);
+    upload(tm->c_ptr, h.c_ptr.data(), h.c_ptr.size(), st);
+    upload(tm->c_col, h.c_col.data(), h.c_col.size(), st);
+    upload(tm->c_val, h.c_val.data(), h.c_val.size(), st);
+    {  // transpose of the prolongation: rows = coarse nodes
+        const int64_t N = tm->N, Nr = R.N;
+        std::vector<int64_t> tp(N + 1, 0);
+        for (int c : h.p_col) ++tp[c + 1];
+        for (int64_t c = 0; c < N; ++c) tp[c + 1] += tp[c];
+        std::vector<int> tc(h.p_col.size());
+        std::vector<double> tv(h.p_col.size());
+        std::vector<int64_t> fill(tp.begin(), tp.end() - 1);
+        for (int64_t r = 0; r < Nr; ++r)
+            for (int64_t e = h.p_ptr[r]; e < h.p_ptr[r + 1]; ++e) {
+                const int64_t at = fill[h.p_col[e]]++;
+                tc[at] = (int)r;
+                tv[at] = h.p_val[e];
+            }
+        upload(tm->pt_ptr, tp.data(), tp.size(), st);
+        upload(tm->pt_col, tc.data(), tc.size(), st);
+        upload(tm->pt_val, tv.data(), tv.size(), st);
+    }
+    DevGeom &d = tm->dg;
+    d.N = R.N;
+    d.x = tm->x.as<double>();
+    d.y = tm->y.as<double>();
+    d.z = tm->z.as<double>();
+    d.w = tm->w.as<double>();
+    DBIE_HIP(hipStreamSynchronize(st));
+}
+
+// U (M x K) = base(P Sigma) + C (P Sigma), everything on the device.
+void tmap_apply_dev(dbie_tmap *tm, int K, const double *dS, int64_t lds, double *dU, int64_t ldu) {
+    make_resident(tm);
+    const TargetMap &h = *tm->h;
+    const int64_t Nr = h.ref.N, M = h.M;
+    hipStream_t st = tm->stream;
+    tm->xr.ensure(sizeof(double) * (size_t)Nr * K);
+    double *Xr = tm->xr.as<double>();
+    csr_spmm(st, Nr, tm->p_ptr.as<int64_t>(), tm->p_col.as<int>(), tm->p_val.as<double>(), K, dS, lds, Xr, Nr, false);
+    if (M > 0) {
+        pottrg(st, tm->dg, M, tm->trg.as<double>(), K, Xr, Nr, dU, ldu);
+        csr_spmm(st, M, tm->c_ptr.as<int64_t>(), tm->c_col.as<int>(), tm->c_val.as<double>(), K, Xr, Nr, dU, ldu, true);
+    }
+    DBIE_HIP(hipGetLastError());
+}
+
+// R (G x K, host, ldr) for densities on the device.
+void reaction_dev(dbie_tmap *tm, int G, const int64_t *grp_ptr, const double *q, int K, const double *dS, int64_t lds,
+                  double *R, int64_t ldr) {
+    make_resident(tm);
+    const TargetMap &h = *tm->h;
+    const int64_t Nr = h.ref.N, M = h.M, N = tm->N;
+    if (G < 1) fail(DBIE_BAD_ARG, "need at least one group");
+    if (!grp_ptr || !q || !R) fail(DBIE_BAD_ARG, "NULL group pointer, weights or output");
+    if (grp_ptr[0] != 0 || grp_ptr[G] != M) fail(DBIE_DIM_MISMATCH, "groups must cover targets 0..M-1");
+    for (int a = 0; a < G; ++a)
+        if (grp_ptr[a + 1] < grp_ptr[a]) fail(DBIE_BAD_ARG, "group pointers must be non-decreasing");
+    if (ldr < G) fail(DBIE_DIM_MISMATCH, "ldr smaller than the number of groups");
+    hipStream_t st = tm->stream;
+    // near-target part of the weights: Qcorr[r, a] = sum_{s in a} q_s C[s, r]  (host, sparse)
+    std::vector<std::vector<std::pair<int64_t, double>>> per(G);
+    parallel_for(G, 0, [&](int64_t a, int) {
+        std::vector<double> acc;
+        std::vector<int> touched;
+        std::vector<char> seen;
+        for (int64_t s = grp_ptr[a]; s < grp_ptr[a + 1]; ++s)
+            for (int64_t e = h.c_ptr[s]; e < h.c_ptr[s + 1]; ++e) {
+                if (acc.empty()) {
+                    acc.assign(Nr, 0.0);
+                    seen.assign(Nr, 0);
+                }
+                const int r = h.c_col[e];
+                if (!seen[r]) {
+                    seen[r] = 1;
+                    touched.push_back(r);
+                }
+                acc[r] += q[s] * h.c_val[e];
+            }
+        std::sort(touched.begin(), touched.end());
+        auto &o = per[a];
+        o.reserve(touched.size());
+        for (int r : touched) o.emplace_back((int64_t)r + a * Nr, acc[r]);
+    });
+    std::vector<int64_t> cidx;
+    std::vector<double> cval;
+    for (auto &o : per)
+        for (auto &e : o) {
+            cidx.push_back(e.first);
+            cval.push_back(e.second);
+        }
+    DBuf dgrp, dq, dQt, dQct, dR, dci, dcv;
+    upload(dgrp, grp_ptr, (size_t)G + 1, st);
+    upload(dq, q, (size_t)M, st);
+    upload(dci, cidx.data(), cidx.size(), st);
+    upload(dcv, cval.data(), cval.size(), st);
+    dQt.ensure(sizeof(double) * (size_t)Nr * G);
+    dQct.ensure(sizeof(double) * (size_t)N * G);
+    dR.ensure(sizeof(double) * (size_t)G * K);
+    reaction_q(st, tm->dg, G, dgrp.as<int64_t>(), tm->trg.as<double>(), dq.as<double>(), dQt.as<double>());
+    scatter_add(st, (int64_t)cidx.size(), dci.as<int64_t>(), dcv.as<double>(), dQt.as<double>());
+    csr_spmm(st, N, tm->pt_ptr.as<int64_t>(), tm->pt_col.as<int>(), tm->pt_val.as<double>(), G, dQt.as<double>(), Nr,
+             dQct.as<double>(), N, false);
+    DBIE_HIP(hipGetLastError());
+    if (!tm->blas) {
+        blas_check(rocblas_create_handle(&tm->blas), "rocblas_create_handle");
+        blas_check(rocblas_set_stream(tm->blas, st), "rocblas_set_stream");
+    }
+    const double one = 1.0, zero = 0.0;
+    blas_check(rocblas_dgemm(tm->blas, rocblas_operation_transpose, rocblas_operation_none, G, K, (rocblas_int)N, &one,
+                             dQct.as<double>(), N, dS, lds, &zero, dR.as<double>(), G),
+               "rocblas_dgemm");
+    DBIE_HIP(hipMemcpy2DAsync(R, sizeof(double) * ldr, dR.p, sizeof(double) * G, sizeof(double) * G, K,
+                              hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+extern "C" {
+
+dbie_status dbie_tmap_create(const dbie_geom *geom, int64_t M, const double *trg, double hcub_atol, double range_factor,
+                             int n_threads, dbie_tmap **out) {
+    return guard([&] {
+        if (!geom || !out) fail(DBIE_BAD_ARG, "NULL geometry or output");
+        *out = nullptr;
+        std::unique_ptr<dbie_tmap> tm(new dbie_tmap);
+        tm->h.reset(build_target_map(geom->g, M, trg, hcub_atol, range_factor, n_threads));
+        tm->N = geom->g.N;
+        *out = tm.release();
+    });
+}
+
+void dbie_tmap_destroy(dbie_tmap *tm) { delete tm; }
+
+dbie_status dbie_tmap_sizes(const dbie_tmap *tm, int64_t *M, int64_t *N, int64_t *N_refined, int64_t *n_pairs,
+                            int64_t *nnz_corr, int64_t *n_capped) {
+    return guard([&] {
+        check_tm(tm);
+        const TargetMap &h = *tm->h;
+        if (M) *M = h.M;
+        if (N) *N = tm->N;
+        if (N_refined) *N_refined = h.ref.N;
+        if (n_pairs) *n_pairs = h.n_pairs;
+        if (nnz_corr) *nnz_corr = (int64_t)h.c_val.size();
+        if (n_capped) *n_capped = h.n_capped;
+    });
+}
+
+dbie_status dbie_tmap_refined(const dbie_tmap *tm, double *xyzw, int *parent, int *from_split) {
+    return guard([&] {
+        check_tm(tm);
+        const TargetMap &h = *tm->h;
+        const Geometry &R = h.ref;
+        if (xyzw) {
+            std::memcpy(xyzw, R.x.data(), sizeof(double) * R.N);
+            std::memcpy(xyzw + R.N, R.y.data(), sizeof(double) * R.N);
+            std::memcpy(xyzw + 2 * R.N, R.z.data(), sizeof(double) * R.N);
+            std::memcpy(xyzw + 3 * R.N, R.w.data(), sizeof(double) * R.N);
+        }
+        if (parent) std::copy(h.parent.begin(), h.parent.end(), parent);
+        if (from_split) std::copy(h.from_split.begin(), h.from_split.end(), from_split);
+    });
+}
+
+dbie_status dbie_tmap_matrices(const dbie_tmap *tm, int64_t *p_ptr, int *p_col, double *p_val, int64_t *c_ptr,
+                               int *c_col, double *c_val) {
+    return guard([&] {
+        check_tm(tm);
+        const TargetMap &h = *tm->h;
+        if (p_ptr) std::copy(h.p_ptr.begin(), h.p_ptr.end(), p_ptr);
+        if (p_col) std::copy(h.p_col.begin(), h.p_col.end(), p_col);
+        if (p_val) std::copy(h.p_val.begin(), h.p_val.end(), p_val);
+        if (c_ptr) std::copy(h.c_ptr.begin(), h.c_ptr.end(), c_ptr);
+        if (c_col) std::copy(h.c_col.begin(), h.c_col.end(), c_col);
+        if (c_val) std::copy(h.c_val.begin(), h.c_val.end(), c_val);
+    });
+}
+
+dbie_status dbie_tmap_apply(dbie_tmap *tm, int K, const double *Sigma, int64_t lds, double *U, int64_t ldu) {
+    return guard([&] {
+        check_tm(tm);
+        const int64_t N = tm->N, M = tm->h->M;
+        if (K < 1 || !Sigma || (!U && M > 0)) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
+        if (lds < N || ldu < M) fail(DBIE_DIM_MISMATCH, "leading dimension too small");
+        make_resident(tm);
+        hipStream_t st = tm->stream;
+        tm->xin.ensure(sizeof(double) * (size_t)N * K);
+        tm->uout.ensure(sizeof(double) * (size_t)std::max<int64_t>(M, 1) * K);
+        DBIE_HIP(hipMemcpy2DAsync(tm->xin.p, sizeof(double) * N, Sigma, sizeof(double) * lds, sizeof(double) * N, K,
+                                  hipMemcpyHostToDevice, st));
+        tmap_apply_dev(tm, K, tm->xin.as<double>(), N, tm->uout.as<double>(), M);
+        if (M > 0)
+            DBIE_HIP(hipMemcpy2DAsync(U, sizeof(double) * ldu, tm->uout.p, sizeof(double) * M, sizeof(double) * M, K,
+                                      hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+    });
+}
+
+dbie_status dbie_tmap_apply_device(dbie_tmap *tm, int K, const double *dSigma, int64_t lds, double *dU, int64_t ldu) {
+    return guard([&] {
+        check_tm(tm);
+        if (K < 1 || !dSigma || !dU) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
+        if (lds < tm->N || ldu < tm->h->M) fail(DBIE_DIM_MISMATCH, "leading dimension too small");
+        tmap_apply_dev(tm, K, dSigma, lds, dU, ldu);
+        DBIE_HIP(hipStreamSynchronize(tm->stream));
+    });
+}
+
+dbie_status dbie_reaction_matrix(dbie_tmap *tm, int G, const int64_t *grp_ptr, const double *q, int K,
+                                 const double *Sigma, int64_t lds, double *R, int64_t ldr) {
+    return guard([&] {
+        check_tm(tm);
+        const int64_t N = tm->N;
+        if (K < 1 || !Sigma) fail(DBIE_BAD_ARG, "K must be >= 1 and Sigma non-NULL");
+        if (lds < N) fail(DBIE_DIM_MISMATCH, "lds smaller than N");
+        make_resident(tm);
+        hipStream_t st = tm->stream;
+        tm->xin.ensure(sizeof(double) * (size_t)N * K);
+        DBIE_HIP(hipMemcpy2DAsync(tm->xin.p, sizeof(double) * N, Sigma, sizeof(double) * lds, sizeof(double) * N, K,
+                                  hipMemcpyHostToDevice, st));
+        reaction_dev(tm, G, grp_ptr, q, K, tm->xin.as<double>(), N, R, ldr);
+    });
+}
+
+dbie_status dbie_reaction_matrix_device(dbie_tmap *tm, int G, const int64_t *grp_ptr, const double *q, int K,
+                                        const double *dSigma, int64_t lds, double *R, int64_t ldr) {
+    return guard([&] {
+        check_tm(tm);
+        if (K < 1 || !dSigma) fail(DBIE_BAD_ARG, "K must be >= 1 and Sigma non-NULL");
+        if (lds < tm->N) fail(DBIE_DIM_MISMATCH, "lds smaller than N");
+        reaction_dev(tm, G, grp_ptr, q, K, dSigma, lds, R, ldr);
+    });
+}
+
+dbie_status dbie_gauss_hermite(int n, double *x, double *w) {
+    return guard([&] {
+        if (!x || !w) fail(DBIE_BAD_ARG, "NULL output");
+        gauss_hermite(n, x, w);
+    });
+}
+
+}  // extern "C"

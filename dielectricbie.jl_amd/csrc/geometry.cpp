@@ -283,4 +283,67 @@ Geometry *make_from_panels(int64_t P, int nq, const double *corners, const doubl
     return g.release();
 }
 
+namespace {
+// psi_n(x) and psi_{n-1}(x) of the orthonormal Hermite functions
+// psi_k = H_k(x) exp(-x^2/2) / sqrt(2^k k! sqrt(pi)), by their three-term recurrence.
+void hermite_fn(int n, double x, double &pn, double &pn1) {
+    double prev = 0.0, cur = std::pow(3.14159265358979311599796346854, -0.25) * std::exp(-x * x / 2);
+    for (int k = 0; k < n; ++k) {
+        const double next = std::sqrt(2.0 / (k + 1)) * x * cur - std::sqrt((double)k / (k + 1)) * prev;
+        prev = cur;
+        cur = next;
+    }
+    pn = cur;
+    pn1 = prev;
+}
+}  // namespace
+
+// Gauss-Hermite nodes (roots of H_n, bracketed on a grid and bisected to adjacent doubles)
+// and weights w_i = exp(-x_i^2) / (n psi_{n-1}(x_i)^2); the rule is symmetrised.
+void gauss_hermite(int n, double *x, double *w) {
+    if (n < 1 || n > 200) fail(DBIE_BAD_ARG, "gauss_hermite: n must be in 1..200");
+    const double L = std::sqrt(2.0 * n + 1) + 1;
+    const int steps = 2000 * n;
+    int found = 0;
+    double xa = -L, fa, dummy;
+    hermite_fn(n, xa, fa, dummy);
+    for (int s = 1; s <= steps && found < n; ++s) {
+        const double xb = -L + 2 * L * s / steps;
+        double fb;
+        hermite_fn(n, xb, fb, dummy);
+        if (fb == 0.0) {
+            x[found++] = xb;
+        } else if (fa != 0.0 && (fa < 0) != (fb < 0)) {
+            double a = xa, b = xb, f_a = fa;
+            for (;;) {
+                const double m = 0.5 * (a + b);
+                if (m <= a || m >= b) break;
+                double fm;
+                hermite_fn(n, m, fm, dummy);
+                if (fm == 0.0) { a = b = m; break; }
+                if ((fm < 0) == (f_a < 0)) { a = m; f_a = fm; } else { b = m; }
+            }
+            x[found++] = 0.5 * (a + b);
+        }
+        xa = xb;
+        fa = fb;
+    }
+    if (found != n) fail(DBIE_BAD_ARG, "gauss_hermite: root bracketing failed");
+    for (int i = 0; i < n / 2; ++i) {
+        const double r = 0.5 * (x[n - 1 - i] - x[i]);
+        x[i] = -r;
+        x[n - 1 - i] = r;
+    }
+    if (n % 2) x[n / 2] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double pn, pn1;
+        hermite_fn(n, x[i], pn, pn1);
+        w[i] = std::exp(-x[i] * x[i]) / (n * pn1 * pn1);
+    }
+    for (int i = 0; i < n / 2; ++i) {
+        const double m = 0.5 * (w[i] + w[n - 1 - i]);
+        w[i] = w[n - 1 - i] = m;
+    }
+}
+
 }  // namespace dbie

@@ -2,7 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -19,6 +22,26 @@ struct Error {
 [[noreturn]] void fail(dbie_status code, const std::string &msg);
 void hip_check(hipError_t e, const char *what);
 #define DBIE_HIP(call) ::dbie::hip_check((call), #call)
+
+// Dynamic-schedule loop over [0, n) on n_threads host threads (0 = all cores);
+// body(i, thread_index).
+template <class F>
+void parallel_for(int64_t n, int n_threads, F &&body) {
+    int nt = n_threads > 0 ? n_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = (int)std::min<int64_t>(nt, std::max<int64_t>(n, 1));
+    std::atomic<int64_t> next{0};
+    auto worker = [&](int tid) {
+        for (;;) {
+            const int64_t i = next.fetch_add(1);
+            if (i >= n) break;
+            body(i, tid);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto &th : pool) th.join();
+}
 
 // ---- geometry --------------------------------------------------------------
 constexpr double kFourPi = 4.0 * 3.14159265358979311599796346854;
@@ -39,6 +62,8 @@ void gauss_legendre(int n, double *x, double *w);  // throws on n < 1
 bool best_grid(double W, double H, double alpha, int *m, int *n);
 Geometry *make_box(double Lx, double Ly, double Lz, int nq, double l_ec, double alpha);
 Geometry *make_from_panels(int64_t P, int nq, const double *corners, const double *normals, const int *is_edge);
+// physicists' Gauss-Hermite rule (weight exp(-x^2)), ascending nodes; throws on n < 1
+void gauss_hermite(int n, double *x, double *w);
 void barycentric_row(int n, const double *x, const double *lam, double xq, double *r);
 
 // ---- near field --------------------------------------------------------------
@@ -72,6 +97,28 @@ struct BlockSparse {
 };
 
 BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearOpts &o);
+
+// ---- off-surface targets ------------------------------------------------------
+// laplace3d_pottrg_fmm3d_corrected_hcubature (src/kernel/laplace3d_near_hcubature.jl:337-375):
+// the interface refined around the targets, the prolongation of densities onto it, and the
+// near-target corrections (adaptive cubature minus the direct quadrature term).
+struct TargetMap {
+    int64_t M = 0;
+    std::vector<double> trg;          // 3 x M
+    Geometry ref;                     // refined interface
+    std::vector<int> parent;          // refined panel -> coarse panel
+    std::vector<int> from_split;      // refined panel came from a split
+    std::vector<int64_t> p_ptr;       // prolongation, CSR: N_ref rows x N columns
+    std::vector<int> p_col;
+    std::vector<double> p_val;
+    std::vector<int64_t> c_ptr;       // corrections, CSR: M rows x N_ref columns (sorted columns)
+    std::vector<int> c_col;
+    std::vector<double> c_val;
+    int64_t n_pairs = 0;              // (panel, target) cubatures
+    int64_t n_capped = 0;             // cubatures that hit the evaluation cap
+};
+TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, double atol, double range_factor,
+                            int n_threads);
 
 }  // namespace dbie
 

@@ -96,4 +96,14 @@ void sub_to(hipStream_t st, const double *D, double *X, int64_t N, int K);
 // copy with leading dimensions
 void copy_cols(hipStream_t st, const double *src, int64_t lds, double *dst, int64_t ldd, int64_t N, int K);
 
+// ---- off-surface targets (targets.hip) ----
+// Y (R x K) = A X (+ Y if accumulate) for CSR A with R rows whose columns index rows of X.
+void csr_spmm(hipStream_t st, int64_t R, const int64_t *ptr, const int *col, const double *val, int K, const double *X,
+              int64_t ldx, double *Y, int64_t ldy, bool accumulate);
+// Qt (g.N x G, column-major) = w_r / (4 pi) sum_{s in group a} q_s / |t_s - y_r|; groups by grp_ptr (G + 1).
+void reaction_q(hipStream_t st, const DevGeom &g, int G, const int64_t *grp_ptr, const double *trg, const double *q,
+                double *Qt);
+// Y[idx[e]] += val[e] for n entries with distinct idx.
+void scatter_add(hipStream_t st, int64_t n, const int64_t *idx, const double *val, double *Y);
+
 }  // namespace dbie

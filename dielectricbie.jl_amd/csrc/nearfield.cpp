@@ -121,24 +121,6 @@ double bernstein_rho(const Frame &f, const double *t) {
     return std::min(rho1, rho2);
 }
 
-template <class F>
-void parallel_for(int64_t n, int n_threads, F &&body) {
-    int nt = n_threads > 0 ? n_threads : (int)std::max(1u, std::thread::hardware_concurrency());
-    nt = (int)std::min<int64_t>(nt, std::max<int64_t>(n, 1));
-    std::atomic<int64_t> next{0};
-    auto worker = [&](int tid) {
-        for (;;) {
-            const int64_t i = next.fetch_add(1);
-            if (i >= n) break;
-            body(i, tid);
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back(worker, t);
-    worker(0);
-    for (auto &th : pool) th.join();
-}
-
 inline int threads_of(int n) { return n > 0 ? n : (int)std::max(1u, std::thread::hardware_concurrency()); }
 
 }  // namespace

@@ -7,6 +7,8 @@ from ._lib import DbieError, NotConverged, lib, LIB_PATH  # noqa: F401
 from .api import (  # noqa: F401
     BatchedDielectricOperator,
     DeviceMatrix,
+    GaussianVolumeSource,
+    TargetPotentialMap,
     DielectricInterface,
     PointSource,
     SharpScreening,
@@ -15,16 +17,21 @@ from .api import (  # noqa: F401
     batched_lhs_dielectric_box3d_fmm3d_corrected,
     best_grid,
     best_grid_mn,
+    blob_lattice_centers,
     build_neighbor_list,
     dielectric_diagonal_scaling,
+    gaussian_blob_gh,
+    gausshermite,
     gausslegendre,
     interface_from_panels,
     lhs_dielectric_box3d,
     lhs_dielectric_box3d_fmm3d,
     lhs_dielectric_box3d_fmm3d_corrected,
+    laplace3d_pottrg_fmm3d_corrected_hcubature,
     num_points,
     rhs_dielectric_box3d,
     rhs_dielectric_box3d_fmm3d,
+    reaction_matrix,
     rhs_point_charges,
     screened_volume_source,
     single_dielectric_box3d,

@@ -95,9 +95,19 @@ SIGNATURES = {
     "dbie_memcpy_h2d": [_vp, _vp, C.c_size_t],
     "dbie_memcpy_d2h": [_vp, _vp, C.c_size_t],
     "dbie_version": [],
+    "dbie_tmap_create": [_vp, C.c_int64, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)],
+    "dbie_tmap_destroy": [_vp],
+    "dbie_tmap_sizes": [_vp, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p],
+    "dbie_tmap_refined": [_vp, _dp, _ip, _ip],
+    "dbie_tmap_matrices": [_vp, _i64p, _ip, _dp, _i64p, _ip, _dp],
+    "dbie_tmap_apply": [_vp, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
+    "dbie_tmap_apply_device": [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64],
+    "dbie_reaction_matrix": [_vp, C.c_int, _i64p, _dp, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
+    "dbie_reaction_matrix_device": [_vp, C.c_int, _i64p, _dp, C.c_int, _vp, C.c_int64, _dp, C.c_int64],
+    "dbie_gauss_hermite": [C.c_int, _dp, _dp],
 }
 _RESTYPES = {"dbie_version": C.c_char_p, "dbie_op_stream": _vp, "dbie_geom_destroy": None,
-             "dbie_operator_destroy": None, "dbie_near_opts_default": None, "dbie_solve_opts_default": None}
+             "dbie_operator_destroy": None, "dbie_tmap_destroy": None, "dbie_near_opts_default": None, "dbie_solve_opts_default": None}
 
 
 def lib():

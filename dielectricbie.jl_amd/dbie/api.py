@@ -576,3 +576,177 @@ def solve_dielectric_box3d_block(interface: DielectricInterface, vss, *, fmm_tol
     F = rhs_dielectric_box3d_fmm3d(interface, list(vss), fmm_tol)
     return op.solve(F, rtol=rtol, atol=atol, itmax=itmax, precondition=precondition, precision=precision,
                     inner_rtol=inner_rtol, max_refine=max_refine)
+
+
+# ---------------------------------------------------------------------------------------------
+# Off-surface targets, Gaussian charge blobs and the reaction matrix
+# ---------------------------------------------------------------------------------------------
+def gausshermite(n: int):
+    """Physicists' Gauss-Hermite rule (weight exp(-x^2)), ascending nodes."""
+    x = np.zeros(n); w = np.zeros(n)
+    L.check(L.lib().dbie_gauss_hermite(int(n), L.dptr(x), L.dptr(w)))
+    return x, w
+
+
+def GaussianVolumeSource(center, sigma: float, n: int, tol: float) -> VolumeSource:
+    """GaussianVolumeSource (src/core/sources.jl:517-548): unit-charge Gaussian of width sigma on an
+    n^3 midpoint grid spanning +-sqrt(2 sigma^2 log(1/tol)) about the centre."""
+    if n < 1 or not sigma > 0 or not (0 < tol < 1):
+        raise ValueError("need n >= 1, sigma > 0 and tol in (0, 1)")
+    two_s2 = 2.0 * sigma * sigma
+    support = math.sqrt(two_s2 * math.log(1.0 / tol))
+    norm = 1.0 / (math.sqrt(2.0 * math.pi) * sigma) ** 3
+    h = 2.0 * support / n
+    off0 = -support + h / 2.0
+    axes = [np.array([c + off0 + i * h for i in range(n)]) for c in center]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    r2 = (X - center[0]) ** 2 + (Y - center[1]) ** 2 + (Z - center[2]) ** 2
+    # point order ix outer, iz inner (_volume_source_positions, src/core/sources.jl:129-151)
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    w = np.full(n ** 3, h * h * h)
+    rho = (norm * np.exp(-r2 / two_s2)).ravel()
+    return VolumeSource.from_points(pts, w, rho)
+
+
+def gaussian_blob_gh(center, sigma: float, n: int = 6) -> VolumeSource:
+    """Unit-charge isotropic Gaussian blob exp(-|y-c|^2 / (2 sigma^2)) / (2 pi sigma^2)^(3/2) as an
+    n x n x n tensor Gauss-Hermite cloud: nodes c + sqrt(2) sigma (x_i, x_j, x_k), charges
+    q = pi^(-3/2) w_i w_j w_k normalised to sum 1, stored as weights q / rho(y) and density rho(y)
+    (so that sum(weights * density) = 1).  Point order: i (x) outer, then j, then k."""
+    x, w = gausshermite(n)
+    s = math.sqrt(2.0) * sigma
+    I, J, K = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    pts = np.stack([center[0] + s * x[I], center[1] + s * x[J], center[2] + s * x[K]], axis=1)
+    q = w[I] * w[J] * w[K]
+    q = q / q.sum()
+    r2 = (s * x[I]) ** 2 + (s * x[J]) ** 2 + (s * x[K]) ** 2
+    rho = np.exp(-r2 / (2.0 * sigma * sigma)) / (2.0 * math.pi * sigma * sigma) ** 1.5
+    return VolumeSource.from_points(pts, q / rho, rho)
+
+
+def blob_lattice_centers(Lx, Ly, Lz, nx=16, ny=8, nz=8):
+    """Centres of an nx x ny x nz lattice interior to the box: x_c = -Lx/2 + (i - 1/2) Lx/nx, ...;
+    order x outer, then y, then z."""
+    xs = [-Lx / 2 + (i + 0.5) * Lx / nx for i in range(nx)]
+    ys = [-Ly / 2 + (j + 0.5) * Ly / ny for j in range(ny)]
+    zs = [-Lz / 2 + (k + 0.5) * Lz / nz for k in range(nz)]
+    return np.array([(x, y, z) for x in xs for y in ys for z in zs])
+
+
+class TargetPotentialMap:
+    """The M x N map sigma -> u[sigma](targets) of laplace3d_pottrg_fmm3d_corrected_hcubature
+    (src/kernel/laplace3d_near_hcubature.jl:337-375): direct sum over the interface refined around
+    the targets plus the adaptive-cubature near-target corrections.  The host builds the
+    refinement and corrections; evaluation runs on the GPU."""
+
+    def __init__(self, interface: DielectricInterface, targets, hcubature_atol: float, range_factor: float = 5.0,
+                 n_threads: int = 0):
+        t = np.asarray(targets, np.float64)
+        if t.ndim == 2 and t.shape[0] == 3 and t.shape[1] != 3:
+            t = t.T
+        t = np.ascontiguousarray(t.reshape(-1, 3))
+        self.M, self.n = t.shape[0], interface.n_points
+        h = C.c_void_p()
+        L.check(L.lib().dbie_tmap_create(interface._h, self.M, L.dptr(t.ravel()), float(hcubature_atol),
+                                         float(range_factor), int(n_threads), C.byref(h)))
+        self._h = h
+        self._iface = interface
+        v = [C.c_int64() for _ in range(6)]
+        L.check(L.lib().dbie_tmap_sizes(self._h, *[C.byref(x) for x in v]))
+        _, _, self.n_refined, self.n_pairs, self.nnz, self.n_capped = [x.value for x in v]
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and L._lib is not None:
+                L._lib.dbie_tmap_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.M, self.n)
+
+    def refined(self):
+        """(nodes 4 x N_refined: x, y, z, w; parent panel and from_split per refined panel)."""
+        npp = self._iface.n_quad ** 2
+        Pr = self.n_refined // npp
+        xyzw = np.zeros(4 * self.n_refined); par = np.zeros(Pr, np.int32); spl = np.zeros(Pr, np.int32)
+        L.check(L.lib().dbie_tmap_refined(self._h, L.dptr(xyzw), L.iptr(par), L.iptr(spl)))
+        return xyzw.reshape(4, -1), par, spl.astype(bool)
+
+    def matrices(self):
+        """(prolongation, corrections) as CSR triples (ptr, col, val)."""
+        Nr, M = self.n_refined, self.M
+        pp = np.zeros(Nr + 1, np.int64)
+        L.check(L.lib().dbie_tmap_matrices(self._h, L.i64ptr(pp), None, None, None, None, None))
+        npn = int(pp[-1])
+        pc = np.zeros(max(npn, 1), np.int32); pv = np.zeros(max(npn, 1))
+        cp = np.zeros(M + 1, np.int64); cc = np.zeros(max(self.nnz, 1), np.int32); cv = np.zeros(max(self.nnz, 1))
+        L.check(L.lib().dbie_tmap_matrices(self._h, L.i64ptr(pp), L.iptr(pc), L.dptr(pv), L.i64ptr(cp), L.iptr(cc),
+                                           L.dptr(cv)))
+        return (pp, pc[:npn], pv[:npn]), (cp, cc[:self.nnz], cv[:self.nnz])
+
+    def apply(self, Sigma):
+        Sg = np.asarray(Sigma, np.float64)
+        vec = Sg.ndim == 1
+        Sf = L.fortran(Sg, self.n)
+        K = Sf.shape[1]
+        U = np.zeros((self.M, K), order="F")
+        L.check(L.lib().dbie_tmap_apply(self._h, K, L.dptr(Sf), self.n, L.dptr(U), max(self.M, 1)))
+        return U[:, 0] if vec else U
+
+    def __matmul__(self, Sigma):
+        return self.apply(Sigma)
+
+    def reaction(self, grp_ptr, q, Sigma):
+        """R[a,b] = sum_{s in group a} q_s u[Sigma_b](t_s) (G x K); Sigma on the host or a DeviceMatrix."""
+        grp = np.ascontiguousarray(grp_ptr, dtype=np.int64)
+        G = len(grp) - 1
+        qv = np.ascontiguousarray(q, dtype=np.float64)
+        if qv.shape != (self.M,):
+            raise ValueError(f"DimensionMismatch: q must have {self.M} entries")
+        if isinstance(Sigma, DeviceMatrix):
+            if Sigma.shape[0] != self.n:
+                raise ValueError("DimensionMismatch: Sigma rows")
+            K = Sigma.shape[1]
+            R = np.zeros((G, K), order="F")
+            L.check(L.lib().dbie_reaction_matrix_device(self._h, G, L.i64ptr(grp), L.dptr(qv), K, Sigma.ptr, self.n,
+                                                        L.dptr(R), G))
+            return R
+        Sf = L.fortran(Sigma, self.n)
+        K = Sf.shape[1]
+        R = np.zeros((G, K), order="F")
+        L.check(L.lib().dbie_reaction_matrix(self._h, G, L.i64ptr(grp), L.dptr(qv), K, L.dptr(Sf), self.n, L.dptr(R),
+                                             G))
+        return R
+
+
+def laplace3d_pottrg_fmm3d_corrected_hcubature(interface: DielectricInterface, targets, fmm_tol: float,
+                                               hcubature_atol: float, range_factor: float = 5.0):
+    """laplace3d_pottrg_fmm3d_corrected_hcubature (src/kernel/laplace3d_near_hcubature.jl:337-375).
+    fmm_tol is accepted and ignored: the base potential is the exact direct sum."""
+    return TargetPotentialMap(interface, targets, hcubature_atol, range_factor)
+
+
+def _sources_targets(sources):
+    pts = np.concatenate([np.asarray(vs.positions, np.float64).T for vs in sources])
+    q = np.concatenate([np.asarray(vs.weights, np.float64) * np.asarray(vs.density, np.float64) for vs in sources])
+    grp = np.zeros(len(sources) + 1, np.int64)
+    grp[1:] = np.cumsum([len(vs.weights) for vs in sources])
+    return pts, q, grp
+
+
+def reaction_matrix(interface: DielectricInterface, sources, Sigma, *, lhs_tol: float = 1e-9,
+                    range_factor: float = 5.0, tmap: TargetPotentialMap | None = None):
+    """Induced term of four_index_matrix (src/solver/multi_rhs.jl:299-321):
+    R[a,b] = sum_{s in source a} w_s rho_s u[Sigma_b](y_s), with u from the near-corrected
+    target map at tolerance lhs_tol.  Unlike the reference, the sources need not share one grid:
+    the targets are the union of all source points.  Returns (R, tmap)."""
+    pts, q, grp = _sources_targets(sources)
+    if tmap is None:
+        tmap = TargetPotentialMap(interface, pts, lhs_tol, range_factor)
+    elif tmap.M != len(q):
+        raise ValueError("DimensionMismatch: target map does not match the sources")
+    return tmap.reaction(grp, q, Sigma), tmap

@@ -189,10 +189,50 @@ dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, d
 dbie_status dbie_pottrg(dbie_op *op, int64_t M, const double *trg, int K, const double *Sigma, int64_t lds,
                         double *U, int64_t ldu);
 
+/* ---- near-corrected potential at off-surface targets -------------------- */
+typedef struct dbie_tmap dbie_tmap;  /* the M x N map Sigma -> u[Sigma](targets) */
+
+/* laplace3d_pottrg_fmm3d_corrected_hcubature(interface, targets, fmm_tol, hcubature_atol,
+   range_factor) (src/kernel/laplace3d_near_hcubature.jl:337-375): refines the interface around
+   the targets (src/kernel/laplace3d_near.jl:99-150), builds the prolongation of densities onto
+   it (:152-225) and the near-target corrections by adaptive cubature to hcub_atol for every
+   non-edge refined panel within range_factor * l / n_quad of a target (:47-70,
+   src/kernel/laplace3d_near_hcubature.jl:193-251).  Host-side build on n_threads threads
+   (0 = all cores); no device is needed until the first evaluation.  trg: 3 x M. */
+dbie_status dbie_tmap_create(const dbie_geom *g, int64_t M, const double *trg, double hcub_atol,
+                             double range_factor, int n_threads, dbie_tmap **out);
+void dbie_tmap_destroy(dbie_tmap *tm);
+/* sizes: targets, coarse nodes, refined nodes, cubature pairs, correction nonzeros, and
+   cubatures that stopped at the evaluation cap (2e6 integrand calls) before reaching hcub_atol */
+dbie_status dbie_tmap_sizes(const dbie_tmap *tm, int64_t *M, int64_t *N, int64_t *N_refined,
+                            int64_t *n_pairs, int64_t *nnz_corr, int64_t *n_capped);
+/* refined nodes as 4 x N_refined rows (x; y; z; w), parent panel and split flag per refined
+   panel (any pointer may be NULL) */
+dbie_status dbie_tmap_refined(const dbie_tmap *tm, double *xyzw, int *parent, int *from_split);
+/* prolongation (CSR, N_refined rows, coarse-node columns) and corrections (CSR, M rows,
+   refined-node columns); sizes from dbie_tmap_sizes (prolongation nnz = p_ptr[N_refined]) */
+dbie_status dbie_tmap_matrices(const dbie_tmap *tm, int64_t *p_ptr, int *p_col, double *p_val,
+                               int64_t *c_ptr, int *c_col, double *c_val);
+/* U (M x K) = u[Sigma] at the targets: the returned LinearMap applied to N x K densities */
+dbie_status dbie_tmap_apply(dbie_tmap *tm, int K, const double *Sigma, int64_t lds, double *U, int64_t ldu);
+dbie_status dbie_tmap_apply_device(dbie_tmap *tm, int K, const double *dSigma, int64_t lds, double *dU,
+                                   int64_t ldu);
+/* Reaction matrix, the induced term of four_index_matrix (src/solver/multi_rhs.jl:299-321):
+   R[a,b] = sum_{s in group a} q_s u[Sigma_b](trg_s), groups = target ranges
+   [grp_ptr[a], grp_ptr[a+1]) covering 0..M-1, q_s = w_s rho_s.  R: G x K (host, ldr). */
+dbie_status dbie_reaction_matrix(dbie_tmap *tm, int G, const int64_t *grp_ptr, const double *q, int K,
+                                 const double *Sigma, int64_t lds, double *R, int64_t ldr);
+dbie_status dbie_reaction_matrix_device(dbie_tmap *tm, int G, const int64_t *grp_ptr, const double *q, int K,
+                                        const double *dSigma, int64_t lds, double *R, int64_t ldr);
+/* physicists' Gauss-Hermite rule (weight exp(-x^2)), ascending nodes; used for Gaussian charge
+   blobs (no counterpart in the reference, whose GaussianVolumeSource uses a midpoint grid,
+   src/core/sources.jl:517-548) */
+dbie_status dbie_gauss_hermite(int n, double *x, double *w);
+
 /* ---- kernel timing (HIP events on the operator's stream) ----------------- */
 /* No counterpart in the reference (it has no profiler hooks); used by the benchmark to
    time the dominant kernels inside a timed region.  kernel ids: 0 = far field FP64,
-   1 = far field low precision (FP32/BF16/BF16X3), 2 = near-field combine.  Reading
+   1 = far field low precision (FP32/BF16/BF16X3/FP16X3), 2 = near-field combine.  Reading
    synchronizes the stream and clears the records. */
 dbie_status dbie_op_profile(dbie_op *op, int enable);
 dbie_status dbie_op_profile_read(dbie_op *op, int kernel, int64_t *launches, double *total_ms);

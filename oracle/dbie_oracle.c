@@ -1019,3 +1019,377 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------ */
+/* Off-surface targets: laplace3d_pottrg_fmm3d_corrected_hcubature      */
+/* (src/kernel/laplace3d_near_hcubature.jl:337-375)                      */
+/* ------------------------------------------------------------------ */
+/* HCubature.hcubature(f, [-1,-1], [1,1]; atol) (HCubature >= 1.8.0, not in the reference
+   tree) is restated from its published algorithm: the two-dimensional Genz-Malik degree-7
+   rule with its embedded degree-5 rule (Genz & Malik 1980); the box with the largest error
+   estimate is halved along the axis with the largest fourth divided difference
+   |f(+-l3) + 12 f(0) - 7 f(+-l2)| (first axis on ties) until the summed error is <= atol.
+   Boxes with equal errors are taken oldest first; the integral is re-summed over the final
+   boxes in creation order. */
+#define GM_MAXV 64
+typedef struct { double a[2], b[2], I[GM_MAXV], E; int kdiv; long seq; } gm_box;
+
+typedef struct {
+    const double *cc, *bma, *dma, *t;
+    double scale;
+    const orc_iface *f;
+} pot_integrand;
+
+/* _pot_integrand_hcubature (src/kernel/laplace3d_near_hcubature.jl:59-84) */
+static void pot_eval(const pot_integrand *pi, double u, double v, double *vals) {
+    int nq = pi->f->nq;
+    double rx[16], ry[16], p[3];
+    bary_row(nq, pi->f->gl_x, pi->f->bary, u, rx);
+    bary_row(nq, pi->f->gl_x, pi->f->bary, v, ry);
+    for (int k = 0; k < 3; ++k) p[k] = pi->cc[k] + pi->bma[k] * (u / 2) + pi->dma[k] * (v / 2);
+    double kv = k_pot(p, pi->t) * pi->scale;
+    for (int ii = 0; ii < nq; ++ii)
+        for (int jj = 0; jj < nq; ++jj) vals[ii * nq + jj] = kv * rx[ii] * ry[jj];
+}
+
+static void gm_rule(const pot_integrand *pi, int m, gm_box *bx) {
+    const double l2 = sqrt(9.0 / 70.0), l3 = sqrt(9.0 / 10.0), l4 = sqrt(9.0 / 10.0), l5 = sqrt(9.0 / 19.0);
+    const double w1 = 4.0 * (12824 - 9120 * 2 + 400 * 4) / 19683, w2 = 4.0 * 980 / 6561,
+                 w3 = 4.0 * (1820 - 400 * 2) / 19683, w4 = 4.0 * 200 / 19683, w5 = 6859.0 / 19683;
+    const double v1 = 4.0 * (729 - 950 * 2 + 50 * 4) / 729, v2 = 4.0 * 245 / 486, v3 = 4.0 * (265 - 100 * 2) / 1458,
+                 v4 = 4.0 * 25 / 729;
+    double c[2], h[2];
+    for (int k = 0; k < 2; ++k) { c[k] = 0.5 * (bx->a[k] + bx->b[k]); h[k] = 0.5 * fabs(bx->b[k] - bx->a[k]); }
+    double V = h[0] * h[1];
+    double f1[GM_MAXV], f2[GM_MAXV] = {0}, f3[GM_MAXV] = {0}, f4[GM_MAXV] = {0}, f5[GM_MAXV] = {0};
+    double ya[GM_MAXV], yb[GM_MAXV], s2[GM_MAXV], s3[GM_MAXV];
+    pot_eval(pi, c[0], c[1], f1);
+    double best = 0.0;
+    bx->kdiv = 0;
+    for (int ax = 0; ax < 2; ++ax) {
+        double d2[2] = {0, 0}, d3[2] = {0, 0};
+        d2[ax] = h[ax] * l2;
+        d3[ax] = h[ax] * l3;
+        pot_eval(pi, c[0] + d2[0], c[1] + d2[1], ya);
+        pot_eval(pi, c[0] - d2[0], c[1] - d2[1], yb);
+        for (int i = 0; i < m; ++i) s2[i] = ya[i] + yb[i];
+        pot_eval(pi, c[0] + d3[0], c[1] + d3[1], ya);
+        pot_eval(pi, c[0] - d3[0], c[1] - d3[1], yb);
+        for (int i = 0; i < m; ++i) s3[i] = ya[i] + yb[i];
+        double dd = 0.0;
+        for (int i = 0; i < m; ++i) {
+            f2[i] += s2[i];
+            f3[i] += s3[i];
+            double e = s3[i] + 12 * f1[i] - 7 * s2[i];
+            dd += e * e;
+        }
+        dd = sqrt(dd);
+        if (dd > best) { best = dd; bx->kdiv = ax; }
+    }
+    for (int sa = -1; sa <= 1; sa += 2)
+        for (int sb = -1; sb <= 1; sb += 2) {
+            pot_eval(pi, c[0] + sa * h[0] * l4, c[1] + sb * h[1] * l4, ya);
+            pot_eval(pi, c[0] + sa * h[0] * l5, c[1] + sb * h[1] * l5, yb);
+            for (int i = 0; i < m; ++i) { f4[i] += ya[i]; f5[i] += yb[i]; }
+        }
+    double e2 = 0.0;
+    for (int i = 0; i < m; ++i) {
+        bx->I[i] = V * (w1 * f1[i] + w2 * f2[i] + w3 * f3[i] + w4 * f4[i] + w5 * f5[i]);
+        double lo = V * (v1 * f1[i] + v2 * f2[i] + v3 * f3[i] + v4 * f4[i]);
+        e2 += (bx->I[i] - lo) * (bx->I[i] - lo);
+    }
+    bx->E = sqrt(e2);
+}
+
+/* returns 1 if atol was reached, 0 if the evaluation cap stopped it */
+static int hcub2(const pot_integrand *pi, int m, double atol, long maxevals, double *I) {
+    long cap = 64, n = 1, evals = 17, seq = 1;
+    gm_box *bx = (gm_box *)malloc(sizeof(gm_box) * cap);
+    bx[0].a[0] = bx[0].a[1] = -1; bx[0].b[0] = bx[0].b[1] = 1; bx[0].seq = 0;
+    gm_rule(pi, m, &bx[0]);
+    int ok;
+    if (bx[0].E <= atol || evals >= maxevals) {
+        memcpy(I, bx[0].I, sizeof(double) * m);
+        ok = bx[0].E <= atol;
+        free(bx);
+        return ok;
+    }
+    double It[GM_MAXV], Et = bx[0].E;
+    memcpy(It, bx[0].I, sizeof(double) * m);
+    for (;;) {
+        long k = 0;
+        for (long q = 1; q < n; ++q)
+            if (bx[q].E > bx[k].E || (bx[q].E == bx[k].E && bx[q].seq < bx[k].seq)) k = q;
+        gm_box old = bx[k];
+        if (n + 1 > cap) { cap *= 2; bx = (gm_box *)realloc(bx, sizeof(gm_box) * cap); }
+        int d = old.kdiv;
+        double mid = old.a[d] + (old.b[d] - old.a[d]) / 2;
+        gm_box c1 = old, c2 = old;
+        c1.b[d] = mid; c2.a[d] = mid;
+        c1.seq = seq++; c2.seq = seq++;
+        gm_rule(pi, m, &c1);
+        gm_rule(pi, m, &c2);
+        evals += 34;
+        for (int i = 0; i < m; ++i) It[i] += c1.I[i] + c2.I[i] - old.I[i];
+        Et += c1.E + c2.E - old.E;
+        bx[k] = bx[n - 1];
+        bx[n - 1] = c1;
+        bx[n++] = c2;
+        if (Et <= atol || evals >= maxevals) break;
+    }
+    /* re-sum in creation order */
+    for (int i = 0; i < m; ++i) I[i] = 0.0;
+    for (long s = 0; s < seq; ++s)
+        for (long q = 0; q < n; ++q)
+            if (bx[q].seq == s) { for (int i = 0; i < m; ++i) I[i] += bx[q].I[i]; break; }
+    ok = Et <= atol;
+    free(bx);
+    return ok;
+}
+
+typedef struct {
+    int M, N, Nr, nq;
+    double *trg;
+    orc_iface *ref;
+    int *parent, *from_split;
+    int *p_ptr, *p_col; double *p_val;     /* prolongation, Nr rows */
+    long *c_ptr; int *c_col; double *c_val; /* corrections, M rows */
+    long n_pairs, n_capped;
+} orc_tmap;
+
+static double panel_maxlen(const double *c) {
+    double l1 = norm3(c[0] - c[3], c[1] - c[4], c[2] - c[5]);
+    double l2 = norm3(c[3] - c[6], c[4] - c[7], c[5] - c[8]);
+    return l1 > l2 ? l1 : l2;
+}
+static void panel_ctr(const double *c, double *cc) {
+    for (int k = 0; k < 3; ++k) cc[k] = (c[k] + c[3 + k] + c[6 + k] + c[9 + k]) / 4;
+}
+/* inrange(tree, c, r) restated as a brute-force ball query |t - c|^2 <= r^2 */
+static int any_in_ball(int M, const double *trg, const double *c, double r) {
+    for (int t = 0; t < M; ++t) {
+        double dx = trg[3 * t] - c[0], dy = trg[3 * t + 1] - c[1], dz = trg[3 * t + 2] - c[2];
+        if (dx * dx + dy * dy + dz * dz <= r * r) return 1;
+    }
+    return 0;
+}
+
+typedef struct { int t, c; double v; } orc_trip;
+static int cmp_trip(const void *a, const void *b) {
+    const orc_trip *x = (const orc_trip *)a, *y = (const orc_trip *)b;
+    if (x->t != y->t) return x->t < y->t ? -1 : 1;
+    return (x->c > y->c) - (x->c < y->c);
+}
+
+orc_tmap *orc_tmap_build(const orc_iface *f, int M, const double *trg, double atol, double range_factor) {
+    int nq = f->nq, npp = nq * nq;
+    orc_tmap *tm = (orc_tmap *)calloc(1, sizeof(orc_tmap));
+    tm->M = M; tm->N = f->N; tm->nq = nq;
+    tm->trg = (double *)malloc(sizeof(double) * 3 * (size_t)(M > 0 ? M : 1));
+    memcpy(tm->trg, trg, sizeof(double) * 3 * (size_t)M);
+    /* _refine_interface_for_targets (src/kernel/laplace3d_near.jl:99-150): stack, pop! from the end */
+    double limit = DBL_MAX;
+    for (int p = 0; p < f->P; ++p) { double l = panel_maxlen(f->corners + 12 * p); if (l < limit) limit = l; }
+    long scap = 2 * (long)f->P + 64, sn = 0, ocap = 2 * (long)f->P + 64, on = 0;
+    double *sc = (double *)malloc(sizeof(double) * 12 * scap);
+    int *spar = (int *)malloc(sizeof(int) * scap), *ssp = (int *)malloc(sizeof(int) * scap);
+    double *oc = (double *)malloc(sizeof(double) * 12 * ocap);
+    int *opar = (int *)malloc(sizeof(int) * ocap), *osp = (int *)malloc(sizeof(int) * ocap);
+    for (int p = 0; p < f->P; ++p) {
+        memcpy(sc + 12 * sn, f->corners + 12 * p, sizeof(double) * 12);
+        spar[sn] = p; ssp[sn] = 0; ++sn;
+    }
+    while (sn > 0) {
+        --sn;
+        double c[12];
+        memcpy(c, sc + 12 * sn, sizeof(double) * 12);
+        int par = spar[sn], spl = ssp[sn];
+        double l = panel_maxlen(c), cc[3];
+        panel_ctr(c, cc);
+        double r = range_factor * l / nq;
+        if (any_in_ball(M, trg, cc, r) && l > limit) {
+            /* _split_panel4 (src/kernel/laplace3d_near.jl:78-97) */
+            const double *a = c, *b = c + 3, *cq = c + 6, *d = c + 9;
+            double ab[3], bc[3], cd[3], da[3], m[3];
+            for (int k = 0; k < 3; ++k) {
+                ab[k] = (a[k] + b[k]) / 2; bc[k] = (b[k] + cq[k]) / 2; cd[k] = (cq[k] + d[k]) / 2;
+                da[k] = (d[k] + a[k]) / 2; m[k] = (a[k] + b[k] + cq[k] + d[k]) / 4;
+            }
+            const double *ch[4][4] = {{a, ab, m, da}, {ab, b, bc, m}, {m, bc, cq, cd}, {da, m, cd, d}};
+            if (sn + 4 > scap) {
+                scap *= 2;
+                sc = (double *)realloc(sc, sizeof(double) * 12 * scap);
+                spar = (int *)realloc(spar, sizeof(int) * scap);
+                ssp = (int *)realloc(ssp, sizeof(int) * scap);
+            }
+            for (int q = 0; q < 4; ++q) {
+                for (int v = 0; v < 4; ++v) memcpy(sc + 12 * sn + 3 * v, ch[q][v], sizeof(double) * 3);
+                spar[sn] = par; ssp[sn] = 1; ++sn;
+            }
+        } else {
+            if (on == ocap) {
+                ocap *= 2;
+                oc = (double *)realloc(oc, sizeof(double) * 12 * ocap);
+                opar = (int *)realloc(opar, sizeof(int) * ocap);
+                osp = (int *)realloc(osp, sizeof(int) * ocap);
+            }
+            memcpy(oc + 12 * on, c, sizeof(double) * 12);
+            opar[on] = par; osp[on] = spl; ++on;
+        }
+    }
+    int Pr = (int)on;
+    double *nr = (double *)malloc(sizeof(double) * 3 * Pr), *ei = (double *)malloc(sizeof(double) * Pr);
+    int *ie = (int *)malloc(sizeof(int) * Pr);
+    for (int q = 0; q < Pr; ++q) {
+        memcpy(nr + 3 * q, f->normal + 3 * opar[q], sizeof(double) * 3);
+        ie[q] = f->is_edge[opar[q]];
+        ei[q] = 1.0;
+    }
+    tm->ref = orc_iface_from_panels(Pr, nq, oc, nr, ie, ei, ei);
+    tm->parent = opar; tm->from_split = osp; tm->Nr = tm->ref->N;
+    free(nr); free(ei); free(ie); free(sc); free(spar); free(ssp); free(oc);
+    const orc_iface *R = tm->ref;
+    /* _refined_interface_prolongation (src/kernel/laplace3d_near.jl:152-225) */
+    tm->p_ptr = (int *)malloc(sizeof(int) * ((size_t)tm->Nr + 1));
+    tm->p_col = (int *)malloc(sizeof(int) * (size_t)tm->Nr * npp);
+    tm->p_val = (double *)malloc(sizeof(double) * (size_t)tm->Nr * npp);
+    int e = 0;
+    tm->p_ptr[0] = 0;
+    for (int q = 0; q < Pr; ++q) {
+        int par = tm->parent[q], col0 = par * npp;
+        if (!tm->from_split[q]) {
+            for (int k = 0; k < npp; ++k) {
+                tm->p_col[e] = col0 + k; tm->p_val[e] = 1.0; ++e;
+                tm->p_ptr[q * npp + k + 1] = e;
+            }
+            continue;
+        }
+        const double *A = f->corners + 12 * par, *B = A + 3, *D = A + 9;
+        double cc[3], bma[3], dma[3];
+        panel_ctr(A, cc);
+        for (int k = 0; k < 3; ++k) { bma[k] = B[k] - A[k]; dma[k] = D[k] - A[k]; }
+        double bb = bma[0] * bma[0] + bma[1] * bma[1] + bma[2] * bma[2];
+        double bd = bma[0] * dma[0] + bma[1] * dma[1] + bma[2] * dma[2];
+        double dd = dma[0] * dma[0] + dma[1] * dma[1] + dma[2] * dma[2];
+        double det = bb * dd - bd * bd, i11 = dd / det, i12 = -bd / det, i22 = bb / det;
+        for (int k = 0; k < npp; ++k) {
+            const double *pt = R->pt + 3 * ((size_t)q * npp + k);
+            double x0 = pt[0] - cc[0], x1 = pt[1] - cc[1], x2 = pt[2] - cc[2];
+            double r1 = bma[0] * x0 + bma[1] * x1 + bma[2] * x2, r2 = dma[0] * x0 + dma[1] * x1 + dma[2] * x2;
+            double s = i11 * r1 + i12 * r2, t = i12 * r1 + i22 * r2;
+            double rx[16], ry[16];
+            bary_row(nq, f->gl_x, f->bary, 2 * s, rx);
+            bary_row(nq, f->gl_x, f->bary, 2 * t, ry);
+            for (int ii = 0; ii < nq; ++ii)
+                for (int jj = 0; jj < nq; ++jj) { tm->p_col[e] = col0 + ii * nq + jj; tm->p_val[e] = rx[ii] * ry[jj]; ++e; }
+            tm->p_ptr[q * npp + k + 1] = e;
+        }
+    }
+    /* build_target_neighbor_list (src/kernel/laplace3d_near.jl:47-70, edges excluded) and
+       laplace3d_pottrg_corrections_hcubature (src/kernel/laplace3d_near_hcubature.jl:193-251) */
+    long tcap = 1024, tn = 0;
+    orc_trip *tr = (orc_trip *)malloc(sizeof(orc_trip) * tcap);
+    for (int q = 0; q < Pr; ++q) {
+        if (R->is_edge[q]) continue;
+        const double *c = R->corners + 12 * q;
+        double cc[3];
+        panel_ctr(c, cc);
+        double r = range_factor * panel_maxlen(c) / nq;
+        pgeom pg;
+        panel_geom(R, q, &pg);
+        for (int t = 0; t < M; ++t) {
+            double dx = trg[3 * t] - cc[0], dy = trg[3 * t + 1] - cc[1], dz = trg[3 * t + 2] - cc[2];
+            if (!(dx * dx + dy * dy + dz * dz <= r * r)) continue;
+            pot_integrand pi = {pg.cc, pg.bma, pg.dma, trg + 3 * t, pg.scale, R};
+            double I[GM_MAXV];
+            if (!hcub2(&pi, npp, atol, 2000000L, I)) ++tm->n_capped;
+            ++tm->n_pairs;
+            for (int k = 0; k < npp; ++k) {
+                int n = q * npp + k;
+                double direct = R->wt[n] * k_pot(R->pt + 3 * n, trg + 3 * t);
+                double v = I[k] - direct;
+                if (v == 0.0) continue;
+                if (tn == tcap) { tcap *= 2; tr = (orc_trip *)realloc(tr, sizeof(orc_trip) * tcap); }
+                tr[tn].t = t; tr[tn].c = n; tr[tn].v = v; ++tn;
+            }
+        }
+    }
+    qsort(tr, (size_t)tn, sizeof(orc_trip), cmp_trip);
+    tm->c_ptr = (long *)calloc((size_t)M + 1, sizeof(long));
+    tm->c_col = (int *)malloc(sizeof(int) * (size_t)(tn > 0 ? tn : 1));
+    tm->c_val = (double *)malloc(sizeof(double) * (size_t)(tn > 0 ? tn : 1));
+    for (long q = 0; q < tn; ++q) { tm->c_ptr[tr[q].t + 1]++; tm->c_col[q] = tr[q].c; tm->c_val[q] = tr[q].v; }
+    for (int t = 0; t < M; ++t) tm->c_ptr[t + 1] += tm->c_ptr[t];
+    free(tr);
+    return tm;
+}
+
+void orc_tmap_sizes(const orc_tmap *tm, int *Nr, int *Pr, long *nnz_c, long *n_pairs, long *n_capped) {
+    *Nr = tm->Nr; *Pr = tm->ref->P; *nnz_c = tm->c_ptr[tm->M]; *n_pairs = tm->n_pairs; *n_capped = tm->n_capped;
+}
+/* refined nodes (x; y; z; w), parents, split flags, prolongation and corrections (CSR) */
+void orc_tmap_get(const orc_tmap *tm, double *xyzw, int *parent, int *from_split, int *p_ptr, int *p_col,
+                  double *p_val, long *c_ptr, int *c_col, double *c_val) {
+    int Nr = tm->Nr;
+    for (int i = 0; i < Nr; ++i) {
+        xyzw[i] = tm->ref->pt[3 * i]; xyzw[Nr + i] = tm->ref->pt[3 * i + 1];
+        xyzw[2 * Nr + i] = tm->ref->pt[3 * i + 2]; xyzw[3 * Nr + i] = tm->ref->wt[i];
+    }
+    memcpy(parent, tm->parent, sizeof(int) * tm->ref->P);
+    memcpy(from_split, tm->from_split, sizeof(int) * tm->ref->P);
+    memcpy(p_ptr, tm->p_ptr, sizeof(int) * ((size_t)Nr + 1));
+    memcpy(p_col, tm->p_col, sizeof(int) * (size_t)tm->p_ptr[Nr]);
+    memcpy(p_val, tm->p_val, sizeof(double) * (size_t)tm->p_ptr[Nr]);
+    memcpy(c_ptr, tm->c_ptr, sizeof(long) * ((size_t)tm->M + 1));
+    memcpy(c_col, tm->c_col, sizeof(int) * (size_t)tm->c_ptr[tm->M]);
+    memcpy(c_val, tm->c_val, sizeof(double) * (size_t)tm->c_ptr[tm->M]);
+}
+
+/* the LinearMap of laplace3d_pottrg_fmm3d_corrected_hcubature applied to S (N x K):
+   U = pot_base(P S) + C (P S), with the direct sum in place of the FMM */
+void orc_tmap_apply(const orc_tmap *tm, int K, const double *S, double *U) {
+    int Nr = tm->Nr, M = tm->M, N = tm->N;
+    double *Xr = (double *)malloc(sizeof(double) * (size_t)Nr * K);
+    for (int k = 0; k < K; ++k)
+        for (int r = 0; r < Nr; ++r) {
+            double acc = 0.0;
+            for (int e = tm->p_ptr[r]; e < tm->p_ptr[r + 1]; ++e) acc += tm->p_val[e] * S[(size_t)k * N + tm->p_col[e]];
+            Xr[(size_t)k * Nr + r] = acc;
+        }
+    orc_pottrg(tm->ref, M, tm->trg, K, Xr, U);
+    for (int k = 0; k < K; ++k)
+        for (int t = 0; t < M; ++t)
+            for (long e = tm->c_ptr[t]; e < tm->c_ptr[t + 1]; ++e)
+                U[(size_t)k * M + t] += tm->c_val[e] * Xr[(size_t)k * Nr + tm->c_col[e]];
+    free(Xr);
+}
+
+/* four_index_matrix (src/solver/multi_rhs.jl:299-321), induced term only:
+   R[a,b] = sum_{s in group a} q_s U[s,b] with U from orc_tmap_apply; R is G x K */
+void orc_reaction(const orc_tmap *tm, int G, const long *grp, const double *q, int K, const double *S, double *R) {
+    int M = tm->M;
+    double *U = (double *)malloc(sizeof(double) * (size_t)M * K);
+    orc_tmap_apply(tm, K, S, U);
+    for (int b = 0; b < K; ++b)
+        for (int a = 0; a < G; ++a) {
+            double acc = 0.0;
+            for (long s = grp[a]; s < grp[a + 1]; ++s) acc += q[s] * U[(size_t)b * M + s];
+            R[(size_t)b * G + a] = acc;
+        }
+    free(U);
+}
+
+void orc_tmap_free(orc_tmap *tm) {
+    if (!tm) return;
+    orc_iface_free(tm->ref);
+    free(tm->trg); free(tm->parent); free(tm->from_split); free(tm->p_ptr); free(tm->p_col); free(tm->p_val);
+    free(tm->c_ptr); free(tm->c_col); free(tm->c_val); free(tm);
+}
+
+/* cubature of the panel potential basis for one (panel, target) pair: I[npp] */
+int orc_pot_panel_hcub(const orc_iface *f, int p, const double *trg, double atol, double *I) {
+    pgeom pg;
+    panel_geom(f, p, &pg);
+    pot_integrand pi = {pg.cc, pg.bma, pg.dma, trg, pg.scale, f};
+    return hcub2(&pi, f->nq * f->nq, atol, 2000000L, I);
+}

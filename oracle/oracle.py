@@ -82,6 +82,17 @@ def lib():
         L.orc_rho_from_pole.restype = C.c_double
         L.orc_rho_from_pole.argtypes = [C.c_double, C.c_double]
         L.orc_num_threads.restype = C.c_int
+        _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.orc_tmap_build.restype = vp
+        L.orc_tmap_build.argtypes = [vp, C.c_int, _dp, C.c_double, C.c_double]
+        L.orc_tmap_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long),
+                                     C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.orc_tmap_get.argtypes = [vp, _dp, _ip, _ip, _ip, _ip, _dp, _lp, _ip, _dp]
+        L.orc_tmap_apply.argtypes = [vp, C.c_int, _dp, _dp]
+        L.orc_reaction.argtypes = [vp, C.c_int, _lp, _dp, C.c_int, _dp, _dp]
+        L.orc_tmap_free.argtypes = [vp]
+        L.orc_pot_panel_hcub.restype = C.c_int
+        L.orc_pot_panel_hcub.argtypes = [vp, C.c_int, _dp, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -378,3 +389,62 @@ def gmres(apply, b, rtol=1e-10, atol=0.0, itmax=500, M=None):
     if M is not None:
         x = x * M
     return x, k, res / beta
+
+
+class TargetMap:
+    """laplace3d_pottrg_fmm3d_corrected_hcubature restated on the CPU
+    (src/kernel/laplace3d_near_hcubature.jl:337-375), direct sum for the FMM."""
+
+    def __init__(self, iface: "Interface", targets, atol: float, range_factor: float = 5.0):
+        t = _f64(np.asarray(targets, np.float64).reshape(-1, 3))
+        self.M = t.shape[0]
+        self.N = iface.N
+        self._iface = iface
+        self.h = C.c_void_p(lib().orc_tmap_build(iface.h, self.M, t.ravel(), atol, range_factor))
+        Nr = C.c_int(); Pr = C.c_int(); nnz = C.c_long(); npairs = C.c_long(); ncap = C.c_long()
+        lib().orc_tmap_sizes(self.h, C.byref(Nr), C.byref(Pr), C.byref(nnz), C.byref(npairs), C.byref(ncap))
+        self.N_refined, self.P_refined = Nr.value, Pr.value
+        self.nnz, self.n_pairs, self.n_capped = nnz.value, npairs.value, ncap.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.orc_tmap_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def parts(self):
+        """(refined nodes 4 x Nr, parent, from_split, P as (ptr, col, val), C as (ptr, col, val))."""
+        Nr, Pr, M = self.N_refined, self.P_refined, self.M
+        npp = Nr // max(Pr, 1)
+        xyzw = np.zeros(4 * Nr); par = np.zeros(Pr, np.int32); spl = np.zeros(Pr, np.int32)
+        pp = np.zeros(Nr + 1, np.int32); pc = np.zeros(Nr * npp, np.int32); pv = np.zeros(Nr * npp)
+        cp = np.zeros(M + 1, np.int64); cc = np.zeros(max(self.nnz, 1), np.int32); cv = np.zeros(max(self.nnz, 1))
+        lib().orc_tmap_get(self.h, xyzw, par, spl, pp, pc, pv, cp, cc, cv)
+        n = int(pp[-1])
+        return (xyzw.reshape(4, Nr), par, spl, (pp, pc[:n], pv[:n]), (cp, cc[:self.nnz], cv[:self.nnz]))
+
+    def apply(self, S):
+        S = np.asarray(S, np.float64).reshape(self.N, -1)
+        K = S.shape[1]
+        U = np.zeros(self.M * K)
+        lib().orc_tmap_apply(self.h, K, _f64(S.T).ravel(), U)
+        return U.reshape(K, self.M).T
+
+    def reaction(self, grp_ptr, q, S):
+        S = np.asarray(S, np.float64).reshape(self.N, -1)
+        K = S.shape[1]
+        grp = np.ascontiguousarray(grp_ptr, dtype=np.int64)
+        G = len(grp) - 1
+        R = np.zeros(G * K)
+        lib().orc_reaction(self.h, G, grp, _f64(q), K, _f64(S.T).ravel(), R)
+        return R.reshape(K, G).T
+
+
+def pot_panel_hcub(iface: "Interface", p: int, target, atol: float):
+    """Adaptive-cubature potential of panel p against its n_quad^2 Lagrange basis at one target
+    (laplace3d_pot_panel_hcubature, src/kernel/laplace3d_near_hcubature.jl:86-150)."""
+    I = np.zeros(iface.nq ** 2)
+    ok = lib().orc_pot_panel_hcub(iface.h, p, _f64(target), atol, I)
+    return I, bool(ok)
