@@ -7,7 +7,8 @@ Corrected operator with the block-solve defaults (up_tol = 1e-9, max_order = 8,
 correct_edges = true), right diagonal preconditioner, rtol = 1e-10.
 
 A step is one block solve of all K right-hand sides, inputs resident in HBM:
-  * mixed:  FP64 iterative refinement around GMRES on the split-BF16 MFMA operator;
+  * mixed:  FP64 iterative refinement around GMRES on the split-FP16 MFMA operator
+            (near panel pairs taken exactly from the FP32 near blocks);
   * fp64:   the plain FP64 GMRES path (the comparator), timed beside it.
 value = K * steps / (time of the timed steps) for the mixed path, in RHS-solves/s.
 
@@ -68,13 +69,72 @@ def cpu_baseline(iters_fp64: int, ksample: int = 8):
                       f"FP64 path)"}
 
 
+def reaction_workload(a, dbie, log):
+    """Box 1.2 x 0.8 x 0.6, n_quad = 5, l_ec = 0.03 (N = 65,800), eps 4/1; K = 1024 unit Gaussian
+    blobs (width 0.02, 6^3 Gauss-Hermite points) on the 16 x 8 x 8 interior lattice; screened
+    volume RHS, mixed-precision and FP64 block solves, and the 1024 x 1024 reaction matrix from each
+    (near-corrected target potential at the 221,184 blob points, hcubature atol 1e-9).
+    A step is: RHS + mixed block solve + reaction matrix, with the operator and target map built."""
+    L = (1.2, 0.8, 0.6)
+    t0 = time.perf_counter()
+    iface = dbie.single_dielectric_box3d(*L, 5, 0.03, 4.0, 1.0)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(iface, 1e-9, 1e-9, 8)
+    t_op = time.perf_counter() - t0
+    C = dbie.blob_lattice_centers(*L)[: a.blobs]
+    srcs = [dbie.gaussian_blob_gh(c, 0.02, 6) for c in C]
+    K = len(srcs)
+    pts = np.concatenate([s.positions.T for s in srcs])
+    q = np.concatenate([s.weights * s.density for s in srcs])
+    grp = np.concatenate([[0], np.cumsum([len(s.weights) for s in srcs])]).astype(np.int64)
+    t1 = time.perf_counter()
+    tm = dbie.TargetPotentialMap(iface, pts, 1e-9)
+    t_tm = time.perf_counter() - t1
+    log(f"N={iface.n_points} K={K} M={len(q)} operator {t_op:.1f}s target map {t_tm:.1f}s "
+        f"(refined {tm.n_refined}, cubatures {tm.n_pairs}, capped {tm.n_capped})")
+    N = iface.n_points
+    dS = dbie.DeviceMatrix(N, K)
+
+    def step(precision):
+        ts = time.perf_counter()
+        F = dbie.rhs_dielectric_box3d_fmm3d(iface, srcs, 1e-9)
+        tr = time.perf_counter()
+        dF = dbie.DeviceMatrix.from_host(F)
+        st = op.solve_device(dF, dS, rtol=1e-10, precision=precision, inner_rtol=a.inner_rtol,
+                             check_residual=False)
+        tsv = time.perf_counter()
+        R = tm.reaction(grp, q, dS)
+        te = time.perf_counter()
+        return R, st, {"rhs_ms": (tr - ts) * 1e3, "solve_ms": (tsv - tr) * 1e3, "reaction_ms": (te - tsv) * 1e3,
+                       "total_ms": (te - ts) * 1e3}
+
+    R64, st64, t64 = step("fp64")
+    log(f"fp64: iters={st64['niter']} {t64}")
+    step(a.precision)  # warm-up
+    lat, out = [], None
+    for _ in range(max(a.steps, 1)):
+        out = step(a.precision)
+        lat.append(out[2]["total_ms"])
+    Rm, stm, tmix = out
+    err = float(np.abs(Rm - R64).max() / np.abs(R64).max())
+    log(f"{a.precision}: iters={stm['niter']} refine={stm['nrefine']} {tmix} max entry err {err:.2e}")
+    print(json.dumps({
+        "workload": "reaction", "metric": "RHS-solves/s (RHS + block solve + reaction matrix), mixed precision",
+        "value": K / (float(np.median(lat)) * 1e-3), "unit": "RHS-solves/s", "dtype": a.precision,
+        "fp64_value": K / (t64["total_ms"] * 1e-3), "config": {"N": N, "K": K, "M": int(len(q)),
+        "N_refined": tm.n_refined, "cubatures": tm.n_pairs},
+        "setup_s": {"operator": t_op, "target_map": t_tm}, "mixed_ms": tmix, "fp64_ms": t64,
+        "mixed_iters": stm["niter"], "mixed_refinements": stm["nrefine"], "fp64_iters": st64["niter"],
+        "max_entry_err_vs_fp64_rel_to_max": err, "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99))}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--K", type=int, default=256)
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16", "bf16x3", "fp16x3"])
+    ap.add_argument("--precision", default="fp16x3", choices=["fp32", "bf16", "bf16x3", "fp16x3"])
     ap.add_argument("--inner-rtol", type=float, default=1e-5)
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -82,6 +142,9 @@ def main():
     ap.add_argument("--mode", default="solve", choices=["solve", "apply"],
                     help="apply: time only operator applies (one per precision) for kernel profiling")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--blobs", type=int, default=1024, help="reaction workload: first B lattice blobs")
+    ap.add_argument("--workload", default="box", choices=["box", "reaction"],
+                    help="reaction: the volume-source reaction-matrix workload (1024 Gauss-Hermite blobs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; the path shards by RHS "
@@ -92,6 +155,10 @@ def main():
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if a.workload == "reaction":
+        reaction_workload(a, dbie, log)
+        return
 
     t0 = time.perf_counter()
     iface = dbie.single_dielectric_box3d(*BOX, NQ, L_EC, EPS_IN, EPS_OUT)

@@ -64,6 +64,10 @@ void ensure_device() {
 
 using namespace dbie;
 
+struct GmresWork {
+    DBuf V, W, Z, U, partial, hbuf, nbuf;
+};
+
 struct dbie_op {
     Geometry g;                  // host copy of the interface
     std::vector<double> diag_h;  // t per node
@@ -71,12 +75,13 @@ struct dbie_op {
     hipStream_t stream = nullptr;
     // resident device data
     DBuf x, y, z, nx, ny, nz, w, xf, yf, zf, nxf, nyf, nzf, wf, plane, diag;
-    DBuf bsr_row, bsr_col, bsr_v64, bsr_v32;
+    DBuf bsr_row, bsr_col, bsr_v64, bsr_v32, bsr_vup32, nmask;
     DevGeom dg;
     DevBSR bsr;
     int adaptive_hits = 0;
     // workspaces (grown on demand)
     DBuf part, xin, yout, tmp, mscratch, gmax;
+    GmresWork gw;                 // GMRES workspaces, reused across solves
     float f16_gscale = 0.0f;  // power-of-two scale of the fp16 kernel entries (0 = not yet computed)
     // optional per-kernel timing: event pairs recorded around launches
     bool profiling = false;
@@ -170,6 +175,7 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
     const int64_t need = far_field_part_elems(N, K);
     op->part.ensure(need * sizeof(double));
     int S;
+    bool exclude = false;  // far field leaves the near panel pairs out (their full interaction is added below)
     {
         ProfScope ps(op, prec == DBIE_FP64 ? 0 : 1);
         if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
@@ -179,8 +185,9 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
                 if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
             }
             op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
+            exclude = prec == DBIE_FP16X3 && op->dg.near_mask && op->bsr.valup32;
             S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p,
-                               op->f16_gscale);
+                               op->f16_gscale, exclude);
         } else if (prec == DBIE_FP64) {
             S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
         } else {
@@ -189,8 +196,8 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
     }
     {
         ProfScope ps(op, 2);
-        near_combine(op->stream, op->bsr, prec != DBIE_FP64, S, op->part.as<double>(), N, K, X, ldx,
-                     op->diag.as<double>(), Y, ldy);
+        const NearVals nv = exclude ? NEAR_UP32 : (prec != DBIE_FP64 ? NEAR_C32 : NEAR_C64);
+        near_combine(op->stream, op->bsr, nv, S, op->part.as<double>(), N, K, X, ldx, op->diag.as<double>(), Y, ldy);
     }
     DBIE_HIP(hipGetLastError());
 }
@@ -219,8 +226,10 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     const int64_t N = op->N();
     const size_t NK = (size_t)N * K;
     const int m = std::max(1, std::min(restart > 0 ? restart : 100, std::max(itmax, 1)));
-    DBuf V, W, Z, U, partial, hbuf, nbuf;
-    V.ensure(sizeof(double) * NK * (m + 1));
+    // workspaces live in the operator and are reused by later solves; the Krylov basis grows
+    // on demand (most solves stop long before the restart length)
+    GmresWork &ws = op->gw;
+    DBuf &W = ws.W, &Z = ws.Z, &U = ws.U, &partial = ws.partial, &hbuf = ws.hbuf, &nbuf = ws.nbuf;
     W.ensure(sizeof(double) * NK);
     U.ensure(sizeof(double) * NK);
     if (dinv) Z.ensure(sizeof(double) * NK);
@@ -228,7 +237,23 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     hbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
     nbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
     hipStream_t st = op->stream;
-    double *Vp = V.as<double>(), *Wp = W.as<double>();
+    double *Wp = W.as<double>();
+    // basis capacity in N x K blocks; growing copies the blocks already built
+    auto basis = [&](int blocks) -> double * {
+        blocks = std::min(blocks, m + 1);
+        if (ws.V.bytes >= sizeof(double) * NK * blocks) return ws.V.as<double>();
+        const int cap = std::min(m + 1, std::max(blocks, std::max(16, 2 * (int)(ws.V.bytes / (sizeof(double) * NK)))));
+        DBuf nv;
+        nv.ensure(sizeof(double) * NK * cap);
+        if (ws.V.p) {
+            DBIE_HIP(hipMemcpyAsync(nv.p, ws.V.p, ws.V.bytes, hipMemcpyDeviceToDevice, st));
+            DBIE_HIP(hipStreamSynchronize(st));
+        }
+        std::swap(ws.V.p, nv.p);
+        std::swap(ws.V.bytes, nv.bytes);
+        return ws.V.as<double>();
+    };
+    double *Vp = basis(2);
 
     DBIE_HIP(hipMemsetAsync(X, 0, sizeof(double) * NK, st));
     // r0 = B (x0 = 0) lives in W
@@ -262,6 +287,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         for (int k = 0; k < K; ++k) gv[(size_t)k * (m + 1)] = active[k] ? beta[k] : 0.0;
         int j = 0;
         for (; j < m && res.iters < itmax; ++j) {
+            Vp = basis(j + 2);
             const double *Vj = Vp + (size_t)j * NK;
             const double *Zin = Vj;
             if (dinv) {
@@ -669,6 +695,36 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
             bs.row_ptr.assign(g.P + 1, 0);
         }
         op->adaptive_hits = bs.adaptive_hits;
+        // full near interaction per block (correction + direct Dt term) for operators whose far
+        // field leaves the near panel pairs out, and the panel-pair mask they use
+        std::vector<double> vup(bs.val.size());
+        const int npp = g.npp(), bd = bs.bd;
+        parallel_for(g.P, o.n_threads, [&](int64_t p, int) {
+            for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b) {
+                const int64_t q = bs.col[b];
+                for (int t = 0; t < bd; ++t) {
+                    const int64_t i = p * npp + t;
+                    const double *n = &g.normal[3 * p];
+                    for (int c = 0; c < bd; ++c) {
+                        const int64_t j = q * npp + c;
+                        const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        double kd = 0.0;
+                        if (r2 > 0.0) {
+                            const double ir = 1.0 / std::sqrt(r2);
+                            kd = (n[0] * dx + n[1] * dy + n[2] * dz) * (ir * ir * ir) / kFourPi * g.w[j];
+                        }
+                        const size_t at = (size_t)b * bd * bd + (size_t)t * bd + c;
+                        vup[at] = bs.val[at] + kd;
+                    }
+                }
+            }
+        });
+        const int mwords = (int)((g.P + 31) / 32) + 2;
+        std::vector<uint32_t> mask((size_t)g.P * mwords, 0u);
+        for (int64_t p = 0; p < g.P; ++p)
+            for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b)
+                mask[(size_t)p * mwords + bs.col[b] / 32] |= 1u << (bs.col[b] % 32);
         const bool mfma_order = bs.bd == 16;
         if (mfma_order) {  // permute each block into MFMA operand order
             std::vector<double> tmpb(256);
@@ -677,6 +733,10 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
                 for (int t = 0; t < 16; ++t)
                     for (int c = 0; c < 16; ++c) tmpb[bsr_storage_index(16, true, t, c)] = blk[t * 16 + c];
                 std::copy(tmpb.begin(), tmpb.end(), blk);
+                double *bu = &vup[(size_t)b * 256];
+                for (int t = 0; t < 16; ++t)
+                    for (int c = 0; c < 16; ++c) tmpb[bsr_storage_index(16, true, t, c)] = bu[t * 16 + c];
+                std::copy(tmpb.begin(), tmpb.end(), bu);
             }
         }
         upload(op->bsr_row, bs.row_ptr.data(), bs.row_ptr.size(), st);
@@ -685,6 +745,9 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         {
             const std::vector<float> v32 = f32(bs.val);
             upload(op->bsr_v32, v32.data(), v32.size(), st);
+            const std::vector<float> u32 = f32(vup);
+            upload(op->bsr_vup32, u32.data(), u32.size(), st);
+            upload(op->nmask, mask.data(), mask.size(), st);
             DBIE_HIP(hipStreamSynchronize(st));
         }
         compute_diag(op.get(), eps_in, eps_out);
@@ -697,6 +760,9 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         d.nxf = op->nxf.as<float>(); d.nyf = op->nyf.as<float>(); d.nzf = op->nzf.as<float>();
         d.wf = op->wf.as<float>();
         d.tile_plane = op->plane.as<int>();
+        d.npp = npp;
+        d.mask_words = mwords;
+        d.near_mask = bs.nblocks() > 0 ? op->nmask.as<uint32_t>() : nullptr;
         DevBSR &c = op->bsr;
         c.P = (int)g.P;
         c.bd = bs.bd;
@@ -705,6 +771,7 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         c.col = op->bsr_col.as<int>();
         c.val64 = op->bsr_v64.as<double>();
         c.val32 = op->bsr_v32.as<float>();
+        c.valup32 = bs.nblocks() > 0 ? op->bsr_vup32.as<float>() : nullptr;
         c.mfma_order = mfma_order;
         DBIE_HIP(hipStreamSynchronize(st));
         *out = op.release();
@@ -762,8 +829,8 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
         int64_t b = 0;
         for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
                               &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
-                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->part, &op->xin, &op->yout, &op->tmp,
-                              &op->mscratch})
+                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->nmask, &op->part,
+                              &op->xin, &op->yout, &op->tmp, &op->mscratch})
             b += (int64_t)d->bytes;
         *bytes = b;
     });
@@ -852,7 +919,7 @@ dbie_status dbie_apply_kernel(dbie_op *op, dbie_kernel kernel, int K, const doub
         DBuf zrow;
         upload(zrow, zeros.data(), zeros.size(), st);
         empty.row_ptr = zrow.as<int>();
-        near_combine(st, empty, false, S, op->part.as<double>(), N, K, op->xin.as<double>(), N, op->tmp.as<double>(),
+        near_combine(st, empty, NEAR_C64, S, op->part.as<double>(), N, K, op->xin.as<double>(), N, op->tmp.as<double>(),
                      op->yout.as<double>(), N);
         DBIE_HIP(hipGetLastError());
         DBIE_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldy, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,

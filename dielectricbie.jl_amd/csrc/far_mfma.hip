@@ -388,9 +388,8 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
 
 // ---------------------------------------------------------------- FP16 x3
 // Two-term fp16 split of both operands (11-bit mantissas, ~22 bits together) with
-// hi*hi + hi*lo + lo*hi products accumulated in FP32: about 2e-7 per entry.  Coordinate
-// differences are formed in FP64 before rounding so that close node pairs keep their
-// accuracy.  Range: entries are scaled by 2^-eG (|G| <= 1/r_min^2 <= 2^14 after scaling)
+// hi*hi + hi*lo + lo*hi products accumulated in FP32: about 2e-7 per entry.  Coordinates are
+// taken relative to the tile's first target in FP64 and then rounded to FP32.  Range: entries are scaled by 2^-eG (|G| <= 1/r_min^2 <= 2^14 after scaling)
 // and each column of w.*X by 2^e_k (max |.| in [2^13, 2^14)); both are undone on output.
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef _Float16 v2h __attribute__((ext_vector_type(2)));
@@ -437,15 +436,29 @@ __global__ void split_tiles_f16(const double *__restrict__ X, int64_t ldx, const
     zl[idx] = l;
 }
 
-template <int NCB>
+// Near-pair exclusion (EXCL > 0): pairs of panels (target panel, source panel) flagged in
+// nmask (one bit per pair, rows of mwords 32-bit words) are left out of the sum; the caller adds
+// their accurate interaction (upsampled/adaptive blocks) in the near-field kernel.  The low
+// precision then never meets the large direct entries of close pairs, whose cancellation
+// against the corrections would amplify its rounding.  EXCL = 1: npp = 16 (a lane's 8 sources in
+// a k-slice lie in one panel); EXCL = 2: any npp (per-source test).
+struct NearMask {
+    const uint32_t *bits = nullptr;
+    int npp = 0;
+    int mwords = 0;
+};
+
+template <int NCB, int EXCL>
 __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int K, int Kp, const _Float16 *__restrict__ zh,
                                                                   const _Float16 *__restrict__ zl,
                                                                   const unsigned *__restrict__ cmax, float gscale,
-                                                                  double *__restrict__ part, int src_per_split) {
+                                                                  double *__restrict__ part, int src_per_split,
+                                                                  NearMask nm) {
     using I = Items<NCB>;
     constexpr int KS = kSb / 32;
     __shared__ uint4 s_g[3][2][kWaves][KS][64];  // [stage % 3][hi/lo][slice][k-slice][lane]
-    __shared__ double4 s_pos[2][kSb];
+    __shared__ float4 s_pos[2][kSb];  // source coordinates relative to the tile origin
+    __shared__ int s_rel[EXCL == 2 ? 2 : 1][kSb];  // source panel - first panel of the stage
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t N = g.N;
     const int64_t i0 = (int64_t)blockIdx.x * kTargets;
@@ -455,8 +468,24 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
 
     int64_t tg = i0 + wave * 16 + (lane & 15);
     tg = tg < N ? tg : N - 1;
-    const double tx = g.x[tg], ty = g.y[tg], tz = g.z[tg];
-    const float tn0 = g.nxf[tg], tn1 = g.nyf[tg], tn2 = g.nzf[tg];
+    // coordinates relative to the first target of the tile, rounded to FP32 after the FP64
+    // subtraction: the difference of two such values carries an error of about
+    // eps_32 (|d| + 2 * tile extent), small against |d| for pairs outside the near list
+    const int64_t io = i0 < N ? i0 : N - 1;
+    const double ox = g.x[io], oy = g.y[io], oz = g.z[io];
+    const float tx = (float)(g.x[tg] - ox), ty = (float)(g.y[tg] - oy), tz = (float)(g.z[tg] - oz);
+    const float tn0 = g.nxf[tg] * gscale, tn1 = g.nyf[tg] * gscale, tn2 = g.nzf[tg] * gscale;
+    const uint32_t *mrow = EXCL ? nm.bits + (tg / nm.npp) * (int64_t)nm.mwords : nullptr;
+    // 64 mask bits starting at the first source panel of the stage at js
+    auto load_win = [&](int64_t js) -> uint64_t {
+        if (!EXCL || js >= walk.jend) return 0;
+        const int64_t sp0 = js / nm.npp;
+        const int64_t w0 = sp0 >> 5;
+        const int sh = (int)(sp0 & 31);
+        const uint64_t lo = (uint64_t)mrow[w0] | ((uint64_t)mrow[w0 + 1] << 32);
+        const uint64_t hi = mrow[w0 + 2];
+        return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+    };
 
     v4f acc[I::kN];
 #pragma unroll
@@ -477,19 +506,24 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
     auto load_pos = [&](int64_t jb, int pb) {
         if (tid < kSb) {
             const int64_t j = jb + tid;
-            s_pos[pb][tid] = j < walk.jend ? make_double4(g.x[j], g.y[j], g.z[j], 0.0)
-                                           : make_double4(1.0e18, 1.0e18, 1.0e18, 0.0);
+            const bool in = j < walk.jend;
+            s_pos[pb][tid] = in ? make_float4((float)(g.x[j] - ox), (float)(g.y[j] - oy), (float)(g.z[j] - oz), 0.f)
+                                : make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.f);
+            if (EXCL == 2) s_rel[pb][tid] = in ? (int)(j / nm.npp - jb / nm.npp) : 0;
         }
     };
-    auto gen2 = [&](int pb, int ks, int e, unsigned &hi, unsigned &lo) {
+    // excl: bit mask of this lane's excluded sources in k-slice ks of the stage in buffer pb
+    auto gen2 = [&](int pb, int ks, int e, uint64_t win, float n0, float n1, float n2, unsigned &hi, unsigned &lo) {
         float v[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            const double4 sp = s_pos[pb][ks * 32 + 8 * (lane >> 4) + e + u];
-            const float dx = (float)(tx - sp.x), dy = (float)(ty - sp.y), dz = (float)(tz - sp.z);
-            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+            const int sidx = ks * 32 + 8 * (lane >> 4) + e + u;
+            const float4 sp = s_pos[pb][sidx];
+            const float dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
+            float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+            if (EXCL == 2) r2 = ((win >> (s_rel[pb][sidx] & 63)) & 1) ? 1.0e30f : r2;
             const float rinv = __builtin_amdgcn_rsqf(r2);
-            v[u] = ((tn0 * dx + tn1 * dy + tn2 * dz) * gscale) * (rinv * rinv * rinv);
+            v[u] = (n0 * dx + n1 * dy + n2 * dz) * (rinv * rinv * rinv);
         }
         const v2h h = __builtin_convertvector(v2f{v[0], v[1]}, v2h);
         const v2f hf = __builtin_convertvector(h, v2f);
@@ -497,28 +531,39 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
         hi = __builtin_bit_cast(unsigned, h);
         lo = __builtin_bit_cast(unsigned, l);
     };
-    auto gen_slice = [&](int pb, int ks, uint4 &h4, uint4 &l4) {
+    auto gen_slice = [&](int pb, int ks, uint64_t win, uint4 &h4, uint4 &l4) {
         unsigned h[4], l[4];
+        float n0 = tn0, n1 = tn1, n2 = tn2;
+        if (EXCL == 1) {  // npp = 16: the 8 sources lie in panel 2 ks + (lane >> 4) / 2 of the stage;
+                          // a zero normal makes their entries exactly zero
+            const bool ex = (win >> (2 * ks + ((lane >> 4) >> 1))) & 1;
+            n0 = ex ? 0.f : n0;
+            n1 = ex ? 0.f : n1;
+            n2 = ex ? 0.f : n2;
+        }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) gen2(pb, ks, 2 * e, h[e], l[e]);
+        for (int e = 0; e < 4; ++e) gen2(pb, ks, 2 * e, win, n0, n1, n2, h[e], l[e]);
         h4 = uint4{h[0], h[1], h[2], h[3]};
         l4 = uint4{l[0], l[1], l[2], l[3]};
     };
 
     int64_t jc = walk.next(jb0);
     int64_t j1 = walk.jend;
+    uint64_t win_nxt = 0;
     if (jc < walk.jend) {
         j1 = walk.next(jc + kSb);
         load_pos(jc, 0);
+        const uint64_t win0 = load_win(jc);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             uint4 h4, l4;
-            gen_slice(0, ks, h4, l4);
+            gen_slice(0, ks, win0, h4, l4);
             s_g[0][0][wave][ks][lane] = h4;
             s_g[0][1][wave][ks][lane] = l4;
         }
         if (j1 < walk.jend) load_pos(j1, 1);
+        win_nxt = load_win(j1);
         load_b(jc, bh_cur, bl_cur);
         __syncthreads();
     }
@@ -527,10 +572,12 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
         const bool more = j1 < walk.jend;
         const int64_t j2 = more ? walk.next(j1 + kSb) : walk.jend;
         load_b(more ? j1 : jc, bh_nxt, bl_nxt);
+        const uint64_t win = win_nxt;
+        win_nxt = load_win(j2);
         uint4 gh[KS], gl[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            gen_slice(pb, ks, gh[ks], gl[ks]);
+            gen_slice(pb, ks, win, gh[ks], gl[ks]);
 #pragma unroll
             for (int s = 0; s < I::kSlicesPerWave; ++s) {
                 const int sl = I::slice(wave, s);
@@ -607,6 +654,7 @@ int mfma_splits(int64_t N, int K, int64_t groups, int64_t part_cap, int64_t *per
     *per_out = per;
     return (int)((N + per - 1) / per);
 }
+
 // max over all node pairs of |n_i . (x_i - y_j)| / r^3, in the arithmetic of the fp16 tile
 // generator (FP64 differences rounded to FP32), as float bits.  grid: (target tiles, source splits)
 __global__ __launch_bounds__(256) void gmax_kernel(DevGeom g, unsigned *__restrict__ out) {
@@ -670,7 +718,7 @@ size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K) {
 }
 
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap, void *scratch, float gscale) {
+                   int64_t part_cap, void *scratch, float gscale, bool exclude_near) {
     const int64_t N = g.N;
     const int64_t bx = (N + kTargets - 1) / kTargets;
     const int ncb = ncb_for(K);
@@ -702,9 +750,17 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         colmax_kernel<<<dim3(64, (unsigned)K), 256, 0, st>>>(X, ldx, g.w, N, cmax);
         split_tiles_f16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
                                                                                                     cmax, zh, zl);
-        switch (ncb) {
-#define DBIE_H3(n) case n: far_mfma_f16x3_kernel<n><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per); break;
-            DBIE_H3(1) DBIE_H3(2) DBIE_H3(4) DBIE_H3(8) DBIE_H3(16)
+        NearMask nm;
+        nm.bits = g.near_mask;
+        nm.npp = g.npp;
+        nm.mwords = g.mask_words;
+        const int ex = (!exclude_near || !g.near_mask) ? 0 : (g.npp == 16 ? 1 : 2);
+        switch (ncb * 4 + ex) {
+#define DBIE_H3(n, e) \
+    case n * 4 + e: far_mfma_f16x3_kernel<n, e><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm); break;
+#define DBIE_H3E(n) DBIE_H3(n, 0) DBIE_H3(n, 1) DBIE_H3(n, 2)
+            DBIE_H3E(1) DBIE_H3E(2) DBIE_H3E(4) DBIE_H3E(8) DBIE_H3E(16)
+#undef DBIE_H3E
 #undef DBIE_H3
         }
         return S;

@@ -17,6 +17,11 @@ struct DevGeom {
     const float *wf = nullptr;
     // face structure for skipping same-plane tiles: plane id per 64-node tile (-1 = mixed)
     const int *tile_plane = nullptr;
+    // near panel pairs: bit (p, q) at near_mask[p * mask_words + q / 32] (target p, source q);
+    // rows padded with two zero words.  npp = nodes per panel.
+    const uint32_t *near_mask = nullptr;
+    int npp = 0;
+    int mask_words = 0;
 };
 
 // Block-sparse near correction on the device.
@@ -26,6 +31,7 @@ struct DevBSR {
     const int *row_ptr = nullptr, *col = nullptr;
     const double *val64 = nullptr;  // nb x bd x bd blocks, element (t, c) at bsr_storage_index(bd, mfma_order, t, c)
     const float *val32 = nullptr;
+    const float *valup32 = nullptr; // nb blocks of the full near interaction (correction + direct), FP32
     bool mfma_order = false;        // bd = 16: blocks stored in MFMA operand order
 };
 // Storage position of block element (t, c): MFMA operand order for bd = 16
@@ -44,9 +50,10 @@ int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int6
 // RHS block.  prec: 0 = FP64 MFMA, 2 = BF16 MFMA, 3 = split BF16 (3 products), 4 = split FP16.
 // scratch: far_field_mfma_scratch_bytes(N, prec, K) bytes of device memory (the
 // pre-scaled RHS block w.*X in FP64, or its tile-major bf16 split).
-// gscale: power-of-two scale of the kernel entries for the fp16 form (prec 4)
+// gscale: power-of-two scale of the kernel entries for the fp16 form (prec 4).  exclude_near
+// (prec 4 only): leave out the panel pairs of g.near_mask, whose full interaction the caller adds.
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap, void *scratch, float gscale);
+                   int64_t part_cap, void *scratch, float gscale, bool exclude_near);
 size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K);
 // Power-of-two scale for the fp16 kernel entries: 2^(14 - e) with max |G_ij| < 2^e over all
 // node pairs.  tmp4: 4 bytes of device memory.  Synchronises the stream; 0 on a HIP error.
@@ -55,7 +62,8 @@ bool far_field_mfma_supported(int prec, int K);
 int64_t far_field_part_elems(int64_t N, int K);  // workspace needed for any far-field launch
 
 // Y[:,k] = sum_s part[s][k] + dscale*diag.*X[:,k] + C X[:,k] (C in FP64 or FP32 values).
-void near_combine(hipStream_t st, const DevBSR &C, bool use_f32, int S, const double *part, int64_t N, int K,
+enum NearVals { NEAR_C64 = 0, NEAR_C32 = 1, NEAR_UP32 = 2 };
+void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
                   const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy);
 
 // Right-hand sides.

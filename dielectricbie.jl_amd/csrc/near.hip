@@ -2,7 +2,9 @@
 // reduction and the diagonal contrast term (mul! of BatchedDielectricOperator,
 // src/solver/multi_rhs.jl:103-130):
 //     Y[i,k] = sum_s part[s][k][i] + t_i X[i,k] + sum_b C_b[i - bd*p, :] . X[bd*src_b + :, k]
-// Block rows are target panels; each block is a dense bd x bd (bd = n_quad^2) tile.
+// Block rows are target panels; each block is a dense bd x bd (bd = n_quad^2) tile.  C_b is the
+// correction block (FP64 or FP32), or, for the split-FP16 operator whose far field leaves the
+// near panel pairs out, the full near interaction (correction + direct term, FP32).
 //
 // Two kernels:
 //  * K >= 16 and bd = 16: FP64 MFMA (v_mfma_f64_16x16x4_f64).  One workgroup per block
@@ -68,8 +70,9 @@ __device__ inline int xcd_row(int w, int P) {
     return r;
 }
 
+template <typename VT>
 __global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__restrict__ row_ptr,
-                                                        const int *__restrict__ col, const double *__restrict__ val,
+                                                        const int *__restrict__ col, const VT *__restrict__ val,
                                                         int S, const double *__restrict__ part, int64_t N, int K,
                                                         const double *__restrict__ X, int64_t ldx,
                                                         const double *__restrict__ diag, double *__restrict__ Y,
@@ -95,11 +98,11 @@ __global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__rest
     // operands of one block: A (4 k-slices) and X (4 k-slices x 4 column blocks), prefetched one block ahead
     double an[4], xn[4][4];
     auto load = [&](int b, double (&a)[4], double (&xv)[4][4]) {
-        const double *v = val + (int64_t)b * 256;
+        const VT *v = val + (int64_t)b * 256;
         const int64_t src_row = (int64_t)col[b] * 16 + (lane >> 4);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            a[kk] = v[kk * 64 + lane];  // C_b[t = lane%16][c = kk*4 + lane/16]
+            a[kk] = (double)v[kk * 64 + lane];  // C_b[t = lane%16][c = kk*4 + lane/16]
 #pragma unroll
             for (int q = 0; q < 4; ++q) xv[kk][q] = X[src_row + kk * 4 + coff[q]];
         }
@@ -140,13 +143,19 @@ __global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__rest
 }
 }  // namespace
 
-void near_combine(hipStream_t st, const DevBSR &C, bool use_f32, int S, const double *part, int64_t N, int K,
+void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
                   const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy) {
     const int bd = C.bd;
+    const float *v32 = vals == NEAR_UP32 ? C.valup32 : (vals == NEAR_C32 ? C.val32 : nullptr);
     if (C.mfma_order && bd == 16 && K >= 16) {
         const int per = (C.P + 7) / 8;
         dim3 grid((unsigned)(8 * per), (unsigned)((K + 255) / 256));
-        near_mfma_kernel<<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.val64, S, part, N, K, X, ldx, diag, Y, ldy);
+        if (vals == NEAR_UP32)
+            near_mfma_kernel<float><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.valup32, S, part, N, K, X, ldx, diag,
+                                                          Y, ldy);
+        else
+            near_mfma_kernel<double><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.val64, S, part, N, K, X, ldx, diag,
+                                                           Y, ldy);
         return;
     }
     const int KT = std::max(1, 256 / bd);
@@ -154,8 +163,8 @@ void near_combine(hipStream_t st, const DevBSR &C, bool use_f32, int S, const do
     const int RP = std::max(1, KT / KTe);
     dim3 grid((unsigned)((C.P + RP - 1) / RP), (unsigned)((K + KTe - 1) / KTe));
     const int mo = C.mfma_order ? 1 : 0;
-    if (use_f32 && C.val32)
-        near_combine_kernel<float><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, C.val32, S, part, N, K, KTe, RP,
+    if (v32)
+        near_combine_kernel<float><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, v32, S, part, N, K, KTe, RP,
                                                           X, ldx, diag, Y, ldy);
     else
         near_combine_kernel<double><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, C.val64, S, part, N, K, KTe,
