@@ -128,6 +128,52 @@ def reaction_workload(a, dbie, log):
         "p99_ms": float(np.percentile(lat, 99))}), flush=True)
 
 
+def serve_workload(a, dbie, log):
+    """Seeded Poisson stream of single-box requests (sides ~U(0.5, 2), l_ec in {0.1, 0.07, 0.05},
+    n_quad 4, eps_in ~U(2, 80), 1-64 charges) through the request server in mixed precision;
+    latency = arrival -> result on the host.  Afterwards each request is solved again in FP64 through
+    the operator API to state the error of the served result."""
+    reqs = dbie.serving_requests(a.requests, seed=a.seed, rate=a.rate)
+    infos, results = [], []
+    with dbie.Server(solve_slots=a.slots) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for r in reqs:
+            while time.perf_counter() - t0 < r["t"]:
+                time.sleep(0.0005)
+            ids.append(srv.submit_box(*r["L"], r["n_quad"], r["l_ec"], r["eps_in"], r["eps_out"], r["points"],
+                                      r["charges"], precision=a.precision, inner_rtol=a.inner_rtol))
+        for i in ids:
+            infos.append(srv.wait(i))
+            results.append(srv.result(i) if infos[-1]["status"] == 0 else None)
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+    ok = [i for i in infos if i["status"] == 0]
+    K_total = sum(i["K"] for i in ok)
+    lat = np.array([i["total_ms"] for i in ok])
+    log(f"{len(ok)}/{len(infos)} ok, wall {wall:.1f}s, stats {stats}")
+    # error of each served result against an FP64 solve of the same request
+    errs = []
+    for r, S in zip(reqs, results):
+        if S is None:
+            continue
+        itf = dbie.single_dielectric_box3d(*r["L"], r["n_quad"], r["l_ec"], r["eps_in"], r["eps_out"])
+        op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(itf, 1e-9, 1e-9, 8)
+        F = dbie.rhs_point_charges(itf, r["points"], r["charges"], r["eps_in"])
+        S64, _ = op.solve(F, rtol=1e-10)
+        errs.append(float(np.max(np.linalg.norm(S - S64, axis=0) / np.linalg.norm(S64, axis=0))))
+    print(json.dumps({
+        "workload": "serve", "metric": "served RHS-solves/s (arrival -> result), mixed precision",
+        "value": K_total / wall, "unit": "RHS-solves/s", "dtype": a.precision,
+        "requests": len(infos), "completed": len(ok), "rhs_total": K_total, "wall_s": wall,
+        "arrival_rate": a.rate, "slots": a.slots, "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)), "mean_setup_ms": float(np.mean([i["setup_ms"] for i in ok])),
+        "mean_solve_ms": float(np.mean([i["solve_ms"] for i in ok])),
+        "mean_queue_ms": float(np.mean([i["queue_ms"] for i in ok])),
+        "N_range": [int(min(i["N"] for i in ok)), int(max(i["N"] for i in ok))],
+        "max_rel_err_vs_fp64": max(errs) if errs else None, "server": stats}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,8 +189,12 @@ def main():
                     help="apply: time only operator applies (one per precision) for kernel profiling")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--blobs", type=int, default=1024, help="reaction workload: first B lattice blobs")
-    ap.add_argument("--workload", default="box", choices=["box", "reaction"],
-                    help="reaction: the volume-source reaction-matrix workload (1024 Gauss-Hermite blobs)")
+    ap.add_argument("--workload", default="box", choices=["box", "reaction", "serve"],
+                    help="reaction: the volume-source reaction-matrix workload (1024 Gauss-Hermite blobs); "
+                         "serve: a seeded Poisson stream of box requests through the request server")
+    ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=2, help="serve: concurrent solve slots")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; the path shards by RHS "
@@ -158,6 +208,9 @@ def main():
 
     if a.workload == "reaction":
         reaction_workload(a, dbie, log)
+        return
+    if a.workload == "serve":
+        serve_workload(a, dbie, log)
         return
 
     t0 = time.perf_counter()

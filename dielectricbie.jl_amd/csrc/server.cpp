@@ -1,0 +1,590 @@
+// server.cpp -- continuous-batching request loop for many block solves on one GPU.
+//
+// No counterpart in the reference (a Julia library whose callers loop over
+// solve_dielectric_box3d_block, src/solver/multi_rhs.jl:270-286).  A request is a single
+// dielectric box (single_dielectric_box3d, src/shape/box3d_single.jl:49-70) with K point charges
+// (rhs_dielectric_box3d, src/solver/dielectric_box3d.jl:95-104); its answer is the N x K density.
+//
+//   arrivals --> [queue] --admission (HBM budget)--> setup workers --> [ready] --> solve slots
+//                                                  (mesh, near field,   (fuse ready requests of one
+//                                                   RHS, on host cores)   operator, block solve on the
+//                                                                         operator's own stream)
+//
+// * Admission: a request enters setup only if the device bytes of the cached operators, the
+//   in-flight reservations and its own estimate (operator + Krylov workspace) fit the budget;
+//   idle cached operators are evicted (least recently used) to make room.  A request that does
+//   not fit even alone fails with DBIE_OOM; the rest keep running.  Admission is first in,
+//   first out: the head of the queue waits for memory rather than being overtaken.
+// * Operator cache: keyed by geometry and permittivities; a hit skips the host build.
+// * Continuous batching: a free slot takes the oldest ready request and every other ready request
+//   on the same operator with the same solve options, up to max_batch_rhs columns, and solves
+//   them as one block (each column converges on its own).  Requests that arrive while others
+//   are on the GPU are set up on the host meanwhile and start as slots free up; the slots run
+//   on separate streams, so their device work overlaps.
+// * Failures (allocation, non-convergence, bad arguments) are per request and reported in its
+//   info; a cancelled request is dropped from the queue or its result discarded.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+
+#include "internal.hpp"
+#include "runtime.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0, Clock::time_point t1) {
+    return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+std::string last_error_text() {
+    char buf[1024];
+    dbie_last_error(buf, sizeof buf);
+    return buf;
+}
+
+using GeoKey = std::tuple<double, double, double, int, double, double, double>;
+
+struct OpEntry {
+    GeoKey key;
+    dbie_op *op = nullptr;
+    int64_t N = 0;
+    size_t bytes = 0;         // device bytes (estimate until built, then measured)
+    bool building = true;
+    bool failed = false;
+    bool busy = false;        // a slot is solving on it
+    int users = 0;            // requests bound to it (setup .. done)
+    uint64_t last_use = 0;
+    ~OpEntry() {
+        if (op) dbie_operator_destroy(op);
+    }
+};
+
+enum class St { Queued, Setup, Ready, Solving, Done, Failed, Cancelled };
+
+struct Request {
+    int64_t id = 0;
+    double L[3] = {0, 0, 0};
+    int nq = 4;
+    double l_ec = 0.1, eps_in = 2, eps_out = 1;
+    int K = 0;
+    std::vector<double> pts, q;
+    dbie_solve_opts sopts;
+    St st = St::Queued;
+    dbie_status status = DBIE_OK;
+    std::string msg;
+    std::shared_ptr<OpEntry> ent;
+    std::vector<double> F, sigma;
+    int64_t N = 0;
+    dbie_solve_stats stats{};
+    size_t pending_ws = 0;     // admission estimates: Krylov workspace, operator
+    size_t pending_op = 0;
+    size_t reserved = 0;       // workspace bytes held while in flight
+    size_t op_reserved = 0;    // operator estimate held while it is being built
+    bool cache_hit = false;
+    bool cancel = false;
+    int batch_K = 0;
+    Clock::time_point t_arrive, t_setup0, t_setup1, t_solve0, t_solve1;
+};
+
+// Device bytes of an operator before it exists: node streams, plane tiles, near blocks (FP64 +
+// two FP32 copies + indices, about 45 blocks of n_quad^4 per panel), and the far-field scratch.
+size_t estimate_op_bytes(int64_t N, int nq) {
+    const int64_t npp = (int64_t)nq * nq;
+    const int64_t P = N / std::max<int64_t>(npp, 1);
+    const double blocks = 50.0 * P;
+    return (size_t)(N * 8 * 24 + blocks * npp * npp * 16.0 + blocks * 8 + 64.0 * 1024 * 1024);
+}
+// Krylov basis (about 30 blocks), work vectors and the far-field partial sums for K columns.
+size_t estimate_ws_bytes(int64_t N, int K) { return (size_t)N * K * 8 * 40 + (size_t)N * K * 8 * 6; }
+
+}  // namespace
+
+struct dbie_server {
+    dbie_server_opts o;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool stop = false;
+    int64_t next_id = 1;
+    uint64_t tick = 0;
+    std::map<int64_t, std::shared_ptr<Request>> reqs;  // every request not yet taken
+    std::deque<std::shared_ptr<Request>> queue;        // waiting for admission (FIFO)
+    std::deque<std::shared_ptr<Request>> ready;        // set up, waiting for a slot
+    std::map<GeoKey, std::shared_ptr<OpEntry>> cache;
+    size_t budget = 0, inflight = 0, peak = 0;
+    int64_t builds = 0, hits = 0, done = 0, failed = 0, batches = 0, fused = 0;
+    std::vector<std::thread> threads;
+
+    size_t cached_bytes() const {
+        size_t b = 0;
+        for (auto &kv : cache) b += kv.second->bytes;
+        return b;
+    }
+    size_t in_use() const { return cached_bytes() + inflight; }
+    void note_peak() { peak = std::max(peak, in_use()); }
+    // drop idle cached operators (oldest first) until `need` more bytes fit
+    bool make_room(size_t need) {
+        while (in_use() + need > budget) {
+            std::shared_ptr<OpEntry> victim;
+            for (auto &kv : cache) {
+                auto &e = kv.second;
+                if (e->users == 0 && !e->building && !e->busy && (!victim || e->last_use < victim->last_use)) victim = e;
+            }
+            if (!victim) return false;
+            cache.erase(victim->key);
+        }
+        return true;
+    }
+    void finish(const std::shared_ptr<Request> &r, St st, dbie_status status, const std::string &msg) {
+        r->st = st;
+        r->status = status;
+        r->msg = msg;
+        r->t_solve1 = Clock::now();
+        inflight -= r->reserved + r->op_reserved;
+        r->reserved = r->op_reserved = 0;
+        if (r->ent) {
+            r->ent->users--;
+            if (r->ent->failed && r->ent->users == 0) cache.erase(r->ent->key);
+            else if (!o.cache_ops && r->ent->users == 0 && !r->ent->busy) cache.erase(r->ent->key);
+            r->ent.reset();
+        }
+        if (st == St::Done) ++done;
+        else ++failed;
+        r->F.clear();
+        r->F.shrink_to_fit();
+        cv.notify_all();
+    }
+
+    // ---------------------------------------------------------------- setup worker
+    void setup_loop() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv.wait(lk, [&] { return stop || pick_head(); });
+            if (stop) return;
+            const std::shared_ptr<Request> r = queue.front();
+            queue.pop_front();
+            // admission: bind or create the operator entry and reserve memory
+            const GeoKey key{r->L[0], r->L[1], r->L[2], r->nq, r->l_ec, r->eps_in, r->eps_out};
+            auto it = cache.find(key);
+            std::shared_ptr<OpEntry> ent;
+            bool build = false;
+            if (it != cache.end()) {
+                ent = it->second;
+                r->cache_hit = true;
+                ++hits;
+            } else {
+                ent = std::make_shared<OpEntry>();
+                ent->key = key;
+                ent->bytes = 0;
+                cache[key] = ent;
+                build = true;
+            }
+            ent->users++;
+            ent->last_use = ++tick;
+            r->ent = ent;
+            r->reserved = r->pending_ws;
+            r->op_reserved = build ? r->pending_op : 0;
+            inflight += r->reserved + r->op_reserved;
+            note_peak();
+            r->st = St::Setup;
+            r->t_setup0 = Clock::now();
+            lk.unlock();
+            dbie_status s = DBIE_OK;
+            std::string msg;
+            if (build) {
+                s = build_op(*r, *ent, msg);
+            }
+            lk.lock();
+            if (!build) {  // wait for a concurrent build of the same operator
+                cv.wait(lk, [&] { return stop || !ent->building; });
+                if (stop) return;
+                if (ent->failed) {
+                    s = DBIE_OOM;
+                    msg = "operator build failed for this geometry";
+                }
+            } else {
+                ent->building = false;
+                inflight -= r->op_reserved;
+                r->op_reserved = 0;
+                if (s != DBIE_OK) ent->failed = true;
+                cv.notify_all();
+            }
+            if (s == DBIE_OK) {
+                r->N = ent->N;
+                lk.unlock();
+                s = make_rhs(*r, *ent, msg);
+                lk.lock();
+            }
+            r->t_setup1 = Clock::now();
+            if (s != DBIE_OK) {
+                finish(r, St::Failed, s, msg);
+                continue;
+            }
+            if (r->cancel) {
+                finish(r, St::Cancelled, DBIE_OK, "cancelled");
+                continue;
+            }
+            r->st = St::Ready;
+            ready.push_back(r);
+            cv.notify_all();
+        }
+    }
+    // True when the head of the queue can be admitted now (called with the lock held).  Cancelled
+    // heads and heads that can never fit the budget are finished here.
+    bool pick_head() {
+        while (!queue.empty()) {
+            const std::shared_ptr<Request> r = queue.front();
+            if (r->cancel) {
+                queue.pop_front();
+                finish(r, St::Cancelled, DBIE_OK, "cancelled");
+                continue;
+            }
+            const GeoKey key{r->L[0], r->L[1], r->L[2], r->nq, r->l_ec, r->eps_in, r->eps_out};
+            const size_t need = r->pending_ws + (cache.count(key) ? 0 : r->pending_op);
+            if (need > budget) {
+                queue.pop_front();
+                finish(r, St::Failed, DBIE_OOM,
+                       "request needs about " + std::to_string(need) + " device bytes; budget " + std::to_string(budget));
+                continue;
+            }
+            return make_room(need);
+        }
+        return false;
+    }
+    dbie_status build_op(Request &r, OpEntry &ent, std::string &msg) {
+        if (o.inject_oom_every > 0 && (++build_count % o.inject_oom_every) == 0) {
+            msg = "injected allocation failure (test hook)";
+            return DBIE_OOM;
+        }
+        dbie_geom *g = nullptr;
+        dbie_status s = dbie_geometry_box(r.L[0], r.L[1], r.L[2], r.nq, r.l_ec, std::sqrt(2.0), &g);
+        if (s != DBIE_OK) {
+            msg = last_error_text();
+            return s;
+        }
+        int64_t P = 0, N = 0;
+        int nq = 0;
+        dbie_geom_sizes(g, &P, &N, &nq);
+        std::vector<double> ei(P, r.eps_in), eo(P, r.eps_out);
+        dbie_near_opts no;
+        dbie_near_opts_default(&no);
+        no.n_threads = o.setup_threads;
+        dbie_op *op = nullptr;
+        s = dbie_operator_create(g, ei.data(), eo.data(), &no, &op);
+        if (s != DBIE_OK) msg = last_error_text();
+        dbie_geom_destroy(g);
+        if (s != DBIE_OK) return s;
+        int64_t b = 0;
+        dbie_op_device_bytes(op, &b);
+        std::lock_guard<std::mutex> lk(mu);
+        ent.op = op;
+        ent.N = N;
+        ent.bytes = (size_t)b;
+        ++builds;
+        return DBIE_OK;
+    }
+    dbie_status make_rhs(Request &r, OpEntry &ent, std::string &msg) {
+        r.F.assign((size_t)ent.N * r.K, 0.0);
+        // the RHS kernel runs on the operator's stream; a slot may be solving on it, so take the
+        // entry's turn like a solve would
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || !ent.busy; });
+        if (stop) return DBIE_HIP_ERROR;
+        ent.busy = true;
+        lk.unlock();
+        const dbie_status s = dbie_rhs_points(ent.op, r.K, r.pts.data(), r.q.data(), r.eps_in, r.F.data(), ent.N);
+        if (s != DBIE_OK) msg = last_error_text();
+        lk.lock();
+        ent.busy = false;
+        cv.notify_all();
+        return s;
+    }
+
+    // ---------------------------------------------------------------- solve slot
+    static bool same_opts(const dbie_solve_opts &a, const dbie_solve_opts &b) {
+        return a.rtol == b.rtol && a.atol == b.atol && a.itmax == b.itmax && a.restart == b.restart &&
+               a.precondition == b.precondition && a.precision == b.precision && a.inner_rtol == b.inner_rtol &&
+               a.max_refine == b.max_refine;
+    }
+    void slot_loop() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            std::vector<std::shared_ptr<Request>> batch;
+            cv.wait(lk, [&] {
+                if (stop) return true;
+                for (auto &r : ready)
+                    if (!r->ent->busy) return true;
+                return false;
+            });
+            if (stop) return;
+            // oldest ready request whose operator is free, plus its companions
+            auto head = ready.end();
+            for (auto it = ready.begin(); it != ready.end(); ++it)
+                if (!(*it)->ent->busy) {
+                    head = it;
+                    break;
+                }
+            const std::shared_ptr<Request> h = *head;
+            int Ktot = 0;
+            for (auto it = ready.begin(); it != ready.end();) {
+                const auto &r = *it;
+                if (r->ent == h->ent && same_opts(r->sopts, h->sopts) &&
+                    (batch.empty() || Ktot + r->K <= o.max_batch_rhs)) {
+                    batch.push_back(r);
+                    Ktot += r->K;
+                    it = ready.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            std::shared_ptr<OpEntry> ent = h->ent;
+            ent->busy = true;
+            ent->last_use = ++tick;
+            ++batches;
+            fused += (int64_t)batch.size() - 1;
+            const auto t0 = Clock::now();
+            for (auto &r : batch) {
+                r->st = St::Solving;
+                r->t_solve0 = t0;
+                r->batch_K = Ktot;
+            }
+            lk.unlock();
+            const int64_t N = ent->N;
+            std::vector<double> F((size_t)N * Ktot), S((size_t)N * Ktot);
+            int64_t col = 0;
+            for (auto &r : batch) {
+                std::memcpy(&F[(size_t)col * N], r->F.data(), sizeof(double) * N * r->K);
+                col += r->K;
+            }
+            dbie_solve_stats st{};
+            dbie_status s = dbie_solve_block(ent->op, Ktot, F.data(), N, S.data(), N, &h->sopts, &st);
+            std::string msg = s == DBIE_OK ? "" : last_error_text();
+            lk.lock();
+            col = 0;
+            for (auto &r : batch) {
+                r->stats = st;
+                if (s == DBIE_OK || s == DBIE_NOT_CONVERGED) {
+                    r->sigma.assign(S.begin() + (size_t)col * N, S.begin() + (size_t)(col + r->K) * N);
+                }
+                col += r->K;
+                if (r->cancel) finish(r, St::Cancelled, DBIE_OK, "cancelled");
+                else if (s == DBIE_OK) finish(r, St::Done, DBIE_OK, "");
+                else finish(r, St::Failed, s, msg);
+            }
+            ent->busy = false;
+            int64_t b = 0;
+            if (ent->op && dbie_op_device_bytes(ent->op, &b) == DBIE_OK) ent->bytes = (size_t)b;
+            if (!o.cache_ops && ent->users == 0) cache.erase(ent->key);
+            note_peak();
+            cv.notify_all();
+        }
+    }
+    int64_t build_count = 0;
+};
+
+extern "C" {
+
+void dbie_server_opts_default(dbie_server_opts *o) {
+    if (!o) return;
+    o->hbm_budget = 0.0;
+    o->setup_workers = 2;
+    o->setup_threads = 0;
+    o->solve_slots = 2;
+    o->max_batch_rhs = 256;
+    o->cache_ops = 1;
+    o->inject_oom_every = 0;
+}
+
+dbie_status dbie_server_create(const dbie_server_opts *opts, dbie_server **out) {
+    return dbie::guard([&] {
+        if (!out) dbie::fail(DBIE_BAD_ARG, "NULL output");
+        *out = nullptr;
+        dbie::ensure_device();
+        std::unique_ptr<dbie_server> s(new dbie_server);
+        if (opts) s->o = *opts;
+        else dbie_server_opts_default(&s->o);
+        dbie_server_opts &o = s->o;
+        if (o.setup_workers < 1 || o.solve_slots < 1 || o.max_batch_rhs < 1)
+            dbie::fail(DBIE_BAD_ARG, "setup_workers, solve_slots and max_batch_rhs must be >= 1");
+        if (o.setup_threads <= 0)
+            o.setup_threads = std::max(1, (int)std::thread::hardware_concurrency() / o.setup_workers);
+        size_t freeb = 0, total = 0;
+        DBIE_HIP(hipMemGetInfo(&freeb, &total));
+        s->budget = o.hbm_budget > 0 ? (size_t)o.hbm_budget : (size_t)(0.8 * (double)freeb);
+        for (int i = 0; i < o.setup_workers; ++i) s->threads.emplace_back([p = s.get()] { p->setup_loop(); });
+        for (int i = 0; i < o.solve_slots; ++i) s->threads.emplace_back([p = s.get()] { p->slot_loop(); });
+        *out = s.release();
+    });
+}
+
+void dbie_server_destroy(dbie_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    for (auto &t : s->threads) t.join();
+    s->reqs.clear();
+    s->queue.clear();
+    s->ready.clear();
+    s->cache.clear();
+    delete s;
+}
+
+dbie_status dbie_server_submit_box(dbie_server *s, double Lx, double Ly, double Lz, int n_quad, double l_ec,
+                                   double eps_in, double eps_out, int K, const double *pts, const double *q,
+                                   const dbie_solve_opts *opts, int64_t *id) {
+    return dbie::guard([&] {
+        if (!s || !id) dbie::fail(DBIE_BAD_ARG, "NULL server or id");
+        if (K < 1 || !pts || !q) dbie::fail(DBIE_BAD_ARG, "need K >= 1 charges with positions and values");
+        if (!(Lx > 0 && Ly > 0 && Lz > 0 && l_ec > 0) || n_quad < 1 || n_quad > 8)
+            dbie::fail(DBIE_BAD_ARG, "invalid box parameters");
+        if (!(eps_in > 0 && eps_out > 0) || eps_in == eps_out)
+            dbie::fail(DBIE_BAD_ARG, "permittivities must be positive and differ (t = (eo+ei)/(2(eo-ei)))");
+        auto r = std::make_shared<Request>();
+        r->L[0] = Lx;
+        r->L[1] = Ly;
+        r->L[2] = Lz;
+        r->nq = n_quad;
+        r->l_ec = l_ec;
+        r->eps_in = eps_in;
+        r->eps_out = eps_out;
+        r->K = K;
+        r->pts.assign(pts, pts + 3 * (size_t)K);
+        r->q.assign(q, q + K);
+        if (opts) r->sopts = *opts;
+        else {
+            dbie_solve_opts_default(&r->sopts);
+            r->sopts.precision = DBIE_FP16X3;
+        }
+        r->sopts.check_residual = 0;
+        // node count of the mesh, for the admission estimate (host mesher, no device work)
+        dbie_geom *g = nullptr;
+        if (dbie_geometry_box(Lx, Ly, Lz, n_quad, l_ec, std::sqrt(2.0), &g) != DBIE_OK)
+            dbie::fail(DBIE_BAD_ARG, last_error_text());
+        int64_t P = 0, N = 0;
+        int nq = 0;
+        dbie_geom_sizes(g, &P, &N, &nq);
+        dbie_geom_destroy(g);
+        r->pending_op = estimate_op_bytes(N, n_quad);
+        r->pending_ws = estimate_ws_bytes(N, std::min(K, s->o.max_batch_rhs));
+        r->t_arrive = Clock::now();
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        s->reqs[r->id] = r;
+        s->queue.push_back(r);
+        *id = r->id;
+        s->cv.notify_all();
+    });
+}
+
+dbie_status dbie_server_wait(dbie_server *s, int64_t id, double timeout_s, dbie_request_info *info) {
+    return dbie::guard([&] {
+        if (!s) dbie::fail(DBIE_BAD_ARG, "NULL server");
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) dbie::fail(DBIE_BAD_ARG, "unknown request id " + std::to_string(id));
+        const std::shared_ptr<Request> r = it->second;
+        auto finished = [&] { return r->st == St::Done || r->st == St::Failed || r->st == St::Cancelled; };
+        if (timeout_s < 0) s->cv.wait(lk, finished);
+        else s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
+        if (info) {
+            std::memset(info, 0, sizeof *info);
+            info->id = r->id;
+            info->status = r->status;
+            info->state = (int)r->st;
+            info->N = r->N;
+            info->K = r->K;
+            info->batch_K = r->batch_K;
+            info->solved = r->stats.solved;
+            info->niter = r->stats.niter;
+            info->nrefine = r->stats.nrefine;
+            info->max_rel_residual = r->stats.max_rel_residual;
+            info->op_cache_hit = r->cache_hit ? 1 : 0;
+            if (finished()) {
+                const bool set = r->t_setup0 != Clock::time_point{};
+                info->queue_ms = set ? ms_since(r->t_arrive, r->t_setup0) : 0.0;
+                info->setup_ms = set ? ms_since(r->t_setup0, r->t_setup1) : 0.0;
+                info->solve_ms = r->t_solve0 != Clock::time_point{} ? ms_since(r->t_solve0, r->t_solve1) : 0.0;
+                info->total_ms = ms_since(r->t_arrive, r->t_solve1);
+            }
+            std::strncpy(info->message, r->msg.c_str(), sizeof info->message - 1);
+        }
+        if (!finished()) dbie::fail(DBIE_PENDING, "request " + std::to_string(id) + " still running");
+    });
+}
+
+dbie_status dbie_server_take(dbie_server *s, int64_t id, double *Sigma, int64_t lds) {
+    return dbie::guard([&] {
+        if (!s) dbie::fail(DBIE_BAD_ARG, "NULL server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) dbie::fail(DBIE_BAD_ARG, "unknown request id " + std::to_string(id));
+        const std::shared_ptr<Request> r = it->second;
+        if (r->st != St::Done && r->st != St::Failed && r->st != St::Cancelled)
+            dbie::fail(DBIE_PENDING, "request still running");
+        if (Sigma) {
+            if (r->sigma.empty()) dbie::fail(DBIE_BAD_ARG, "request has no result: " + r->msg);
+            if (lds < r->N) dbie::fail(DBIE_DIM_MISMATCH, "lds smaller than N");
+            for (int k = 0; k < r->K; ++k)
+                std::memcpy(Sigma + (size_t)k * lds, &r->sigma[(size_t)k * r->N], sizeof(double) * r->N);
+        }
+        s->reqs.erase(it);
+    });
+}
+
+dbie_status dbie_server_cancel(dbie_server *s, int64_t id) {
+    return dbie::guard([&] {
+        if (!s) dbie::fail(DBIE_BAD_ARG, "NULL server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) dbie::fail(DBIE_BAD_ARG, "unknown request id " + std::to_string(id));
+        const std::shared_ptr<Request> r = it->second;
+        r->cancel = true;
+        if (r->st == St::Queued) {
+            for (auto q = s->queue.begin(); q != s->queue.end(); ++q)
+                if (*q == r) {
+                    s->queue.erase(q);
+                    break;
+                }
+            s->finish(r, St::Cancelled, DBIE_OK, "cancelled");
+        } else if (r->st == St::Ready) {
+            for (auto q = s->ready.begin(); q != s->ready.end(); ++q)
+                if (*q == r) {
+                    s->ready.erase(q);
+                    break;
+                }
+            s->finish(r, St::Cancelled, DBIE_OK, "cancelled");
+        }
+        s->cv.notify_all();
+    });
+}
+
+dbie_status dbie_server_stats(dbie_server *s, dbie_server_summary *out) {
+    return dbie::guard([&] {
+        if (!s || !out) dbie::fail(DBIE_BAD_ARG, "NULL argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        out->completed = s->done;
+        out->failed = s->failed;
+        out->operator_builds = s->builds;
+        out->cache_hits = s->hits;
+        out->batches = s->batches;
+        out->fused_requests = s->fused;
+        out->cached_operators = (int64_t)s->cache.size();
+        out->bytes_in_use = (int64_t)s->in_use();
+        out->peak_bytes = (int64_t)s->peak;
+        out->budget_bytes = (int64_t)s->budget;
+    });
+}
+
+}  // extern "C"

@@ -11,6 +11,7 @@ from .api import (  # noqa: F401
     TargetPotentialMap,
     DielectricInterface,
     PointSource,
+    Server,
     SharpScreening,
     VolumeSource,
     all_weights,
@@ -34,6 +35,7 @@ from .api import (  # noqa: F401
     reaction_matrix,
     rhs_point_charges,
     screened_volume_source,
+    serving_requests,
     single_dielectric_box3d,
     solve_dielectric_box3d_block,
     solve_gmres,

@@ -13,7 +13,7 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdbie.so")
 
-OK, BAD_ARG, DIM_MISMATCH, OOM, HIP_ERROR, NOT_CONVERGED, NO_DEVICE = range(7)
+OK, BAD_ARG, DIM_MISMATCH, OOM, HIP_ERROR, NOT_CONVERGED, NO_DEVICE, PENDING = range(8)
 FP64, FP32, BF16, BF16X3, FP16X3 = 0, 1, 2, 3, 4
 OP_DT, OP_S, OP_D = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16x3": BF16X3, "fp16x3": FP16X3}
@@ -44,6 +44,37 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("niter", C.c_int), ("nrefine", C.c_int), ("solved", C.c_int), ("max_rel_residual", C.c_double),
                 ("time_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_double), ("setup_workers", C.c_int), ("setup_threads", C.c_int),
+                ("solve_slots", C.c_int), ("max_batch_rhs", C.c_int), ("cache_ops", C.c_int),
+                ("inject_oom_every", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("state", C.c_int), ("N", C.c_int64), ("K", C.c_int),
+                ("batch_K", C.c_int), ("solved", C.c_int), ("niter", C.c_int), ("nrefine", C.c_int),
+                ("max_rel_residual", C.c_double), ("op_cache_hit", C.c_int), ("queue_ms", C.c_double),
+                ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("total_ms", C.c_double),
+                ("message", C.c_char * 256)]
+
+    STATES = ("queued", "setup", "ready", "solving", "done", "failed", "cancelled")
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["message"] = self.message.decode(errors="replace")
+        d["state"] = self.STATES[self.state] if 0 <= self.state < len(self.STATES) else str(self.state)
+        return d
+
+
+class ServerSummary(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("completed", "failed", "operator_builds", "cache_hits", "batches",
+                                         "fused_requests", "cached_operators", "bytes_in_use", "peak_bytes",
+                                         "budget_bytes")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -105,9 +136,18 @@ SIGNATURES = {
     "dbie_reaction_matrix": [_vp, C.c_int, _i64p, _dp, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
     "dbie_reaction_matrix_device": [_vp, C.c_int, _i64p, _dp, C.c_int, _vp, C.c_int64, _dp, C.c_int64],
     "dbie_gauss_hermite": [C.c_int, _dp, _dp],
+    "dbie_server_opts_default": [C.POINTER(ServerOpts)],
+    "dbie_server_create": [C.POINTER(ServerOpts), C.POINTER(_vp)],
+    "dbie_server_destroy": [_vp],
+    "dbie_server_submit_box": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                               C.c_double, C.c_int, _dp, _dp, C.POINTER(SolveOpts), _i64p],
+    "dbie_server_wait": [_vp, C.c_int64, C.c_double, C.POINTER(RequestInfo)],
+    "dbie_server_take": [_vp, C.c_int64, _dp, C.c_int64],
+    "dbie_server_cancel": [_vp, C.c_int64],
+    "dbie_server_stats": [_vp, C.POINTER(ServerSummary)],
 }
 _RESTYPES = {"dbie_version": C.c_char_p, "dbie_op_stream": _vp, "dbie_geom_destroy": None,
-             "dbie_operator_destroy": None, "dbie_tmap_destroy": None, "dbie_near_opts_default": None, "dbie_solve_opts_default": None}
+             "dbie_operator_destroy": None, "dbie_tmap_destroy": None, "dbie_server_destroy": None, "dbie_server_opts_default": None, "dbie_near_opts_default": None, "dbie_solve_opts_default": None}
 
 
 def lib():

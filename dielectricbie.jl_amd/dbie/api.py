@@ -750,3 +750,112 @@ def reaction_matrix(interface: DielectricInterface, sources, Sigma, *, lhs_tol: 
     elif tmap.M != len(q):
         raise ValueError("DimensionMismatch: target map does not match the sources")
     return tmap.reaction(grp, q, Sigma), tmap
+
+
+# ---------------------------------------------------------------------------------------------
+# Request server (continuous batching of many block solves)
+# ---------------------------------------------------------------------------------------------
+class Server:
+    """Continuous-batching request loop on the GPU (libdbie's dbie_server_*).
+
+    submit_box() queues a single dielectric box with K point charges (eps_src = eps_in) and
+    returns an id at once; wait() blocks until it is done (or a timeout) and returns its info;
+    result() returns the N x K density and forgets the request.  Requests are admitted by device
+    memory footprint, set up on host threads while others run on the GPU, and ready requests on
+    one cached operator with equal options are fused into one block solve."""
+
+    def __init__(self, *, hbm_budget: float = 0.0, setup_workers: int = 2, setup_threads: int = 0,
+                 solve_slots: int = 2, max_batch_rhs: int = 256, cache_ops: bool = True, inject_oom_every: int = 0):
+        o = L.ServerOpts()
+        L.lib().dbie_server_opts_default(C.byref(o))
+        o.hbm_budget, o.setup_workers, o.setup_threads = float(hbm_budget), int(setup_workers), int(setup_threads)
+        o.solve_slots, o.max_batch_rhs, o.cache_ops = int(solve_slots), int(max_batch_rhs), int(bool(cache_ops))
+        o.inject_oom_every = int(inject_oom_every)
+        h = C.c_void_p()
+        L.check(L.lib().dbie_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and L._lib is not None:
+            L._lib.dbie_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit_box(self, Lx, Ly, Lz, n_quad, l_ec, eps_in, eps_out, points, charges, *, precision="fp16x3",
+                   rtol=1e-10, itmax=500, inner_rtol=1e-5, max_refine=10) -> int:
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(charges, np.float64).ravel()
+        if len(q) != pts.shape[0]:
+            raise ValueError("points and charges must have the same length")
+        o = L.SolveOpts()
+        L.lib().dbie_solve_opts_default(C.byref(o))
+        o.rtol, o.itmax = float(rtol), int(itmax)
+        o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.inner_rtol, o.max_refine = float(inner_rtol), int(max_refine)
+        rid = C.c_int64()
+        L.check(L.lib().dbie_server_submit_box(self._h, Lx, Ly, Lz, int(n_quad), l_ec, eps_in, eps_out, len(q),
+                                               L.dptr(pts.ravel()), L.dptr(q), C.byref(o), C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout: float | None = None):
+        """Info dict of the request; None if it is still running after `timeout` seconds."""
+        info = L.RequestInfo()
+        st = L.lib().dbie_server_wait(self._h, int(rid), -1.0 if timeout is None else float(timeout), C.byref(info))
+        if st == L.PENDING:
+            return None
+        L.check(st)
+        return info.as_dict()
+
+    def result(self, rid: int) -> np.ndarray:
+        """N x K density of a finished request (raises if it failed); the request is forgotten."""
+        info = self.wait(rid)
+        if info["status"] not in (L.OK,):
+            L.lib().dbie_server_take(self._h, int(rid), None, 0)
+            raise L.DbieError(info["status"], f"request {rid} failed: {info['message']}")
+        S = np.zeros((info["N"], info["K"]), order="F")
+        L.check(L.lib().dbie_server_take(self._h, int(rid), L.dptr(S), info["N"]))
+        return S
+
+    def forget(self, rid: int):
+        L.check(L.lib().dbie_server_take(self._h, int(rid), None, 0))
+
+    def cancel(self, rid: int):
+        L.check(L.lib().dbie_server_cancel(self._h, int(rid)))
+
+    def stats(self) -> dict:
+        s = L.ServerSummary()
+        L.check(L.lib().dbie_server_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+
+def serving_requests(n: int, seed: int = 0, rate: float = 8.0, l_ecs=(0.1, 0.07, 0.05), kmax: int = 64):
+    """Seeded request stream of the serving workload: Poisson arrivals at `rate` per second, box sides
+    ~U(0.5, 2.0), l_ec uniform over l_ecs, n_quad 4, eps_in ~U(2, 80), eps_out 1, 1..kmax charges
+    uniform in the box at least 0.05 from every face, q ~U(-1, 1).  Returns a list of dicts."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate)
+        L3 = rng.uniform(0.5, 2.0, 3)
+        l_ec = float(l_ecs[rng.integers(len(l_ecs))])
+        eps_in = float(rng.uniform(2.0, 80.0))
+        K = int(rng.integers(1, kmax + 1))
+        half = L3 / 2 - 0.05
+        pts = rng.uniform(-half, half, (K, 3))
+        q = rng.uniform(-1.0, 1.0, K)
+        out.append({"t": t, "L": tuple(float(v) for v in L3), "l_ec": l_ec, "eps_in": eps_in, "eps_out": 1.0,
+                    "n_quad": 4, "points": pts, "charges": q})
+    return out
+

@@ -36,7 +36,8 @@ enum {
     DBIE_OOM = 3,            /* device or host allocation failed */
     DBIE_HIP_ERROR = 4,      /* a HIP runtime call failed */
     DBIE_NOT_CONVERGED = 5,  /* iterative solve hit itmax (Krylov stats.solved == false) */
-    DBIE_NO_DEVICE = 6       /* no usable gfx950 device */
+    DBIE_NO_DEVICE = 6,      /* no usable gfx950 device */
+    DBIE_PENDING = 7         /* request server: the request has not finished yet */
 };
 
 /* Precision of the far-field kernel x RHS-block contraction. */
@@ -228,6 +229,56 @@ dbie_status dbie_reaction_matrix_device(dbie_tmap *tm, int G, const int64_t *grp
    blobs (no counterpart in the reference, whose GaussianVolumeSource uses a midpoint grid,
    src/core/sources.jl:517-548) */
 dbie_status dbie_gauss_hermite(int n, double *x, double *w);
+
+/* ---- request server: continuous batching of many block solves ----------- */
+/* No counterpart in the reference, whose callers loop over solve_dielectric_box3d_block
+   (src/solver/multi_rhs.jl:270-286).  Requests (a box and its point charges) are admitted by
+   device-memory footprint, set up on host threads (mesh, near field, RHS) while other requests
+   run, and solved by a few concurrent slots; ready requests on one cached operator with the same
+   options are fused into one block solve.  Results are the same alone and in a mix, within the
+   solve tolerance. */
+typedef struct dbie_server dbie_server;
+typedef struct {
+    double hbm_budget;     /* device bytes for operators + workspaces (0 = 80% of free memory) */
+    int setup_workers;     /* concurrent request setups (default 2) */
+    int setup_threads;     /* host threads per near-field build (0 = cores / setup_workers) */
+    int solve_slots;       /* concurrent block solves, each on its operator's stream (default 2) */
+    int max_batch_rhs;     /* columns fused into one block solve (default 256) */
+    int cache_ops;         /* keep idle operators for later requests (default 1) */
+    int inject_oom_every;  /* test hook: every n-th operator build fails with DBIE_OOM (0 = off) */
+} dbie_server_opts;
+typedef struct {
+    int64_t id;
+    dbie_status status;    /* the request's own outcome (DBIE_NOT_CONVERGED, DBIE_OOM, ...) */
+    int state;             /* 0 queued, 1 setup, 2 ready, 3 solving, 4 done, 5 failed, 6 cancelled */
+    int64_t N;
+    int K, batch_K;        /* own columns; columns of the block solve it was part of */
+    int solved, niter, nrefine;
+    double max_rel_residual;
+    int op_cache_hit;
+    double queue_ms, setup_ms, solve_ms, total_ms;  /* arrival -> setup, setup, solve, arrival -> result */
+    char message[256];
+} dbie_request_info;
+typedef struct {
+    int64_t completed, failed, operator_builds, cache_hits, batches, fused_requests, cached_operators;
+    int64_t bytes_in_use, peak_bytes, budget_bytes;
+} dbie_server_summary;
+
+void dbie_server_opts_default(dbie_server_opts *o);
+dbie_status dbie_server_create(const dbie_server_opts *o, dbie_server **out);
+/* stops the workers after the solves in progress and frees every operator */
+void dbie_server_destroy(dbie_server *s);
+/* single_dielectric_box3d(Lx, Ly, Lz, n_quad, l_ec, eps_in, eps_out) with K point charges (pts 3 x K,
+   q) at eps_src = eps_in; opts NULL = defaults with DBIE_FP16X3.  Returns at once with an id. */
+dbie_status dbie_server_submit_box(dbie_server *s, double Lx, double Ly, double Lz, int n_quad, double l_ec,
+                                   double eps_in, double eps_out, int K, const double *pts, const double *q,
+                                   const dbie_solve_opts *opts, int64_t *id);
+/* waits up to timeout_s seconds (< 0: no limit); DBIE_PENDING if unfinished.  info may be NULL. */
+dbie_status dbie_server_wait(dbie_server *s, int64_t id, double timeout_s, dbie_request_info *info);
+/* copies the N x K density of a finished request (Sigma may be NULL) and forgets the request */
+dbie_status dbie_server_take(dbie_server *s, int64_t id, double *Sigma, int64_t lds);
+dbie_status dbie_server_cancel(dbie_server *s, int64_t id);
+dbie_status dbie_server_stats(dbie_server *s, dbie_server_summary *st);
 
 /* ---- kernel timing (HIP events on the operator's stream) ----------------- */
 /* No counterpart in the reference (it has no profiler hooks); used by the benchmark to

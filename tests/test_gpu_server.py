@@ -1,0 +1,133 @@
+"""The request server on the GPU: results are the same alone and in a mix, failures stay with
+their request, memory admission and cancellation (run with -m gpu)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+BOX_A = (1.0, 0.8, 0.6, 4, 0.1, 10.0, 1.0)
+
+
+def charges(seed, K, L):
+    rng = np.random.default_rng(seed)
+    half = np.array(L) / 2 - 0.05
+    return rng.uniform(-half, half, (K, 3)), rng.uniform(-1, 1, K)
+
+
+def test_alone_equals_mix(dbie):
+    pa, qa = charges(1, 5, BOX_A[:3])
+    say("server: request A alone (FP64 and split-FP16)")
+    with dbie.Server(solve_slots=2) as srv:
+        ia = srv.submit_box(*BOX_A, pa, qa, precision="fp64", rtol=1e-12)
+        ib = srv.submit_box(*BOX_A, pa, qa, precision="fp16x3", rtol=1e-11)
+        S64_alone = srv.result(ia)
+        Sm_alone = srv.result(ib)
+    # reference: the same solve through the operator API
+    itf = dbie.single_dielectric_box3d(*BOX_A[:5], BOX_A[5], BOX_A[6])
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(itf, 1e-9, 1e-9, 8)
+    F = dbie.rhs_point_charges(itf, pa, qa, BOX_A[5])
+    S_ref, _ = op.solve(F, rtol=1e-12)
+    assert rel(S64_alone, S_ref) < 1e-10
+    for k in range(5):
+        assert rel(Sm_alone[:, k], S_ref[:, k]) < 1e-8
+
+    say("server: A inside a mix of 8 requests (3 share its operator, 4 other boxes)")
+    # the first request keeps the single slot busy while A and its companions are set up, so that
+    # they are fused into one block solve
+    others = [((1.6, 1.2, 1.0, 4, 0.05, 4.0, 1.0), 3), ((0.6, 0.6, 0.6, 4, 0.07, 40.0, 1.0), 7),
+              ((1.1, 0.9, 0.5, 4, 0.1, 2.5, 1.0), 1), ((0.9, 0.9, 0.9, 4, 0.1, 77.0, 1.0), 12)]
+    with dbie.Server(solve_slots=1, setup_workers=3) as srv:
+        ids = {}
+        for j, (box, K) in enumerate(others[:2]):
+            p, q = charges(10 + j, K, box[:3])
+            ids[f"o{j}"] = srv.submit_box(*box, p, q, precision="fp64", rtol=1e-12)
+        for j in range(2):  # companions on A's operator
+            p, q = charges(20 + j, 4 + 3 * j, BOX_A[:3])
+            ids[f"c{j}"] = srv.submit_box(*BOX_A, p, q, precision="fp64", rtol=1e-12)
+        ids["A"] = srv.submit_box(*BOX_A, pa, qa, precision="fp64", rtol=1e-12)
+        ids["Am"] = srv.submit_box(*BOX_A, pa, qa, precision="fp16x3", rtol=1e-11)
+        for j, (box, K) in enumerate(others[2:]):
+            p, q = charges(30 + j, K, box[:3])
+            ids[f"p{j}"] = srv.submit_box(*box, p, q, precision="fp16x3", rtol=1e-11)
+        infos = {k: srv.wait(v) for k, v in ids.items()}
+        assert all(i["status"] == 0 and i["state"] == "done" for i in infos.values()), infos
+        S64_mix = srv.result(ids["A"])
+        Sm_mix = srv.result(ids["Am"])
+        st = srv.stats()
+    say(f"  stats {st}; A batch columns {infos['A']['batch_K']}")
+    assert st["completed"] == len(ids) and st["failed"] == 0
+    assert st["cache_hits"] >= 2 and st["operator_builds"] == 5
+    assert st["fused_requests"] >= 1 and infos["A"]["batch_K"] > 5  # A shared a block solve
+    assert rel(S64_mix, S64_alone) < 1e-10
+    for k in range(5):
+        assert rel(Sm_mix[:, k], S64_alone[:, k]) < 1e-8
+        assert rel(Sm_mix[:, k], Sm_alone[:, k]) < 1e-8
+
+
+def test_failures_stay_with_their_request(dbie):
+    say("server: injected allocation failure, non-convergence, over-budget, cancellation")
+    boxes = [(1.0 + 0.1 * j, 0.8, 0.6, 4, 0.1, 10.0, 1.0) for j in range(4)]
+    with dbie.Server(inject_oom_every=2, solve_slots=2, setup_workers=1) as srv:
+        ids = []
+        for j, box in enumerate(boxes):
+            p, q = charges(40 + j, 3, box[:3])
+            ids.append(srv.submit_box(*box, p, q, precision="fp64"))
+        p, q = charges(50, 2, boxes[0][:3])
+        nc = srv.submit_box(*boxes[0], p, q, precision="fp64", itmax=1)
+        infos = [srv.wait(i) for i in ids]
+        oom = [i for i in infos if i["status"] == 3]
+        ok = [i for i in infos if i["status"] == 0]
+        assert len(oom) == 2 and len(ok) == 2, infos
+        for i in ok:
+            assert srv.result(i["id"]).shape == (i["N"], 3)
+        for i in oom:
+            with pytest.raises(Exception):
+                srv.result(i["id"])
+        inc = srv.wait(nc)
+        assert inc["status"] in (5, 3), inc  # not converged (or its operator build was the failing one)
+    with dbie.Server(hbm_budget=1e6) as srv:
+        p, q = charges(60, 2, boxes[0][:3])
+        r = srv.submit_box(*boxes[0], p, q)
+        info = srv.wait(r)
+        assert info["status"] == 3 and "budget" in info["message"]
+    with dbie.Server(setup_workers=1, solve_slots=1) as srv:
+        rs = []
+        for j in range(4):
+            p, q = charges(70 + j, 2, boxes[j][:3])
+            rs.append(srv.submit_box(*boxes[j], p, q))
+        srv.cancel(rs[-1])
+        last = srv.wait(rs[-1])
+        assert last["state"] == "cancelled"
+        assert all(srv.wait(r)["state"] == "done" for r in rs[:-1])
+
+
+def test_serving_stream_small(dbie):
+    """A short seeded stream of the serving workload (boxes scaled to keep the test quick)."""
+    reqs = dbie.serving_requests(10, seed=3, rate=50.0, l_ecs=(0.2, 0.15), kmax=16)
+    say("server: 10-request stream, Poisson arrivals")
+    import time
+    with dbie.Server(solve_slots=2) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for r in reqs:
+            while time.perf_counter() - t0 < r["t"]:
+                time.sleep(0.001)
+            L = tuple(0.5 * v for v in r["L"])
+            pts = r["points"] * 0.5
+            ids.append(srv.submit_box(*L, r["n_quad"], r["l_ec"], r["eps_in"], r["eps_out"], pts, r["charges"]))
+        infos = [srv.wait(i) for i in ids]
+        assert all(i["status"] == 0 for i in infos), infos
+        lat = [i["total_ms"] for i in infos]
+        say(f"  p50 {np.percentile(lat, 50):.1f} ms  p99 {np.percentile(lat, 99):.1f} ms  stats {srv.stats()}")
