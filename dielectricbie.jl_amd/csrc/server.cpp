@@ -114,6 +114,7 @@ struct dbie_server {
     std::mutex mu;
     std::condition_variable cv;
     bool stop = false;
+    bool paused = false;   // slots start no new batch while set
     int64_t next_id = 1;
     uint64_t tick = 0;
     std::map<int64_t, std::shared_ptr<Request>> reqs;  // every request not yet taken
@@ -321,6 +322,7 @@ struct dbie_server {
             std::vector<std::shared_ptr<Request>> batch;
             cv.wait(lk, [&] {
                 if (stop) return true;
+                if (paused) return false;
                 for (auto &r : ready)
                     if (!r->ent->busy) return true;
                 return false;
@@ -566,6 +568,15 @@ dbie_status dbie_server_cancel(dbie_server *s, int64_t id) {
                 }
             s->finish(r, St::Cancelled, DBIE_OK, "cancelled");
         }
+        s->cv.notify_all();
+    });
+}
+
+dbie_status dbie_server_pause(dbie_server *s, int paused) {
+    return dbie::guard([&] {
+        if (!s) dbie::fail(DBIE_BAD_ARG, "NULL server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->paused = paused != 0;
         s->cv.notify_all();
     });
 }

@@ -144,6 +144,7 @@ SIGNATURES = {
     "dbie_server_wait": [_vp, C.c_int64, C.c_double, C.POINTER(RequestInfo)],
     "dbie_server_take": [_vp, C.c_int64, _dp, C.c_int64],
     "dbie_server_cancel": [_vp, C.c_int64],
+    "dbie_server_pause": [_vp, C.c_int],
     "dbie_server_stats": [_vp, C.POINTER(ServerSummary)],
 }
 _RESTYPES = {"dbie_version": C.c_char_p, "dbie_op_stream": _vp, "dbie_geom_destroy": None,

@@ -817,6 +817,18 @@ class Server:
         L.check(st)
         return info.as_dict()
 
+    def info(self, rid: int) -> dict:
+        """Info dict of the request in whatever state it is (no waiting)."""
+        info = L.RequestInfo()
+        st = L.lib().dbie_server_wait(self._h, int(rid), 0.0, C.byref(info))
+        if st not in (L.OK, L.PENDING):
+            L.check(st)
+        return info.as_dict()
+
+    def pause(self, paused: bool = True):
+        """Stop (or resume) starting new block solves; setup goes on, ready requests accumulate."""
+        L.check(L.lib().dbie_server_pause(self._h, int(bool(paused))))
+
     def result(self, rid: int) -> np.ndarray:
         """N x K density of a finished request (raises if it failed); the request is forgotten."""
         info = self.wait(rid)

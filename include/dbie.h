@@ -278,6 +278,9 @@ dbie_status dbie_server_wait(dbie_server *s, int64_t id, double timeout_s, dbie_
 /* copies the N x K density of a finished request (Sigma may be NULL) and forgets the request */
 dbie_status dbie_server_take(dbie_server *s, int64_t id, double *Sigma, int64_t lds);
 dbie_status dbie_server_cancel(dbie_server *s, int64_t id);
+/* paused != 0: solve slots start no new block solve (setup continues), e.g. to drain the device or
+   to let ready requests accumulate; 0 resumes */
+dbie_status dbie_server_pause(dbie_server *s, int paused);
 dbie_status dbie_server_stats(dbie_server *s, dbie_server_summary *st);
 
 /* ---- kernel timing (HIP events on the operator's stream) ----------------- */

@@ -44,11 +44,12 @@ def test_alone_equals_mix(dbie):
         assert rel(Sm_alone[:, k], S_ref[:, k]) < 1e-8
 
     say("server: A inside a mix of 8 requests (3 share its operator, 4 other boxes)")
-    # the first request keeps the single slot busy while A and its companions are set up, so that
-    # they are fused into one block solve
-    others = [((1.6, 1.2, 1.0, 4, 0.05, 4.0, 1.0), 3), ((0.6, 0.6, 0.6, 4, 0.07, 40.0, 1.0), 7),
+    # slots are paused until every request is set up, so that A and its companions are fused
+    # into one block solve
+    others = [((1.3, 0.7, 0.9, 4, 0.1, 4.0, 1.0), 3), ((0.6, 0.6, 0.6, 4, 0.07, 40.0, 1.0), 7),
               ((1.1, 0.9, 0.5, 4, 0.1, 2.5, 1.0), 1), ((0.9, 0.9, 0.9, 4, 0.1, 77.0, 1.0), 12)]
-    with dbie.Server(solve_slots=1, setup_workers=3) as srv:
+    with dbie.Server(solve_slots=2, setup_workers=3) as srv:
+        srv.pause(True)
         ids = {}
         for j, (box, K) in enumerate(others[:2]):
             p, q = charges(10 + j, K, box[:3])
@@ -61,6 +62,12 @@ def test_alone_equals_mix(dbie):
         for j, (box, K) in enumerate(others[2:]):
             p, q = charges(30 + j, K, box[:3])
             ids[f"p{j}"] = srv.submit_box(*box, p, q, precision="fp16x3", rtol=1e-11)
+        import time
+        t0 = time.perf_counter()
+        while any(srv.info(v)["state"] != "ready" for v in ids.values()):
+            assert time.perf_counter() - t0 < 120, [srv.info(v) for v in ids.values()]
+            time.sleep(0.01)
+        srv.pause(False)
         infos = {k: srv.wait(v) for k, v in ids.items()}
         assert all(i["status"] == 0 and i["state"] == "done" for i in infos.values()), infos
         S64_mix = srv.result(ids["A"])
@@ -69,7 +76,7 @@ def test_alone_equals_mix(dbie):
     say(f"  stats {st}; A batch columns {infos['A']['batch_K']}")
     assert st["completed"] == len(ids) and st["failed"] == 0
     assert st["cache_hits"] >= 2 and st["operator_builds"] == 5
-    assert st["fused_requests"] >= 1 and infos["A"]["batch_K"] > 5  # A shared a block solve
+    assert st["fused_requests"] >= 2 and infos["A"]["batch_K"] == 5 + 4 + 7  # A shared a block solve
     assert rel(S64_mix, S64_alone) < 1e-10
     for k in range(5):
         assert rel(Sm_mix[:, k], S64_alone[:, k]) < 1e-8
