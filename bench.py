@@ -234,8 +234,13 @@ def main():
             op.profile(False)
             res[prec] = {k: v[1] / max(v[0], 1) for k, v in prof.items() if v[0]}
             log(f"apply {prec}: per-launch ms {res[prec]}")
+        # calibration for HBM-traffic counters: one copy kernel of known size (read 1 GiB, write 1 GiB)
+        src = dbie.DeviceMatrix(2 ** 24, 8)
+        dst = dbie.DeviceMatrix(2 ** 24, 8)
+        dst.copy_from(src)
         print(json.dumps({"mode": "apply", "N": N, "K": a.K, "per_launch_ms": res,
-                          "cross_plane_pairs": cross_plane_pairs(iface)}), flush=True)
+                          "cross_plane_pairs": cross_plane_pairs(iface), "calibration_copy_bytes": 2 ** 30}),
+              flush=True)
         return
 
     # plain FP64 comparator (also the reference solution for the error)
@@ -291,11 +296,13 @@ def main():
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": None, "kernel": ("far_mfma_f64" if name == "far_fp64" else f"far_mfma_{a.precision}"),
                 "avg_launch_ms": avg_ms, "launches": launches, "algorithmic_flops_per_launch": flops}
+    # HBM bytes per launch of the dominant kernel, from the committed FETCH_SIZE/WRITE_SIZE passes
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         tr = json.load(open(traffic_file)).get(roof.get("kernel"))
-        if tr:
-            roof["traffic"] = tr
+        if tr and a.K == 256:
+            roof["traffic"] = tr["bytes"]
+            roof["traffic_fetch_write"] = [tr["fetch_bytes"], tr["write_bytes"]]
     roof["kernels_ms"] = {k: v[1] for k, v in prof.items()}
 
     cpu = None if a.no_cpu else cpu_baseline(iters64)

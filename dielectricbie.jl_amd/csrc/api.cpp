@@ -176,19 +176,26 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
     op->part.ensure(need * sizeof(double));
     int S;
     bool exclude = false;  // far field leaves the near panel pairs out (their full interaction is added below)
-    {
+    if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
+        // profiling events bracket the contraction kernel alone (not the operand preparation)
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (op->profiling) {
+            DBIE_HIP(hipEventCreate(&e0));
+            DBIE_HIP(hipEventCreate(&e1));
+            op->prof[prec == DBIE_FP64 ? 0 : 1].push_back({e0, e1});
+        }
+        if (prec == DBIE_FP16X3 && op->f16_gscale == 0.0f) {
+            op->gmax.ensure(sizeof(unsigned));
+            op->f16_gscale = far_field_f16_scale(op->stream, op->dg, op->gmax.p);
+            if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
+        }
+        op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
+        exclude = prec == DBIE_FP16X3 && op->dg.near_mask && op->bsr.valup32;
+        S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p,
+                           op->f16_gscale, exclude, e0, e1);
+    } else {
         ProfScope ps(op, prec == DBIE_FP64 ? 0 : 1);
-        if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
-            if (prec == DBIE_FP16X3 && op->f16_gscale == 0.0f) {
-                op->gmax.ensure(sizeof(unsigned));
-                op->f16_gscale = far_field_f16_scale(op->stream, op->dg, op->gmax.p);
-                if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
-            }
-            op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
-            exclude = prec == DBIE_FP16X3 && op->dg.near_mask && op->bsr.valup32;
-            S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p,
-                               op->f16_gscale, exclude);
-        } else if (prec == DBIE_FP64) {
+        if (prec == DBIE_FP64) {
             S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
         } else {
             S = far_field_f32(op->stream, op->dg, K, X, ldx, op->part.as<double>(), need);
@@ -1150,6 +1157,19 @@ dbie_status dbie_memcpy_h2d(void *dst, const void *src, size_t bytes) {
     return guard([&] {
         if (bytes && (!dst || !src)) fail(DBIE_BAD_ARG, "NULL pointer");
         if (bytes) DBIE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+dbie_status dbie_copy_block_device(int64_t N, int K, const double *dsrc, int64_t lds, double *ddst, int64_t ldd) {
+    return guard([&] {
+        ensure_device();
+        if (N < 0 || K < 0 || ((N > 0 && K > 0) && (!dsrc || !ddst))) fail(DBIE_BAD_ARG, "bad copy arguments");
+        if (lds < N || ldd < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        if (N > 0 && K > 0) {
+            copy_cols(nullptr, dsrc, lds, ddst, ldd, N, K);
+            DBIE_HIP(hipGetLastError());
+            DBIE_HIP(hipStreamSynchronize(nullptr));
+        }
     });
 }
 

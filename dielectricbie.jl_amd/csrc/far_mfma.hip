@@ -24,6 +24,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include <algorithm>
 
@@ -399,15 +400,20 @@ __global__ void zero_u32_kernel(unsigned *__restrict__ p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-// per-column max |w_j X_jk| as float bits (non-negative floats order like unsigned ints)
-__global__ void colmax_kernel(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
-                              unsigned *__restrict__ cmax) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
+// per-column max |w_j X_jk| as float bits (non-negative floats order like unsigned ints);
+// one workgroup per column
+__global__ __launch_bounds__(256) void colmax_kernel(const double *__restrict__ X, int64_t ldx,
+                                                     const double *__restrict__ w, int64_t N,
+                                                     unsigned *__restrict__ cmax) {
+    __shared__ float s_m[4];
+    const int k = blockIdx.x;
+    const double *xk = X + (int64_t)k * ldx;
     float m = 0.0f;
-    for (int64_t jj = j; jj < N; jj += (int64_t)gridDim.x * blockDim.x) m = fmaxf(m, fabsf((float)(w[jj] * X[jj + (int64_t)k * ldx])));
+    for (int64_t j = threadIdx.x; j < N; j += 256) m = fmaxf(m, fabsf((float)(w[j] * xk[j])));
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(cmax + k, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) cmax[k] = __float_as_uint(fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3])));
 }
 
 __device__ inline int col_exponent(unsigned bits) {
@@ -448,6 +454,32 @@ struct NearMask {
     int mwords = 0;
 };
 
+// Lane n of each 16-lane row broadcast to the whole row (DPP row_newbcast).
+template <int N>
+__device__ inline float row_bcast_c(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + N, 0xF, 0xF, true));
+}
+__device__ inline float row_bcast(float v, int n) {  // n is a constant after unrolling
+    switch (n) {
+    case 0: return row_bcast_c<0>(v);
+    case 1: return row_bcast_c<1>(v);
+    case 2: return row_bcast_c<2>(v);
+    case 3: return row_bcast_c<3>(v);
+    case 4: return row_bcast_c<4>(v);
+    case 5: return row_bcast_c<5>(v);
+    case 6: return row_bcast_c<6>(v);
+    case 7: return row_bcast_c<7>(v);
+    case 8: return row_bcast_c<8>(v);
+    case 9: return row_bcast_c<9>(v);
+    case 10: return row_bcast_c<10>(v);
+    case 11: return row_bcast_c<11>(v);
+    case 12: return row_bcast_c<12>(v);
+    case 13: return row_bcast_c<13>(v);
+    case 14: return row_bcast_c<14>(v);
+    default: return row_bcast_c<15>(v);
+    }
+}
+
 template <int NCB, int EXCL>
 __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int K, int Kp, const _Float16 *__restrict__ zh,
                                                                   const _Float16 *__restrict__ zl,
@@ -456,8 +488,12 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
                                                                   NearMask nm) {
     using I = Items<NCB>;
     constexpr int KS = kSb / 32;
-    __shared__ uint4 s_g[3][2][kWaves][KS][64];  // [stage % 3][hi/lo][slice][k-slice][lane]
-    __shared__ float4 s_pos[2][kSb];  // source coordinates relative to the tile origin
+    __shared__ uint4 s_g[2][2][kWaves][KS][64];  // [stage parity][hi/lo][slice][k-slice][lane]
+    // source coordinates relative to the tile origin, staged in LDS.  Unless EXCL = 2, lane
+    // 16 g + 8 ks + e of each wave reads only source 32 ks + 8 g + e of the stage and the lanes of
+    // row g take the others by a row broadcast (one LDS read per lane and stage)
+    constexpr bool kRegPos = EXCL != 2;
+    __shared__ float4 s_pos[2][kSb];
     __shared__ int s_rel[EXCL == 2 ? 2 : 1][kSb];  // source panel - first panel of the stage
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t N = g.N;
@@ -490,8 +526,9 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
     v4f acc[I::kN];
 #pragma unroll
     for (int q = 0; q < I::kN; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
-    uint4 bh_cur[I::kCbPerWave][KS], bh_nxt[I::kCbPerWave][KS];
-    uint4 bl_cur[I::kCbPerWave][KS], bl_nxt[I::kCbPerWave][KS];
+    // B operands of the current and the next stage; the loop is unrolled by two so that the two
+    // halves swap roles without register copies (a copy would wait for the loads in flight)
+    uint4 bh[2][I::kCbPerWave][KS], bl[2][I::kCbPerWave][KS];
     auto load_b = [&](int64_t jb, uint4 (&bh)[I::kCbPerWave][KS], uint4 (&bl)[I::kCbPerWave][KS]) {
         const int64_t base = ((jb / kSb) * Kp + k0) * kSb;
 #pragma unroll
@@ -502,6 +539,14 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
                 bh[q][ks] = *reinterpret_cast<const uint4 *>(zh + off);
                 bl[q][ks] = *reinterpret_cast<const uint4 *>(zl + off);
             }
+    };
+    const int my_src = 32 * ((lane >> 3) & 1) + 8 * (lane >> 4) + (lane & 7);
+    float px = 0.f, py = 0.f, pz = 0.f;  // relative FP32 coordinates of my_src in the generated stage
+    auto take_pos = [&](int pb) {
+        const float4 sp = s_pos[pb][my_src];
+        px = sp.x;
+        py = sp.y;
+        pz = sp.z;
     };
     auto load_pos = [&](int64_t jb, int pb) {
         if (tid < kSb) {
@@ -518,9 +563,21 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int sidx = ks * 32 + 8 * (lane >> 4) + e + u;
-            const float4 sp = s_pos[pb][sidx];
-            const float dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
-            float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+            float sx, sy, sz;
+            if (kRegPos) {
+                sx = row_bcast(px, 8 * ks + e + u);
+                sy = row_bcast(py, 8 * ks + e + u);
+                sz = row_bcast(pz, 8 * ks + e + u);
+            } else {
+                const float4 sp = s_pos[pb][sidx];
+                sx = sp.x;
+                sy = sp.y;
+                sz = sp.z;
+            }
+            const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
+            // + 1e-20 keeps the self pair finite (d = 0: rinv^3 = 1e30, entry 0 * 1e30 = 0) at the
+            // cost of no instruction: it folds into the first fused multiply-add
+            float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-20f)));
             if (EXCL == 2) r2 = ((win >> (s_rel[pb][sidx] & 63)) & 1) ? 1.0e30f : r2;
             const float rinv = __builtin_amdgcn_rsqf(r2);
             v[u] = (n0 * dx + n1 * dy + n2 * dz) * (rinv * rinv * rinv);
@@ -549,12 +606,13 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
 
     int64_t jc = walk.next(jb0);
     int64_t j1 = walk.jend;
-    uint64_t win_nxt = 0;
+    uint64_t win[2] = {0, 0};  // exclusion window of the stage generated in an iteration of parity C
     if (jc < walk.jend) {
         j1 = walk.next(jc + kSb);
         load_pos(jc, 0);
         const uint64_t win0 = load_win(jc);
         __syncthreads();
+        if (kRegPos) take_pos(0);
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             uint4 h4, l4;
@@ -563,34 +621,39 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
             s_g[0][1][wave][ks][lane] = l4;
         }
         if (j1 < walk.jend) load_pos(j1, 1);
-        win_nxt = load_win(j1);
-        load_b(jc, bh_cur, bl_cur);
+        win[0] = load_win(j1);
+        load_b(jc, bh[0], bl[0]);
         __syncthreads();
     }
-    int sg = 0, pb = 1;
-    while (jc < walk.jend) {
+    // one stage: MFMAs on stage jc (LDS tile s_g[C], B operands bh/bl[C]) while stage j1 is generated
+    // into s_g[1 - C]; loads for the stage after are issued into the other parity
+    auto step = [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
         const bool more = j1 < walk.jend;
         const int64_t j2 = more ? walk.next(j1 + kSb) : walk.jend;
-        load_b(more ? j1 : jc, bh_nxt, bl_nxt);
-        const uint64_t win = win_nxt;
-        win_nxt = load_win(j2);
-        uint4 gh[KS], gl[KS];
+        load_b(more ? j1 : jc, bh[1 - C], bl[1 - C]);
+        win[1 - C] = load_win(j2);
+        if (kRegPos) take_pos(1 - C);
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            gen_slice(pb, ks, win, gh[ks], gl[ks]);
+            uint4 gh, gl;
+            gen_slice(1 - C, ks, win[C], gh, gl);
+            // s_g[1 - C] is not read in this stage: store the generated slice at once
+            s_g[1 - C][0][wave][ks][lane] = gh;
+            s_g[1 - C][1][wave][ks][lane] = gl;
 #pragma unroll
             for (int s = 0; s < I::kSlicesPerWave; ++s) {
                 const int sl = I::slice(wave, s);
-                const v8h ah = __builtin_bit_cast(v8h, s_g[sg][0][sl][ks][lane]);
-                const v8h al = __builtin_bit_cast(v8h, s_g[sg][1][sl][ks][lane]);
+                const v8h ah = __builtin_bit_cast(v8h, s_g[C][0][sl][ks][lane]);
+                const v8h al = __builtin_bit_cast(v8h, s_g[C][1][sl][ks][lane]);
 #pragma unroll
                 for (int q = 0; q < I::kCbPerWave; ++q) {
                     v4f &c = acc[q * I::kSlicesPerWave + s];
-                    const v8h bh = __builtin_bit_cast(v8h, bh_cur[q][ks]);
-                    const v8h bl = __builtin_bit_cast(v8h, bl_cur[q][ks]);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
+                    const v8h vbh = __builtin_bit_cast(v8h, bh[C][q][ks]);
+                    const v8h vbl = __builtin_bit_cast(v8h, bl[C][q][ks]);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, vbh, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, vbl, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, vbh, c, 0, 0, 0);
                 }
             }
         }
@@ -599,25 +662,15 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
         }
-        const int sn = sg == 2 ? 0 : sg + 1;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            s_g[sn][0][wave][ks][lane] = gh[ks];
-            s_g[sn][1][wave][ks][lane] = gl[ks];
-        }
-        if (j2 < walk.jend) load_pos(j2, pb ^ 1);
+        if (j2 < walk.jend) load_pos(j2, C);
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < I::kCbPerWave; ++q)
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                bh_cur[q][ks] = bh_nxt[q][ks];
-                bl_cur[q][ks] = bl_nxt[q][ks];
-            }
-        sg = sn;
-        pb ^= 1;
         jc = j1;
         j1 = j2;
+    };
+    while (jc < walk.jend) {
+        step(std::integral_constant<int, 0>{});
+        if (!(jc < walk.jend)) break;
+        step(std::integral_constant<int, 1>{});
     }
     const double out_scale = kInv4Pi / (double)gscale;
 #pragma unroll
@@ -718,7 +771,10 @@ size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K) {
 }
 
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap, void *scratch, float gscale, bool exclude_near) {
+                   int64_t part_cap, void *scratch, float gscale, bool exclude_near, hipEvent_t ev0, hipEvent_t ev1) {
+    auto mark = [&](hipEvent_t e) {
+        if (e) (void)hipEventRecord(e, st);
+    };
     const int64_t N = g.N;
     const int64_t bx = (N + kTargets - 1) / kTargets;
     const int ncb = ncb_for(K);
@@ -733,11 +789,13 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         const int64_t ntile = (N + kS64 - 1) / kS64;
         scale_tiles_f64<<<dim3((unsigned)((ntile * kS64 + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K,
                                                                                                      Kp, Zt);
+        mark(ev0);
         switch (ncb) {
 #define DBIE_F64(n) case n: far_mfma_f64_kernel<n><<<grid, kThreads, 0, st>>>(g, K, Kp, Zt, part, (int)per); break;
             DBIE_F64(1) DBIE_F64(2) DBIE_F64(4) DBIE_F64(8) DBIE_F64(16)
 #undef DBIE_F64
         }
+        mark(ev1);
         return S;
     }
     const int64_t ntile = (N + kSb - 1) / kSb;
@@ -746,8 +804,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         _Float16 *zh = static_cast<_Float16 *>(scratch);
         _Float16 *zl = zh + zelems;
         unsigned *cmax = reinterpret_cast<unsigned *>(zl + zelems);
-        zero_u32_kernel<<<(Kp + 255) / 256, 256, 0, st>>>(cmax, Kp);
-        colmax_kernel<<<dim3(64, (unsigned)K), 256, 0, st>>>(X, ldx, g.w, N, cmax);
+        colmax_kernel<<<(unsigned)K, 256, 0, st>>>(X, ldx, g.w, N, cmax);
         split_tiles_f16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
                                                                                                     cmax, zh, zl);
         NearMask nm;
@@ -755,6 +812,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         nm.npp = g.npp;
         nm.mwords = g.mask_words;
         const int ex = (!exclude_near || !g.near_mask) ? 0 : (g.npp == 16 ? 1 : 2);
+        mark(ev0);
         switch (ncb * 4 + ex) {
 #define DBIE_H3(n, e) \
     case n * 4 + e: far_mfma_f16x3_kernel<n, e><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm); break;
@@ -763,6 +821,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
 #undef DBIE_H3E
 #undef DBIE_H3
         }
+        mark(ev1);
         return S;
     }
     const bool x3 = prec == 3;
@@ -770,6 +829,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
     __bf16 *zl = x3 ? zh + zelems : nullptr;
     split_tiles_bf16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
                                                                                                  zh, zl);
+    mark(ev0);
     if (x3) {
         switch (ncb) {
 #define DBIE_B3(n) case n: far_mfma_bf16_kernel<n, true><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, zl, part, (int)per); break;
@@ -783,6 +843,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
 #undef DBIE_B1
         }
     }
+    mark(ev1);
     return S;
 }
 

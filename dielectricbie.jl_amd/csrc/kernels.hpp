@@ -52,8 +52,10 @@ int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int6
 // pre-scaled RHS block w.*X in FP64, or its tile-major bf16 split).
 // gscale: power-of-two scale of the kernel entries for the fp16 form (prec 4).  exclude_near
 // (prec 4 only): leave out the panel pairs of g.near_mask, whose full interaction the caller adds.
+// ev0/ev1 (optional) are recorded immediately before and after the contraction kernel.
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap, void *scratch, float gscale, bool exclude_near);
+                   int64_t part_cap, void *scratch, float gscale, bool exclude_near, hipEvent_t ev0 = nullptr,
+                   hipEvent_t ev1 = nullptr);
 size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K);
 // Power-of-two scale for the fp16 kernel entries: 2^(14 - e) with max |G_ij| < 2^e over all
 // node pairs.  tmp4: 4 bytes of device memory.  Synchronises the stream; 0 on a HIP error.

@@ -125,6 +125,7 @@ SIGNATURES = {
     "dbie_device_free": [_vp],
     "dbie_memcpy_h2d": [_vp, _vp, C.c_size_t],
     "dbie_memcpy_d2h": [_vp, _vp, C.c_size_t],
+    "dbie_copy_block_device": [C.c_int64, C.c_int, _vp, C.c_int64, _vp, C.c_int64],
     "dbie_version": [],
     "dbie_tmap_create": [_vp, C.c_int64, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)],
     "dbie_tmap_destroy": [_vp],

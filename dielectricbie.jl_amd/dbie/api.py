@@ -399,6 +399,13 @@ class DeviceMatrix:
         L.check(L.lib().dbie_memcpy_h2d(d.ptr, A.ctypes.data_as(C.c_void_p), d.nbytes))
         return d
 
+    def copy_from(self, other: "DeviceMatrix"):
+        """Device-to-device copy (a copy kernel) of an equally shaped block."""
+        if other.shape != self.shape:
+            raise ValueError("DimensionMismatch: shapes differ")
+        L.check(L.lib().dbie_copy_block_device(self.shape[0], self.shape[1], other.ptr, self.shape[0], self.ptr,
+                                               self.shape[0]))
+
     def to_host(self) -> np.ndarray:
         out = np.zeros(self.shape, order="F")
         L.check(L.lib().dbie_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))

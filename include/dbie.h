@@ -305,6 +305,8 @@ dbie_status dbie_device_alloc(size_t bytes, void **dptr);
 dbie_status dbie_device_free(void *dptr);
 dbie_status dbie_memcpy_h2d(void *dst, const void *src, size_t bytes);
 dbie_status dbie_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy of an N x K column-major block by a kernel (synchronous). */
+dbie_status dbie_copy_block_device(int64_t N, int K, const double *dsrc, int64_t lds, double *ddst, int64_t ldd);
 
 #ifdef __cplusplus
 }

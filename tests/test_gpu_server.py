@@ -135,6 +135,7 @@ def test_serving_stream_small(dbie):
             pts = r["points"] * 0.5
             ids.append(srv.submit_box(*L, r["n_quad"], r["l_ec"], r["eps_in"], r["eps_out"], pts, r["charges"]))
         infos = [srv.wait(i) for i in ids]
-        assert all(i["status"] == 0 for i in infos), infos
+        bad = [(r["L"], r["l_ec"], len(r["charges"]), i) for r, i in zip(reqs, infos) if i["status"] != 0]
+        assert not bad, bad
         lat = [i["total_ms"] for i in infos]
         say(f"  p50 {np.percentile(lat, 50):.1f} ms  p99 {np.percentile(lat, 99):.1f} ms  stats {srv.stats()}")
