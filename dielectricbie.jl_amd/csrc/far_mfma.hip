@@ -646,15 +646,20 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
                 const int sl = I::slice(wave, s);
                 const v8h ah = __builtin_bit_cast(v8h, s_g[C][0][sl][ks][lane]);
                 const v8h al = __builtin_bit_cast(v8h, s_g[C][1][sl][ks][lane]);
+                // the three products of one accumulator are issued kCbPerWave apart, so that
+                // consecutive MFMAs never depend on each other
 #pragma unroll
-                for (int q = 0; q < I::kCbPerWave; ++q) {
-                    v4f &c = acc[q * I::kSlicesPerWave + s];
-                    const v8h vbh = __builtin_bit_cast(v8h, bh[C][q][ks]);
-                    const v8h vbl = __builtin_bit_cast(v8h, bl[C][q][ks]);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, vbh, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, vbl, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, vbh, c, 0, 0, 0);
-                }
+                for (int q = 0; q < I::kCbPerWave; ++q)
+                    acc[q * I::kSlicesPerWave + s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        ah, __builtin_bit_cast(v8h, bh[C][q][ks]), acc[q * I::kSlicesPerWave + s], 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < I::kCbPerWave; ++q)
+                    acc[q * I::kSlicesPerWave + s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        ah, __builtin_bit_cast(v8h, bl[C][q][ks]), acc[q * I::kSlicesPerWave + s], 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < I::kCbPerWave; ++q)
+                    acc[q * I::kSlicesPerWave + s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        al, __builtin_bit_cast(v8h, bh[C][q][ks]), acc[q * I::kSlicesPerWave + s], 0, 0, 0);
             }
         }
 #pragma unroll
