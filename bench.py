@@ -220,7 +220,7 @@ def main():
     pts, q = charges(a.K, a.seed)
     F = dbie.rhs_point_charges(iface, pts, q, EPS_IN)
     N = iface.n_points
-    log(f"setup N={N} blocks={op.n_blocks} {t_setup:.2f}s")
+    log(f"setup N={N} blocks={op.n_blocks} {t_setup:.2f}s {op.setup_ms()}")
     if a.mode == "apply":
         X = np.random.default_rng(2).standard_normal((N, a.K))
         res = {}
@@ -319,7 +319,7 @@ def main():
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_rel_err_vs_fp64": float(err.max()), "median_rel_err_vs_fp64": float(np.median(err)),
         "mixed_inner_iters": st["niter"], "mixed_refinements": st["nrefine"],
-        "mixed_max_rel_residual": st["max_rel_residual"], "setup_s": t_setup,
+        "mixed_max_rel_residual": st["max_rel_residual"], "setup_s": t_setup, "setup_ms": op.setup_ms(),
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

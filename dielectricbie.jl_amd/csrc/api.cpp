@@ -80,8 +80,9 @@ struct dbie_op {
     DevBSR bsr;
     int adaptive_hits = 0;
     // workspaces (grown on demand)
-    DBuf part, xin, yout, tmp, mscratch, gmax;
+    DBuf part, xin, yout, tmp, mscratch, gmax, xt32;
     GmresWork gw;                 // GMRES workspaces, reused across solves
+    double setup_ms[4] = {0, 0, 0, 0};  // near pairs, correction blocks, near-field derived data, uploads
     float f16_gscale = 0.0f;  // power-of-two scale of the fp16 kernel entries (0 = not yet computed)
     // optional per-kernel timing: event pairs recorded around launches
     bool profiling = false;
@@ -204,7 +205,9 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
     {
         ProfScope ps(op, 2);
         const NearVals nv = exclude ? NEAR_UP32 : (prec != DBIE_FP64 ? NEAR_C32 : NEAR_C64);
-        near_combine(op->stream, op->bsr, nv, S, op->part.as<double>(), N, K, X, ldx, op->diag.as<double>(), Y, ldy);
+        if (nv == NEAR_UP32) op->xt32.ensure(near_xt_bytes(N, K));
+        near_combine(op->stream, op->bsr, nv, S, op->part.as<double>(), N, K, X, ldx, op->diag.as<double>(), Y, ldy,
+                     nv == NEAR_UP32 ? op->xt32.as<float>() : nullptr);
     }
     DBIE_HIP(hipGetLastError());
 }
@@ -686,8 +689,15 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         // near-field corrections
         BlockSparse bs;
         bs.bd = g.npp();
+        using Clk = std::chrono::steady_clock;
+        auto ms = [](Clk::time_point a, Clk::time_point b) {
+            return std::chrono::duration<double, std::milli>(b - a).count();
+        };
+        const auto t0 = Clk::now();
+        auto t1 = t0, t2 = t0;
         if (o.build_corrections) {
             const PairList pl = build_pair_list(g, o.max_order, o.up_tol, o.correct_edges != 0, o.n_threads);
+            t1 = Clk::now();
             NearOpts no;
             no.up_tol = o.up_tol;
             no.max_order = o.max_order;
@@ -698,9 +708,12 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
             no.adaptive_max_depth = o.adaptive_max_depth;
             no.n_threads = o.n_threads;
             bs = build_corrections(g, pl, no);
+            t2 = Clk::now();
         } else {
             bs.row_ptr.assign(g.P + 1, 0);
         }
+        op->setup_ms[0] = ms(t0, t1);
+        op->setup_ms[1] = ms(t1, t2);
         op->adaptive_hits = bs.adaptive_hits;
         // full near interaction per block (correction + direct Dt term) for operators whose far
         // field leaves the near panel pairs out, and the panel-pair mask they use
@@ -733,6 +746,8 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
             for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b)
                 mask[(size_t)p * mwords + bs.col[b] / 32] |= 1u << (bs.col[b] % 32);
         const bool mfma_order = bs.bd == 16;
+        const auto t3 = Clk::now();
+        op->setup_ms[2] = ms(t2 > t0 ? t2 : t0, t3);
         if (mfma_order) {  // permute each block into MFMA operand order
             std::vector<double> tmpb(256);
             for (int64_t b = 0; b < bs.nblocks(); ++b) {
@@ -742,7 +757,7 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
                 std::copy(tmpb.begin(), tmpb.end(), blk);
                 double *bu = &vup[(size_t)b * 256];
                 for (int t = 0; t < 16; ++t)
-                    for (int c = 0; c < 16; ++c) tmpb[bsr_storage_index(16, true, t, c)] = bu[t * 16 + c];
+                    for (int c = 0; c < 16; ++c) tmpb[bsr_lane_major_index(t, c)] = bu[t * 16 + c];
                 std::copy(tmpb.begin(), tmpb.end(), bu);
             }
         }
@@ -781,6 +796,7 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         c.valup32 = bs.nblocks() > 0 ? op->bsr_vup32.as<float>() : nullptr;
         c.mfma_order = mfma_order;
         DBIE_HIP(hipStreamSynchronize(st));
+        op->setup_ms[3] = ms(t3, Clk::now());
         *out = op.release();
     });
 }
@@ -829,6 +845,14 @@ dbie_status dbie_op_corrections(const dbie_op *op, int *src, int *trg, double *b
     });
 }
 
+dbie_status dbie_op_setup_ms(const dbie_op *op, double *t) {
+    return guard([&] {
+        check_op(op);
+        if (!t) fail(DBIE_BAD_ARG, "NULL output");
+        for (int i = 0; i < 4; ++i) t[i] = op->setup_ms[i];
+    });
+}
+
 dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
     return guard([&] {
         check_op(op);
@@ -837,7 +861,7 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
         for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
                               &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
                               &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->nmask, &op->part,
-                              &op->xin, &op->yout, &op->tmp, &op->mscratch})
+                              &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32})
             b += (int64_t)d->bytes;
         *bytes = b;
     });

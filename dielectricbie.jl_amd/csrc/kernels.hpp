@@ -65,8 +65,12 @@ int64_t far_field_part_elems(int64_t N, int K);  // workspace needed for any far
 
 // Y[:,k] = sum_s part[s][k] + dscale*diag.*X[:,k] + C X[:,k] (C in FP64 or FP32 values).
 enum NearVals { NEAR_C64 = 0, NEAR_C32 = 1, NEAR_UP32 = 2 };
+// NEAR_UP32 with 16 x 16 blocks and K >= 16 uses xt, near_xt_bytes(N, K) bytes of scratch for a
+// tile-major FP32 copy of X; the full near blocks are then stored lane-major (bsr_lane_major_index).
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
-                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy);
+                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt = nullptr);
+size_t near_xt_bytes(int64_t N, int K);
+int64_t bsr_lane_major_index(int t, int c);
 
 // Right-hand sides.
 void rhs_points(hipStream_t st, const DevGeom &g, int K, const double *src, const double *q, double eps_src,

@@ -26,8 +26,12 @@ namespace dbie {
 __host__ __device__ inline int blk_idx(int bd, bool mfma_order, int t, int c) {
     return mfma_order ? (c >> 2) * 64 + (c & 3) * 16 + t : t * bd + c;
 }
+// Lane-major order of a 16 x 16 block (full near blocks of the split-FP16 operator): lane
+// (c/4)*16 + t holds columns 4(c/4) .. 4(c/4)+3 contiguously, one 16-byte load per lane.
+__host__ __device__ inline int lane_major_idx(int t, int c) { return ((c >> 2) * 16 + t) * 4 + (c & 3); }
 
 int64_t bsr_storage_index(int bd, bool mfma_order, int t, int c) { return blk_idx(bd, mfma_order, t, c); }
+int64_t bsr_lane_major_index(int t, int c) { return lane_major_idx(t, c); }
 
 namespace {
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -57,7 +61,8 @@ __global__ __launch_bounds__(256) void near_combine_kernel(int P, int bd, int mf
     for (int b = row_ptr[p]; b < b1; ++b) {
         const VT *v = val + (int64_t)b * bd * bd;
         const double *xs = xk + (int64_t)col[b] * bd;
-        for (int c = 0; c < bd; ++c) corr += (double)v[blk_idx(bd, mfma_order != 0, row, c)] * xs[c];
+        for (int c = 0; c < bd; ++c)
+            corr += (double)v[mfma_order == 2 ? lane_major_idx(row, c) : blk_idx(bd, mfma_order != 0, row, c)] * xs[c];
     }
     Y[i + (int64_t)k * ldy] = acc + corr;
 }
@@ -141,28 +146,105 @@ __global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__rest
         Y[i + (int64_t)c * ldy] = y;
     }
 }
+
+// X (N x K, FP64) -> tile-major FP32 copy Xt[((j / 16) * Kpad + k) * 16 + j % 16], zero beyond K
+__global__ void x_tiles_f32_kernel(const double *__restrict__ X, int64_t ldx, int64_t N, int K, int Kpad,
+                                   float *__restrict__ Xt) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    if (j >= N) return;
+    Xt[((j >> 4) * Kpad + k) * 16 + (j & 15)] = k < K ? (float)X[j + (int64_t)k * ldx] : 0.0f;
+}
+
+// Full near blocks (FP32, lane-major) times the tile-major FP32 copy of X on the FP64 MFMA: per
+// block one 16-byte load of A and one of B per column block and lane.  MFMA kk contracts source
+// columns c = 4 (lane / 16) + kk, a permutation of the block's 16 columns.
+__global__ __launch_bounds__(256) void near_up32_kernel(int P, const int *__restrict__ row_ptr,
+                                                        const int *__restrict__ col, const float *__restrict__ val,
+                                                        int S, const double *__restrict__ part, int64_t N, int K,
+                                                        const float *__restrict__ Xt, int Kpad,
+                                                        const double *__restrict__ X, int64_t ldx,
+                                                        const double *__restrict__ diag, double *__restrict__ Y,
+                                                        int64_t ldy) {
+    __shared__ double s_out[256][17];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int p = xcd_row(blockIdx.x, P);
+    const int k0 = blockIdx.y * 256;
+    if (p >= P) return;  // uniform per workgroup
+    v4d acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
+    float4 an, xn[4];
+    auto load = [&](int b, float4 &a, float4 (&xv)[4]) {
+        a = *reinterpret_cast<const float4 *>(val + (int64_t)b * 256 + lane * 4);
+        const float *xs = Xt + (int64_t)col[b] * Kpad * 16 + 4 * (lane >> 4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            xv[q] = *reinterpret_cast<const float4 *>(xs + (int64_t)(k0 + (wave + 4 * q) * 16 + (lane & 15)) * 16);
+    };
+    if (b0 < b1) load(b0, an, xn);
+    for (int b = b0; b < b1; ++b) {
+        const float4 a = an;
+        float4 xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xv[q] = xn[q];
+        if (b + 1 < b1) load(b + 1, an, xn);
+        const double av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double bv = kk == 0 ? xv[q].x : kk == 1 ? xv[q].y : kk == 2 ? xv[q].z : xv[q].w;
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv, acc[q], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s_out[(wave + 4 * q) * 16 + (lane & 15)][4 * r + (lane >> 4)] = acc[q][r];
+    __syncthreads();
+    const int row = tid & 15;
+    const int64_t i = (int64_t)p * 16 + row;
+    const double t = diag[i];
+    for (int m = 0; m < 16; ++m) {
+        const int cl = (tid >> 4) + 16 * m;
+        const int c = k0 + cl;
+        if (c >= K) continue;
+        double y = s_out[cl][row] + t * X[i + (int64_t)c * ldx];
+        for (int s = 0; s < S; ++s) y += part[((int64_t)s * K + c) * N + i];
+        Y[i + (int64_t)c * ldy] = y;
+    }
+}
 }  // namespace
 
+size_t near_xt_bytes(int64_t N, int K) { return sizeof(float) * (size_t)N * (size_t)((K + 255) / 256 * 256); }
+
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
-                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy) {
+                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt) {
     const int bd = C.bd;
     const float *v32 = vals == NEAR_UP32 ? C.valup32 : (vals == NEAR_C32 ? C.val32 : nullptr);
+    if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 16) {  // xt: near_xt_bytes(N, K)
+        const int Kpad = (K + 255) / 256 * 256;
+        x_tiles_f32_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)Kpad), 256, 0, st>>>(X, ldx, N, K, Kpad, xt);
+        const int per = (C.P + 7) / 8;
+        dim3 grid((unsigned)(8 * per), (unsigned)(Kpad / 256));
+        near_up32_kernel<<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.valup32, S, part, N, K, xt, Kpad, X, ldx, diag,
+                                               Y, ldy);
+        return;
+    }
     if (C.mfma_order && bd == 16 && K >= 16) {
         const int per = (C.P + 7) / 8;
         dim3 grid((unsigned)(8 * per), (unsigned)((K + 255) / 256));
-        if (vals == NEAR_UP32)
-            near_mfma_kernel<float><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.valup32, S, part, N, K, X, ldx, diag,
-                                                          Y, ldy);
-        else
-            near_mfma_kernel<double><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.val64, S, part, N, K, X, ldx, diag,
-                                                           Y, ldy);
+        near_mfma_kernel<double><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.val64, S, part, N, K, X, ldx, diag, Y,
+                                                       ldy);
         return;
     }
     const int KT = std::max(1, 256 / bd);
     const int KTe = std::min(K, KT);
     const int RP = std::max(1, KT / KTe);
     dim3 grid((unsigned)((C.P + RP - 1) / RP), (unsigned)((K + KTe - 1) / KTe));
-    const int mo = C.mfma_order ? 1 : 0;
+    const int mo = (vals == NEAR_UP32 && C.mfma_order) ? 2 : (C.mfma_order ? 1 : 0);
     if (v32)
         near_combine_kernel<float><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, v32, S, part, N, K, KTe, RP,
                                                           X, ldx, diag, Y, ldy);

@@ -272,6 +272,12 @@ class BatchedDielectricOperator:
         L.check(L.lib().dbie_op_device_bytes(self._h, C.byref(b)))
         return b.value
 
+    def setup_ms(self) -> dict:
+        """Host time of the operator build: near pairs, correction blocks, derived data, uploads (ms)."""
+        t = np.zeros(4)
+        L.check(L.lib().dbie_op_setup_ms(self._h, L.dptr(t)))
+        return dict(zip(("pairs", "corrections", "derived", "upload"), map(float, t)))
+
     def profile(self, enable=True):
         """Record HIP events around the far-field and combine launches (kernel timing)."""
         L.check(L.lib().dbie_op_profile(self._h, int(bool(enable))))

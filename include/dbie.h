@@ -144,6 +144,10 @@ dbie_status dbie_operator_create(const dbie_geom *g, const double *eps_in, const
                                  const dbie_near_opts *opts, dbie_op **out);
 void dbie_operator_destroy(dbie_op *op);
 dbie_status dbie_op_sizes(const dbie_op *op, int64_t *N, int64_t *n_blocks, int *block_dim);
+/* Host time of the operator build in ms (no counterpart in the reference): t[0] near-pair list,
+   t[1] correction blocks, t[2] derived near-field data (full near blocks, pair mask), t[3] layout
+   permutation + uploads. */
+dbie_status dbie_op_setup_ms(const dbie_op *op, double *t);
 /* Update eps (diag only; geometry and corrections depend on geometry alone). */
 dbie_status dbie_op_set_eps(dbie_op *op, const double *eps_in, const double *eps_out);
 /* Export the correction blocks: pairs (src, trg) and blocks[b][t + c*bd] (FP64). */
