@@ -82,7 +82,7 @@ struct dbie_op {
     // workspaces (grown on demand)
     DBuf part, xin, yout, tmp, mscratch, gmax, xt32;
     GmresWork gw;                 // GMRES workspaces, reused across solves
-    double setup_ms[4] = {0, 0, 0, 0};  // near pairs, correction blocks, near-field derived data, uploads
+    double setup_ms[4] = {0, 0, 0, 0};  // near pairs, correction blocks (host part), derived data, uploads + device block build
     float f16_gscale = 0.0f;  // power-of-two scale of the fp16 kernel entries (0 = not yet computed)
     // optional per-kernel timing: event pairs recorded around launches
     bool profiling = false;
@@ -100,6 +100,9 @@ struct dbie_op {
 
 namespace dbie {
 namespace {
+
+// quadtree nodes one device walk of a touching pair may visit before that pair goes to the host
+constexpr int kQuadtreeNodeBudget = 1 << 16;
 
 void check_op(const dbie_op *op) {
     if (!op) fail(DBIE_BAD_ARG, "operator handle is NULL");
@@ -536,6 +539,7 @@ void dbie_near_opts_default(dbie_near_opts *o) {
     o->adaptive_max_depth = 20;
     o->build_corrections = 1;
     o->n_threads = 0;
+    o->device_build = 1;
 }
 
 void dbie_solve_opts_default(dbie_solve_opts *o) {
@@ -695,83 +699,162 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         };
         const auto t0 = Clk::now();
         auto t1 = t0, t2 = t0;
+        const int npp = g.npp(), bd = bs.bd;
+        const bool mfma_order = bd == 16;
+        NearOpts no;
+        no.up_tol = o.up_tol;
+        no.max_order = o.max_order;
+        no.correct_edges = o.correct_edges != 0;
+        no.adaptive_atol = o.adaptive_atol;
+        no.adaptive_rtol = o.adaptive_rtol;
+        no.adaptive_n_GL = o.adaptive_n_GL;
+        no.adaptive_max_depth = o.adaptive_max_depth;
+        no.n_threads = o.n_threads;
+        // the device builder keeps a block's kernel values in LDS
+        const bool on_device = o.build_corrections && o.device_build &&
+                               (double)bd * o.max_order * o.max_order * sizeof(double) <= (double)kNearBuildLds;
+        bool device_built = false;
         if (o.build_corrections) {
             const PairList pl = build_pair_list(g, o.max_order, o.up_tol, o.correct_edges != 0, o.n_threads);
             t1 = Clk::now();
-            NearOpts no;
-            no.up_tol = o.up_tol;
-            no.max_order = o.max_order;
-            no.correct_edges = o.correct_edges != 0;
-            no.adaptive_atol = o.adaptive_atol;
-            no.adaptive_rtol = o.adaptive_rtol;
-            no.adaptive_n_GL = o.adaptive_n_GL;
-            no.adaptive_max_depth = o.adaptive_max_depth;
-            no.n_threads = o.n_threads;
-            bs = build_corrections(g, pl, no);
-            t2 = Clk::now();
+            if (on_device) {
+                // host: pair order, source caches, touching-pair quadtrees; device: every block
+                const int nGL = o.adaptive_n_GL == 0 ? g.nq : o.adaptive_n_GL;
+                const bool dev_quad = adaptive_blocks_lds(g.nq, nGL, o.adaptive_max_depth) <= kNearBuildLds &&
+                                      o.adaptive_max_depth >= 0;
+                const CorrectionPlan cp = plan_corrections(g, pl, no, !dev_quad);
+                t2 = Clk::now();
+                bs.row_ptr = cp.row_ptr;
+                bs.col = cp.col;
+                bs.adaptive_hits = cp.adaptive_hits;
+                const size_t nv = (size_t)cp.nblocks() * bd * bd;
+                op->bsr_v64.ensure(sizeof(double) * std::max<size_t>(nv, 1));
+                op->bsr_v32.ensure(sizeof(float) * std::max<size_t>(nv, 1));
+                op->bsr_vup32.ensure(sizeof(float) * std::max<size_t>(nv, 1));
+                DBuf us, ut, ul, co, cn, ca, as, at, al, ak;
+                upload(us, cp.up_src.data(), cp.up_src.size(), st);
+                upload(ut, cp.up_trg.data(), cp.up_trg.size(), st);
+                upload(ul, cp.up_slot.data(), cp.up_slot.size(), st);
+                upload(co, cp.cache_off.data(), cp.cache_off.size(), st);
+                upload(cn, cp.cache_nu.data(), cp.cache_nu.size(), st);
+                upload(ca, cp.caches.data(), cp.caches.size(), st);
+                upload(as, cp.adp_src.data(), cp.adp_src.size(), st);
+                upload(at, cp.adp_trg.data(), cp.adp_trg.size(), st);
+                upload(al, cp.adp_slot.data(), cp.adp_slot.size(), st);
+                int max_nu = 0;
+                for (int v : cp.cache_nu) max_nu = std::max(max_nu, v);
+                DevGeom dgb;
+                dgb.N = N;
+                dgb.x = op->x.as<double>(); dgb.y = op->y.as<double>(); dgb.z = op->z.as<double>();
+                dgb.nx = op->nx.as<double>(); dgb.ny = op->ny.as<double>(); dgb.nz = op->nz.as<double>();
+                dgb.w = op->w.as<double>();
+                const int n_adp = (int)cp.adp_src.size();
+                DBuf ar, am, ru, ah;
+                std::vector<int> hits;
+                if (dev_quad) {
+                    // touching pairs: quadtrees on the device, K_up blocks into ak
+                    const std::vector<double> rules = quadtree_rules(g, nGL);
+                    upload(ar, cp.adp_rep.data(), cp.adp_rep.size(), st);
+                    upload(am, cp.adp_map.data(), cp.adp_map.size(), st);
+                    upload(ru, rules.data(), rules.size(), st);
+                    ak.ensure(sizeof(double) * std::max<size_t>((size_t)n_adp * bd * bd, 1));
+                    ah.ensure(sizeof(int) * std::max(n_adp, 1));
+                    if (!adaptive_blocks(st, dgb, g.nq, nGL, n_adp, as.as<int>(), at.as<int>(), ar.as<int>(),
+                                         am.as<double>(), ru.as<double>(), o.adaptive_atol, o.adaptive_rtol,
+                                         o.adaptive_max_depth, kQuadtreeNodeBudget, ak.as<double>(), ah.as<int>()))
+                        fail(DBIE_BAD_ARG, "near block build: quadtree too deep for LDS");
+                    DBIE_HIP(hipGetLastError());
+                    hits.resize(n_adp);
+                    if (n_adp > 0) {
+                        DBIE_HIP(hipMemcpyAsync(hits.data(), ah.p, sizeof(int) * n_adp, hipMemcpyDeviceToHost, st));
+                        DBIE_HIP(hipStreamSynchronize(st));
+                    }
+                    bs.adaptive_hits = 0;
+                    std::vector<double> blk((size_t)bd * bd);
+                    for (int k = 0; k < n_adp; ++k) {
+                        if (hits[k] == 2) {  // walk over the node budget: this block on the host
+                            hits[k] = touching_block(g, cp.adp_src[k], cp.adp_trg[k], no, blk.data()) ? 1 : 0;
+                            h2d(ak.as<double>() + (size_t)k * bd * bd, blk.data(), sizeof(double) * blk.size(), st);
+                        }
+                        bs.adaptive_hits += hits[k];
+                    }
+                } else {
+                    upload(ak, cp.adp_kup.data(), cp.adp_kup.size(), st);
+                }
+                const bool ok = build_near_blocks(st, dgb, bd, mfma_order, (int)cp.up_src.size(), us.as<int>(), ut.as<int>(),
+                                  ul.as<int>(), co.as<int>(), cn.as<int>(), ca.as<double>(), max_nu,
+                                  n_adp, as.as<int>(), at.as<int>(), al.as<int>(), ak.as<double>(),
+                                  op->bsr_v64.as<double>(), op->bsr_v32.as<float>(), op->bsr_vup32.as<float>());
+                if (!ok) fail(DBIE_BAD_ARG, "near block build: upsampling order too high for LDS");
+                DBIE_HIP(hipGetLastError());
+                DBIE_HIP(hipStreamSynchronize(st));
+                device_built = true;
+                op->setup_ms[3] = ms(t2, Clk::now());
+            } else {
+                bs = build_corrections(g, pl, no);
+                t2 = Clk::now();
+            }
         } else {
             bs.row_ptr.assign(g.P + 1, 0);
         }
         op->setup_ms[0] = ms(t0, t1);
         op->setup_ms[1] = ms(t1, t2);
         op->adaptive_hits = bs.adaptive_hits;
-        // full near interaction per block (correction + direct Dt term) for operators whose far
-        // field leaves the near panel pairs out, and the panel-pair mask they use
-        std::vector<double> vup(bs.val.size());
-        const int npp = g.npp(), bd = bs.bd;
-        parallel_for(g.P, o.n_threads, [&](int64_t p, int) {
-            for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b) {
-                const int64_t q = bs.col[b];
-                for (int t = 0; t < bd; ++t) {
-                    const int64_t i = p * npp + t;
-                    const double *n = &g.normal[3 * p];
-                    for (int c = 0; c < bd; ++c) {
-                        const int64_t j = q * npp + c;
-                        const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        double kd = 0.0;
-                        if (r2 > 0.0) {
-                            const double ir = 1.0 / std::sqrt(r2);
-                            kd = (n[0] * dx + n[1] * dy + n[2] * dz) * (ir * ir * ir) / kFourPi * g.w[j];
-                        }
-                        const size_t at = (size_t)b * bd * bd + (size_t)t * bd + c;
-                        vup[at] = bs.val[at] + kd;
-                    }
-                }
-            }
-        });
+        // panel-pair mask of the near list (the split-FP16 far field leaves these pairs out)
         const int mwords = (int)((g.P + 31) / 32) + 2;
         std::vector<uint32_t> mask((size_t)g.P * mwords, 0u);
         for (int64_t p = 0; p < g.P; ++p)
             for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b)
                 mask[(size_t)p * mwords + bs.col[b] / 32] |= 1u << (bs.col[b] % 32);
-        const bool mfma_order = bs.bd == 16;
         const auto t3 = Clk::now();
-        op->setup_ms[2] = ms(t2 > t0 ? t2 : t0, t3);
-        if (mfma_order) {  // permute each block into MFMA operand order
-            std::vector<double> tmpb(256);
-            for (int64_t b = 0; b < bs.nblocks(); ++b) {
-                double *blk = &bs.val[(size_t)b * 256];
-                for (int t = 0; t < 16; ++t)
-                    for (int c = 0; c < 16; ++c) tmpb[bsr_storage_index(16, true, t, c)] = blk[t * 16 + c];
-                std::copy(tmpb.begin(), tmpb.end(), blk);
-                double *bu = &vup[(size_t)b * 256];
-                for (int t = 0; t < 16; ++t)
-                    for (int c = 0; c < 16; ++c) tmpb[bsr_lane_major_index(t, c)] = bu[t * 16 + c];
-                std::copy(tmpb.begin(), tmpb.end(), bu);
-            }
-        }
+        op->setup_ms[2] = ms(t2 > t0 ? t2 : t0, t3) - op->setup_ms[3];
         upload(op->bsr_row, bs.row_ptr.data(), bs.row_ptr.size(), st);
         upload(op->bsr_col, bs.col.data(), bs.col.size(), st);
-        upload(op->bsr_v64, bs.val.data(), bs.val.size(), st);
-        {
+        upload(op->nmask, mask.data(), mask.size(), st);
+        if (!device_built) {
+            // host-built blocks: full near interaction (correction + direct Dt term), storage orders
+            std::vector<double> vup(bs.val.size());
+            parallel_for(g.P, o.n_threads, [&](int64_t p, int) {
+                for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b) {
+                    const int64_t q = bs.col[b];
+                    for (int t = 0; t < bd; ++t) {
+                        const int64_t i = p * npp + t;
+                        const double *n = &g.normal[3 * p];
+                        for (int c = 0; c < bd; ++c) {
+                            const int64_t j = q * npp + c;
+                            const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            double kd = 0.0;
+                            if (r2 > 0.0) {
+                                const double ir = 1.0 / std::sqrt(r2);
+                                kd = (n[0] * dx + n[1] * dy + n[2] * dz) * (ir * ir * ir) / kFourPi * g.w[j];
+                            }
+                            const size_t at = (size_t)b * bd * bd + (size_t)t * bd + c;
+                            vup[at] = bs.val[at] + kd;
+                        }
+                    }
+                }
+            });
+            if (mfma_order) {  // permute each block into the kernels' operand orders
+                parallel_for(bs.nblocks(), o.n_threads, [&](int64_t b, int) {
+                    double tmpb[256];
+                    double *blk = &bs.val[(size_t)b * 256];
+                    for (int t = 0; t < 16; ++t)
+                        for (int c = 0; c < 16; ++c) tmpb[bsr_storage_index(16, true, t, c)] = blk[t * 16 + c];
+                    std::copy(tmpb, tmpb + 256, blk);
+                    double *bu = &vup[(size_t)b * 256];
+                    for (int t = 0; t < 16; ++t)
+                        for (int c = 0; c < 16; ++c) tmpb[bsr_lane_major_index(t, c)] = bu[t * 16 + c];
+                    std::copy(tmpb, tmpb + 256, bu);
+                });
+            }
+            upload(op->bsr_v64, bs.val.data(), bs.val.size(), st);
             const std::vector<float> v32 = f32(bs.val);
             upload(op->bsr_v32, v32.data(), v32.size(), st);
             const std::vector<float> u32 = f32(vup);
             upload(op->bsr_vup32, u32.data(), u32.size(), st);
-            upload(op->nmask, mask.data(), mask.size(), st);
-            DBIE_HIP(hipStreamSynchronize(st));
         }
+        DBIE_HIP(hipStreamSynchronize(st));
         compute_diag(op.get(), eps_in, eps_out);
         DevGeom &d = op->dg;
         d.N = N;
@@ -796,7 +879,7 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         c.valup32 = bs.nblocks() > 0 ? op->bsr_vup32.as<float>() : nullptr;
         c.mfma_order = mfma_order;
         DBIE_HIP(hipStreamSynchronize(st));
-        op->setup_ms[3] = ms(t3, Clk::now());
+        op->setup_ms[3] += ms(t3, Clk::now());
         *out = op.release();
     });
 }

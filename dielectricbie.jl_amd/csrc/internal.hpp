@@ -98,6 +98,32 @@ struct BlockSparse {
 
 BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearOpts &o);
 
+// Host part of the device correction build: BSR structure, the pair items with their BSR slots,
+// per-source-panel caches (upsampled nodes, interpolation-quadrature matrix Mt) for the upsampled
+// pairs, and the adaptive (touching-pair) K_up blocks, row-major, without the direct term.
+struct CorrectionPlan {
+    int bd = 0;
+    std::vector<int> row_ptr, col;
+    std::vector<int> up_src, up_trg, up_slot;
+    std::vector<int> cache_off, cache_nu;   // per source panel (-1 / 0: no upsampled pairs)
+    std::vector<double> caches;             // per panel: upos (3 n_up^2), then Mt (bd x n_up^2)
+    std::vector<int> adp_src, adp_trg, adp_slot;
+    std::vector<double> adp_kup;            // per touching pair: bd x bd (host quadtrees only)
+    std::vector<int> adp_rep;               // per touching pair: representative target node in its panel
+    std::vector<double> adp_map;            // per touching pair: source panel map cc[3], bma[3], dma[3], scale
+    int adaptive_hits = 0;
+    int64_t nblocks() const { return (int64_t)col.size(); }
+};
+// host_quadtrees = false leaves the touching pairs' quadtrees to the device (adp_rep, adp_map filled,
+// adp_kup empty).
+CorrectionPlan plan_corrections(const Geometry &g, const PairList &pl, const NearOpts &o, bool host_quadtrees);
+// Gauss-Legendre rule and barycentric data the device quadtree walk needs: glx[nq], bary[nq],
+// ns[nGL], ws[nGL] (nGL = adaptive_n_GL, or nq when 0).
+std::vector<double> quadtree_rules(const Geometry &g, int nGL);
+// K_up block of one touching pair on the host (source panel i, target panel j; bd x bd row-major,
+// no direct term); returns the max-depth flag.
+bool touching_block(const Geometry &g, int64_t i, int64_t j, const NearOpts &o, double *blk);
+
 // ---- off-surface targets ------------------------------------------------------
 // laplace3d_pottrg_fmm3d_corrected_hcubature (src/kernel/laplace3d_near_hcubature.jl:337-375):
 // the interface refined around the targets, the prolongation of densities onto it, and the

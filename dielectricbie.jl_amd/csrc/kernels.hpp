@@ -120,4 +120,24 @@ void reaction_q(hipStream_t st, const DevGeom &g, int G, const int64_t *grp_ptr,
 // Y[idx[e]] += val[e] for n entries with distinct idx.
 void scatter_add(hipStream_t st, int64_t n, const int64_t *idx, const double *val, double *Y);
 
+// ---- near-field correction blocks on the device (nearbuild.hip) ----
+// Upsampled pairs from per-source-panel caches (upos then Mt, at cache_off, order cache_nu) and
+// touching pairs from host K_up blocks (bd x bd row-major): writes the correction (FP64, FP32) and
+// the full near interaction (FP32) of each pair's BSR slot.  max_nu: largest cache order (LDS size).
+// Returns false (nothing launched) when bd x max_nu^2 doubles exceed kNearBuildLds.
+constexpr size_t kNearBuildLds = 64 * 1024;
+// Touching pairs on the device: K_up blocks (n_adp x bd x bd, row-major) and max-depth flags of the
+// quadtree rule, one wave per pair.  adp_map: 10 doubles per pair (cc, bma, dma, scale of the source
+// panel); rules: glx[nq], bary[nq], ns[nGL], ws[nGL].  Returns false (nothing launched) when the
+// walk's LDS, adaptive_blocks_lds(), exceeds kNearBuildLds.  hit[k]: 1 = a cell reached max_depth
+// unconverged, 2 = the walk stopped after max_nodes nodes (block incomplete; build it on the host).
+size_t adaptive_blocks_lds(int nq, int nGL, int max_depth);
+bool adaptive_blocks(hipStream_t st, const DevGeom &g, int nq, int nGL, int n_adp, const int *adp_src,
+                     const int *adp_trg, const int *adp_rep, const double *adp_map, const double *rules, double atol,
+                     double rtol, int max_depth, int max_nodes, double *kup, int *hit);
+bool build_near_blocks(hipStream_t st, const DevGeom &g, int bd, bool mfma_order, int n_up, const int *up_src,
+                       const int *up_trg, const int *up_slot, const int *cache_off, const int *cache_nu,
+                       const double *caches, int max_nu, int n_adp, const int *adp_src, const int *adp_trg,
+                       const int *adp_slot, const double *adp_kup, double *val64, float *val32, float *vup32);
+
 }  // namespace dbie

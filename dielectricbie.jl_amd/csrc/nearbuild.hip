@@ -1,0 +1,304 @@
+// nearbuild.hip -- correction blocks of the upsampled near pairs built on the device.
+//
+// For a near pair (source panel i, target panel j) with upsampling order n_up
+// (_laplace3d_corrections, src/kernel/laplace3d_near_upsampling.jl:127-276):
+//     K_up[t, c] = sum_a Mt_i[c, a] * G(u_a, x_t, n_j)        (u_a: n_up^2 upsampled nodes of panel i)
+//     C[t, c]    = K_up[t, c] - w_c G(y_c, x_t, n_j)           (direct quadrature term)
+// with G = laplace3d_grad (src/kernel/laplace3d.jl:9-15).  Mt_i and u_a come from the host
+// (per-source-panel caches); one workgroup builds one block: the kernel values for all
+// (target, upsampled node) pairs go to LDS, then each thread forms its (t, c) sums in the
+// order of the host builder.  Outputs: the correction C in FP64 and FP32 and the full near
+// interaction K_up in FP32, in the storage orders of the near-field kernels.  Touching pairs
+// arrive as K_up blocks (from the quadtree kernel below, or from the host) and only get their
+// direct term here.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace dbie {
+namespace {
+constexpr double kFourPiB = 4.0 * 3.14159265358979311599796346854;
+
+// laplace3d_grad(s, t, n), with the operation order of the host builder
+__device__ inline double grad_k(double s0, double s1, double s2, double t0, double t1, double t2, double n0,
+                                double n1, double n2) {
+#pragma clang fp contract(off)
+    const double d0 = s0 - t0, d1 = s1 - t1, d2 = s2 - t2;
+    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double ir = 1.0 / r;
+    const double ir3 = ir * ir * ir;
+    return (n0 * (ir3 * (t0 - s0)) + n1 * (ir3 * (t1 - s1)) + n2 * (ir3 * (t2 - s2))) / kFourPiB;
+}
+
+__device__ inline int store_idx(int bd, bool mfma, int t, int c) {
+    return mfma ? (c >> 2) * 64 + (c & 3) * 16 + t : t * bd + c;
+}
+__device__ inline int store_up_idx(int bd, bool mfma, int t, int c) {
+    return mfma ? ((c >> 2) * 16 + t) * 4 + (c & 3) : t * bd + c;
+}
+
+// One workgroup per item.  up = 1: upsampled pair (K_up from the cache); up = 0: K_up given (kup).
+template <bool UP>
+__global__ __launch_bounds__(256) void near_blocks_kernel(DevGeom g, int bd, int mfma, const int *__restrict__ src,
+                                                          const int *__restrict__ trg, const int *__restrict__ slot,
+                                                          const int *__restrict__ cache_off,
+                                                          const int *__restrict__ cache_nu,
+                                                          const double *__restrict__ caches,
+                                                          const double *__restrict__ kup, int mt_lds,
+                                                          double *__restrict__ val64, float *__restrict__ val32,
+                                                          float *__restrict__ vup32) {
+#pragma clang fp contract(off)
+    extern __shared__ double sm[];  // UP: kv (bd x nu2), then Mt (bd x nu2) when mt_lds
+    const int it = blockIdx.x;
+    const int i = src[it], j = trg[it];
+    const int64_t sb = (int64_t)i * bd, tb = (int64_t)j * bd;
+    const double n0 = g.nx[tb], n1 = g.ny[tb], n2 = g.nz[tb];
+    const size_t out0 = (size_t)slot[it] * bd * bd;
+    int nu2 = 0;
+    const double *Mt = nullptr;
+    double *kv = sm;
+    if (UP) {
+        const int nu = cache_nu[i];
+        nu2 = nu * nu;
+        const double *upos = caches + cache_off[i];
+        Mt = upos + 3 * nu2;
+        double *sMt = sm + bd * nu2;
+        for (int q = threadIdx.x; q < bd * nu2; q += blockDim.x) {
+            const int t = q / nu2, a = q - t * nu2;
+            kv[q] = grad_k(upos[3 * a], upos[3 * a + 1], upos[3 * a + 2], g.x[tb + t], g.y[tb + t], g.z[tb + t], n0,
+                           n1, n2);
+            if (mt_lds) sMt[q] = Mt[q];
+        }
+        if (mt_lds) Mt = sMt;
+        __syncthreads();
+    }
+    for (int o = threadIdx.x; o < bd * bd; o += blockDim.x) {
+        const int t = o / bd, c = o - t * bd;
+        double s = 0.0;
+        if (UP) {
+            const double *mr = Mt + (size_t)c * nu2;
+            const double *kr = kv + (size_t)t * nu2;
+            for (int a = 0; a < nu2; ++a) s += mr[a] * kr[a];
+        } else {
+            s = kup[(size_t)it * bd * bd + o];
+        }
+        const double direct = grad_k(g.x[sb + c], g.y[sb + c], g.z[sb + c], g.x[tb + t], g.y[tb + t], g.z[tb + t], n0,
+                                     n1, n2) *
+                              g.w[sb + c];
+        const double v = s - direct;
+        val64[out0 + store_idx(bd, mfma != 0, t, c)] = v;
+        val32[out0 + store_idx(bd, mfma != 0, t, c)] = (float)v;
+        vup32[out0 + store_up_idx(bd, mfma != 0, t, c)] = (float)s;
+    }
+}
+// barycentric_row (geometry.cpp), same operations: row of Lagrange basis values at xq
+__device__ inline void bary_row(int n, const double *x, const double *lam, double xq, double *r) {
+#pragma clang fp contract(off)
+    const double rtol = 0x1p-26;  // sqrt(DBL_EPSILON)
+    for (int i = 0; i < n; ++i) {
+        const double ax = fabs(xq), bx = fabs(x[i]);
+        if (xq == x[i] || fabs(xq - x[i]) <= rtol * (ax < bx ? bx : ax)) {
+            for (int k = 0; k < n; ++k) r[k] = 0.0;
+            r[i] = 1.0;
+            return;
+        }
+    }
+    double den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        r[i] = lam[i] / (xq - x[i]);
+        den += r[i];
+    }
+    const double inv = 1.0 / den;
+    for (int i = 0; i < n; ++i) r[i] *= inv;
+}
+
+// Touching pairs: the quadtree of walk_adaptive_tree (src/kernel/laplace3d_near_adaptive.jl:119-224)
+// driven by the representative target, depth first with an explicit stack in LDS, and the block
+// moments of adaptive_block_moments_inplace! (:308-387) accumulated leaf by leaf in walk order.
+// Every sum runs in the order of the host quadtree (Quadtree::walk / adaptive_block), so the
+// split decisions and the blocks are those of the host.  One wave per pair; kup gets K_up
+// (bd x bd row-major, no direct term), hit the max-depth flag (1), or 2 when the walk stopped at
+// max_nodes nodes (the host then builds that block).
+__global__ __launch_bounds__(64) void adaptive_blocks_kernel(DevGeom g, int nq, int nGL, const int *__restrict__ src,
+                                                             const int *__restrict__ trg, const int *__restrict__ rep,
+                                                             const double *__restrict__ pmap,
+                                                             const double *__restrict__ rules, double atol,
+                                                             double rtol, int max_depth, int max_nodes,
+                                                             double *__restrict__ kup, int *__restrict__ hit) {
+#pragma clang fp contract(off)
+    extern __shared__ double sm[];
+    __shared__ int s_top, s_leaf;
+    const int it = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int npp = nq * nq, bd = npp, G2 = nGL * nGL;
+    const int ent = 5 + npp, cap = 3 * max_depth + 4;
+    double *stk = sm;                   // cap x (ulo, uhi, vlo, vhi, depth, moments[npp])
+    double *cm = stk + cap * ent;       // 4 x npp: moments of the four cells in cb
+    double *wk = cm + 4 * npp;          // 4 x G2: weight x kernel at the representative target
+    double *wuv = wk + 4 * G2;          // 4 x G2: weights
+    double *rxs = wuv + 4 * G2;         // 4 x nGL x nq: interpolation rows in u
+    double *rys = rxs + 4 * nGL * nq;   // 4 x nGL x nq: interpolation rows in v
+    double *pts = rys + 4 * nGL * nq;   // 4 x G2 x 3: quadrature points
+    double *kt = pts + 12 * G2;         // G2 x bd: kernel at the panel's targets (one leaf)
+    double *blk = kt + G2 * bd;         // bd x bd
+    double *cb = blk + bd * bd;         // 4 x (ulo, uhi, vlo, vhi)
+
+    const double *mp = pmap + (size_t)it * 10;
+    const double cc0 = mp[0], cc1 = mp[1], cc2 = mp[2], bm0 = mp[3], bm1 = mp[4], bm2 = mp[5];
+    const double dm0 = mp[6], dm1 = mp[7], dm2 = mp[8], scale = mp[9];
+    const double *glx = rules, *bary = rules + nq, *ns = rules + 2 * nq, *ws = ns + nGL;
+    const int64_t tb = (int64_t)trg[it] * bd;
+    const int64_t rn = tb + rep[it];
+    const double tx = g.x[rn], ty = g.y[rn], tz = g.z[rn];
+    const double n0 = g.nx[tb], n1 = g.ny[tb], n2 = g.nz[tb];
+    (void)src;
+
+    // moments of the first ncell cells of cb into cm (Quadtree::moments)
+    auto moments = [&](int ncell) {
+        for (int idx = tid; idx < ncell * G2; idx += nt) {
+            const int k = idx / G2, ab = idx - k * G2, a = ab / nGL, b = ab - a * nGL;
+            const double ulo = cb[4 * k], uhi = cb[4 * k + 1], vlo = cb[4 * k + 2], vhi = cb[4 * k + 3];
+            const double hu = (uhi - ulo) / 2, hv = (vhi - vlo) / 2, mu = (uhi + ulo) / 2, mv = (vhi + vlo) / 2;
+            const double sc = hu * hv * scale;
+            const double u = mu + hu * ns[a], v = mv + hv * ns[b];
+            if (b == 0) bary_row(nq, glx, bary, u, rxs + (k * nGL + a) * nq);
+            if (a == 0) bary_row(nq, glx, bary, v, rys + (k * nGL + b) * nq);
+            const double y0 = cc0 + bm0 * (u / 2) + dm0 * (v / 2);
+            const double y1 = cc1 + bm1 * (u / 2) + dm1 * (v / 2);
+            const double y2 = cc2 + bm2 * (u / 2) + dm2 * (v / 2);
+            const double w = ws[a] * ws[b] * sc;
+            pts[3 * idx] = y0;
+            pts[3 * idx + 1] = y1;
+            pts[3 * idx + 2] = y2;
+            wuv[idx] = w;
+            wk[idx] = w * grad_k(y0, y1, y2, tx, ty, tz, n0, n1, n2);
+        }
+        __syncthreads();
+        for (int idx = tid; idx < ncell * npp; idx += nt) {
+            const int k = idx / npp, q = idx - k * npp, mx = q / nq, my = q - mx * nq;
+            double s = 0.0;
+            for (int a = 0; a < nGL; ++a) {
+                const double rx = rxs[(k * nGL + a) * nq + mx];
+                for (int b = 0; b < nGL; ++b) s += wk[k * G2 + a * nGL + b] * rx * rys[(k * nGL + b) * nq + my];
+            }
+            cm[idx] = s;
+        }
+        __syncthreads();
+    };
+
+    for (int e = tid; e < bd * bd; e += nt) blk[e] = 0.0;
+    if (tid == 0) {
+        cb[0] = -1.0; cb[1] = 1.0; cb[2] = -1.0; cb[3] = 1.0;
+    }
+    __syncthreads();
+    moments(1);
+    for (int q = tid; q < ent; q += nt) stk[q] = q < 4 ? cb[q] : (q == 4 ? 0.0 : cm[q - 5]);
+    if (tid == 0) s_top = 1;
+    int hitflag = 0, nodes = 0;
+    __syncthreads();
+    while (s_top > 0 && nodes < max_nodes) {
+        ++nodes;
+        const int top = s_top;
+        const double *nd = stk + (size_t)(top - 1) * ent;
+        const double ulo = nd[0], uhi = nd[1], vlo = nd[2], vhi = nd[3];
+        const int depth = (int)nd[4];
+        const double mu = (uhi + ulo) / 2, mv = (vhi + vlo) / 2;
+        if (tid == 0) {
+            const double c[16] = {ulo, mu, vlo, mv, mu, uhi, vlo, mv, ulo, mu, mv, vhi, mu, uhi, mv, vhi};
+            for (int q = 0; q < 16; ++q) cb[q] = c[q];
+        }
+        __syncthreads();
+        moments(4);
+        if (tid == 0) {
+            double err = 0.0, pmax = 0.0;
+            for (int q = 0; q < npp; ++q) {
+                const double e = fabs(nd[5 + q] - (cm[q] + cm[npp + q] + cm[2 * npp + q] + cm[3 * npp + q]));
+                if (q == 0 || e > err) err = e;
+                const double a = fabs(nd[5 + q]);
+                if (q == 0 || a > pmax) pmax = a;
+            }
+            const double rp = rtol * pmax;
+            const double tl = atol < rp ? rp : atol;
+            if (depth == max_depth && err >= tl) hitflag = 1;
+            s_leaf = (err < tl || depth == max_depth) ? 1 : 0;
+            s_top = top - 1;
+        }
+        __syncthreads();
+        if (s_leaf) {
+            // the four cells are leaves: block moments at every target of the panel, cell by cell
+            for (int k = 0; k < 4; ++k) {
+                for (int idx = tid; idx < G2 * bd; idx += nt) {
+                    const int ab = idx / bd, t = idx - ab * bd;
+                    const double *y = pts + 3 * (k * G2 + ab);
+                    kt[idx] = grad_k(y[0], y[1], y[2], g.x[tb + t], g.y[tb + t], g.z[tb + t], n0, n1, n2);
+                }
+                __syncthreads();
+                for (int e = tid; e < bd * bd; e += nt) {
+                    const int t = e / bd, c = e - t * bd, mx = c / nq, my = c - mx * nq;
+                    double s = blk[e];
+                    for (int a = 0; a < nGL; ++a) {
+                        const double rx = rxs[(k * nGL + a) * nq + mx];
+                        for (int b = 0; b < nGL; ++b) {
+                            const double wm = wuv[k * G2 + a * nGL + b] * rx * rys[(k * nGL + b) * nq + my];
+                            s += wm * kt[(a * nGL + b) * bd + t];
+                        }
+                    }
+                    blk[e] = s;
+                }
+                __syncthreads();
+            }
+        } else {
+            // push the children last-to-first so that the first child is walked next
+            const int base = top - 1;
+            for (int idx = tid; idx < 4 * ent; idx += nt) {
+                const int sl = idx / ent, q = idx - sl * ent, k = 3 - sl;
+                stk[(size_t)(base + sl) * ent + q] =
+                    q < 4 ? cb[4 * k + q] : (q == 4 ? (double)(depth + 1) : cm[k * npp + q - 5]);
+            }
+            __syncthreads();
+            if (tid == 0) s_top = base + 4;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < bd * bd; e += nt) kup[(size_t)it * bd * bd + e] = blk[e];
+    if (tid == 0) hit[it] = s_top > 0 ? 2 : hitflag;  // 2: node budget spent, block incomplete
+}
+}  // namespace
+
+size_t adaptive_blocks_lds(int nq, int nGL, int max_depth) {
+    const size_t npp = (size_t)nq * nq, G2 = (size_t)nGL * nGL;
+    const size_t cap = 3 * (size_t)max_depth + 4;
+    return sizeof(double) * (cap * (5 + npp) + 4 * npp + 8 * G2 + 8 * (size_t)nGL * nq + 12 * G2 + G2 * npp +
+                             npp * npp + 16);
+}
+
+bool adaptive_blocks(hipStream_t st, const DevGeom &g, int nq, int nGL, int n_adp, const int *adp_src,
+                     const int *adp_trg, const int *adp_rep, const double *adp_map, const double *rules, double atol,
+                     double rtol, int max_depth, int max_nodes, double *kup, int *hit) {
+    const size_t lds = adaptive_blocks_lds(nq, nGL, max_depth);
+    if (lds > kNearBuildLds || max_depth < 0) return false;
+    if (n_adp > 0)
+        adaptive_blocks_kernel<<<n_adp, 64, lds, st>>>(g, nq, nGL, adp_src, adp_trg, adp_rep, adp_map, rules, atol,
+                                                       rtol, max_depth, max_nodes, kup, hit);
+    return true;
+}
+
+bool build_near_blocks(hipStream_t st, const DevGeom &g, int bd, bool mfma_order, int n_up, const int *up_src,
+                       const int *up_trg, const int *up_slot, const int *cache_off, const int *cache_nu,
+                       const double *caches, int max_nu, int n_adp, const int *adp_src, const int *adp_trg,
+                       const int *adp_slot, const double *adp_kup, double *val64, float *val32, float *vup32) {
+    if (n_up > 0) {
+        // kernel values in LDS; the quadrature matrix too when both fit in 64 KiB
+        const size_t kv = sizeof(double) * (size_t)bd * max_nu * max_nu;
+        if (kv > kNearBuildLds) return false;
+        const int mt_lds = 2 * kv <= kNearBuildLds ? 1 : 0;
+        near_blocks_kernel<true><<<n_up, 256, (1 + mt_lds) * kv, st>>>(g, bd, mfma_order ? 1 : 0, up_src, up_trg,
+                                                                       up_slot, cache_off, cache_nu, caches, nullptr,
+                                                                       mt_lds, val64, val32, vup32);
+    }
+    if (n_adp > 0)
+        near_blocks_kernel<false><<<n_adp, 256, 0, st>>>(g, bd, mfma_order ? 1 : 0, adp_src, adp_trg, adp_slot, nullptr,
+                                                         nullptr, nullptr, adp_kup, 0, val64, val32, vup32);
+    return true;
+}
+
+}  // namespace dbie

@@ -313,6 +313,197 @@ struct Quadtree {
 
 }  // namespace
 
+namespace {
+// Slots of the merged pair list in BSR order (block rows = target panels, ascending source panel).
+struct PairItems {
+    struct Item { int src, trg, order; };  // order 0 = touching (adaptive)
+    std::vector<Item> items;               // sorted by (src, trg)
+    std::vector<int64_t> slot, first;      // BSR slot per item; items of source panel i: [first[i], first[i+1])
+    std::vector<int> row_ptr, col;
+};
+PairItems order_pairs(const Geometry &g, const PairList &pl) {
+    const int64_t P = g.P;
+    PairItems pi;
+    auto &items = pi.items;
+    items.reserve(pl.up_src.size() + pl.adp_src.size());
+    for (size_t k = 0; k < pl.up_src.size(); ++k) items.push_back({pl.up_src[k], pl.up_trg[k], pl.up_order[k]});
+    for (size_t k = 0; k < pl.adp_src.size(); ++k) items.push_back({pl.adp_src[k], pl.adp_trg[k], 0});
+    std::sort(items.begin(), items.end(), [](const PairItems::Item &a, const PairItems::Item &b) {
+        return a.src != b.src ? a.src < b.src : a.trg < b.trg;
+    });
+    pi.row_ptr.assign(P + 1, 0);
+    for (const auto &it : items) pi.row_ptr[it.trg + 1]++;
+    for (int64_t p = 0; p < P; ++p) pi.row_ptr[p + 1] += pi.row_ptr[p];
+    const int64_t nb = (int64_t)items.size();
+    pi.col.resize(nb);
+    pi.slot.resize(nb);
+    std::vector<int> fill(pi.row_ptr.begin(), pi.row_ptr.end() - 1);
+    for (int64_t k = 0; k < nb; ++k) {
+        pi.slot[k] = fill[items[k].trg]++;
+        pi.col[pi.slot[k]] = items[k].src;
+    }
+    pi.first.assign(P + 1, 0);
+    for (const auto &it : items) pi.first[it.src + 1]++;
+    for (int64_t p = 0; p < P; ++p) pi.first[p + 1] += pi.first[p];
+    return pi;
+}
+
+// Source cache of panel i at order n_up (build_source_cache, src/kernel/laplace3d_near_upsampling.jl:8-50):
+// upos (n_up^2 x 3) and Mt (bd x n_up^2, row = moment).
+void source_cache(const Geometry &g, int64_t i, int n_up, double *upos, double *Mt) {
+    const int nq = g.nq, bd = nq * nq;
+    const PanelMap m = panel_map(g, i);
+    double ns[16], ws[16], Ex[16][16];
+    gauss_legendre(n_up, ns, ws);
+    for (int a = 0; a < n_up; ++a) barycentric_row(nq, g.glx, g.bary, ns[a], Ex[a]);
+    const int nu2 = n_up * n_up;
+    for (int a = 0; a < n_up; ++a)
+        for (int b = 0; b < n_up; ++b) {
+            const int al = a * n_up + b;
+            for (int k = 0; k < 3; ++k) upos[3 * al + k] = m.cc[k] + m.bma[k] * (ns[a] / 2) + m.dma[k] * (ns[b] / 2);
+        }
+    for (int mx = 0; mx < nq; ++mx)
+        for (int my = 0; my < nq; ++my)
+            for (int a = 0; a < n_up; ++a)
+                for (int b = 0; b < n_up; ++b)
+                    Mt[(size_t)(mx * nq + my) * nu2 + a * n_up + b] = m.scale * ws[a] * ws[b] * Ex[a][mx] * Ex[b][my];
+}
+
+// Touching pair: the adaptive block moments of source panel i at the targets of panel j, driven by the
+// representative target (walk_adaptive_tree + adaptive_block_moments_inplace!,
+// src/kernel/laplace3d_near_adaptive.jl:119-224,308-387).  blk: bd x bd row-major (K_up, no direct term).
+bool adaptive_block(const Geometry &g, int64_t i, int64_t j, const NearOpts &o, double *blk) {
+    const int nq = g.nq, bd = nq * nq;
+    const int nGL = o.adaptive_n_GL == 0 ? nq : o.adaptive_n_GL;
+    const PanelMap m = panel_map(g, i);
+    const double *tn = &g.normal[3 * j];
+    double tpts[64][3];
+    for (int t = 0; t < bd; ++t) node(g, j * bd + t, tpts[t]);
+    Quadtree qt{g, m, nGL, {}, {}, nullptr, tn, o.adaptive_atol, o.adaptive_rtol, o.adaptive_max_depth, {}, false};
+    gauss_legendre(nGL, qt.ns, qt.ws);
+    const double *c1 = &g.corners[12 * i];
+    int rep = 0;
+    double best = INFINITY;
+    for (int t = 0; t < bd; ++t) {
+        const double d = len3(tpts[t][0] - c1[0], tpts[t][1] - c1[1], tpts[t][2] - c1[2]);
+        if (d < best) { best = d; rep = t; }
+    }
+    qt.trg = tpts[rep];
+    qt.walk(-1.0, 1.0, -1.0, 1.0, 0);
+    std::fill(blk, blk + bd * bd, 0.0);
+    double rx[16], ry[16], kt[64];
+    for (size_t l = 0; l < qt.leaves.size(); l += 4) {
+        const double ulo = qt.leaves[l], uhi = qt.leaves[l + 1], vlo = qt.leaves[l + 2], vhi = qt.leaves[l + 3];
+        const double hu = (uhi - ulo) / 2, hv = (vhi - vlo) / 2, mu = (uhi + ulo) / 2, mv = (vhi + vlo) / 2;
+        const double sc = hu * hv * m.scale;
+        for (int a = 0; a < nGL; ++a) {
+            const double u = mu + hu * qt.ns[a], wu = qt.ws[a];
+            barycentric_row(nq, g.glx, g.bary, u, rx);
+            for (int b = 0; b < nGL; ++b) {
+                const double v = mv + hv * qt.ns[b], wv = qt.ws[b];
+                barycentric_row(nq, g.glx, g.bary, v, ry);
+                double y[3];
+                for (int kk = 0; kk < 3; ++kk) y[kk] = m.cc[kk] + m.bma[kk] * (u / 2) + m.dma[kk] * (v / 2);
+                const double wuv = wu * wv * sc;
+                for (int t = 0; t < bd; ++t) kt[t] = grad_kernel(y, tpts[t], tn);
+                for (int mx = 0; mx < nq; ++mx)
+                    for (int my = 0; my < nq; ++my) {
+                        const int c = mx * nq + my;
+                        const double wm = wuv * rx[mx] * ry[my];
+                        for (int t = 0; t < bd; ++t) blk[t * bd + c] += wm * kt[t];
+                    }
+            }
+        }
+    }
+    return qt.hit;
+}
+}  // namespace
+
+bool touching_block(const Geometry &g, int64_t i, int64_t j, const NearOpts &o, double *blk) {
+    return adaptive_block(g, i, j, o, blk);
+}
+
+std::vector<double> quadtree_rules(const Geometry &g, int nGL) {
+    std::vector<double> r(2 * g.nq + 2 * nGL);
+    std::copy(g.glx, g.glx + g.nq, r.begin());
+    std::copy(g.bary, g.bary + g.nq, r.begin() + g.nq);
+    gauss_legendre(nGL, &r[2 * g.nq], &r[2 * g.nq + nGL]);
+    return r;
+}
+
+CorrectionPlan plan_corrections(const Geometry &g, const PairList &pl, const NearOpts &o, bool host_quadtrees) {
+    const int64_t P = g.P;
+    const int bd = g.npp();
+    PairItems pi = order_pairs(g, pl);
+    CorrectionPlan cp;
+    cp.bd = bd;
+    cp.row_ptr = std::move(pi.row_ptr);
+    cp.col = std::move(pi.col);
+    cp.cache_off.assign(P, -1);
+    cp.cache_nu.assign(P, 0);
+    // cache layout and the item lists
+    int64_t off = 0;
+    for (int64_t i = 0; i < P; ++i) {
+        int n_up = 0;
+        for (int64_t k = pi.first[i]; k < pi.first[i + 1]; ++k) n_up = std::max(n_up, pi.items[k].order);
+        if (n_up > 0) {
+            cp.cache_off[i] = (int)off;
+            cp.cache_nu[i] = n_up;
+            off += (int64_t)n_up * n_up * (3 + bd);
+        }
+        for (int64_t k = pi.first[i]; k < pi.first[i + 1]; ++k) {
+            const auto &it = pi.items[k];
+            if (it.order > 0) {
+                cp.up_src.push_back(it.src);
+                cp.up_trg.push_back(it.trg);
+                cp.up_slot.push_back((int)pi.slot[k]);
+            } else {
+                cp.adp_src.push_back(it.src);
+                cp.adp_trg.push_back(it.trg);
+                cp.adp_slot.push_back((int)pi.slot[k]);
+            }
+        }
+    }
+    if (off > INT32_MAX) fail(DBIE_BAD_ARG, "source caches too large");
+    cp.caches.resize((size_t)off);
+    parallel_for(P, o.n_threads, [&](int64_t i, int) {
+        if (cp.cache_nu[i] == 0) return;
+        const int nu = cp.cache_nu[i];
+        double *c = &cp.caches[(size_t)cp.cache_off[i]];
+        source_cache(g, i, nu, c, c + 3 * nu * nu);
+    });
+    if (!host_quadtrees) {
+        // representative target: the target node nearest the source panel's first corner
+        const size_t na = cp.adp_src.size();
+        cp.adp_rep.resize(na);
+        cp.adp_map.resize(na * 10);
+        for (size_t k = 0; k < na; ++k) {
+            const int i = cp.adp_src[k], j = cp.adp_trg[k];
+            const PanelMap m = panel_map(g, i);
+            double *mp = &cp.adp_map[k * 10];
+            for (int c = 0; c < 3; ++c) { mp[c] = m.cc[c]; mp[3 + c] = m.bma[c]; mp[6 + c] = m.dma[c]; }
+            mp[9] = m.scale;
+            const double *c1 = &g.corners[12 * (size_t)i];
+            int rep = 0;
+            double best = INFINITY;
+            for (int t = 0; t < bd; ++t) {
+                const int64_t n = (int64_t)j * bd + t;
+                const double d = len3(g.x[n] - c1[0], g.y[n] - c1[1], g.z[n] - c1[2]);
+                if (d < best) { best = d; rep = t; }
+            }
+            cp.adp_rep[k] = rep;
+        }
+        return cp;
+    }
+    cp.adp_kup.resize(cp.adp_src.size() * (size_t)bd * bd);
+    std::atomic<int> hits{0};
+    parallel_for((int64_t)cp.adp_src.size(), o.n_threads, [&](int64_t k, int) {
+        if (adaptive_block(g, cp.adp_src[k], cp.adp_trg[k], o, &cp.adp_kup[(size_t)k * bd * bd])) hits.fetch_add(1);
+    });
+    cp.adaptive_hits = hits.load();
+    return cp;
+}
+
 BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearOpts &o) {
     const int64_t P = g.P;
     const int nq = g.nq, bd = nq * nq;

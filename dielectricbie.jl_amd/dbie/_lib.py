@@ -32,7 +32,8 @@ class NotConverged(DbieError):
 class NearOpts(C.Structure):
     _fields_ = [("up_tol", C.c_double), ("max_order", C.c_int), ("correct_edges", C.c_int),
                 ("adaptive_atol", C.c_double), ("adaptive_rtol", C.c_double), ("adaptive_n_GL", C.c_int),
-                ("adaptive_max_depth", C.c_int), ("build_corrections", C.c_int), ("n_threads", C.c_int)]
+                ("adaptive_max_depth", C.c_int), ("build_corrections", C.c_int), ("n_threads", C.c_int),
+                ("device_build", C.c_int)]
 
 
 class SolveOpts(C.Structure):

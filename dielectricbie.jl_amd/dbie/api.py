@@ -216,7 +216,7 @@ class BatchedDielectricOperator:
 
     def __init__(self, interface: DielectricInterface, *, corrected=True, up_tol=1e-9, max_order=8,
                  correct_edges=True, adaptive_atol=None, adaptive_rtol=SQRT_EPS, adaptive_n_GL=0,
-                 adaptive_max_depth=20, n_threads=0):
+                 adaptive_max_depth=20, n_threads=0, device_build=True):
         o = L.NearOpts()
         L.lib().dbie_near_opts_default(C.byref(o))
         o.up_tol = float(up_tol)
@@ -228,6 +228,7 @@ class BatchedDielectricOperator:
         o.adaptive_max_depth = int(adaptive_max_depth)
         o.build_corrections = int(bool(corrected))
         o.n_threads = int(n_threads)
+        o.device_build = int(bool(device_build))
         h = C.c_void_p()
         L.check(L.lib().dbie_operator_create(interface._h, L.dptr(interface.eps_in), L.dptr(interface.eps_out),
                                              C.byref(o), C.byref(h)))

@@ -73,6 +73,7 @@ typedef struct {
     int adaptive_max_depth;  /* default 20 */
     int build_corrections;   /* 0 = plain (uncorrected) operator, as lhs_dielectric_box3d_fmm3d */
     int n_threads;           /* host threads for the near-field build (0 = all cores) */
+    int device_build;        /* build the correction blocks on the GPU (default 1; 0 = host threads) */
 } dbie_near_opts;
 
 /* Solve options: keywords of solve_dielectric_box3d_block (src/solver/multi_rhs.jl:270-276)

@@ -231,3 +231,26 @@ def test_device_resident_solve(dbie, cfg1):
     Sm = dS.to_host()
     for k in range(32):
         assert rel(Sm[:, k], S_host[:, k]) < 1e-8
+
+
+@pytest.mark.parametrize("nq,max_order", [(4, 8), (4, 16), (5, 16)])
+def test_device_built_blocks_match_host(dbie, nq, max_order):
+    """Correction blocks built on the device equal the host-built ones (same sums, same order)."""
+    say(f"device-built vs host-built correction blocks, n_quad={nq}, max_order={max_order}")
+    g = dbie.single_dielectric_box3d(1.0, 0.8, 0.6, nq, 0.2, 4.0, 1.0)
+    opd = dbie.BatchedDielectricOperator(g, max_order=max_order, device_build=True)
+    oph = dbie.BatchedDielectricOperator(g, max_order=max_order, device_build=False)
+    sd, td, bd_ = opd.corrections()
+    sh, th, bh = oph.corrections()
+    assert np.array_equal(sd, sh) and np.array_equal(td, th)
+    scale = np.abs(bh).max()
+    say(f"  max |device - host| / scale {np.abs(bd_ - bh).max() / scale:.2e}, "
+        f"bit-equal entries {np.mean(bd_ == bh):.6f}")
+    assert np.abs(bd_ - bh).max() <= 1e-14 * scale
+    X = np.random.default_rng(21).standard_normal((g.n_points, 32))
+    assert rel(opd @ X, oph @ X) < 1e-13
+    Y64 = oph @ X
+    for prec in ("fp16x3", "fp32"):
+        Yd, Yh = opd.apply(X, prec), oph.apply(X, prec)
+        assert rel(Yd, Yh) < 1e-7 * (1 + 100 * (prec == "fp32")), prec
+        assert rel(Yd, Y64) < (2e-6 if prec == "fp16x3" else 1e-5), prec
