@@ -104,6 +104,45 @@ namespace {
 // quadtree nodes one device walk of a touching pair may visit before that pair goes to the host
 constexpr int kQuadtreeNodeBudget = 1 << 16;
 
+// Near-pair list with the candidate scan on the device (pair_prep and finish_pair_list on the host).
+PairList device_pair_list(const Geometry &g, const DevGeom &dg, int max_order, double atol, bool correct_edges,
+                          hipStream_t st) {
+    const int64_t P = g.P;
+    if (P > INT32_MAX) fail(DBIE_BAD_ARG, "too many panels");
+    const PairPrep pp = pair_prep(g, max_order, atol, correct_edges);
+    DBuf pd, co, ie, cnt, dptr, dtrg, drho, dtouch;
+    upload(pd, pp.pdata.data(), pp.pdata.size(), st);
+    upload(co, g.corners.data(), g.corners.size(), st);
+    upload(ie, g.is_edge.data(), g.is_edge.size(), st);
+    cnt.ensure(sizeof(int64_t) * P);
+    pair_scan(st, dg, (int)P, g.npp(), pd.as<double>(), co.as<double>(), ie.as<int>(), pp.tc2, correct_edges,
+              cnt.as<int64_t>(), nullptr, nullptr, nullptr, nullptr);
+    DBIE_HIP(hipGetLastError());
+    PairCandidates c;
+    c.ptr.assign(P + 1, 0);
+    DBIE_HIP(hipMemcpyAsync(c.ptr.data() + 1, cnt.p, sizeof(int64_t) * P, hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < P; ++i) c.ptr[i + 1] += c.ptr[i];
+    const int64_t total = c.ptr[P];
+    upload(dptr, c.ptr.data(), c.ptr.size(), st);
+    dtrg.ensure(sizeof(int) * std::max<int64_t>(total, 1));
+    drho.ensure(sizeof(double) * std::max<int64_t>(total, 1));
+    dtouch.ensure(std::max<int64_t>(total, 1));
+    pair_scan(st, dg, (int)P, g.npp(), pd.as<double>(), co.as<double>(), ie.as<int>(), pp.tc2, correct_edges,
+              nullptr, dptr.as<int64_t>(), dtrg.as<int>(), drho.as<double>(), dtouch.as<unsigned char>());
+    DBIE_HIP(hipGetLastError());
+    c.trg.resize(total);
+    c.rho.resize(total);
+    c.touch.resize(total);
+    if (total) {
+        DBIE_HIP(hipMemcpyAsync(c.trg.data(), dtrg.p, sizeof(int) * total, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(c.rho.data(), drho.p, sizeof(double) * total, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(c.touch.data(), dtouch.p, total, hipMemcpyDeviceToHost, st));
+    }
+    DBIE_HIP(hipStreamSynchronize(st));
+    return finish_pair_list(g, pp, c);
+}
+
 void check_op(const dbie_op *op) {
     if (!op) fail(DBIE_BAD_ARG, "operator handle is NULL");
 }
@@ -715,7 +754,12 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
                                (double)bd * o.max_order * o.max_order * sizeof(double) <= (double)kNearBuildLds;
         bool device_built = false;
         if (o.build_corrections) {
-            const PairList pl = build_pair_list(g, o.max_order, o.up_tol, o.correct_edges != 0, o.n_threads);
+            DevGeom dgp;
+            dgp.N = N;
+            dgp.x = op->x.as<double>(); dgp.y = op->y.as<double>(); dgp.z = op->z.as<double>();
+            const PairList pl = o.device_build
+                                    ? device_pair_list(g, dgp, o.max_order, o.up_tol, o.correct_edges != 0, st)
+                                    : build_pair_list(g, o.max_order, o.up_tol, o.correct_edges != 0, o.n_threads);
             t1 = Clk::now();
             if (on_device) {
                 // host: pair order, source caches, touching-pair quadtrees; device: every block

@@ -86,6 +86,26 @@ struct PairList {
 
 PairList build_pair_list(const Geometry &g, int max_order, double atol, bool correct_edges, int n_threads);
 
+// The pair list in three steps (build_pair_list runs all three on the host; the device scan
+// replaces the middle one).  Per-panel data, kPairData doubles per panel: centre[3], ri^2,
+// node radius, Bernstein frame cc[3] e1[3] e2[3] n[3] L1 L2, ri (kPairData = 20).
+struct PairPrep {
+    int max_order = 8;
+    double atol = 1e-9, rho_star = 0, tc2 = 0;
+    bool correct_edges = true;
+    std::vector<double> len, off, pdata;
+};
+PairPrep pair_prep(const Geometry &g, int max_order, double atol, bool correct_edges);
+// candidates per source panel (CSR over source panels, ascending target panel): the smallest
+// Bernstein radius over the target's nodes in the ball (inf if none) and the corner-touch flag
+struct PairCandidates {
+    std::vector<int64_t> ptr;
+    std::vector<int> trg;
+    std::vector<double> rho;
+    std::vector<unsigned char> touch;
+};
+PairList finish_pair_list(const Geometry &g, const PairPrep &pp, const PairCandidates &c);
+
 // Block-sparse (BSR) correction matrix: block rows = target panels.
 struct BlockSparse {
     int bd = 0;                    // block dimension = nq^2

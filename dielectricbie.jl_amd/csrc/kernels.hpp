@@ -126,6 +126,14 @@ void scatter_add(hipStream_t st, int64_t n, const int64_t *idx, const double *va
 // the full near interaction (FP32) of each pair's BSR slot.  max_nu: largest cache order (LDS size).
 // Returns false (nothing launched) when bd x max_nu^2 doubles exceed kNearBuildLds.
 constexpr size_t kNearBuildLds = 64 * 1024;
+// Near-pair candidate scan on the device (the middle step of the pair list).  pdata: kPairData
+// doubles per panel (pair_prep); corners: 12 per panel; is_edge: per panel.  With ptr == nullptr it
+// writes count[i] (candidates of source panel i); with ptr (P + 1 offsets) it writes the candidates:
+// target panel, smallest Bernstein radius over its nodes in the ball (inf if none), touch flag.
+constexpr int kPairData = 20;
+void pair_scan(hipStream_t st, const DevGeom &g, int P, int npp, const double *pdata, const double *corners,
+               const int *is_edge, double tc2, bool correct_edges, int64_t *count, const int64_t *ptr, int *ctrg,
+               double *crho, unsigned char *ctouch);
 // Touching pairs on the device: K_up blocks (n_adp x bd x bd, row-major) and max-depth flags of the
 // quadtree rule, one wave per pair.  adp_map: 10 doubles per pair (cc, bma, dma, scale of the source
 // panel); rules: glx[nq], bary[nq], ns[nGL], ws[nGL].  Returns false (nothing launched) when the

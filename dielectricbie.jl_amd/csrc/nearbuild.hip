@@ -13,6 +13,8 @@
 // direct term here.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include "kernels.hpp"
 
 namespace dbie {
@@ -262,7 +264,179 @@ __global__ __launch_bounds__(64) void adaptive_blocks_kernel(DevGeom g, int nq, 
     for (int e = tid; e < bd * bd; e += nt) kup[(size_t)it * bd * bd + e] = blk[e];
     if (tid == 0) hit[it] = s_top > 0 ? 2 : hitflag;  // 2: node budget spent, block incomplete
 }
+// ---- near-pair scan (build_neighbor_list, src/kernel/laplace3d_near.jl:240-392) ----------------
+// The scalar helpers below repeat the host's (nearfield.cpp) operations one for one, so that the
+// Bernstein radii, and with them every pair decision, are bit-identical.
+
+// Julia Base.Math.hypot, FMA-corrected branch
+__device__ double hypot_cr_d(double x, double y) {
+#pragma clang fp contract(off)
+    if (isinf(x) || isinf(y)) return INFINITY;
+    double ax = fabs(x), ay = fabs(y);
+    if (ay > ax) { const double t = ax; ax = ay; ay = t; }
+    if (ay <= ax * sqrt(DBL_EPSILON / 2.0)) return ax;
+    double scale = DBL_EPSILON * sqrt(DBL_MIN);
+    if (ax > sqrt(DBL_MAX / 2.0)) {
+        ax *= scale; ay *= scale; scale = 1.0 / scale;
+    } else if (ay < sqrt(DBL_MIN)) {
+        ax /= scale; ay /= scale;
+    } else {
+        scale = 1.0;
+    }
+    double h = sqrt(fma(ax, ax, ay * ay));
+    const double h2 = h * h, ax2 = ax * ax;
+    h -= (fma(-ay, ay, h2 - ax2) + fma(h, h, -h2) - fma(ax, ax, -ax2)) / (2.0 * h);
+    return h * scale;
+}
+
+// Julia Base sqrt(::Complex{Float64})
+__device__ void complex_sqrt_d(double x, double y, double &re, double &im) {
+#pragma clang fp contract(off)
+    if (x == 0.0 && y == 0.0) { re = 0.0; im = y; return; }
+    int k = 0;
+    double rho = x * x + y * y;
+    if (!isfinite(rho) && (isinf(x) || isinf(y))) {
+        rho = INFINITY;
+    } else if (isinf(rho) || (rho == 0.0 && (x != 0.0 || y != 0.0)) ||
+               rho < 4.9406564584124654e-324 / (2.0 * DBL_EPSILON * DBL_EPSILON)) {
+        const double fx = fabs(x), fy = fabs(y);
+        const double m = fx < fy ? fy : fx;
+        k = m == 0.0 ? 0 : ilogb(m);
+        const double xk = ldexp(x, -k), yk = ldexp(y, -k);
+        rho = xk * xk + yk * yk;
+    }
+    if (isfinite(x)) rho = ldexp(fabs(x), -k) + sqrt(rho);
+    if (k % 2 != 0) {
+        k = (k - 1) / 2;
+    } else {
+        k = k / 2 - 1;
+        rho += rho;
+    }
+    rho = ldexp(sqrt(rho), k);
+    double xi = rho, eta = y;
+    if (rho != 0.0) {
+        if (isfinite(eta)) eta = (eta / rho) / 2.0;
+        if (x < 0.0) { xi = fabs(eta); eta = copysign(rho, y); }
+    }
+    re = xi; im = eta;
+}
+
+// bernstein_rho_from_pole (src/utils/bernstein.jl:12-17)
+__device__ double rho_pole_d(double zr, double zi) {
+#pragma clang fp contract(off)
+    double sr, si;
+    complex_sqrt_d(zr * zr - zi * zi - 1.0, zr * zi + zi * zr, sr, si);
+    const double a = hypot_cr_d(zr + sr, zi + si), b = hypot_cr_d(zr - sr, zi - si);
+    return a < b ? b : a;
+}
+
+// bernstein_rho_panel (src/utils/bernstein.jl:33-62); f: panel data (frame at offset 5)
+__device__ double bernstein_rho_d(const double *f, double t0, double t1, double t2) {
+#pragma clang fp contract(off)
+    const double r0 = t0 - f[5], r1 = t1 - f[6], r2 = t2 - f[7];
+    const double xt = r0 * f[8] + r1 * f[9] + r2 * f[10];
+    const double yt = r0 * f[11] + r1 * f[12] + r2 * f[13];
+    const double zt = r0 * f[14] + r1 * f[15] + r2 * f[16];
+    const double L1 = f[17], L2 = f[18];
+    const double cy0 = yt < -L2 ? -L2 : yt;
+    const double cy = L2 < cy0 ? L2 : cy0;
+    const double dy = yt - cy;
+    const double rho1 = rho_pole_d(xt / L1, sqrt(dy * dy + zt * zt) / L1);
+    const double cx0 = xt < -L1 ? -L1 : xt;
+    const double cx = L1 < cx0 ? L1 : cx0;
+    const double dx = xt - cx;
+    const double rho2 = rho_pole_d(yt / L2, sqrt(dx * dx + zt * zt) / L2);
+    return rho2 < rho1 ? rho2 : rho1;
+}
+
+// One workgroup per source panel i, one thread per target panel j in chunks of 256: candidates are
+// target panels with a node in the ball around i (smallest Bernstein radius over those nodes, in
+// node order) or, with correct_edges, a corner within tol_corner of one of i's.  Pass WRITE = false
+// counts them; WRITE = true writes them, ascending j, at ptr[i].
+template <bool WRITE>
+__global__ __launch_bounds__(256) void pair_scan_kernel(DevGeom g, int P, int npp, const double *__restrict__ pdata,
+                                                        const double *__restrict__ corners,
+                                                        const int *__restrict__ is_edge, double tc2, int correct_edges,
+                                                        int64_t *__restrict__ count, const int64_t *__restrict__ ptr,
+                                                        int *__restrict__ ctrg, double *__restrict__ crho,
+                                                        unsigned char *__restrict__ ctouch) {
+#pragma clang fp contract(off)
+    __shared__ int s_wave[4];
+    const int i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (!correct_edges && is_edge[i]) {
+        if (!WRITE && tid == 0) count[i] = 0;
+        return;
+    }
+    double fi[kPairData];
+    for (int k = 0; k < kPairData; ++k) fi[k] = pdata[(size_t)i * kPairData + k];
+    const double cx = fi[0], cy = fi[1], cz = fi[2], ri2 = fi[3], ri = fi[19];
+    int64_t running = 0;
+    for (int base = 0; base < P; base += 256) {
+        const int j = base + tid;
+        bool seen = false, touch = false;
+        double rmin = INFINITY;
+        if (j < P && j != i) {
+            const double *fj = pdata + (size_t)j * kPairData;
+            const double e0 = fj[0] - cx, e1 = fj[1] - cy, e2 = fj[2] - cz;
+            const double dc = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            // no node of j can be in the ball when dc - rad_j > ri (with slack for rounding)
+            if (dc - fj[4] <= ri * (1.0 + 1e-9) + 1e-300) {
+                for (int t = 0; t < npp; ++t) {
+                    const int64_t n = (int64_t)j * npp + t;
+                    const double x = g.x[n], y = g.y[n], z = g.z[n];
+                    const double d0 = x - cx, d1 = y - cy, d2 = z - cz;
+                    if (d0 * d0 + d1 * d1 + d2 * d2 > ri2) continue;
+                    const double rho = bernstein_rho_d(fi, x, y, z);
+                    if (!seen) { seen = true; rmin = rho; }
+                    else if (rho < rmin) rmin = rho;
+                }
+            }
+            if (correct_edges) {
+                const double *ci = corners + (size_t)i * 12, *cj = corners + (size_t)j * 12;
+                for (int a = 0; a < 4 && !touch; ++a)
+                    for (int b = 0; b < 4 && !touch; ++b) {
+                        const double d0 = cj[3 * b] - ci[3 * a], d1 = cj[3 * b + 1] - ci[3 * a + 1],
+                                     d2 = cj[3 * b + 2] - ci[3 * a + 2];
+                        touch = d0 * d0 + d1 * d1 + d2 * d2 <= tc2;
+                    }
+            }
+        }
+        const bool flag = seen || touch;
+        const uint64_t mask = __ballot(flag);
+        const int before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wave[wv] = __popcll(mask);
+        __syncthreads();
+        int woff = 0, total = 0;
+        for (int w = 0; w < 4; ++w) {
+            if (w < wv) woff += s_wave[w];
+            total += s_wave[w];
+        }
+        if (WRITE && flag) {
+            const int64_t o = ptr[i] + running + woff + before;
+            if (o < ptr[i + 1]) {
+                ctrg[o] = j;
+                crho[o] = seen ? rmin : INFINITY;
+                ctouch[o] = touch ? 1 : 0;
+            }
+        }
+        running += total;
+        __syncthreads();
+    }
+    if (!WRITE && tid == 0) count[i] = running;
+}
 }  // namespace
+
+void pair_scan(hipStream_t st, const DevGeom &g, int P, int npp, const double *pdata, const double *corners,
+               const int *is_edge, double tc2, bool correct_edges, int64_t *count, const int64_t *ptr, int *ctrg,
+               double *crho, unsigned char *ctouch) {
+    if (P <= 0) return;
+    if (ptr == nullptr)
+        pair_scan_kernel<false><<<P, 256, 0, st>>>(g, P, npp, pdata, corners, is_edge, tc2, correct_edges ? 1 : 0,
+                                                   count, nullptr, nullptr, nullptr, nullptr);
+    else
+        pair_scan_kernel<true><<<P, 256, 0, st>>>(g, P, npp, pdata, corners, is_edge, tc2, correct_edges ? 1 : 0,
+                                                  nullptr, ptr, ctrg, crho, ctouch);
+}
 
 size_t adaptive_blocks_lds(int nq, int nGL, int max_depth) {
     const size_t npp = (size_t)nq * nq, G2 = (size_t)nGL * nGL;

@@ -16,6 +16,7 @@
 #include <thread>
 
 #include "internal.hpp"
+#include "kernels.hpp"
 
 namespace dbie {
 
@@ -125,108 +126,158 @@ inline int threads_of(int n) { return n > 0 ? n : (int)std::max(1u, std::thread:
 
 }  // namespace
 
-PairList build_pair_list(const Geometry &g, int max_order, double atol, bool correct_edges, int n_threads) {
-    const int64_t P = g.P, N = g.N;
-    const int npp = g.npp();
-    std::vector<double> ctr(3 * P), len(P), off(P);
+PairPrep pair_prep(const Geometry &g, int max_order, double atol, bool correct_edges) {
+    const int64_t P = g.P;
+    PairPrep pp;
+    pp.max_order = max_order;
+    pp.atol = atol;
+    pp.correct_edges = correct_edges;
+    pp.len.resize(P);
+    pp.off.resize(P);
+    pp.pdata.resize((size_t)P * kPairData);
     for (int64_t p = 0; p < P; ++p) {
         const double *c = &g.corners[12 * p];
-        for (int k = 0; k < 3; ++k) ctr[3 * p + k] = (c[k] + c[3 + k] + c[6 + k] + c[9 + k]) / 4;
-        len[p] = std::max(len3(c[0] - c[3], c[1] - c[4], c[2] - c[5]), len3(c[3] - c[6], c[4] - c[7], c[5] - c[8]));
+        pp.len[p] = std::max(len3(c[0] - c[3], c[1] - c[4], c[2] - c[5]), len3(c[3] - c[6], c[4] - c[7], c[5] - c[8]));
         const double *n = &g.normal[3 * p];
-        off[p] = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
+        pp.off[p] = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
     }
-    const double same_tol = std::sqrt(DBL_EPSILON);
     double Lmax = 1.0;
-    for (double l : len) Lmax = std::max(Lmax, l);
+    for (double l : pp.len) Lmax = std::max(Lmax, l);
     const double tol_corner = std::sqrt(DBL_EPSILON) * Lmax;
-    const double log_atol = std::log(atol);
+    pp.tc2 = tol_corner * tol_corner;
+    pp.rho_star = std::pow(atol, -1.0 / (2 * g.nq));
+    const double As = (pp.rho_star + 1.0 / pp.rho_star) / 2, Bs = (pp.rho_star - 1.0 / pp.rho_star) / 2;
+    const int npp = g.npp();
+    for (int64_t p = 0; p < P; ++p) {
+        const double *c = &g.corners[12 * p];
+        double *d = &pp.pdata[(size_t)p * kPairData];
+        const Frame f = make_frame(g, p);
+        for (int k = 0; k < 3; ++k) {
+            d[k] = (c[k] + c[3 + k] + c[6 + k] + c[9 + k]) / 4;  // centre
+            d[5 + k] = f.cc[k];
+            d[8 + k] = f.e1[k];
+            d[11 + k] = f.e2[k];
+            d[14 + k] = f.n[k];
+        }
+        d[17] = f.L1;
+        d[18] = f.L2;
+        const double ri = 105.0 / 100.0 * (pp.len[p] / 2) * std::sqrt(As * As + (1.0 + Bs) * (1.0 + Bs) + Bs * Bs);
+        d[3] = ri * ri;
+        // largest distance from the centre to a node of the panel (target-side pruning)
+        double rad = 0.0;
+        for (int t = 0; t < npp; ++t) {
+            const int64_t n = p * npp + t;
+            rad = std::max(rad, len3(g.x[n] - d[0], g.y[n] - d[1], g.z[n] - d[2]));
+        }
+        d[4] = rad;
+        d[19] = ri;
+    }
+    return pp;
+}
 
-    struct Out {
-        std::vector<int> up_trg, up_ord, adp_trg;
-    };
-    std::vector<Out> out(P);
+namespace {
+// candidate scan of one source panel on the host: target panels with a node in the ball (with
+// the smallest Bernstein radius over those nodes) or a corner within tol_corner, ascending
+void scan_source(const Geometry &g, const PairPrep &pp, int64_t i, std::vector<double> &rho_min,
+                 std::vector<char> &seen, std::vector<char> &touch, std::vector<int> &cand, PairCandidates &out) {
+    const int64_t P = g.P, N = g.N;
+    const int npp = g.npp();
+    cand.clear();
+    const double *d = &pp.pdata[(size_t)i * kPairData];
+    const double ri2 = d[3];
+    const Frame fr = make_frame(g, i);
+    const double cx = d[0], cy = d[1], cz = d[2];
+    // ball query over target nodes (inrange on the node KD-tree, :322)
+    for (int64_t t = 0; t < N; ++t) {
+        const int64_t j = t / npp;
+        if (j == i) continue;
+        const double d0 = g.x[t] - cx, d1 = g.y[t] - cy, d2 = g.z[t] - cz;
+        if (d0 * d0 + d1 * d1 + d2 * d2 > ri2) continue;
+        const double tp[3] = {g.x[t], g.y[t], g.z[t]};
+        const double rho = bernstein_rho(fr, tp);
+        if (!seen[j]) { seen[j] = 1; rho_min[j] = rho; cand.push_back((int)j); }
+        else if (rho < rho_min[j]) rho_min[j] = rho;
+    }
+    // corner-touching panels (Phase B, :335-352)
+    if (pp.correct_edges) {
+        for (int64_t j = 0; j < P; ++j) {
+            if (j == i) continue;
+            bool hit = false;
+            for (int a = 0; a < 4 && !hit; ++a)
+                for (int b = 0; b < 4 && !hit; ++b) {
+                    const double *p = &g.corners[12 * i + 3 * a], *q = &g.corners[12 * j + 3 * b];
+                    const double d0 = q[0] - p[0], d1 = q[1] - p[1], d2 = q[2] - p[2];
+                    hit = d0 * d0 + d1 * d1 + d2 * d2 <= pp.tc2;
+                }
+            if (hit) {
+                touch[j] = 1;
+                if (!seen[j]) { seen[j] = 1; rho_min[j] = INFINITY; cand.push_back((int)j); }
+            }
+        }
+    }
+    std::sort(cand.begin(), cand.end());
+    for (int j : cand) {
+        out.trg.push_back(j);
+        out.rho.push_back(rho_min[j]);
+        out.touch.push_back(touch[j]);
+        seen[j] = 0;
+        touch[j] = 0;
+    }
+}
+}  // namespace
+
+PairList finish_pair_list(const Geometry &g, const PairPrep &pp, const PairCandidates &c) {
+    const int64_t P = g.P;
+    const double same_tol = std::sqrt(DBL_EPSILON);
+    const double log_atol = std::log(pp.atol);
+    PairList pl;
+    for (int64_t i = 0; i < P; ++i) {
+        if (!pp.correct_edges && g.is_edge[i]) continue;
+        const double li = pp.len[i];
+        for (int64_t k = c.ptr[i]; k < c.ptr[i + 1]; ++k) {
+            const int j = c.trg[k];
+            if (!pp.correct_edges && g.is_edge[j]) continue;
+            const double *ni = &g.normal[3 * i], *nj = &g.normal[3 * j];
+            const double dn = ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2];
+            if (dn > 1 - same_tol && std::fabs(pp.off[i] - pp.off[j]) <= same_tol * std::max(1.0, li)) continue;
+            if (pp.correct_edges && c.touch[k]) {
+                pl.adp_src.push_back((int)i);
+                pl.adp_trg.push_back(j);
+                continue;
+            }
+            const double rm = c.rho[k];
+            if (!(rm <= pp.rho_star)) continue;
+            int p_up = rm > 1.0 ? (int)std::ceil(-log_atol / (2 * std::log(rm))) : pp.max_order;
+            p_up = std::min(std::max(p_up, g.nq), pp.max_order);
+            pl.up_src.push_back((int)i);
+            pl.up_trg.push_back(j);
+            pl.up_order.push_back(p_up);
+        }
+    }
+    return pl;
+}
+
+PairList build_pair_list(const Geometry &g, int max_order, double atol, bool correct_edges, int n_threads) {
+    const int64_t P = g.P;
+    const PairPrep pp = pair_prep(g, max_order, atol, correct_edges);
     const int nt = threads_of(n_threads);
     std::vector<std::vector<double>> rho_min_t(nt, std::vector<double>(P));
     std::vector<std::vector<char>> seen_t(nt, std::vector<char>(P)), touch_t(nt, std::vector<char>(P));
     std::vector<std::vector<int>> cand_t(nt);
-
+    std::vector<PairCandidates> per(P);
     parallel_for(P, nt, [&](int64_t i, int tid) {
         if (!correct_edges && g.is_edge[i]) return;
-        auto &rho_min = rho_min_t[tid];
-        auto &seen = seen_t[tid];
-        auto &touch = touch_t[tid];
-        auto &cand = cand_t[tid];
-        cand.clear();
-        const double li = len[i];
-        const double rho_star = std::pow(atol, -1.0 / (2 * g.nq));
-        const double As = (rho_star + 1.0 / rho_star) / 2, Bs = (rho_star - 1.0 / rho_star) / 2;
-        const double ri = 105.0 / 100.0 * (li / 2) * std::sqrt(As * As + (1.0 + Bs) * (1.0 + Bs) + Bs * Bs);
-        const double ri2 = ri * ri;
-        const Frame fr = make_frame(g, i);
-        const double cx = ctr[3 * i], cy = ctr[3 * i + 1], cz = ctr[3 * i + 2];
-        // ball query over target nodes (inrange on the node KD-tree, :322)
-        for (int64_t t = 0; t < N; ++t) {
-            const int64_t j = t / npp;
-            if (j == i) continue;
-            const double d0 = g.x[t] - cx, d1 = g.y[t] - cy, d2 = g.z[t] - cz;
-            if (d0 * d0 + d1 * d1 + d2 * d2 > ri2) continue;
-            const double tp[3] = {g.x[t], g.y[t], g.z[t]};
-            const double rho = bernstein_rho(fr, tp);
-            if (!seen[j]) { seen[j] = 1; rho_min[j] = rho; cand.push_back((int)j); }
-            else if (rho < rho_min[j]) rho_min[j] = rho;
-        }
-        // corner-touching panels (Phase B, :335-352)
-        if (correct_edges) {
-            const double tc2 = tol_corner * tol_corner;
-            for (int64_t j = 0; j < P; ++j) {
-                if (j == i) continue;
-                bool hit = false;
-                for (int a = 0; a < 4 && !hit; ++a)
-                    for (int b = 0; b < 4 && !hit; ++b) {
-                        const double *p = &g.corners[12 * i + 3 * a], *q = &g.corners[12 * j + 3 * b];
-                        const double d0 = q[0] - p[0], d1 = q[1] - p[1], d2 = q[2] - p[2];
-                        hit = d0 * d0 + d1 * d1 + d2 * d2 <= tc2;
-                    }
-                if (hit) {
-                    touch[j] = 1;
-                    if (!seen[j]) { seen[j] = 1; rho_min[j] = INFINITY; cand.push_back((int)j); }
-                }
-            }
-        }
-        Out &o = out[i];
-        std::sort(cand.begin(), cand.end());
-        for (int j : cand) {
-            const bool tj = touch[j];
-            seen[j] = 0;
-            touch[j] = 0;
-            if (!correct_edges && g.is_edge[j]) continue;
-            const double *ni = &g.normal[3 * i], *nj = &g.normal[3 * j];
-            const double dn = ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2];
-            if (dn > 1 - same_tol && std::fabs(off[i] - off[j]) <= same_tol * std::max(1.0, li)) continue;
-            if (correct_edges && tj) { o.adp_trg.push_back(j); continue; }
-            const double rm = rho_min[j];
-            if (!(rm <= rho_star)) continue;
-            int p_up = rm > 1.0 ? (int)std::ceil(-log_atol / (2 * std::log(rm))) : max_order;
-            p_up = std::min(std::max(p_up, g.nq), max_order);
-            o.up_trg.push_back(j);
-            o.up_ord.push_back(p_up);
-        }
+        scan_source(g, pp, i, rho_min_t[tid], seen_t[tid], touch_t[tid], cand_t[tid], per[i]);
     });
-
-    PairList pl;
+    PairCandidates c;
+    c.ptr.assign(P + 1, 0);
+    for (int64_t i = 0; i < P; ++i) c.ptr[i + 1] = c.ptr[i] + (int64_t)per[i].trg.size();
     for (int64_t i = 0; i < P; ++i) {
-        for (size_t k = 0; k < out[i].up_trg.size(); ++k) {
-            pl.up_src.push_back((int)i);
-            pl.up_trg.push_back(out[i].up_trg[k]);
-            pl.up_order.push_back(out[i].up_ord[k]);
-        }
-        for (int j : out[i].adp_trg) {
-            pl.adp_src.push_back((int)i);
-            pl.adp_trg.push_back(j);
-        }
+        c.trg.insert(c.trg.end(), per[i].trg.begin(), per[i].trg.end());
+        c.rho.insert(c.rho.end(), per[i].rho.begin(), per[i].rho.end());
+        c.touch.insert(c.touch.end(), per[i].touch.begin(), per[i].touch.end());
     }
-    return pl;
+    return finish_pair_list(g, pp, c);
 }
 
 namespace {

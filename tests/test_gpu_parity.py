@@ -233,13 +233,14 @@ def test_device_resident_solve(dbie, cfg1):
         assert rel(Sm[:, k], S_host[:, k]) < 1e-8
 
 
-@pytest.mark.parametrize("nq,max_order", [(4, 8), (4, 16), (5, 16)])
-def test_device_built_blocks_match_host(dbie, nq, max_order):
-    """Correction blocks built on the device equal the host-built ones (same sums, same order)."""
-    say(f"device-built vs host-built correction blocks, n_quad={nq}, max_order={max_order}")
+@pytest.mark.parametrize("nq,max_order,edges", [(4, 8, True), (4, 16, True), (5, 16, True), (4, 8, False)])
+def test_device_built_blocks_match_host(dbie, nq, max_order, edges):
+    """Pair list and correction blocks built on the device equal the host-built ones (same
+    decisions, same sums in the same order)."""
+    say(f"device-built vs host-built correction blocks, n_quad={nq}, max_order={max_order}, edges={edges}")
     g = dbie.single_dielectric_box3d(1.0, 0.8, 0.6, nq, 0.2, 4.0, 1.0)
-    opd = dbie.BatchedDielectricOperator(g, max_order=max_order, device_build=True)
-    oph = dbie.BatchedDielectricOperator(g, max_order=max_order, device_build=False)
+    opd = dbie.BatchedDielectricOperator(g, max_order=max_order, correct_edges=edges, device_build=True)
+    oph = dbie.BatchedDielectricOperator(g, max_order=max_order, correct_edges=edges, device_build=False)
     sd, td, bd_ = opd.corrections()
     sh, th, bh = oph.corrections()
     assert np.array_equal(sd, sh) and np.array_equal(td, th)
