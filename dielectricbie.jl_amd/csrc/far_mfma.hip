@@ -694,6 +694,258 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
         }
 }
 
+// Warp-specialised split-FP16 far field for 256-column blocks: 128 targets x 256 columns per
+// workgroup, 32 sources per stage, 12 waves.  Waves 0-3 (one per SIMD) generate the hi/lo A tiles of
+// the next stage into LDS (VALU: 2 target slices x 8 sources per lane; each lane loads its own
+// source coordinates and mask bits two stages ahead into registers); waves 4-11 (two per SIMD) run
+// the MFMAs of the current stage, each on 4 target slices x 4 column blocks, with the stage's B
+// operands copied global -> LDS one stage ahead.  One barrier per stage.  Generation, scaling,
+// splitting and near-pair exclusion are those of far_mfma_f16x3_kernel.
+constexpr int kWsSb = 32;        // sources per stage
+constexpr int kWsThreads = 768;  // 4 producer + 8 consumer waves
+constexpr int kWsMaskWords = 512;  // near-mask words per target panel staged in LDS (16384 source panels)
+
+// Workgroup barrier for the producers: their LDS writes are complete (lgkmcnt(0); s_waitcnt
+// immediate in the gfx9 encoding, vmcnt left at its maximum) while their global loads for later
+// stages stay in flight.  __syncthreads would also wait for those.
+__device__ inline void ws_producer_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    asm volatile("s_barrier" ::: "memory");
+}
+
+template <int EXCL>
+__global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int K, int Kp,
+                                                                  const _Float16 *__restrict__ zh,
+                                                                  const _Float16 *__restrict__ zl,
+                                                                  const unsigned *__restrict__ cmax, float gscale,
+                                                                  double *__restrict__ part, int src_per_split,
+                                                                  NearMask nm, const int *__restrict__ tplane) {
+    static_assert(EXCL == 0 || EXCL == 1, "per-source exclusion runs in far_mfma_f16x3_kernel");
+    __shared__ uint4 s_a[2][8][2][64];   // [stage parity][target slice][hi/lo][lane]
+    __shared__ uint4 s_b[2][16][2][64];  // [stage parity][column block][hi/lo][lane]
+    __shared__ uint32_t s_mask[EXCL ? 8 : 1][EXCL ? kWsMaskWords : 1];  // near-mask rows of the 8 target panels
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool producer = wave < 4;
+    const int64_t N = g.N;
+    const int64_t i0 = (int64_t)blockIdx.x * kTargets;
+    const int k0 = blockIdx.y * 256;
+    const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
+    // tplane (the plane ids of the 256-node tiles) is a restrict kernel argument, so that its
+    // wave-uniform reads go through the scalar cache and never wait on vector loads in flight
+    const StageWalk walk{min(N, jb0 + src_per_split), tplane ? tplane[i0 / 256] : -1, tplane};
+    // Both roles walk the same stages (jc, then j1, j2) and pass one barrier per stage; their
+    // loops are separate so that each keeps only its own state in registers.
+    auto after = [&](int64_t j) { return j < walk.jend ? walk.next(j + kWsSb) : walk.jend; };
+    int64_t jc = walk.next(jb0);
+    int64_t j1 = after(jc);
+
+    if (producer) {
+        const int pw = wave;  // slices 2 pw, 2 pw + 1
+        const int64_t io = i0 < N ? i0 : N - 1;
+        const double ox = g.x[io], oy = g.y[io], oz = g.z[io];
+        // two targets per lane, FP32 coordinates relative to the tile's first target, normals
+        // scaled by gscale
+        float tx[2], ty[2], tz[2], tn0[2], tn1[2], tn2[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            int64_t tg = i0 + (2 * pw + u) * 16 + (lane & 15);
+            tg = tg < N ? tg : N - 1;
+            tx[u] = (float)(g.x[tg] - ox);
+            ty[u] = (float)(g.y[tg] - oy);
+            tz[u] = (float)(g.z[tg] - oz);
+            tn0[u] = g.nxf[tg] * gscale;
+            tn1[u] = g.nyf[tg] * gscale;
+            tn2[u] = g.nzf[tg] * gscale;
+        }
+        // near-mask rows of the 8 target panels over the split's source panels, into LDS (the
+        // host keeps them within kWsMaskWords); read after the first barrier
+        const int64_t wbase = EXCL ? (jb0 / nm.npp) >> 5 : 0;
+        if (EXCL && walk.jend > jb0) {
+            const int nw = (int)((((walk.jend - 1) / nm.npp) >> 5) - wbase + 1);
+            for (int idx = tid; idx < 8 * nw; idx += 256) {
+                const int sl = idx / nw, w = idx - sl * nw;
+                int64_t tg = i0 + sl * 16;
+                tg = tg < N ? tg : N - 1;
+                s_mask[sl][w] = nm.bits[(tg / nm.npp) * (int64_t)nm.mwords + wbase + w];
+            }
+        }
+        // Per stage: this lane's source (8 r + e of row r, lane 16 r + e and 16 r + e + 8) in FP64,
+        // loaded two stages ahead and consumed as FP32 relative coordinates
+        // (the load is unconditional, from a clamped index, and the padding is applied at use, so
+        // that no register is written while its earlier load may be in flight)
+        struct Ahead {
+            double x, y, z;
+            bool in;
+        };
+        auto fetch = [&](int64_t js) {
+            const int64_t j = js + 8 * (lane >> 4) + (lane & 7);
+            const bool in = j < walk.jend;
+            const int64_t jl = in ? j : (walk.jend > 0 ? walk.jend - 1 : 0);
+            return Ahead{g.x[jl], g.y[jl], g.z[jl], in};
+        };
+        // A tiles of stage js (data f) into s_a[pb]
+        auto generate = [&](int pb, int64_t js, const Ahead &f) {
+            const float px = f.in ? (float)(f.x - ox) : 1.0e18f;
+            const float py = f.in ? (float)(f.y - oy) : 1.0e18f;
+            const float pz = f.in ? (float)(f.z - oz) : 1.0e18f;
+            float n0[2], n1[2], n2[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                n0[u] = tn0[u];
+                n1[u] = tn1[u];
+                n2[u] = tn2[u];
+                if (EXCL == 1) {  // the lane's 8 sources lie in panel js / 16 + (lane >> 4) / 2
+                    const int64_t sp0 = js / nm.npp;  // even: both panels' bits lie in one word
+                    const bool ex = (s_mask[2 * pw + u][(sp0 >> 5) - wbase] >> ((sp0 & 31) + ((lane >> 4) >> 1))) & 1u;
+                    n0[u] = ex ? 0.f : n0[u];
+                    n1[u] = ex ? 0.f : n1[u];
+                    n2[u] = ex ? 0.f : n2[u];
+                }
+            }
+            unsigned h[2][4], l[2][4];
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                float v[2][2];
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    const float sx = row_bcast(px, e + d), sy = row_bcast(py, e + d), sz = row_bcast(pz, e + d);
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const float dx = tx[u] - sx, dy = ty[u] - sy, dz = tz[u] - sz;
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-20f)));
+                        const float rinv = __builtin_amdgcn_rsqf(r2);
+                        v[u][d] = (n0[u] * dx + n1[u] * dy + n2[u] * dz) * (rinv * rinv * rinv);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const v2h hh = __builtin_convertvector(v2f{v[u][0], v[u][1]}, v2h);
+                    const v2f hf = __builtin_convertvector(hh, v2f);
+                    const v2h ll = __builtin_convertvector(v2f{v[u][0] - hf[0], v[u][1] - hf[1]}, v2h);
+                    h[u][e / 2] = __builtin_bit_cast(unsigned, hh);
+                    l[u][e / 2] = __builtin_bit_cast(unsigned, ll);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                s_a[pb][2 * pw + u][0][lane] = uint4{h[u][0], h[u][1], h[u][2], h[u][3]};
+                s_a[pb][2 * pw + u][1][lane] = uint4{l[u][0], l[u][1], l[u][2], l[u][3]};
+            }
+        };
+        Ahead fa{}, fb{};
+        if (jc < walk.jend) {
+            const Ahead f0 = fetch(jc);
+            fa = fetch(j1);
+            ws_producer_barrier();
+            generate(0, jc, f0);
+            ws_producer_barrier();
+        }
+        // stage jc (parity C): generate stage j1 into s_a[1 - C] while the consumers multiply stage
+        // jc, and fetch the data of stage j2.  Unrolled by two so that the fetched registers are
+        // never copied (a copy would wait for the loads).
+        auto step = [&](auto cc, const Ahead &cur, Ahead &next) {
+            constexpr int C = decltype(cc)::value;
+            const int64_t j2 = after(j1);
+            next = fetch(j2);
+            generate(1 - C, j1, cur);
+            ws_producer_barrier();
+            jc = j1;
+            j1 = j2;
+        };
+        while (jc < walk.jend) {
+            step(std::integral_constant<int, 0>{}, fa, fb);
+            if (!(jc < walk.jend)) break;
+            step(std::integral_constant<int, 1>{}, fb, fa);
+        }
+        return;
+    }
+
+    // An (empty) statement that clobbers an accumulation register makes the compiler keep the
+    // MFMA accumulators in AGPRs, which leaves the VGPR file to the producers' VALU work on the
+    // same SIMD (with the accumulators in VGPRs the two hardly overlap).
+    asm volatile("" ::: "a0");
+    const int cw = wave - 4;
+    const int sg = cw >> 2, cg = cw & 3;  // target slices 4 sg .. 4 sg + 3, column blocks 4 cg .. 4 cg + 3
+    v4f acc[4][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[s][q] = v4f{0.f, 0.f, 0.f, 0.f};
+    // B operands of stage js go global -> LDS directly (buffer loads: a wave-uniform stage offset
+    // plus a fixed 32-bit lane offset): this wave copies column blocks 4 cg + 2 sg and
+    // 4 cg + 2 sg + 1, its partner wave of the other slice group the other two
+    typedef __attribute__((address_space(3))) void *lptr_t;
+    const int zbytes = (int)(((N + kSb - 1) / kSb) * Kp * kSb * 2);  // the host keeps this below 2^31
+    const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc((void *)zh, (short)0, zbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void *)zl, (short)0, zbytes, 0x00020000);
+    int loff[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) loff[q] = (((4 * cg + 2 * sg + q) * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+    auto dma_b = [&](int64_t js, int pb) {
+        const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int cb = 4 * cg + 2 * sg + q;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rh, (lptr_t)&s_b[pb][cb][0][0], 16, loff[q], soff, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rl, (lptr_t)&s_b[pb][cb][1][0], 16, loff[q], soff, 0, 0);
+        }
+    };
+    if (jc < walk.jend) {
+        dma_b(jc, 0);
+        __syncthreads();
+        __syncthreads();
+    }
+    // stage jc: A in s_a[c], B in s_b[c]; the B operands of stage j1 are copied into s_b[c ^ 1]
+    for (int c = 0; jc < walk.jend; c ^= 1) {
+        const bool more = j1 < walk.jend;
+        const int64_t j2 = after(j1);
+        uint4 bh[4], bl[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            bh[q] = s_b[c][4 * cg + q][0][lane];
+            bl[q] = s_b[c][4 * cg + q][1][lane];
+        }
+        if (more) dma_b(j1, c ^ 1);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const v8h ah = __builtin_bit_cast(v8h, s_a[c][4 * sg + s][0][lane]);
+            const v8h al = __builtin_bit_cast(v8h, s_a[c][4 * sg + s][1][lane]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(v8h, bh[q]), acc[s][q], 0, 0,
+                                                                   0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(v8h, bl[q]), acc[s][q], 0, 0,
+                                                                   0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, __builtin_bit_cast(v8h, bh[q]), acc[s][q], 0, 0,
+                                                                   0);
+        }
+        // keep this stage's MFMAs (and the registers they read) on this side of the barrier
+        __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();  // also waits for stage j1's copies
+        jc = j1;
+        j1 = j2;
+    }
+    const double out_scale = kInv4Pi / (double)gscale;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int col = k0 + (4 * cg + q) * 16 + (lane & 15);
+        if (col >= K) continue;
+        const double cs = ldexp(out_scale, -col_exponent(cmax[col]));
+        double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t i = i0 + (4 * sg + s) * 16 + 4 * (lane >> 4) + r;
+                if (i < N) dst[i] = (double)acc[s][q][r] * cs;
+            }
+    }
+}
+
 // 16-column blocks per workgroup for K columns (power of two, at most 16)
 int ncb_for(int K) {
     const int nb = (K + 15) / 16;
@@ -818,6 +1070,18 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         nm.mwords = g.mask_words;
         const int ex = (!exclude_near || !g.near_mask) ? 0 : (g.npp == 16 ? 1 : 2);
         mark(ev0);
+        const int64_t mask_words_split = (per / std::max(g.npp, 1)) / 32 + 2;
+        if (ncb == 16 && ex != 2 && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
+            (ex == 0 || mask_words_split <= kWsMaskWords)) {
+            if (ex == 1)
+                far_ws_f16x3_kernel<1><<<grid, kWsThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm,
+                                                                     g.tile_plane);
+            else
+                far_ws_f16x3_kernel<0><<<grid, kWsThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm,
+                                                                     g.tile_plane);
+            mark(ev1);
+            return S;
+        }
         switch (ncb * 4 + ex) {
 #define DBIE_H3(n, e) \
     case n * 4 + e: far_mfma_f16x3_kernel<n, e><<<grid, kThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm); break;

@@ -126,29 +126,44 @@ def test_mfma_apply_parity(cfg1):
             assert errs[k] < bound, (prec, k)
 
 
-def test_fp16x3_localized_inputs(cfg1):
+@pytest.mark.parametrize("K", [32, 160])
+def test_fp16x3_localized_inputs(cfg1, K):
     """Inputs concentrated on edge panels (like iterative-refinement corrections): the split-FP16
     operator leaves the near panel pairs to the full FP32 near blocks, so its error stays at the
-    entry level instead of being amplified by the near-pair cancellation."""
+    entry level instead of being amplified by the near-pair cancellation.  K = 160 takes the
+    256-column (warp-specialised) kernel."""
     g, op, ref_if, ref = cfg1
     rng = np.random.default_rng(13)
     _, _, is_edge = g.panels()
     mask = np.repeat(is_edge, g.n_quad ** 2).astype(float)
-    X = rng.standard_normal((g.n_points, 32)) * mask[:, None]
+    X = rng.standard_normal((g.n_points, K)) * mask[:, None]
     Y64 = op @ X
     errs = {}
     for prec in ("fp16x3", "bf16x3"):
         Yp = op.apply(X, prec)
-        errs[prec] = max(rel(Yp[:, k], Y64[:, k]) for k in range(32))
-    say(f"localized inputs: max column error {errs}")
+        errs[prec] = max(rel(Yp[:, k], Y64[:, k]) for k in range(K))
+    say(f"localized inputs K={K}: max column error {errs}")
     assert errs["fp16x3"] < 1e-6
 
 
-def test_fp16x3_column_scaling(cfg1):
+@pytest.mark.parametrize("K", [256, 300])
+def test_fp16x3_wide_blocks(cfg1, K):
+    """256-column blocks (one full block; a full and a partial one) against the FP64 operator."""
+    g, op, ref_if, ref = cfg1
+    say(f"split-FP16 apply K={K} vs FP64")
+    X = np.random.default_rng(14).standard_normal((g.n_points, K))
+    Y64 = op @ X
+    Yp = op.apply(X, "fp16x3")
+    errs = np.array([rel(Yp[:, k], Y64[:, k]) for k in range(K)])
+    say(f"  max column error {errs.max():.3e}")
+    assert errs.max() < 2e-6
+
+
+@pytest.mark.parametrize("K", [32, 256])
+def test_fp16x3_column_scaling(cfg1, K):
     """Columns spanning 40 decades keep their relative accuracy (per-column power-of-two scaling)."""
     g, op, ref_if, ref = cfg1
     rng = np.random.default_rng(12)
-    K = 32
     X = rng.standard_normal((g.n_points, K)) * (10.0 ** np.linspace(-20, 20, K))[None, :]
     X[:, 5] = 0.0  # an all-zero column stays exactly zero
     Y64 = op @ X
