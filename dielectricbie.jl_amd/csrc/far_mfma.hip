@@ -697,19 +697,24 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
 // Warp-specialised split-FP16 far field for 256-column blocks: 128 targets x 256 columns per
 // workgroup, 32 sources per stage, 12 waves.  Waves 0-3 (one per SIMD) generate the hi/lo A tiles of
 // the next stage into LDS (VALU: 2 target slices x 8 sources per lane; each lane loads its own
-// source coordinates and mask bits two stages ahead into registers); waves 4-11 (two per SIMD) run
-// the MFMAs of the current stage, each on 4 target slices x 4 column blocks, with the stage's B
-// operands copied global -> LDS one stage ahead.  One barrier per stage.  Generation, scaling,
-// splitting and near-pair exclusion are those of far_mfma_f16x3_kernel.
+// source coordinates two stages ahead into registers) and copy the B operands of the stage after
+// next global -> LDS (three LDS buffers); waves 4-11 (two per SIMD) run the MFMAs of the current
+// stage, each on 4 target slices x 4 column blocks, reading A and B from LDS only.  One barrier per
+// stage.  Generation, scaling, splitting and near-pair exclusion are those of
+// far_mfma_f16x3_kernel.
 constexpr int kWsSb = 32;        // sources per stage
 constexpr int kWsThreads = 768;  // 4 producer + 8 consumer waves
 constexpr int kWsMaskWords = 512;  // near-mask words per target panel staged in LDS (16384 source panels)
 
-// Workgroup barrier for the producers: their LDS writes are complete (lgkmcnt(0); s_waitcnt
-// immediate in the gfx9 encoding, vmcnt left at its maximum) while their global loads for later
-// stages stay in flight.  __syncthreads would also wait for those.
+// Workgroup barrier for the producers: their LDS writes are complete (lgkmcnt(0)) and all but
+// their newest VM operations (the copies and loads issued for the stage after next) have landed,
+// so that the B operands copied a stage earlier are in LDS for every wave after the barrier.
+// s_waitcnt immediates in the gfx9 encoding (vmcnt [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+// __syncthreads would wait for every VM operation in flight.
+template <int VM>  // VM = 63: no wait on VM operations
 __device__ inline void ws_producer_barrier() {
-    __builtin_amdgcn_s_waitcnt(0xC07F);
+    static_assert((VM >= 0 && VM < 16) || VM == 63, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt(VM == 63 ? 0xC07F : 0x0070 | VM);
     asm volatile("s_barrier" ::: "memory");
 }
 
@@ -722,7 +727,8 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
                                                                   NearMask nm, const int *__restrict__ tplane) {
     static_assert(EXCL == 0 || EXCL == 1, "per-source exclusion runs in far_mfma_f16x3_kernel");
     __shared__ uint4 s_a[2][8][2][64];   // [stage parity][target slice][hi/lo][lane]
-    __shared__ uint4 s_b[2][16][2][64];  // [stage parity][column block][hi/lo][lane]
+    __shared__ uint4 s_b[3][16][2][64];  // [stage % 3][column block][hi/lo][lane]
+    __shared__ double s_pos[3][3][kWsSb];  // [stage % 3][x, y, z][source]
     __shared__ uint32_t s_mask[EXCL ? 8 : 1][EXCL ? kWsMaskWords : 1];  // near-mask rows of the 8 target panels
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool producer = wave < 4;
@@ -769,25 +775,42 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
                 s_mask[sl][w] = nm.bits[(tg / nm.npp) * (int64_t)nm.mwords + wbase + w];
             }
         }
-        // Per stage: this lane's source (8 r + e of row r, lane 16 r + e and 16 r + e + 8) in FP64,
-        // loaded two stages ahead and consumed as FP32 relative coordinates
-        // (the load is unconditional, from a clamped index, and the padding is applied at use, so
-        // that no register is written while its earlier load may be in flight)
-        struct Ahead {
-            double x, y, z;
-            bool in;
+        // Every VM operation of the producers is a global -> LDS copy of the stage after next (B
+        // operands and source coordinates), so the compiler never waits on one for a register; the
+        // barriers wait for the previous stage's copies.  This wave copies column blocks
+        // 4 pw .. 4 pw + 3 (eight 1 KiB copies per stage) and, for pw < 3, coordinate pw of the
+        // stage's 32 sources (one 256-byte copy).
+        typedef __attribute__((address_space(3))) void *lptr_t;
+        const int zbytes = (int)(((N + kSb - 1) / kSb) * Kp * kSb * 2);  // the host keeps this below 2^31
+        const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc((void *)zh, (short)0, zbytes, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void *)zl, (short)0, zbytes, 0x00020000);
+        const double *coord = pw == 0 ? g.x : (pw == 1 ? g.y : g.z);
+        const __amdgpu_buffer_rsrc_t rc =
+            __builtin_amdgcn_make_buffer_rsrc((void *)coord, (short)0, (int)(N * (int64_t)sizeof(double)), 0x00020000);
+        const int loff = ((4 * pw * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+        auto copy_stage = [&](int64_t js, int pb) {
+            const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int cb = 4 * pw + q;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rh, (lptr_t)&s_b[pb][cb][0][0], 16, loff + q * 16 * kSb * 2,
+                                                         soff, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rl, (lptr_t)&s_b[pb][cb][1][0], 16, loff + q * 16 * kSb * 2,
+                                                         soff, 0, 0);
+            }
+            // 64 lanes x 4 bytes = the stage's 32 doubles (reads past the end return zero)
+            if (pw < 3)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rc, (lptr_t)&s_pos[pb][pw][0], 4, lane * 4,
+                                                         (int)(js * (int64_t)sizeof(double)), 0, 0);
         };
-        auto fetch = [&](int64_t js) {
-            const int64_t j = js + 8 * (lane >> 4) + (lane & 7);
-            const bool in = j < walk.jend;
-            const int64_t jl = in ? j : (walk.jend > 0 ? walk.jend - 1 : 0);
-            return Ahead{g.x[jl], g.y[jl], g.z[jl], in};
-        };
-        // A tiles of stage js (data f) into s_a[pb]
-        auto generate = [&](int pb, int64_t js, const Ahead &f) {
-            const float px = f.in ? (float)(f.x - ox) : 1.0e18f;
-            const float py = f.in ? (float)(f.y - oy) : 1.0e18f;
-            const float pz = f.in ? (float)(f.z - oz) : 1.0e18f;
+        // A tiles of stage js (coordinates in s_pos[cb]) into s_a[pb].  Lane 16 r + e (and
+        // 16 r + e + 8) takes source 8 r + e; the row broadcast hands it to the whole row.
+        auto generate = [&](int pb, int cb, int64_t js) {
+            const int sl = 8 * (lane >> 4) + (lane & 7);
+            const bool in = js + sl < walk.jend;
+            const float px = in ? (float)(s_pos[cb][0][sl] - ox) : 1.0e18f;
+            const float py = in ? (float)(s_pos[cb][1][sl] - oy) : 1.0e18f;
+            const float pz = in ? (float)(s_pos[cb][2][sl] - oz) : 1.0e18f;
             float n0[2], n1[2], n2[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -832,30 +855,39 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
                 s_a[pb][2 * pw + u][1][lane] = uint4{l[u][0], l[u][1], l[u][2], l[u][3]};
             }
         };
-        Ahead fa{}, fb{};
+        int bc = 0;  // LDS buffer of stage jc (B operands, coordinates)
         if (jc < walk.jend) {
-            const Ahead f0 = fetch(jc);
-            fa = fetch(j1);
-            ws_producer_barrier();
-            generate(0, jc, f0);
-            ws_producer_barrier();
+            copy_stage(jc, 0);
+            if (j1 < walk.jend) {
+                copy_stage(j1, 1);
+                if (pw < 3) ws_producer_barrier<9>();  // stage jc's copies (and the mask rows) are in
+                else ws_producer_barrier<8>();         // LDS; stage j1's may fly
+            } else {
+                ws_producer_barrier<0>();
+            }
+            generate(0, 0, jc);
+            ws_producer_barrier<63>();
         }
-        // stage jc (parity C): generate stage j1 into s_a[1 - C] while the consumers multiply stage
-        // jc, and fetch the data of stage j2.  Unrolled by two so that the fetched registers are
-        // never copied (a copy would wait for the loads).
-        auto step = [&](auto cc, const Ahead &cur, Ahead &next) {
+        // stage jc (parity C, buffer bc): generate stage j1 into s_a[1 - C] from s_pos[bc + 1] while
+        // the consumers multiply stage jc; copy stage j2 into buffer bc + 2 (mod 3).  Before the
+        // barrier, stage j1's copies have landed; only this stage's may fly.
+        auto step = [&](auto cc) {
             constexpr int C = decltype(cc)::value;
             const int64_t j2 = after(j1);
-            next = fetch(j2);
-            generate(1 - C, j1, cur);
-            ws_producer_barrier();
+            const bool more2 = j2 < walk.jend;
+            if (more2) copy_stage(j2, bc == 0 ? 2 : bc - 1);
+            generate(1 - C, bc == 2 ? 0 : bc + 1, j1);
+            if (!more2) ws_producer_barrier<0>();
+            else if (pw < 3) ws_producer_barrier<9>();
+            else ws_producer_barrier<8>();
             jc = j1;
             j1 = j2;
+            bc = bc == 2 ? 0 : bc + 1;
         };
         while (jc < walk.jend) {
-            step(std::integral_constant<int, 0>{}, fa, fb);
+            step(std::integral_constant<int, 0>{});
             if (!(jc < walk.jend)) break;
-            step(std::integral_constant<int, 1>{}, fb, fa);
+            step(std::integral_constant<int, 1>{});
         }
         return;
     }
@@ -871,41 +903,18 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[s][q] = v4f{0.f, 0.f, 0.f, 0.f};
-    // B operands of stage js go global -> LDS directly (buffer loads: a wave-uniform stage offset
-    // plus a fixed 32-bit lane offset): this wave copies column blocks 4 cg + 2 sg and
-    // 4 cg + 2 sg + 1, its partner wave of the other slice group the other two
-    typedef __attribute__((address_space(3))) void *lptr_t;
-    const int zbytes = (int)(((N + kSb - 1) / kSb) * Kp * kSb * 2);  // the host keeps this below 2^31
-    const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc((void *)zh, (short)0, zbytes, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void *)zl, (short)0, zbytes, 0x00020000);
-    int loff[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) loff[q] = (((4 * cg + 2 * sg + q) * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
-    auto dma_b = [&](int64_t js, int pb) {
-        const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int cb = 4 * cg + 2 * sg + q;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rh, (lptr_t)&s_b[pb][cb][0][0], 16, loff[q], soff, 0, 0);
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rl, (lptr_t)&s_b[pb][cb][1][0], 16, loff[q], soff, 0, 0);
-        }
-    };
     if (jc < walk.jend) {
-        dma_b(jc, 0);
         __syncthreads();
         __syncthreads();
     }
-    // stage jc: A in s_a[c], B in s_b[c]; the B operands of stage j1 are copied into s_b[c ^ 1]
-    for (int c = 0; jc < walk.jend; c ^= 1) {
-        const bool more = j1 < walk.jend;
-        const int64_t j2 = after(j1);
+    // stage jc: A in s_a[c], B in s_b[bc] (the producers have made both complete before the barrier)
+    for (int c = 0, bc = 0; jc < walk.jend; c ^= 1, bc = bc == 2 ? 0 : bc + 1) {
         uint4 bh[4], bl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            bh[q] = s_b[c][4 * cg + q][0][lane];
-            bl[q] = s_b[c][4 * cg + q][1][lane];
+            bh[q] = s_b[bc][4 * cg + q][0][lane];
+            bl[q] = s_b[bc][4 * cg + q][1][lane];
         }
-        if (more) dma_b(j1, c ^ 1);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const v8h ah = __builtin_bit_cast(v8h, s_a[c][4 * sg + s][0][lane]);
@@ -925,9 +934,9 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
         }
         // keep this stage's MFMAs (and the registers they read) on this side of the barrier
         __builtin_amdgcn_sched_barrier(0);
-        __syncthreads();  // also waits for stage j1's copies
+        __syncthreads();
         jc = j1;
-        j1 = j2;
+        j1 = after(j1);
     }
     const double out_scale = kInv4Pi / (double)gscale;
 #pragma unroll
