@@ -919,18 +919,14 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
         for (int s = 0; s < 4; ++s) {
             const v8h ah = __builtin_bit_cast(v8h, s_a[c][4 * sg + s][0][lane]);
             const v8h al = __builtin_bit_cast(v8h, s_a[c][4 * sg + s][1][lane]);
+            // the three products of an accumulator written as one chain: the compiler then keeps
+            // every accumulator in its own AGPRs (it interleaves the chains by itself)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(v8h, bh[q]), acc[s][q], 0, 0,
-                                                                   0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(v8h, bl[q]), acc[s][q], 0, 0,
-                                                                   0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, __builtin_bit_cast(v8h, bh[q]), acc[s][q], 0, 0,
-                                                                   0);
+            for (int q = 0; q < 4; ++q) {
+                v4f t = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(v8h, bh[q]), acc[s][q], 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, __builtin_bit_cast(v8h, bl[q]), t, 0, 0, 0);
+                acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, __builtin_bit_cast(v8h, bh[q]), t, 0, 0, 0);
+            }
         }
         // keep this stage's MFMAs (and the registers they read) on this side of the barrier
         __builtin_amdgcn_sched_barrier(0);
