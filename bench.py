@@ -90,7 +90,7 @@ def reaction_workload(a, dbie, log):
     tm = dbie.TargetPotentialMap(iface, pts, 1e-9)
     t_tm = time.perf_counter() - t1
     log(f"N={iface.n_points} K={K} M={len(q)} operator {t_op:.1f}s target map {t_tm:.1f}s "
-        f"(refined {tm.n_refined}, cubatures {tm.n_pairs}, capped {tm.n_capped})")
+        f"(refined {tm.n_refined}, cubatures {tm.n_pairs}, capped {tm.n_capped}; phases ms {tm.setup_ms()})")
     N = iface.n_points
     dS = dbie.DeviceMatrix(N, K)
 
@@ -122,7 +122,7 @@ def reaction_workload(a, dbie, log):
         "value": K / (float(np.median(lat)) * 1e-3), "unit": "RHS-solves/s", "dtype": a.precision,
         "fp64_value": K / (t64["total_ms"] * 1e-3), "config": {"N": N, "K": K, "M": int(len(q)),
         "N_refined": tm.n_refined, "cubatures": tm.n_pairs},
-        "setup_s": {"operator": t_op, "target_map": t_tm}, "mixed_ms": tmix, "fp64_ms": t64,
+        "setup_s": {"operator": t_op, "target_map": t_tm}, "target_map_phases_ms": tm.setup_ms(), "mixed_ms": tmix, "fp64_ms": t64,
         "mixed_iters": stm["niter"], "mixed_refinements": stm["nrefine"], "fp64_iters": st64["niter"],
         "max_entry_err_vs_fp64_rel_to_max": err, "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99))}), flush=True)

@@ -5,11 +5,13 @@
 //   dbie_reaction_matrix  four_index_matrix (src/solver/multi_rhs.jl:299-321), induced term
 //
 // dbie_tmap_create builds the refined interface, the prolongation and the near-target
-// corrections on the host (targets.cpp, no device needed); the first evaluation uploads them and
-// the device evaluates U or R.
+// corrections on the host (targets.cpp, no device needed); dbie_tmap_create_ex can integrate the
+// near-target cubatures on the device (bit-identical results).  The first evaluation uploads the
+// map and the device evaluates U or R.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <array>
 #include <cstring>
 #include <memory>
 
@@ -184,11 +186,60 @@ extern "C" {
 
 dbie_status dbie_tmap_create(const dbie_geom *geom, int64_t M, const double *trg, double hcub_atol, double range_factor,
                              int n_threads, dbie_tmap **out) {
+    return dbie_tmap_create_ex(geom, M, trg, hcub_atol, range_factor, n_threads, 0, out);
+}
+
+namespace {
+// The near-target cubatures on the device (target_cubatures); pairs whose boxes outgrow the
+// per-workgroup storage come back with status 2 and are integrated on the host.
+void device_cubatures(const TargetCubatureJob &job, const Geometry &g, double *I, int *status) {
+    ensure_device();
+    const int npairs = (int)job.panel.size();
+    const int nq = job.nq, npp = nq * nq;
+    if (npp > kCubatureMaxNpp) fail(DBIE_BAD_ARG, "n_quad too large for the device cubature");
+    hipStream_t st = nullptr;
+    DBIE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    std::vector<double> rules(2 * nq);
+    std::copy(g.glx, g.glx + nq, rules.begin());
+    std::copy(g.bary, g.bary + nq, rules.begin() + nq);
+    const std::array<double, 13> r = genz_malik_rule();
+    CubatureRule rule;
+    rule.l2 = r[0];
+    rule.l3 = r[1];
+    rule.l4 = r[2];
+    rule.l5 = r[3];
+    for (int i = 0; i < 5; ++i) rule.w[i] = r[4 + i];
+    for (int i = 0; i < 4; ++i) rule.wp[i] = r[9 + i];
+    constexpr int kBoxCap = 512, kMaxWg = 2048;
+    const int nwg = std::min(npairs, kMaxWg);
+    DBuf pd, ru, bx, io, so;
+    upload(pd, job.pdat.data(), job.pdat.size(), st);
+    upload(ru, rules.data(), rules.size(), st);
+    bx.ensure(target_cubature_scratch(nwg, nq, kBoxCap));
+    io.ensure(sizeof(double) * (size_t)npairs * npp);
+    so.ensure(sizeof(int) * (size_t)npairs);
+    target_cubatures(st, npairs, nq, pd.as<double>(), ru.as<double>(), rule, job.atol, (long long)job.maxevals, kBoxCap,
+                     nwg, bx.as<double>(), io.as<double>(), so.as<int>());
+    DBIE_HIP(hipGetLastError());
+    DBIE_HIP(hipMemcpyAsync(I, io.p, sizeof(double) * (size_t)npairs * npp, hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipMemcpyAsync(status, so.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipStreamSynchronize(st));
+}
+}  // namespace
+
+dbie_status dbie_tmap_create_ex(const dbie_geom *geom, int64_t M, const double *trg, double hcub_atol,
+                                double range_factor, int n_threads, int device_cubature, dbie_tmap **out) {
     return guard([&] {
         if (!geom || !out) fail(DBIE_BAD_ARG, "NULL geometry or output");
         *out = nullptr;
         std::unique_ptr<dbie_tmap> tm(new dbie_tmap);
-        tm->h.reset(build_target_map(geom->g, M, trg, hcub_atol, range_factor, n_threads));
+        TargetCubatureFn dev;
+        if (device_cubature) dev = device_cubatures;
+        tm->h.reset(build_target_map(geom->g, M, trg, hcub_atol, range_factor, n_threads, dev));
         tm->N = geom->g.N;
         *out = tm.release();
     });
@@ -207,6 +258,14 @@ dbie_status dbie_tmap_sizes(const dbie_tmap *tm, int64_t *M, int64_t *N, int64_t
         if (n_pairs) *n_pairs = h.n_pairs;
         if (nnz_corr) *nnz_corr = (int64_t)h.c_val.size();
         if (n_capped) *n_capped = h.n_capped;
+    });
+}
+
+dbie_status dbie_tmap_setup_ms(const dbie_tmap *tm, double *t) {
+    return guard([&] {
+        check_tm(tm);
+        if (!t) fail(DBIE_BAD_ARG, "NULL output");
+        for (int i = 0; i < 4; ++i) t[i] = tm->h->setup_ms[i];
     });
 }
 

@@ -1,0 +1,30 @@
+// devmath.hpp -- device restatements of host helpers whose floating-point decisions must match
+// the host bit for bit (same operations, same order, no contraction).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dbie {
+
+// barycentric_row (geometry.cpp): row of Lagrange basis values at xq for the nodes x with
+// barycentric weights lam (src/utils/barycentric.jl:6-95)
+__device__ inline void bary_row(int n, const double *x, const double *lam, double xq, double *r) {
+#pragma clang fp contract(off)
+    const double rtol = 0x1p-26;  // sqrt(DBL_EPSILON)
+    for (int i = 0; i < n; ++i) {
+        const double ax = fabs(xq), bx = fabs(x[i]);
+        if (xq == x[i] || fabs(xq - x[i]) <= rtol * (ax < bx ? bx : ax)) {
+            for (int k = 0; k < n; ++k) r[k] = 0.0;
+            r[i] = 1.0;
+            return;
+        }
+    }
+    double den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        r[i] = lam[i] / (xq - x[i]);
+        den += r[i];
+    }
+    const double inv = 1.0 / den;
+    for (int i = 0; i < n; ++i) r[i] *= inv;
+}
+
+}  // namespace dbie

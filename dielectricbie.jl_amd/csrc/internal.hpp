@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <functional>
 #include <cstdint>
 #include <thread>
 #include <string>
@@ -162,9 +164,25 @@ struct TargetMap {
     std::vector<double> c_val;
     int64_t n_pairs = 0;              // (panel, target) cubatures
     int64_t n_capped = 0;             // cubatures that hit the evaluation cap
+    int64_t n_device = 0;             // cubatures integrated on the device
+    double setup_ms[4] = {0, 0, 0, 0};  // refinement, prolongation, pairs + cubatures, assembly
 };
+// The near-target cubatures of a target map: pair k = (refined panel panel[k], target target[k]);
+// pdat: panel centre, b - a, d - a, area scale, target (13 doubles per pair).
+struct TargetCubatureJob {
+    int nq = 0;
+    double atol = 0;
+    int64_t maxevals = 0;
+    std::vector<int> panel, target;
+    std::vector<double> pdat;
+};
+// Optional device integrator: fills I (n_quad^2 per pair) and status (0 converged, 1 stopped at
+// maxevals, 2 not computed: the host integrates those pairs).  g: the coarse geometry (rules).
+// Genz-Malik two-dimensional rule: l2, l3, l4, l5, w[5], wp[4] (the host cubature's constants)
+std::array<double, 13> genz_malik_rule();
+using TargetCubatureFn = std::function<void(const TargetCubatureJob &, const Geometry &g, double *I, int *status)>;
 TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, double atol, double range_factor,
-                            int n_threads);
+                            int n_threads, const TargetCubatureFn &device = nullptr);
 
 }  // namespace dbie
 

@@ -148,4 +148,20 @@ bool build_near_blocks(hipStream_t st, const DevGeom &g, int bd, bool mfma_order
                        const double *caches, int max_nu, int n_adp, const int *adp_src, const int *adp_trg,
                        const int *adp_slot, const double *adp_kup, double *val64, float *val32, float *vup32);
 
+// Near-target cubatures on the device (laplace3d_pottrg_corrections_hcubature): for each pair,
+// the adaptive Genz-Malik cubature over [-1, 1]^2 of the panel potential against the n_quad^2
+// Lagrange basis functions, summed error <= atol, in the host cubature's operation order.
+// pdat: 13 doubles per pair (panel centre, b - a, d - a, area scale, target); rules: glx[nq],
+// bary[nq].  Iout: n_quad^2 per pair; status: 0 converged, 1 stopped at maxevals, 2 more than
+// bcap boxes (not computed).  boxes: target_cubature_scratch(nwg, nq, bcap) bytes.
+constexpr int kCubatureMaxNpp = 64;
+struct CubatureRule {
+    double l2, l3, l4, l5;
+    double w[5], wp[4];
+};
+size_t target_cubature_scratch(int nwg, int nq, int bcap);
+void target_cubatures(hipStream_t st, int npairs, int nq, const double *pdat, const double *rules,
+                      const CubatureRule &rule, double atol, long long maxevals, int bcap, int nwg, double *boxes,
+                      double *Iout, int *status);
+
 }  // namespace dbie

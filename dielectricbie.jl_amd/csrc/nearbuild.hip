@@ -15,6 +15,7 @@
 
 #include <cfloat>
 
+#include "devmath.hpp"
 #include "kernels.hpp"
 
 namespace dbie {
@@ -93,27 +94,6 @@ __global__ __launch_bounds__(256) void near_blocks_kernel(DevGeom g, int bd, int
         vup32[out0 + store_up_idx(bd, mfma != 0, t, c)] = (float)s;
     }
 }
-// barycentric_row (geometry.cpp), same operations: row of Lagrange basis values at xq
-__device__ inline void bary_row(int n, const double *x, const double *lam, double xq, double *r) {
-#pragma clang fp contract(off)
-    const double rtol = 0x1p-26;  // sqrt(DBL_EPSILON)
-    for (int i = 0; i < n; ++i) {
-        const double ax = fabs(xq), bx = fabs(x[i]);
-        if (xq == x[i] || fabs(xq - x[i]) <= rtol * (ax < bx ? bx : ax)) {
-            for (int k = 0; k < n; ++k) r[k] = 0.0;
-            r[i] = 1.0;
-            return;
-        }
-    }
-    double den = 0.0;
-    for (int i = 0; i < n; ++i) {
-        r[i] = lam[i] / (xq - x[i]);
-        den += r[i];
-    }
-    const double inv = 1.0 / den;
-    for (int i = 0; i < n; ++i) r[i] *= inv;
-}
-
 // Touching pairs: the quadtree of walk_adaptive_tree (src/kernel/laplace3d_near_adaptive.jl:119-224)
 // driven by the representative target, depth first with an explicit stack in LDS, and the block
 // moments of adaptive_block_moments_inplace! (:308-387) accumulated leaf by leaf in walk order.

@@ -18,6 +18,8 @@
 // HCubature.jl's hcubature(f, a, b; atol), which the reference calls.  Range queries use a
 // uniform cell grid over the targets with the test |t - c|^2 <= r^2.
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <memory>
@@ -271,8 +273,13 @@ bool hcubature2(const GMRule &g, F &f, int m, double atol, int64_t maxevals, dou
 
 }  // namespace
 
+std::array<double, 13> genz_malik_rule() {
+    const GMRule g;
+    return {g.l2, g.l3, g.l4, g.l5, g.w[0], g.w[1], g.w[2], g.w[3], g.w[4], g.wp[0], g.wp[1], g.wp[2], g.wp[3]};
+}
+
 TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, double atol, double range_factor,
-                            int n_threads) {
+                            int n_threads, const TargetCubatureFn &device) {
     if (M < 0 || (M > 0 && !trg)) fail(DBIE_BAD_ARG, "targets: need M >= 0 and a 3 x M array");
     if (!(atol > 0)) fail(DBIE_BAD_ARG, "hcubature atol must be positive");
     if (!(range_factor > 0)) fail(DBIE_BAD_ARG, "range_factor must be positive");
@@ -286,6 +293,9 @@ TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, dou
     // panel size limit = smallest panel (max side)
     double limit = DBL_MAX;
     for (int64_t p = 0; p < g.P; ++p) limit = std::min(limit, panel_max_length(&g.corners[12 * p]));
+    using Clk = std::chrono::steady_clock;
+    auto ms_since = [](Clk::time_point a) { return std::chrono::duration<double, std::milli>(Clk::now() - a).count(); };
+    auto tp = Clk::now();
     const TargetGrid grid(tm->trg.data(), M, range_factor * limit / nq);
 
     // ---- refinement around the targets (stack, last in first out) ----
@@ -339,21 +349,32 @@ TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, dou
     }
     const Geometry &R = tm->ref;
 
+    tm->setup_ms[0] = ms_since(tp);
+    tp = Clk::now();
     // ---- prolongation (coarse density -> refined nodes) ----
-    tm->p_ptr.assign(1, 0);
+    // a refined node of an unsplit panel copies its coarse node; one of a split panel takes the
+    // tensor Lagrange interpolant of its parent panel (n_quad^2 entries).  Rows are counted first,
+    // then filled in parallel.
+    tm->p_ptr.assign((size_t)Pr * npp + 1, 0);
+    for (int64_t q = 0; q < Pr; ++q)
+        for (int k = 0; k < npp; ++k) tm->p_ptr[q * npp + k + 1] = tm->from_split[q] ? npp : 1;
+    for (int64_t r = 0; r < Pr * npp; ++r) tm->p_ptr[r + 1] += tm->p_ptr[r];
+    tm->p_col.resize(tm->p_ptr.back());
+    tm->p_val.resize(tm->p_ptr.back());
     {
-        double rx[16], ry[16];
-        for (int64_t q = 0; q < Pr; ++q) {
+        const int nt = n_threads > 0 ? n_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+        parallel_for(Pr, nt, [&](int64_t q, int) {
             const int par = tm->parent[q];
             const int64_t col0 = (int64_t)par * npp;
             if (!tm->from_split[q]) {
                 for (int k = 0; k < npp; ++k) {
-                    tm->p_col.push_back((int)(col0 + k));
-                    tm->p_val.push_back(1.0);
-                    tm->p_ptr.push_back((int64_t)tm->p_col.size());
+                    const int64_t e = tm->p_ptr[q * npp + k];
+                    tm->p_col[e] = (int)(col0 + k);
+                    tm->p_val[e] = 1.0;
                 }
-                continue;
+                return;
             }
+            double rx[16], ry[16];
             const double *c = &g.corners[12 * (int64_t)par];
             const double *a = c, *b = c + 3, *d = c + 9;
             double cc[3], bma[3], dma[3];
@@ -376,68 +397,96 @@ TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, dou
                 const double t = inv12 * rhs1 + inv22 * rhs2;
                 barycentric_row(nq, g.glx, g.bary, 2 * s, rx);
                 barycentric_row(nq, g.glx, g.bary, 2 * t, ry);
+                int64_t e = tm->p_ptr[n];
                 for (int ii = 0; ii < nq; ++ii)
-                    for (int jj = 0; jj < nq; ++jj) {
-                        tm->p_col.push_back((int)(col0 + ii * nq + jj));
-                        tm->p_val.push_back(rx[ii] * ry[jj]);
+                    for (int jj = 0; jj < nq; ++jj, ++e) {
+                        tm->p_col[e] = (int)(col0 + ii * nq + jj);
+                        tm->p_val[e] = rx[ii] * ry[jj];
                     }
-                tm->p_ptr.push_back((int64_t)tm->p_col.size());
             }
-        }
+        });
     }
-
+    tm->setup_ms[1] = ms_since(tp);
+    tp = Clk::now();
     // ---- near-target corrections on the refined interface ----
-    struct Trip {
-        int t, c;
-        double v;
-    };
+    // pairs (refined panel, target) in panel order, targets ascending per panel
     const int nt = n_threads > 0 ? n_threads : (int)std::max(1u, std::thread::hardware_concurrency());
-    std::vector<std::vector<Trip>> out(nt);
-    std::vector<int64_t> pairs(nt, 0), capped(nt, 0);
-    const GMRule rule;
-    const int64_t maxevals = 2000000;
-    parallel_for(Pr, nt, [&](int64_t q, int tid) {
+    std::vector<std::vector<int>> near_of(Pr);
+    parallel_for(Pr, nt, [&](int64_t q, int) {
         if (R.is_edge[q] || M == 0) return;
         const double *c = &R.corners[12 * q];
         double cc[3];
         panel_center(c, cc);
         const double l = panel_max_length(c);
-        std::vector<int> near;
+        std::vector<int> &near = near_of[q];
         grid.ball(cc, range_factor * l / nq, [&](int id) { near.push_back(id); return true; });
-        if (near.empty()) return;
         std::sort(near.begin(), near.end());
-        const double *a = c, *b = c + 3, *d = c + 9;
-        double bma[3], dma[3];
-        for (int k = 0; k < 3; ++k) {
-            bma[k] = b[k] - a[k];
-            dma[k] = d[k] - a[k];
+    });
+    TargetCubatureJob job;
+    job.nq = nq;
+    job.atol = atol;
+    job.maxevals = 2000000;
+    for (int64_t q = 0; q < Pr; ++q)
+        for (const int tg : near_of[q]) {
+            job.panel.push_back((int)q);
+            job.target.push_back(tg);
         }
-        const double scale = len3(bma[0], bma[1], bma[2]) * len3(dma[0], dma[1], dma[2]) / 4;
+    std::vector<std::vector<int>>().swap(near_of);
+    const int64_t npairs = (int64_t)job.panel.size();
+    if (npairs > INT32_MAX) fail(DBIE_BAD_ARG, "too many near-target pairs");
+    job.pdat.resize((size_t)npairs * 13);
+    parallel_for(npairs, nt, [&](int64_t k, int) {
+        const double *c = &R.corners[12 * (int64_t)job.panel[k]];
+        double *d = &job.pdat[(size_t)k * 13];
+        panel_center(c, d);
+        for (int i = 0; i < 3; ++i) {
+            d[3 + i] = c[3 + i] - c[i];  // b - a
+            d[6 + i] = c[9 + i] - c[i];  // d - a
+        }
+        d[9] = len3(d[3], d[4], d[5]) * len3(d[6], d[7], d[8]) / 4;
+        for (int i = 0; i < 3; ++i) d[10 + i] = tm->trg[3 * (int64_t)job.target[k] + i];
+    });
+    std::vector<double> I((size_t)npairs * npp);
+    std::vector<int> st(npairs, 2);
+    if (device && npairs > 0) device(job, g, I.data(), st.data());
+    // the host integrates every pair the device did not (all of them without a device)
+    const GMRule rule;
+    parallel_for(npairs, nt, [&](int64_t k, int) {
+        if (st[k] != 2) return;
+        const double *d = &job.pdat[(size_t)k * 13];
+        auto f = [&](double u, double v, double *vals) {
+            double rx[16], ry[16];
+            barycentric_row(nq, g.glx, g.bary, u, rx);
+            barycentric_row(nq, g.glx, g.bary, v, ry);
+            double p[3];
+            for (int i = 0; i < 3; ++i) p[i] = d[i] + d[3 + i] * (u / 2) + d[6 + i] * (v / 2);
+            const double dx = p[0] - d[10], dy = p[1] - d[11], dz = p[2] - d[12];
+            const double kv = (1.0 / std::sqrt(dx * dx + dy * dy + dz * dz)) / kFourPi * d[9];
+            for (int ii = 0; ii < nq; ++ii)
+                for (int jj = 0; jj < nq; ++jj) vals[ii * nq + jj] = kv * rx[ii] * ry[jj];
+        };
         std::vector<Box> heap;
-        double I[kMaxVals];
-        for (const int tg : near) {
-            const double *tp = &tm->trg[3 * (int64_t)tg];
-            auto f = [&](double u, double v, double *vals) {
-                double rx[16], ry[16];
-                barycentric_row(nq, g.glx, g.bary, u, rx);
-                barycentric_row(nq, g.glx, g.bary, v, ry);
-                double p[3];
-                for (int k = 0; k < 3; ++k) p[k] = cc[k] + bma[k] * (u / 2) + dma[k] * (v / 2);
-                const double dx = p[0] - tp[0], dy = p[1] - tp[1], dz = p[2] - tp[2];
-                const double kv = (1.0 / std::sqrt(dx * dx + dy * dy + dz * dz)) / kFourPi * scale;
-                for (int ii = 0; ii < nq; ++ii)
-                    for (int jj = 0; jj < nq; ++jj) vals[ii * nq + jj] = kv * rx[ii] * ry[jj];
-            };
-            if (!hcubature2(rule, f, npp, atol, maxevals, I, heap)) ++capped[tid];
-            ++pairs[tid];
-            for (int k = 0; k < npp; ++k) {
-                const int64_t n = q * npp + k;
-                const double dx = R.x[n] - tp[0], dy = R.y[n] - tp[1], dz = R.z[n] - tp[2];
-                const double direct = R.w[n] * ((1.0 / std::sqrt(dx * dx + dy * dy + dz * dz)) / kFourPi);
-                const double v = I[k] - direct;
-                if (v == 0.0) continue;
-                out[tid].push_back(Trip{tg, (int)n, v});
-            }
+        st[k] = hcubature2(rule, f, npp, atol, job.maxevals, &I[(size_t)k * npp], heap) ? 0 : 1;
+    });
+    tm->setup_ms[2] = ms_since(tp);
+    tp = Clk::now();
+    // corrections: integral minus the direct quadrature term
+    struct Trip {
+        int t, c;
+        double v;
+    };
+    std::vector<std::vector<Trip>> out(nt);
+    parallel_for(npairs, nt, [&](int64_t k, int tid) {
+        const int64_t q = job.panel[k];
+        const int tg = job.target[k];
+        const double *tp = &tm->trg[3 * (int64_t)tg];
+        for (int i = 0; i < npp; ++i) {
+            const int64_t n = q * npp + i;
+            const double dx = R.x[n] - tp[0], dy = R.y[n] - tp[1], dz = R.z[n] - tp[2];
+            const double direct = R.w[n] * ((1.0 / std::sqrt(dx * dx + dy * dy + dz * dz)) / kFourPi);
+            const double v = I[(size_t)k * npp + i] - direct;
+            if (v == 0.0) continue;
+            out[tid].push_back(Trip{tg, (int)n, v});
         }
     });
     size_t total = 0;
@@ -458,8 +507,10 @@ TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, dou
         tm->c_val[e] = all[e].v;
     }
     for (int64_t t = 0; t < M; ++t) tm->c_ptr[t + 1] += tm->c_ptr[t];
-    tm->n_pairs = std::accumulate(pairs.begin(), pairs.end(), (int64_t)0);
-    tm->n_capped = std::accumulate(capped.begin(), capped.end(), (int64_t)0);
+    tm->setup_ms[3] = ms_since(tp);
+    tm->n_pairs = npairs;
+    tm->n_capped = std::count(st.begin(), st.end(), 1);
+    tm->n_device = device ? npairs - std::count(st.begin(), st.end(), 2) : 0;
     return tm.release();
 }
 

@@ -13,8 +13,13 @@
 //        R[a, b] = sum_{s in a} q_s u[Sigma_b](t_s)
 //    becomes R = (P^T (Qt + Qcorr))^T Sigma: one N x G weight block and one DGEMM instead of the
 //    potential at every target (2 M N K flops).  FP64 VALU, LDS-staged source points.
+//  * target_cubatures: the adaptive two-dimensional Genz-Malik cubatures of the near-target
+//    corrections (one workgroup of one wave per (refined panel, target) pair), with every sum in
+//    the order of the host cubature (targets.cpp, hcubature2 / gm_apply), so the integrals and
+//    the evaluation counts are those of the host build.
 #include <hip/hip_runtime.h>
 
+#include "devmath.hpp"
 #include "kernels.hpp"
 
 namespace dbie {
@@ -65,6 +70,210 @@ __global__ void scatter_add_kernel(int64_t n, const int64_t *__restrict__ idx, c
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e < n) Y[idx[e]] += val[e];
 }
+
+// ---- near-target cubatures ------------------------------------------------------------------
+// Box storage per slot: a0, a1, b0, b1, E (-1 once split), kdiv, I[npp].  Slots are appended in
+// creation order, so slot order is the host's sequence order.
+constexpr int kBoxHead = 6;
+
+// One 64-lane workgroup per pair (grid-stride over the pairs).  pdat per pair: panel centre,
+// b - a, d - a, area scale, target (13 doubles).  status: 0 converged, 1 stopped at maxevals,
+// 2 box storage full (the host redoes that pair).
+__global__ __launch_bounds__(64) void target_cubature_kernel(int npairs, int nq, const double *__restrict__ pdat,
+                                                             const double *__restrict__ rules, CubatureRule gm,
+                                                             double atol, long long maxevals, int bcap,
+                                                             double *__restrict__ boxes, double *__restrict__ Iout,
+                                                             int *__restrict__ status) {
+#pragma clang fp contract(off)
+    __shared__ double s_v[34][kCubatureMaxNpp];  // integrand at the 17 points of two boxes
+    __shared__ double s_sq[2][3][kCubatureMaxNpp];  // per-component squares of the two boxes
+    __shared__ double s_ab[2][4];                   // the boxes being evaluated
+    __shared__ double s_red[2][3];                  // E, kdiv, per box
+    __shared__ double s_sel[2];                     // argmax: E, slot
+    const int lane = threadIdx.x;
+    const int npp = nq * nq, stride = kBoxHead + npp;
+    const double *glx = rules, *bary = rules + nq;
+    double *bx = boxes + (size_t)blockIdx.x * bcap * stride;
+
+    for (int pr = blockIdx.x; pr < npairs; pr += gridDim.x) {
+        const double *pd = pdat + (size_t)pr * 13;
+        const double cc0 = pd[0], cc1 = pd[1], cc2 = pd[2], bm0 = pd[3], bm1 = pd[4], bm2 = pd[5];
+        const double dm0 = pd[6], dm1 = pd[7], dm2 = pd[8], scale = pd[9], t0 = pd[10], t1 = pd[11], t2 = pd[12];
+
+        // the integrand at point (u, v): kernel times the tensor Lagrange basis (targets.cpp, f)
+        auto integrand = [&](double u, double v, double *vals) {
+            double rx[16], ry[16];
+            bary_row(nq, glx, bary, u, rx);
+            bary_row(nq, glx, bary, v, ry);
+            const double p0 = cc0 + bm0 * (u / 2) + dm0 * (v / 2);
+            const double p1 = cc1 + bm1 * (u / 2) + dm1 * (v / 2);
+            const double p2 = cc2 + bm2 * (u / 2) + dm2 * (v / 2);
+            const double dx = p0 - t0, dy = p1 - t1, dz = p2 - t2;
+            const double kv = (1.0 / sqrt(dx * dx + dy * dy + dz * dz)) / kFourPiT * scale;
+            for (int ii = 0; ii < nq; ++ii)
+                for (int jj = 0; jj < nq; ++jj) vals[ii * nq + jj] = kv * rx[ii] * ry[jj];
+        };
+        // rule applications on the nb boxes in s_ab: I into slots dst[k], E and kdiv into s_red
+        auto apply = [&](int nb, const int *dst) {
+            if (lane < 17 * nb) {
+                const int k = lane / 17, j = lane - 17 * k;
+                const double *ab = s_ab[k];
+                const double c0 = 0.5 * (ab[0] + ab[2]), c1 = 0.5 * (ab[1] + ab[3]);
+                const double D0 = 0.5 * fabs(ab[2] - ab[0]), D1 = 0.5 * fabs(ab[3] - ab[1]);
+                double u = c0, v = c1;
+                if (j >= 1 && j < 9) {  // axis points: +-D l2, +-D l3 along axis (j - 1) / 4
+                    const int ax = (j - 1) / 4, r = (j - 1) % 4;
+                    const double p = (ax == 0 ? D0 : D1) * (r < 2 ? gm.l2 : gm.l3);
+                    const double pu = ax == 0 ? p : 0.0, pv = ax == 0 ? 0.0 : p;
+                    if (r % 2 == 0) {
+                        u = c0 + pu;
+                        v = c1 + pv;
+                    } else {
+                        u = c0 - pu;
+                        v = c1 - pv;
+                    }
+                } else if (j >= 9) {  // corner points, s0 outer, s1 inner; l4 then l5
+                    const int q = (j - 9) / 2, l5 = (j - 9) % 2;
+                    const double s0 = q < 2 ? -1.0 : 1.0, s1 = q % 2 == 0 ? -1.0 : 1.0;
+                    const double lam = l5 ? gm.l5 : gm.l4;
+                    u = c0 + s0 * D0 * lam;
+                    v = c1 + s1 * D1 * lam;
+                }
+                integrand(u, v, s_v[lane]);
+            }
+            __syncthreads();
+            if (lane < npp * nb) {
+                const int k = lane / npp, i = lane - k * npp;
+                const double *ab = s_ab[k];
+                const double D0 = 0.5 * fabs(ab[2] - ab[0]), D1 = 0.5 * fabs(ab[3] - ab[1]);
+                const double V = D0 * D1;
+                const double(*fv)[kCubatureMaxNpp] = s_v + 17 * k;
+                const double f1 = fv[0][i];
+                double f2 = 0.0, f3 = 0.0, f4 = 0.0, f5 = 0.0;
+                for (int ax = 0; ax < 2; ++ax) {
+                    const double f2i = fv[1 + 4 * ax][i] + fv[2 + 4 * ax][i];
+                    const double f3i = fv[3 + 4 * ax][i] + fv[4 + 4 * ax][i];
+                    f2 += f2i;
+                    f3 += f3i;
+                    const double e = f3i + 12 * f1 - 7 * f2i;  // fourth divided difference
+                    s_sq[k][ax][i] = e * e;
+                }
+                for (int q = 0; q < 4; ++q) {
+                    f4 += fv[9 + 2 * q][i];
+                    f5 += fv[10 + 2 * q][i];
+                }
+                const double I = V * (gm.w[0] * f1 + gm.w[1] * f2 + gm.w[2] * f3 + gm.w[3] * f4 + gm.w[4] * f5);
+                const double Ip = V * (gm.wp[0] * f1 + gm.wp[1] * f2 + gm.wp[2] * f3 + gm.wp[3] * f4);
+                s_sq[k][2][i] = (I - Ip) * (I - Ip);
+                bx[(size_t)dst[k] * stride + kBoxHead + i] = I;
+            }
+            __syncthreads();
+            if (lane < nb) {  // serial sums in component order, as on the host
+                const int k = lane;
+                double d0 = 0.0, d1 = 0.0, e2 = 0.0;
+                for (int i = 0; i < npp; ++i) d0 += s_sq[k][0][i];
+                for (int i = 0; i < npp; ++i) d1 += s_sq[k][1][i];
+                for (int i = 0; i < npp; ++i) e2 += s_sq[k][2][i];
+                d0 = sqrt(d0);
+                d1 = sqrt(d1);
+                double maxdd = 0.0;
+                int kdiv = 0;
+                if (d0 > maxdd) {
+                    maxdd = d0;
+                    kdiv = 0;
+                }
+                if (d1 > maxdd) {
+                    maxdd = d1;
+                    kdiv = 1;
+                }
+                const double E = sqrt(e2);
+                s_red[k][0] = E;
+                s_red[k][1] = (double)kdiv;
+                double *h = bx + (size_t)dst[k] * stride;
+                h[0] = s_ab[k][0];
+                h[1] = s_ab[k][1];
+                h[2] = s_ab[k][2];
+                h[3] = s_ab[k][3];
+                h[4] = E;
+                h[5] = (double)kdiv;
+            }
+            __syncthreads();
+        };
+
+        if (lane == 0) {
+            s_ab[0][0] = -1.0;
+            s_ab[0][1] = -1.0;
+            s_ab[0][2] = 1.0;
+            s_ab[0][3] = 1.0;
+        }
+        __syncthreads();
+        const int d0[2] = {0, 0};
+        apply(1, d0);
+        long long evals = 17;
+        int n = 1, st = 0;
+        double Etot = s_red[0][0];
+        if (!(Etot <= atol || evals >= maxevals)) {
+            for (;;) {
+                // the box with the largest error, the oldest among equal errors
+                double be = -2.0;
+                int bs = 0x7fffffff;
+                for (int sl = lane; sl < n; sl += 64) {
+                    const double e = bx[(size_t)sl * stride + 4];
+                    if (e > be) {
+                        be = e;
+                        bs = sl;
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double oe = __shfl_xor(be, off, 64);
+                    const int os = __shfl_xor(bs, off, 64);
+                    if (oe > be || (oe == be && os < bs)) {
+                        be = oe;
+                        bs = os;
+                    }
+                }
+                if (n + 2 > bcap) {
+                    st = 2;
+                    break;
+                }
+                const double *old = bx + (size_t)bs * stride;
+                const double oE = old[4];
+                const int kd = (int)old[5];
+                if (lane == 0) {
+                    const double a0 = old[0], a1 = old[1], b0 = old[2], b1 = old[3];
+                    const double ak = kd == 0 ? a0 : a1, bk = kd == 0 ? b0 : b1;
+                    const double mid = ak + (bk - ak) / 2;
+                    s_ab[0][0] = a0;
+                    s_ab[0][1] = a1;
+                    s_ab[0][2] = kd == 0 ? mid : b0;
+                    s_ab[0][3] = kd == 0 ? b1 : mid;
+                    s_ab[1][0] = kd == 0 ? mid : a0;
+                    s_ab[1][1] = kd == 0 ? a1 : mid;
+                    s_ab[1][2] = b0;
+                    s_ab[1][3] = b1;
+                }
+                __syncthreads();
+                if (lane == 0) bx[(size_t)bs * stride + 4] = -1.0;  // split: no longer a leaf
+                const int dst[2] = {n, n + 1};
+                apply(2, dst);
+                n += 2;
+                evals += 34;
+                Etot += s_red[0][0] + s_red[1][0] - oE;
+                if (Etot <= atol || evals >= maxevals) break;
+            }
+        }
+        if (st == 0) st = Etot <= atol ? 0 : 1;
+        // sum over the leaves in creation order
+        if (lane < npp) {
+            double acc = 0.0;
+            for (int sl = 0; sl < n; ++sl)
+                if (bx[(size_t)sl * stride + 4] >= 0.0) acc += bx[(size_t)sl * stride + kBoxHead + lane];
+            Iout[(size_t)pr * npp + lane] = acc;
+        }
+        if (lane == 0) status[pr] = st;
+        __syncthreads();
+    }
+}
 }  // namespace
 
 void csr_spmm(hipStream_t st, int64_t R, const int64_t *ptr, const int *col, const double *val, int K, const double *X,
@@ -83,6 +292,18 @@ void reaction_q(hipStream_t st, const DevGeom &g, int G, const int64_t *grp_ptr,
 void scatter_add(hipStream_t st, int64_t n, const int64_t *idx, const double *val, double *Y) {
     if (n <= 0) return;
     scatter_add_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, idx, val, Y);
+}
+
+size_t target_cubature_scratch(int nwg, int nq, int bcap) {
+    return sizeof(double) * (size_t)nwg * bcap * (kBoxHead + nq * nq);
+}
+
+void target_cubatures(hipStream_t st, int npairs, int nq, const double *pdat, const double *rules,
+                      const CubatureRule &rule, double atol, long long maxevals, int bcap, int nwg, double *boxes,
+                      double *Iout, int *status) {
+    if (npairs <= 0) return;
+    target_cubature_kernel<<<(unsigned)nwg, 64, 0, st>>>(npairs, nq, pdat, rules, rule, atol, maxevals, bcap, boxes,
+                                                         Iout, status);
 }
 
 }  // namespace dbie

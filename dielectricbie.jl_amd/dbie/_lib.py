@@ -130,6 +130,8 @@ SIGNATURES = {
     "dbie_copy_block_device": [C.c_int64, C.c_int, _vp, C.c_int64, _vp, C.c_int64],
     "dbie_version": [],
     "dbie_tmap_create": [_vp, C.c_int64, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)],
+    "dbie_tmap_create_ex": [_vp, C.c_int64, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)],
+    "dbie_tmap_setup_ms": [_vp, _dp],
     "dbie_tmap_destroy": [_vp],
     "dbie_tmap_sizes": [_vp, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p],
     "dbie_tmap_refined": [_vp, _dp, _ip, _ip],

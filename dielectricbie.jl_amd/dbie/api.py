@@ -652,18 +652,21 @@ class TargetPotentialMap:
     """The M x N map sigma -> u[sigma](targets) of laplace3d_pottrg_fmm3d_corrected_hcubature
     (src/kernel/laplace3d_near_hcubature.jl:337-375): direct sum over the interface refined around
     the targets plus the adaptive-cubature near-target corrections.  The host builds the
-    refinement and corrections; evaluation runs on the GPU."""
+    refinement and the prolongation; the near-target cubatures run on the GPU
+    (device_cubature=True, bit-identical to the host cubature) or on n_threads host threads;
+    evaluation runs on the GPU."""
 
     def __init__(self, interface: DielectricInterface, targets, hcubature_atol: float, range_factor: float = 5.0,
-                 n_threads: int = 0):
+                 n_threads: int = 0, device_cubature: bool = True):
         t = np.asarray(targets, np.float64)
         if t.ndim == 2 and t.shape[0] == 3 and t.shape[1] != 3:
             t = t.T
         t = np.ascontiguousarray(t.reshape(-1, 3))
         self.M, self.n = t.shape[0], interface.n_points
         h = C.c_void_p()
-        L.check(L.lib().dbie_tmap_create(interface._h, self.M, L.dptr(t.ravel()), float(hcubature_atol),
-                                         float(range_factor), int(n_threads), C.byref(h)))
+        L.check(L.lib().dbie_tmap_create_ex(interface._h, self.M, L.dptr(t.ravel()), float(hcubature_atol),
+                                            float(range_factor), int(n_threads), int(bool(device_cubature)),
+                                            C.byref(h)))
         self._h = h
         self._iface = interface
         v = [C.c_int64() for _ in range(6)]
@@ -681,6 +684,12 @@ class TargetPotentialMap:
     @property
     def shape(self):
         return (self.M, self.n)
+
+    def setup_ms(self):
+        """Build phases in ms: refinement, prolongation, pairs + cubatures, assembly."""
+        t = np.zeros(4)
+        L.check(L.lib().dbie_tmap_setup_ms(self._h, L.dptr(t)))
+        return dict(zip(("refine", "prolong", "cubatures", "assembly"), t.tolist()))
 
     def refined(self):
         """(nodes 4 x N_refined: x, y, z, w; parent panel and from_split per refined panel)."""

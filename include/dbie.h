@@ -207,9 +207,17 @@ typedef struct dbie_tmap dbie_tmap;  /* the M x N map Sigma -> u[Sigma](targets)
    (0 = all cores); no device is needed until the first evaluation.  trg: 3 x M. */
 dbie_status dbie_tmap_create(const dbie_geom *g, int64_t M, const double *trg, double hcub_atol,
                              double range_factor, int n_threads, dbie_tmap **out);
+/* As dbie_tmap_create (same reference function); device_cubature = 1 integrates the near-target
+   cubatures on the GPU (one wave per panel-target pair, the host cubature's operation order, so
+   the map is bit-identical); the refinement and the prolongation stay on the host. */
+dbie_status dbie_tmap_create_ex(const dbie_geom *g, int64_t M, const double *trg, double hcub_atol,
+                                double range_factor, int n_threads, int device_cubature, dbie_tmap **out);
 void dbie_tmap_destroy(dbie_tmap *tm);
 /* sizes: targets, coarse nodes, refined nodes, cubature pairs, correction nonzeros, and
    cubatures that stopped at the evaluation cap (2e6 integrand calls) before reaching hcub_atol */
+/* No counterpart in the reference: wall-clock ms of the build phases (refinement, prolongation,
+   near-target pairs + cubatures, correction assembly), for setup timing. */
+dbie_status dbie_tmap_setup_ms(const dbie_tmap *tm, double *t /* [4] */);
 dbie_status dbie_tmap_sizes(const dbie_tmap *tm, int64_t *M, int64_t *N, int64_t *N_refined,
                             int64_t *n_pairs, int64_t *nnz_corr, int64_t *n_capped);
 /* refined nodes as 4 x N_refined rows (x; y; z; w), parent panel and split flag per refined

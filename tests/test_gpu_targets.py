@@ -96,3 +96,20 @@ def test_reaction_matrix_mixed_vs_fp64(dbie, O):
     inside = np.array([np.all(np.abs(s.positions.T) < np.array([Lx, Ly, Lz]) / 2) for s in srcs])
     Ri = R64[np.ix_(inside, inside)]
     assert inside.sum() >= 4 and np.abs(Ri - Ri.T).max() < 1e-2 * np.abs(Ri).max()
+
+
+@pytest.mark.parametrize("nq,l_ec,atol", [(4, 0.2, 1e-10), (5, 0.2, 1e-9)])
+def test_device_cubatures_match_host(dbie, nq, l_ec, atol):
+    """Near-target cubatures integrated on the device give the host-built map bit for bit."""
+    say(f"target map: device vs host cubatures (n_quad={nq}, atol={atol})")
+    centers = [(0.2, 0.1, 0.05), (0.55, -0.25, 0.27), (-0.58, 0.0, 0.0)]
+    srcs, pos, q, grp = blob_cloud(dbie, centers, 0.02)
+    itf = dbie.single_dielectric_box3d(*BOX1, nq, l_ec, 4.0, 1.0)
+    td = dbie.TargetPotentialMap(itf, pos, atol, device_cubature=True)
+    th = dbie.TargetPotentialMap(itf, pos, atol, device_cubature=False)
+    say(f"  pairs {td.n_pairs}, capped {td.n_capped}, nnz {td.nnz}")
+    assert (td.n_refined, td.n_pairs, td.nnz, td.n_capped) == (th.n_refined, th.n_pairs, th.nnz, th.n_capped)
+    (Pp, Pc, Pv), (Cp, Cc, Cv) = td.matrices()
+    (pp, pc, pv), (cp, cc, cv) = th.matrices()
+    assert np.array_equal(Cp, cp) and np.array_equal(Cc, cc)
+    assert np.array_equal(Cv, cv), np.abs(Cv - cv).max()

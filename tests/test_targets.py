@@ -203,7 +203,7 @@ def test_host_target_map_equals_oracle(dbie, O):
     for (Lx, Ly, Lz, nq, l_ec), pos, atol, rf in _host_cases(dbie):
         itf = dbie.single_dielectric_box3d(Lx, Ly, Lz, nq, l_ec, 4.0, 1.0)
         f = O.Interface.box(Lx, Ly, Lz, nq, l_ec, 4.0, 1.0)
-        tm = dbie.TargetPotentialMap(itf, pos, atol, rf, n_threads=3)
+        tm = dbie.TargetPotentialMap(itf, pos, atol, rf, n_threads=3, device_cubature=False)
         to = O.TargetMap(f, pos, atol, rf)
         assert (tm.n_refined, tm.n_pairs, tm.nnz, tm.n_capped) == (to.N_refined, to.n_pairs, to.nnz, to.n_capped)
         xyzw, par, spl = tm.refined()
@@ -218,9 +218,9 @@ def test_host_target_map_equals_oracle(dbie, O):
 def test_target_map_bad_args(dbie):
     itf = dbie.single_dielectric_box3d(1.0, 1.0, 1.0, 3, 0.3, 4.0, 1.0)
     with pytest.raises(ValueError):
-        dbie.TargetPotentialMap(itf, np.zeros((2, 3)), 0.0)
+        dbie.TargetPotentialMap(itf, np.zeros((2, 3)), 0.0, device_cubature=False)
     with pytest.raises(ValueError):
-        dbie.TargetPotentialMap(itf, np.zeros((2, 3)), 1e-8, range_factor=-1.0)
+        dbie.TargetPotentialMap(itf, np.zeros((2, 3)), 1e-8, range_factor=-1.0, device_cubature=False)
 
 
 def test_oracle_reaction_reciprocity(dbie, O):
