@@ -233,7 +233,7 @@ def main():
             prof = op.profile_read()
             op.profile(False)
             res[prec] = {k: v[1] / max(v[0], 1) for k, v in prof.items() if v[0]}
-            log(f"apply {prec}: per-launch ms {res[prec]}")
+            log(f"apply {prec}: per-launch ms {res[prec]} tuned shape {op.tuned_shape(a.K, prec)}")
         # calibration for HBM-traffic counters: one copy kernel of known size (read 1 GiB, write 1 GiB)
         src = dbie.DeviceMatrix(2 ** 24, 8)
         dst = dbie.DeviceMatrix(2 ** 24, 8)
@@ -321,6 +321,7 @@ def main():
         "mixed_inner_iters": st["niter"], "mixed_refinements": st["nrefine"],
         "mixed_max_rel_residual": st["max_rel_residual"], "setup_s": t_setup, "setup_ms": op.setup_ms(),
         "roofline": roof, "cpu_baseline": cpu,
+        "tuned_shapes": {p: op.tuned_shape(a.K, p) for p in ("fp64", a.precision)},
     }
     print(json.dumps(out), flush=True)
 

@@ -16,6 +16,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <atomic>
+#include <tuple>
+#include <map>
 #include <mutex>
 
 #include "internal.hpp"
@@ -213,9 +216,124 @@ void check_precision(int p) {
 }
 
 // Y = A X on the device (X, Y: N x K with leading dims).
-void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
+// ---- launch-shape autotuner ------------------------------------------------------------------
+// The far-field contraction's source split count (and, for the split-FP16 form at K > 128, the
+// kernel variant) is picked per shape class by timing the candidates on the first apply of that
+// class: (precision, near exclusion, nodes per panel, K in 32-column groups, N to a quarter
+// octave).  The choice is kept for the process.  Different split counts sum the partial sums in
+// a different order, so results can differ from an untuned run in the last bits (well inside
+// every solve tolerance).
+struct TuneKey {
+    int prec, excl, npp, kgroups, nbucket;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(prec, excl, npp, kgroups, nbucket) <
+               std::tie(o.prec, o.excl, o.npp, o.kgroups, o.nbucket);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, FarTune> g_tune;
+std::atomic<int> g_autotune{-1};  // -1: not read yet (DBIE_AUTOTUNE=0 switches it off)
+
+bool autotune_on() {
+    int v = g_autotune.load();
+    if (v < 0) {
+        const char *e = std::getenv("DBIE_AUTOTUNE");
+        v = (e && e[0] == '0') ? 0 : 1;
+        g_autotune.store(v);
+    }
+    return v != 0;
+}
+
+// whether the far field of this apply leaves the near panel pairs out (split-FP16 MFMA form)
+bool excludes_near(const dbie_op *op, int K, dbie_precision prec) {
+    return prec == DBIE_FP16X3 && op->dg.near_mask && op->bsr.valup32 && far_field_mfma_supported((int)prec, K);
+}
+
+TuneKey tune_key(const dbie_op *op, int K, dbie_precision prec, bool exclude) {
+    return TuneKey{(int)prec, exclude ? 1 : 0, op->g.npp(), (K + 31) / 32,
+                   (int)std::lround(4.0 * std::log2((double)std::max<int64_t>(op->N(), 1)))};
+}
+
+// Candidate launch shapes for one apply: split counts the launch accepts whose partial sums fit
+// in 2 GB, each with the default kernel and (split-FP16, K > 128) the general kernel.
+std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
+    std::vector<FarTune> cand;
+    const int smax = far_field_max_splits(N);
+    const bool ws = prec == DBIE_FP16X3 && K > 128;
+    for (int sp : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) {
+        if (sp > smax || (double)sp * K * N * sizeof(double) > 2.0e9) break;
+        cand.push_back(FarTune{sp, -1});
+        if (ws) cand.push_back(FarTune{sp, 0});
+    }
+    return cand;
+}
+
+void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                   const FarTune &tune);
+
+// The tuned launch shape for this apply (tuning it first if its class is new).  Y receives
+// scratch results while the candidates are timed; the caller's apply overwrites it.
+FarTune tuned_shape(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                    bool exclude) {
+    const TuneKey key = tune_key(op, K, prec, exclude);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) return it->second;
+    }
     const int64_t N = op->N();
-    const int64_t need = far_field_part_elems(N, K);
+    const std::vector<FarTune> cand = tune_candidates(N, K, prec);
+    FarTune best = cand.empty() ? FarTune() : cand[0];
+    if (cand.size() > 1) {
+        // buffers at their largest before anything is timed; one untimed apply loads the code
+        op->part.ensure(sizeof(double) * ((int64_t)cand.back().splits * K * N + 64 * N));
+        const bool prof = op->profiling;
+        op->profiling = false;  // tuning launches stay out of the kernel profile
+        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, FarTune());
+        hipEvent_t e0, e1;
+        DBIE_HIP(hipEventCreate(&e0));
+        DBIE_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        for (const FarTune &t : cand) {
+            DBIE_HIP(hipEventRecord(e0, op->stream));
+            apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, t);
+            DBIE_HIP(hipEventRecord(e1, op->stream));
+            DBIE_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DBIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        op->profiling = prof;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+    return best;
+}
+
+void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
+    FarTune t;
+    if (autotune_on() && K > 0) {
+        const bool exclude = excludes_near(op, K, prec);
+        t = tuned_shape(op, K, X, ldx, Y, ldy, prec, exclude);
+    }
+    apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, t);
+}
+
+void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                   const FarTune &tune_in) {
+    const int64_t N = op->N();
+    // a forced split count stays within what the launch accepts and 2 GB of partial sums
+    FarTune tune = tune_in;
+    if (tune.splits > 0) {
+        const int64_t cap = std::max<int64_t>(1, (int64_t)(2.0e9 / (8.0 * std::max(K, 1) * N)));
+        tune.splits = (int)std::min<int64_t>({(int64_t)tune.splits, (int64_t)far_field_max_splits(N), cap});
+    }
+    const int64_t need = std::max<int64_t>(far_field_part_elems(N, K), (int64_t)tune.splits * std::max(K, 1) * N + 64 * N);
     op->part.ensure(need * sizeof(double));
     int S;
     bool exclude = false;  // far field leaves the near panel pairs out (their full interaction is added below)
@@ -233,15 +351,15 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
             if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
         }
         op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
-        exclude = prec == DBIE_FP16X3 && op->dg.near_mask && op->bsr.valup32;
+        exclude = excludes_near(op, K, prec);
         S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p,
-                           op->f16_gscale, exclude, e0, e1);
+                           op->f16_gscale, exclude, e0, e1, tune);
     } else {
         ProfScope ps(op, prec == DBIE_FP64 ? 0 : 1);
         if (prec == DBIE_FP64) {
-            S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need);
+            S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need, tune);
         } else {
-            S = far_field_f32(op->stream, op->dg, K, X, ldx, op->part.as<double>(), need);
+            S = far_field_f32(op->stream, op->dg, K, X, ldx, op->part.as<double>(), need, tune);
         }
     }
     {
@@ -995,6 +1113,50 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
 }
 
 void *dbie_op_stream(dbie_op *op) { return op ? (void *)op->stream : nullptr; }
+
+dbie_status dbie_autotune(int enable, int clear) {
+    return guard([&] {
+        if (enable < -1 || enable > 1) fail(DBIE_BAD_ARG, "enable must be -1, 0 or 1");
+        if (enable >= 0) {
+            autotune_on();  // read the environment first so that it cannot override this call
+            g_autotune.store(enable);
+        }
+        if (clear) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune.clear();
+        }
+    });
+}
+
+dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precision, int *splits, int *variant) {
+    return guard([&] {
+        check_op(op);
+        if (K < 1) fail(DBIE_BAD_ARG, "K must be >= 1");
+        check_precision(precision);
+        const bool exclude = excludes_near(op, K, precision);
+        FarTune t;
+        t.splits = 0;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(tune_key(op, K, precision, exclude));
+            if (it != g_tune.end()) t = it->second;
+        }
+        if (splits) *splits = t.splits;
+        if (variant) *variant = t.variant;
+    });
+}
+
+dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision, int splits, int variant) {
+    return guard([&] {
+        check_op(op);
+        if (K < 1 || splits < 1) fail(DBIE_BAD_ARG, "K and splits must be >= 1");
+        if (variant != -1 && variant != 0) fail(DBIE_BAD_ARG, "variant must be -1 or 0");
+        check_precision(precision);
+        const bool exclude = excludes_near(op, K, precision);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[tune_key(op, K, precision, exclude)] = FarTune{splits, variant};
+    });
+}
 
 dbie_status dbie_op_profile(dbie_op *op, int enable) {
     return guard([&] {

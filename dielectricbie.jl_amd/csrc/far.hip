@@ -141,10 +141,10 @@ int choose_splits(int64_t N, int64_t groups) {
 }
 
 template <class Launch>
-int far_launch(int64_t N, int K, int KC, int64_t part_cap, Launch &&launch) {
+int far_launch(int64_t N, int K, int KC, int64_t part_cap, int forced, Launch &&launch) {
     const int64_t bx = (N + kTile - 1) / kTile;
     const int by = (K + KC - 1) / KC;
-    int S = choose_splits(N, bx * by);
+    int S = forced > 0 ? (int)std::min<int64_t>(forced, std::max<int64_t>(1, N / kTile)) : choose_splits(N, bx * by);
     while (S > 1 && (int64_t)S * K * N > part_cap) --S;
     // splits start on tile boundaries so that the plane table applies
     int64_t per = (N + S - 1) / S;
@@ -161,9 +161,9 @@ int64_t far_field_part_elems(int64_t N, int K) {
 }
 
 int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const double *X, int64_t ldx,
-                  double *part, int64_t part_cap) {
+                  double *part, int64_t part_cap, const FarTune &tune) {
     const int KC = K >= 8 ? 8 : (K >= 4 ? 4 : (K >= 2 ? 2 : 1));
-    return far_launch(g.N, K, KC, part_cap, [&](dim3 grid, int per) {
+    return far_launch(g.N, K, KC, part_cap, tune.splits, [&](dim3 grid, int per) {
 #define DBIE_FAR(M, C) far_f64_kernel<M, C><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per)
 #define DBIE_FAR_KC(M)           \
     switch (KC) {                \
@@ -181,9 +181,9 @@ int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const d
 }
 
 int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int64_t ldx, double *part,
-                  int64_t part_cap) {
+                  int64_t part_cap, const FarTune &tune) {
     const int KC = K >= 8 ? 8 : (K >= 4 ? 4 : (K >= 2 ? 2 : 1));
-    return far_launch(g.N, K, KC, part_cap, [&](dim3 grid, int per) {
+    return far_launch(g.N, K, KC, part_cap, tune.splits, [&](dim3 grid, int per) {
         switch (KC) {
         case 1: far_f32_kernel<1><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
         case 2: far_f32_kernel<2><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;

@@ -960,9 +960,10 @@ int ncb_for(int K) {
 }
 
 // Source split count (splits start on 256-node tiles).
-int mfma_splits(int64_t N, int K, int64_t groups, int64_t part_cap, int64_t *per_out) {
+int mfma_splits(int64_t N, int K, int64_t groups, int64_t part_cap, int forced, int64_t *per_out) {
     int64_t s = std::max<int64_t>(1, (768 + groups - 1) / groups);
     s = std::min<int64_t>(s, std::max<int64_t>(1, N / 2048));
+    if (forced > 0) s = std::min<int64_t>(forced, std::max<int64_t>(1, N / 256));
     while (s > 1 && s * K * N > part_cap) --s;
     int64_t per = (N + s - 1) / s;
     per = (per + 255) / 256 * 256;
@@ -1032,8 +1033,11 @@ size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K) {
     return sizeof(__bf16) * zelems * (prec == 3 ? 2 : 1);
 }
 
+int far_field_max_splits(int64_t N) { return (int)std::max<int64_t>(1, N / 256); }
+
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
-                   int64_t part_cap, void *scratch, float gscale, bool exclude_near, hipEvent_t ev0, hipEvent_t ev1) {
+                   int64_t part_cap, void *scratch, float gscale, bool exclude_near, hipEvent_t ev0, hipEvent_t ev1,
+                   const FarTune &tune) {
     auto mark = [&](hipEvent_t e) {
         if (e) (void)hipEventRecord(e, st);
     };
@@ -1044,7 +1048,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
     const int by = (K + CB - 1) / CB;
     const int Kp = by * CB;
     int64_t per;
-    const int S = mfma_splits(N, K, bx * by, part_cap, &per);
+    const int S = mfma_splits(N, K, bx * by, part_cap, tune.splits, &per);
     dim3 grid((unsigned)bx, (unsigned)by, (unsigned)S);
     if (prec == 0) {
         double *Zt = static_cast<double *>(scratch);
@@ -1076,7 +1080,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         const int ex = (!exclude_near || !g.near_mask) ? 0 : (g.npp == 16 ? 1 : 2);
         mark(ev0);
         const int64_t mask_words_split = (per / std::max(g.npp, 1)) / 32 + 2;
-        if (ncb == 16 && ex != 2 && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
+        if (tune.variant != 0 && ncb == 16 && ex != 2 && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
             (ex == 0 || mask_words_split <= kWsMaskWords)) {
             if (ex == 1)
                 far_ws_f16x3_kernel<1><<<grid, kWsThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm,

@@ -40,12 +40,20 @@ int64_t bsr_storage_index(int bd, bool mfma_order, int t, int c);
 
 enum FarMode { FAR_DT = 0, FAR_S = 1, FAR_D = 2 };
 
+// Launch shape of a far-field contraction, as picked by the autotuner: splits = number of
+// source splits (0: the built-in heuristic), variant = -1: the warp-specialised split-FP16
+// kernel where it applies, 0: never.
+struct FarTune {
+    int splits = 0;
+    int variant = -1;
+};
+
 // Far-field partial sums: part[s][k][i] = sum over sources in split s of G(i,j) w_j X[j,k]
 // (G of the chosen mode, self pair excluded, 1/(4 pi) applied).  Returns the split count S.
 int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const double *X, int64_t ldx,
-                  double *part, int64_t part_cap);
+                  double *part, int64_t part_cap, const FarTune &tune = FarTune());
 int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int64_t ldx, double *part,
-                  int64_t part_cap);
+                  int64_t part_cap, const FarTune &tune = FarTune());
 // MFMA contraction of the DT far field: tiles generated on the fly and contracted with the
 // RHS block.  prec: 0 = FP64 MFMA, 2 = BF16 MFMA, 3 = split BF16 (3 products), 4 = split FP16.
 // scratch: far_field_mfma_scratch_bytes(N, prec, K) bytes of device memory (the
@@ -55,7 +63,9 @@ int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int6
 // ev0/ev1 (optional) are recorded immediately before and after the contraction kernel.
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
                    int64_t part_cap, void *scratch, float gscale, bool exclude_near, hipEvent_t ev0 = nullptr,
-                   hipEvent_t ev1 = nullptr);
+                   hipEvent_t ev1 = nullptr, const FarTune &tune = FarTune());
+// largest useful split count of a far-field launch (splits hold >= 256 sources)
+int far_field_max_splits(int64_t N);
 size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K);
 // Power-of-two scale for the fp16 kernel entries: 2^(14 - e) with max |G_ij| < 2^e over all
 // node pairs.  tmp4: 4 bytes of device memory.  Synchronises the stream; 0 on a HIP error.

@@ -15,6 +15,7 @@ from .api import (  # noqa: F401
     SharpScreening,
     VolumeSource,
     all_weights,
+    autotune,
     batched_lhs_dielectric_box3d_fmm3d_corrected,
     best_grid,
     best_grid_mn,

@@ -279,6 +279,18 @@ class BatchedDielectricOperator:
         L.check(L.lib().dbie_op_setup_ms(self._h, L.dptr(t)))
         return dict(zip(("pairs", "corrections", "derived", "upload"), map(float, t)))
 
+    def tuned_shape(self, K, precision="fp64"):
+        """The autotuner's cached far-field launch shape for K columns: (splits, variant), or None."""
+        sp = C.c_int(); var = C.c_int()
+        L.check(L.lib().dbie_autotune_choice(self._h, int(K), L.PRECISIONS[precision] if isinstance(precision, str)
+                                             else int(precision), C.byref(sp), C.byref(var)))
+        return (sp.value, var.value) if sp.value > 0 else None
+
+    def set_tuned_shape(self, K, precision, splits, variant=-1):
+        """Pin the far-field launch shape for this operator's class at K columns."""
+        L.check(L.lib().dbie_autotune_set(self._h, int(K), L.PRECISIONS[precision] if isinstance(precision, str)
+                                          else int(precision), int(splits), int(variant)))
+
     def profile(self, enable=True):
         """Record HIP events around the far-field and combine launches (kernel timing)."""
         L.check(L.lib().dbie_op_profile(self._h, int(bool(enable))))
@@ -425,6 +437,13 @@ class DeviceMatrix:
                 self.ptr = None
         except Exception:
             pass
+
+
+def autotune(enable=None, clear=False):
+    """Switch the far-field launch-shape autotuner on/off (None: unchanged); clear=True forgets
+    every cached choice.  The tuner times candidate source split counts (and, for "fp16x3" at
+    K > 128, the kernel variant) on the first apply of each shape class."""
+    L.check(L.lib().dbie_autotune(-1 if enable is None else int(bool(enable)), int(bool(clear))))
 
 
 def _op_for(interface, key, **kw):

@@ -304,6 +304,22 @@ dbie_status dbie_server_stats(dbie_server *s, dbie_server_summary *st);
 dbie_status dbie_op_profile(dbie_op *op, int enable);
 dbie_status dbie_op_profile_read(dbie_op *op, int kernel, int64_t *launches, double *total_ms);
 
+/* ---- launch-shape autotuner ---------------------------------------------- */
+/* No counterpart in the reference (its far field is a CPU loop with no launch shape).  The
+   far-field launch shape (source split count; for the split-FP16 form at K > 128 also the kernel
+   variant) is timed on the first apply of each shape class (precision, near exclusion, nodes per
+   panel, K in 32-column groups, N to a quarter octave) and kept for the process.  enable: 1 on
+   (the default; environment DBIE_AUTOTUNE=0 starts it off), 0 off (default shapes), -1 unchanged;
+   clear != 0 forgets the cached choices. */
+dbie_status dbie_autotune(int enable, int clear);
+/* the cached choice for this operator's class at K columns: *splits = 0 when none is cached
+   (either output may be NULL) */
+dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precision, int *splits, int *variant);
+/* pins the choice for this operator's class at K columns (e.g. one saved from an earlier run):
+   splits >= 1 (clamped to what the launch accepts), variant -1 = default kernel, 0 = the general
+   split-FP16 kernel */
+dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision, int splits, int variant);
+
 /* ---- device-pointer variants (stream-ordered on the operator's stream) ---- */
 void *dbie_op_stream(dbie_op *op);
 dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,

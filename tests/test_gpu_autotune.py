@@ -1,0 +1,87 @@
+"""GPU checks of the far-field launch-shape autotuner: every shape it may pick gives the same
+operator (to the bar of its precision), and the tuned shape is cached per shape class.
+
+Bars: 1e-12 relative against the oracle for FP64 applies, 1e-5 for FP32 and 2e-6 for the
+split-FP16 form against the FP64 apply (per column)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CFG1 = (1.2, 0.8, 0.6, 4, 0.2)
+BOUND = {"fp32": 1e-5, "fp16x3": 2e-6}
+
+
+def say(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def col_err(Y, Yref):
+    return max(rel(Y[:, k], Yref[:, k]) for k in range(Y.shape[1]))
+
+
+@pytest.fixture(scope="module")
+def cfg(dbie, O):
+    say("build config-1 operator (N=2560) for the autotuner checks")
+    g = dbie.single_dielectric_box3d(*CFG1, 4.0, 1.0)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
+    ref = O.CorrectedOperator(O.Interface.box(*CFG1, 4.0, 1.0))
+    yield g, op, ref
+    dbie.autotune(True, clear=True)  # pinned shapes stay out of later tests
+
+
+@pytest.mark.parametrize("prec,K", [("fp64", 8), ("fp64", 40), ("fp32", 40), ("fp16x3", 32), ("fp16x3", 256)])
+def test_every_split_count_gives_the_same_operator(dbie, cfg, prec, K):
+    g, op, ref = cfg
+    X = np.random.default_rng(21).standard_normal((g.n_points, K))
+    Yref = ref @ X
+    variants = (-1, 0) if prec == "fp16x3" and K > 128 else (-1,)
+    for splits in (1, 2, 3, 4, 8):
+        for var in variants:
+            say(f"pinned shape {prec} K={K} splits={splits} variant={var}")
+            op.set_tuned_shape(K, prec, splits, var)
+            assert op.tuned_shape(K, prec) == (splits, var)
+            Y = op.apply(X, prec)
+            if prec == "fp64":
+                assert rel(Y, Yref) < 1e-12
+            else:
+                assert col_err(Y, Yref) < BOUND[prec]
+
+
+@pytest.mark.parametrize("prec,K", [("fp64", 24), ("fp16x3", 48), ("fp16x3", 256)])
+def test_tuner_picks_and_caches_a_shape(dbie, cfg, prec, K):
+    g, op, ref = cfg
+    X = np.random.default_rng(22).standard_normal((g.n_points, K))
+    Yref = ref @ X
+    dbie.autotune(True, clear=True)
+    assert op.tuned_shape(K, prec) is None
+    say(f"tuned apply {prec} K={K}")
+    Y1 = op.apply(X, prec)  # first apply of the class times the candidates
+    shape = op.tuned_shape(K, prec)
+    say(f"  chosen shape {shape}")
+    assert shape is not None and shape[0] >= 1
+    Y2 = op.apply(X, prec)  # cached: same shape, same launches, same bits
+    assert np.array_equal(Y1, Y2)
+    assert op.tuned_shape(K, prec) == shape
+    if prec == "fp64":
+        assert rel(Y1, Yref) < 1e-12
+    else:
+        assert col_err(Y1, Yref) < BOUND[prec]
+    # switched off: default shapes, cached choices kept
+    dbie.autotune(False)
+    try:
+        Y3 = op.apply(X, prec)
+        assert op.tuned_shape(K, prec) == shape
+        if prec == "fp64":
+            assert rel(Y3, Y1) < 1e-13
+        else:
+            assert col_err(Y3, Yref) < BOUND[prec]
+    finally:
+        dbie.autotune(True)

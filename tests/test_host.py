@@ -96,6 +96,9 @@ def test_bad_arguments_raise(dbie):
     g = dbie.single_dielectric_box3d(1.0, 1.0, 1.0, 2, 0.5, 4.0, 1.0)
     with pytest.raises(ValueError):
         dbie.build_neighbor_list(g, 8, 2.0)
+    from dbie import _lib
+    assert _lib.lib().dbie_autotune(5, 0) == _lib.BAD_ARG
+    assert _lib.lib().dbie_autotune(-1, 1) == 0
 
 
 def test_no_cpu_fallback_without_gpu(dbie):
