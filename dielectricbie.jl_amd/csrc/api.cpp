@@ -372,6 +372,139 @@ void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
     DBIE_HIP(hipGetLastError());
 }
 
+// ---- screening boxes ---------------------------------------------------------------------------
+// Boxes of permittivity for the screened volume sources, on the host and (after upload) on the device.
+struct HostBoxes {
+    std::vector<double> lo, hi, eps;
+    ScreenBoxes sb;  // device pointers filled by upload_boxes
+};
+
+void check_screen_mode(int mode, double bandwidth) {
+    if (mode < SCREEN_SHARP || mode > SCREEN_SOFT_MIX_INV) fail(DBIE_BAD_ARG, "screening mode must be 0, 1 or 2");
+    if (mode != SCREEN_SHARP && !(bandwidth > 0)) fail(DBIE_BAD_ARG, "bandwidth must be positive");
+}
+
+// screened_volume_source(boxes, epses, eps_out, vs, mode; tol) (src/core/sources.jl:495-515): box b spans
+// centre -/+ sides/2; tol < 0 takes the default sqrt(eps) * the largest side of any box.  form 1 selects
+// the one-region formulas (screened_volume_source(Lx, Ly, Lz, vs, eps_in, eps_out, mode), :368-388).
+HostBoxes box_list(int nb, const double *boxes, const double *epses, double eps_out, int mode, double bandwidth,
+                   double tol, int form) {
+    if (nb < 1 || !boxes || !epses) fail(DBIE_BAD_ARG, "need at least one box and one permittivity per box");
+    if (form != 0 && form != 1) fail(DBIE_BAD_ARG, "form must be 0 (several boxes) or 1 (one region)");
+    if (form == 1 && nb != 1) fail(DBIE_BAD_ARG, "the one-region form takes one box");
+    check_screen_mode(mode, bandwidth);
+    HostBoxes hb;
+    double max_dim = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const double *bx = boxes + 6 * b;
+        for (int k = 0; k < 3; ++k) {
+            hb.lo.push_back(bx[k] - bx[3 + k] / 2);
+            hb.hi.push_back(bx[k] + bx[3 + k] / 2);
+        }
+        max_dim = std::max(max_dim, std::max(std::max(std::fabs(bx[3]), std::fabs(bx[4])), std::fabs(bx[5])));
+        hb.eps.push_back(epses[b]);
+    }
+    hb.sb.nb = nb;
+    hb.sb.eps_out = eps_out;
+    hb.sb.tol = tol >= 0 ? tol : std::sqrt(2.220446049250313e-16) * max_dim;
+    hb.sb.mode = mode;
+    hb.sb.bandwidth = bandwidth;
+    hb.sb.single = form == 1;
+    return hb;
+}
+
+// the interface's bounding box with its uniform permittivities (_interface_box_bounds,
+// _uniform_interface_eps, src/core/sources.jl:283-313) and the default tolerance (:390-400)
+HostBoxes interface_box(const dbie_op *op) {
+    const Geometry &g = op->g;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t p = 0; p < g.P; ++p)
+        for (int c = 0; c < 4; ++c)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], g.corners[12 * p + 3 * c + k]);
+                hi[k] = std::max(hi[k], g.corners[12 * p + 3 * c + k]);
+            }
+    for (int64_t p = 1; p < g.P; ++p)
+        if (op->eps_in[p] != op->eps_in[0] || op->eps_out[p] != op->eps_out[0])
+            fail(DBIE_BAD_ARG, "screened_volume_source requires a uniform eps across the interface");
+    HostBoxes hb;
+    hb.lo.assign(lo, lo + 3);
+    hb.hi.assign(hi, hi + 3);
+    hb.eps.assign(1, op->eps_in[0]);
+    hb.sb.nb = 1;
+    hb.sb.eps_out = op->eps_out[0];
+    hb.sb.tol = std::sqrt(2.220446049250313e-16) *
+                std::max(std::max(std::fabs(hi[0] - lo[0]), std::fabs(hi[1] - lo[1])), std::fabs(hi[2] - lo[2]));
+    hb.sb.mode = SCREEN_SHARP;
+    hb.sb.single = true;
+    return hb;
+}
+
+// copies the box arrays into dbuf (lo, hi, eps back to back) and points sb at them
+ScreenBoxes upload_boxes(const HostBoxes &hb, DBuf &dbuf, hipStream_t st) {
+    std::vector<double> all(hb.lo);
+    all.insert(all.end(), hb.hi.begin(), hb.hi.end());
+    all.insert(all.end(), hb.eps.begin(), hb.eps.end());
+    upload(dbuf, all.data(), all.size(), st);
+    ScreenBoxes sb = hb.sb;
+    sb.lo = dbuf.as<double>();
+    sb.hi = sb.lo + 3 * sb.nb;
+    sb.eps = sb.hi + 3 * sb.nb;
+    return sb;
+}
+
+// screened densities of S host points (device arithmetic)
+void screen_host(hipStream_t st, int64_t S, const double *xyz, const double *rho, const HostBoxes &hb,
+                 double *rho_out) {
+    if (S == 0) return;
+    DBuf dpos, drho, dout, dbox;
+    upload(dpos, xyz, 3 * (size_t)S, st);
+    upload(drho, rho, (size_t)S, st);
+    dout.ensure(sizeof(double) * S);
+    screen_boxes(st, S, dpos.as<double>(), drho.as<double>(), nullptr, upload_boxes(hb, dbox, st), dout.as<double>());
+    DBIE_HIP(hipGetLastError());
+    DBIE_HIP(hipMemcpyAsync(rho_out, dout.p, sizeof(double) * S, hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipStreamSynchronize(st));
+}
+
+// rhs_dielectric_box3d for K volume sources (src/solver/dielectric_box3d.jl:106-124,
+// src/solver/multi_rhs.jl:151-211): column k sums the points col_ptr[k] .. col_ptr[k+1]-1, each
+// with charge w * rho' (rho' screened by hb, or rho itself when hb is NULL), times 1 / eps_src.
+void rhs_volume_host(dbie_op *op, int K, const int64_t *col_ptr, const double *xyz, const double *w,
+                     const double *rho, const HostBoxes *hb, double eps_src, double *F, int64_t ldf) {
+    const int64_t N = op->N();
+    if (K < 1 || !col_ptr || !F) fail(DBIE_BAD_ARG, "K >= 1, col_ptr and F required");
+    if (ldf < N) fail(DBIE_DIM_MISMATCH, "ldf < N");
+    const int64_t S = col_ptr[K];
+    if (col_ptr[0] != 0 || S < 0) fail(DBIE_BAD_ARG, "col_ptr must start at 0");
+    for (int k = 0; k < K; ++k)
+        if (col_ptr[k + 1] < col_ptr[k]) fail(DBIE_BAD_ARG, "col_ptr must be non-decreasing");
+    if (S > 0 && (!xyz || !w || !rho)) fail(DBIE_BAD_ARG, "source arrays required");
+    hipStream_t st = op->stream;
+    DBuf dcp, dpos, dw, drho, dwr, dbox;
+    upload(dcp, col_ptr, (size_t)K + 1, st);
+    upload(dpos, xyz, 3 * (size_t)S, st);
+    upload(dw, w, (size_t)S, st);
+    upload(drho, rho, (size_t)S, st);
+    dwr.ensure(sizeof(double) * std::max<int64_t>(S, 1));
+    if (hb) {
+        screen_boxes(st, S, dpos.as<double>(), drho.as<double>(), dw.as<double>(), upload_boxes(*hb, dbox, st),
+                     dwr.as<double>());
+    } else {
+        // charges w .* rho, 1 / eps_src applied at the end
+        const double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
+        screen_sharp(st, S, dpos.as<double>(), drho.as<double>(), dw.as<double>(), lo, hi, 0.0, 1.0, 1.0,
+                     dwr.as<double>());
+    }
+    op->yout.ensure(sizeof(double) * N * K);
+    rhs_volume(st, op->dg, K, dcp.as<int64_t>(), dpos.as<double>(), dwr.as<double>(), op->yout.as<double>(), N,
+               1.0 / eps_src);
+    DBIE_HIP(hipGetLastError());
+    DBIE_HIP(hipMemcpy2DAsync(F, sizeof(double) * ldf, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
+                              hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipStreamSynchronize(st));
+}
+
 // Column 2-norms of an N x K block (contiguous), returned on the host.
 void col_norms(dbie_op *op, const double *V, int64_t N, int K, DBuf &partial, DBuf &h, std::vector<double> &out) {
     partial.ensure(dots_partial_elems(N, K, 1) * sizeof(double));
@@ -747,6 +880,27 @@ dbie_status dbie_geometry_from_panels(int64_t P, int n_quad, const double *corne
         auto *h = new dbie_geom;
         h->g = std::move(*g);
         *out = h;
+    });
+}
+
+dbie_status dbie_geometry_multi_box(int n_quad, double l_ec, int n_boxes, const double *boxes, const double *epses,
+                                    double eps_out, double alpha, dbie_geom **out) {
+    return guard([&] {
+        if (!out) fail(DBIE_BAD_ARG, "NULL output");
+        std::unique_ptr<dbie_geom> h(new dbie_geom);
+        std::unique_ptr<Geometry> g(
+            make_multi_box(n_quad, l_ec, n_boxes, boxes, epses, eps_out, alpha, h->eps_in, h->eps_out));
+        h->g = std::move(*g);
+        *out = h.release();
+    });
+}
+
+dbie_status dbie_geom_eps(const dbie_geom *gh, double *eps_in, double *eps_out) {
+    return guard([&] {
+        if (!gh) fail(DBIE_BAD_ARG, "geometry handle is NULL");
+        if (gh->eps_in.size() != (size_t)gh->g.P) fail(DBIE_BAD_ARG, "this geometry carries no permittivities");
+        if (eps_in) std::memcpy(eps_in, gh->eps_in.data(), sizeof(double) * gh->g.P);
+        if (eps_out) std::memcpy(eps_out, gh->eps_out.data(), sizeof(double) * gh->g.P);
     });
 }
 
@@ -1322,53 +1476,45 @@ dbie_status dbie_rhs_volume(dbie_op *op, int K, const int64_t *col_ptr, const do
                             const double *rho, int screening, double eps_src, double *F, int64_t ldf) {
     return guard([&] {
         check_op(op);
-        const int64_t N = op->N();
-        if (K < 1 || !col_ptr || !F) fail(DBIE_BAD_ARG, "K >= 1, col_ptr and F required");
-        if (ldf < N) fail(DBIE_DIM_MISMATCH, "ldf < N");
-        const int64_t S = col_ptr[K];
-        if (col_ptr[0] != 0 || S < 0) fail(DBIE_BAD_ARG, "col_ptr must start at 0");
-        for (int k = 0; k < K; ++k)
-            if (col_ptr[k + 1] < col_ptr[k]) fail(DBIE_BAD_ARG, "col_ptr must be non-decreasing");
-        if (S > 0 && (!xyz || !w || !rho)) fail(DBIE_BAD_ARG, "source arrays required");
         if (!screening && eps_src == 0.0) fail(DBIE_BAD_ARG, "eps_src must be nonzero");
-        hipStream_t st = op->stream;
-        DBuf dcp, dpos, dw, drho, dwr;
-        upload(dcp, col_ptr, (size_t)K + 1, st);
-        upload(dpos, xyz, 3 * (size_t)S, st);
-        upload(dw, w, (size_t)S, st);
-        upload(drho, rho, (size_t)S, st);
-        dwr.ensure(sizeof(double) * std::max<int64_t>(S, 1));
-        double scale = 1.0;
-        if (screening) {
-            const Geometry &g = op->g;
-            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (int64_t p = 0; p < g.P; ++p)
-                for (int c = 0; c < 4; ++c)
-                    for (int k = 0; k < 3; ++k) {
-                        lo[k] = std::min(lo[k], g.corners[12 * p + 3 * c + k]);
-                        hi[k] = std::max(hi[k], g.corners[12 * p + 3 * c + k]);
-                    }
-            for (int64_t p = 1; p < g.P; ++p)
-                if (op->eps_in[p] != op->eps_in[0] || op->eps_out[p] != op->eps_out[0])
-                    fail(DBIE_BAD_ARG, "screened_volume_source requires a uniform eps across the interface");
-            const double tol = std::sqrt(2.220446049250313e-16) *
-                               std::max(std::max(std::fabs(hi[0] - lo[0]), std::fabs(hi[1] - lo[1])), std::fabs(hi[2] - lo[2]));
-            screen_sharp(st, S, dpos.as<double>(), drho.as<double>(), dw.as<double>(), lo, hi, tol, op->eps_in[0],
-                         op->eps_out[0], dwr.as<double>());
-        } else {
-            // charges w.*rho, 1/eps_src applied at the end
-            const double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
-            screen_sharp(st, S, dpos.as<double>(), drho.as<double>(), dw.as<double>(), lo, hi, 0.0, 1.0, 1.0,
-                         dwr.as<double>());
-            scale = 1.0 / eps_src;
-        }
-        op->yout.ensure(sizeof(double) * N * K);
-        rhs_volume(st, op->dg, K, dcp.as<int64_t>(), dpos.as<double>(), dwr.as<double>(), op->yout.as<double>(), N,
-                   scale);
-        DBIE_HIP(hipGetLastError());
-        DBIE_HIP(hipMemcpy2DAsync(F, sizeof(double) * ldf, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
-                                  hipMemcpyDeviceToHost, st));
-        DBIE_HIP(hipStreamSynchronize(st));
+        HostBoxes hb;
+        if (screening) hb = interface_box(op);
+        rhs_volume_host(op, K, col_ptr, xyz, w, rho, screening ? &hb : nullptr, screening ? 1.0 : eps_src, F, ldf);
+    });
+}
+
+dbie_status dbie_rhs_volume_boxes(dbie_op *op, int K, const int64_t *col_ptr, const double *xyz, const double *w,
+                                  const double *rho, int n_boxes, const double *boxes, const double *epses,
+                                  double eps_out, int mode, double bandwidth, double *F, int64_t ldf) {
+    return guard([&] {
+        check_op(op);
+        const HostBoxes hb = box_list(n_boxes, boxes, epses, eps_out, mode, bandwidth, -1.0, 0);
+        rhs_volume_host(op, K, col_ptr, xyz, w, rho, &hb, 1.0, F, ldf);
+    });
+}
+
+dbie_status dbie_screen_boxes(int64_t S, const double *xyz, const double *rho, int n_boxes, const double *boxes,
+                              const double *epses, double eps_out, int mode, double bandwidth, double tol, int form,
+                              double *rho_out) {
+    return guard([&] {
+        ensure_device();
+        if (S < 0 || (S > 0 && (!xyz || !rho || !rho_out))) fail(DBIE_BAD_ARG, "S >= 0 and source arrays required");
+        const HostBoxes hb = box_list(n_boxes, boxes, epses, eps_out, mode, bandwidth, tol, form);
+        screen_host(nullptr, S, xyz, rho, hb, rho_out);
+    });
+}
+
+dbie_status dbie_screen_interface(dbie_op *op, int64_t S, const double *xyz, const double *rho, int mode,
+                                  double bandwidth, double tol, double *rho_out) {
+    return guard([&] {
+        check_op(op);
+        if (S < 0 || (S > 0 && (!xyz || !rho || !rho_out))) fail(DBIE_BAD_ARG, "S >= 0 and source arrays required");
+        HostBoxes hb = interface_box(op);
+        check_screen_mode(mode, bandwidth);
+        hb.sb.mode = mode;
+        hb.sb.bandwidth = bandwidth;
+        if (tol >= 0) hb.sb.tol = tol;
+        screen_host(op->stream, S, xyz, rho, hb, rho_out);
     });
 }
 

@@ -4,6 +4,7 @@
 // Node coordinates feed 1e-12 operator parity, so every formula keeps the
 // reference's arithmetic order and this file is compiled with
 // -ffp-contract=off (no fused multiply-add where the reference has none).
+#include <algorithm>
 #include <array>
 #include <cfloat>
 #include <cmath>
@@ -216,53 +217,298 @@ void fill_nodes(Geometry &g) {
 }
 }  // namespace
 
+namespace {
+// rect_panel3d_adaptive_panels (src/shape/box3d.jl:99-133): a best_grid rough split of the root,
+// then 2x2 splits (first in, first out) of every edge/corner panel longer than l_ec.  Appends the
+// fine panels with normal nrm to g.
+void append_adaptive(Geometry &g, const Cell &root, const V3 &nrm, double alpha, double l_ec) {
+    const double Lab = len3(root.b[0] - root.a[0], root.b[1] - root.a[1], root.b[2] - root.a[2]);
+    const double Lda = len3(root.a[0] - root.d[0], root.a[1] - root.d[1], root.a[2] - root.d[2]);
+    int nx, ny;
+    if (!best_grid(Lab, Lda, alpha, &nx, &ny)) fail(DBIE_BAD_ARG, "best_grid found no tiling");
+    std::vector<Cell> scratch;
+    split_cell(root, nx, ny, scratch);
+    std::deque<Cell> queue(scratch.begin(), scratch.end());  // popfirst! / append!
+    while (!queue.empty()) {
+        const Cell t = queue.front();
+        queue.pop_front();
+        const double lab = len3(t.b[0] - t.a[0], t.b[1] - t.a[1], t.b[2] - t.a[2]);
+        const double lda = len3(t.a[0] - t.d[0], t.a[1] - t.d[1], t.a[2] - t.d[2]);
+        if (t.any() && std::max(lab, lda) > l_ec) {
+            scratch.clear();
+            split_cell(t, 2, 2, scratch);
+            queue.insert(queue.end(), scratch.begin(), scratch.end());
+        } else {
+            for (const V3 *v : {&t.a, &t.b, &t.c, &t.d}) g.corners.insert(g.corners.end(), v->begin(), v->end());
+            g.normal.insert(g.normal.end(), nrm.begin(), nrm.end());
+            g.is_edge.push_back(t.any() ? 1 : 0);
+        }
+    }
+}
+
+void check_mesh_args(int nq, double l_ec) {
+    if (!(l_ec > 0)) fail(DBIE_BAD_ARG, "l_ec must be positive");
+    if (nq < 1 || nq > 8) fail(DBIE_BAD_ARG, "n_quad must be in 1..8");
+}
+
+// corner order of the six faces of a box and their outward normals (src/shape/box3d_single.jl:5-32,
+// src/shape/box3d_multi.jl:11-27): vertices 0..7 = (+,+,+) (-,+,+) (-,-,+) (+,-,+) (+,+,-) (-,+,-)
+// (-,-,-) (+,-,-) relative to the centre
+constexpr int kFaceVtx[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {7, 4, 0, 3}, {6, 2, 1, 5}, {5, 1, 0, 4}, {6, 7, 3, 2}};
+const V3 kFaceNrm[6] = {{0, 0, 1}, {0, 0, -1}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+}  // namespace
+
 // single_dielectric_box3d (src/shape/box3d_single.jl:1-70) with the per-face
 // rect_panel3d_adaptive_panels refinement (src/shape/box3d.jl:99-133).
 Geometry *make_box(double Lx, double Ly, double Lz, int nq, double l_ec, double alpha) {
     if (!(Lx > 0 && Ly > 0 && Lz > 0)) fail(DBIE_BAD_ARG, "box sides must be positive");
-    if (!(l_ec > 0)) fail(DBIE_BAD_ARG, "l_ec must be positive");
-    if (nq < 1 || nq > 8) fail(DBIE_BAD_ARG, "n_quad must be in 1..8");
+    check_mesh_args(nq, l_ec);
     const V3 vtx[8] = {{Lx / 2, Ly / 2, Lz / 2},   {-Lx / 2, Ly / 2, Lz / 2},  {-Lx / 2, -Ly / 2, Lz / 2},
                        {Lx / 2, -Ly / 2, Lz / 2},  {Lx / 2, Ly / 2, -Lz / 2},  {-Lx / 2, Ly / 2, -Lz / 2},
                        {-Lx / 2, -Ly / 2, -Lz / 2}, {Lx / 2, -Ly / 2, -Lz / 2}};
-    const int faces[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {7, 4, 0, 3}, {6, 2, 1, 5}, {5, 1, 0, 4}, {6, 7, 3, 2}};
-    const V3 nrm[6] = {{0, 0, 1}, {0, 0, -1}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
-
     std::unique_ptr<Geometry> g(new Geometry);
     g->nq = nq;
     init_rules(*g);
-    std::vector<Cell> fine, scratch;
     for (int f = 0; f < 6; ++f) {
-        Cell root{vtx[faces[f][0]], vtx[faces[f][1]], vtx[faces[f][2]], vtx[faces[f][3]],
-                  true, true, true, true, true, true, true, true};
-        const double Lab = len3(root.b[0] - root.a[0], root.b[1] - root.a[1], root.b[2] - root.a[2]);
-        const double Lda = len3(root.a[0] - root.d[0], root.a[1] - root.d[1], root.a[2] - root.d[2]);
-        int nx, ny;
-        if (!best_grid(Lab, Lda, alpha, &nx, &ny)) fail(DBIE_BAD_ARG, "best_grid found no tiling");
-        scratch.clear();
-        split_cell(root, nx, ny, scratch);
-        std::deque<Cell> queue(scratch.begin(), scratch.end());  // popfirst! / append!
-        fine.clear();
-        while (!queue.empty()) {
-            const Cell t = queue.front();
-            queue.pop_front();
-            const double lab = len3(t.b[0] - t.a[0], t.b[1] - t.a[1], t.b[2] - t.a[2]);
-            const double lda = len3(t.a[0] - t.d[0], t.a[1] - t.d[1], t.a[2] - t.d[2]);
-            if (t.any() && std::max(lab, lda) > l_ec) {
-                scratch.clear();
-                split_cell(t, 2, 2, scratch);
-                queue.insert(queue.end(), scratch.begin(), scratch.end());
-            } else {
-                fine.push_back(t);
-            }
-        }
-        for (const Cell &t : fine) {
-            for (const V3 *v : {&t.a, &t.b, &t.c, &t.d}) g->corners.insert(g->corners.end(), v->begin(), v->end());
-            g->normal.insert(g->normal.end(), nrm[f].begin(), nrm[f].end());
-            g->is_edge.push_back(t.any() ? 1 : 0);
-        }
+        const Cell root{vtx[kFaceVtx[f][0]], vtx[kFaceVtx[f][1]], vtx[kFaceVtx[f][2]], vtx[kFaceVtx[f][3]],
+                        true, true, true, true, true, true, true, true};
+        append_adaptive(*g, root, kFaceNrm[f], alpha, l_ec);
     }
     g->P = (int64_t)g->is_edge.size();
+    fill_nodes(*g);
+    return g.release();
+}
+
+// ---- several touching boxes (src/shape/box3d_multi.jl) -----------------------------------------
+namespace {
+using Quad = std::array<V3, 4>;  // corners a, b, c, d
+constexpr double kFaceTol = 1e-10;
+
+// Julia's min and max on floats (-0.0 orders below 0.0)
+double jmin(double x, double y) { return (y < x || (y == x && std::signbit(y))) ? y : x; }
+double jmax(double x, double y) { return (y > x || (y == x && !std::signbit(y))) ? y : x; }
+
+// in-plane axes (u, v) and normal axis of an axis-aligned rectangle
+struct Axes {
+    int u, v, n;
+};
+Axes plane_axes(const V3 &nrm) {
+    const double a0 = std::fabs(nrm[0]), a1 = std::fabs(nrm[1]), a2 = std::fabs(nrm[2]);
+    Axes ax = a0 > 0.5 ? Axes{1, 2, 0} : (a1 > 0.5 ? Axes{0, 2, 1} : Axes{0, 1, 2});
+    if (!(a0 > 0.5 || a1 > 0.5 || a2 > 0.5)) fail(DBIE_BAD_ARG, "box faces must be axis-aligned");
+    ax.n = a0 > 0.5 ? 0 : (a1 > 0.5 ? 1 : 2);
+    return ax;
+}
+
+V3 in_plane(const Axes &ax, double u, double v, double w) {
+    V3 p{0.0, 0.0, 0.0};
+    p[ax.u] = u;
+    p[ax.v] = v;
+    p[ax.n] = w;
+    return p;
+}
+
+// extent of a rectangle along one axis
+void extent(const Quad &q, int k, double &lo, double &hi) {
+    lo = jmin(jmin(jmin(q[0][k], q[1][k]), q[2][k]), q[3][k]);
+    hi = jmax(jmax(jmax(q[0][k], q[1][k]), q[2][k]), q[3][k]);
+}
+
+// The six faces of a box (centre c, sides L) (_box3d_faces_at_center, src/shape/box3d_multi.jl:4-30).
+void box_faces(const double *box, Quad faces[6]) {
+    const double cx = box[0], cy = box[1], cz = box[2], hx = box[3] / 2, hy = box[4] / 2, hz = box[5] / 2;
+    const V3 v[8] = {{cx + hx, cy + hy, cz + hz}, {cx - hx, cy + hy, cz + hz}, {cx - hx, cy - hy, cz + hz},
+                     {cx + hx, cy - hy, cz + hz}, {cx + hx, cy + hy, cz - hz}, {cx - hx, cy + hy, cz - hz},
+                     {cx - hx, cy - hy, cz - hz}, {cx + hx, cy - hy, cz - hz}};
+    for (int f = 0; f < 6; ++f)
+        for (int k = 0; k < 4; ++k) faces[f][k] = v[kFaceVtx[f][k]];
+}
+
+// Overlap of two coplanar rectangles with opposite normals, wound like the first
+// (_rect_overlap_3d, src/shape/box3d_multi.jl:36-132).
+bool rect_overlap(const Quad &f1, const V3 &n1, const Quad &f2, const V3 &n2, Quad &out) {
+    if (len3(n1[0] + n2[0], n1[1] + n2[1], n1[2] + n2[2]) > kFaceTol) return false;
+    for (const V3 &p : f2)
+        if (std::fabs(n1[0] * (p[0] - f1[0][0]) + n1[1] * (p[1] - f1[0][1]) + n1[2] * (p[2] - f1[0][2])) > kFaceTol)
+            return false;
+    const Axes ax = plane_axes(n1);
+    double u1lo, u1hi, v1lo, v1hi, u2lo, u2hi, v2lo, v2hi;
+    extent(f1, ax.u, u1lo, u1hi);
+    extent(f1, ax.v, v1lo, v1hi);
+    extent(f2, ax.u, u2lo, u2hi);
+    extent(f2, ax.v, v2lo, v2hi);
+    const double lo_u = jmax(u1lo, u2lo), hi_u = jmin(u1hi, u2hi);
+    const double lo_v = jmax(v1lo, v2lo), hi_v = jmin(v1hi, v2hi);
+    if (hi_u - lo_u < kFaceTol || hi_v - lo_v < kFaceTol) return false;
+    const double w = f1[0][ax.n];
+    const double ab_u = f1[1][ax.u] - f1[0][ax.u], ab_v = f1[1][ax.v] - f1[0][ax.v];
+    const double ad_u = f1[3][ax.u] - f1[0][ax.u], ad_v = f1[3][ax.v] - f1[0][ax.v];
+    if (std::fabs(ab_u) > kFaceTol) {  // a -> b runs along u
+        const double us = ab_u >= 0 ? lo_u : hi_u, ue = ab_u >= 0 ? hi_u : lo_u;
+        const double vs = ad_v >= 0 ? lo_v : hi_v, ve = ad_v >= 0 ? hi_v : lo_v;
+        out = {in_plane(ax, us, vs, w), in_plane(ax, ue, vs, w), in_plane(ax, ue, ve, w), in_plane(ax, us, ve, w)};
+    } else {  // a -> b runs along v
+        const double us = ad_u >= 0 ? lo_u : hi_u, ue = ad_u >= 0 ? hi_u : lo_u;
+        const double vs = ab_v >= 0 ? lo_v : hi_v, ve = ab_v >= 0 ? hi_v : lo_v;
+        out = {in_plane(ax, us, vs, w), in_plane(ax, us, ve, w), in_plane(ax, ue, ve, w), in_plane(ax, ue, vs, w)};
+    }
+    return true;
+}
+
+struct SharedFace {
+    Quad region;
+    int lo, hi;  // box ids, lo < hi
+    V3 nrm;      // from box hi towards box lo
+};
+
+// _detect_shared_faces_3d (src/shape/box3d_multi.jl:139-162)
+std::vector<SharedFace> shared_faces(int nb, const double *boxes) {
+    std::vector<SharedFace> out;
+    Quad fi[6], fj[6];
+    for (int i = 0; i < nb; ++i) {
+        box_faces(boxes + 6 * i, fi);
+        for (int j = i + 1; j < nb; ++j) {
+            box_faces(boxes + 6 * j, fj);
+            for (int a = 0; a < 6; ++a)
+                for (int b = 0; b < 6; ++b) {
+                    Quad reg;
+                    if (rect_overlap(fi[a], kFaceNrm[a], fj[b], kFaceNrm[b], reg))
+                        out.push_back({reg, i, j, V3{-kFaceNrm[a][0], -kFaceNrm[a][1], -kFaceNrm[a][2]}});
+                }
+        }
+    }
+    return out;
+}
+
+// sort!(unique!(x)) under Julia's isequal / isless: -0.0 and 0.0 stay distinct, -0.0 first
+void sort_unique(std::vector<double> &x) {
+    auto less = [](double a, double b) { return a < b || (a == b && std::signbit(a) && !std::signbit(b)); };
+    std::sort(x.begin(), x.end(), less);
+    x.erase(std::unique(x.begin(), x.end(),
+                        [](double a, double b) { return a == b && std::signbit(a) == std::signbit(b); }),
+            x.end());
+}
+
+// A face minus the rectangles it shares with other boxes, as the cells of the grid their edges
+// span, each with the flags of its physical edges (ab, bc, cd, da) and corners (a, b, c, d)
+// (_subtract_rects_from_face_3d, src/shape/box3d_multi.jl:169-308).
+void subtract_rects(const Quad &face, const V3 &nrm, const std::vector<Quad> &cut, std::vector<Cell> &out) {
+    if (cut.empty()) {
+        out.push_back(Cell{face[0], face[1], face[2], face[3], true, true, true, true, true, true, true, true});
+        return;
+    }
+    const Axes ax = plane_axes(nrm);
+    const double w = face[0][ax.n];
+    double fu0, fu1, fv0, fv1;
+    extent(face, ax.u, fu0, fu1);
+    extent(face, ax.v, fv0, fv1);
+    std::vector<double> us{fu0, fu1}, vs{fv0, fv1};
+    for (const Quad &r : cut)
+        for (const V3 &p : r) {
+            us.push_back(p[ax.u]);
+            vs.push_back(p[ax.v]);
+        }
+    sort_unique(us);
+    sort_unique(vs);
+    us.erase(std::remove_if(us.begin(), us.end(), [&](double u) { return !(fu0 - kFaceTol <= u && u <= fu1 + kFaceTol); }),
+             us.end());
+    vs.erase(std::remove_if(vs.begin(), vs.end(), [&](double v) { return !(fv0 - kFaceTol <= v && v <= fv1 + kFaceTol); }),
+             vs.end());
+    const int nu = (int)us.size() - 1, nv = (int)vs.size() - 1;
+    if (nu < 1 || nv < 1) return;
+    std::vector<char> sh((size_t)nu * nv);
+    auto S = [&](int i, int j) -> char & { return sh[(size_t)i * nv + j]; };
+    for (int i = 0; i < nu; ++i)
+        for (int j = 0; j < nv; ++j) {
+            if (us[i + 1] - us[i] < kFaceTol || vs[j + 1] - vs[j] < kFaceTol) {
+                S(i, j) = 1;  // degenerate cell
+                continue;
+            }
+            const double um = (us[i] + us[i + 1]) / 2, vm = (vs[j] + vs[j + 1]) / 2;
+            char hit = 0;
+            for (const Quad &r : cut) {
+                double ru0, ru1, rv0, rv1;
+                extent(r, ax.u, ru0, ru1);
+                extent(r, ax.v, rv0, rv1);
+                if (ru0 - kFaceTol <= um && um <= ru1 + kFaceTol && rv0 - kFaceTol <= vm && vm <= rv1 + kFaceTol) {
+                    hit = 1;
+                    break;
+                }
+            }
+            S(i, j) = hit;
+        }
+    // a grid point is physical when any cell around it is shared or outside the face
+    auto blocked = [&](int i, int j) { return i < 0 || i >= nu || j < 0 || j >= nv || S(i, j); };
+    auto corner = [&](int gi, int gj) {
+        return blocked(gi, gj) || blocked(gi - 1, gj) || blocked(gi, gj - 1) || blocked(gi - 1, gj - 1);
+    };
+    for (int i = 0; i < nu; ++i)
+        for (int j = 0; j < nv; ++j) {
+            if (S(i, j)) continue;
+            Cell c;
+            c.a = in_plane(ax, us[i], vs[j], w);
+            c.b = in_plane(ax, us[i + 1], vs[j], w);
+            c.c = in_plane(ax, us[i + 1], vs[j + 1], w);
+            c.d = in_plane(ax, us[i], vs[j + 1], w);
+            // an edge is physical unless the cell across it is another remaining cell
+            c.eab = j == 0 || S(i, j - 1);
+            c.ebc = i == nu - 1 || S(i + 1, j);
+            c.ecd = j == nv - 1 || S(i, j + 1);
+            c.eda = i == 0 || S(i - 1, j);
+            c.ca = corner(i, j);
+            c.cb = corner(i + 1, j);
+            c.cc = corner(i + 1, j + 1);
+            c.cd = corner(i, j + 1);
+            out.push_back(c);
+        }
+}
+}  // namespace
+
+// multi_dielectric_box3d (src/shape/box3d_multi.jl:313-386): the external pieces of every box's
+// faces (box order, face order, cell order) then the shared faces (detection order), each meshed by
+// rect_panel3d_adaptive_panels.  eps_in / eps_out receive the permittivities of every panel.
+Geometry *make_multi_box(int nq, double l_ec, int nb, const double *boxes, const double *epses, double eps_out,
+                         double alpha, std::vector<double> &eps_in_p, std::vector<double> &eps_out_p) {
+    if (nb < 1 || !boxes || !epses) fail(DBIE_BAD_ARG, "need at least one box and one permittivity per box");
+    check_mesh_args(nq, l_ec);
+    for (int b = 0; b < nb; ++b)
+        if (!(boxes[6 * b + 3] > 0 && boxes[6 * b + 4] > 0 && boxes[6 * b + 5] > 0))
+            fail(DBIE_BAD_ARG, "box sides must be positive");
+    std::unique_ptr<Geometry> g(new Geometry);
+    g->nq = nq;
+    init_rules(*g);
+    eps_in_p.clear();
+    eps_out_p.clear();
+    auto mesh = [&](const Cell &root, const V3 &nrm, double ei, double eo) {
+        append_adaptive(*g, root, nrm, alpha, l_ec);
+        eps_in_p.resize(g->is_edge.size(), ei);
+        eps_out_p.resize(g->is_edge.size(), eo);
+    };
+    const std::vector<SharedFace> shared = shared_faces(nb, boxes);
+    Quad faces[6];
+    std::vector<Quad> cut;
+    std::vector<Cell> cells;
+    for (int b = 0; b < nb; ++b) {
+        box_faces(boxes + 6 * b, faces);
+        for (int f = 0; f < 6; ++f) {
+            const V3 &fn = kFaceNrm[f];
+            cut.clear();
+            for (const SharedFace &sf : shared) {
+                if (sf.lo != b && sf.hi != b) continue;
+                Quad ov;
+                if (rect_overlap(faces[f], fn, sf.region, V3{-fn[0], -fn[1], -fn[2]}, ov)) cut.push_back(ov);
+            }
+            cells.clear();
+            subtract_rects(faces[f], fn, cut, cells);
+            for (const Cell &c : cells) mesh(c, fn, epses[b], eps_out);
+        }
+    }
+    for (const SharedFace &sf : shared)
+        mesh(Cell{sf.region[0], sf.region[1], sf.region[2], sf.region[3], true, true, true, true, true, true, true,
+                  true},
+             sf.nrm, epses[sf.hi], epses[sf.lo]);
+    g->P = (int64_t)g->is_edge.size();
+    if (g->P == 0) fail(DBIE_BAD_ARG, "the boxes leave no surface");
     fill_nodes(*g);
     return g.release();
 }

@@ -64,6 +64,9 @@ void gauss_legendre(int n, double *x, double *w);  // throws on n < 1
 bool best_grid(double W, double H, double alpha, int *m, int *n);
 Geometry *make_box(double Lx, double Ly, double Lz, int nq, double l_ec, double alpha);
 Geometry *make_from_panels(int64_t P, int nq, const double *corners, const double *normals, const int *is_edge);
+// multi_dielectric_box3d: boxes nb x 6 (centre x, y, z, Lx, Ly, Lz); per-panel permittivities out
+Geometry *make_multi_box(int nq, double l_ec, int nb, const double *boxes, const double *epses, double eps_out,
+                         double alpha, std::vector<double> &eps_in_p, std::vector<double> &eps_out_p);
 // physicists' Gauss-Hermite rule (weight exp(-x^2)), ascending nodes; throws on n < 1
 void gauss_hermite(int n, double *x, double *w);
 void barycentric_row(int n, const double *x, const double *lam, double xq, double *r);
@@ -189,4 +192,5 @@ TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, dou
 // Opaque handles of the C ABI.
 struct dbie_geom {
     dbie::Geometry g;
+    std::vector<double> eps_in, eps_out;  // per panel, when the mesher assigns permittivities
 };

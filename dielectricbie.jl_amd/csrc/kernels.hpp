@@ -90,6 +90,21 @@ void rhs_volume(hipStream_t st, const DevGeom &g, int K, const int64_t *col_ptr,
 void screen_sharp(hipStream_t st, int64_t S, const double *pos, const double *rho, const double *w,
                   const double *lo, const double *hi, double tol, double eps_in, double eps_out, double *wrho);
 
+// Screening of volume source densities by boxes of permittivity (device arrays lo, hi: nb x 3,
+// eps: nb).  mode: SCREEN_SHARP, SCREEN_SOFT_MIX (eps mixed), SCREEN_SOFT_MIX_INV (1/eps mixed);
+// single = the one-region formulas of the interface overloads (nb = 1).
+enum ScreenMode { SCREEN_SHARP = 0, SCREEN_SOFT_MIX = 1, SCREEN_SOFT_MIX_INV = 2 };
+struct ScreenBoxes {
+    int nb = 0;
+    const double *lo = nullptr, *hi = nullptr, *eps = nullptr;
+    double eps_out = 1.0, tol = 0.0, bandwidth = 1.0;
+    int mode = SCREEN_SHARP;
+    bool single = false;
+};
+// out[s] = w[s] * rho'_s (w != NULL) or rho'_s, rho' the screened density
+void screen_boxes(hipStream_t st, int64_t S, const double *pos, const double *rho, const double *w,
+                  const ScreenBoxes &sb, double *out);
+
 // Off-surface single-layer potential U[t,k] = sum_i w_i X[i,k]/(4 pi |t - x_i|).
 void pottrg(hipStream_t st, const DevGeom &g, int64_t M, const double *trg, int K, const double *X, int64_t ldx,
             double *U, int64_t ldu);

@@ -71,6 +71,57 @@ __global__ void screen_sharp_kernel(int64_t S, const double *__restrict__ pos, c
     wrho[s] = w[s] * (rho[s] / (in ? eps_in : eps_out));
 }
 
+// (1 + erf(d / bandwidth)) / 2 (_smooth_screen_factor, src/core/sources.jl:321-323)
+__device__ inline double smooth_step(double d, double bw) { return (1.0 + erf(d / bw)) / 2.0; }
+
+// Screened density rho / eps(y) of source points (src/core/sources.jl:315-349 for one region,
+// :408-493 for several boxes).  Sharp: eps of the first box whose tol-widened closure holds the
+// point, else eps_out.  Soft: the boxes' smooth indicators P_b (products of six error-function
+// steps) mix eps (or 1/eps) with eps_out; each sum runs in box order from zero like the reference.
+__global__ void screen_boxes_kernel(int64_t S, const double *__restrict__ pos, const double *__restrict__ rho,
+                                    const double *__restrict__ w, ScreenBoxes sb, double *__restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const double p[3] = {pos[3 * s], pos[3 * s + 1], pos[3 * s + 2]};
+    double r;
+    if (sb.mode == SCREEN_SHARP) {
+        double eps = sb.eps_out;
+        for (int b = 0; b < sb.nb; ++b) {
+            const double *lo = sb.lo + 3 * b, *hi = sb.hi + 3 * b;
+            if ((lo[0] - sb.tol <= p[0] && p[0] <= hi[0] + sb.tol) && (lo[1] - sb.tol <= p[1] && p[1] <= hi[1] + sb.tol) &&
+                (lo[2] - sb.tol <= p[2] && p[2] <= hi[2] + sb.tol)) {
+                eps = sb.eps[b];
+                break;
+            }
+        }
+        r = rho[s] / eps;
+    } else {
+        double mix = 0.0, sum_p = 0.0;
+        for (int b = 0; b < sb.nb; ++b) {
+            const double *lo = sb.lo + 3 * b, *hi = sb.hi + 3 * b;
+            const double P = smooth_step(p[0] - lo[0], sb.bandwidth) * smooth_step(hi[0] - p[0], sb.bandwidth) *
+                             smooth_step(p[1] - lo[1], sb.bandwidth) * smooth_step(hi[1] - p[1], sb.bandwidth) *
+                             smooth_step(p[2] - lo[2], sb.bandwidth) * smooth_step(hi[2] - p[2], sb.bandwidth);
+            if (sb.single) {
+                mix = P;  // one region: the smooth indicator s itself
+            } else {
+                mix += sb.mode == SCREEN_SOFT_MIX ? sb.eps[b] * P : P / sb.eps[b];
+                sum_p += P;
+            }
+        }
+        if (sb.single) {
+            const double eps = sb.mode == SCREEN_SOFT_MIX ? sb.eps[0] * mix + sb.eps_out * (1.0 - mix)
+                                                          : 1.0 / (mix / sb.eps[0] + (1.0 - mix) / sb.eps_out);
+            r = rho[s] / eps;
+        } else if (sb.mode == SCREEN_SOFT_MIX) {
+            r = rho[s] / (mix + sb.eps_out * (1.0 - sum_p));
+        } else {
+            r = rho[s] * (mix + (1.0 - sum_p) / sb.eps_out);
+        }
+    }
+    out[s] = w ? w[s] * r : r;
+}
+
 // laplace3d_pottrg (src/kernel/laplace3d.jl:69-82): off-surface targets, KC columns per thread.
 template <int KC>
 __global__ __launch_bounds__(kTile) void pottrg_kernel(DevGeom g, int64_t M, const double *__restrict__ trg, int K,
@@ -250,6 +301,12 @@ void screen_sharp(hipStream_t st, int64_t S, const double *pos, const double *rh
     if (S <= 0) return;
     screen_sharp_kernel<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, pos, rho, w, lo[0], lo[1], lo[2], hi[0],
                                                                       hi[1], hi[2], tol, eps_in, eps_out, wrho);
+}
+
+void screen_boxes(hipStream_t st, int64_t S, const double *pos, const double *rho, const double *w,
+                  const ScreenBoxes &sb, double *out) {
+    if (S <= 0) return;
+    screen_boxes_kernel<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, pos, rho, w, sb, out);
 }
 
 void pottrg(hipStream_t st, const DevGeom &g, int64_t M, const double *trg, int K, const double *X, int64_t ldx,

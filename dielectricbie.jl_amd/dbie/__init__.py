@@ -13,6 +13,8 @@ from .api import (  # noqa: F401
     PointSource,
     Server,
     SharpScreening,
+    SoftMixInversePermittivity,
+    SoftMixPermittivity,
     VolumeSource,
     all_weights,
     autotune,
@@ -20,6 +22,7 @@ from .api import (  # noqa: F401
     best_grid,
     best_grid_mn,
     blob_lattice_centers,
+    box_array,
     build_neighbor_list,
     dielectric_diagonal_scaling,
     gaussian_blob_gh,
@@ -27,6 +30,7 @@ from .api import (  # noqa: F401
     gausslegendre,
     interface_from_panels,
     lhs_dielectric_box3d,
+    multi_dielectric_box3d,
     lhs_dielectric_box3d_fmm3d,
     lhs_dielectric_box3d_fmm3d_corrected,
     laplace3d_pottrg_fmm3d_corrected_hcubature,
@@ -35,6 +39,7 @@ from .api import (  # noqa: F401
     rhs_dielectric_box3d_fmm3d,
     reaction_matrix,
     rhs_point_charges,
+    rhs_volume_boxes,
     screened_volume_source,
     serving_requests,
     single_dielectric_box3d,

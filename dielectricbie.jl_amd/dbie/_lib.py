@@ -97,6 +97,8 @@ SIGNATURES = {
     "dbie_best_grid": [C.c_double, C.c_double, C.c_double, _ip, _ip],
     "dbie_geometry_box": [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(_vp)],
     "dbie_geometry_from_panels": [C.c_int64, C.c_int, _dp, _dp, _ip, C.POINTER(_vp)],
+    "dbie_geometry_multi_box": [C.c_int, C.c_double, C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(_vp)],
+    "dbie_geom_eps": [_vp, _dp, _dp],
     "dbie_geom_sizes": [_vp, _i64p, _i64p, _ip],
     "dbie_geom_nodes": [_vp, _dp],
     "dbie_geom_panels": [_vp, _dp, _dp, _ip],
@@ -115,6 +117,11 @@ SIGNATURES = {
     "dbie_solve_lu": [C.c_int64, _dp, C.c_int64, C.c_int, _dp, C.c_int64],
     "dbie_rhs_points": [_vp, C.c_int, _dp, _dp, C.c_double, _dp, C.c_int64],
     "dbie_rhs_volume": [_vp, C.c_int, _i64p, _dp, _dp, _dp, C.c_int, C.c_double, _dp, C.c_int64],
+    "dbie_rhs_volume_boxes": [_vp, C.c_int, _i64p, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int,
+                              C.c_double, _dp, C.c_int64],
+    "dbie_screen_boxes": [C.c_int64, _dp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double,
+                          C.c_int, _dp],
+    "dbie_screen_interface": [_vp, C.c_int64, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp],
     "dbie_solve_block": [_vp, C.c_int, _dp, C.c_int64, _dp, C.c_int64, C.POINTER(SolveOpts), C.POINTER(SolveStats)],
     "dbie_pottrg": [_vp, C.c_int64, _dp, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
     "dbie_op_stream": [_vp],

@@ -89,6 +89,47 @@ class VolumeSource:
 
 class SharpScreening:
     """SharpScreening (src/core/sources.jl:265): rho / eps_in inside the box, rho / eps_out outside."""
+    code = 0
+    bandwidth = 0.0
+
+
+class SoftMixPermittivity:
+    """SoftMixPermittivity(bandwidth) (src/core/sources.jl:267-273): eps mixed by the box's smooth
+    indicator (products of (1 + erf(d / bandwidth)) / 2 over the six faces)."""
+    code = 1
+
+    def __init__(self, bandwidth: float):
+        if not bandwidth > 0:
+            raise ValueError("bandwidth must be positive")
+        self.bandwidth = float(bandwidth)
+
+
+class SoftMixInversePermittivity(SoftMixPermittivity):
+    """SoftMixInversePermittivity(bandwidth) (src/core/sources.jl:275-281): 1/eps mixed instead."""
+    code = 2
+
+
+def _screen_mode(mode):
+    if mode is None:
+        raise TypeError("screened_volume_source requires a screening mode (MethodError in the reference)")
+    if not isinstance(mode, (SharpScreening, SoftMixPermittivity)):
+        raise TypeError(f"unknown screening mode {type(mode).__name__}")
+    return mode.code, mode.bandwidth
+
+
+def box_array(boxes) -> np.ndarray:
+    """n x 6 (centre x, y, z, Lx, Ly, Lz) from the reference's (center = (x, y, z), Lx, Ly, Lz)
+    boxes, given as dicts, objects with those attributes or (center, Lx, Ly, Lz) tuples."""
+    rows = []
+    for b in boxes:
+        if isinstance(b, dict):
+            c, lx, ly, lz = b["center"], b["Lx"], b["Ly"], b["Lz"]
+        elif hasattr(b, "center"):
+            c, lx, ly, lz = b.center, b.Lx, b.Ly, b.Lz
+        else:
+            c, lx, ly, lz = b
+        rows.append([float(c[0]), float(c[1]), float(c[2]), float(lx), float(ly), float(lz)])
+    return np.ascontiguousarray(np.array(rows, np.float64).reshape(-1, 6))
 
 
 # --------------------------------------------------------------------------- #
@@ -173,6 +214,24 @@ def single_dielectric_box3d(Lx, Ly, Lz, n_quad: int, l_ec, eps_in, eps_out, T=np
     L.check(L.lib().dbie_geometry_box(float(Lx), float(Ly), float(Lz), int(n_quad), float(l_ec), float(alpha),
                                       C.byref(h)))
     return DielectricInterface(h.value, eps_in, eps_out)
+
+
+def multi_dielectric_box3d(n_quad: int, l_ec, boxes, epses, eps_out=1.0, *, alpha=math.sqrt(2.0)):
+    """multi_dielectric_box3d (src/shape/box3d_multi.jl:358-386): touching axis-aligned boxes with
+    permittivity epses[b] inside box b and eps_out outside; shared faces carry
+    (eps_in, eps_out) = (eps of the higher-numbered box, eps of the lower)."""
+    B = box_array(boxes)
+    e = np.ascontiguousarray(epses, np.float64).ravel()
+    if len(e) != B.shape[0]:
+        raise ValueError("Number of boxes must match number of permittivities")
+    h = C.c_void_p()
+    L.check(L.lib().dbie_geometry_multi_box(int(n_quad), float(l_ec), B.shape[0], L.dptr(B.ravel()), L.dptr(e),
+                                            float(eps_out), float(alpha), C.byref(h)))
+    P = C.c_int64()
+    L.check(L.lib().dbie_geom_sizes(h, C.byref(P), None, None))
+    ei = np.zeros(P.value); eo = np.zeros(P.value)
+    L.check(L.lib().dbie_geom_eps(h, L.dptr(ei), L.dptr(eo)))
+    return DielectricInterface(h.value, ei, eo)
 
 
 def interface_from_panels(corners, normals, n_quad: int, eps_in, eps_out, is_edge=None):
@@ -493,29 +552,59 @@ def _rhs_op(interface):
     return _op_for(interface, ("plain",), corrected=False)
 
 
-def screened_volume_source(interface: DielectricInterface, vs: VolumeSource, mode=None) -> VolumeSource:
-    """screened_volume_source(interface, vs, SharpScreening()) (src/core/sources.jl:390-406).
-    The screening itself runs on the GPU inside the RHS kernels; this host form exists for
-    callers that need the screened density (interface bounds + sqrt(eps)*max side tolerance)."""
-    if mode is None:
-        raise TypeError("screened_volume_source requires a screening mode (MethodError in the reference)")
-    if not isinstance(mode, SharpScreening):
-        raise NotImplementedError("only SharpScreening is on the GPU path")
-    ei, eo = interface.eps_in, interface.eps_out
-    if not (np.all(ei == ei[0]) and np.all(eo == eo[0])):
-        raise ValueError("screened_volume_source requires a uniform eps across the interface")
-    corners = interface.panels()[0].reshape(-1, 3)
-    lo, hi = corners.min(0), corners.max(0)
-    tol = SQRT_EPS * np.max(np.abs(hi - lo))
-    p = vs.positions
-    inside = np.all((lo[:, None] - tol <= p) & (p <= hi[:, None] + tol), axis=0)
-    rho = vs.density / np.where(inside, ei[0], eo[0])
-    return VolumeSource(p.copy(), vs.weights.copy(), rho)
+def screened_volume_source(*args, tol=None) -> VolumeSource:
+    """screened_volume_source (src/core/sources.jl:368-515), computed on the GPU.  Forms:
+
+    * ``(interface, vs, mode)``: the interface's bounding box and uniform permittivities;
+    * ``(boxes, epses, eps_out, vs, mode)``: several boxes (first box holding a point wins for
+      SharpScreening; partition-of-unity mixing for the soft modes);
+    * ``(Lx, Ly, Lz, vs, eps_in, eps_out, mode)``: one box centred at the origin.
+
+    ``tol`` (SharpScreening's widening of the boxes) defaults to sqrt(eps) times the largest side."""
+    t = -1.0 if tol is None else float(tol)
+    if len(args) in (2, 3) and isinstance(args[0], DielectricInterface):
+        interface, vs = args[0], args[1]
+        code, bw = _screen_mode(args[2] if len(args) == 3 else None)
+        op = _rhs_op(interface)
+        rho = np.zeros(vs.density.size)
+        L.check(L.lib().dbie_screen_interface(op._h, vs.density.size, L.dptr(np.ascontiguousarray(vs.positions.T)),
+                                              L.dptr(vs.density), code, float(bw), t, L.dptr(rho)))
+    elif len(args) == 5:
+        boxes, epses, eps_out, vs, mode = args
+        B = box_array(boxes)
+        e = np.ascontiguousarray(epses, np.float64).ravel()
+        if len(e) != B.shape[0]:
+            raise ValueError("Number of boxes must match number of permittivities")
+        rho = _screen_boxes(vs, B, e, float(eps_out), mode, t, 0)
+    elif len(args) == 7:
+        Lx, Ly, Lz, vs, eps_in, eps_out, mode = args
+        B = np.array([[0.0, 0.0, 0.0, float(Lx), float(Ly), float(Lz)]])
+        rho = _screen_boxes(vs, B, np.array([float(eps_in)]), float(eps_out), mode, t, 1)
+    else:
+        raise TypeError("screened_volume_source(interface, vs, mode), (boxes, epses, eps_out, vs, mode) or "
+                        "(Lx, Ly, Lz, vs, eps_in, eps_out, mode)")
+    return VolumeSource(vs.positions.copy(), vs.weights.copy(), rho)
 
 
-def rhs_dielectric_box3d(interface: DielectricInterface, source, eps_src=None) -> np.ndarray:
+def _screen_boxes(vs, B, e, eps_out, mode, tol, form):
+    code, bw = _screen_mode(mode)
+    S = vs.density.size
+    rho = np.zeros(S)
+    pos = np.ascontiguousarray(vs.positions.T) if S else np.zeros((1, 3))
+    dens = vs.density if S else np.zeros(1)
+    L.check(L.lib().dbie_screen_boxes(S, L.dptr(pos.ravel()), L.dptr(dens), B.shape[0], L.dptr(B.ravel()), L.dptr(e),
+                                      eps_out, code, float(bw), tol, form, L.dptr(rho) if S else L.dptr(np.zeros(1))))
+    return rho
+
+
+def rhs_dielectric_box3d(interface: DielectricInterface, source, eps_src=None, *rest) -> np.ndarray:
     """rhs_dielectric_box3d for a PointSource (src/solver/dielectric_box3d.jl:95-104), a VolumeSource
-    with eps_src (:106-120) or a VolumeSource screened by the interface (:122-124)."""
+    with eps_src (:106-120), a VolumeSource screened by the interface (:122-124), or
+    ``(interface, boxes, epses, eps_out, vs)``: screened box by box (:224-232)."""
+    if rest:
+        if len(rest) != 2:
+            raise TypeError("rhs_dielectric_box3d(interface, boxes, epses, eps_out, vs)")
+        return rhs_volume_boxes(interface, [rest[1]], source, eps_src, rest[0])[:, 0]
     op = _rhs_op(interface)
     N = interface.n_points
     if isinstance(source, PointSource):
@@ -545,28 +634,62 @@ def rhs_point_charges(interface: DielectricInterface, points, charges, eps_src: 
     return F
 
 
-def _rhs_volume(op, vss, screening: bool, eps_src: float = 1.0) -> np.ndarray:
+def rhs_volume_boxes(interface: DielectricInterface, vss, boxes, epses, eps_out=1.0, mode=None) -> np.ndarray:
+    """N x K right-hand side of K volume sources screened box by box (default SharpScreening), eps_src = 1:
+    rhs_dielectric_box3d_fmm3d(interface, vss, thresh; screen_boxes, screen_epses, screen_eps_out)
+    (src/solver/multi_rhs.jl:168-211) with the direct sum for the FMM."""
+    code, bw = _screen_mode(SharpScreening() if mode is None else mode)
+    B = box_array(boxes)
+    e = np.ascontiguousarray(epses, np.float64).ravel()
+    if len(e) != B.shape[0]:
+        raise ValueError("Number of boxes must match number of permittivities")
+    op = _rhs_op(interface)
+    col_ptr, xyz, w, rho = _pack_sources(vss)
+    K, N = len(vss), op.n
+    if K == 0:
+        return np.zeros((N, 0))
+    F = np.zeros((N, K), order="F")
+    L.check(L.lib().dbie_rhs_volume_boxes(op._h, K, L.i64ptr(col_ptr), L.dptr(xyz.ravel()), L.dptr(w), L.dptr(rho),
+                                          B.shape[0], L.dptr(B.ravel()), L.dptr(e), float(eps_out), code, float(bw),
+                                          L.dptr(F), N))
+    return F
+
+
+def _pack_sources(vss):
     K = len(vss)
     col_ptr = np.zeros(K + 1, np.int64)
     col_ptr[1:] = np.cumsum([v.positions.shape[1] for v in vss])
     xyz = np.ascontiguousarray(np.concatenate([v.positions.T for v in vss], axis=0)) if K else np.zeros((0, 3))
     w = np.ascontiguousarray(np.concatenate([v.weights for v in vss])) if K else np.zeros(0)
     rho = np.ascontiguousarray(np.concatenate([v.density for v in vss])) if K else np.zeros(0)
-    N = op.n
-    F = np.zeros((N, max(K, 1)), order="F")
-    if K == 0:
-        return np.zeros((N, 0))
     if xyz.size == 0:
         xyz = np.zeros((1, 3)); w = np.zeros(1); rho = np.zeros(1)
+    return col_ptr, xyz, w, rho
+
+
+def _rhs_volume(op, vss, screening: bool, eps_src: float = 1.0) -> np.ndarray:
+    K, N = len(vss), op.n
+    if K == 0:
+        return np.zeros((N, 0))
+    col_ptr, xyz, w, rho = _pack_sources(vss)
+    F = np.zeros((N, K), order="F")
     L.check(L.lib().dbie_rhs_volume(op._h, K, L.i64ptr(col_ptr), L.dptr(xyz.ravel()), L.dptr(w), L.dptr(rho),
                                     int(bool(screening)), float(eps_src), L.dptr(F), N))
     return F
 
 
-def rhs_dielectric_box3d_fmm3d(interface: DielectricInterface, sources, *args):
+def rhs_dielectric_box3d_fmm3d(interface: DielectricInterface, sources, *args, screen_boxes=None,
+                               screen_epses=None, screen_eps_out=1.0):
     """rhs_dielectric_box3d_fmm3d: single VolumeSource (src/solver/dielectric_box3d.jl:127-159) or
     Vector{VolumeSource} -> N x K (src/solver/multi_rhs.jl:168-211).  Forms:
-    (interface, vs, eps_src, thresh), (interface, vs, thresh), (interface, vss, thresh)."""
+    (interface, vs, eps_src, thresh), (interface, vs, thresh), (interface, vss, thresh), the latter with
+    optional box-by-box screening (screen_boxes, screen_epses, screen_eps_out), and
+    (interface, boxes, epses, eps_out, vs, thresh) (src/solver/dielectric_box3d.jl:234-243)."""
+    if len(args) == 4 and isinstance(args[2], VolumeSource):
+        return rhs_volume_boxes(interface, [args[2]], sources, args[0], args[1])[:, 0]
+    if screen_boxes is not None:
+        vss = [sources] if isinstance(sources, VolumeSource) else list(sources)
+        return rhs_volume_boxes(interface, vss, screen_boxes, screen_epses, screen_eps_out)
     op = _rhs_op(interface)
     if isinstance(sources, VolumeSource):
         if len(args) == 2:

@@ -122,6 +122,16 @@ dbie_status dbie_geometry_box(double Lx, double Ly, double Lz, int n_quad, doubl
    (lb, rb, rt, lt), normals P x 3, is_edge P (may be NULL). */
 dbie_status dbie_geometry_from_panels(int64_t P, int n_quad, const double *corners, const double *normals,
                                       const int *is_edge, dbie_geom **out);
+/* multi_dielectric_box3d (src/shape/box3d_multi.jl:358-386): n_boxes axis-aligned boxes, boxes
+   n_boxes x 6 (centre x, y, z, Lx, Ly, Lz; the reference's (center, Lx, Ly, Lz) NamedTuples),
+   permittivity epses[b] inside box b and eps_out outside.  Panels: the pieces of every box face not
+   shared with another box (box order, face order), then the shared faces; each region meshed by
+   rect_panel3d_adaptive_panels.  dbie_geom_eps returns the panels' permittivities. */
+dbie_status dbie_geometry_multi_box(int n_quad, double l_ec, int n_boxes, const double *boxes, const double *epses,
+                                    double eps_out, double alpha, dbie_geom **out);
+/* per-panel eps_in / eps_out (interface.eps_in / eps_out, src/core/panels.jl:64-68) of a geometry
+   whose mesher assigned them (dbie_geometry_multi_box); DBIE_BAD_ARG for other geometries */
+dbie_status dbie_geom_eps(const dbie_geom *g, double *eps_in, double *eps_out);
 dbie_status dbie_geom_sizes(const dbie_geom *g, int64_t *P, int64_t *N, int *n_quad);
 /* node SoA, 7 x N rows x,y,z,nx,ny,nz,w (eachpoint + all_weights, src/core/panels.jl:85-118) */
 dbie_status dbie_geom_nodes(const dbie_geom *g, double *xyz_nrm_w);
@@ -180,6 +190,27 @@ dbie_status dbie_rhs_points(dbie_op *op, int K, const double *src_xyz, const dou
    through the interface bounds (src/core/sources.jl:390-406) and eps_src is then 1. */
 dbie_status dbie_rhs_volume(dbie_op *op, int K, const int64_t *col_ptr, const double *xyz, const double *w,
                             const double *rho, int screening, double eps_src, double *F, int64_t ldf);
+/* rhs_dielectric_box3d(interface, boxes, epses, eps_out, vs) (src/solver/dielectric_box3d.jl:224-232)
+   and rhs_dielectric_box3d_fmm3d(interface, vss; screen_boxes, screen_epses, screen_eps_out)
+   (src/solver/multi_rhs.jl:168-211): the sources screened box by box
+   (screened_volume_source(boxes, epses, eps_out, vs, mode), src/core/sources.jl:408-515; boxes
+   n_boxes x 6 = centre x, y, z, Lx, Ly, Lz; mode 0 SharpScreening, 1 SoftMixPermittivity(bandwidth),
+   2 SoftMixInversePermittivity(bandwidth); default tolerance), then eps_src = 1. */
+dbie_status dbie_rhs_volume_boxes(dbie_op *op, int K, const int64_t *col_ptr, const double *xyz, const double *w,
+                                  const double *rho, int n_boxes, const double *boxes, const double *epses,
+                                  double eps_out, int mode, double bandwidth, double *F, int64_t ldf);
+/* screened_volume_source on the GPU: rho_out[s] = the density of point s (xyz 3 x S) screened by
+   the permittivity at its position.  form 0: the several-box overload
+   (src/core/sources.jl:408-515); form 1: the one-region overload screened_volume_source(Lx, Ly, Lz,
+   vs, eps_in, eps_out, mode) (:315-388) with one box (epses[0] = eps_in).  tol < 0 takes the
+   reference's default (sqrt(eps) x the largest side). */
+dbie_status dbie_screen_boxes(int64_t S, const double *xyz, const double *rho, int n_boxes, const double *boxes,
+                              const double *epses, double eps_out, int mode, double bandwidth, double tol, int form,
+                              double *rho_out);
+/* screened_volume_source(interface, vs, mode; tol) (src/core/sources.jl:390-406) through the
+   operator's interface bounds and uniform permittivities (DBIE_BAD_ARG when they vary). */
+dbie_status dbie_screen_interface(dbie_op *op, int64_t S, const double *xyz, const double *rho, int mode,
+                                  double bandwidth, double tol, double *rho_out);
 
 /* ---- solves ------------------------------------------------------------ */
 /* solve_dielectric_box3d_block / Krylov.block_gmres (src/solver/multi_rhs.jl:270-286) and

@@ -348,6 +348,50 @@ static void divide_tpanel(const tpanel *t, int nx, int ny, tpanel *out) {
         }
 }
 
+/* rect_panel3d_adaptive_panels (src/shape/box3d.jl:99-133): best_grid rough split of the root,
+   then a FIFO of 2x2 splits of edge/corner panels longer than l_ec.  Appends the fine panels
+   (with normal nrm) to *fine/*fn (count *n, capacity *cap).  Returns 0, or -1 when best_grid
+   finds no tiling. */
+static int rect_adaptive(const tpanel *root, const double nrm[3], double l_ec, double alpha, tpanel **fine,
+                         double (**fn)[3], size_t *n, size_t *cap) {
+    double Lab = norm3(root->b[0] - root->a[0], root->b[1] - root->a[1], root->b[2] - root->a[2]);
+    double Lda = norm3(root->a[0] - root->d[0], root->a[1] - root->d[1], root->a[2] - root->d[2]);
+    int nx, ny;
+    if (orc_best_grid(Lab, Lda, alpha, &nx, &ny) != 0) return -1;
+    /* FIFO queue (popfirst!/append!) */
+    size_t qcap = (size_t)nx * ny + 4096, qh = 0, qt = 0;
+    tpanel *q = (tpanel *)malloc(sizeof(tpanel) * qcap);
+    divide_tpanel(root, nx, ny, q);
+    qt = (size_t)nx * ny;
+    while (qh < qt) {
+        tpanel t = q[qh++];
+        int has_ec = t.ca || t.cb || t.cc || t.cd || t.eab || t.ebc || t.ecd || t.eda;
+        double L_ab = norm3(t.b[0] - t.a[0], t.b[1] - t.a[1], t.b[2] - t.a[2]);
+        double L_da = norm3(t.a[0] - t.d[0], t.a[1] - t.d[1], t.a[2] - t.d[2]);
+        double mx = L_ab > L_da ? L_ab : L_da;
+        if (has_ec && mx > l_ec) {
+            if (qt + 4 > qcap) {
+                /* compact then grow */
+                memmove(q, q + qh, sizeof(tpanel) * (qt - qh)); qt -= qh; qh = 0;
+                if (qt + 4 > qcap) { qcap *= 2; q = (tpanel *)realloc(q, sizeof(tpanel) * qcap); }
+            }
+            divide_tpanel(&t, 2, 2, q + qt);
+            qt += 4;
+        } else {
+            if (*n == *cap) {
+                *cap *= 2;
+                *fine = (tpanel *)realloc(*fine, sizeof(tpanel) * *cap);
+                *fn = (double (*)[3])realloc(*fn, sizeof(double) * 3 * *cap);
+            }
+            (*fine)[*n] = t;
+            memcpy((*fn)[*n], nrm, sizeof(double) * 3);
+            ++*n;
+        }
+    }
+    free(q);
+    return 0;
+}
+
 /* Returns the panel count for a box, or <0 on error.  Panels are appended in
    face order; each face is meshed by rect_panel3d_adaptive_panels. */
 static int box_panels(double Lx, double Ly, double Lz, double l_ec, double alpha, tpanel **out_panels,
@@ -368,44 +412,37 @@ static int box_panels(double Lx, double Ly, double Lz, double l_ec, double alpha
         memcpy(root.d, V[F[f][3] - 1], sizeof(double) * 3);
         root.ca = root.cb = root.cc = root.cd = 1;
         root.eab = root.ebc = root.ecd = root.eda = 1;
-        double Lab = norm3(root.b[0] - root.a[0], root.b[1] - root.a[1], root.b[2] - root.a[2]);
-        double Lda = norm3(root.a[0] - root.d[0], root.a[1] - root.d[1], root.a[2] - root.d[2]);
-        int nx, ny;
-        if (orc_best_grid(Lab, Lda, alpha, &nx, &ny) != 0) { free(fine); free(fn); return -1; }
-        /* FIFO queue (popfirst!/append!) */
-        size_t qcap = (size_t)nx * ny + 4096, qh = 0, qt = 0;
-        tpanel *q = (tpanel *)malloc(sizeof(tpanel) * qcap);
-        divide_tpanel(&root, nx, ny, q);
-        qt = (size_t)nx * ny;
-        while (qh < qt) {
-            tpanel t = q[qh++];
-            int has_ec = t.ca || t.cb || t.cc || t.cd || t.eab || t.ebc || t.ecd || t.eda;
-            double L_ab = norm3(t.b[0] - t.a[0], t.b[1] - t.a[1], t.b[2] - t.a[2]);
-            double L_da = norm3(t.a[0] - t.d[0], t.a[1] - t.d[1], t.a[2] - t.d[2]);
-            double mx = L_ab > L_da ? L_ab : L_da;
-            if (has_ec && mx > l_ec) {
-                if (qt + 4 > qcap) {
-                    /* compact then grow */
-                    memmove(q, q + qh, sizeof(tpanel) * (qt - qh)); qt -= qh; qh = 0;
-                    if (qt + 4 > qcap) { qcap *= 2; q = (tpanel *)realloc(q, sizeof(tpanel) * qcap); }
-                }
-                divide_tpanel(&t, 2, 2, q + qt);
-                qt += 4;
-            } else {
-                if (n == cap) {
-                    cap *= 2;
-                    fine = (tpanel *)realloc(fine, sizeof(tpanel) * cap);
-                    fn = (double (*)[3])realloc(fn, sizeof(double) * 3 * cap);
-                }
-                fine[n] = t;
-                memcpy(fn[n], Nrm[f], sizeof(double) * 3);
-                ++n;
-            }
-        }
-        free(q);
+        if (rect_adaptive(&root, Nrm[f], l_ec, alpha, &fine, &fn, &n, &cap) != 0) { free(fine); free(fn); return -1; }
     }
     *out_panels = fine; *out_norm = fn;
     return (int)n;
+}
+
+/* rect_panel3d_adaptive_panels of one rectangle (corners a, b, c, d = abcd[0..11], normal nrm,
+   edge flags ab, bc, cd, da and corner flags a, b, c, d).  Writes up to cap panels (corners
+   12 each, is_edge) and returns the panel count (which may exceed cap: call again with a larger
+   buffer), or -1 when best_grid finds no tiling. */
+int orc_rect_adaptive_panels(const double *abcd, const double *nrm, const int *edge, const int *corner,
+                             double l_ec, double alpha, int cap_out, double *corners, int *is_edge) {
+    tpanel root;
+    memcpy(root.a, abcd, 24); memcpy(root.b, abcd + 3, 24); memcpy(root.c, abcd + 6, 24); memcpy(root.d, abcd + 9, 24);
+    root.eab = edge[0]; root.ebc = edge[1]; root.ecd = edge[2]; root.eda = edge[3];
+    root.ca = corner[0]; root.cb = corner[1]; root.cc = corner[2]; root.cd = corner[3];
+    size_t cap = 256, n = 0;
+    tpanel *fine = (tpanel *)malloc(sizeof(tpanel) * cap);
+    double (*fn)[3] = (double (*)[3])malloc(sizeof(double) * 3 * cap);
+    int rc = rect_adaptive(&root, nrm, l_ec, alpha, &fine, &fn, &n, &cap);
+    if (rc == 0) {
+        for (size_t p = 0; p < n && (int)p < cap_out; ++p) {
+            double *C = corners + 12 * p;
+            memcpy(C, fine[p].a, 24); memcpy(C + 3, fine[p].b, 24); memcpy(C + 6, fine[p].c, 24);
+            memcpy(C + 9, fine[p].d, 24);
+            is_edge[p] = fine[p].eab || fine[p].ebc || fine[p].ecd || fine[p].eda || fine[p].ca || fine[p].cb ||
+                         fine[p].cc || fine[p].cd;
+        }
+    }
+    free(fine); free(fn);
+    return rc == 0 ? (int)n : -1;
 }
 
 orc_iface *orc_iface_box(double Lx, double Ly, double Lz, int nq, double l_ec, double eps_in,

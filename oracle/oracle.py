@@ -16,6 +16,7 @@ adds the pieces that are plain linear algebra:
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -51,6 +52,8 @@ def lib():
         L.orc_iface_from_panels.restype = vp
         L.orc_iface_from_panels.argtypes = [C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp]
         L.orc_iface_free.argtypes = [vp]
+        L.orc_rect_adaptive_panels.restype = C.c_int
+        L.orc_rect_adaptive_panels.argtypes = [_dp, _dp, _ip, _ip, C.c_double, C.c_double, C.c_int, _dp, _ip]
         L.orc_iface_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_iface_nodes.argtypes = [vp, _dp]
         L.orc_iface_panels.argtypes = [vp, _dp, _dp, _ip]
@@ -130,8 +133,300 @@ def bary_row(x, w, xq):
     return r
 
 
+# --------------------------------------------------------------------------- #
+# Multi-box geometry (src/shape/box3d_multi.jl:4-386), restated in plain Python.
+# Boxes are (center (x, y, z), Lx, Ly, Lz); points are 3-tuples of floats.
+# --------------------------------------------------------------------------- #
+def _norm3(v):
+    return math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+
+
+def _dot3(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def _jmin2(x, y):
+    """Julia's min on floats: -0.0 counts as below 0.0."""
+    return y if (y < x or (y == x and math.copysign(1.0, y) < 0)) else x
+
+
+def _jmax2(x, y):
+    return y if (y > x or (y == x and math.copysign(1.0, y) > 0)) else x
+
+
+def _jmin(it):
+    it = list(it)
+    m = it[0]
+    for v in it[1:]:
+        m = _jmin2(m, v)
+    return m
+
+
+def _jmax(it):
+    it = list(it)
+    m = it[0]
+    for v in it[1:]:
+        m = _jmax2(m, v)
+    return m
+
+
+def box_faces_at_center(center, Lx, Ly, Lz):
+    """_box3d_faces_at_center (src/shape/box3d_multi.jl:4-30): six (a, b, c, d, normal)."""
+    cx, cy, cz = center
+    hx, hy, hz = Lx / 2, Ly / 2, Lz / 2
+    v = [None,
+         (cx + hx, cy + hy, cz + hz), (cx - hx, cy + hy, cz + hz), (cx - hx, cy - hy, cz + hz),
+         (cx + hx, cy - hy, cz + hz), (cx + hx, cy + hy, cz - hz), (cx - hx, cy + hy, cz - hz),
+         (cx - hx, cy - hy, cz - hz), (cx + hx, cy - hy, cz - hz)]
+    return [(v[1], v[2], v[3], v[4], (0.0, 0.0, 1.0)), (v[5], v[8], v[7], v[6], (0.0, 0.0, -1.0)),
+            (v[8], v[5], v[1], v[4], (1.0, 0.0, 0.0)), (v[7], v[3], v[2], v[6], (-1.0, 0.0, 0.0)),
+            (v[6], v[2], v[1], v[5], (0.0, 1.0, 0.0)), (v[7], v[8], v[4], v[3], (0.0, -1.0, 0.0))]
+
+
+def _plane_axes(n):
+    an = (abs(n[0]), abs(n[1]), abs(n[2]))
+    if an[0] > 0.5:
+        ax = (1, 2)
+    elif an[1] > 0.5:
+        ax = (0, 2)
+    else:
+        ax = (0, 1)
+    nax = next(k for k in range(3) if an[k] > 0.5)
+    return ax[0], ax[1], nax
+
+
+def _point(ax1, ax2, nax, u, v, w):
+    p = [0.0, 0.0, 0.0]
+    p[ax1] = u
+    p[ax2] = v
+    p[nax] = w
+    return tuple(p)
+
+
+def rect_overlap_3d(a1, b1, c1, d1, n1, a2, b2, c2, d2, n2, tol=1e-10):
+    """_rect_overlap_3d (src/shape/box3d_multi.jl:36-132) -> (bool, (a, b, c, d) or None)."""
+    if _norm3((n1[0] + n2[0], n1[1] + n2[1], n1[2] + n2[2])) > tol:
+        return False, None
+    for q in (a2, b2, c2, d2):
+        if abs(_dot3(n1, (q[0] - a1[0], q[1] - a1[1], q[2] - a1[2]))) > tol:
+            return False, None
+    ax1, ax2, nax = _plane_axes(n1)
+    c1s, c2s = (a1, b1, c1, d1), (a2, b2, c2, d2)
+    lo_u = _jmax2(_jmin(c[ax1] for c in c1s), _jmin(c[ax1] for c in c2s))
+    hi_u = _jmin2(_jmax(c[ax1] for c in c1s), _jmax(c[ax1] for c in c2s))
+    lo_v = _jmax2(_jmin(c[ax2] for c in c1s), _jmin(c[ax2] for c in c2s))
+    hi_v = _jmin2(_jmax(c[ax2] for c in c1s), _jmax(c[ax2] for c in c2s))
+    if hi_u - lo_u < tol or hi_v - lo_v < tol:
+        return False, None
+    w = a1[nax]
+    e_ab = (b1[ax1] - a1[ax1], b1[ax2] - a1[ax2])
+    e_ad = (d1[ax1] - a1[ax1], d1[ax2] - a1[ax2])
+    mk = lambda u, v: _point(ax1, ax2, nax, u, v, w)  # noqa: E731
+    if abs(e_ab[0]) > tol:
+        us, ue = (lo_u, hi_u) if e_ab[0] >= 0 else (hi_u, lo_u)
+        vs, ve = (lo_v, hi_v) if e_ad[1] >= 0 else (hi_v, lo_v)
+        return True, (mk(us, vs), mk(ue, vs), mk(ue, ve), mk(us, ve))
+    us, ue = (lo_u, hi_u) if e_ad[0] >= 0 else (hi_u, lo_u)
+    vs, ve = (lo_v, hi_v) if e_ab[1] >= 0 else (hi_v, lo_v)
+    return True, (mk(us, vs), mk(us, ve), mk(ue, ve), mk(ue, vs))
+
+
+def detect_shared_faces_3d(boxes):
+    """_detect_shared_faces_3d (src/shape/box3d_multi.jl:139-162) -> [(region, i, j, normal)]
+    with 0-based i < j and the normal pointing from box j to box i."""
+    out = []
+    for i in range(len(boxes)):
+        fi = box_faces_at_center(*boxes[i])
+        for j in range(i + 1, len(boxes)):
+            fj = box_faces_at_center(*boxes[j])
+            for (a1, b1, c1, d1, n1) in fi:
+                for (a2, b2, c2, d2, n2) in fj:
+                    ok, reg = rect_overlap_3d(a1, b1, c1, d1, n1, a2, b2, c2, d2, n2)
+                    if ok:
+                        out.append((reg, i, j, (-n1[0], -n1[1], -n1[2])))
+    return out
+
+
+def _sorted_unique(xs):
+    """sort!(unique!(xs)) with Julia's isequal/isless on floats (-0.0 before 0.0, kept apart)."""
+    seen, out = set(), []
+    for x in xs:
+        key = (x, math.copysign(1.0, x))
+        if key not in seen:
+            seen.add(key)
+            out.append(x)
+    return sorted(out, key=lambda x: (x, math.copysign(1.0, x)))
+
+
+def subtract_rects_from_face_3d(a, b, c, d, normal, shared, tol=1e-10):
+    """_subtract_rects_from_face_3d (src/shape/box3d_multi.jl:169-308) -> [(a, b, c, d, edges, corners)]."""
+    if not shared:
+        return [(a, b, c, d, (True,) * 4, (True,) * 4)]
+    ax1, ax2, nax = _plane_axes(normal)
+    w = a[nax]
+    fc = (a, b, c, d)
+    fu0, fu1 = _jmin(p[ax1] for p in fc), _jmax(p[ax1] for p in fc)
+    fv0, fv1 = _jmin(p[ax2] for p in fc), _jmax(p[ax2] for p in fc)
+    us, vs = [fu0, fu1], [fv0, fv1]
+    for r in shared:
+        for q in r:
+            us.append(q[ax1])
+            vs.append(q[ax2])
+    us = [u for u in _sorted_unique(us) if fu0 - tol <= u <= fu1 + tol]
+    vs = [v for v in _sorted_unique(vs) if fv0 - tol <= v <= fv1 + tol]
+
+    def is_shared(um, vm):
+        for r in shared:
+            if (_jmin(q[ax1] for q in r) - tol <= um <= _jmax(q[ax1] for q in r) + tol and
+                    _jmin(q[ax2] for q in r) - tol <= vm <= _jmax(q[ax2] for q in r) + tol):
+                return True
+        return False
+
+    nu, nv = len(us) - 1, len(vs) - 1
+    sh = [[False] * nv for _ in range(nu)]
+    for i in range(nu):
+        for j in range(nv):
+            if us[i + 1] - us[i] < tol or vs[j + 1] - vs[j] < tol:
+                sh[i][j] = True
+            else:
+                sh[i][j] = is_shared((us[i] + us[i + 1]) / 2, (vs[j] + vs[j + 1]) / 2)
+
+    def blocked(ci, cj):
+        return ci < 0 or ci >= nu or cj < 0 or cj >= nv or sh[ci][cj]
+
+    pc = [[any(blocked(gi + di, gj + dj) for di, dj in ((0, 0), (-1, 0), (0, -1), (-1, -1)))
+           for gj in range(nv + 1)] for gi in range(nu + 1)]
+    out = []
+    for i in range(nu):
+        for j in range(nv):
+            if sh[i][j]:
+                continue
+            mk = lambda u, v: _point(ax1, ax2, nax, u, v, w)  # noqa: E731
+            edges = (j == 0 or sh[i][j - 1], i == nu - 1 or sh[i + 1][j],
+                     j == nv - 1 or sh[i][j + 1], i == 0 or sh[i - 1][j])
+            corners = (pc[i][j], pc[i + 1][j], pc[i + 1][j + 1], pc[i][j + 1])
+            out.append((mk(us[i], vs[j]), mk(us[i + 1], vs[j]), mk(us[i + 1], vs[j + 1]), mk(us[i], vs[j + 1]),
+                        edges, corners))
+    return out
+
+
+def multi_box3d_face_regions(boxes, epses, eps_out):
+    """_multi_box3d_face_regions (src/shape/box3d_multi.jl:313-356) ->
+    [(a, b, c, d, normal, eps_in, eps_out, edges, corners)]: external pieces (box order, face
+    order) then the shared faces."""
+    shared = detect_shared_faces_3d(boxes)
+    out = []
+    for bid, box in enumerate(boxes):
+        for (fa, fb, fc, fd, fn) in box_faces_at_center(*box):
+            mine = []
+            for (reg, lo, hi, _n) in shared:
+                if bid in (lo, hi):
+                    ok, ov = rect_overlap_3d(fa, fb, fc, fd, fn, *reg, (-fn[0], -fn[1], -fn[2]))
+                    if ok:
+                        mine.append(ov)
+            for (ra, rb, rc, rd, ie, ic) in subtract_rects_from_face_3d(fa, fb, fc, fd, fn, mine):
+                out.append((ra, rb, rc, rd, fn, epses[bid], eps_out, ie, ic))
+    for (reg, lo, hi, n) in shared:
+        out.append((*reg, n, epses[hi], epses[lo], (True,) * 4, (True,) * 4))
+    return out
+
+
+def rect_adaptive_panels(a, b, c, d, normal, edges, corners, alpha, l_ec):
+    """rect_panel3d_adaptive_panels (src/shape/box3d.jl:99-133) -> (corners P x 4 x 3, is_edge P)."""
+    abcd = np.array([*a, *b, *c, *d], np.float64)
+    e = np.array([int(x) for x in edges], np.int32)
+    k = np.array([int(x) for x in corners], np.int32)
+    cap = 1024
+    while True:
+        cs = np.zeros(12 * cap)
+        ie = np.zeros(cap, np.int32)
+        n = lib().orc_rect_adaptive_panels(abcd, _f64(normal), e, k, float(l_ec), float(alpha), cap, cs, ie)
+        if n < 0:
+            raise ValueError("best_grid found no tiling")
+        if n <= cap:
+            return cs[:12 * n].reshape(n, 4, 3), ie[:n]
+        cap = n
+
+
+# --------------------------------------------------------------------------- #
+# Screened volume sources (src/core/sources.jl:315-349, 408-515), plain Python loops.
+# --------------------------------------------------------------------------- #
+def _smooth(d, bw):
+    """_smooth_screen_factor (src/core/sources.jl:321-323)."""
+    return (1.0 + math.erf(d / bw)) / 2.0
+
+
+def _box_screen(p, lo, hi, bw):
+    """_box_screen (src/core/sources.jl:325-332)."""
+    return (_smooth(p[0] - lo[0], bw) * _smooth(hi[0] - p[0], bw) * _smooth(p[1] - lo[1], bw) *
+            _smooth(hi[1] - p[1], bw) * _smooth(p[2] - lo[2], bw) * _smooth(hi[2] - p[2], bw))
+
+
+def _in_box(p, lo, hi, tol):
+    """_point_in_box (src/core/sources.jl:315-319)."""
+    return all(lo[k] - tol <= p[k] <= hi[k] + tol for k in range(3))
+
+
+def screen_one_region(pos, rho, lo, hi, eps_in, eps_out, mode, bandwidth=0.0, tol=0.0):
+    """_screened_volume_density (src/core/sources.jl:334-366): mode 'sharp', 'mix' or 'inv'."""
+    out = np.zeros(len(rho))
+    for s in range(len(rho)):
+        p = tuple(pos[s])
+        if mode == "sharp":
+            e = eps_in if _in_box(p, lo, hi, tol) else eps_out
+        else:
+            sc = _box_screen(p, lo, hi, bandwidth)
+            e = eps_in * sc + eps_out * (1.0 - sc) if mode == "mix" else 1.0 / (sc / eps_in + (1.0 - sc) / eps_out)
+        out[s] = rho[s] / e
+    return out
+
+
+def screen_boxes(pos, rho, boxes, epses, eps_out, mode, bandwidth=0.0, tol=None):
+    """_screened_volume_density_multibox (src/core/sources.jl:408-493) with the tol default of
+    screened_volume_source(boxes, ...) (:495-515); boxes = [(center, Lx, Ly, Lz)]."""
+    if tol is None:
+        tol = math.sqrt(np.finfo(float).eps) * max(max(abs(b[1]), abs(b[2]), abs(b[3])) for b in boxes)
+    los = [tuple(b[0][k] - b[1 + k] / 2 for k in range(3)) for b in boxes]
+    his = [tuple(b[0][k] + b[1 + k] / 2 for k in range(3)) for b in boxes]
+    out = np.zeros(len(rho))
+    for s in range(len(rho)):
+        p = tuple(pos[s])
+        if mode == "sharp":
+            e = eps_out
+            for b in range(len(boxes)):
+                if _in_box(p, los[b], his[b], tol):
+                    e = epses[b]
+                    break
+            out[s] = rho[s] / e
+            continue
+        acc = 0.0
+        sum_p = 0.0
+        for b in range(len(boxes)):
+            P = _box_screen(p, los[b], his[b], bandwidth)
+            acc += epses[b] * P if mode == "mix" else P / epses[b]
+            sum_p += P
+        out[s] = rho[s] / (acc + eps_out * (1.0 - sum_p)) if mode == "mix" else rho[s] * (acc + (1.0 - sum_p) / eps_out)
+    return out
+
+
 class Interface:
     """A DielectricInterface restated on the CPU (uniform n_quad)."""
+
+    @classmethod
+    def multi_box(cls, n_quad, l_ec, boxes, epses, eps_out=1.0, alpha=float(np.sqrt(2.0))):
+        """multi_dielectric_box3d (src/shape/box3d_multi.jl:358-386); boxes = [(center, Lx, Ly, Lz)]."""
+        if len(boxes) != len(epses) or not boxes:
+            raise ValueError("need one permittivity per box and at least one box")
+        cs, ns, es, ei, eo = [], [], [], [], []
+        for (ra, rb, rc, rd, fn, e_in, e_out, ie, ic) in multi_box3d_face_regions(boxes, epses, eps_out):
+            c, e = rect_adaptive_panels(ra, rb, rc, rd, fn, ie, ic, alpha, l_ec)
+            cs.append(c)
+            es.append(e)
+            ns.append(np.tile(np.asarray(fn, np.float64), (len(e), 1)))
+            ei.append(np.full(len(e), float(e_in)))
+            eo.append(np.full(len(e), float(e_out)))
+        return cls.from_panels(np.concatenate(cs), np.concatenate(ns), n_quad, np.concatenate(ei),
+                               np.concatenate(eo), np.concatenate(es))
 
     def __init__(self, handle):
         if not handle:
