@@ -48,4 +48,17 @@ from .api import (  # noqa: F401
     solve_lu,
 )
 
+from .lattice import (  # noqa: F401
+    BATCH_FORMAT_VERSION,
+    BatchResult,
+    LatticeBatch,
+    batch_result,
+    batch_volume_sources,
+    envelope_volume_source,
+    is_complete_batch,
+    load_batch_result,
+    save_batch_result,
+    solve_dielectric_lattice_batch,
+)
+
 __version__ = "0.1.0"

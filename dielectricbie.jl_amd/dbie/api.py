@@ -724,12 +724,15 @@ def solve_gmres(A, b, atol: float = 1e-6, rtol: float = 1e-6, itmax: int = 1000)
 
 def solve_dielectric_box3d_block(interface: DielectricInterface, vss, *, fmm_tol=1e-9, up_tol=1e-9, max_order=8,
                                  rtol=1e-10, atol=0.0, itmax=500, precondition=True, correct_edges=True,
+                                 screen_boxes=None, screen_epses=None, screen_eps_out=1.0,
                                  precision="fp64", inner_rtol=1e-4, max_refine=10):
     """solve_dielectric_box3d_block (src/solver/multi_rhs.jl:270-286) -> (Σ, stats).
+    With ``screen_boxes`` the sources are screened box by box (several permittivity regions);
     ``precision`` other than "fp64" selects the mixed-precision refinement solve."""
     op = batched_lhs_dielectric_box3d_fmm3d_corrected(interface, fmm_tol, up_tol, max_order,
                                                       correct_edges=correct_edges)
-    F = rhs_dielectric_box3d_fmm3d(interface, list(vss), fmm_tol)
+    F = rhs_dielectric_box3d_fmm3d(interface, list(vss), fmm_tol, screen_boxes=screen_boxes,
+                                   screen_epses=screen_epses, screen_eps_out=screen_eps_out)
     return op.solve(F, rtol=rtol, atol=atol, itmax=itmax, precondition=precondition, precision=precision,
                     inner_rtol=inner_rtol, max_refine=max_refine)
 
