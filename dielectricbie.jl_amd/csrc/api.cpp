@@ -218,11 +218,13 @@ void check_precision(int p) {
 // Y = A X on the device (X, Y: N x K with leading dims).
 // ---- launch-shape autotuner ------------------------------------------------------------------
 // The far-field contraction's source split count (and, for the split-FP16 form at K > 128, the
-// kernel variant) is picked per shape class by timing the candidates on the first apply of that
-// class: (precision, near exclusion, nodes per panel, K in 32-column groups, N to a quarter
-// octave).  The choice is kept for the process.  Different split counts sum the partial sums in
-// a different order, so results can differ from an untuned run in the last bits (well inside
-// every solve tolerance).
+// kernel variant) is picked per shape class: (precision, near exclusion, nodes per panel, K in
+// 32-column groups, N to a quarter octave).  The tuner is online: the first applies of a class
+// each run the next untried candidate with HIP events around the whole apply (far field and
+// combine), nothing is run twice and nothing waits for a timing; timings are collected when their
+// events have completed, and once every candidate has one the fastest is kept for the process.
+// Different split counts sum the partial sums in a different order, so results differ at the
+// rounding level between candidates (well inside every solve tolerance).
 struct TuneKey {
     int prec, excl, npp, kgroups, nbucket;
     bool operator<(const TuneKey &o) const {
@@ -230,8 +232,21 @@ struct TuneKey {
                std::tie(o.prec, o.excl, o.npp, o.kgroups, o.nbucket);
     }
 };
+struct TuneState {
+    std::vector<FarTune> cand;
+    std::vector<float> ms;  // per candidate; < 0 until timed
+    size_t next = 0;        // next candidate to try
+    bool done = false;
+    FarTune best;
+};
+struct PendingTiming {
+    TuneKey key;
+    size_t idx;
+    hipEvent_t e0, e1;
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, FarTune> g_tune;
+std::map<TuneKey, TuneState> g_tune;
+std::vector<PendingTiming> g_pending;
 std::atomic<int> g_autotune{-1};  // -1: not read yet (DBIE_AUTOTUNE=0 switches it off)
 
 bool autotune_on() {
@@ -268,60 +283,97 @@ std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
     return cand;
 }
 
+// the fastest timed candidate (the first one while none is timed)
+void settle(TuneState &ts) {
+    size_t bi = 0;
+    float bm = -1.f;
+    bool all = ts.next >= ts.cand.size();
+    for (size_t i = 0; i < ts.cand.size(); ++i) {
+        if (ts.ms[i] < 0) {
+            all = false;
+            continue;
+        }
+        if (bm < 0 || ts.ms[i] < bm) {
+            bm = ts.ms[i];
+            bi = i;
+        }
+    }
+    ts.best = ts.cand.empty() ? FarTune() : ts.cand[bi];
+    ts.done = all;
+}
+
+// Collects the timings whose events have completed (caller holds g_tune_mu).
+void poll_timings_locked() {
+    for (size_t i = 0; i < g_pending.size();) {
+        PendingTiming &p = g_pending[i];
+        const hipError_t q = hipEventQuery(p.e1);
+        if (q == hipErrorNotReady) {
+            ++i;
+            continue;
+        }
+        float ms = 1e30f;  // a failed query rules the candidate out
+        if (q != hipSuccess || hipEventElapsedTime(&ms, p.e0, p.e1) != hipSuccess) ms = 1e30f;
+        auto it = g_tune.find(p.key);
+        if (it != g_tune.end() && p.idx < it->second.ms.size() && !it->second.done) {
+            it->second.ms[p.idx] = ms;
+            settle(it->second);
+        }
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+        p = g_pending.back();
+        g_pending.pop_back();
+    }
+}
+
 void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
                    const FarTune &tune);
 
-// The tuned launch shape for this apply (tuning it first if its class is new).  Y receives
-// scratch results while the candidates are timed; the caller's apply overwrites it.
-FarTune tuned_shape(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
-                    bool exclude) {
-    const TuneKey key = tune_key(op, K, prec, exclude);
+void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
+    if (!autotune_on() || K < 1) {
+        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, FarTune());
+        return;
+    }
+    const TuneKey key = tune_key(op, K, prec, excludes_near(op, K, prec));
+    FarTune use;
+    bool timed = false;
+    size_t idx = 0;
+    int max_splits = 0;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
+        poll_timings_locked();
         auto it = g_tune.find(key);
-        if (it != g_tune.end()) return it->second;
-    }
-    const int64_t N = op->N();
-    const std::vector<FarTune> cand = tune_candidates(N, K, prec);
-    FarTune best = cand.empty() ? FarTune() : cand[0];
-    if (cand.size() > 1) {
-        // buffers at their largest before anything is timed; one untimed apply loads the code
-        op->part.ensure(sizeof(double) * ((int64_t)cand.back().splits * K * N + 64 * N));
-        const bool prof = op->profiling;
-        op->profiling = false;  // tuning launches stay out of the kernel profile
-        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, FarTune());
-        hipEvent_t e0, e1;
-        DBIE_HIP(hipEventCreate(&e0));
-        DBIE_HIP(hipEventCreate(&e1));
-        float best_ms = 1e30f;
-        for (const FarTune &t : cand) {
-            DBIE_HIP(hipEventRecord(e0, op->stream));
-            apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, t);
-            DBIE_HIP(hipEventRecord(e1, op->stream));
-            DBIE_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            DBIE_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = t;
-            }
+        if (it == g_tune.end()) {
+            TuneState ts;
+            ts.cand = tune_candidates(op->N(), K, prec);
+            ts.ms.assign(ts.cand.size(), -1.f);
+            if (ts.cand.size() <= 1) ts.next = ts.cand.size();
+            settle(ts);
+            it = g_tune.emplace(key, ts).first;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        op->profiling = prof;
+        TuneState &ts = it->second;
+        if (!ts.done && ts.next < ts.cand.size()) {
+            idx = ts.next++;
+            use = ts.cand[idx];
+            timed = true;
+            for (const FarTune &c : ts.cand) max_splits = std::max(max_splits, c.splits);
+        } else {
+            use = ts.best;
+        }
     }
+    if (!timed) {
+        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, use);
+        return;
+    }
+    // partial sums at their largest first, so that no timed apply allocates
+    op->part.ensure(sizeof(double) * ((int64_t)max_splits * K * op->N() + 64 * op->N()));
+    hipEvent_t e0, e1;
+    DBIE_HIP(hipEventCreate(&e0));
+    DBIE_HIP(hipEventCreate(&e1));
+    DBIE_HIP(hipEventRecord(e0, op->stream));
+    apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, use);
+    DBIE_HIP(hipEventRecord(e1, op->stream));
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = best;
-    return best;
-}
-
-void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
-    FarTune t;
-    if (autotune_on() && K > 0) {
-        const bool exclude = excludes_near(op, K, prec);
-        t = tuned_shape(op, K, X, ldx, Y, ldy, prec, exclude);
-    }
-    apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, t);
+    g_pending.push_back(PendingTiming{key, idx, e0, e1});
 }
 
 void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
@@ -1277,6 +1329,11 @@ dbie_status dbie_autotune(int enable, int clear) {
         }
         if (clear) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
+            for (PendingTiming &p : g_pending) {  // events of launches still in flight are released
+                (void)hipEventDestroy(p.e0);      // when those complete
+                (void)hipEventDestroy(p.e1);
+            }
+            g_pending.clear();
             g_tune.clear();
         }
     });
@@ -1287,13 +1344,13 @@ dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precis
         check_op(op);
         if (K < 1) fail(DBIE_BAD_ARG, "K must be >= 1");
         check_precision(precision);
-        const bool exclude = excludes_near(op, K, precision);
         FarTune t;
         t.splits = 0;
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
-            auto it = g_tune.find(tune_key(op, K, precision, exclude));
-            if (it != g_tune.end()) t = it->second;
+            poll_timings_locked();
+            auto it = g_tune.find(tune_key(op, K, precision, excludes_near(op, K, precision)));
+            if (it != g_tune.end() && it->second.done) t = it->second.best;
         }
         if (splits) *splits = t.splits;
         if (variant) *variant = t.variant;
@@ -1306,9 +1363,14 @@ dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision
         if (K < 1 || splits < 1) fail(DBIE_BAD_ARG, "K and splits must be >= 1");
         if (variant != -1 && variant != 0) fail(DBIE_BAD_ARG, "variant must be -1 or 0");
         check_precision(precision);
-        const bool exclude = excludes_near(op, K, precision);
+        TuneState ts;
+        ts.cand = {FarTune{splits, variant}};
+        ts.ms = {0.f};
+        ts.next = 1;
+        ts.done = true;
+        ts.best = ts.cand[0];
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[tune_key(op, K, precision, exclude)] = FarTune{splits, variant};
+        g_tune[tune_key(op, K, precision, excludes_near(op, K, precision))] = ts;
     });
 }
 

@@ -338,13 +338,14 @@ dbie_status dbie_op_profile_read(dbie_op *op, int kernel, int64_t *launches, dou
 /* ---- launch-shape autotuner ---------------------------------------------- */
 /* No counterpart in the reference (its far field is a CPU loop with no launch shape).  The
    far-field launch shape (source split count; for the split-FP16 form at K > 128 also the kernel
-   variant) is timed on the first apply of each shape class (precision, near exclusion, nodes per
-   panel, K in 32-column groups, N to a quarter octave) and kept for the process.  enable: 1 on
-   (the default; environment DBIE_AUTOTUNE=0 starts it off), 0 off (default shapes), -1 unchanged;
-   clear != 0 forgets the cached choices. */
+   variant) is tuned online per shape class (precision, near exclusion, nodes per panel, K in
+   32-column groups, N to a quarter octave): the first applies of a class each run one untried
+   candidate under HIP events, without extra launches or waits, and once all are timed the fastest
+   is kept for the process.  enable: 1 on (the default; environment DBIE_AUTOTUNE=0 starts it off),
+   0 off (default shapes), -1 unchanged; clear != 0 forgets the cached choices. */
 dbie_status dbie_autotune(int enable, int clear);
-/* the cached choice for this operator's class at K columns: *splits = 0 when none is cached
-   (either output may be NULL) */
+/* the settled choice for this operator's class at K columns: *splits = 0 while the class is still
+   being tuned or was never applied (either output may be NULL) */
 dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precision, int *splits, int *variant);
 /* pins the choice for this operator's class at K columns (e.g. one saved from an earlier run):
    splits >= 1 (clamped to what the launch accepts), variant -1 = default kernel, 0 = the general

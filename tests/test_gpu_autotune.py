@@ -1,5 +1,5 @@
 """GPU checks of the far-field launch-shape autotuner: every shape it may pick gives the same
-operator (to the bar of its precision), and the tuned shape is cached per shape class.
+operator (to the bar of its precision), and the online tuner settles on one shape per class.
 
 Bars: 1e-12 relative against the oracle for FP64 applies, 1e-5 for FP32 and 2e-6 for the
 split-FP16 form against the FP64 apply (per column)."""
@@ -57,31 +57,38 @@ def test_every_split_count_gives_the_same_operator(dbie, cfg, prec, K):
 
 @pytest.mark.parametrize("prec,K", [("fp64", 24), ("fp16x3", 48), ("fp16x3", 256)])
 def test_tuner_picks_and_caches_a_shape(dbie, cfg, prec, K):
+    """The online tuner tries one candidate per apply (every result stays within the bar), settles
+    once every candidate is timed, and then replays the chosen shape bit for bit."""
     g, op, ref = cfg
     X = np.random.default_rng(22).standard_normal((g.n_points, K))
     Yref = ref @ X
     dbie.autotune(True, clear=True)
     assert op.tuned_shape(K, prec) is None
-    say(f"tuned apply {prec} K={K}")
-    Y1 = op.apply(X, prec)  # first apply of the class times the candidates
-    shape = op.tuned_shape(K, prec)
-    say(f"  chosen shape {shape}")
+
+    def check(Y):
+        if prec == "fp64":
+            assert rel(Y, Yref) < 1e-12
+        else:
+            assert col_err(Y, Yref) < BOUND[prec]
+
+    say(f"online tuning {prec} K={K}")
+    shape = None
+    for n in range(60):
+        check(op.apply(X, prec))
+        shape = op.tuned_shape(K, prec)
+        if shape is not None:
+            break
+    say(f"  chosen shape {shape} after {n + 1} applies")
     assert shape is not None and shape[0] >= 1
-    Y2 = op.apply(X, prec)  # cached: same shape, same launches, same bits
+    Y1 = op.apply(X, prec)
+    Y2 = op.apply(X, prec)  # settled: same shape, same launches, same bits
     assert np.array_equal(Y1, Y2)
     assert op.tuned_shape(K, prec) == shape
-    if prec == "fp64":
-        assert rel(Y1, Yref) < 1e-12
-    else:
-        assert col_err(Y1, Yref) < BOUND[prec]
+    check(Y1)
     # switched off: default shapes, cached choices kept
     dbie.autotune(False)
     try:
-        Y3 = op.apply(X, prec)
+        check(op.apply(X, prec))
         assert op.tuned_shape(K, prec) == shape
-        if prec == "fp64":
-            assert rel(Y3, Y1) < 1e-13
-        else:
-            assert col_err(Y3, Yref) < BOUND[prec]
     finally:
         dbie.autotune(True)
