@@ -270,7 +270,8 @@ TuneKey tune_key(const dbie_op *op, int K, dbie_precision prec, bool exclude) {
 }
 
 // Candidate launch shapes for one apply: split counts the launch accepts whose partial sums fit
-// in 2 GB, each with the default kernel and (split-FP16, K > 128) the general kernel.
+// in 2 GB, each with the default kernel and (split-FP16, K > 128) the general kernel and the
+// warp-specialised kernel with one consumer wave per SIMD.
 std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
     std::vector<FarTune> cand;
     const int smax = far_field_max_splits(N);
@@ -278,7 +279,10 @@ std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
     for (int sp : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) {
         if (sp > smax || (double)sp * K * N * sizeof(double) > 2.0e9) break;
         cand.push_back(FarTune{sp, -1});
-        if (ws) cand.push_back(FarTune{sp, 0});
+        if (ws) {
+            cand.push_back(FarTune{sp, 0});
+            cand.push_back(FarTune{sp, 1});
+        }
     }
     return cand;
 }
@@ -1361,7 +1365,7 @@ dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision
     return guard([&] {
         check_op(op);
         if (K < 1 || splits < 1) fail(DBIE_BAD_ARG, "K and splits must be >= 1");
-        if (variant != -1 && variant != 0) fail(DBIE_BAD_ARG, "variant must be -1 or 0");
+        if (variant < -1 || variant > 1) fail(DBIE_BAD_ARG, "variant must be -1, 0 or 1");
         check_precision(precision);
         TuneState ts;
         ts.cand = {FarTune{splits, variant}};

@@ -703,7 +703,11 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
 // stage.  Generation, scaling, splitting and near-pair exclusion are those of
 // far_mfma_f16x3_kernel.
 constexpr int kWsSb = 32;        // sources per stage
-constexpr int kWsThreads = 768;  // 4 producer + 8 consumer waves
+// CW consumer waves: 8 (two per SIMD, each 4 target slices x 4 column blocks) or 4 (one per SIMD,
+// each 8 target slices x 4 column blocks: a quarter less LDS traffic per MFMA, twice the
+// accumulator registers); the autotuner picks between them
+template <int CW>
+constexpr int ws_threads() { return (4 + CW) * 64; }
 constexpr int kWsMaskWords = 512;  // near-mask words per target panel staged in LDS (16384 source panels)
 
 // Workgroup barrier for the producers: their LDS writes are complete (lgkmcnt(0)) and all but
@@ -718,14 +722,16 @@ __device__ inline void ws_producer_barrier() {
     asm volatile("s_barrier" ::: "memory");
 }
 
-template <int EXCL>
-__global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int K, int Kp,
+template <int EXCL, int CW>
+__global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom g, int K, int Kp,
                                                                   const _Float16 *__restrict__ zh,
                                                                   const _Float16 *__restrict__ zl,
                                                                   const unsigned *__restrict__ cmax, float gscale,
                                                                   double *__restrict__ part, int src_per_split,
                                                                   NearMask nm, const int *__restrict__ tplane) {
     static_assert(EXCL == 0 || EXCL == 1, "per-source exclusion runs in far_mfma_f16x3_kernel");
+    static_assert(CW == 4 || CW == 8, "4 or 8 consumer waves");
+    constexpr int RS = 32 / CW;  // target slices per consumer wave
     __shared__ uint4 s_a[2][8][2][64];   // [stage parity][target slice][hi/lo][lane]
     __shared__ uint4 s_b[3][16][2][64];  // [stage % 3][column block][hi/lo][lane]
     __shared__ double s_pos[3][3][kWsSb];  // [stage % 3][x, y, z][source]
@@ -897,10 +903,10 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
     // same SIMD (with the accumulators in VGPRs the two hardly overlap).
     asm volatile("" ::: "a0");
     const int cw = wave - 4;
-    const int sg = cw >> 2, cg = cw & 3;  // target slices 4 sg .. 4 sg + 3, column blocks 4 cg .. 4 cg + 3
-    v4f acc[4][4];
+    const int sg = cw >> 2, cg = cw & 3;  // target slices RS sg .. RS sg + RS - 1, column blocks 4 cg .. 4 cg + 3
+    v4f acc[RS][4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < RS; ++s)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[s][q] = v4f{0.f, 0.f, 0.f, 0.f};
     if (jc < walk.jend) {
@@ -916,9 +922,9 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
             bl[q] = s_b[bc][4 * cg + q][1][lane];
         }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const v8h ah = __builtin_bit_cast(v8h, s_a[c][4 * sg + s][0][lane]);
-            const v8h al = __builtin_bit_cast(v8h, s_a[c][4 * sg + s][1][lane]);
+        for (int s = 0; s < RS; ++s) {
+            const v8h ah = __builtin_bit_cast(v8h, s_a[c][RS * sg + s][0][lane]);
+            const v8h al = __builtin_bit_cast(v8h, s_a[c][RS * sg + s][1][lane]);
             // the three products of an accumulator written as one chain: the compiler then keeps
             // every accumulator in its own AGPRs (it interleaves the chains by itself)
 #pragma unroll
@@ -942,10 +948,10 @@ __global__ __launch_bounds__(kWsThreads) void far_ws_f16x3_kernel(DevGeom g, int
         const double cs = ldexp(out_scale, -col_exponent(cmax[col]));
         double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < RS; ++s)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t i = i0 + (4 * sg + s) * 16 + 4 * (lane >> 4) + r;
+                const int64_t i = i0 + (RS * sg + s) * 16 + 4 * (lane >> 4) + r;
                 if (i < N) dst[i] = (double)acc[s][q][r] * cs;
             }
     }
@@ -1082,12 +1088,17 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         const int64_t mask_words_split = (per / std::max(g.npp, 1)) / 32 + 2;
         if (tune.variant != 0 && ncb == 16 && ex != 2 && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
             (ex == 0 || mask_words_split <= kWsMaskWords)) {
-            if (ex == 1)
-                far_ws_f16x3_kernel<1><<<grid, kWsThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm,
-                                                                     g.tile_plane);
-            else
-                far_ws_f16x3_kernel<0><<<grid, kWsThreads, 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm,
-                                                                     g.tile_plane);
+#define DBIE_WS(E, C)                                                                                         \
+    far_ws_f16x3_kernel<E, C><<<grid, ws_threads<C>(), 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm, \
+                                                               g.tile_plane)
+            if (tune.variant == 1) {
+                if (ex == 1) DBIE_WS(1, 4);
+                else DBIE_WS(0, 4);
+            } else {
+                if (ex == 1) DBIE_WS(1, 8);
+                else DBIE_WS(0, 8);
+            }
+#undef DBIE_WS
             mark(ev1);
             return S;
         }

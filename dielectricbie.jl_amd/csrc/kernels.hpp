@@ -42,7 +42,8 @@ enum FarMode { FAR_DT = 0, FAR_S = 1, FAR_D = 2 };
 
 // Launch shape of a far-field contraction, as picked by the autotuner: splits = number of
 // source splits (0: the built-in heuristic), variant = -1: the warp-specialised split-FP16
-// kernel where it applies, 0: never.
+// kernel where it applies (two consumer waves per SIMD), 1: the same with one consumer wave per
+// SIMD, 0: never.
 struct FarTune {
     int splits = 0;
     int variant = -1;

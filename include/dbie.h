@@ -349,7 +349,7 @@ dbie_status dbie_autotune(int enable, int clear);
 dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precision, int *splits, int *variant);
 /* pins the choice for this operator's class at K columns (e.g. one saved from an earlier run):
    splits >= 1 (clamped to what the launch accepts), variant -1 = default kernel, 0 = the general
-   split-FP16 kernel */
+   split-FP16 kernel, 1 = the warp-specialised split-FP16 kernel with one consumer wave per SIMD */
 dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision, int splits, int variant);
 
 /* ---- device-pointer variants (stream-ordered on the operator's stream) ---- */

@@ -42,7 +42,7 @@ def test_every_split_count_gives_the_same_operator(dbie, cfg, prec, K):
     g, op, ref = cfg
     X = np.random.default_rng(21).standard_normal((g.n_points, K))
     Yref = ref @ X
-    variants = (-1, 0) if prec == "fp16x3" and K > 128 else (-1,)
+    variants = (-1, 0, 1) if prec == "fp16x3" and K > 128 else (-1,)
     for splits in (1, 2, 3, 4, 8):
         for var in variants:
             say(f"pinned shape {prec} K={K} splits={splits} variant={var}")
