@@ -90,7 +90,31 @@ struct dbie_op {
     // optional per-kernel timing: event pairs recorded around launches
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[3];
+    // operator applies captured as HIP graphs: an apply whose arguments (buffers, K, precision,
+    // launch shape) repeat is captured on its second call and replayed from then on
+    struct ApplyGraph {
+        const double *X;
+        int64_t ldx;
+        double *Y;
+        int64_t ldy;
+        int K, prec, splits, variant;
+        hipGraphExec_t exec;    // null: seen once, not captured yet
+        void *part, *scratch, *xt;  // workspaces the graph writes; a reallocation retires it
+        uint64_t last_use;
+    };
+    bool graphs = true;
+    bool graphs_failed = false;  // a capture failed: applies stay eager
+    std::vector<ApplyGraph> agraphs;
+    uint64_t graph_tick = 0;
+    int64_t graph_captures = 0, graph_replays = 0;
+    void drop_graphs() {
+        for (auto &a : agraphs)
+            if (a.exec) (void)hipGraphExecDestroy(a.exec);
+        agraphs.clear();
+    }
     ~dbie_op() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         for (auto &v : prof)
             for (auto &e : v) {
                 (void)hipEventDestroy(e.first);
@@ -332,9 +356,88 @@ void poll_timings_locked() {
 void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
                    const FarTune &tune);
 
+// HIP-graph form of apply_dev_cfg for repeated applies (GMRES calls the operator on the same
+// buffers every iteration).  The first call with a set of arguments runs eagerly (it also sizes
+// the workspaces and computes the fp16 scale); the second is captured on the operator's stream and
+// instantiated; later ones replay the graph.  The kernels need no memsets or host synchronisation,
+// so the graph holds kernel nodes only.  A graph is retired when a workspace it writes has been
+// reallocated by another apply; at most 8 are kept per operator (least recently used goes).
+void apply_graphed(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                   const FarTune &tune) {
+    if (!op->graphs || op->graphs_failed || op->profiling) {
+        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, tune);
+        return;
+    }
+    using AG = dbie_op::ApplyGraph;
+    auto live = [&](const AG &a) { return a.part == op->part.p && a.scratch == op->mscratch.p && a.xt == op->xt32.p; };
+    AG *hit = nullptr;
+    for (size_t i = 0; i < op->agraphs.size();) {
+        AG &a = op->agraphs[i];
+        if (a.exec && !live(a)) {  // a workspace moved: retire
+            (void)hipGraphExecDestroy(a.exec);
+            a = op->agraphs.back();
+            op->agraphs.pop_back();
+            continue;
+        }
+        if (a.X == X && a.ldx == ldx && a.Y == Y && a.ldy == ldy && a.K == K && a.prec == (int)prec &&
+            a.splits == tune.splits && a.variant == tune.variant)
+            hit = &a;
+        ++i;
+    }
+    if (hit && hit->exec) {
+        hit->last_use = ++op->graph_tick;
+        DBIE_HIP(hipGraphLaunch(hit->exec, op->stream));
+        ++op->graph_replays;
+        return;
+    }
+    if (!hit) {  // first sighting: run eagerly and remember the arguments
+        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, tune);
+        if (op->agraphs.size() >= 8) {
+            auto lru = std::min_element(op->agraphs.begin(), op->agraphs.end(),
+                                        [](const AG &a, const AG &b) { return a.last_use < b.last_use; });
+            if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+            *lru = op->agraphs.back();
+            op->agraphs.pop_back();
+        }
+        op->agraphs.push_back(AG{X, ldx, Y, ldy, K, (int)prec, tune.splits, tune.variant, nullptr, nullptr, nullptr,
+                                 nullptr, ++op->graph_tick});
+        return;
+    }
+    // second sighting: capture, instantiate, launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ok = hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        try {
+            apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, tune);
+        } catch (...) {
+            ok = false;
+        }
+        ok = hipStreamEndCapture(op->stream, &graph) == hipSuccess && ok && graph;
+        ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    if (!ok) {  // capture is not available here: stay eager for this operator
+        (void)hipGetLastError();
+        if (exec) (void)hipGraphExecDestroy(exec);
+        op->graphs_failed = true;
+        op->drop_graphs();
+        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, tune);
+        return;
+    }
+    hit->exec = exec;
+    hit->part = op->part.p;
+    hit->scratch = op->mscratch.p;
+    hit->xt = op->xt32.p;
+    hit->last_use = ++op->graph_tick;
+    ++op->graph_captures;
+    DBIE_HIP(hipGraphLaunch(exec, op->stream));
+    ++op->graph_replays;
+}
+
 void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
     if (!autotune_on() || K < 1) {
-        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, FarTune());
+        apply_graphed(op, K, X, ldx, Y, ldy, prec, FarTune());
         return;
     }
     const TuneKey key = tune_key(op, K, prec, excludes_near(op, K, prec));
@@ -365,7 +468,7 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
         }
     }
     if (!timed) {
-        apply_dev_cfg(op, K, X, ldx, Y, ldy, prec, use);
+        apply_graphed(op, K, X, ldx, Y, ldy, prec, use);
         return;
     }
     // partial sums at their largest first, so that no timed apply allocates
@@ -1027,6 +1130,10 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         op->g = gh->g;
         const Geometry &g = op->g;
         DBIE_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+        {
+            const char *e = std::getenv("DBIE_GRAPHS");
+            op->graphs = !(e && e[0] == '0');
+        }
         hipStream_t st = op->stream;
         const int64_t N = g.N;
         // node streams
@@ -1375,6 +1482,18 @@ dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision
         ts.best = ts.cand[0];
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune[tune_key(op, K, precision, excludes_near(op, K, precision))] = ts;
+    });
+}
+
+dbie_status dbie_op_graphs(dbie_op *op, int enable, int64_t *captures, int64_t *replays) {
+    return guard([&] {
+        check_op(op);
+        if (enable >= 0) {
+            DBIE_HIP(hipStreamSynchronize(op->stream));
+            op->graphs = enable != 0;  // captured graphs stay cached while switched off
+        }
+        if (captures) *captures = op->graph_captures;
+        if (replays) *replays = op->graph_replays;
     });
 }
 

@@ -350,6 +350,13 @@ class BatchedDielectricOperator:
         L.check(L.lib().dbie_autotune_set(self._h, int(K), L.PRECISIONS[precision] if isinstance(precision, str)
                                           else int(precision), int(splits), int(variant)))
 
+    def graphs(self, enable=None):
+        """Switch HIP-graph replay of repeated applies on/off (None: unchanged); returns the
+        operator's (captures, replays) so far."""
+        c = C.c_int64(); r = C.c_int64()
+        L.check(L.lib().dbie_op_graphs(self._h, -1 if enable is None else int(bool(enable)), C.byref(c), C.byref(r)))
+        return c.value, r.value
+
     def profile(self, enable=True):
         """Record HIP events around the far-field and combine launches (kernel timing)."""
         L.check(L.lib().dbie_op_profile(self._h, int(bool(enable))))

@@ -352,6 +352,14 @@ dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precis
    split-FP16 kernel, 1 = the warp-specialised split-FP16 kernel with one consumer wave per SIMD */
 dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision, int splits, int variant);
 
+/* ---- HIP graphs ------------------------------------------------------------ */
+/* No counterpart in the reference.  An operator apply whose arguments (buffers, K, precision,
+   launch shape) repeat, as in every GMRES iteration, is captured as a HIP graph on its second call
+   and replayed from then on (on by default; environment DBIE_GRAPHS=0 starts operators with it
+   off).  enable: 1 on, 0 off (eager launches; captured graphs stay cached), -1 unchanged.
+   captures / replays (may be NULL) receive the operator's counts so far. */
+dbie_status dbie_op_graphs(dbie_op *op, int enable, int64_t *captures, int64_t *replays);
+
 /* ---- device-pointer variants (stream-ordered on the operator's stream) ---- */
 void *dbie_op_stream(dbie_op *op);
 dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
