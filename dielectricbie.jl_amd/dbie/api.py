@@ -432,6 +432,13 @@ class BatchedDielectricOperator:
             L.check(status)
         return (S[:, 0] if vec else S), stats
 
+    def apply_device(self, dX: "DeviceMatrix", dY: "DeviceMatrix", precision="fp64"):
+        """Y = A X with X and Y resident in device memory (stream-ordered; returns once queued)."""
+        if dX.shape[0] != self.n or dY.shape[0] != self.n or dX.shape[1] != dY.shape[1]:
+            raise ValueError("DimensionMismatch: X and Y must be N x K")
+        L.check(L.lib().dbie_apply_device(self._h, dX.shape[1], dX.ptr, self.n, dY.ptr, self.n,
+                                          L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)))
+
     def solve_device(self, dF: "DeviceMatrix", dSigma: "DeviceMatrix", *, rtol=1e-10, atol=0.0, itmax=500,
                      restart=0, precondition=True, precision="fp64", inner_rtol=1e-4, max_refine=10,
                      check_residual=True, raise_on_fail=True):
