@@ -69,6 +69,9 @@ using namespace dbie;
 
 struct GmresWork {
     DBuf V, W, Z, U, partial, hbuf, nbuf;
+    DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
+    DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
+    DBuf PV, PT;          // block GMRES: Householder vectors (2K x K) and T factors (K x K) per Hessenberg step
 };
 
 struct dbie_op {
@@ -102,6 +105,7 @@ struct dbie_op {
         void *part, *scratch, *xt;  // workspaces the graph writes; a reallocation retires it
         uint64_t last_use;
     };
+    rocblas_handle blas = nullptr;  // on the operator's stream (block GMRES), created on first use
     bool graphs = true;
     bool graphs_failed = false;  // a capture failed: applies stay eager
     std::vector<ApplyGraph> agraphs;
@@ -115,6 +119,7 @@ struct dbie_op {
     ~dbie_op() {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graphs();
+        if (blas) (void)rocblas_destroy_handle(blas);
         for (auto &v : prof)
             for (auto &e : v) {
                 (void)hipEventDestroy(e.first);
@@ -830,6 +835,252 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     return res;
 }
 
+void blas_ok(rocblas_status st, const char *what) {
+    if (st != rocblas_status_success) fail(DBIE_HIP_ERROR, std::string(what) + ": rocBLAS status " + std::to_string((int)st));
+}
+
+rocblas_handle op_blas(dbie_op *op) {
+    if (!op->blas) {
+        blas_ok(rocblas_create_handle(&op->blas), "rocblas_create_handle");
+        blas_ok(rocblas_set_stream(op->blas, op->stream), "rocblas_set_stream");
+    }
+    return op->blas;
+}
+
+// C (p x q, leading dimension ldc) = beta C + A^T B for A (N x p) and B (N x q), both with leading
+// dimension N: a GEMM whose reduction runs over the long dimension, split into row chunks computed as
+// one strided-batched GEMM and summed by a kernel (one GEMM call over all N rows runs at a small
+// fraction of the FP64 matrix peak for these shapes).
+void gram(dbie_op *op, GmresWork &ws, const double *A, int p, const double *B, int q, int64_t N, double *C,
+          int64_t ldc, double beta) {
+    rocblas_handle h = op_blas(op);
+    const double one = 1.0, zero = 0.0;
+    const int64_t nc = std::max<int64_t>(1, std::min<int64_t>(64, N / 2048));
+    const int64_t rc = N / nc;  // rows per chunk; the last N - nc rc rows go in one more GEMM
+    ws.gpart.ensure(sizeof(double) * (size_t)nc * p * q);
+    blas_ok(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, p, q, (rocblas_int)rc,
+                                          &one, A, (rocblas_int)N, rc, B, (rocblas_int)N, rc, &zero,
+                                          ws.gpart.as<double>(), p, (rocblas_stride)p * q, (rocblas_int)nc),
+            "rocblas_dgemm_strided_batched");
+    sum_blocks(op->stream, (int)nc, p, q, ws.gpart.as<double>(), beta, C, ldc);
+    const int64_t tail = N - nc * rc;
+    if (tail > 0)
+        blas_ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, p, q, (rocblas_int)tail, &one,
+                              A + nc * rc, (rocblas_int)N, B + nc * rc, (rocblas_int)N, &one, C, (rocblas_int)ldc),
+                "rocblas_dgemm");
+}
+
+// W (N x K) = Q R with Q overwriting W, R upper triangular (K x K, leading dimension ldr): Cholesky QR
+// twice (G = W^T W = R1^T R1, W <- W R1^-1, again, R = R2 R1).  Returns false, with W unchanged in
+// meaning, when a Gram matrix is not numerically positive definite.
+bool chol_qr2(dbie_op *op, GmresWork &ws, double *W, int64_t N, int K, double *R, int64_t ldr) {
+    rocblas_handle h = op_blas(op);
+    hipStream_t st = op->stream;
+    const double one = 1.0;
+    ws.R.ensure(sizeof(double) * (size_t)K * K);
+    ws.info.ensure(sizeof(rocblas_int));
+    double *G = ws.R.as<double>();
+    for (int pass = 0; pass < 2; ++pass) {
+        gram(op, ws, W, K, W, K, N, G, K, 0.0);
+        blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, K, G, K, ws.info.as<rocblas_int>()), "rocsolver_dpotrf");
+        rocblas_int info = 0;
+        DBIE_HIP(hipMemcpyAsync(&info, ws.info.p, sizeof(info), hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+        if (info != 0) return false;
+        blas_ok(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, (rocblas_int)N, K, &one, G, K, W, (rocblas_int)N),
+                "rocblas_dtrsm");
+        if (pass == 0) {
+            triu_copy(st, K, G, K, R, ldr);
+        } else {  // R = R2 R1
+            blas_ok(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, K, K, &one, G, K, R, (rocblas_int)ldr, R, (rocblas_int)ldr),
+                    "rocblas_dtrmm");
+        }
+    }
+    return true;
+}
+
+// Block GMRES(m) (Krylov.block_gmres as called by _block_gmres_solve, src/solver/multi_rhs.jl:144-148):
+// one block Krylov space span{R, A M R, (A M)^2 R, ...} for all K columns, right preconditioner M =
+// diag(dinv).  Block Arnoldi with classical Gram-Schmidt twice as two pairs of GEMMs over the whole
+// basis; each new block is orthonormalised by Cholesky QR twice (Householder QR, rocSOLVER
+// geqrf/orgqr, when a Gram matrix is not positive definite), whose R is the sub-diagonal block of
+// the block Hessenberg Hbar.  The small least-squares problem min ||E1 S0 - Hbar Y|| is solved through
+// the normal equations with an incrementally extended Cholesky factor of Hbar^T Hbar (Hbar = V^T A M V
+// has the conditioning of the operator on the Krylov space, a second-kind operator here, so squaring
+// it costs no accuracy the 1e-10 stopping test can see), and the residual E1 S0 - Hbar Y is formed
+// explicitly: its column norms are the residual norms of the K columns.  Stops when every column meets
+// atol + rtol ||B_k|| (the reference's Krylov.jl tests the Frobenius norm of the block residual; the
+// per-column test is the stricter of the two).  X receives the solution (initial guess zero).
+GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol,
+                     int itmax, int restart, const double *dinv) {
+    const int64_t N = op->N();
+    const size_t NK = (size_t)N * K;
+    const int m = std::max(1, std::min(restart > 0 ? restart : 40, std::max(itmax, 1)));
+    GmresWork &ws = op->gw;
+    hipStream_t st = op->stream;
+    rocblas_handle h = op_blas(op);
+    const rocblas_int KK = K;
+    const int64_t ldh = (int64_t)(m + 1) * K;  // Hbar: (m+1)K x mK; residual (m+1)K x K
+    const int64_t ldr = (int64_t)m * K;        // Cholesky factor of Hbar^T Hbar: mK x mK (upper)
+    ws.W.ensure(sizeof(double) * NK);
+    ws.U.ensure(sizeof(double) * NK);
+    if (dinv) ws.Z.ensure(sizeof(double) * NK);
+    ws.V.ensure(sizeof(double) * NK * (m + 1));
+    ws.H.ensure(sizeof(double) * (size_t)ldh * m * K);
+    ws.G.ensure(sizeof(double) * (size_t)ldh * K);   // residual E1 S0 - Hbar Y
+    ws.Y.ensure(sizeof(double) * (size_t)ldh * K);   // CGS coefficients, then Y
+    ws.tau.ensure(sizeof(double) * (size_t)K);
+    ws.PV.ensure(sizeof(double) * (size_t)ldr * ldr);  // Cholesky factor
+    ws.PT.ensure(sizeof(double) * (size_t)(ldr + 2 * K) * K);  // right-hand side z, S0, K x K scratch
+    ws.partial.ensure(sizeof(double) * std::max(dots_partial_elems(N, K, 1), dots_partial_elems(ldh, K, 1)));
+    ws.nbuf.ensure(sizeof(double) * K);
+    ws.info.ensure(sizeof(rocblas_int));
+    double *Vp = ws.V.as<double>(), *Wp = ws.W.as<double>(), *Hp = ws.H.as<double>(), *Res = ws.G.as<double>();
+    double *Yp = ws.Y.as<double>(), *taup = ws.tau.as<double>(), *Rp = ws.PV.as<double>();
+    double *zp = ws.PT.as<double>(), *S0 = zp + (size_t)ldr * K, *Dp = S0 + (size_t)K * K;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    std::vector<double> bnorm, tol(K), rn;
+    col_norms(op, B, N, K, ws.partial, ws.nbuf, bnorm);
+    for (int k = 0; k < K; ++k) {
+        if (!std::isfinite(bnorm[k])) fail(DBIE_BAD_ARG, "right-hand side contains NaN or Inf");
+        tol[k] = atol + rtol * bnorm[k];
+    }
+    DBIE_HIP(hipMemsetAsync(X, 0, sizeof(double) * NK, st));
+    DBIE_HIP(hipMemcpyAsync(Wp, B, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
+    GmresOut res;
+    // V (N x K) = Q R in place (V holds a copy of W), R (upper triangular) into Rdst
+    auto orthonormalise = [&](double *V, double *Rdst, int64_t ld) {
+        if (chol_qr2(op, ws, V, N, K, Rdst, ld)) return;
+        // the Gram matrix lost definiteness (nearly dependent directions): Householder QR
+        DBIE_HIP(hipMemcpyAsync(V, Wp, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
+        blas_ok(rocsolver_dgeqrf(h, (rocblas_int)N, KK, V, (rocblas_int)N, taup), "rocsolver_dgeqrf");
+        triu_copy(st, K, V, N, Rdst, ld);
+        blas_ok(rocsolver_dorgqr(h, (rocblas_int)N, KK, KK, V, (rocblas_int)N, taup), "rocsolver_dorgqr");
+    };
+    for (;;) {
+        // residual check of the current iterate (W holds R = B - A X)
+        col_norms(op, Wp, N, K, ws.partial, ws.nbuf, rn);
+        bool all = true;
+        for (int k = 0; k < K; ++k) all = all && rn[k] <= tol[k];
+        if (all) { res.solved = true; break; }
+        if (res.iters >= itmax) break;
+        // R = V0 S0
+        DBIE_HIP(hipMemcpyAsync(Vp, Wp, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
+        orthonormalise(Vp, S0, K);
+        int j = 0;
+        bool conv = false, breakdown = false;
+        for (; j < m && res.iters < itmax; ++j) {
+            double *Vj = Vp + (size_t)j * NK, *Vn = Vp + (size_t)(j + 1) * NK;
+            double *Hc = Hp + (size_t)j * K * ldh;  // block column j of Hbar
+            const rocblas_int nb = (rocblas_int)((j + 1) * K), jk = (rocblas_int)(j * K);
+            const double *Zin = Vj;
+            if (dinv) {
+                row_scale(st, dinv, Vj, ws.Z.as<double>(), N, K);
+                Zin = ws.Z.as<double>();
+            }
+            apply_dev(op, K, Zin, N, Wp, N, prec);
+            // block CGS2: Hc = V^T W, W -= V Hc, twice (the second correction accumulated into Hc)
+            DBIE_HIP(hipMemsetAsync(Hc, 0, sizeof(double) * (size_t)ldh * K, st));
+            for (int pass = 0; pass < 2; ++pass) {
+                gram(op, ws, Vp, nb, Wp, K, N, Yp, ldh, 0.0);
+                blas_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, KK, nb, &mone,
+                                      Vp, (rocblas_int)N, Yp, (rocblas_int)ldh, &one, Wp, (rocblas_int)N),
+                        "rocblas_dgemm");
+                blas_ok(rocblas_dgeam(h, rocblas_operation_none, rocblas_operation_none, nb, KK, &one, Hc,
+                                      (rocblas_int)ldh, &one, Yp, (rocblas_int)ldh, Hc, (rocblas_int)ldh),
+                        "rocblas_dgeam");
+            }
+            // W = V_{j+1} R: R is the sub-diagonal block of block column j
+            DBIE_HIP(hipMemcpyAsync(Vn, Wp, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
+            orthonormalise(Vn, Hc + (size_t)(j + 1) * K, ldh);
+            // extend the Cholesky factor R of Hbar^T Hbar by block column j:
+            //   Rc = R^-T (Hbar_old^T h),  Rd = chol(h^T h - Rc^T Rc)
+            double *Rc = Rp + (size_t)jk * ldr, *Rd = Rc + jk;
+            if (j > 0) {
+                blas_ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, jk, KK, nb, &one, Hp,
+                                      (rocblas_int)ldh, Hc, (rocblas_int)ldh, &zero, Rc, (rocblas_int)ldr),
+                        "rocblas_dgemm");
+                blas_ok(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                      rocblas_diagonal_non_unit, jk, KK, &one, Rp, (rocblas_int)ldr, Rc, (rocblas_int)ldr),
+                        "rocblas_dtrsm");
+            }
+            blas_ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, KK, KK, nb + KK, &one, Hc,
+                                  (rocblas_int)ldh, Hc, (rocblas_int)ldh, &zero, Rd, (rocblas_int)ldr),
+                    "rocblas_dgemm");
+            if (j > 0)
+                blas_ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, KK, KK, jk, &mone, Rc,
+                                      (rocblas_int)ldr, Rc, (rocblas_int)ldr, &one, Rd, (rocblas_int)ldr),
+                        "rocblas_dgemm");
+            blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, KK, Rd, (rocblas_int)ldr, ws.info.as<rocblas_int>()),
+                    "rocsolver_dpotrf");
+            // right-hand side z = Hbar^T E1 S0: block j is Hbar(0:K, block j)^T S0
+            blas_ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, KK, KK, KK, &one, Hc,
+                                  (rocblas_int)ldh, S0, KK, &zero, zp + jk, (rocblas_int)ldr),
+                    "rocblas_dgemm");
+            ++res.iters;
+            rocblas_int info = 0;
+            DBIE_HIP(hipMemcpyAsync(&info, ws.info.p, sizeof(info), hipMemcpyDeviceToHost, st));
+            DBIE_HIP(hipStreamSynchronize(st));
+            if (info != 0) {  // Hbar lost rank: the Krylov space is exhausted; solve with the previous blocks
+                breakdown = true;
+                break;
+            }
+            // Y = (R^T R)^-1 z, residual E1 S0 - Hbar Y and its column norms
+            blas_ok(rocblas_dgeam(h, rocblas_operation_none, rocblas_operation_none, nb, KK, &one, zp, (rocblas_int)ldr,
+                                  &zero, zp, (rocblas_int)ldr, Yp, (rocblas_int)ldh),
+                    "rocblas_dgeam");
+            blas_ok(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                  rocblas_diagonal_non_unit, nb, KK, &one, Rp, (rocblas_int)ldr, Yp, (rocblas_int)ldh),
+                    "rocblas_dtrsm");
+            blas_ok(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, nb, KK, &one, Rp, (rocblas_int)ldr, Yp, (rocblas_int)ldh),
+                    "rocblas_dtrsm");
+            const rocblas_int nr = nb + KK;  // rows of Hbar's first j + 1 block columns
+            blas_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nr, KK, nb, &mone, Hp,
+                                  (rocblas_int)ldh, Yp, (rocblas_int)ldh, &zero, Res, nr),
+                    "rocblas_dgemm");
+            blas_ok(rocblas_dgeam(h, rocblas_operation_none, rocblas_operation_none, KK, KK, &one, Res, nr, &one, S0, KK,
+                                  Res, nr),
+                    "rocblas_dgeam");
+            std::vector<double> resn;
+            col_norms(op, Res, nr, K, ws.partial, ws.nbuf, resn);
+            conv = true;
+            for (int k = 0; k < K && conv; ++k) conv = resn[k] <= tol[k];
+            if (conv) { ++j; break; }
+        }
+        if (breakdown && j > 0) {  // Y for the first j blocks (the factor of those is intact)
+            const rocblas_int nb = (rocblas_int)(j * K);
+            blas_ok(rocblas_dgeam(h, rocblas_operation_none, rocblas_operation_none, nb, KK, &one, zp, (rocblas_int)ldr,
+                                  &zero, zp, (rocblas_int)ldr, Yp, (rocblas_int)ldh),
+                    "rocblas_dgeam");
+            blas_ok(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                  rocblas_diagonal_non_unit, nb, KK, &one, Rp, (rocblas_int)ldr, Yp, (rocblas_int)ldh),
+                    "rocblas_dtrsm");
+            blas_ok(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, nb, KK, &one, Rp, (rocblas_int)ldr, Yp, (rocblas_int)ldh),
+                    "rocblas_dtrsm");
+        }
+        // X += M V(:, 0:jK) Y
+        const rocblas_int nj = (rocblas_int)(j * K);
+        if (nj > 0) {
+            blas_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, KK, nj, &one, Vp,
+                                  (rocblas_int)N, Yp, (rocblas_int)ldh, &zero, ws.U.as<double>(), (rocblas_int)N),
+                    "rocblas_dgemm");
+            if (dinv) row_scale(st, dinv, ws.U.as<double>(), ws.U.as<double>(), N, K);
+            add_to(st, ws.U.as<double>(), X, N, K);
+        }
+        if (conv) { res.solved = true; break; }
+        if (res.iters >= itmax || (breakdown && j == 0)) break;
+        // restart (or continue after a breakdown): W = B - A X
+        apply_dev(op, K, X, N, Wp, N, prec);
+        sub_from(st, B, Wp, N, K);
+    }
+    DBIE_HIP(hipStreamSynchronize(st));
+    return res;
+}
+
 // Solve A X = F for K columns, F and X on the device (N x K, ld = N).  Returns DBIE_OK or
 // DBIE_NOT_CONVERGED (X and stats filled either way); throws on argument or runtime errors.
 dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const dbie_solve_opts *opts,
@@ -841,6 +1092,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     if (o.itmax < 1) fail(DBIE_BAD_ARG, "itmax must be >= 1");
     if (!(o.rtol >= 0) || !(o.atol >= 0)) fail(DBIE_BAD_ARG, "tolerances must be >= 0");
     check_precision(o.precision);
+    if (o.krylov != 0 && o.krylov != 1) fail(DBIE_BAD_ARG, "krylov must be 0 (per column) or 1 (block)");
     hipStream_t st = op->stream;
     const size_t NK = (size_t)N * K;
     DBuf dR, dD, dT, dinv, partial, hb;
@@ -858,7 +1110,8 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     dbie_solve_stats s{};
     std::vector<double> bnorm, rn;
     if (o.precision == DBIE_FP64) {
-        const GmresOut r = gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp);
+        const GmresOut r = o.krylov == 1 ? block_gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp)
+                                         : gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp);
         s.niter = r.iters;
         s.solved = r.solved;
     } else {
@@ -905,8 +1158,11 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
             hb.ensure(sizeof(double) * K);
             h2d(hb.p, mask.data(), sizeof(double) * K, st);
             col_scale(st, dR.as<double>(), hb.as<double>(), false, dR.as<double>(), N, K);
-            const GmresOut r = gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
-                                     o.itmax - s.niter, o.restart, dinvp);
+            const GmresOut r =
+                o.krylov == 1 ? block_gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
+                                            o.itmax - s.niter, o.restart, dinvp)
+                              : gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
+                                      o.itmax - s.niter, o.restart, dinvp);
             s.niter += r.iters;
             if (trace) std::fprintf(stderr, "[dbie] refine %d: inner iters %d, residual %.3e (%s)\n", it, r.iters,
                                     worst(rn), exact ? "fp64" : "updated");
@@ -1002,6 +1258,7 @@ void dbie_solve_opts_default(dbie_solve_opts *o) {
     o->inner_rtol = 1e-4;
     o->max_refine = 10;
     o->check_residual = 1;
+    o->krylov = 0;
 }
 
 dbie_status dbie_gauss_legendre(int n, double *x, double *w) {

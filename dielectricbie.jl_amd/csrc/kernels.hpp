@@ -135,6 +135,12 @@ void add_to(hipStream_t st, const double *D, double *X, int64_t N, int K);
 void sub_to(hipStream_t st, const double *D, double *X, int64_t N, int K);
 // copy with leading dimensions
 void copy_cols(hipStream_t st, const double *src, int64_t lds, double *dst, int64_t ldd, int64_t N, int K);
+// C = beta C + sum of nb partial p x q blocks stored back to back in P (C has leading dimension ldc)
+void sum_blocks(hipStream_t st, int nb, int p, int q, const double *P, double beta, double *C, int64_t ldc);
+// dst (rows x cols) = unit lower trapezoid of src (Householder vectors below a geqrf R)
+void unit_lower_copy(hipStream_t st, int rows, int cols, const double *src, int64_t lds, double *dst, int64_t ldd);
+// dst = upper triangle of the n x n block src (zeros below the diagonal)
+void triu_copy(hipStream_t st, int n, const double *src, int64_t lds, double *dst, int64_t ldd);
 
 // ---- off-surface targets (targets.hip) ----
 // Y (R x K) = A X (+ Y if accumulate) for CSR A with R rows whose columns index rows of X.

@@ -283,6 +283,36 @@ __global__ void copy_cols_kernel(const double *__restrict__ src, int64_t lds, do
     if (n < N) dst[n + (int64_t)k * ldd] = src[n + (int64_t)k * lds];
 }
 
+// dst(0:n, 0:n) = upper triangle of src(0:n, 0:n), zeros below the diagonal
+__global__ void triu_copy_kernel(int n, const double *__restrict__ src, int64_t lds, double *__restrict__ dst,
+                                 int64_t ldd) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r < n) dst[r + (int64_t)c * ldd] = r <= c ? src[r + (int64_t)c * lds] : 0.0;
+}
+
+// dst(0:rows, 0:cols) = the unit lower trapezoid of src (ones on the diagonal, zeros above): the
+// Householder vectors that geqrf leaves below R
+__global__ void unit_lower_copy_kernel(int rows, const double *__restrict__ src, int64_t lds, double *__restrict__ dst,
+                                       int64_t ldd) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r < rows) dst[r + (int64_t)c * ldd] = r > c ? src[r + (int64_t)c * lds] : (r == c ? 1.0 : 0.0);
+}
+
+// C(i, j) = beta C(i, j) + sum_b P_b(i, j) over nb partial p x q blocks stored back to back
+__global__ void sum_blocks_kernel(int nb, int p, int q, const double *__restrict__ P, double beta,
+                                  double *__restrict__ C, int64_t ldc) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t pq = (int64_t)p * q;
+    if (e >= pq) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += P[(int64_t)b * pq + e];
+    const int64_t i = e % p, j = e / p;
+    double *c = C + i + j * ldc;
+    *c = beta == 0.0 ? s : beta * *c + s;
+}
+
 inline dim3 grid2(int64_t N, int K) { return dim3((unsigned)((N + 255) / 256), (unsigned)K); }
 }  // namespace
 
@@ -362,6 +392,19 @@ void sub_to(hipStream_t st, const double *D, double *X, int64_t N, int K) {
 
 void copy_cols(hipStream_t st, const double *src, int64_t lds, double *dst, int64_t ldd, int64_t N, int K) {
     copy_cols_kernel<<<grid2(N, K), 256, 0, st>>>(src, lds, dst, ldd, N);
+}
+
+void sum_blocks(hipStream_t st, int nb, int p, int q, const double *P, double beta, double *C, int64_t ldc) {
+    const int64_t pq = (int64_t)p * q;
+    if (pq > 0) sum_blocks_kernel<<<(unsigned)((pq + 255) / 256), 256, 0, st>>>(nb, p, q, P, beta, C, ldc);
+}
+
+void unit_lower_copy(hipStream_t st, int rows, int cols, const double *src, int64_t lds, double *dst, int64_t ldd) {
+    if (rows > 0 && cols > 0) unit_lower_copy_kernel<<<grid2(rows, cols), 256, 0, st>>>(rows, src, lds, dst, ldd);
+}
+
+void triu_copy(hipStream_t st, int n, const double *src, int64_t lds, double *dst, int64_t ldd) {
+    if (n > 0) triu_copy_kernel<<<grid2(n, n), 256, 0, st>>>(n, src, lds, dst, ldd);
 }
 
 }  // namespace dbie

@@ -314,7 +314,7 @@ struct dbie_server {
     static bool same_opts(const dbie_solve_opts &a, const dbie_solve_opts &b) {
         return a.rtol == b.rtol && a.atol == b.atol && a.itmax == b.itmax && a.restart == b.restart &&
                a.precondition == b.precondition && a.precision == b.precision && a.inner_rtol == b.inner_rtol &&
-               a.max_refine == b.max_refine;
+               a.max_refine == b.max_refine && a.krylov == b.krylov;
     }
     void slot_loop() {
         std::unique_lock<std::mutex> lk(mu);

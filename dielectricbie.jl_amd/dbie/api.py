@@ -409,8 +409,10 @@ class BatchedDielectricOperator:
         return A
 
     def solve(self, F, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precondition=True, precision="fp64",
-              inner_rtol=1e-4, max_refine=10, check_residual=True, raise_on_fail=True):
-        """Block GMRES-type solve of A Σ = F on the GPU; returns (Σ, stats dict)."""
+              inner_rtol=1e-4, max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns"):
+        """GMRES solve of A Σ = F for all columns of F on the GPU; returns (Σ, stats dict).
+        ``krylov``: "columns" (one Krylov space per column, shared block matvecs) or "block"
+        (one block Krylov space for all columns, as Krylov.block_gmres)."""
         F = np.asarray(F, np.float64)
         vec = F.ndim == 1
         Ff = L.fortran(F, self.n)
@@ -422,6 +424,7 @@ class BatchedDielectricOperator:
         o.precondition = int(bool(precondition))
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
+        o.krylov = _krylov_code(krylov)
         st = L.SolveStats()
         status = L.lib().dbie_solve_block(self._h, K, L.dptr(Ff), self.n, L.dptr(S), self.n, C.byref(o),
                                           C.byref(st))
@@ -441,7 +444,7 @@ class BatchedDielectricOperator:
 
     def solve_device(self, dF: "DeviceMatrix", dSigma: "DeviceMatrix", *, rtol=1e-10, atol=0.0, itmax=500,
                      restart=0, precondition=True, precision="fp64", inner_rtol=1e-4, max_refine=10,
-                     check_residual=True, raise_on_fail=True):
+                     check_residual=True, raise_on_fail=True, krylov="columns"):
         """Solve with F and Σ resident in device memory (dbie_solve_block_device); returns stats."""
         if dF.shape[0] != self.n or dSigma.shape != dF.shape:
             raise ValueError("DimensionMismatch: device blocks must be N x K and equal in shape")
@@ -451,6 +454,7 @@ class BatchedDielectricOperator:
         o.precondition = int(bool(precondition))
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
+        o.krylov = _krylov_code(krylov)
         st = L.SolveStats()
         status = L.lib().dbie_solve_block_device(self._h, dF.shape[1], dF.ptr, self.n, dSigma.ptr, self.n,
                                                  C.byref(o), C.byref(st))
@@ -472,6 +476,13 @@ class BatchedDielectricOperator:
         U = np.zeros((M, K), order="F")
         L.check(L.lib().dbie_pottrg(self._h, M, L.dptr(t.ravel()), K, L.dptr(Sf), self.n, L.dptr(U), M))
         return U[:, 0] if vec else U
+
+
+def _krylov_code(krylov) -> int:
+    codes = {"columns": 0, "block": 1}
+    if krylov not in codes:
+        raise ValueError('krylov must be "columns" or "block"')
+    return codes[krylov]
 
 
 class DeviceMatrix:

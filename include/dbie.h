@@ -89,6 +89,9 @@ typedef struct {
     int max_refine;          /* mixed: maximum FP64 refinement steps (default 10) */
     int check_residual;      /* FP64 path: evaluate the true FP64 residual after the solve
                                 (one extra matvec; fills max_rel_residual; default 1) */
+    int krylov;              /* 0 = GMRES per column on a shared block matvec (default),
+                                1 = block GMRES: one block Krylov space for all K columns
+                                (Krylov.block_gmres, src/solver/multi_rhs.jl:144-148) */
 } dbie_solve_opts;
 
 typedef struct {

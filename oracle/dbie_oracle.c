@@ -350,7 +350,7 @@ static void divide_tpanel(const tpanel *t, int nx, int ny, tpanel *out) {
 
 /* rect_panel3d_adaptive_panels (src/shape/box3d.jl:99-133): best_grid rough split of the root,
    then a FIFO of 2x2 splits of edge/corner panels longer than l_ec.  Appends the fine panels
-   (with normal nrm) to *fine/*fn (count *n, capacity *cap).  Returns 0, or -1 when best_grid
+   (with normal nrm) to the arrays fine and fn (count n, capacity cap).  Returns 0, or -1 when best_grid
    finds no tiling. */
 static int rect_adaptive(const tpanel *root, const double nrm[3], double l_ec, double alpha, tpanel **fine,
                          double (**fn)[3], size_t *n, size_t *cap) {
