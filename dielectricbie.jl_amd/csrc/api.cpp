@@ -67,8 +67,23 @@ void ensure_device() {
 
 using namespace dbie;
 
+// The Krylov space of one low-precision inner solve, kept for the refinements after it: the
+// basis V (nblk N x K blocks, in GmresWork::V1) and per column the Arnoldi steps k1, the Givens
+// rotations cs/sn that reduced the (k1+1) x k1 Hessenberg H to R1 (stored in R, layout of the
+// solve's Hessenberg), and omega = Omega^T e_k1 with Omega = G_(k1-1) ... G_0, the unit vector
+// orthogonal to range(H).  (A M) V_k1 = V_(k1+1) H, so C = V_(k1+1) Omega^T[:, :k1] is an orthonormal
+// basis of the image of U = V_k1 R1^-1 (A M U = C), the recycled pair of GCRO.
+struct KrylovRecycle {
+    bool valid = false;
+    int nblk = 0, m = 0;
+    std::vector<int> k1;
+    std::vector<double> cs, sn, R, omega;
+};
+
 struct GmresWork {
     DBuf V, W, Z, U, partial, hbuf, nbuf;
+    DBuf V1, abuf;       // recycled Krylov basis of an earlier inner solve, its projection coefficients
+    KrylovRecycle rec;
     DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
     DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
     DBuf PV, PT;          // block GMRES: Householder vectors (2K x K) and T factors (K x K) per Hessenberg step
@@ -688,8 +703,14 @@ struct GmresOut {
 // GMRES(m) for K independent columns sharing each block matvec.  Right
 // preconditioner: diagonal dinv (dielectric_diagonal_scaling).  X receives the
 // solution (initial guess zero).  tol_k = atol + rtol * ||B_k||.
+//
+// keep: store this solve's last Krylov cycle for later solves with the same operator.
+// use: GCRO deflation with a stored space (C, U), A M U = C: every cycle starts from
+// r0 = (I - C C^T) r with c = C^T r, each new Arnoldi vector is made orthogonal to C as well
+// (b_j = C^T A M v_j), and the cycle's correction is M (U (c - B y) + V y), which minimises the
+// residual over span(U) + the new Krylov space: the refinement continues the earlier space.
 GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol, int itmax,
-               int restart, const double *dinv) {
+               int restart, const double *dinv, KrylovRecycle *keep = nullptr, const KrylovRecycle *use = nullptr) {
     const int64_t N = op->N();
     const size_t NK = (size_t)N * K;
     const int m = std::max(1, std::min(restart > 0 ? restart : 100, std::max(itmax, 1)));
@@ -697,14 +718,18 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     // on demand (most solves stop long before the restart length)
     GmresWork &ws = op->gw;
     DBuf &W = ws.W, &Z = ws.Z, &U = ws.U, &partial = ws.partial, &hbuf = ws.hbuf, &nbuf = ws.nbuf;
+    const bool defl = use && use->valid && use->nblk > 0;
+    const int nb1 = defl ? use->nblk : 0, um = defl ? use->m : 0;
     W.ensure(sizeof(double) * NK);
     U.ensure(sizeof(double) * NK);
     if (dinv) Z.ensure(sizeof(double) * NK);
-    partial.ensure(sizeof(double) * std::max(dots_partial_elems(N, K, m + 1), dots_partial_elems(N, K, 1)));
+    partial.ensure(sizeof(double) * std::max(dots_partial_elems(N, K, std::max(m + 1, nb1)), dots_partial_elems(N, K, 1)));
     hbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
     nbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
+    if (defl) ws.abuf.ensure(sizeof(double) * (size_t)nb1 * K);
     hipStream_t st = op->stream;
     double *Wp = W.as<double>();
+    const double *V1p = defl ? ws.V1.as<double>() : nullptr;
     // basis capacity in N x K blocks; growing copies the blocks already built
     auto basis = [&](int blocks) -> double * {
         blocks = std::min(blocks, m + 1);
@@ -722,20 +747,96 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     };
     double *Vp = basis(2);
 
+    // Wd <- (I - C C^T) Wd; t (K x (um+1)) receives Omega V1^T Wd per column, whose first k1
+    // entries are C^T Wd.  C C^T = V1 (I - omega omega^T) V1^T on the span of V1_(k1+1).
+    std::vector<double> tproj;
+    auto deflate = [&](double *Wd) {
+        block_dots(st, V1p, nb1, Wd, N, K, partial.as<double>(), ws.abuf.as<double>());
+        std::vector<double> a((size_t)nb1 * K), coef((size_t)nb1 * K, 0.0);
+        DBIE_HIP(hipMemcpyAsync(a.data(), ws.abuf.p, sizeof(double) * a.size(), hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipStreamSynchronize(st));
+        tproj.assign((size_t)(um + 1) * K, 0.0);
+        for (int k = 0; k < K; ++k) {
+            const int k1 = use->k1[k];
+            if (k1 <= 0) continue;
+            const double *om = &use->omega[(size_t)k * (um + 1)];
+            double wa = 0.0;
+            for (int i = 0; i <= k1; ++i) wa += om[i] * a[(size_t)i * K + k];
+            for (int i = 0; i <= k1; ++i) coef[(size_t)i * K + k] = a[(size_t)i * K + k] - om[i] * wa;
+            double *tk = &tproj[(size_t)k * (um + 1)];
+            for (int i = 0; i <= k1; ++i) tk[i] = a[(size_t)i * K + k];
+            const double *c = &use->cs[(size_t)k * um], *s = &use->sn[(size_t)k * um];
+            for (int i = 0; i < k1; ++i) {
+                const double u = c[i] * tk[i] + s[i] * tk[i + 1];
+                tk[i + 1] = -s[i] * tk[i] + c[i] * tk[i + 1];
+                tk[i] = u;
+            }
+        }
+        h2d(ws.abuf.p, coef.data(), sizeof(double) * coef.size(), st);
+        block_subtract(st, V1p, nb1, ws.abuf.as<double>(), Wd, N, K);
+    };
+
     DBIE_HIP(hipMemsetAsync(X, 0, sizeof(double) * NK, st));
     // r0 = B (x0 = 0) lives in W
     DBIE_HIP(hipMemcpyAsync(Wp, B, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
     std::vector<double> beta, tol(K), H((size_t)(m + 1) * m * K), cs((size_t)m * K), sn((size_t)m * K),
         gv((size_t)(m + 1) * K), hh((size_t)(m + 1) * K), yv((size_t)m * K);
+    std::vector<double> cvec, Bm;  // deflation: C^T r of the cycle's start (um per column), B (um x m per column)
     std::vector<int> nconv(K);
     std::vector<char> active(K), conv(K);
     GmresOut res;
+    if (defl) {  // tolerances from ||B||, before the first projection
+        col_norms(op, Wp, N, K, partial, nbuf, beta);
+        for (int k = 0; k < K; ++k) {
+            if (!std::isfinite(beta[k])) fail(DBIE_BAD_ARG, "right-hand side contains NaN or Inf");
+            tol[k] = atol + rtol * beta[k];
+        }
+    }
+    // X += M (V y + U (c - B y)) for the cycle just ended (y = 0 for inactive columns)
+    auto update = [&](int nmax) {
+        DBIE_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * NK, st));
+        if (nmax > 0) {
+            h2d(hbuf.p, yv.data(), sizeof(double) * (size_t)nmax * K, st);
+            block_combine(st, Vp, nmax, hbuf.as<double>(), U.as<double>(), N, K);
+        }
+        if (defl) {
+            std::vector<double> z((size_t)nb1 * K, 0.0);
+            for (int k = 0; k < K; ++k) {
+                const int k1 = use->k1[k];
+                if (k1 <= 0) continue;
+                std::vector<double> rhs(k1);
+                for (int i = 0; i < k1; ++i) {
+                    double v = cvec[(size_t)k * um + i];
+                    for (int j = 0; j < nconv[k]; ++j)
+                        v -= Bm[((size_t)k * um + i) * m + j] * yv[(size_t)j * K + k];
+                    rhs[i] = v;
+                }
+                const double *Rk = &use->R[(size_t)k * (um + 1) * um];
+                for (int i = k1 - 1; i >= 0; --i) {
+                    double v = rhs[i];
+                    for (int l = i + 1; l < k1; ++l) v -= Rk[(size_t)l * (um + 1) + i] * z[(size_t)l * K + k];
+                    z[(size_t)i * K + k] = v / Rk[(size_t)i * (um + 1) + i];
+                }
+            }
+            h2d(ws.abuf.p, z.data(), sizeof(double) * z.size(), st);
+            block_combine(st, V1p, nb1, ws.abuf.as<double>(), U.as<double>(), N, K);
+        }
+        if (dinv) row_scale(st, dinv, U.as<double>(), U.as<double>(), N, K);
+        add_to(st, U.as<double>(), X, N, K);
+    };
     bool first = true;
     for (;;) {
+        if (defl) {
+            deflate(Wp);
+            cvec.assign((size_t)um * K, 0.0);
+            for (int k = 0; k < K; ++k)
+                for (int i = 0; i < use->k1[k]; ++i) cvec[(size_t)k * um + i] = tproj[(size_t)k * (um + 1) + i];
+            Bm.assign((size_t)K * um * m, 0.0);
+        }
         col_norms(op, Wp, N, K, partial, nbuf, beta);
         bool any = false;
         for (int k = 0; k < K; ++k) {
-            if (first) {
+            if (first && !defl) {
                 if (!std::isfinite(beta[k])) fail(DBIE_BAD_ARG, "right-hand side contains NaN or Inf");
                 tol[k] = atol + rtol * beta[k];
             }
@@ -745,8 +846,14 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
             any = any || active[k];
         }
         first = false;
-        if (!any) { res.solved = true; break; }
-        if (res.iters >= itmax) break;
+        if (!any || res.iters >= itmax) {
+            if (defl) {  // the projection alone: X += M U c
+                std::fill(yv.begin(), yv.end(), 0.0);
+                update(0);
+            }
+            res.solved = !any;
+            break;
+        }
         // V0 = r / beta
         h2d(nbuf.p, beta.data(), sizeof(double) * K, st);
         col_scale(st, Wp, nbuf.as<double>(), true, Vp, N, K);
@@ -762,6 +869,12 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                 Zin = Z.as<double>();
             }
             apply_dev(op, K, Zin, N, Wp, N, prec);
+            if (defl) {
+                deflate(Wp);
+                for (int k = 0; k < K; ++k)
+                    for (int i = 0; i < use->k1[k]; ++i)
+                        Bm[((size_t)k * um + i) * m + j] = tproj[(size_t)k * (um + 1) + i];
+            }
             // classical Gram-Schmidt, twice (CGS2)
             block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), hbuf.as<double>());
             block_subtract(st, Vp, j + 1, hbuf.as<double>(), Wp, N, K);
@@ -818,11 +931,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         }
         int nmax = 0;
         for (int k = 0; k < K; ++k) nmax = std::max(nmax, nconv[k]);
-        h2d(hbuf.p, yv.data(), sizeof(double) * (size_t)std::max(nmax, 1) * K, st);
-        DBIE_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * NK, st));
-        block_combine(st, Vp, nmax, hbuf.as<double>(), U.as<double>(), N, K);
-        if (dinv) row_scale(st, dinv, U.as<double>(), U.as<double>(), N, K);
-        add_to(st, U.as<double>(), X, N, K);
+        update(nmax);
         bool all_conv = true;
         for (int k = 0; k < K; ++k) all_conv = all_conv && conv[k];
         if (all_conv) { res.solved = true; break; }
@@ -832,6 +941,35 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         sub_from(st, B, Wp, N, K);
     }
     DBIE_HIP(hipStreamSynchronize(st));
+    if (keep) {
+        keep->valid = false;
+        int nmax = 0;
+        for (int k = 0; k < K; ++k) nmax = std::max(nmax, nconv[k]);
+        if (nmax > 0 && !defl) {
+            // the last cycle's basis moves to V1 (its buffer is reused by the next solve's basis)
+            std::swap(ws.V.p, ws.V1.p);
+            std::swap(ws.V.bytes, ws.V1.bytes);
+            keep->nblk = nmax + 1;
+            keep->m = m;
+            keep->k1 = nconv;
+            keep->omega.assign((size_t)K * (m + 1), 0.0);
+            for (int k = 0; k < K; ++k) {
+                const int k1 = nconv[k];
+                double *om = &keep->omega[(size_t)k * (m + 1)];
+                om[k1] = 1.0;
+                const double *c = &cs[(size_t)k * m], *s = &sn[(size_t)k * m];
+                for (int i = k1 - 1; i >= 0; --i) {  // omega = G_0^T ... G_(k1-1)^T e_k1
+                    const double x0 = om[i], x1 = om[i + 1];
+                    om[i] = c[i] * x0 - s[i] * x1;
+                    om[i + 1] = s[i] * x0 + c[i] * x1;
+                }
+            }
+            keep->cs = std::move(cs);
+            keep->sn = std::move(sn);
+            keep->R = std::move(H);
+            keep->valid = true;
+        }
+    }
     return res;
 }
 
@@ -1158,11 +1296,16 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
             hb.ensure(sizeof(double) * K);
             h2d(hb.p, mask.data(), sizeof(double) * K, st);
             col_scale(st, dR.as<double>(), hb.as<double>(), false, dR.as<double>(), N, K);
+            // the first inner solve keeps its Krylov space, the later ones deflate it
+            KrylovRecycle &rec = op->gw.rec;
+            if (it == 0) rec.valid = false;
+            const bool recyc = o.recycle != 0 && o.krylov == 0;
             const GmresOut r =
                 o.krylov == 1 ? block_gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
                                             o.itmax - s.niter, o.restart, dinvp)
                               : gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
-                                      o.itmax - s.niter, o.restart, dinvp);
+                                      o.itmax - s.niter, o.restart, dinvp, recyc && it == 0 ? &rec : nullptr,
+                                      recyc && it > 0 ? &rec : nullptr);
             s.niter += r.iters;
             if (trace) std::fprintf(stderr, "[dbie] refine %d: inner iters %d, residual %.3e (%s)\n", it, r.iters,
                                     worst(rn), exact ? "fp64" : "updated");
@@ -1259,6 +1402,7 @@ void dbie_solve_opts_default(dbie_solve_opts *o) {
     o->max_refine = 10;
     o->check_residual = 1;
     o->krylov = 0;
+    o->recycle = 1;
 }
 
 dbie_status dbie_gauss_legendre(int n, double *x, double *w) {

@@ -104,7 +104,8 @@ size_t estimate_op_bytes(int64_t N, int nq) {
     const double blocks = 50.0 * P;
     return (size_t)(N * 8 * 24 + blocks * npp * npp * 16.0 + blocks * 8 + 64.0 * 1024 * 1024);
 }
-// Krylov basis (about 30 blocks), work vectors and the far-field partial sums for K columns.
+// Krylov basis and the recycled basis of mixed-precision refinements (about 30 blocks together),
+// work vectors and the far-field partial sums for K columns.
 size_t estimate_ws_bytes(int64_t N, int K) { return (size_t)N * K * 8 * 40 + (size_t)N * K * 8 * 6; }
 
 }  // namespace
@@ -314,7 +315,8 @@ struct dbie_server {
     static bool same_opts(const dbie_solve_opts &a, const dbie_solve_opts &b) {
         return a.rtol == b.rtol && a.atol == b.atol && a.itmax == b.itmax && a.restart == b.restart &&
                a.precondition == b.precondition && a.precision == b.precision && a.inner_rtol == b.inner_rtol &&
-               a.max_refine == b.max_refine && a.krylov == b.krylov;
+               a.max_refine == b.max_refine && a.krylov == b.krylov &&
+               a.recycle == b.recycle;
     }
     void slot_loop() {
         std::unique_lock<std::mutex> lk(mu);

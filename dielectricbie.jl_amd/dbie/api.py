@@ -409,10 +409,13 @@ class BatchedDielectricOperator:
         return A
 
     def solve(self, F, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precondition=True, precision="fp64",
-              inner_rtol=1e-4, max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns"):
+              inner_rtol=1e-4, max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns",
+              recycle=True):
         """GMRES solve of A Σ = F for all columns of F on the GPU; returns (Σ, stats dict).
         ``krylov``: "columns" (one Krylov space per column, shared block matvecs) or "block"
-        (one block Krylov space for all columns, as Krylov.block_gmres)."""
+        (one block Krylov space for all columns, as Krylov.block_gmres).  ``recycle`` (mixed
+        precision, per-column Krylov): refinements after the first deflate the first inner solve's
+        Krylov space instead of starting a new one."""
         F = np.asarray(F, np.float64)
         vec = F.ndim == 1
         Ff = L.fortran(F, self.n)
@@ -425,6 +428,7 @@ class BatchedDielectricOperator:
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
         o.krylov = _krylov_code(krylov)
+        o.recycle = int(bool(recycle))
         st = L.SolveStats()
         status = L.lib().dbie_solve_block(self._h, K, L.dptr(Ff), self.n, L.dptr(S), self.n, C.byref(o),
                                           C.byref(st))
@@ -444,7 +448,7 @@ class BatchedDielectricOperator:
 
     def solve_device(self, dF: "DeviceMatrix", dSigma: "DeviceMatrix", *, rtol=1e-10, atol=0.0, itmax=500,
                      restart=0, precondition=True, precision="fp64", inner_rtol=1e-4, max_refine=10,
-                     check_residual=True, raise_on_fail=True, krylov="columns"):
+                     check_residual=True, raise_on_fail=True, krylov="columns", recycle=True):
         """Solve with F and Σ resident in device memory (dbie_solve_block_device); returns stats."""
         if dF.shape[0] != self.n or dSigma.shape != dF.shape:
             raise ValueError("DimensionMismatch: device blocks must be N x K and equal in shape")
@@ -455,6 +459,7 @@ class BatchedDielectricOperator:
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
         o.krylov = _krylov_code(krylov)
+        o.recycle = int(bool(recycle))
         st = L.SolveStats()
         status = L.lib().dbie_solve_block_device(self._h, dF.shape[1], dF.ptr, self.n, dSigma.ptr, self.n,
                                                  C.byref(o), C.byref(st))

@@ -92,6 +92,10 @@ typedef struct {
     int krylov;              /* 0 = GMRES per column on a shared block matvec (default),
                                 1 = block GMRES: one block Krylov space for all K columns
                                 (Krylov.block_gmres, src/solver/multi_rhs.jl:144-148) */
+    int recycle;             /* mixed precision with krylov = 0: the inner solves after the first
+                                deflate the first one's Krylov space (GCRO: minimise over it and
+                                keep new directions orthogonal to its image), so the refinement
+                                continues that space instead of restarting (default 1) */
 } dbie_solve_opts;
 
 typedef struct {

@@ -188,6 +188,31 @@ def test_mixed_precision_solve(dbie, cfg1, precision, K):
         assert rel(Sm[:, k], S64[:, k]) < 1e-8
 
 
+@pytest.mark.parametrize("precision,restart", [("fp16x3", 0), ("fp16x3", 4), ("bf16", 0), ("fp32", 3)])
+def test_recycled_refinement(dbie, cfg1, precision, restart):
+    """Refinements that deflate the first inner solve's Krylov space (GCRO) reach the same
+    FP64-verified tolerance as restarted ones, with no more inner iterations; a zero column and
+    short restart cycles (deflation re-applied at each cycle start) included."""
+    g, op, ref_if, ref = cfg1
+    K = 12
+    say(f"recycled vs restarted refinement ({precision}, restart={restart})")
+    rng = np.random.default_rng(14)
+    F = dbie.rhs_point_charges(g, rng.uniform(-0.25, 0.25, (K, 3)), rng.uniform(-1, 1, K), 4.0)
+    F[:, 3] = 0.0
+    S64, _ = op.solve(F, rtol=1e-12)
+    out = {}
+    for rec in (False, True):
+        Sm, st = op.solve(F, rtol=1e-11, precision=precision, inner_rtol=1e-5, restart=restart, recycle=rec)
+        assert st["solved"] and st["max_rel_residual"] <= 1e-11, st
+        assert np.all(Sm[:, 3] == 0.0)
+        for k in range(K):
+            if k != 3:
+                assert rel(Sm[:, k], S64[:, k]) < 1e-8, (rec, k)
+        out[rec] = st
+    say(f"  inner iterations: restarted {out[False]['niter']}, recycled {out[True]['niter']}")
+    assert out[True]["niter"] <= out[False]["niter"] + 1
+
+
 def test_rhs_volume_screened(dbie, O, cfg1):
     g, op, ref_if, ref = cfg1
     say("volume RHS with sharp screening vs oracle")
