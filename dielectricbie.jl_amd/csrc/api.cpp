@@ -720,6 +720,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     DBuf &W = ws.W, &Z = ws.Z, &U = ws.U, &partial = ws.partial, &hbuf = ws.hbuf, &nbuf = ws.nbuf;
     const bool defl = use && use->valid && use->nblk > 0;
     const int nb1 = defl ? use->nblk : 0, um = defl ? use->m : 0;
+    const int cgs_passes = prec == DBIE_FP64 ? 2 : 1;
     W.ensure(sizeof(double) * NK);
     U.ensure(sizeof(double) * NK);
     if (dinv) Z.ensure(sizeof(double) * NK);
@@ -875,14 +876,18 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                     for (int i = 0; i < use->k1[k]; ++i)
                         Bm[((size_t)k * um + i) * m + j] = tproj[(size_t)k * (um + 1) + i];
             }
-            // classical Gram-Schmidt, twice (CGS2)
+            // classical Gram-Schmidt, twice (CGS2) for FP64 solves; once for the low-precision inner
+            // solves of the refinement, whose operator is accurate to ~1e-7 and whose tolerance is
+            // ~1e-5: the orthogonality CGS1 keeps is far below both
             block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), hbuf.as<double>());
             block_subtract(st, Vp, j + 1, hbuf.as<double>(), Wp, N, K);
-            block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), nbuf.as<double>());
-            block_subtract(st, Vp, j + 1, nbuf.as<double>(), Wp, N, K);
-            std::vector<double> h1((size_t)(j + 1) * K), h2((size_t)(j + 1) * K);
+            std::vector<double> h1((size_t)(j + 1) * K), h2((size_t)(j + 1) * K, 0.0);
+            if (cgs_passes == 2) {
+                block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), nbuf.as<double>());
+                block_subtract(st, Vp, j + 1, nbuf.as<double>(), Wp, N, K);
+                DBIE_HIP(hipMemcpyAsync(h2.data(), nbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
+            }
             DBIE_HIP(hipMemcpyAsync(h1.data(), hbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
-            DBIE_HIP(hipMemcpyAsync(h2.data(), nbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
             std::vector<double> hn;
             col_norms(op, Wp, N, K, partial, nbuf, hn);  // synchronizes the stream
             h2d(nbuf.p, hn.data(), sizeof(double) * K, st);
