@@ -213,6 +213,28 @@ def test_recycled_refinement(dbie, cfg1, precision, restart):
     assert out[True]["niter"] <= out[False]["niter"] + 1
 
 
+@pytest.mark.parametrize("nq", [2, 3, 5])
+def test_near_mfma_row_major_blocks(dbie, O, nq):
+    """n_quad = 2, 3, 5 (4x4, 9x9, 25x25 near blocks, row-major): the FP64 MFMA near kernel (K >= 16;
+    256-column groups, a partial last group) matches the oracle to 1e-12 and the one-thread-per-output
+    kernel (K < 16) to rounding; the FP32-block forms stay within their precision's bar."""
+    say(f"near MFMA kernel, row-major {nq * nq}x{nq * nq} blocks")
+    g = dbie.single_dielectric_box3d(1.2, 0.8, 0.6, nq, 0.2, 4.0, 1.0)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
+    ref = O.CorrectedOperator(O.Interface.box(1.2, 0.8, 0.6, nq, 0.2, 4.0, 1.0))
+    rng = np.random.default_rng(30 + nq)
+    for K in (16, 40, 300):
+        X = rng.standard_normal((g.n_points, K))
+        Y = op.apply(X, "fp64")
+        assert rel(Y, ref @ X) < 1e-12, K
+        Yv = op.apply(X[:, :8], "fp64")  # K < 16: one thread per output
+        assert rel(Y[:, :8], Yv) < 1e-14, K
+        for prec, bound in (("fp32", 1e-5), ("fp16x3", 2e-6)):
+            Yp = op.apply(X, prec)
+            err = max(rel(Yp[:, k], Y[:, k]) for k in range(K))
+            assert err < bound, (K, prec, err)
+
+
 def test_rhs_volume_screened(dbie, O, cfg1):
     g, op, ref_if, ref = cfg1
     say("volume RHS with sharp screening vs oracle")
