@@ -29,6 +29,8 @@ struct dbie_tmap {
     DBuf x, y, z, w, trg;                      // refined nodes, targets
     DBuf p_ptr, p_col, p_val;                  // prolongation (N_ref rows)
     DBuf pt_ptr, pt_col, pt_val;               // transposed prolongation (N rows)
+    DBuf par_first, par_count;                 // refined nodes of each coarse panel (consecutive)
+    bool par_blocks = false;                   // the prolongation is block-structured by coarse panel
     DBuf c_ptr, c_col, c_val;                  // corrections (M rows)
     DBuf xr, xin, uout;                        // workspaces
     DevGeom dg;
@@ -84,6 +86,34 @@ void make_resident(dbie_tmap *tm) {
         upload(tm->pt_ptr, tp.data(), tp.size(), st);
         upload(tm->pt_col, tc.data(), tc.size(), st);
         upload(tm->pt_val, tv.data(), tv.size(), st);
+    }
+    {  // refined nodes per coarse panel: consecutive (the refinement finishes a panel's subtree
+       // before the next panel), interpolating from that panel's nodes only
+        const int64_t P = tm->N / std::max(1, h.ref.nq * h.ref.nq);
+        const int npp = h.ref.nq * h.ref.nq;
+        std::vector<int64_t> first(P, -1);
+        std::vector<int> count(P, 0);
+        bool ok = P * npp == tm->N;
+        int prev = -1;
+        for (size_t qp = 0; qp < h.parent.size() && ok; ++qp) {
+            const int c = h.parent[qp];
+            if (c < 0 || c >= P) { ok = false; break; }
+            if (c != prev) {
+                if (first[c] >= 0) { ok = false; break; }  // a second run of the same parent
+                first[c] = (int64_t)qp * npp;
+                prev = c;
+            }
+            count[c] += npp;
+            for (int64_t r = (int64_t)qp * npp; r < (int64_t)(qp + 1) * npp && ok; ++r)
+                for (int64_t e = h.p_ptr[r]; e < h.p_ptr[r + 1]; ++e)
+                    if (h.p_col[e] / npp != c) { ok = false; break; }
+        }
+        for (int64_t c = 0; c < P && ok; ++c) ok = first[c] >= 0;
+        tm->par_blocks = ok && npp <= 32;
+        if (tm->par_blocks) {
+            upload(tm->par_first, first.data(), first.size(), st);
+            upload(tm->par_count, count.data(), count.size(), st);
+        }
     }
     DevGeom &d = tm->dg;
     d.N = R.N;
@@ -164,8 +194,12 @@ void reaction_dev(dbie_tmap *tm, int G, const int64_t *grp_ptr, const double *q,
     dR.ensure(sizeof(double) * (size_t)G * K);
     reaction_q(st, tm->dg, G, dgrp.as<int64_t>(), tm->trg.as<double>(), dq.as<double>(), dQt.as<double>());
     scatter_add(st, (int64_t)cidx.size(), dci.as<int64_t>(), dcv.as<double>(), dQt.as<double>());
-    csr_spmm(st, N, tm->pt_ptr.as<int64_t>(), tm->pt_col.as<int>(), tm->pt_val.as<double>(), G, dQt.as<double>(), Nr,
-             dQct.as<double>(), N, false);
+    const int npp = h.ref.nq * h.ref.nq;
+    if (!(tm->par_blocks && prolong_t(st, (int)(N / npp), npp, tm->par_first.as<int64_t>(), tm->par_count.as<int>(),
+                                      tm->p_ptr.as<int64_t>(), tm->p_col.as<int>(), tm->p_val.as<double>(), G,
+                                      dQt.as<double>(), Nr, dQct.as<double>(), N)))
+        csr_spmm(st, N, tm->pt_ptr.as<int64_t>(), tm->pt_col.as<int>(), tm->pt_val.as<double>(), G, dQt.as<double>(),
+                 Nr, dQct.as<double>(), N, false);
     DBIE_HIP(hipGetLastError());
     if (!tm->blas) {
         blas_check(rocblas_create_handle(&tm->blas), "rocblas_create_handle");

@@ -147,6 +147,11 @@ void triu_copy(hipStream_t st, int n, const double *src, int64_t lds, double *ds
 void csr_spmm(hipStream_t st, int64_t R, const int64_t *ptr, const int *col, const double *val, int K, const double *X,
               int64_t ldx, double *Y, int64_t ldy, bool accumulate);
 // Qt (g.N x G, column-major) = w_r / (4 pi) sum_{s in group a} q_s / |t_s - y_r|; groups by grp_ptr (G + 1).
+// Qc (N x G, ld N) = P^T Qr (Qr: Nr x G, ld Nr) for a prolongation whose refined nodes of coarse
+// panel c are first[c] .. first[c] + count[c] - 1 and interpolate from that panel's npp nodes only.
+// Returns false (nothing launched) for npp > 32.
+bool prolong_t(hipStream_t st, int P, int npp, const int64_t *first, const int *count, const int64_t *p_ptr,
+               const int *p_col, const double *p_val, int G, const double *Qr, int64_t Nr, double *Qc, int64_t N);
 void reaction_q(hipStream_t st, const DevGeom &g, int G, const int64_t *grp_ptr, const double *trg, const double *q,
                 double *Qt);
 // Y[idx[e]] += val[e] for n entries with distinct idx.

@@ -7,6 +7,10 @@
 //    and the transposed prolongation of the reaction weights.  One thread per (row, column):
 //    rows are short (1..n_quad^2 entries for the prolongation, a few panels' nodes for the
 //    corrections), so the kernel is bound by the gathers of X from HBM/L2.
+//  * prolong_t: Qc = P^T Qr for the reaction weights, per coarse panel one dense
+//    n_quad^2 x (refined nodes of the panel) block of P^T times those rows of Qr on FP64 MFMA
+//    (the refined panels of one coarse panel are consecutive, and their nodes interpolate from
+//    that panel's nodes only).
 //  * reaction_q: Qt[r, a] = w_r / (4 pi) * sum_{s in group a} q_s / |t_s - y_r|, the weight of
 //    refined node r in group a's potential integral.  With it the reaction matrix of
 //    four_index_matrix (src/solver/multi_rhs.jl:299-321, induced term)
@@ -25,6 +29,7 @@
 namespace dbie {
 namespace {
 constexpr double kFourPiT = 4.0 * 3.14159265358979311599796346854;
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void csr_spmm_kernel(int64_t R, const int64_t *__restrict__ ptr,
                                                        const int *__restrict__ col, const double *__restrict__ val,
@@ -38,6 +43,16 @@ __global__ __launch_bounds__(256) void csr_spmm_kernel(int64_t R, const int64_t 
     for (int64_t e = ptr[r]; e < ptr[r + 1]; ++e) acc += val[e] * xk[col[e]];
     double *y = Y + r + k * ldy;
     *y = accumulate ? *y + acc : acc;
+}
+
+// 1 / sqrt(d2) for d2 in the normal FP32 range: the FP32 hardware estimate (relative error
+// ~1e-7) refined by two Newton steps y <- y (3 - d2 y^2) / 2, each squaring the relative error,
+// so the result is within a few ulp of the correctly rounded value
+__device__ inline double rsqrt_f64(double d2) {
+    double y = (double)__builtin_amdgcn_rsqf((float)d2);
+    y = y * fma(-0.5 * d2, y * y, 1.5);
+    y = y * fma(-0.5 * d2, y * y, 1.5);
+    return y;
 }
 
 __global__ __launch_bounds__(256) void reaction_q_kernel(DevGeom g, const int64_t *__restrict__ grp_ptr,
@@ -59,10 +74,95 @@ __global__ __launch_bounds__(256) void reaction_q_kernel(DevGeom g, const int64_
         for (int u = 0; u < n; ++u) {
             const double4 p = s_src[u];
             const double dx = p.x - x, dy = p.y - y, dz = p.z - z;
-            acc += p.w / sqrt(dx * dx + dy * dy + dz * dz);
+            acc = fma(p.w, rsqrt_f64(dx * dx + dy * dy + dz * dz), acc);
         }
     }
     if (j < g.N) Qt[j + (int64_t)a * g.N] = g.w[j] * acc / kFourPiT;
+}
+
+// XCD-aware order of coarse panels: workgroup w runs on XCD w % 8; each XCD takes a contiguous
+// range of panels.
+__device__ inline int xcd_panel(int w, int P) { return (w % 8) * ((P + 7) / 8) + w / 8; }
+
+// Qc[c npp + k, a] = sum_{r refined node of coarse panel c} P[r, c npp + k] Qr[r, a]: one workgroup
+// per coarse panel and 256 columns.  Chunks of 32 refined nodes: the P^T chunk (npp x 32, zero
+// beyond npp rows and the panel's nodes) goes to LDS, the matching rows of Qr are the B operands of
+// v_mfma_f64_16x16x4_f64 (RT row tiles of 16 coarse nodes, 4 column blocks of 16 per wave).
+template <int RT>
+__global__ __launch_bounds__(256) void prolong_t_kernel(int P, int npp, const int64_t *__restrict__ first,
+                                                        const int *__restrict__ count, const int64_t *__restrict__ p_ptr,
+                                                        const int *__restrict__ p_col, const double *__restrict__ p_val,
+                                                        int G, const double *__restrict__ Qr, int64_t Nr,
+                                                        double *__restrict__ Qc, int64_t N) {
+    constexpr int CH = 32;
+    constexpr int LP = CH + 1;
+    constexpr int LO = 16 * RT + 1;
+    __shared__ double s_p[16 * RT][LP];  // [coarse node k][refined node within the chunk]
+    __shared__ double s_out[256][LO];    // [column within the 256-column group][coarse node]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = xcd_panel(blockIdx.x, P);
+    const int a0 = blockIdx.y * 256;
+    if (c >= P) return;  // uniform per workgroup
+    v4d acc[RT][4];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[rt][q] = v4d{0.0, 0.0, 0.0, 0.0};
+    int64_t coff[4];
+    double cmask[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int a = a0 + (wave + 4 * q) * 16 + (lane & 15);
+        cmask[q] = a < G ? 1.0 : 0.0;
+        coff[q] = (int64_t)(a < G ? a : G - 1) * Nr;
+    }
+    const int64_t r0 = first[c], col0 = (int64_t)c * npp;
+    const int n = count[c];
+    for (int base = 0; base < n; base += CH) {
+        __syncthreads();  // the previous chunk has been read
+        for (int idx = tid; idx < 16 * RT * LP; idx += 256) (&s_p[0][0])[idx] = 0.0;
+        __syncthreads();
+        const int nc = min(CH, n - base);
+        for (int idx = tid; idx < nc * npp; idx += 256) {
+            const int u = idx / npp, j = idx - u * npp;
+            const int64_t r = r0 + base + u;
+            const int64_t e = p_ptr[r] + j;
+            if (e < p_ptr[r + 1]) {
+                const int64_t k = p_col[e] - col0;
+                if (k >= 0 && k < npp) s_p[k][u] = p_val[e];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CH / 4; ++kk) {
+            const int u = kk * 4 + (lane >> 4);
+            double xv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[q] = u < nc ? Qr[r0 + base + u + coff[q]] : 0.0;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                const double a = s_p[rt * 16 + (lane & 15)][u];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc[rt][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xv[q], acc[rt][q], 0, 0, 0);
+            }
+        }
+    }
+    // D(row = 4r + lane/16, col = lane%16) of row tile rt -> LDS [col][16 rt + row]
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                s_out[(wave + 4 * q) * 16 + (lane & 15)][16 * rt + 4 * r + (lane >> 4)] = acc[rt][q][r] * cmask[q];
+    __syncthreads();
+    for (int idx = tid; idx < 256 * npp; idx += 256) {
+        const int cl = idx / npp, k = idx - cl * npp;
+        const int a = a0 + cl;
+        if (a >= G) break;  // idx only grows
+        Qc[col0 + k + (int64_t)a * N] = s_out[cl][k];
+    }
 }
 
 __global__ void scatter_add_kernel(int64_t n, const int64_t *__restrict__ idx, const double *__restrict__ val,
@@ -287,6 +387,19 @@ void reaction_q(hipStream_t st, const DevGeom &g, int G, const int64_t *grp_ptr,
                 double *Qt) {
     if (g.N <= 0 || G <= 0) return;
     reaction_q_kernel<<<dim3((unsigned)((g.N + 255) / 256), (unsigned)G), 256, 0, st>>>(g, grp_ptr, trg, q, Qt);
+}
+
+bool prolong_t(hipStream_t st, int P, int npp, const int64_t *first, const int *count, const int64_t *p_ptr,
+               const int *p_col, const double *p_val, int G, const double *Qr, int64_t Nr, double *Qc, int64_t N) {
+    if (npp > 32) return false;
+    if (P <= 0 || G <= 0) return true;
+    const int per = (P + 7) / 8;
+    dim3 grid((unsigned)(8 * per), (unsigned)((G + 255) / 256));
+    if (npp <= 16)
+        prolong_t_kernel<1><<<grid, 256, 0, st>>>(P, npp, first, count, p_ptr, p_col, p_val, G, Qr, Nr, Qc, N);
+    else
+        prolong_t_kernel<2><<<grid, 256, 0, st>>>(P, npp, first, count, p_ptr, p_col, p_val, G, Qr, Nr, Qc, N);
+    return true;
 }
 
 void scatter_add(hipStream_t st, int64_t n, const int64_t *idx, const double *val, double *Y) {
