@@ -729,13 +729,15 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
                                                                   const unsigned *__restrict__ cmax, float gscale,
                                                                   double *__restrict__ part, int src_per_split,
                                                                   NearMask nm, const int *__restrict__ tplane) {
-    static_assert(EXCL == 0 || EXCL == 1, "per-source exclusion runs in far_mfma_f16x3_kernel");
+    static_assert(EXCL >= 0 && EXCL <= 2, "exclusion mode");
     static_assert(CW == 4 || CW == 8, "4 or 8 consumer waves");
     constexpr int RS = 32 / CW;  // target slices per consumer wave
     __shared__ uint4 s_a[2][8][2][64];   // [stage parity][target slice][hi/lo][lane]
     __shared__ uint4 s_b[3][16][2][64];  // [stage % 3][column block][hi/lo][lane]
     __shared__ double s_pos[3][3][kWsSb];  // [stage % 3][x, y, z][source]
-    __shared__ uint32_t s_mask[EXCL ? 8 : 1][EXCL ? kWsMaskWords : 1];  // near-mask rows of the 8 target panels
+    // near-mask rows of the tile's target panels: the 8 panels of the 8 target slices (EXCL = 1,
+    // npp = 16), or the at most 8 panels the 128 targets touch (EXCL = 2, npp >= 19)
+    __shared__ uint32_t s_mask[EXCL ? 8 : 1][EXCL ? kWsMaskWords : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool producer = wave < 4;
     const int64_t N = g.N;
@@ -758,10 +760,12 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
         // two targets per lane, FP32 coordinates relative to the tile's first target, normals
         // scaled by gscale
         float tx[2], ty[2], tz[2], tn0[2], tn1[2], tn2[2];
+        int tpr[2] = {0, 0};  // EXCL = 2: mask row (target panel - first panel of the tile)
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             int64_t tg = i0 + (2 * pw + u) * 16 + (lane & 15);
             tg = tg < N ? tg : N - 1;
+            if (EXCL == 2) tpr[u] = (int)(tg / nm.npp - i0 / nm.npp);
             tx[u] = (float)(g.x[tg] - ox);
             ty[u] = (float)(g.y[tg] - oy);
             tz[u] = (float)(g.z[tg] - oz);
@@ -774,11 +778,18 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
         const int64_t wbase = EXCL ? (jb0 / nm.npp) >> 5 : 0;
         if (EXCL && walk.jend > jb0) {
             const int nw = (int)((((walk.jend - 1) / nm.npp) >> 5) - wbase + 1);
+            const int64_t tp_last = (N - 1) / nm.npp;
             for (int idx = tid; idx < 8 * nw; idx += 256) {
                 const int sl = idx / nw, w = idx - sl * nw;
-                int64_t tg = i0 + sl * 16;
-                tg = tg < N ? tg : N - 1;
-                s_mask[sl][w] = nm.bits[(tg / nm.npp) * (int64_t)nm.mwords + wbase + w];
+                int64_t tp;
+                if (EXCL == 1) {
+                    const int64_t tg = i0 + sl * 16;
+                    tp = (tg < N ? tg : N - 1) / nm.npp;
+                } else {
+                    tp = i0 / nm.npp + sl;
+                    tp = tp < tp_last ? tp : tp_last;
+                }
+                s_mask[sl][w] = nm.bits[tp * (int64_t)nm.mwords + wbase + w];
             }
         }
         // Every VM operation of the producers is a global -> LDS copy of the stage after next (B
@@ -831,6 +842,25 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
                     n2[u] = ex ? 0.f : n2[u];
                 }
             }
+            // EXCL = 2 (npp >= 19): the lane's 8 consecutive sources js + 8 (lane >> 4) + e lie in
+            // panel pa for e < bnd and in panel pa + 1 otherwise; exA / exB are those panels' bits
+            int bnd = 8;
+            bool exA[2] = {false, false}, exB[2] = {false, false};
+            if (EXCL == 2) {
+                const int64_t s0 = js + 8 * (lane >> 4);
+                const int64_t pa = s0 / nm.npp;
+                bnd = (int)((pa + 1) * nm.npp - s0);
+                auto bit = [&](int row, int64_t sp) -> bool {
+                    int w = (int)((sp >> 5) - wbase);  // padding sources past the split may point
+                    w = w < 0 ? 0 : (w > kWsMaskWords - 1 ? kWsMaskWords - 1 : w);  // one word beyond
+                    return (s_mask[row][w] >> (sp & 31)) & 1u;
+                };
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    exA[u] = bit(tpr[u], pa);
+                    exB[u] = bit(tpr[u], pa + 1);
+                }
+            }
             unsigned h[2][4], l[2][4];
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
@@ -844,6 +874,7 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
                         const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-20f)));
                         const float rinv = __builtin_amdgcn_rsqf(r2);
                         v[u][d] = (n0[u] * dx + n1[u] * dy + n2[u] * dz) * (rinv * rinv * rinv);
+                        if (EXCL == 2) v[u][d] = (e + d < bnd ? exA[u] : exB[u]) ? 0.f : v[u][d];
                     }
                 }
 #pragma unroll
@@ -1086,16 +1117,20 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         const int ex = (!exclude_near || !g.near_mask) ? 0 : (g.npp == 16 ? 1 : 2);
         mark(ev0);
         const int64_t mask_words_split = (per / std::max(g.npp, 1)) / 32 + 2;
-        if (tune.variant != 0 && ncb == 16 && ex != 2 && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
+        // the warp-specialised kernel: exclusion per lane (npp = 16) or per source for panels of at
+        // least 19 nodes (the 128 targets of a tile then touch at most 8 target panels)
+        if (tune.variant != 0 && ncb == 16 && (ex != 2 || g.npp >= 19) && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
             (ex == 0 || mask_words_split <= kWsMaskWords)) {
 #define DBIE_WS(E, C)                                                                                         \
     far_ws_f16x3_kernel<E, C><<<grid, ws_threads<C>(), 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm, \
                                                                g.tile_plane)
             if (tune.variant == 1) {
-                if (ex == 1) DBIE_WS(1, 4);
+                if (ex == 2) DBIE_WS(2, 4);
+                else if (ex == 1) DBIE_WS(1, 4);
                 else DBIE_WS(0, 4);
             } else {
-                if (ex == 1) DBIE_WS(1, 8);
+                if (ex == 2) DBIE_WS(2, 8);
+                else if (ex == 1) DBIE_WS(1, 8);
                 else DBIE_WS(0, 8);
             }
 #undef DBIE_WS

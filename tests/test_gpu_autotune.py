@@ -92,3 +92,28 @@ def test_tuner_picks_and_caches_a_shape(dbie, cfg, prec, K):
         assert op.tuned_shape(K, prec) == shape
     finally:
         dbie.autotune(True)
+
+
+@pytest.mark.parametrize("K", [256, 300])
+def test_warp_specialised_kernel_per_source_exclusion(dbie, O, K):
+    """n_quad = 5 (25-node panels): the warp-specialised split-FP16 kernel leaves the near pairs
+    out source by source (a lane's 8 sources may straddle two panels); both consumer layouts and
+    the general kernel give the same operator to the split-FP16 bar."""
+    say(f"n_quad=5 split-FP16 far field, K={K}: variants -1, 0, 1")
+    g = dbie.single_dielectric_box3d(1.2, 0.8, 0.6, 5, 0.2, 4.0, 1.0)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
+    ref = O.CorrectedOperator(O.Interface.box(1.2, 0.8, 0.6, 5, 0.2, 4.0, 1.0))
+    X = np.random.default_rng(23).standard_normal((g.n_points, K))
+    Yref = ref @ X
+    out = {}
+    try:
+        for splits in (1, 3):
+            for var in (-1, 0, 1):
+                op.set_tuned_shape(K, "fp16x3", splits, var)
+                Y = op.apply(X, "fp16x3")
+                err = col_err(Y, Yref)
+                assert err < BOUND["fp16x3"], (splits, var, err)
+                out[(splits, var)] = Y
+    finally:
+        dbie.autotune(True, clear=True)
+    say(f"  max column error vs oracle {max(col_err(Y, Yref) for Y in out.values()):.2e}")
