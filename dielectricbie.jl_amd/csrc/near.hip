@@ -13,8 +13,8 @@
 //    matching 16-row slice of X.  Block rows are remapped so that consecutive rows (which
 //    share most source panels) run on the same XCD and reuse its L2.
 //  * K >= 16 and bd = 4, 9 or 25 (n_quad = 2, 3, 5; row-major blocks): FP64 MFMA on the block
-//    padded to RT x 16 rows and KS x 4 source columns (zeros staged once in LDS), one workgroup per
-//    block row and 256 columns, the block staged in LDS as FP64 and shared by the four waves.
+//    padded to RT x 16 rows and KS x 4 source columns, one workgroup per block row and 256
+//    columns, operands loaded straight from the block and X (no LDS, no barrier).
 //  * otherwise: one thread per (row, column), VALU.
 #include <hip/hip_runtime.h>
 
@@ -150,11 +150,11 @@ __global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__rest
     }
 }
 
-// Row-major bd x bd blocks (bd <= 16 RT, bd <= 4 KS) on v_mfma_f64_16x16x4_f64: the block is staged
-// in LDS as FP64 inside a zero frame of RT x 16 rows and 4 KS columns, so the padding rows and
-// columns contract to zero; each wave multiplies it with 4 column blocks of 16 columns of the
-// source panel's rows of X.  The output goes through LDS so that each column's bd rows are written
-// (and the partial sums, X and the diagonal read) contiguously.
+// Row-major bd x bd blocks (bd <= 16 RT, bd <= 4 KS) on v_mfma_f64_16x16x4_f64, padded to RT x 16
+// rows and 4 KS source columns: each lane loads its A operands straight from the block (L1-resident:
+// the four waves read the same block) with zeros outside it, and its B operands from the source
+// panel's rows of X, so the loop has no barrier and no LDS; each wave multiplies 4 column blocks of
+// 16 columns.  The epilogue adds the partial sums and the diagonal term from the registers.
 template <typename VT, int RT, int KS>
 __global__ __launch_bounds__(256) void near_mfma_rm_kernel(int P, int bd, const int *__restrict__ row_ptr,
                                                            const int *__restrict__ col, const VT *__restrict__ val,
@@ -162,86 +162,68 @@ __global__ __launch_bounds__(256) void near_mfma_rm_kernel(int P, int bd, const 
                                                            const double *__restrict__ X, int64_t ldx,
                                                            const double *__restrict__ diag, double *__restrict__ Y,
                                                            int64_t ldy) {
-    constexpr int LA = 4 * KS + 1;  // padded row length of the staged block (odd: no bank conflicts)
-    constexpr int LO = 16 * RT + 1;
-    __shared__ double s_a[16 * RT][LA];
-    __shared__ double s_out[256][LO];  // [column within the 256-column group][row]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int p = xcd_row(blockIdx.x, P);
     const int k0 = blockIdx.y * 256;
     if (p >= P) return;  // uniform per workgroup
-    for (int idx = tid; idx < 16 * RT * LA; idx += 256) (&s_a[0][0])[idx] = 0.0;
     v4d acc[RT][4];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[rt][q] = v4d{0.0, 0.0, 0.0, 0.0};
     int64_t coff[4];
-    double cmask[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int c = k0 + (wave + 4 * q) * 16 + (lane & 15);
-        cmask[q] = c < K ? 1.0 : 0.0;
         coff[q] = (int64_t)(c < K ? c : K - 1) * ldx;
     }
+    // this lane's A element (t, c) of row tile rt and k-slice kk: t = 16 rt + lane % 16,
+    // c = 4 kk + lane / 16; offsets into the block, or -1 outside it
+    int aoff[RT][KS];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const int t = 16 * rt + (lane & 15), c = 4 * kk + (lane >> 4);
+            aoff[rt][kk] = t < bd && c < bd ? t * bd + c : -1;
+        }
     const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
     const int nv = bd * bd;
-    // X operands of one block (KS k-slices x 4 column blocks), prefetched one block ahead; source
-    // columns c >= bd (padding) load nothing and contract with the zero frame
-    double xn[KS][4];
-    auto load_x = [&](int b, double (&xv)[KS][4]) {
+    for (int b = b0; b < b1; ++b) {
+        const VT *v = val + (int64_t)b * nv;
         const int64_t src = (int64_t)col[b] * bd;
 #pragma unroll
         for (int kk = 0; kk < KS; ++kk) {
             const int c = kk * 4 + (lane >> 4);
+            double xv[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) xv[kk][q] = c < bd ? X[src + c + coff[q]] : 0.0;
-        }
-    };
-    if (b0 < b1) load_x(b0, xn);
-    for (int b = b0; b < b1; ++b) {
-        __syncthreads();  // the previous block's A fragments have been read
-        const VT *v = val + (int64_t)b * nv;
-        for (int idx = tid; idx < nv; idx += 256) {
-            const int t = idx / bd;
-            s_a[t][idx - t * bd] = (double)v[idx];
-        }
-        double xv[KS][4];
-#pragma unroll
-        for (int kk = 0; kk < KS; ++kk)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) xv[kk][q] = xn[kk][q];
-        if (b + 1 < b1) load_x(b + 1, xn);
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KS; ++kk)
+            for (int q = 0; q < 4; ++q) xv[q] = c < bd ? X[src + c + coff[q]] : 0.0;
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
-                const double a = s_a[rt * 16 + (lane & 15)][kk * 4 + (lane >> 4)];
+                const double a = aoff[rt][kk] >= 0 ? (double)v[aoff[rt][kk]] : 0.0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    acc[rt][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xv[kk][q], acc[rt][q], 0, 0, 0);
+                    acc[rt][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xv[q], acc[rt][q], 0, 0, 0);
             }
+        }
     }
-    // D(row = 4r + lane/16, col = lane%16) of row tile rt -> LDS [col][16 rt + row]
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                s_out[(wave + 4 * q) * 16 + (lane & 15)][16 * rt + 4 * r + (lane >> 4)] = acc[rt][q][r] * cmask[q];
-    __syncthreads();
-    // consecutive threads take consecutive rows of one column
+    // D(row = 4r + lane/16, col = lane%16) of row tile rt: row 16 rt + 4 r + lane / 16 of the block row
     const int64_t ib = (int64_t)p * bd;
-    for (int idx = tid; idx < 256 * bd; idx += 256) {
-        const int cl = idx / bd, row = idx - cl * bd;
-        const int c = k0 + cl;
-        if (c >= K) break;  // idx only grows
-        const int64_t i = ib + row;
-        double y = s_out[cl][row] + diag[i] * X[i + (int64_t)c * ldx];
-        for (int s2 = 0; s2 < S; ++s2) y += part[((int64_t)s2 * K + c) * N + i];
-        Y[i + (int64_t)c * ldy] = y;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = k0 + (wave + 4 * q) * 16 + (lane & 15);
+        if (c >= K) continue;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rt + 4 * r + (lane >> 4);
+                if (row >= bd) continue;
+                const int64_t i = ib + row;
+                double y = acc[rt][q][r] + diag[i] * X[i + (int64_t)c * ldx];
+                for (int s2 = 0; s2 < S; ++s2) y += part[((int64_t)s2 * K + c) * N + i];
+                Y[i + (int64_t)c * ldy] = y;
+            }
     }
 }
 
