@@ -128,6 +128,47 @@ def reaction_workload(a, dbie, log):
         "p99_ms": float(np.percentile(lat, 99))}), flush=True)
 
 
+def single_workload(a, dbie, log):
+    """Box 2.4 x 1.6 x 1.2, l_ec = 0.05, n_quad = 4, eps_in = 10, eps_out = 1 (N = 64,384); one unit point
+    charge at a seeded uniform-random interior point; plain FP64 GMRES to a relative residual of 1e-10
+    (matrix-free far field, near corrections).  A step is one solve with the operator built; the
+    single-column far field runs on the FP64 VALU (one thread per target, sources staged in LDS)."""
+    t0 = time.perf_counter()
+    iface = dbie.single_dielectric_box3d(*BOX, NQ, L_EC, 10.0, EPS_OUT)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(iface, 1e-9, 1e-9, 8)
+    t_op = time.perf_counter() - t0
+    rng = np.random.default_rng(a.seed)
+    src = rng.uniform(-0.5, 0.5, (1, 3)) * np.array(BOX)
+    F = dbie.rhs_point_charges(iface, src, np.ones(1), 10.0)
+    N = iface.n_points
+    dF = dbie.DeviceMatrix.from_host(F.reshape(N, 1))
+    dS = dbie.DeviceMatrix(N, 1)
+    for _ in range(a.warmup):
+        op.solve_device(dF, dS, rtol=1e-10, precision="fp64")
+    op.profile(True)
+    op.profile_read()
+    lat, st = [], None
+    for _ in range(max(a.steps, 1)):
+        t1 = time.perf_counter()
+        st = op.solve_device(dF, dS, rtol=1e-10, precision="fp64")
+        lat.append((time.perf_counter() - t1) * 1e3)
+    prof = op.profile_read()
+    op.profile(False)
+    pairs = cross_plane_pairs(iface)
+    n_far, far_ms = prof.get("far_fp64", (0, 0.0))
+    avg = far_ms / max(n_far, 1)
+    log(f"single FP64 solve: N={N} iters={st['niter']} residual {st['max_rel_residual']:.2e} "
+        f"{np.median(lat):.1f} ms; far field {avg:.3f} ms per launch ({n_far} launches), prof={prof}")
+    print(json.dumps({
+        "workload": "single", "metric": "FP64 solve time, one point charge, relative residual 1e-10",
+        "value": float(np.median(lat)), "unit": "ms", "higher_is_better": False, "dtype": "f64",
+        "config": {"N": N, "eps_in": 10.0, "source": src.ravel().tolist()}, "iters": st["niter"],
+        "max_rel_residual": st["max_rel_residual"], "setup_s": t_op,
+        "far_field": {"avg_launch_ms": avg, "pair_evaluations": pairs,
+                      "pairs_per_s": pairs / (avg * 1e-3) if avg > 0 else None},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}), flush=True)
+
+
 def serve_workload(a, dbie, log):
     """Seeded Poisson stream of single-box requests (sides ~U(0.5, 2), l_ec in {0.1, 0.07, 0.05},
     n_quad 4, eps_in ~U(2, 80), 1-64 charges) through the request server in mixed precision;
@@ -189,9 +230,10 @@ def main():
                     help="apply: time only operator applies (one per precision) for kernel profiling")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--blobs", type=int, default=1024, help="reaction workload: first B lattice blobs")
-    ap.add_argument("--workload", default="box", choices=["box", "reaction", "serve"],
+    ap.add_argument("--workload", default="box", choices=["box", "reaction", "serve", "single"],
                     help="reaction: the volume-source reaction-matrix workload (1024 Gauss-Hermite blobs); "
-                         "serve: a seeded Poisson stream of box requests through the request server")
+                         "serve: a seeded Poisson stream of box requests through the request server; "
+                         "single: one point charge, eps_in 10, FP64 GMRES to 1e-10 (solve time)")
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=2, help="serve: concurrent solve slots")
@@ -211,6 +253,9 @@ def main():
         return
     if a.workload == "serve":
         serve_workload(a, dbie, log)
+        return
+    if a.workload == "single":
+        single_workload(a, dbie, log)
         return
 
     t0 = time.perf_counter()
