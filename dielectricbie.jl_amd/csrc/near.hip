@@ -39,34 +39,48 @@ int64_t bsr_lane_major_index(int t, int c) { return lane_major_idx(t, c); }
 namespace {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// One output (row, column) per CS threads: thread cs of the group takes the block columns
+// c = cs, cs + CS, ... of every block of the row, and the group's partial sums meet in LDS.  Thread
+// tid = row + bd (cs + CS (kc + KTe pp)).
 template <typename VT>
-__global__ __launch_bounds__(256) void near_combine_kernel(int P, int bd, int mfma_order, const int *__restrict__ row_ptr,
+__global__ __launch_bounds__(256) void near_combine_kernel(int P, int bd, int mfma_order, int CS,
+                                                           const int *__restrict__ row_ptr,
                                                            const int *__restrict__ col, const VT *__restrict__ val,
                                                            int S, const double *__restrict__ part, int64_t N, int K,
                                                            int KTe, int RP, const double *__restrict__ X, int64_t ldx,
                                                            const double *__restrict__ diag, double *__restrict__ Y,
                                                            int64_t ldy) {
+    __shared__ double s_red[256];
     const int tid = threadIdx.x;
     const int row = tid % bd;
-    const int rest = tid / bd;
+    int rest = tid / bd;
+    const int cs = rest % CS;
+    rest /= CS;
     const int kc = rest % KTe;
     const int pp = rest / KTe;
-    if (pp >= RP) return;
     const int p = blockIdx.x * RP + pp;
     const int k = blockIdx.y * KTe + kc;
-    if (p >= P || k >= K) return;
-    const int64_t i = (int64_t)p * bd + row;
-    const double *xk = X + (int64_t)k * ldx;
-    double acc = diag[i] * xk[i];
-    for (int s = 0; s < S; ++s) acc += part[((int64_t)s * K + k) * N + i];
+    const bool valid = pp < RP && p < P && k < K;
+    const int64_t i = (int64_t)(valid ? p : 0) * bd + row;
+    const double *xk = X + (int64_t)(valid ? k : 0) * ldx;
     double corr = 0.0;
-    const int b1 = row_ptr[p + 1];
-    for (int b = row_ptr[p]; b < b1; ++b) {
-        const VT *v = val + (int64_t)b * bd * bd;
-        const double *xs = xk + (int64_t)col[b] * bd;
-        for (int c = 0; c < bd; ++c)
-            corr += (double)v[mfma_order == 2 ? lane_major_idx(row, c) : blk_idx(bd, mfma_order != 0, row, c)] * xs[c];
+    if (valid) {
+        const int b1 = row_ptr[p + 1];
+        for (int b = row_ptr[p]; b < b1; ++b) {
+            const VT *v = val + (int64_t)b * bd * bd;
+            const double *xs = xk + (int64_t)col[b] * bd;
+            for (int c = cs; c < bd; c += CS)
+                corr += (double)v[mfma_order == 2 ? lane_major_idx(row, c) : blk_idx(bd, mfma_order != 0, row, c)] * xs[c];
+        }
     }
+    if (CS > 1) {
+        s_red[tid] = corr;
+        __syncthreads();
+    }
+    if (!valid || cs != 0) return;
+    for (int j = 1; j < CS; ++j) corr += s_red[tid + j * bd];
+    double acc = diag[i] * xk[i];
+    for (int s2 = 0; s2 < S; ++s2) acc += part[((int64_t)s2 * K + k) * N + i];
     Y[i + (int64_t)k * ldy] = acc + corr;
 }
 
@@ -338,17 +352,23 @@ void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const d
 #undef DBIE_NEAR_RM
         return;
     }
-    const int KT = std::max(1, 256 / bd);
+    // CS threads per (row, column) output (up to 4, within a 64-thread workgroup for blocks of at
+    // most 64 rows), 64 / (bd CS) block rows times columns per workgroup, so that a few columns
+    // still spread over every CU with short per-thread loops
+    const int nt = bd <= 64 ? 64 : 256;
+    int CS = 1;
+    while (CS < 4 && bd * CS * 2 <= nt) CS *= 2;
+    const int KT = std::max(1, nt / (bd * CS));
     const int KTe = std::min(K, KT);
     const int RP = std::max(1, KT / KTe);
     dim3 grid((unsigned)((C.P + RP - 1) / RP), (unsigned)((K + KTe - 1) / KTe));
     const int mo = (vals == NEAR_UP32 && C.mfma_order) ? 2 : (C.mfma_order ? 1 : 0);
     if (v32)
-        near_combine_kernel<float><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, v32, S, part, N, K, KTe, RP,
-                                                          X, ldx, diag, Y, ldy);
+        near_combine_kernel<float><<<grid, nt, 0, st>>>(C.P, bd, mo, CS, C.row_ptr, C.col, v32, S, part, N, K, KTe,
+                                                         RP, X, ldx, diag, Y, ldy);
     else
-        near_combine_kernel<double><<<grid, 256, 0, st>>>(C.P, bd, mo, C.row_ptr, C.col, C.val64, S, part, N, K, KTe,
-                                                           RP, X, ldx, diag, Y, ldy);
+        near_combine_kernel<double><<<grid, nt, 0, st>>>(C.P, bd, mo, CS, C.row_ptr, C.col, C.val64, S, part, N, K,
+                                                          KTe, RP, X, ldx, diag, Y, ldy);
 }
 
 }  // namespace dbie
