@@ -276,10 +276,14 @@ struct TuneKey {
                std::tie(o.prec, o.excl, o.npp, o.kgroups, o.nbucket);
     }
 };
+// Every candidate is timed kTuneRounds times (round robin) and keeps its fastest time: a single
+// timing of a launch can be off by several per cent (clock ramps, the first touch of a buffer).
+constexpr size_t kTuneRounds = 2;
 struct TuneState {
     std::vector<FarTune> cand;
-    std::vector<float> ms;  // per candidate; < 0 until timed
-    size_t next = 0;        // next candidate to try
+    std::vector<float> ms;  // per candidate: fastest time so far; < 0 until timed
+    std::vector<int> nt;    // per candidate: timings received
+    size_t next = 0;        // next trial (candidate next % cand.size())
     bool done = false;
     FarTune best;
 };
@@ -335,8 +339,9 @@ std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
 void settle(TuneState &ts) {
     size_t bi = 0;
     float bm = -1.f;
-    bool all = ts.next >= ts.cand.size();
+    bool all = ts.next >= kTuneRounds * ts.cand.size();
     for (size_t i = 0; i < ts.cand.size(); ++i) {
+        if (i < ts.nt.size() && ts.nt[i] < (int)kTuneRounds) all = false;
         if (ts.ms[i] < 0) {
             all = false;
             continue;
@@ -363,8 +368,10 @@ void poll_timings_locked() {
         if (q != hipSuccess || hipEventElapsedTime(&ms, p.e0, p.e1) != hipSuccess) ms = 1e30f;
         auto it = g_tune.find(p.key);
         if (it != g_tune.end() && p.idx < it->second.ms.size() && !it->second.done) {
-            it->second.ms[p.idx] = ms;
-            settle(it->second);
+            TuneState &ts = it->second;
+            ts.ms[p.idx] = ts.ms[p.idx] < 0 ? ms : std::min(ts.ms[p.idx], ms);
+            if (p.idx < ts.nt.size()) ++ts.nt[p.idx];
+            settle(ts);
         }
         (void)hipEventDestroy(p.e0);
         (void)hipEventDestroy(p.e1);
@@ -473,13 +480,17 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
             TuneState ts;
             ts.cand = tune_candidates(op->N(), K, prec);
             ts.ms.assign(ts.cand.size(), -1.f);
-            if (ts.cand.size() <= 1) ts.next = ts.cand.size();
+            ts.nt.assign(ts.cand.size(), 0);
+            if (ts.cand.size() <= 1) {
+                ts.next = kTuneRounds * ts.cand.size();
+                ts.nt.assign(ts.cand.size(), (int)kTuneRounds);
+            }
             settle(ts);
             it = g_tune.emplace(key, ts).first;
         }
         TuneState &ts = it->second;
-        if (!ts.done && ts.next < ts.cand.size()) {
-            idx = ts.next++;
+        if (!ts.done && ts.next < kTuneRounds * ts.cand.size()) {
+            idx = ts.next++ % ts.cand.size();
             use = ts.cand[idx];
             timed = true;
             for (const FarTune &c : ts.cand) max_splits = std::max(max_splits, c.splits);
@@ -1902,7 +1913,8 @@ dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision
         TuneState ts;
         ts.cand = {FarTune{splits, variant}};
         ts.ms = {0.f};
-        ts.next = 1;
+        ts.nt = {(int)kTuneRounds};
+        ts.next = kTuneRounds;
         ts.done = true;
         ts.best = ts.cand[0];
         std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -31,7 +31,7 @@ def test_graph_replays_equal_eager(dbie, op, prec, K):
     rng = np.random.default_rng(40 + K)
     # tune the class first so that the launch shape is settled (the tuner's candidates run eagerly)
     X0 = rng.standard_normal((n, K))
-    for _ in range(40):
+    for _ in range(80):
         op.apply(X0, prec)
         if op.tuned_shape(K, prec) is not None:
             break
