@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--mode", default="solve", choices=["solve", "apply"],
                     help="apply: time only operator applies (one per precision) for kernel profiling")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--shape", default="", help="apply mode: pin the low-precision far-field launch shape "
+                                                "'splits,variant' instead of tuning it")
     ap.add_argument("--blobs", type=int, default=1024, help="reaction workload: first B lattice blobs")
     ap.add_argument("--workload", default="box", choices=["box", "reaction", "serve", "single"],
                     help="reaction: the volume-source reaction-matrix workload (1024 Gauss-Hermite blobs); "
@@ -270,8 +272,12 @@ def main():
         X = np.random.default_rng(2).standard_normal((N, a.K))
         res = {}
         for prec in ("fp64", a.precision):
-            # let the launch-shape tuner settle (it tries one candidate per apply), then time
-            for _ in range(64):
+            # let the launch-shape tuner settle (it tries each candidate twice), then time; or pin
+            # the low-precision shape given by --shape
+            if a.shape and prec == a.precision:
+                sp, var = (int(v) for v in a.shape.split(","))
+                op.set_tuned_shape(a.K, prec, sp, var)
+            for _ in range(96):
                 op.apply(X, prec)
                 if op.tuned_shape(a.K, prec) is not None:
                     break
@@ -289,6 +295,7 @@ def main():
         dst = dbie.DeviceMatrix(2 ** 24, 8)
         dst.copy_from(src)
         print(json.dumps({"mode": "apply", "N": N, "K": a.K, "per_launch_ms": res,
+                          "shapes": {p: op.tuned_shape(a.K, p) for p in ("fp64", a.precision)},
                           "cross_plane_pairs": cross_plane_pairs(iface), "calibration_copy_bytes": 2 ** 30}),
               flush=True)
         return
@@ -371,9 +378,15 @@ def main():
                 "traffic": None, "kernel": ("far_mfma_f64" if name == "far_fp64" else f"far_mfma_{a.precision}"),
                 "avg_launch_ms": avg_ms, "launches": launches, "algorithmic_flops_per_launch": flops}
     # HBM bytes per launch of the dominant kernel, from the committed FETCH_SIZE/WRITE_SIZE passes
+    # (split-FP16 far field: one entry per launch shape, s<splits>v<variant>; the entry of the shape
+    # the tuner chose in this run, if that shape was profiled)
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         tr = json.load(open(traffic_file)).get(roof.get("kernel"))
+        if tr and "bytes" not in tr:
+            shp = op.tuned_shape(a.K, "fp64" if roof.get("kernel") == "far_mfma_f64" else a.precision)
+            roof["traffic_shape"] = f"s{shp[0]}v{shp[1]}" if shp else None
+            tr = tr.get(roof["traffic_shape"]) if shp else None
         if tr and a.K == 256:
             roof["traffic"] = tr["bytes"]
             roof["traffic_fetch_write"] = [tr["fetch_bytes"], tr["write_bytes"]]
