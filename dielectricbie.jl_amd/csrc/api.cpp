@@ -82,7 +82,8 @@ struct KrylovRecycle {
 
 struct GmresWork {
     DBuf V, W, Z, U, partial, hbuf, nbuf;
-    DBuf V1, abuf;       // recycled Krylov basis of an earlier inner solve, its projection coefficients
+    DBuf V1, abuf, cbuf;  // recycled Krylov basis of an earlier inner solve, its projection coefficients
+    DBuf om, n1;          // the recycled space's unit normals omega and block counts on the device
     KrylovRecycle rec;
     DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
     DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
@@ -740,7 +741,22 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     partial.ensure(sizeof(double) * std::max(dots_partial_elems(N, K, std::max(m + 1, nb1)), dots_partial_elems(N, K, 1)));
     hbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
     nbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
-    if (defl) ws.abuf.ensure(sizeof(double) * (size_t)nb1 * K);
+    std::vector<double> a_h;  // V1^T W of the last deflation, copied back with the iteration's sync
+    if (defl) {
+        ws.abuf.ensure(sizeof(double) * (size_t)nb1 * K);
+        ws.cbuf.ensure(sizeof(double) * (size_t)nb1 * K);
+        std::vector<double> om_h((size_t)nb1 * K, 0.0);
+        std::vector<int> n1_h(K, 0);
+        for (int k = 0; k < K; ++k) {
+            const int k1 = use->k1[k];
+            if (k1 <= 0) continue;
+            n1_h[k] = k1 + 1;
+            for (int i = 0; i <= k1; ++i) om_h[(size_t)i * K + k] = use->omega[(size_t)k * (um + 1) + i];
+        }
+        upload(ws.om, om_h.data(), om_h.size(), op->stream);
+        upload(ws.n1, n1_h.data(), n1_h.size(), op->stream);
+        a_h.resize((size_t)nb1 * K);
+    }
     hipStream_t st = op->stream;
     double *Wp = W.as<double>();
     const double *V1p = defl ? ws.V1.as<double>() : nullptr;
@@ -761,24 +777,23 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     };
     double *Vp = basis(2);
 
-    // Wd <- (I - C C^T) Wd; t (K x (um+1)) receives Omega V1^T Wd per column, whose first k1
-    // entries are C^T Wd.  C C^T = V1 (I - omega omega^T) V1^T on the span of V1_(k1+1).
+    // Wd <- (I - C C^T) Wd with C C^T = V1 (I - omega omega^T) V1^T on the span of V1_(k1+1), on the
+    // device; V1^T Wd is copied back for the projections, which project() forms after the next sync:
+    // tproj (K x (um+1)) = Omega V1^T Wd per column, whose first k1 entries are C^T Wd
     std::vector<double> tproj;
     auto deflate = [&](double *Wd) {
         block_dots(st, V1p, nb1, Wd, N, K, partial.as<double>(), ws.abuf.as<double>());
-        std::vector<double> a((size_t)nb1 * K), coef((size_t)nb1 * K, 0.0);
-        DBIE_HIP(hipMemcpyAsync(a.data(), ws.abuf.p, sizeof(double) * a.size(), hipMemcpyDeviceToHost, st));
-        DBIE_HIP(hipStreamSynchronize(st));
+        deflate_coef(st, ws.abuf.as<double>(), ws.om.as<double>(), ws.n1.as<int>(), nb1, K, ws.cbuf.as<double>());
+        block_subtract(st, V1p, nb1, ws.cbuf.as<double>(), Wd, N, K);
+        DBIE_HIP(hipMemcpyAsync(a_h.data(), ws.abuf.p, sizeof(double) * a_h.size(), hipMemcpyDeviceToHost, st));
+    };
+    auto project = [&]() {
         tproj.assign((size_t)(um + 1) * K, 0.0);
         for (int k = 0; k < K; ++k) {
             const int k1 = use->k1[k];
             if (k1 <= 0) continue;
-            const double *om = &use->omega[(size_t)k * (um + 1)];
-            double wa = 0.0;
-            for (int i = 0; i <= k1; ++i) wa += om[i] * a[(size_t)i * K + k];
-            for (int i = 0; i <= k1; ++i) coef[(size_t)i * K + k] = a[(size_t)i * K + k] - om[i] * wa;
             double *tk = &tproj[(size_t)k * (um + 1)];
-            for (int i = 0; i <= k1; ++i) tk[i] = a[(size_t)i * K + k];
+            for (int i = 0; i <= k1; ++i) tk[i] = a_h[(size_t)i * K + k];
             const double *c = &use->cs[(size_t)k * um], *s = &use->sn[(size_t)k * um];
             for (int i = 0; i < k1; ++i) {
                 const double u = c[i] * tk[i] + s[i] * tk[i + 1];
@@ -786,8 +801,6 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                 tk[i] = u;
             }
         }
-        h2d(ws.abuf.p, coef.data(), sizeof(double) * coef.size(), st);
-        block_subtract(st, V1p, nb1, ws.abuf.as<double>(), Wd, N, K);
     };
 
     DBIE_HIP(hipMemsetAsync(X, 0, sizeof(double) * NK, st));
@@ -840,14 +853,15 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     };
     bool first = true;
     for (;;) {
+        if (defl) deflate(Wp);
+        col_norms(op, Wp, N, K, partial, nbuf, beta);  // synchronizes the stream
         if (defl) {
-            deflate(Wp);
+            project();
             cvec.assign((size_t)um * K, 0.0);
             for (int k = 0; k < K; ++k)
                 for (int i = 0; i < use->k1[k]; ++i) cvec[(size_t)k * um + i] = tproj[(size_t)k * (um + 1) + i];
             Bm.assign((size_t)K * um * m, 0.0);
         }
-        col_norms(op, Wp, N, K, partial, nbuf, beta);
         bool any = false;
         for (int k = 0; k < K; ++k) {
             if (first && !defl) {
@@ -884,12 +898,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                 Zin = Z.as<double>();
             }
             apply_dev(op, K, Zin, N, Wp, N, prec);
-            if (defl) {
-                deflate(Wp);
-                for (int k = 0; k < K; ++k)
-                    for (int i = 0; i < use->k1[k]; ++i)
-                        Bm[((size_t)k * um + i) * m + j] = tproj[(size_t)k * (um + 1) + i];
-            }
+            if (defl) deflate(Wp);
             // classical Gram-Schmidt, twice (CGS2) for FP64 solves; once for the low-precision inner
             // solves of the refinement, whose operator is accurate to ~1e-7 and whose tolerance is
             // ~1e-5: the orthogonality CGS1 keeps is far below both
@@ -902,10 +911,20 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                 DBIE_HIP(hipMemcpyAsync(h2.data(), nbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
             }
             DBIE_HIP(hipMemcpyAsync(h1.data(), hbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
-            std::vector<double> hn;
-            col_norms(op, Wp, N, K, partial, nbuf, hn);  // synchronizes the stream
-            h2d(nbuf.p, hn.data(), sizeof(double) * K, st);
-            col_scale(st, Wp, nbuf.as<double>(), true, Vp + (size_t)(j + 1) * NK, N, K);
+            // the next basis vector from the squared norms on the device; one sync brings the
+            // coefficients, the norms and the deflation projections to the host
+            block_dots(st, Wp, 1, Wp, N, K, partial.as<double>(), nbuf.as<double>());
+            col_normalize(st, Wp, nbuf.as<double>(), Vp + (size_t)(j + 1) * NK, N, K);
+            std::vector<double> hn(K);
+            DBIE_HIP(hipMemcpyAsync(hn.data(), nbuf.p, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+            DBIE_HIP(hipStreamSynchronize(st));
+            for (auto &v : hn) v = std::sqrt(std::max(v, 0.0));
+            if (defl) {
+                project();
+                for (int k = 0; k < K; ++k)
+                    for (int i = 0; i < use->k1[k]; ++i)
+                        Bm[((size_t)k * um + i) * m + j] = tproj[(size_t)k * (um + 1) + i];
+            }
             ++res.iters;
             bool all = true;
             for (int k = 0; k < K; ++k) {

@@ -123,6 +123,11 @@ void block_dots(hipStream_t st, const double *V, int nv, const double *W, int64_
 void block_subtract(hipStream_t st, const double *V, int nv, const double *h, double *W, int64_t N, int K);
 // out[:,k] = in[:,k] * s[k] (s may be inverted: out = in / s when invert)
 void col_scale(hipStream_t st, const double *in, const double *s, bool invert, double *out, int64_t N, int K);
+// out[:, k] = in[:, k] / sqrt(sq[k]) (zero where sq[k] <= 0)
+void col_normalize(hipStream_t st, const double *in, const double *sq, double *out, int64_t N, int K);
+// coef[i, k] = a[i, k] - om[i, k] sum_{l < n1[k]} om[l, k] a[l, k] for i < n1[k], zero for n1[k] <= i < nb
+// (a, om, coef: nb x K, element (i, k) at i K + k)
+void deflate_coef(hipStream_t st, const double *a, const double *om, const int *n1, int nb, int K, double *coef);
 // out[:,k] = d .* in[:,k]  (d per row)
 void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K);
 // X[:,k] += sum_i V_i[:,k] y[i*K+k]

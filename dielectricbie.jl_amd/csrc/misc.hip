@@ -251,6 +251,30 @@ __global__ void col_scale_kernel(const double *__restrict__ in, const double *__
     out[(int64_t)k * N + n] = r;
 }
 
+// out[:, k] = in[:, k] / sqrt(sq[k]) (zero where sq[k] <= 0): a new Arnoldi vector from its squared norm
+// without a host round trip
+__global__ void col_normalize_kernel(const double *__restrict__ in, const double *__restrict__ sq,
+                                     double *__restrict__ out, int64_t N, int K) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (n >= N) return;
+    const double s = sq[k];
+    out[(int64_t)k * N + n] = s > 0.0 ? in[(int64_t)k * N + n] / sqrt(s) : 0.0;
+}
+
+// coef[i, k] = a[i, k] - om[i, k] sum_l om[l, k] a[l, k] for i < n1[k], zero otherwise (the projection
+// onto a subspace of the span of nb basis blocks given by its unit normal om, column by column)
+__global__ void deflate_coef_kernel(const double *__restrict__ a, const double *__restrict__ om,
+                                    const int *__restrict__ n1, int nb, int K, double *__restrict__ coef) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int n = n1[k];
+    double wa = 0.0;
+    for (int i = 0; i < n; ++i) wa += om[(int64_t)i * K + k] * a[(int64_t)i * K + k];
+    for (int i = 0; i < nb; ++i)
+        coef[(int64_t)i * K + k] = i < n ? a[(int64_t)i * K + k] - om[(int64_t)i * K + k] * wa : 0.0;
+}
+
 __global__ void row_scale_kernel(const double *__restrict__ d, const double *__restrict__ in, double *__restrict__ out,
                                  int64_t N, int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -372,6 +396,14 @@ void block_combine(hipStream_t st, const double *V, int nv, const double *y, dou
 
 void col_scale(hipStream_t st, const double *in, const double *s, bool invert, double *out, int64_t N, int K) {
     col_scale_kernel<<<grid2(N, K), 256, 0, st>>>(in, s, invert ? 1 : 0, out, N, K);
+}
+
+void col_normalize(hipStream_t st, const double *in, const double *sq, double *out, int64_t N, int K) {
+    col_normalize_kernel<<<grid2(N, K), 256, 0, st>>>(in, sq, out, N, K);
+}
+
+void deflate_coef(hipStream_t st, const double *a, const double *om, const int *n1, int nb, int K, double *coef) {
+    deflate_coef_kernel<<<(unsigned)((K + 63) / 64), 64, 0, st>>>(a, om, n1, nb, K, coef);
 }
 
 void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K) {
