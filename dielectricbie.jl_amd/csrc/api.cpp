@@ -1050,16 +1050,16 @@ void gram(dbie_op *op, GmresWork &ws, const double *A, int p, const double *B, i
 }
 
 // W (N x K) = Q R with Q overwriting W, R upper triangular (K x K, leading dimension ldr): Cholesky QR
-// twice (G = W^T W = R1^T R1, W <- W R1^-1, again, R = R2 R1).  Returns false, with W unchanged in
-// meaning, when a Gram matrix is not numerically positive definite.
-bool chol_qr2(dbie_op *op, GmresWork &ws, double *W, int64_t N, int K, double *R, int64_t ldr) {
+// twice (G = W^T W = R1^T R1, W <- W R1^-1, again, R = R2 R1), or once with npass = 1.  Returns false,
+// with W unchanged in meaning, when a Gram matrix is not numerically positive definite.
+bool chol_qr2(dbie_op *op, GmresWork &ws, double *W, int64_t N, int K, double *R, int64_t ldr, int npass = 2) {
     rocblas_handle h = op_blas(op);
     hipStream_t st = op->stream;
     const double one = 1.0;
     ws.R.ensure(sizeof(double) * (size_t)K * K);
     ws.info.ensure(sizeof(rocblas_int));
     double *G = ws.R.as<double>();
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < npass; ++pass) {
         gram(op, ws, W, K, W, K, N, G, K, 0.0);
         blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, K, G, K, ws.info.as<rocblas_int>()), "rocsolver_dpotrf");
         rocblas_int info = 0;
@@ -1130,8 +1130,11 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
     DBIE_HIP(hipMemcpyAsync(Wp, B, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
     GmresOut res;
     // V (N x K) = Q R in place (V holds a copy of W), R (upper triangular) into Rdst
+    // FP64 solves orthogonalise twice (block CGS2, Cholesky QR twice); the low-precision inner solves
+    // of the refinement once each: their operator is accurate to ~1e-7 and their tolerance ~1e-5
+    const int npass = prec == DBIE_FP64 ? 2 : 1;
     auto orthonormalise = [&](double *V, double *Rdst, int64_t ld) {
-        if (chol_qr2(op, ws, V, N, K, Rdst, ld)) return;
+        if (chol_qr2(op, ws, V, N, K, Rdst, ld, npass)) return;
         // the Gram matrix lost definiteness (nearly dependent directions): Householder QR
         DBIE_HIP(hipMemcpyAsync(V, Wp, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
         blas_ok(rocsolver_dgeqrf(h, (rocblas_int)N, KK, V, (rocblas_int)N, taup), "rocsolver_dgeqrf");
@@ -1162,7 +1165,7 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
             apply_dev(op, K, Zin, N, Wp, N, prec);
             // block CGS2: Hc = V^T W, W -= V Hc, twice (the second correction accumulated into Hc)
             DBIE_HIP(hipMemsetAsync(Hc, 0, sizeof(double) * (size_t)ldh * K, st));
-            for (int pass = 0; pass < 2; ++pass) {
+            for (int pass = 0; pass < npass; ++pass) {
                 gram(op, ws, Vp, nb, Wp, K, N, Yp, ldh, 0.0);
                 blas_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, KK, nb, &mone,
                                       Vp, (rocblas_int)N, Yp, (rocblas_int)ldh, &one, Wp, (rocblas_int)N),
