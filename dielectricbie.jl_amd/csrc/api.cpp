@@ -710,8 +710,8 @@ void col_norms(dbie_op *op, const double *V, int64_t N, int K, DBuf &partial, DB
 struct GmresOut {
     int iters = 0;
     bool solved = false;
-    std::vector<double> resid;  // per column: the residual norm the Krylov process ended with (GMRES per
-                                // column only; empty otherwise)
+    std::vector<double> resid;  // per column: the residual norm the Krylov process ended with (from the
+                                // Givens recurrence per column, the explicit block residual for block GMRES)
 };
 
 // GMRES(m) for K independent columns sharing each block matvec.  Right
@@ -1144,6 +1144,7 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
     for (;;) {
         // residual check of the current iterate (W holds R = B - A X)
         col_norms(op, Wp, N, K, ws.partial, ws.nbuf, rn);
+        res.resid = rn;
         bool all = true;
         for (int k = 0; k < K; ++k) all = all && rn[k] <= tol[k];
         if (all) { res.solved = true; break; }
@@ -1228,6 +1229,7 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
                     "rocblas_dgeam");
             std::vector<double> resn;
             col_norms(op, Res, nr, K, ws.partial, ws.nbuf, resn);
+            res.resid = resn;
             conv = true;
             for (int k = 0; k < K && conv; ++k) conv = resn[k] <= tol[k];
             if (conv) { ++j; break; }
