@@ -219,7 +219,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--K", type=int, default=256)
     ap.add_argument("--precision", default="fp16x3", choices=["fp32", "bf16", "bf16x3", "fp16x3"])
     ap.add_argument("--inner-rtol", type=float, default=1e-5)
