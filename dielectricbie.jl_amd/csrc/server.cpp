@@ -104,9 +104,9 @@ size_t estimate_op_bytes(int64_t N, int nq) {
     const double blocks = 50.0 * P;
     return (size_t)(N * 8 * 24 + blocks * npp * npp * 16.0 + blocks * 8 + 64.0 * 1024 * 1024);
 }
-// Krylov basis and the recycled basis of mixed-precision refinements (about 30 blocks together),
-// work vectors and the far-field partial sums for K columns.
-size_t estimate_ws_bytes(int64_t N, int K) { return (size_t)N * K * 8 * 40 + (size_t)N * K * 8 * 6; }
+// Krylov basis and the recycled basis of mixed-precision refinements (up to about 2 x 25 blocks for
+// inner solves of up to 24 iterations), work vectors and the far-field partial sums for K columns.
+size_t estimate_ws_bytes(int64_t N, int K) { return (size_t)N * K * 8 * 56 + (size_t)N * K * 8 * 6; }
 
 }  // namespace
 
