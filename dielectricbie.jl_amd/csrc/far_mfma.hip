@@ -805,8 +805,15 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
         const __amdgpu_buffer_rsrc_t rc =
             __builtin_amdgcn_make_buffer_rsrc((void *)coord, (short)0, (int)(N * (int64_t)sizeof(double)), 0x00020000);
         const int loff = ((4 * pw * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+        // The coordinate copy goes first: a barrier preceded by vmcnt(8) then has every wave's
+        // coordinates of the stage landed (the other producer waves read them in generate) while
+        // the 8 B-operand copies, read by the consumers a stage later, may still fly.
         auto copy_stage = [&](int64_t js, int pb) {
             const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+            // 64 lanes x 4 bytes = the stage's 32 doubles (reads past the end return zero)
+            if (pw < 3)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rc, (lptr_t)&s_pos[pb][pw][0], 4, lane * 4,
+                                                         (int)(js * (int64_t)sizeof(double)), 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int cb = 4 * pw + q;
@@ -815,10 +822,6 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(rl, (lptr_t)&s_b[pb][cb][1][0], 16, loff + q * 16 * kSb * 2,
                                                          soff, 0, 0);
             }
-            // 64 lanes x 4 bytes = the stage's 32 doubles (reads past the end return zero)
-            if (pw < 3)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rc, (lptr_t)&s_pos[pb][pw][0], 4, lane * 4,
-                                                         (int)(js * (int64_t)sizeof(double)), 0, 0);
         };
         // A tiles of stage js (coordinates in s_pos[cb]) into s_a[pb].  Lane 16 r + e (and
         // 16 r + e + 8) takes source 8 r + e; the row broadcast hands it to the whole row.
@@ -897,8 +900,8 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
             copy_stage(jc, 0);
             if (j1 < walk.jend) {
                 copy_stage(j1, 1);
-                if (pw < 3) ws_producer_barrier<9>();  // stage jc's copies (and the mask rows) are in
-                else ws_producer_barrier<8>();         // LDS; stage j1's may fly
+                ws_producer_barrier<8>();  // stage jc's copies, stage j1's coordinates (and the mask
+                                           // rows) are in LDS; stage j1's B operands may fly
             } else {
                 ws_producer_barrier<0>();
             }
@@ -906,8 +909,10 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
             ws_producer_barrier<63>();
         }
         // stage jc (parity C, buffer bc): generate stage j1 into s_a[1 - C] from s_pos[bc + 1] while
-        // the consumers multiply stage jc; copy stage j2 into buffer bc + 2 (mod 3).  Before the
-        // barrier, stage j1's copies have landed; only this stage's may fly.
+        // the consumers multiply stage jc; copy stage j2 into buffer bc + 2 (mod 3).  Stage j1's
+        // coordinates landed before the previous barrier (in every wave); before this barrier
+        // stage j1's B operands and stage j2's coordinates land, and only stage j2's B operands
+        // (this wave's newest 8 copies) may fly.
         auto step = [&](auto cc) {
             constexpr int C = decltype(cc)::value;
             const int64_t j2 = after(j1);
@@ -915,7 +920,6 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
             if (more2) copy_stage(j2, bc == 0 ? 2 : bc - 1);
             generate(1 - C, bc == 2 ? 0 : bc + 1, j1);
             if (!more2) ws_producer_barrier<0>();
-            else if (pw < 3) ws_producer_barrier<9>();
             else ws_producer_barrier<8>();
             jc = j1;
             j1 = j2;

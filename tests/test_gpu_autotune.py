@@ -117,3 +117,26 @@ def test_warp_specialised_kernel_per_source_exclusion(dbie, O, K):
     finally:
         dbie.autotune(True, clear=True)
     say(f"  max column error vs oracle {max(col_err(Y, Yref) for Y in out.values()):.2e}")
+
+
+def test_split_fp16_applies_are_bit_reproducible(dbie):
+    """Repeated split-FP16 applies at N = 64,384, K = 256 give the same bits, eager and replayed as a
+    HIP graph, for the warp-specialised kernel at 1 source split (the longest stage walk: 2012 stages
+    of 32 sources per workgroup) and at 6 splits.  Each stage's source coordinates are copied into LDS
+    by three producer waves and read by all four, so the copies must have landed in every wave before
+    the barrier that precedes their use."""
+    say("bit-reproducible split-FP16 applies, N=64384, K=256")
+    g = dbie.single_dielectric_box3d(2.4, 1.6, 1.2, 4, 0.05, 4.0, 1.0)
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
+    X = np.random.default_rng(24).standard_normal((g.n_points, 256))
+    try:
+        for graphs in (False, True):
+            op.graphs(graphs)
+            for splits, var in ((1, 1), (1, -1), (6, 1)):
+                op.set_tuned_shape(256, "fp16x3", splits, var)
+                ref = op.apply(X, "fp16x3")
+                for it in range(6):
+                    assert np.array_equal(op.apply(X, "fp16x3"), ref), (graphs, splits, var, it)
+    finally:
+        op.graphs(True)
+        dbie.autotune(True, clear=True)
