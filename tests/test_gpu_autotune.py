@@ -121,7 +121,8 @@ def test_warp_specialised_kernel_per_source_exclusion(dbie, O, K):
 
 def test_split_fp16_applies_are_bit_reproducible(dbie):
     """Repeated split-FP16 applies at N = 64,384, K = 256 give the same bits, eager and replayed as a
-    HIP graph, for the warp-specialised kernel at 1 source split (the longest stage walk: 2012 stages
+    HIP graph, for the warp-specialised kernel with one (variant 1) and two (variant -1) consumer waves
+    per SIMD and the general kernel (variant 0), at 1 source split (the longest stage walk: 2012 stages
     of 32 sources per workgroup) and at 6 splits.  Each stage's source coordinates are copied into LDS
     by three producer waves and read by all four, so the copies must have landed in every wave before
     the barrier that precedes their use."""
@@ -132,7 +133,7 @@ def test_split_fp16_applies_are_bit_reproducible(dbie):
     try:
         for graphs in (False, True):
             op.graphs(graphs)
-            for splits, var in ((1, 1), (1, -1), (6, 1)):
+            for splits, var in ((1, 1), (1, -1), (1, 0), (6, 1), (6, -1), (6, 0)):
                 op.set_tuned_shape(256, "fp16x3", splits, var)
                 ref = op.apply(X, "fp16x3")
                 for it in range(6):
