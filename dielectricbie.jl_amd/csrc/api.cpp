@@ -515,17 +515,37 @@ void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int6
     g_pending.push_back(PendingTiming{key, idx, e0, e1});
 }
 
-void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
-                   const FarTune &tune_in) {
-    const int64_t N = op->N();
-    // a forced split count stays within what the launch accepts and 2 GB of partial sums
+// Workspaces of one apply: the far-field partial sums (part_cap doubles), the far-field operand
+// staging (w.*X tile-major, or its fp16/bf16 split), the tile-major FP32 copy of X for the full
+// near blocks, and 4 bytes for the fp16 scale reduction.
+struct ApplyWs {
+    double *part = nullptr;
+    int64_t part_cap = 0;
+    void *mscratch = nullptr;
+    float *xt32 = nullptr;
+    void *tmp4 = nullptr;
+};
+
+// Partial-sum elements an apply with this launch shape needs (the launchers reduce the split
+// count to what part_cap holds, never below one split of K x N).
+int64_t apply_part_need(int64_t N, int K, const FarTune &tune) {
+    return std::max<int64_t>(far_field_part_elems(N, K), (int64_t)tune.splits * std::max(K, 1) * N + 64 * N);
+}
+
+// a forced split count stays within what the launch accepts and 2 GB of partial sums
+FarTune clamp_tune(int64_t N, int K, const FarTune &tune_in) {
     FarTune tune = tune_in;
     if (tune.splits > 0) {
         const int64_t cap = std::max<int64_t>(1, (int64_t)(2.0e9 / (8.0 * std::max(K, 1) * N)));
         tune.splits = (int)std::min<int64_t>({(int64_t)tune.splits, (int64_t)far_field_max_splits(N), cap});
     }
-    const int64_t need = std::max<int64_t>(far_field_part_elems(N, K), (int64_t)tune.splits * std::max(K, 1) * N + 64 * N);
-    op->part.ensure(need * sizeof(double));
+    return tune;
+}
+
+// Y = A X with the given workspaces (no allocation here).
+void apply_core(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                const FarTune &tune, const ApplyWs &ws) {
+    const int64_t N = op->N();
     int S;
     bool exclude = false;  // far field leaves the near panel pairs out (their full interaction is added below)
     if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
@@ -537,30 +557,51 @@ void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
             op->prof[prec == DBIE_FP64 ? 0 : 1].push_back({e0, e1});
         }
         if (prec == DBIE_FP16X3 && op->f16_gscale == 0.0f) {
-            op->gmax.ensure(sizeof(unsigned));
-            op->f16_gscale = far_field_f16_scale(op->stream, op->dg, op->gmax.p);
+            op->f16_gscale = far_field_f16_scale(op->stream, op->dg, ws.tmp4);
             if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
         }
-        op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
         exclude = excludes_near(op, K, prec);
-        S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, op->part.as<double>(), need, op->mscratch.p,
+        S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, ws.part, ws.part_cap, ws.mscratch,
                            op->f16_gscale, exclude, e0, e1, tune);
     } else {
         ProfScope ps(op, prec == DBIE_FP64 ? 0 : 1);
         if (prec == DBIE_FP64) {
-            S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, op->part.as<double>(), need, tune);
+            S = far_field_f64(op->stream, op->dg, FAR_DT, K, X, ldx, ws.part, ws.part_cap, tune);
         } else {
-            S = far_field_f32(op->stream, op->dg, K, X, ldx, op->part.as<double>(), need, tune);
+            S = far_field_f32(op->stream, op->dg, K, X, ldx, ws.part, ws.part_cap, tune);
         }
     }
     {
         ProfScope ps(op, 2);
         const NearVals nv = exclude ? NEAR_UP32 : (prec != DBIE_FP64 ? NEAR_C32 : NEAR_C64);
-        if (nv == NEAR_UP32) op->xt32.ensure(near_xt_bytes(N, K));
-        near_combine(op->stream, op->bsr, nv, S, op->part.as<double>(), N, K, X, ldx, op->diag.as<double>(), Y, ldy,
-                     nv == NEAR_UP32 ? op->xt32.as<float>() : nullptr);
+        near_combine(op->stream, op->bsr, nv, S, ws.part, N, K, X, ldx, op->diag.as<double>(), Y, ldy,
+                     nv == NEAR_UP32 ? ws.xt32 : nullptr);
     }
     DBIE_HIP(hipGetLastError());
+}
+
+void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                   const FarTune &tune_in) {
+    const int64_t N = op->N();
+    const FarTune tune = clamp_tune(N, K, tune_in);
+    const int64_t need = apply_part_need(N, K, tune);
+    op->part.ensure(need * sizeof(double));
+    ApplyWs ws;
+    ws.part = op->part.as<double>();
+    ws.part_cap = need;
+    if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
+        if (prec == DBIE_FP16X3 && op->f16_gscale == 0.0f) {
+            op->gmax.ensure(sizeof(unsigned));
+            ws.tmp4 = op->gmax.p;
+        }
+        op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
+        ws.mscratch = op->mscratch.p;
+        if (excludes_near(op, K, prec)) {
+            op->xt32.ensure(near_xt_bytes(N, K));
+            ws.xt32 = op->xt32.as<float>();
+        }
+    }
+    apply_core(op, K, X, ldx, Y, ldy, prec, tune, ws);
 }
 
 // ---- screening boxes ---------------------------------------------------------------------------
@@ -1415,6 +1456,60 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
 }
 
 }  // namespace
+
+size_t apply_noalloc_bytes(const dbie_op *op, int K, dbie_precision prec) {
+    const int64_t N = op->N();
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t b = 256;  // fp16 scale reduction
+    if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
+        b += up(far_field_mfma_scratch_bytes(N, (int)prec, K));
+        if (excludes_near(op, K, prec)) b += up(near_xt_bytes(N, K));
+    }
+    return b + sizeof(double) * (size_t)std::max(K, 1) * (size_t)N;  // one split of partial sums
+}
+
+void apply_noalloc(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                   void *scratch, size_t bytes) {
+    const int64_t N = op->N();
+    const size_t need = apply_noalloc_bytes(op, K, prec);
+    if (need > bytes)
+        fail(DBIE_BAD_ARG, "apply without allocation: N = " + std::to_string(N) + ", K = " + std::to_string(K) +
+                               " needs " + std::to_string(need) + " bytes of scratch, the library holds " +
+                               std::to_string(bytes));
+    // the launch shape the tuner settled on for this class (no exploration: it would time the
+    // launch with events), else the built-in default
+    FarTune tune;
+    if (autotune_on()) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(tune_key(op, K, prec, excludes_near(op, K, prec)));
+        if (it != g_tune.end() && it->second.done) tune = it->second.best;
+    }
+    tune = clamp_tune(N, K, tune);
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char *base = static_cast<char *>(scratch);
+    size_t off = 256;
+    ApplyWs ws;
+    ws.tmp4 = base;
+    if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
+        ws.mscratch = base + off;
+        off += up(far_field_mfma_scratch_bytes(N, (int)prec, K));
+        if (excludes_near(op, K, prec)) {
+            ws.xt32 = reinterpret_cast<float *>(base + off);
+            off += up(near_xt_bytes(N, K));
+        }
+    }
+    ws.part = reinterpret_cast<double *>(base + off);
+    ws.part_cap = (int64_t)((bytes - off) / sizeof(double));  // >= K x N: at least one split fits
+    // no kernel timing here: event creation is a library allocation
+    struct NoProf {
+        dbie_op *op;
+        bool was;
+        ~NoProf() { op->profiling = was; }
+    } np{op, op->profiling};
+    op->profiling = false;
+    apply_core(op, K, X, ldx, Y, ldy, prec, tune, ws);
+}
+
 }  // namespace dbie
 
 // ============================== C ABI ========================================
@@ -1891,6 +1986,13 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
 }
 
 void *dbie_op_stream(dbie_op *op) { return op ? (void *)op->stream : nullptr; }
+
+dbie_status dbie_op_synchronize(dbie_op *op) {
+    return guard([&] {
+        check_op(op);
+        DBIE_HIP(hipStreamSynchronize(op->stream));
+    });
+}
 
 dbie_status dbie_autotune(int enable, int clear) {
     return guard([&] {

@@ -187,6 +187,17 @@ using TargetCubatureFn = std::function<void(const TargetCubatureJob &, const Geo
 TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, double atol, double range_factor,
                             int n_threads, const TargetCubatureFn &device = nullptr);
 
+// ---- operator apply without allocation (noalloc.hip) ----------------------------
+// Scratch bytes dbie_apply_device needs when every workspace comes from one caller buffer: the
+// far-field operand staging, the tile-major X copy of the full near blocks, one source split of
+// partial sums and 256 bytes for the fp16 scale reduction.
+size_t apply_noalloc_bytes(const dbie_op *op, int K, dbie_precision prec);
+// Y = A X on the operator's stream with its workspaces carved out of scratch (bytes long, device
+// memory, 256-byte aligned); throws DBIE_BAD_ARG when apply_noalloc_bytes exceeds bytes.  Uses the
+// tuner's settled launch shape (never explores), no HIP graphs, no kernel timing.
+void apply_noalloc(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
+                   void *scratch, size_t bytes);
+
 }  // namespace dbie
 
 // Opaque handles of the C ABI.

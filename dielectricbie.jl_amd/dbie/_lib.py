@@ -126,6 +126,7 @@ SIGNATURES = {
     "dbie_solve_block": [_vp, C.c_int, _dp, C.c_int64, _dp, C.c_int64, C.POINTER(SolveOpts), C.POINTER(SolveStats)],
     "dbie_pottrg": [_vp, C.c_int64, _dp, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
     "dbie_op_stream": [_vp],
+    "dbie_op_synchronize": [_vp],
     "dbie_autotune": [C.c_int, C.c_int],
     "dbie_autotune_choice": [_vp, C.c_int, C.c_int, _ip, _ip],
     "dbie_autotune_set": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],

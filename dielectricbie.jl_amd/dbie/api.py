@@ -446,6 +446,10 @@ class BatchedDielectricOperator:
         L.check(L.lib().dbie_apply_device(self._h, dX.shape[1], dX.ptr, self.n, dY.ptr, self.n,
                                           L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)))
 
+    def synchronize(self):
+        """Wait for the work queued on the operator's stream (apply_device returns once queued)."""
+        L.check(L.lib().dbie_op_synchronize(self._h))
+
     def solve_device(self, dF: "DeviceMatrix", dSigma: "DeviceMatrix", *, rtol=1e-10, atol=0.0, itmax=500,
                      restart=0, precondition=True, precision="fp64", inner_rtol=1e-4, max_refine=10,
                      check_residual=True, raise_on_fail=True, krylov="columns", recycle=True):

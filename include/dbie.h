@@ -369,6 +369,8 @@ dbie_status dbie_op_graphs(dbie_op *op, int enable, int64_t *captures, int64_t *
 
 /* ---- device-pointer variants (stream-ordered on the operator's stream) ---- */
 void *dbie_op_stream(dbie_op *op);
+/* waits until the work queued on the operator's stream (device-pointer applies) has finished */
+dbie_status dbie_op_synchronize(dbie_op *op);
 dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
                               dbie_precision precision);
 /* dbie_solve_block with F and Sigma resident in device memory (solve_dielectric_box3d_block,
