@@ -134,6 +134,7 @@ SIGNATURES = {
     "dbie_op_profile": [_vp, C.c_int],
     "dbie_op_profile_read": [_vp, C.c_int, _i64p, _dp],
     "dbie_apply_device": [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, C.c_int],
+    "dbie_apply_device_noalloc": [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, C.c_int],
     "dbie_solve_block_device": [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, C.POINTER(SolveOpts),
                                 C.POINTER(SolveStats)],
     "dbie_device_alloc": [C.c_size_t, C.POINTER(_vp)],

@@ -373,6 +373,18 @@ void *dbie_op_stream(dbie_op *op);
 dbie_status dbie_op_synchronize(dbie_op *op);
 dbie_status dbie_apply_device(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
                               dbie_precision precision);
+/* No counterpart in the reference: dbie_apply_device for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+   workspace (operand staging, partial sums, X copy) comes from one fixed 256 MiB device array the
+   library holds per GPU until the process ends; the call allocates nothing and launches on the
+   operator's stream (stream-ordered; dbie_op_synchronize waits).  DBIE_BAD_ARG when the apply needs
+   more scratch (one source split of N x K FP64 partial sums plus the operand staging; N = 64,384,
+   K = 256 fits).  The launch shape is the autotuner's settled one (no tuning here), graphs are not
+   used.  Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+   two threads) race.  The first split-FP16 call on an operator also computes its fp16 scale and
+   synchronizes the stream. */
+dbie_status dbie_apply_device_noalloc(dbie_op *op, int K, const double *dX, int64_t ldx, double *dY, int64_t ldy,
+                                      dbie_precision precision);
 /* dbie_solve_block with F and Sigma resident in device memory (solve_dielectric_box3d_block,
    src/solver/multi_rhs.jl:270-286); returns after the solve completes. */
 dbie_status dbie_solve_block_device(dbie_op *op, int K, const double *dF, int64_t ldf, double *dSigma, int64_t lds,
