@@ -98,6 +98,9 @@ struct dbie_op {
     // resident device data
     DBuf x, y, z, nx, ny, nz, w, xf, yf, zf, nxf, nyf, nzf, wf, plane, diag;
     DBuf bsr_row, bsr_col, bsr_v64, bsr_v32, bsr_vup32, nmask;
+    dbie_near_opts nopts{};       // near options the operator was built with
+    DBuf bsr_d64;                 // correction blocks of the double layer D (built on first use)
+    bool d_built = false;
     DevGeom dg;
     DevBSR bsr;
     int adaptive_hits = 0;
@@ -236,6 +239,154 @@ void compute_diag(dbie_op *op, const double *eps_in, const double *eps_out) {
         for (int j = 0; j < npp; ++j) op->diag_h[p * npp + j] = t;
     }
     upload(op->diag, op->diag_h.data(), g.N, op->stream);
+}
+
+// Correction blocks of one kernel mode (0: laplace3d_DT_corrections, 1: laplace3d_D_corrections,
+// src/kernel/laplace3d_near_upsampling.jl:127-282) for the operator's geometry with the near options o:
+// the near-pair list (build_neighbor_list, src/kernel/laplace3d_near.jl:240-392; device candidate scan
+// when o.device_build), then the blocks.  On the device path the values land in v64 (and v32, vup32
+// when given) in the near kernels' storage order and true is returned; otherwise bs.val holds them
+// (row-major blocks) and false is returned.  bs gets the BSR structure either way.  t: ms of the pair
+// list, the host part of the block build, and the device block build.
+bool build_correction_blocks(dbie_op *op, const dbie_near_opts &o, int mode, BlockSparse &bs, DBuf *v64, DBuf *v32,
+                             DBuf *vup32, double *t) {
+    using Clk = std::chrono::steady_clock;
+    auto ms = [](Clk::time_point a, Clk::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    const Geometry &g = op->g;
+    const int64_t N = g.N;
+    hipStream_t st = op->stream;
+    const int bd = g.npp();
+    const bool mfma_order = bd == 16;
+    bs.bd = bd;
+    NearOpts no;
+    no.up_tol = o.up_tol;
+    no.max_order = o.max_order;
+    no.correct_edges = o.correct_edges != 0;
+    no.adaptive_atol = o.adaptive_atol;
+    no.adaptive_rtol = o.adaptive_rtol;
+    no.adaptive_n_GL = o.adaptive_n_GL;
+    no.adaptive_max_depth = o.adaptive_max_depth;
+    no.n_threads = o.n_threads;
+    no.mode = mode;
+    // the device builder keeps a block's kernel values in LDS
+    const bool on_device =
+        o.device_build && (double)bd * o.max_order * o.max_order * sizeof(double) <= (double)kNearBuildLds;
+    const auto t0 = Clk::now();
+    DevGeom dgp;
+    dgp.N = N;
+    dgp.x = op->x.as<double>(); dgp.y = op->y.as<double>(); dgp.z = op->z.as<double>();
+    const PairList pl = o.device_build ? device_pair_list(g, dgp, o.max_order, o.up_tol, o.correct_edges != 0, st)
+                                       : build_pair_list(g, o.max_order, o.up_tol, o.correct_edges != 0, o.n_threads);
+    const auto t1 = Clk::now();
+    t[0] = ms(t0, t1);
+    if (!on_device) {
+        bs = build_corrections(g, pl, no);
+        t[1] = ms(t1, Clk::now());
+        t[2] = 0.0;
+        return false;
+    }
+    // host: pair order, source caches, touching-pair quadtrees; device: every block
+    const int nGL = o.adaptive_n_GL == 0 ? g.nq : o.adaptive_n_GL;
+    const bool dev_quad = adaptive_blocks_lds(g.nq, nGL, o.adaptive_max_depth) <= kNearBuildLds &&
+                          o.adaptive_max_depth >= 0;
+    const CorrectionPlan cp = plan_corrections(g, pl, no, !dev_quad);
+    const auto t2 = Clk::now();
+    t[1] = ms(t1, t2);
+    bs.row_ptr = cp.row_ptr;
+    bs.col = cp.col;
+    bs.adaptive_hits = cp.adaptive_hits;
+    const size_t nv = (size_t)cp.nblocks() * bd * bd;
+    v64->ensure(sizeof(double) * std::max<size_t>(nv, 1));
+    if (v32) v32->ensure(sizeof(float) * std::max<size_t>(nv, 1));
+    if (vup32) vup32->ensure(sizeof(float) * std::max<size_t>(nv, 1));
+    DBuf us, ut, ul, co, cn, ca, as, at, al, ak;
+    upload(us, cp.up_src.data(), cp.up_src.size(), st);
+    upload(ut, cp.up_trg.data(), cp.up_trg.size(), st);
+    upload(ul, cp.up_slot.data(), cp.up_slot.size(), st);
+    upload(co, cp.cache_off.data(), cp.cache_off.size(), st);
+    upload(cn, cp.cache_nu.data(), cp.cache_nu.size(), st);
+    upload(ca, cp.caches.data(), cp.caches.size(), st);
+    upload(as, cp.adp_src.data(), cp.adp_src.size(), st);
+    upload(at, cp.adp_trg.data(), cp.adp_trg.size(), st);
+    upload(al, cp.adp_slot.data(), cp.adp_slot.size(), st);
+    int max_nu = 0;
+    for (int v : cp.cache_nu) max_nu = std::max(max_nu, v);
+    DevGeom dgb;
+    dgb.N = N;
+    dgb.x = op->x.as<double>(); dgb.y = op->y.as<double>(); dgb.z = op->z.as<double>();
+    dgb.nx = op->nx.as<double>(); dgb.ny = op->ny.as<double>(); dgb.nz = op->nz.as<double>();
+    dgb.w = op->w.as<double>();
+    const int n_adp = (int)cp.adp_src.size();
+    DBuf ar, am, ru, ah;
+    std::vector<int> hits;
+    if (dev_quad) {
+        // touching pairs: quadtrees on the device, K_up blocks into ak
+        const std::vector<double> rules = quadtree_rules(g, nGL);
+        upload(ar, cp.adp_rep.data(), cp.adp_rep.size(), st);
+        upload(am, cp.adp_map.data(), cp.adp_map.size(), st);
+        upload(ru, rules.data(), rules.size(), st);
+        ak.ensure(sizeof(double) * std::max<size_t>((size_t)n_adp * bd * bd, 1));
+        ah.ensure(sizeof(int) * std::max(n_adp, 1));
+        if (!adaptive_blocks(st, dgb, g.nq, nGL, n_adp, as.as<int>(), at.as<int>(), ar.as<int>(), am.as<double>(),
+                             ru.as<double>(), o.adaptive_atol, o.adaptive_rtol, o.adaptive_max_depth,
+                             kQuadtreeNodeBudget, ak.as<double>(), ah.as<int>(), mode))
+            fail(DBIE_BAD_ARG, "near block build: quadtree too deep for LDS");
+        DBIE_HIP(hipGetLastError());
+        hits.resize(n_adp);
+        if (n_adp > 0) {
+            DBIE_HIP(hipMemcpyAsync(hits.data(), ah.p, sizeof(int) * n_adp, hipMemcpyDeviceToHost, st));
+            DBIE_HIP(hipStreamSynchronize(st));
+        }
+        bs.adaptive_hits = 0;
+        std::vector<double> blk((size_t)bd * bd);
+        for (int k = 0; k < n_adp; ++k) {
+            if (hits[k] == 2) {  // walk over the node budget: this block on the host
+                hits[k] = touching_block(g, cp.adp_src[k], cp.adp_trg[k], no, blk.data()) ? 1 : 0;
+                h2d(ak.as<double>() + (size_t)k * bd * bd, blk.data(), sizeof(double) * blk.size(), st);
+            }
+            bs.adaptive_hits += hits[k];
+        }
+    } else {
+        upload(ak, cp.adp_kup.data(), cp.adp_kup.size(), st);
+    }
+    const bool ok = build_near_blocks(st, dgb, bd, mfma_order, (int)cp.up_src.size(), us.as<int>(), ut.as<int>(),
+                                      ul.as<int>(), co.as<int>(), cn.as<int>(), ca.as<double>(), max_nu, n_adp,
+                                      as.as<int>(), at.as<int>(), al.as<int>(), ak.as<double>(), v64->as<double>(),
+                                      v32 ? v32->as<float>() : nullptr, vup32 ? vup32->as<float>() : nullptr, mode);
+    if (!ok) fail(DBIE_BAD_ARG, "near block build: upsampling order too high for LDS");
+    DBIE_HIP(hipGetLastError());
+    DBIE_HIP(hipStreamSynchronize(st));
+    t[2] = ms(t2, Clk::now());
+    return true;
+}
+
+// The near corrections of the double layer D (laplace3d_D_corrections,
+// src/kernel/laplace3d_near_upsampling.jl:278-282) on the operator's near-pair list, built on first use
+// into bsr_d64 in the storage order of the FP64 near kernels.  The pair list depends on the geometry
+// alone, so the blocks sit in the slots of the Dt corrections.
+void ensure_d_corrections(dbie_op *op) {
+    if (op->d_built) return;
+    if (!op->nopts.build_corrections) fail(DBIE_BAD_ARG, "the operator was created without near corrections");
+    BlockSparse bs;
+    double t[3];
+    const bool dev = build_correction_blocks(op, op->nopts, 1, bs, &op->bsr_d64, nullptr, nullptr, t);
+    if (bs.nblocks() != op->bsr.nb) fail(DBIE_HIP_ERROR, "D corrections: pair list differs from the operator's");
+    if (!dev) {
+        const int bd = bs.bd;
+        if (bd == 16) {
+            parallel_for(bs.nblocks(), op->nopts.n_threads, [&](int64_t b, int) {
+                double tmpb[256];
+                double *blk = &bs.val[(size_t)b * 256];
+                for (int tt = 0; tt < 16; ++tt)
+                    for (int c = 0; c < 16; ++c) tmpb[bsr_storage_index(16, true, tt, c)] = blk[tt * 16 + c];
+                std::copy(tmpb, tmpb + 256, blk);
+            });
+        }
+        upload(op->bsr_d64, bs.val.data(), std::max<size_t>(bs.val.size(), 1), op->stream);
+    }
+    op->d_built = true;
 }
 
 // Event pair around a launch when profiling is on (kernel id: 0 far FP64, 1 far low precision, 2 combine).
@@ -1728,112 +1879,21 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
             return std::chrono::duration<double, std::milli>(b - a).count();
         };
         const auto t0 = Clk::now();
-        auto t1 = t0, t2 = t0;
+        auto t2 = t0;
         const int npp = g.npp(), bd = bs.bd;
         const bool mfma_order = bd == 16;
-        NearOpts no;
-        no.up_tol = o.up_tol;
-        no.max_order = o.max_order;
-        no.correct_edges = o.correct_edges != 0;
-        no.adaptive_atol = o.adaptive_atol;
-        no.adaptive_rtol = o.adaptive_rtol;
-        no.adaptive_n_GL = o.adaptive_n_GL;
-        no.adaptive_max_depth = o.adaptive_max_depth;
-        no.n_threads = o.n_threads;
-        // the device builder keeps a block's kernel values in LDS
-        const bool on_device = o.build_corrections && o.device_build &&
-                               (double)bd * o.max_order * o.max_order * sizeof(double) <= (double)kNearBuildLds;
         bool device_built = false;
         if (o.build_corrections) {
-            DevGeom dgp;
-            dgp.N = N;
-            dgp.x = op->x.as<double>(); dgp.y = op->y.as<double>(); dgp.z = op->z.as<double>();
-            const PairList pl = o.device_build
-                                    ? device_pair_list(g, dgp, o.max_order, o.up_tol, o.correct_edges != 0, st)
-                                    : build_pair_list(g, o.max_order, o.up_tol, o.correct_edges != 0, o.n_threads);
-            t1 = Clk::now();
-            if (on_device) {
-                // host: pair order, source caches, touching-pair quadtrees; device: every block
-                const int nGL = o.adaptive_n_GL == 0 ? g.nq : o.adaptive_n_GL;
-                const bool dev_quad = adaptive_blocks_lds(g.nq, nGL, o.adaptive_max_depth) <= kNearBuildLds &&
-                                      o.adaptive_max_depth >= 0;
-                const CorrectionPlan cp = plan_corrections(g, pl, no, !dev_quad);
-                t2 = Clk::now();
-                bs.row_ptr = cp.row_ptr;
-                bs.col = cp.col;
-                bs.adaptive_hits = cp.adaptive_hits;
-                const size_t nv = (size_t)cp.nblocks() * bd * bd;
-                op->bsr_v64.ensure(sizeof(double) * std::max<size_t>(nv, 1));
-                op->bsr_v32.ensure(sizeof(float) * std::max<size_t>(nv, 1));
-                op->bsr_vup32.ensure(sizeof(float) * std::max<size_t>(nv, 1));
-                DBuf us, ut, ul, co, cn, ca, as, at, al, ak;
-                upload(us, cp.up_src.data(), cp.up_src.size(), st);
-                upload(ut, cp.up_trg.data(), cp.up_trg.size(), st);
-                upload(ul, cp.up_slot.data(), cp.up_slot.size(), st);
-                upload(co, cp.cache_off.data(), cp.cache_off.size(), st);
-                upload(cn, cp.cache_nu.data(), cp.cache_nu.size(), st);
-                upload(ca, cp.caches.data(), cp.caches.size(), st);
-                upload(as, cp.adp_src.data(), cp.adp_src.size(), st);
-                upload(at, cp.adp_trg.data(), cp.adp_trg.size(), st);
-                upload(al, cp.adp_slot.data(), cp.adp_slot.size(), st);
-                int max_nu = 0;
-                for (int v : cp.cache_nu) max_nu = std::max(max_nu, v);
-                DevGeom dgb;
-                dgb.N = N;
-                dgb.x = op->x.as<double>(); dgb.y = op->y.as<double>(); dgb.z = op->z.as<double>();
-                dgb.nx = op->nx.as<double>(); dgb.ny = op->ny.as<double>(); dgb.nz = op->nz.as<double>();
-                dgb.w = op->w.as<double>();
-                const int n_adp = (int)cp.adp_src.size();
-                DBuf ar, am, ru, ah;
-                std::vector<int> hits;
-                if (dev_quad) {
-                    // touching pairs: quadtrees on the device, K_up blocks into ak
-                    const std::vector<double> rules = quadtree_rules(g, nGL);
-                    upload(ar, cp.adp_rep.data(), cp.adp_rep.size(), st);
-                    upload(am, cp.adp_map.data(), cp.adp_map.size(), st);
-                    upload(ru, rules.data(), rules.size(), st);
-                    ak.ensure(sizeof(double) * std::max<size_t>((size_t)n_adp * bd * bd, 1));
-                    ah.ensure(sizeof(int) * std::max(n_adp, 1));
-                    if (!adaptive_blocks(st, dgb, g.nq, nGL, n_adp, as.as<int>(), at.as<int>(), ar.as<int>(),
-                                         am.as<double>(), ru.as<double>(), o.adaptive_atol, o.adaptive_rtol,
-                                         o.adaptive_max_depth, kQuadtreeNodeBudget, ak.as<double>(), ah.as<int>()))
-                        fail(DBIE_BAD_ARG, "near block build: quadtree too deep for LDS");
-                    DBIE_HIP(hipGetLastError());
-                    hits.resize(n_adp);
-                    if (n_adp > 0) {
-                        DBIE_HIP(hipMemcpyAsync(hits.data(), ah.p, sizeof(int) * n_adp, hipMemcpyDeviceToHost, st));
-                        DBIE_HIP(hipStreamSynchronize(st));
-                    }
-                    bs.adaptive_hits = 0;
-                    std::vector<double> blk((size_t)bd * bd);
-                    for (int k = 0; k < n_adp; ++k) {
-                        if (hits[k] == 2) {  // walk over the node budget: this block on the host
-                            hits[k] = touching_block(g, cp.adp_src[k], cp.adp_trg[k], no, blk.data()) ? 1 : 0;
-                            h2d(ak.as<double>() + (size_t)k * bd * bd, blk.data(), sizeof(double) * blk.size(), st);
-                        }
-                        bs.adaptive_hits += hits[k];
-                    }
-                } else {
-                    upload(ak, cp.adp_kup.data(), cp.adp_kup.size(), st);
-                }
-                const bool ok = build_near_blocks(st, dgb, bd, mfma_order, (int)cp.up_src.size(), us.as<int>(), ut.as<int>(),
-                                  ul.as<int>(), co.as<int>(), cn.as<int>(), ca.as<double>(), max_nu,
-                                  n_adp, as.as<int>(), at.as<int>(), al.as<int>(), ak.as<double>(),
-                                  op->bsr_v64.as<double>(), op->bsr_v32.as<float>(), op->bsr_vup32.as<float>());
-                if (!ok) fail(DBIE_BAD_ARG, "near block build: upsampling order too high for LDS");
-                DBIE_HIP(hipGetLastError());
-                DBIE_HIP(hipStreamSynchronize(st));
-                device_built = true;
-                op->setup_ms[3] = ms(t2, Clk::now());
-            } else {
-                bs = build_corrections(g, pl, no);
-                t2 = Clk::now();
-            }
+            double tb[3] = {0, 0, 0};
+            device_built = build_correction_blocks(op.get(), o, 0, bs, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, tb);
+            op->setup_ms[0] = tb[0];
+            op->setup_ms[1] = tb[1];
+            op->setup_ms[3] = tb[2];
+            t2 = Clk::now();
         } else {
             bs.row_ptr.assign(g.P + 1, 0);
         }
-        op->setup_ms[0] = ms(t0, t1);
-        op->setup_ms[1] = ms(t1, t2);
+        op->nopts = o;
         op->adaptive_hits = bs.adaptive_hits;
         // panel-pair mask of the near list (the split-FP16 far field leaves these pairs out)
         const int mwords = (int)((g.P + 31) / 32) + 2;
@@ -1842,7 +1902,7 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
             for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b)
                 mask[(size_t)p * mwords + bs.col[b] / 32] |= 1u << (bs.col[b] % 32);
         const auto t3 = Clk::now();
-        op->setup_ms[2] = ms(t2 > t0 ? t2 : t0, t3) - op->setup_ms[3];
+        op->setup_ms[2] = ms(t2, t3);
         upload(op->bsr_row, bs.row_ptr.data(), bs.row_ptr.size(), st);
         upload(op->bsr_col, bs.col.data(), bs.col.size(), st);
         upload(op->nmask, mask.data(), mask.size(), st);
@@ -1939,28 +1999,40 @@ dbie_status dbie_op_set_eps(dbie_op *op, const double *eps_in, const double *eps
     });
 }
 
-dbie_status dbie_op_corrections(const dbie_op *op, int *src, int *trg, double *blocks) {
-    return guard([&] {
-        check_op(op);
-        const DevBSR &c = op->bsr;
-        std::vector<int> rp(c.P + 1), col(c.nb);
-        DBIE_HIP(hipMemcpy(rp.data(), c.row_ptr, sizeof(int) * (c.P + 1), hipMemcpyDeviceToHost));
-        if (c.nb) DBIE_HIP(hipMemcpy(col.data(), c.col, sizeof(int) * c.nb, hipMemcpyDeviceToHost));
-        for (int p = 0; p < c.P; ++p)
-            for (int b = rp[p]; b < rp[p + 1]; ++b) {
-                if (src) src[b] = col[b];
-                if (trg) trg[b] = p;
-            }
-        if (blocks && c.nb) {
-            const size_t bb = (size_t)c.bd * c.bd;
-            std::vector<double> raw(c.nb * bb);
-            DBIE_HIP(hipMemcpy(raw.data(), c.val64, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
-            for (int64_t b = 0; b < c.nb; ++b)
-                for (int t = 0; t < c.bd; ++t)
-                    for (int cc = 0; cc < c.bd; ++cc)
-                        blocks[b * bb + (size_t)t * c.bd + cc] = raw[b * bb + bsr_storage_index(c.bd, c.mfma_order, t, cc)];
+static void export_corrections(dbie_op *op, dbie_kernel kernel, int *src, int *trg, double *blocks) {
+    check_op(op);
+    if (kernel != DBIE_OP_DT && kernel != DBIE_OP_D) fail(DBIE_BAD_ARG, "corrections exist for DBIE_OP_DT and DBIE_OP_D");
+    const DevBSR &c = op->bsr;
+    const double *v64 = c.val64;
+    if (kernel == DBIE_OP_D) {
+        ensure_d_corrections(op);
+        v64 = op->bsr_d64.as<double>();
+    }
+    std::vector<int> rp(c.P + 1), col(c.nb);
+    DBIE_HIP(hipMemcpy(rp.data(), c.row_ptr, sizeof(int) * (c.P + 1), hipMemcpyDeviceToHost));
+    if (c.nb) DBIE_HIP(hipMemcpy(col.data(), c.col, sizeof(int) * c.nb, hipMemcpyDeviceToHost));
+    for (int p = 0; p < c.P; ++p)
+        for (int b = rp[p]; b < rp[p + 1]; ++b) {
+            if (src) src[b] = col[b];
+            if (trg) trg[b] = p;
         }
-    });
+    if (blocks && c.nb) {
+        const size_t bb = (size_t)c.bd * c.bd;
+        std::vector<double> raw(c.nb * bb);
+        DBIE_HIP(hipMemcpy(raw.data(), v64, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+        for (int64_t b = 0; b < c.nb; ++b)
+            for (int t = 0; t < c.bd; ++t)
+                for (int cc = 0; cc < c.bd; ++cc)
+                    blocks[b * bb + (size_t)t * c.bd + cc] = raw[b * bb + bsr_storage_index(c.bd, c.mfma_order, t, cc)];
+    }
+}
+
+dbie_status dbie_op_corrections(const dbie_op *op, int *src, int *trg, double *blocks) {
+    return guard([&] { export_corrections(const_cast<dbie_op *>(op), DBIE_OP_DT, src, trg, blocks); });
+}
+
+dbie_status dbie_op_corrections_kernel(dbie_op *op, dbie_kernel kernel, int *src, int *trg, double *blocks) {
+    return guard([&] { export_corrections(op, kernel, src, trg, blocks); });
 }
 
 dbie_status dbie_op_setup_ms(const dbie_op *op, double *t) {
@@ -1978,7 +2050,7 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
         int64_t b = 0;
         for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
                               &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
-                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->nmask, &op->part,
+                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_d64, &op->nmask, &op->part,
                               &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32})
             b += (int64_t)d->bytes;
         *bytes = b;
@@ -2118,37 +2190,55 @@ dbie_status dbie_apply(dbie_op *op, int K, const double *X, int64_t ldx, double 
     });
 }
 
+// Plain boundary operator (corrected = false) or operator with its near corrections (corrected = true),
+// no diagonal: far-field partial sums reduced by the combine with zero diag and the given blocks.
+static void apply_kernel_host(dbie_op *op, dbie_kernel kernel, bool corrected, int K, const double *X, int64_t ldx,
+                              double *Y, int64_t ldy) {
+    check_op(op);
+    const int64_t N = op->N();
+    if (K < 1 || !X || !Y) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
+    if (ldx < N || ldy < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+    if (kernel < DBIE_OP_DT || kernel > DBIE_OP_D) fail(DBIE_BAD_ARG, "unknown kernel");
+    if (corrected && kernel == DBIE_OP_S) fail(DBIE_BAD_ARG, "near corrections exist for DBIE_OP_DT and DBIE_OP_D");
+    hipStream_t st = op->stream;
+    DevBSR C = op->bsr;
+    DBuf zrow;
+    if (corrected) {
+        if (!op->nopts.build_corrections) fail(DBIE_BAD_ARG, "the operator was created without near corrections");
+        if (kernel == DBIE_OP_D) {
+            ensure_d_corrections(op);
+            C.val64 = op->bsr_d64.as<double>();
+        }
+    } else {
+        std::vector<int> zeros(op->g.P + 1, 0);
+        upload(zrow, zeros.data(), zeros.size(), st);
+        C.row_ptr = zrow.as<int>();
+    }
+    op->xin.ensure(sizeof(double) * N * K);
+    op->yout.ensure(sizeof(double) * N * K);
+    const int64_t need = far_field_part_elems(N, K);
+    op->part.ensure(need * sizeof(double));
+    DBIE_HIP(hipMemcpy2DAsync(op->xin.p, sizeof(double) * N, X, sizeof(double) * ldx, sizeof(double) * N, K,
+                              hipMemcpyHostToDevice, st));
+    const int S = far_field_f64(st, op->dg, (FarMode)kernel, K, op->xin.as<double>(), N, op->part.as<double>(), need);
+    op->tmp.ensure(sizeof(double) * N);
+    DBIE_HIP(hipMemsetAsync(op->tmp.p, 0, sizeof(double) * N, st));
+    near_combine(st, C, NEAR_C64, S, op->part.as<double>(), N, K, op->xin.as<double>(), N, op->tmp.as<double>(),
+                 op->yout.as<double>(), N);
+    DBIE_HIP(hipGetLastError());
+    DBIE_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldy, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
+                              hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipStreamSynchronize(st));
+}
+
 dbie_status dbie_apply_kernel(dbie_op *op, dbie_kernel kernel, int K, const double *X, int64_t ldx, double *Y,
                               int64_t ldy) {
-    return guard([&] {
-        check_op(op);
-        const int64_t N = op->N();
-        if (K < 1 || !X || !Y) fail(DBIE_BAD_ARG, "K must be >= 1 and buffers non-NULL");
-        if (ldx < N || ldy < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
-        if (kernel < DBIE_OP_DT || kernel > DBIE_OP_D) fail(DBIE_BAD_ARG, "unknown kernel");
-        hipStream_t st = op->stream;
-        op->xin.ensure(sizeof(double) * N * K);
-        op->yout.ensure(sizeof(double) * N * K);
-        const int64_t need = far_field_part_elems(N, K);
-        op->part.ensure(need * sizeof(double));
-        DBIE_HIP(hipMemcpy2DAsync(op->xin.p, sizeof(double) * N, X, sizeof(double) * ldx, sizeof(double) * N, K,
-                                  hipMemcpyHostToDevice, st));
-        // the plain operator: far-field partial sums reduced with no corrections and zero diag
-        const int S = far_field_f64(st, op->dg, (FarMode)kernel, K, op->xin.as<double>(), N, op->part.as<double>(), need);
-        op->tmp.ensure(sizeof(double) * N);
-        DBIE_HIP(hipMemsetAsync(op->tmp.p, 0, sizeof(double) * N, st));
-        DevBSR empty = op->bsr;
-        std::vector<int> zeros(op->g.P + 1, 0);
-        DBuf zrow;
-        upload(zrow, zeros.data(), zeros.size(), st);
-        empty.row_ptr = zrow.as<int>();
-        near_combine(st, empty, NEAR_C64, S, op->part.as<double>(), N, K, op->xin.as<double>(), N, op->tmp.as<double>(),
-                     op->yout.as<double>(), N);
-        DBIE_HIP(hipGetLastError());
-        DBIE_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldy, op->yout.p, sizeof(double) * N, sizeof(double) * N, K,
-                                  hipMemcpyDeviceToHost, st));
-        DBIE_HIP(hipStreamSynchronize(st));
-    });
+    return guard([&] { apply_kernel_host(op, kernel, false, K, X, ldx, Y, ldy); });
+}
+
+dbie_status dbie_apply_kernel_corrected(dbie_op *op, dbie_kernel kernel, int K, const double *X, int64_t ldx,
+                                        double *Y, int64_t ldy) {
+    return guard([&] { apply_kernel_host(op, kernel, true, K, X, ldx, Y, ldy); });
 }
 
 dbie_status dbie_lhs_dense(dbie_op *op, double *A, int64_t lda) {

@@ -81,6 +81,7 @@ struct NearOpts {
     int adaptive_n_GL = 0;
     int adaptive_max_depth = 20;
     int n_threads = 0;
+    int mode = 0;  // correction kernel: 0 = laplace3d_DT_corrections, 1 = laplace3d_D_corrections
 };
 
 struct PairList {

@@ -176,6 +176,8 @@ constexpr int kPairData = 20;
 void pair_scan(hipStream_t st, const DevGeom &g, int P, int npp, const double *pdata, const double *corners,
                const int *is_edge, double tc2, bool correct_edges, int64_t *count, const int64_t *ptr, int *ctrg,
                double *crho, unsigned char *ctouch);
+// mode (both builders): 0 = the Dt correction blocks, 1 = the D blocks (source normal, FP64 values
+// only: val32 and vup32 may be null).
 // Touching pairs on the device: K_up blocks (n_adp x bd x bd, row-major) and max-depth flags of the
 // quadtree rule, one wave per pair.  adp_map: 10 doubles per pair (cc, bma, dma, scale of the source
 // panel); rules: glx[nq], bary[nq], ns[nGL], ws[nGL].  Returns false (nothing launched) when the
@@ -184,11 +186,12 @@ void pair_scan(hipStream_t st, const DevGeom &g, int P, int npp, const double *p
 size_t adaptive_blocks_lds(int nq, int nGL, int max_depth);
 bool adaptive_blocks(hipStream_t st, const DevGeom &g, int nq, int nGL, int n_adp, const int *adp_src,
                      const int *adp_trg, const int *adp_rep, const double *adp_map, const double *rules, double atol,
-                     double rtol, int max_depth, int max_nodes, double *kup, int *hit);
+                     double rtol, int max_depth, int max_nodes, double *kup, int *hit, int mode = 0);
 bool build_near_blocks(hipStream_t st, const DevGeom &g, int bd, bool mfma_order, int n_up, const int *up_src,
                        const int *up_trg, const int *up_slot, const int *cache_off, const int *cache_nu,
                        const double *caches, int max_nu, int n_adp, const int *adp_src, const int *adp_trg,
-                       const int *adp_slot, const double *adp_kup, double *val64, float *val32, float *vup32);
+                       const int *adp_slot, const double *adp_kup, double *val64, float *val32, float *vup32,
+                       int mode = 0);
 
 // Near-target cubatures on the device (laplace3d_pottrg_corrections_hcubature): for each pair,
 // the adaptive Genz-Malik cubature over [-1, 1]^2 of the panel potential against the n_quad^2

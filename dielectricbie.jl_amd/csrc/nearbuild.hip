@@ -33,6 +33,15 @@ __device__ inline double grad_k(double s0, double s1, double s2, double t0, doub
     return (n0 * (ir3 * (t0 - s0)) + n1 * (ir3 * (t1 - s1)) + n2 * (ir3 * (t2 - s2))) / kFourPiB;
 }
 
+// Kernel of a correction block between a point y of the source panel and a target x: MODE 0 (:DT)
+// laplace3d_grad(y, x, n_trg), MODE 1 (:D) laplace3d_grad(x, y, n_src)
+// (src/kernel/laplace3d_near_upsampling.jl:183-197, src/kernel/laplace3d_near_adaptive.jl:169-175)
+template <int MODE>
+__device__ inline double corr_k(double y0, double y1, double y2, double x0, double x1, double x2, double tn0,
+                                double tn1, double tn2, double sn0, double sn1, double sn2) {
+    return MODE == 1 ? grad_k(x0, x1, x2, y0, y1, y2, sn0, sn1, sn2) : grad_k(y0, y1, y2, x0, x1, x2, tn0, tn1, tn2);
+}
+
 __device__ inline int store_idx(int bd, bool mfma, int t, int c) {
     return mfma ? (c >> 2) * 64 + (c & 3) * 16 + t : t * bd + c;
 }
@@ -41,7 +50,8 @@ __device__ inline int store_up_idx(int bd, bool mfma, int t, int c) {
 }
 
 // One workgroup per item.  up = 1: upsampled pair (K_up from the cache); up = 0: K_up given (kup).
-template <bool UP>
+// val32 and vup32 may be null (MODE 1 builds the FP64 blocks only).
+template <bool UP, int MODE>
 __global__ __launch_bounds__(256) void near_blocks_kernel(DevGeom g, int bd, int mfma, const int *__restrict__ src,
                                                           const int *__restrict__ trg, const int *__restrict__ slot,
                                                           const int *__restrict__ cache_off,
@@ -56,6 +66,7 @@ __global__ __launch_bounds__(256) void near_blocks_kernel(DevGeom g, int bd, int
     const int i = src[it], j = trg[it];
     const int64_t sb = (int64_t)i * bd, tb = (int64_t)j * bd;
     const double n0 = g.nx[tb], n1 = g.ny[tb], n2 = g.nz[tb];
+    const double s0 = g.nx[sb], s1 = g.ny[sb], s2 = g.nz[sb];
     const size_t out0 = (size_t)slot[it] * bd * bd;
     int nu2 = 0;
     const double *Mt = nullptr;
@@ -68,8 +79,8 @@ __global__ __launch_bounds__(256) void near_blocks_kernel(DevGeom g, int bd, int
         double *sMt = sm + bd * nu2;
         for (int q = threadIdx.x; q < bd * nu2; q += blockDim.x) {
             const int t = q / nu2, a = q - t * nu2;
-            kv[q] = grad_k(upos[3 * a], upos[3 * a + 1], upos[3 * a + 2], g.x[tb + t], g.y[tb + t], g.z[tb + t], n0,
-                           n1, n2);
+            kv[q] = corr_k<MODE>(upos[3 * a], upos[3 * a + 1], upos[3 * a + 2], g.x[tb + t], g.y[tb + t], g.z[tb + t],
+                                 n0, n1, n2, s0, s1, s2);
             if (mt_lds) sMt[q] = Mt[q];
         }
         if (mt_lds) Mt = sMt;
@@ -85,13 +96,13 @@ __global__ __launch_bounds__(256) void near_blocks_kernel(DevGeom g, int bd, int
         } else {
             s = kup[(size_t)it * bd * bd + o];
         }
-        const double direct = grad_k(g.x[sb + c], g.y[sb + c], g.z[sb + c], g.x[tb + t], g.y[tb + t], g.z[tb + t], n0,
-                                     n1, n2) *
+        const double direct = corr_k<MODE>(g.x[sb + c], g.y[sb + c], g.z[sb + c], g.x[tb + t], g.y[tb + t],
+                                           g.z[tb + t], n0, n1, n2, s0, s1, s2) *
                               g.w[sb + c];
         const double v = s - direct;
         val64[out0 + store_idx(bd, mfma != 0, t, c)] = v;
-        val32[out0 + store_idx(bd, mfma != 0, t, c)] = (float)v;
-        vup32[out0 + store_up_idx(bd, mfma != 0, t, c)] = (float)s;
+        if (val32) val32[out0 + store_idx(bd, mfma != 0, t, c)] = (float)v;
+        if (vup32) vup32[out0 + store_up_idx(bd, mfma != 0, t, c)] = (float)s;
     }
 }
 // Touching pairs: the quadtree of walk_adaptive_tree (src/kernel/laplace3d_near_adaptive.jl:119-224)
@@ -100,7 +111,8 @@ __global__ __launch_bounds__(256) void near_blocks_kernel(DevGeom g, int bd, int
 // Every sum runs in the order of the host quadtree (Quadtree::walk / adaptive_block), so the
 // split decisions and the blocks are those of the host.  One wave per pair; kup gets K_up
 // (bd x bd row-major, no direct term), hit the max-depth flag (1), or 2 when the walk stopped at
-// max_nodes nodes (the host then builds that block).
+// max_nodes nodes (the host then builds that block).  MODE as in near_blocks_kernel.
+template <int MODE>
 __global__ __launch_bounds__(64) void adaptive_blocks_kernel(DevGeom g, int nq, int nGL, const int *__restrict__ src,
                                                              const int *__restrict__ trg, const int *__restrict__ rep,
                                                              const double *__restrict__ pmap,
@@ -132,7 +144,8 @@ __global__ __launch_bounds__(64) void adaptive_blocks_kernel(DevGeom g, int nq, 
     const int64_t rn = tb + rep[it];
     const double tx = g.x[rn], ty = g.y[rn], tz = g.z[rn];
     const double n0 = g.nx[tb], n1 = g.ny[tb], n2 = g.nz[tb];
-    (void)src;
+    const int64_t sbn = (int64_t)src[it] * bd;
+    const double s0 = g.nx[sbn], s1 = g.ny[sbn], s2 = g.nz[sbn];
 
     // moments of the first ncell cells of cb into cm (Quadtree::moments)
     auto moments = [&](int ncell) {
@@ -152,7 +165,7 @@ __global__ __launch_bounds__(64) void adaptive_blocks_kernel(DevGeom g, int nq, 
             pts[3 * idx + 1] = y1;
             pts[3 * idx + 2] = y2;
             wuv[idx] = w;
-            wk[idx] = w * grad_k(y0, y1, y2, tx, ty, tz, n0, n1, n2);
+            wk[idx] = w * corr_k<MODE>(y0, y1, y2, tx, ty, tz, n0, n1, n2, s0, s1, s2);
         }
         __syncthreads();
         for (int idx = tid; idx < ncell * npp; idx += nt) {
@@ -211,7 +224,8 @@ __global__ __launch_bounds__(64) void adaptive_blocks_kernel(DevGeom g, int nq, 
                 for (int idx = tid; idx < G2 * bd; idx += nt) {
                     const int ab = idx / bd, t = idx - ab * bd;
                     const double *y = pts + 3 * (k * G2 + ab);
-                    kt[idx] = grad_k(y[0], y[1], y[2], g.x[tb + t], g.y[tb + t], g.z[tb + t], n0, n1, n2);
+                    kt[idx] = corr_k<MODE>(y[0], y[1], y[2], g.x[tb + t], g.y[tb + t], g.z[tb + t], n0, n1, n2, s0,
+                                           s1, s2);
                 }
                 __syncthreads();
                 for (int e = tid; e < bd * bd; e += nt) {
@@ -427,31 +441,47 @@ size_t adaptive_blocks_lds(int nq, int nGL, int max_depth) {
 
 bool adaptive_blocks(hipStream_t st, const DevGeom &g, int nq, int nGL, int n_adp, const int *adp_src,
                      const int *adp_trg, const int *adp_rep, const double *adp_map, const double *rules, double atol,
-                     double rtol, int max_depth, int max_nodes, double *kup, int *hit) {
+                     double rtol, int max_depth, int max_nodes, double *kup, int *hit, int mode) {
     const size_t lds = adaptive_blocks_lds(nq, nGL, max_depth);
     if (lds > kNearBuildLds || max_depth < 0) return false;
-    if (n_adp > 0)
-        adaptive_blocks_kernel<<<n_adp, 64, lds, st>>>(g, nq, nGL, adp_src, adp_trg, adp_rep, adp_map, rules, atol,
-                                                       rtol, max_depth, max_nodes, kup, hit);
+    if (n_adp > 0) {
+        if (mode == 1)
+            adaptive_blocks_kernel<1><<<n_adp, 64, lds, st>>>(g, nq, nGL, adp_src, adp_trg, adp_rep, adp_map, rules,
+                                                              atol, rtol, max_depth, max_nodes, kup, hit);
+        else
+            adaptive_blocks_kernel<0><<<n_adp, 64, lds, st>>>(g, nq, nGL, adp_src, adp_trg, adp_rep, adp_map, rules,
+                                                              atol, rtol, max_depth, max_nodes, kup, hit);
+    }
     return true;
 }
 
 bool build_near_blocks(hipStream_t st, const DevGeom &g, int bd, bool mfma_order, int n_up, const int *up_src,
                        const int *up_trg, const int *up_slot, const int *cache_off, const int *cache_nu,
                        const double *caches, int max_nu, int n_adp, const int *adp_src, const int *adp_trg,
-                       const int *adp_slot, const double *adp_kup, double *val64, float *val32, float *vup32) {
+                       const int *adp_slot, const double *adp_kup, double *val64, float *val32, float *vup32,
+                       int mode) {
+    const int mo = mfma_order ? 1 : 0;
     if (n_up > 0) {
         // kernel values in LDS; the quadrature matrix too when both fit in 64 KiB
         const size_t kv = sizeof(double) * (size_t)bd * max_nu * max_nu;
         if (kv > kNearBuildLds) return false;
         const int mt_lds = 2 * kv <= kNearBuildLds ? 1 : 0;
-        near_blocks_kernel<true><<<n_up, 256, (1 + mt_lds) * kv, st>>>(g, bd, mfma_order ? 1 : 0, up_src, up_trg,
-                                                                       up_slot, cache_off, cache_nu, caches, nullptr,
-                                                                       mt_lds, val64, val32, vup32);
+        const size_t lds = (1 + mt_lds) * kv;
+        if (mode == 1)
+            near_blocks_kernel<true, 1><<<n_up, 256, lds, st>>>(g, bd, mo, up_src, up_trg, up_slot, cache_off, cache_nu,
+                                                                caches, nullptr, mt_lds, val64, val32, vup32);
+        else
+            near_blocks_kernel<true, 0><<<n_up, 256, lds, st>>>(g, bd, mo, up_src, up_trg, up_slot, cache_off, cache_nu,
+                                                                caches, nullptr, mt_lds, val64, val32, vup32);
     }
-    if (n_adp > 0)
-        near_blocks_kernel<false><<<n_adp, 256, 0, st>>>(g, bd, mfma_order ? 1 : 0, adp_src, adp_trg, adp_slot, nullptr,
-                                                         nullptr, nullptr, adp_kup, 0, val64, val32, vup32);
+    if (n_adp > 0) {
+        if (mode == 1)
+            near_blocks_kernel<false, 1><<<n_adp, 256, 0, st>>>(g, bd, mo, adp_src, adp_trg, adp_slot, nullptr, nullptr,
+                                                                nullptr, adp_kup, 0, val64, val32, vup32);
+        else
+            near_blocks_kernel<false, 0><<<n_adp, 256, 0, st>>>(g, bd, mo, adp_src, adp_trg, adp_slot, nullptr, nullptr,
+                                                                nullptr, adp_kup, 0, val64, val32, vup32);
+    }
     return true;
 }
 

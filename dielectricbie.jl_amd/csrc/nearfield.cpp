@@ -299,6 +299,13 @@ PanelMap panel_map(const Geometry &g, int64_t p) {
 
 inline void node(const Geometry &g, int64_t n, double *p) { p[0] = g.x[n]; p[1] = g.y[n]; p[2] = g.z[n]; }
 
+// Kernel of a correction block between a point y of the source panel and a target x: mode 0 (:DT)
+// laplace3d_grad(y, x, n_trg), mode 1 (:D) laplace3d_grad(x, y, n_src)
+// (src/kernel/laplace3d_near_upsampling.jl:183-197, src/kernel/laplace3d_near_adaptive.jl:169-175)
+inline double corr_kernel(int mode, const double *y, const double *x, const double *tn, const double *sn) {
+    return mode == 1 ? grad_kernel(x, y, sn) : grad_kernel(y, x, tn);
+}
+
 // Quadtree of the touching-pair rule (walk_adaptive_tree, src/kernel/laplace3d_near_adaptive.jl:119-224)
 struct Quadtree {
     const Geometry &g;
@@ -310,6 +317,8 @@ struct Quadtree {
     int max_depth;
     std::vector<double> leaves;  // (ulo, uhi, vlo, vhi) per leaf
     bool hit = false;
+    const double *sn = nullptr;  // source panel normal (mode 1)
+    int mode = 0;
 
     void moments(double ulo, double uhi, double vlo, double vhi, double *out) const {
         const int nq = g.nq, npp = nq * nq;
@@ -325,7 +334,7 @@ struct Quadtree {
                 barycentric_row(nq, g.glx, g.bary, v, ry);
                 double y[3];
                 for (int k = 0; k < 3; ++k) y[k] = m.cc[k] + m.bma[k] * (u / 2) + m.dma[k] * (v / 2);
-                const double wk = wu * wv * sc * grad_kernel(y, trg, tn);
+                const double wk = wu * wv * sc * corr_kernel(mode, y, trg, tn, sn);
                 int q = 0;
                 for (int mx = 0; mx < nq; ++mx)
                     for (int my = 0; my < nq; ++my, ++q) out[q] += wk * rx[mx] * ry[my];
@@ -427,10 +436,11 @@ bool adaptive_block(const Geometry &g, int64_t i, int64_t j, const NearOpts &o, 
     const int nq = g.nq, bd = nq * nq;
     const int nGL = o.adaptive_n_GL == 0 ? nq : o.adaptive_n_GL;
     const PanelMap m = panel_map(g, i);
-    const double *tn = &g.normal[3 * j];
+    const double *tn = &g.normal[3 * j], *sn = &g.normal[3 * i];
     double tpts[64][3];
     for (int t = 0; t < bd; ++t) node(g, j * bd + t, tpts[t]);
-    Quadtree qt{g, m, nGL, {}, {}, nullptr, tn, o.adaptive_atol, o.adaptive_rtol, o.adaptive_max_depth, {}, false};
+    Quadtree qt{g, m, nGL, {}, {}, nullptr, tn, o.adaptive_atol, o.adaptive_rtol, o.adaptive_max_depth, {}, false,
+                sn, o.mode};
     gauss_legendre(nGL, qt.ns, qt.ws);
     const double *c1 = &g.corners[12 * i];
     int rep = 0;
@@ -456,7 +466,7 @@ bool adaptive_block(const Geometry &g, int64_t i, int64_t j, const NearOpts &o, 
                 double y[3];
                 for (int kk = 0; kk < 3; ++kk) y[kk] = m.cc[kk] + m.bma[kk] * (u / 2) + m.dma[kk] * (v / 2);
                 const double wuv = wu * wv * sc;
-                for (int t = 0; t < bd; ++t) kt[t] = grad_kernel(y, tpts[t], tn);
+                for (int t = 0; t < bd; ++t) kt[t] = corr_kernel(o.mode, y, tpts[t], tn, sn);
                 for (int mx = 0; mx < nq; ++mx)
                     for (int my = 0; my < nq; ++my) {
                         const int c = mx * nq + my;
@@ -622,14 +632,14 @@ BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearO
         std::vector<double> kv, blk(bd * bd);
         for (int64_t k = first[i]; k < first[i + 1]; ++k) {
             const int j = items[k].trg;
-            const double *tn = &g.normal[3 * j];
+            const double *tn = &g.normal[3 * j], *sn = &g.normal[3 * i];
             double tpts[64][3];
             for (int t = 0; t < bd; ++t) node(g, (int64_t)j * bd + t, tpts[t]);
             if (items[k].order > 0) {
                 const int nu2 = n_up * n_up;
                 kv.resize((size_t)nu2);
                 for (int t = 0; t < bd; ++t) {
-                    for (int al = 0; al < nu2; ++al) kv[al] = grad_kernel(&upos[3 * al], tpts[t], tn);
+                    for (int al = 0; al < nu2; ++al) kv[al] = corr_kernel(o.mode, &upos[3 * al], tpts[t], tn, sn);
                     for (int c = 0; c < bd; ++c) {
                         const double *mrow = &Mt[(size_t)c * nu2];
                         double s = 0.0;
@@ -638,7 +648,8 @@ BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearO
                     }
                 }
             } else {
-                Quadtree qt{g, m, nGL, {}, {}, nullptr, tn, o.adaptive_atol, o.adaptive_rtol, o.adaptive_max_depth, {}, false};
+                Quadtree qt{g, m, nGL, {}, {}, nullptr, tn, o.adaptive_atol, o.adaptive_rtol, o.adaptive_max_depth, {},
+                            false, sn, o.mode};
                 gauss_legendre(nGL, qt.ns, qt.ws);
                 const double *c1 = &g.corners[12 * i];
                 int rep = 0;
@@ -666,7 +677,7 @@ BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearO
                             double y[3];
                             for (int kk = 0; kk < 3; ++kk) y[kk] = m.cc[kk] + m.bma[kk] * (u / 2) + m.dma[kk] * (v / 2);
                             const double wuv = wu * wv * sc;
-                            for (int t = 0; t < bd; ++t) kt[t] = grad_kernel(y, tpts[t], tn);
+                            for (int t = 0; t < bd; ++t) kt[t] = corr_kernel(o.mode, y, tpts[t], tn, sn);
                             for (int mx = 0; mx < nq; ++mx)
                                 for (int my = 0; my < nq; ++my) {
                                     const int c = mx * nq + my;
@@ -677,11 +688,13 @@ BlockSparse build_corrections(const Geometry &g, const PairList &pl, const NearO
                     }
                 }
             }
-            // subtract the direct panel block (laplace3d_DT_panel, src/kernel/laplace3d_near.jl:25-34)
+            // subtract the direct panel block (laplace3d_DT_panel / laplace3d_D_panel,
+            // src/kernel/laplace3d_near.jl:25-45)
             double *dst = &bs.val[(size_t)slot[k] * bd * bd];
             for (int t = 0; t < bd; ++t)
                 for (int c = 0; c < bd; ++c)
-                    dst[t * bd + c] = blk[t * bd + c] - grad_kernel(src_pts[c], tpts[t], tn) * g.w[i * bd + c];
+                    dst[t * bd + c] =
+                        blk[t * bd + c] - corr_kernel(o.mode, src_pts[c], tpts[t], tn, sn) * g.w[i * bd + c];
         }
     });
     bs.adaptive_hits = hits.load();

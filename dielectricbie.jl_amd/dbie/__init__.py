@@ -33,6 +33,8 @@ from .api import (  # noqa: F401
     multi_dielectric_box3d,
     lhs_dielectric_box3d_fmm3d,
     lhs_dielectric_box3d_fmm3d_corrected,
+    laplace3d_D_fmm3d_corrected,
+    laplace3d_DT_fmm3d_corrected,
     laplace3d_pottrg_fmm3d_corrected_hcubature,
     num_points,
     rhs_dielectric_box3d,

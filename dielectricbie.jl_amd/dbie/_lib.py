@@ -110,10 +110,12 @@ SIGNATURES = {
     "dbie_op_sizes": [_vp, _i64p, _i64p, _ip],
     "dbie_op_set_eps": [_vp, _dp, _dp],
     "dbie_op_corrections": [_vp, _ip, _ip, _dp],
+    "dbie_op_corrections_kernel": [_vp, C.c_int, _ip, _ip, _dp],
     "dbie_op_device_bytes": [_vp, _i64p],
     "dbie_op_setup_ms": [_vp, _dp],
     "dbie_apply": [_vp, C.c_int, _dp, C.c_int64, _dp, C.c_int64, C.c_int],
     "dbie_apply_kernel": [_vp, C.c_int, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
+    "dbie_apply_kernel_corrected": [_vp, C.c_int, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
     "dbie_lhs_dense": [_vp, _dp, C.c_int64],
     "dbie_solve_lu": [C.c_int64, _dp, C.c_int64, C.c_int, _dp, C.c_int64],
     "dbie_rhs_points": [_vp, C.c_int, _dp, _dp, C.c_double, _dp, C.c_int64],

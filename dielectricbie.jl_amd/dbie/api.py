@@ -320,11 +320,14 @@ class BatchedDielectricOperator:
         eo = np.ascontiguousarray(np.broadcast_to(np.asarray(eps_out, np.float64), (self.interface.n_panels,)))
         L.check(L.lib().dbie_op_set_eps(self._h, L.dptr(ei), L.dptr(eo)))
 
-    def corrections(self):
-        """Correction blocks: (src panels, trg panels, blocks[b, t, c]) with C[trg*bd+t, src*bd+c] = blocks[b,t,c]."""
+    def corrections(self, kernel="DT"):
+        """Correction blocks of laplace3d_DT_corrections (kernel "DT") or laplace3d_D_corrections ("D"),
+        src/kernel/laplace3d_near_upsampling.jl:272-282: (src panels, trg panels, blocks[b, t, c]) with
+        C[trg*bd+t, src*bd+c] = blocks[b,t,c]."""
         nb, bd = self.n_blocks, self.block_dim
         src = np.zeros(nb, np.int32); trg = np.zeros(nb, np.int32); blk = np.zeros(max(nb, 1) * bd * bd)
-        L.check(L.lib().dbie_op_corrections(self._h, L.iptr(src), L.iptr(trg), L.dptr(blk)))
+        kid = {"DT": L.OP_DT, "D": L.OP_D}[kernel]
+        L.check(L.lib().dbie_op_corrections_kernel(self._h, kid, L.iptr(src), L.iptr(trg), L.dptr(blk)))
         return src, trg, blk[: nb * bd * bd].reshape(nb, bd, bd)
 
     def device_bytes(self) -> int:
@@ -380,15 +383,18 @@ class BatchedDielectricOperator:
                                    L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)))
         return Y[:, 0] if vec else Y
 
-    def apply_kernel(self, X, kernel="DT"):
-        """Plain laplace3d_DT / _S / _D matvec (no corrections, no diagonal)."""
+    def apply_kernel(self, X, kernel="DT", corrected=False):
+        """Plain laplace3d_DT / _S / _D matvec (no corrections, no diagonal); corrected=True adds the near
+        corrections of that kernel (laplace3d_DT_fmm3d_corrected / laplace3d_D_fmm3d_corrected,
+        src/kernel/laplace3d_near.jl:395-445)."""
         X = np.asarray(X, np.float64)
         vec = X.ndim == 1
         Xf = L.fortran(X, self.n)
         K = Xf.shape[1]
         Y = np.zeros((self.n, K), order="F")
         kid = {"DT": L.OP_DT, "S": L.OP_S, "D": L.OP_D}[kernel]
-        L.check(L.lib().dbie_apply_kernel(self._h, kid, K, L.dptr(Xf), self.n, L.dptr(Y), self.n))
+        fn = L.lib().dbie_apply_kernel_corrected if corrected else L.lib().dbie_apply_kernel
+        L.check(fn(self._h, kid, K, L.dptr(Xf), self.n, L.dptr(Y), self.n))
         return Y[:, 0] if vec else Y
 
     def __matmul__(self, X):
@@ -572,6 +578,43 @@ def batched_lhs_dielectric_box3d_fmm3d_corrected(interface: DielectricInterface,
                                                  max_order: int, **kw):
     """batched_lhs_dielectric_box3d_fmm3d_corrected (src/solver/multi_rhs.jl:60-100)."""
     return lhs_dielectric_box3d_fmm3d_corrected(interface, fmm_tol, up_tol, max_order, **kw)
+
+
+class _KernelMap:
+    """LinearMap{Float64} of a corrected layer potential (K + C_K, no diagonal) on a resident operator."""
+
+    def __init__(self, op: BatchedDielectricOperator, kernel: str):
+        self.op, self.kernel = op, kernel
+        self.shape = op.shape
+
+    def __matmul__(self, X):
+        return self.op.apply_kernel(X, self.kernel, corrected=True)
+
+    def __mul__(self, X):
+        return self @ X
+
+
+def laplace3d_DT_fmm3d_corrected(interface: DielectricInterface, fmm_tol: float, up_tol: float, max_order: int, *,
+                                 correct_edges=True, adaptive_atol=None, adaptive_rtol=SQRT_EPS, adaptive_n_GL=0,
+                                 adaptive_max_depth=20):
+    """laplace3d_DT_fmm3d_corrected (src/kernel/laplace3d_near.jl:395-419): x -> Dt x + C x (the far
+    field is the exact direct sum; fmm_tol is accepted for the reference's signature)."""
+    op = BatchedDielectricOperator(interface, up_tol=up_tol, max_order=max_order, correct_edges=correct_edges,
+                                   adaptive_atol=adaptive_atol, adaptive_rtol=adaptive_rtol,
+                                   adaptive_n_GL=adaptive_n_GL, adaptive_max_depth=adaptive_max_depth)
+    return _KernelMap(op, "DT")
+
+
+def laplace3d_D_fmm3d_corrected(interface: DielectricInterface, fmm_tol: float, up_tol: float, max_order: int, *,
+                                correct_edges=True, adaptive_atol=None, adaptive_rtol=SQRT_EPS, adaptive_n_GL=0,
+                                adaptive_max_depth=20):
+    """laplace3d_D_fmm3d_corrected (src/kernel/laplace3d_near.jl:421-445): x -> D x + C_D x with the
+    double-layer corrections laplace3d_D_corrections (source normal,
+    src/kernel/laplace3d_near_upsampling.jl:278-282) on the same near-pair list."""
+    op = BatchedDielectricOperator(interface, up_tol=up_tol, max_order=max_order, correct_edges=correct_edges,
+                                   adaptive_atol=adaptive_atol, adaptive_rtol=adaptive_rtol,
+                                   adaptive_n_GL=adaptive_n_GL, adaptive_max_depth=adaptive_max_depth)
+    return _KernelMap(op, "D")
 
 
 def dielectric_diagonal_scaling(interface: DielectricInterface) -> np.ndarray:

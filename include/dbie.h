@@ -170,6 +170,10 @@ dbie_status dbie_op_setup_ms(const dbie_op *op, double *t);
 dbie_status dbie_op_set_eps(dbie_op *op, const double *eps_in, const double *eps_out);
 /* Export the correction blocks: pairs (src, trg) and blocks[b][t + c*bd] (FP64). */
 dbie_status dbie_op_corrections(const dbie_op *op, int *src, int *trg, double *blocks);
+/* As dbie_op_corrections for the corrections of kernel DBIE_OP_DT or DBIE_OP_D (laplace3d_DT_corrections /
+   laplace3d_D_corrections, src/kernel/laplace3d_near_upsampling.jl:272-282); the D blocks are built on the
+   first call. */
+dbie_status dbie_op_corrections_kernel(dbie_op *op, dbie_kernel kernel, int *src, int *trg, double *blocks);
 /* HBM bytes the operator holds. */
 dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes);
 
@@ -181,6 +185,13 @@ dbie_status dbie_apply(dbie_op *op, int K, const double *X, int64_t ldx, double 
    matrix-free (src/kernel/laplace3d.jl:18-52 and the *_fmm3d maps :84-137). */
 dbie_status dbie_apply_kernel(dbie_op *op, dbie_kernel kernel, int K, const double *X, int64_t ldx,
                               double *Y, int64_t ldy);
+/* laplace3d_DT_fmm3d_corrected / laplace3d_D_fmm3d_corrected (src/kernel/laplace3d_near.jl:395-445):
+   Y = K X + C_K X for kernel DBIE_OP_DT or DBIE_OP_D (no diagonal), C_K the near corrections of that
+   kernel on the operator's near-pair list (laplace3d_DT_corrections / laplace3d_D_corrections,
+   src/kernel/laplace3d_near_upsampling.jl:272-282; the D blocks use the source normal and are built on
+   the first call).  DBIE_BAD_ARG for DBIE_OP_S or an operator created without corrections. */
+dbie_status dbie_apply_kernel_corrected(dbie_op *op, dbie_kernel kernel, int K, const double *X, int64_t ldx,
+                                        double *Y, int64_t ldy);
 /* Dense lhs_dielectric_box3d (src/solver/dielectric_box3d.jl:1-17): A = DT + diag(t), N x N. */
 dbie_status dbie_lhs_dense(dbie_op *op, double *A, int64_t lda);
 /* solve_lu (src/utils/linear_algebra.jl:1): LU with partial pivoting on the GPU. */

@@ -798,7 +798,14 @@ void orc_nlist_free(orc_nlist *l) {
 /* Correction blocks (src/kernel/laplace3d_near_upsampling.jl:1-276,   */
 /* src/kernel/laplace3d_near_adaptive.jl:119-224,308-387)             */
 /* ------------------------------------------------------------------ */
-typedef struct { double atol, rtol; int n_GL, max_depth; } orc_adaptive_cfg;
+typedef struct { double atol, rtol; int n_GL, max_depth, mode; } orc_adaptive_cfg;
+
+/* Kernel of the correction blocks between a point y of the source panel and a target x:
+   mode 0 (:DT) laplace3d_grad(y, x, n_trg), mode 1 (:D) laplace3d_grad(x, y, n_src)
+   (src/kernel/laplace3d_near_upsampling.jl:183-197, src/kernel/laplace3d_near_adaptive.jl:169-175) */
+static inline double corr_k(int mode, const double *y, const double *x, const double *tn, const double *sn) {
+    return mode == 1 ? k_grad(x, y, sn) : k_grad(y, x, tn);
+}
 
 /* panel geometry helpers shared by the upsampled and adaptive rules */
 typedef struct { double cc[3], bma[3], dma[3], scale; } pgeom;
@@ -809,14 +816,14 @@ static void panel_geom(const orc_iface *f, int p, pgeom *g) {
     g->scale = Lx * Ly / 4;
 }
 
-/* laplace3d_DT_panel (src/kernel/laplace3d_near.jl:25-34): Kd[t + c*ntrg] */
-static void dt_panel_direct(const orc_iface *f, int src, int trg, double *Kd) {
+/* laplace3d_DT_panel / laplace3d_D_panel (src/kernel/laplace3d_near.jl:25-45): Kd[t + c*ntrg] */
+static void panel_direct(const orc_iface *f, int mode, int src, int trg, double *Kd) {
     int npp = f->nq * f->nq;
-    const double *nt = f->normal + 3 * trg;
+    const double *nt = f->normal + 3 * trg, *ns = f->normal + 3 * src;
     for (int c = 0; c < npp; ++c)
         for (int t = 0; t < npp; ++t) {
             int gs = src * npp + c, gt = trg * npp + t;
-            Kd[t + c * npp] = k_grad(f->pt + 3 * gs, f->pt + 3 * gt, nt) * f->wt[gs];
+            Kd[t + c * npp] = corr_k(mode, f->pt + 3 * gs, f->pt + 3 * gt, nt, ns) * f->wt[gs];
         }
 }
 
@@ -852,7 +859,8 @@ static void source_cache(const orc_iface *f, int p, int n_up, double *pos /*n_up
 
 /* cell moments of one target over a (u,v) cell (walk_adaptive_tree's cell_moments_inplace!) */
 static void cell_moments(const orc_iface *f, const pgeom *g, int nGL, const double *nsG, const double *wsG,
-                         const double *trg, const double *tn, double ulo, double uhi, double vlo, double vhi, double *out) {
+                         const double *trg, const double *tn, const double *sn, int mode, double ulo, double uhi,
+                         double vlo, double vhi, double *out) {
     int nq = f->nq, npp = nq * nq;
     double rx[16], ry[16];
     for (int m = 0; m < npp; ++m) out[m] = 0.0;
@@ -866,7 +874,7 @@ static void cell_moments(const orc_iface *f, const pgeom *g, int nGL, const doub
             bary_row(nq, f->gl_x, f->bary, v, ry);
             double y[3];
             for (int k = 0; k < 3; ++k) y[k] = g->cc[k] + g->bma[k] * (u / 2) + g->dma[k] * (v / 2);
-            double kv = k_grad(y, trg, tn);
+            double kv = corr_k(mode, y, trg, tn, sn);
             double w = wu * wv * sc * kv;
             int m = 0;
             for (int mx = 0; mx < nq; ++mx)
@@ -883,16 +891,16 @@ static void leaf_push(leafbuf *L, double a, double b, double c, double d) {
 }
 /* walk_adaptive_tree's recurse_walk! (src/kernel/laplace3d_near_adaptive.jl:191-220) */
 static void walk(const orc_iface *f, const pgeom *g, int nGL, const double *nsG, const double *wsG, const double *trg,
-                 const double *tn, const orc_adaptive_cfg *cfg, double ulo, double uhi, double vlo, double vhi, int depth,
-                 leafbuf *L) {
+                 const double *tn, const double *sn, const orc_adaptive_cfg *cfg, double ulo, double uhi, double vlo,
+                 double vhi, int depth, leafbuf *L) {
     int npp = f->nq * f->nq;
     double par[64], c1[64], c2[64], c3[64], c4[64];
-    cell_moments(f, g, nGL, nsG, wsG, trg, tn, ulo, uhi, vlo, vhi, par);
+    cell_moments(f, g, nGL, nsG, wsG, trg, tn, sn, cfg->mode, ulo, uhi, vlo, vhi, par);
     double mu = (uhi + ulo) / 2, mv = (vhi + vlo) / 2;
-    cell_moments(f, g, nGL, nsG, wsG, trg, tn, ulo, mu, vlo, mv, c1);
-    cell_moments(f, g, nGL, nsG, wsG, trg, tn, mu, uhi, vlo, mv, c2);
-    cell_moments(f, g, nGL, nsG, wsG, trg, tn, ulo, mu, mv, vhi, c3);
-    cell_moments(f, g, nGL, nsG, wsG, trg, tn, mu, uhi, mv, vhi, c4);
+    cell_moments(f, g, nGL, nsG, wsG, trg, tn, sn, cfg->mode, ulo, mu, vlo, mv, c1);
+    cell_moments(f, g, nGL, nsG, wsG, trg, tn, sn, cfg->mode, mu, uhi, vlo, mv, c2);
+    cell_moments(f, g, nGL, nsG, wsG, trg, tn, sn, cfg->mode, ulo, mu, mv, vhi, c3);
+    cell_moments(f, g, nGL, nsG, wsG, trg, tn, sn, cfg->mode, mu, uhi, mv, vhi, c4);
     double err = 0.0, pmax = 0.0;
     for (int m = 0; m < npp; ++m) {
         double ch = c1[m] + c2[m] + c3[m] + c4[m];
@@ -911,10 +919,10 @@ static void walk(const orc_iface *f, const pgeom *g, int nGL, const double *nsG,
         if (depth == cfg->max_depth && err >= tl) L->hit = 1;
         return;
     }
-    walk(f, g, nGL, nsG, wsG, trg, tn, cfg, ulo, mu, vlo, mv, depth + 1, L);
-    walk(f, g, nGL, nsG, wsG, trg, tn, cfg, mu, uhi, vlo, mv, depth + 1, L);
-    walk(f, g, nGL, nsG, wsG, trg, tn, cfg, ulo, mu, mv, vhi, depth + 1, L);
-    walk(f, g, nGL, nsG, wsG, trg, tn, cfg, mu, uhi, mv, vhi, depth + 1, L);
+    walk(f, g, nGL, nsG, wsG, trg, tn, sn, cfg, ulo, mu, vlo, mv, depth + 1, L);
+    walk(f, g, nGL, nsG, wsG, trg, tn, sn, cfg, mu, uhi, vlo, mv, depth + 1, L);
+    walk(f, g, nGL, nsG, wsG, trg, tn, sn, cfg, ulo, mu, mv, vhi, depth + 1, L);
+    walk(f, g, nGL, nsG, wsG, trg, tn, sn, cfg, mu, uhi, mv, vhi, depth + 1, L);
 }
 
 /* Adaptive block for one touching pair: Kb[t + m*npp] (adaptive_block_moments_inplace!) */
@@ -924,7 +932,7 @@ static int adaptive_block(const orc_iface *f, int src, int trg, const orc_adapti
     double nsG[32], wsG[32];
     orc_gauss_legendre(nGL, nsG, wsG);
     pgeom g; panel_geom(f, src, &g);
-    const double *tn = f->normal + 3 * trg;
+    const double *tn = f->normal + 3 * trg, *sn = f->normal + 3 * src;
     /* representative target: node of trg closest to first corner of src (first argmin) */
     const double *c1 = f->corners + 12 * src;
     int rep = 0; double best = INFINITY;
@@ -934,7 +942,7 @@ static int adaptive_block(const orc_iface *f, int src, int trg, const orc_adapti
         if (d < best) { best = d; rep = t; }
     }
     leafbuf L = {0};
-    walk(f, &g, nGL, nsG, wsG, f->pt + 3 * (size_t)(trg * npp + rep), tn, cfg, -1.0, 1.0, -1.0, 1.0, 0, &L);
+    walk(f, &g, nGL, nsG, wsG, f->pt + 3 * (size_t)(trg * npp + rep), tn, sn, cfg, -1.0, 1.0, -1.0, 1.0, 0, &L);
     for (int m = 0; m < npp * npp; ++m) Kb[m] = 0.0;
     double rx[16], ry[16], kv[64];
     for (int l = 0; l < L.n; ++l) {
@@ -950,7 +958,7 @@ static int adaptive_block(const orc_iface *f, int src, int trg, const orc_adapti
                 double y[3];
                 for (int k = 0; k < 3; ++k) y[k] = g.cc[k] + g.bma[k] * (u / 2) + g.dma[k] * (v / 2);
                 double wuv = wu * wv * sc;
-                for (int t = 0; t < npp; ++t) kv[t] = k_grad(y, f->pt + 3 * (size_t)(trg * npp + t), tn);
+                for (int t = 0; t < npp; ++t) kv[t] = corr_k(cfg->mode, y, f->pt + 3 * (size_t)(trg * npp + t), tn, sn);
                 for (int mx = 0; mx < nq; ++mx)
                     for (int my = 0; my < nq; ++my) {
                         int m = mx * nq + my;
@@ -970,9 +978,11 @@ static int adaptive_block(const orc_iface *f, int src, int trg, const orc_adapti
    nodes of panel j, cols = source nodes of panel i) per pair.  Upsampled pairs
    use the per-source-panel maximum n_up (laplace3d_near_upsampling.jl:165-183).
    out_blocks is (n_up + n_adp) x npp x npp, pairs in nlist order (upsampled
-   first, then adaptive).  Returns the number of adaptive pairs that hit max_depth. */
-int orc_corrections(const orc_iface *f, const orc_nlist *nl, double adp_atol, double adp_rtol, int adp_nGL,
-                    int adp_max_depth, double *out_blocks) {
+   first, then adaptive).  Returns the number of adaptive pairs that hit max_depth.
+   mode 0: laplace3d_DT_corrections, mode 1: laplace3d_D_corrections
+   (src/kernel/laplace3d_near_upsampling.jl:272-282). */
+int orc_corrections_mode(const orc_iface *f, const orc_nlist *nl, int mode, double adp_atol, double adp_rtol,
+                         int adp_nGL, int adp_max_depth, double *out_blocks) {
     int P = f->P, nq = f->nq, npp = nq * nq;
     int *nmax = (int *)calloc(P, sizeof(int));
     for (int k = 0; k < nl->n_up; ++k) if (nl->up_p[k] > nmax[nl->up_i[k]]) nmax[nl->up_i[k]] = nl->up_p[k];
@@ -992,11 +1002,11 @@ int orc_corrections(const orc_iface *f, const orc_nlist *nl, double adp_atol, do
             source_cache(f, i, n_up, pos, Mt);
             for (int k = start[i]; k < start[i + 1]; ++k) {
                 int j = nl->up_j[k];
-                const double *tn = f->normal + 3 * j;
+                const double *tn = f->normal + 3 * j, *sn = f->normal + 3 * i;
                 for (int t = 0; t < npp; ++t)
                     for (int al = 0; al < nu2; ++al)
-                        Kmat[al + t * nu2] = k_grad(pos + 3 * al, f->pt + 3 * (size_t)(j * npp + t), tn);
-                dt_panel_direct(f, i, j, Kd);
+                        Kmat[al + t * nu2] = corr_k(mode, pos + 3 * al, f->pt + 3 * (size_t)(j * npp + t), tn, sn);
+                panel_direct(f, mode, i, j, Kd);
                 double *blk = out_blocks + (size_t)k * npp * npp;
                 for (int c = 0; c < npp; ++c)
                     for (int t = 0; t < npp; ++t) {
@@ -1008,7 +1018,7 @@ int orc_corrections(const orc_iface *f, const orc_nlist *nl, double adp_atol, do
         }
         free(pos); free(Mt); free(Kmat); free(Kd);
     }
-    orc_adaptive_cfg cfg = {adp_atol, adp_rtol, adp_nGL, adp_max_depth};
+    orc_adaptive_cfg cfg = {adp_atol, adp_rtol, adp_nGL, adp_max_depth, mode};
     int hits = 0;
 #pragma omp parallel
     {
@@ -1017,7 +1027,7 @@ int orc_corrections(const orc_iface *f, const orc_nlist *nl, double adp_atol, do
         for (int k = 0; k < nl->n_adp; ++k) {
             int i = nl->adp_i[k], j = nl->adp_j[k], nlv;
             hits += adaptive_block(f, i, j, &cfg, Kb, &nlv);
-            dt_panel_direct(f, i, j, Kd);
+            panel_direct(f, mode, i, j, Kd);
             double *blk = out_blocks + (size_t)(nl->n_up + k) * npp * npp;
             for (int m = 0; m < npp * npp; ++m) blk[m] = Kb[m] - Kd[m];
         }
@@ -1025,6 +1035,11 @@ int orc_corrections(const orc_iface *f, const orc_nlist *nl, double adp_atol, do
     }
     free(nmax); free(start);
     return hits;
+}
+
+int orc_corrections(const orc_iface *f, const orc_nlist *nl, double adp_atol, double adp_rtol, int adp_nGL,
+                    int adp_max_depth, double *out_blocks) {
+    return orc_corrections_mode(f, nl, 0, adp_atol, adp_rtol, adp_nGL, adp_max_depth, out_blocks);
 }
 
 /* Apply C (as blocks) to X: Y[rows of j, k] += blk * X[cols of i, k] */

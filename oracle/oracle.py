@@ -76,6 +76,8 @@ def lib():
         L.orc_nlist_free.argtypes = [vp]
         L.orc_corrections.restype = C.c_int
         L.orc_corrections.argtypes = [vp, vp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
+        L.orc_corrections_mode.restype = C.c_int
+        L.orc_corrections_mode.argtypes = [vp, vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
         L.orc_corrections_apply.argtypes = [vp, vp, _dp, C.c_int, _dp, _dp]
         L.orc_laplace3d_grad.restype = C.c_double
         L.orc_laplace3d_grad.argtypes = [_dp, _dp, _dp]
@@ -578,11 +580,16 @@ class NeighborList:
     def adaptive(self):
         return {(int(i), int(j)) for i, j in zip(self.adp_i, self.adp_j)}
 
-    def corrections(self, adp_atol=1e-9, adp_rtol=float(np.sqrt(np.finfo(float).eps)), adp_nGL=0, adp_max_depth=20):
+    def corrections(self, adp_atol=1e-9, adp_rtol=float(np.sqrt(np.finfo(float).eps)), adp_nGL=0, adp_max_depth=20,
+                    mode="DT"):
+        """laplace3d_DT_corrections (mode "DT") or laplace3d_D_corrections (mode "D")
+        (src/kernel/laplace3d_near_upsampling.jl:127-282); the blocks are kept for apply()."""
         npp = self.iface.nq ** 2
         nb = self.n_up + self.n_adp
         out = np.zeros(max(nb, 1) * npp * npp)
-        self.hits = lib().orc_corrections(self.iface.h, self.h, adp_atol, adp_rtol, adp_nGL, adp_max_depth, out)
+        m = {"DT": 0, "D": 1}[mode]
+        self.hits = lib().orc_corrections_mode(self.iface.h, self.h, m, adp_atol, adp_rtol, adp_nGL, adp_max_depth,
+                                               out)
         self.blocks = out[: nb * npp * npp].reshape(nb, npp, npp).transpose(0, 2, 1)  # [b, t, c]
         return self.blocks
 
