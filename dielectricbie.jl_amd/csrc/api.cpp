@@ -1749,6 +1749,20 @@ dbie_status dbie_geometry_from_panels(int64_t P, int n_quad, const double *corne
     });
 }
 
+dbie_status dbie_geometry_from_panel_nodes(int64_t P, int n_quad, const double *corners, const double *normals,
+                                           const int *is_edge, const double *points, const double *weights,
+                                           const double *gl_xs, const double *gl_ws, const double *bary_weights,
+                                           dbie_geom **out) {
+    return guard([&] {
+        if (!out) fail(DBIE_BAD_ARG, "NULL output");
+        std::unique_ptr<Geometry> g(
+            make_from_panel_nodes(P, n_quad, corners, normals, is_edge, points, weights, gl_xs, gl_ws, bary_weights));
+        auto *h = new dbie_geom;
+        h->g = std::move(*g);
+        *out = h;
+    });
+}
+
 dbie_status dbie_geometry_multi_box(int n_quad, double l_ec, int n_boxes, const double *boxes, const double *epses,
                                     double eps_out, double alpha, dbie_geom **out) {
     return guard([&] {

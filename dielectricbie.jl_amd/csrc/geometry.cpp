@@ -529,6 +529,31 @@ Geometry *make_from_panels(int64_t P, int nq, const double *corners, const doubl
     return g.release();
 }
 
+Geometry *make_from_panel_nodes(int64_t P, int nq, const double *corners, const double *normals, const int *is_edge,
+                                const double *points, const double *weights, const double *gl_xs, const double *gl_ws,
+                                const double *bary) {
+    if (!points || !weights || !gl_xs || !gl_ws)
+        fail(DBIE_BAD_ARG, "from_panel_nodes: points, weights, gl_xs and gl_ws are required");
+    std::unique_ptr<Geometry> g(make_from_panels(P, nq, corners, normals, is_edge));
+    // the caller's rule and nodes, as given (FlatPanel.gl_xs / gl_ws / bary_weights / points / weights)
+    for (int i = 0; i < nq; ++i) {
+        g->glx[i] = gl_xs[i];
+        g->glw[i] = gl_ws[i];
+    }
+    if (bary) {
+        for (int i = 0; i < nq; ++i) g->bary[i] = bary[i];
+    } else {
+        barycentric_weights(nq, g->glx, g->glw, g->bary);
+    }
+    for (int64_t n = 0; n < g->N; ++n) {
+        g->x[n] = points[3 * n];
+        g->y[n] = points[3 * n + 1];
+        g->z[n] = points[3 * n + 2];
+        g->w[n] = weights[n];
+    }
+    return g.release();
+}
+
 namespace {
 // psi_n(x) and psi_{n-1}(x) of the orthonormal Hermite functions
 // psi_k = H_k(x) exp(-x^2/2) / sqrt(2^k k! sqrt(pi)), by their three-term recurrence.

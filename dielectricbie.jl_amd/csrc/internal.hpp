@@ -64,6 +64,11 @@ void gauss_legendre(int n, double *x, double *w);  // throws on n < 1
 bool best_grid(double W, double H, double alpha, int *m, int *n);
 Geometry *make_box(double Lx, double Ly, double Lz, int nq, double l_ec, double alpha);
 Geometry *make_from_panels(int64_t P, int nq, const double *corners, const double *normals, const int *is_edge);
+// as make_from_panels with the caller's nodes (points 3 x N interleaved, weights N) and one rule for every
+// panel (gl_xs, gl_ws; bary may be null: computed from them) taken as given
+Geometry *make_from_panel_nodes(int64_t P, int nq, const double *corners, const double *normals, const int *is_edge,
+                                const double *points, const double *weights, const double *gl_xs, const double *gl_ws,
+                                const double *bary);
 // multi_dielectric_box3d: boxes nb x 6 (centre x, y, z, Lx, Ly, Lz); per-panel permittivities out
 Geometry *make_multi_box(int nq, double l_ec, int nb, const double *boxes, const double *epses, double eps_out,
                          double alpha, std::vector<double> &eps_in_p, std::vector<double> &eps_out_p);

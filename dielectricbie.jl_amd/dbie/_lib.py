@@ -98,6 +98,7 @@ SIGNATURES = {
     "dbie_best_grid": [C.c_double, C.c_double, C.c_double, _ip, _ip],
     "dbie_geometry_box": [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(_vp)],
     "dbie_geometry_from_panels": [C.c_int64, C.c_int, _dp, _dp, _ip, C.POINTER(_vp)],
+    "dbie_geometry_from_panel_nodes": [C.c_int64, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, C.POINTER(_vp)],
     "dbie_geometry_multi_box": [C.c_int, C.c_double, C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(_vp)],
     "dbie_geom_eps": [_vp, _dp, _dp],
     "dbie_geom_sizes": [_vp, _i64p, _i64p, _ip],

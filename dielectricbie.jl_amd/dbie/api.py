@@ -234,16 +234,30 @@ def multi_dielectric_box3d(n_quad: int, l_ec, boxes, epses, eps_out=1.0, *, alph
     return DielectricInterface(h.value, ei, eo)
 
 
-def interface_from_panels(corners, normals, n_quad: int, eps_in, eps_out, is_edge=None):
+def interface_from_panels(corners, normals, n_quad: int, eps_in, eps_out, is_edge=None, *, points=None,
+                          weights=None, gl_xs=None, gl_ws=None, bary_weights=None):
     """DielectricInterface(panels, eps_in, eps_out) of rect_panel3d_discretize panels
-    (src/shape/box3d.jl:22-64); corners P x 4 x 3 ordered (a, b, c, d)."""
+    (src/shape/box3d.jl:22-64); corners P x 4 x 3 ordered (a, b, c, d).  With points (N x 3),
+    weights (N), gl_xs and gl_ws (n_quad; bary_weights optional) the panels' own quadrature
+    (FlatPanel fields, src/core/panels.jl:5-22) is used as given instead of being regenerated."""
     c = np.ascontiguousarray(corners, np.float64).reshape(-1, 12)
     P = c.shape[0]
     n = np.ascontiguousarray(normals, np.float64).reshape(P, 3)
     e = np.zeros(P, np.int32) if is_edge is None else np.ascontiguousarray(is_edge, np.int32)
     h = C.c_void_p()
-    L.check(L.lib().dbie_geometry_from_panels(P, int(n_quad), L.dptr(c.ravel()), L.dptr(n.ravel()), L.iptr(e),
-                                              C.byref(h)))
+    if points is None:
+        L.check(L.lib().dbie_geometry_from_panels(P, int(n_quad), L.dptr(c.ravel()), L.dptr(n.ravel()), L.iptr(e),
+                                                  C.byref(h)))
+    else:
+        Npts = P * int(n_quad) ** 2
+        pts = np.ascontiguousarray(points, np.float64).reshape(Npts, 3)
+        w = np.ascontiguousarray(weights, np.float64).reshape(Npts)
+        gx = np.ascontiguousarray(gl_xs, np.float64).reshape(int(n_quad))
+        gw = np.ascontiguousarray(gl_ws, np.float64).reshape(int(n_quad))
+        bw = None if bary_weights is None else L.dptr(np.ascontiguousarray(bary_weights, np.float64).reshape(int(n_quad)))
+        L.check(L.lib().dbie_geometry_from_panel_nodes(P, int(n_quad), L.dptr(c.ravel()), L.dptr(n.ravel()),
+                                                       L.iptr(e), L.dptr(pts.ravel()), L.dptr(w), L.dptr(gx),
+                                                       L.dptr(gw), bw, C.byref(h)))
     return DielectricInterface(h.value, eps_in, eps_out)
 
 

@@ -129,6 +129,15 @@ dbie_status dbie_geometry_box(double Lx, double Ly, double Lz, int n_quad, doubl
    (lb, rb, rt, lt), normals P x 3, is_edge P (may be NULL). */
 dbie_status dbie_geometry_from_panels(int64_t P, int n_quad, const double *corners, const double *normals,
                                       const int *is_edge, dbie_geom **out);
+/* DielectricInterface(panels, ...) with the panels' own quadrature (FlatPanel.points, .weights, .gl_xs,
+   .gl_ws, .bary_weights; src/core/panels.jl:5-22,64-68), taken as given rather than regenerated:
+   points 3 x N (x, y, z of node n at 3n .. 3n+2, panel order, node order of the panels), weights N,
+   gl_xs / gl_ws / bary_weights n_quad (one rule for every panel; bary_weights may be NULL: computed from
+   gl_xs, gl_ws).  Corners, normals and is_edge as for dbie_geometry_from_panels. */
+dbie_status dbie_geometry_from_panel_nodes(int64_t P, int n_quad, const double *corners, const double *normals,
+                                           const int *is_edge, const double *points, const double *weights,
+                                           const double *gl_xs, const double *gl_ws, const double *bary_weights,
+                                           dbie_geom **out);
 /* multi_dielectric_box3d (src/shape/box3d_multi.jl:358-386): n_boxes axis-aligned boxes, boxes
    n_boxes x 6 (centre x, y, z, Lx, Ly, Lz; the reference's (center, Lx, Ly, Lz) NamedTuples),
    permittivity epses[b] inside box b and eps_out outside.  Panels: the pieces of every box face not

@@ -317,3 +317,29 @@ def test_device_built_blocks_match_host(dbie, nq, max_order, edges):
         Yd, Yh = opd.apply(X, prec), oph.apply(X, prec)
         assert rel(Yd, Yh) < 1e-7 * (1 + 100 * (prec == "fp32")), prec
         assert rel(Yd, Y64) < (2e-6 if prec == "fp16x3" else 1e-5), prec
+
+
+def test_operator_from_caller_nodes(dbie, O, cfg1):
+    """An interface built from the panels' own quadrature (points, weights, GL rule as given) gives the
+    mesher's operator bit for bit when the nodes are the mesher's; with nodes moved by 1e-9 the plain Dt
+    apply follows the caller's nodes (numpy direct sum on those nodes)."""
+    g, op, ref_if, ref = cfg1
+    say("operator from caller-supplied panel nodes")
+    corners, normals, is_edge = g.panels()
+    nd = g.nodes
+    x, w = dbie.gausslegendre(4)
+    g2 = dbie.interface_from_panels(corners, normals, 4, 4.0, 1.0, is_edge, points=nd[0:3].T, weights=nd[6],
+                                    gl_xs=x, gl_ws=w)
+    op2 = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g2, 1e-9, 1e-9, 8)
+    X = np.random.default_rng(60).standard_normal((g.n_points, 3))
+    assert np.array_equal(op2 @ X, op @ X)
+    pts = nd[0:3].T + 1e-9 * np.random.default_rng(61).standard_normal((g.n_points, 3))
+    g3 = dbie.interface_from_panels(corners, normals, 4, 4.0, 1.0, is_edge, points=pts, weights=nd[6],
+                                    gl_xs=x, gl_ws=w)
+    op3 = dbie.BatchedDielectricOperator(g3, corrected=False)
+    d = pts[:, None, :] - pts[None, :, :]
+    r = np.linalg.norm(d, axis=2)
+    np.fill_diagonal(r, np.inf)
+    nrm = nd[3:6].T
+    DT = np.einsum("ik,ijk->ij", nrm, d) / (4 * np.pi * r ** 3) * nd[6][None, :]
+    assert rel(op3.apply_kernel(X, "DT"), DT @ X) < 1e-12
