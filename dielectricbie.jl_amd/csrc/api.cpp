@@ -101,6 +101,17 @@ struct dbie_op {
     dbie_near_opts nopts{};       // near options the operator was built with
     DBuf bsr_d64;                 // correction blocks of the double layer D (built on first use)
     bool d_built = false;
+    // residual history of the last solve: (kind, max_k ||r_k|| / ||F_k||) per entry; kind 0 = a Krylov
+    // iteration's residual estimate, 1 = the true FP64 residual, 2 = a residual updated with the
+    // low-precision operator or taken from the inner solve's estimate.  hist_scale: 1 / ||F_k||
+    std::vector<double> hist, hist_scale;
+    void note(int kind, const std::vector<double> &r) {
+        if (hist_scale.size() != r.size()) return;
+        double m = 0.0;
+        for (size_t k = 0; k < r.size(); ++k) m = std::max(m, r[k] * hist_scale[k]);
+        hist.push_back((double)kind);
+        hist.push_back(m);
+    }
     DevGeom dg;
     DevBSR bsr;
     int adaptive_hits = 0;
@@ -135,8 +146,22 @@ struct dbie_op {
             if (a.exec) (void)hipGraphExecDestroy(a.exec);
         agraphs.clear();
     }
+    // A fused operator (dbie_solve_fused): no geometry of its own; its N x K blocks hold the columns of
+    // several member operators (member b: columns c0 .. c0 + K - 1, rows 0 .. its N - 1, the rows past
+    // that zero), and an apply runs every member's apply on the member's stream, forked from and joined
+    // back to this operator's stream.
+    struct Member {
+        dbie_op *op;
+        int c0, K;
+        hipEvent_t done;
+    };
+    std::vector<Member> members;
+    hipEvent_t fork_ev = nullptr;
     ~dbie_op() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &m : members)
+            if (m.done) (void)hipEventDestroy(m.done);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
         drop_graphs();
         if (blas) (void)rocblas_destroy_handle(blas);
         for (auto &v : prof)
@@ -198,31 +223,32 @@ void check_op(const dbie_op *op) {
     if (!op) fail(DBIE_BAD_ARG, "operator handle is NULL");
 }
 
-// Plane id per 256-node tile: nodes whose panels share one plane (normal and
-// n.corner offset equal) get the same id; tiles spanning two planes get -1.
+// Plane id per 256-node tile: nodes with the same panel normal and the same n.x (computed as the
+// kernels' entries are) share a plane id; a tile gets that id when all its nodes share it, else -1.
+// For axis-aligned normals two nodes of one plane id have n.(x - y) = 0 exactly, so the far-field
+// kernels skip same-plane tile pairs at no cost to accuracy.  The key is taken per node, not per panel,
+// so that nodes a caller supplies off their panel's plane never have their entries skipped.
 std::vector<int> tile_planes(const Geometry &g) {
     const int npp = g.npp();
-    std::vector<int> pid(g.P);
-    std::vector<std::array<double, 4>> planes;
-    for (int64_t p = 0; p < g.P; ++p) {
-        const double *n = &g.normal[3 * p], *c = &g.corners[12 * p];
-        const std::array<double, 4> key{n[0], n[1], n[2], n[0] * c[0] + n[1] * c[1] + n[2] * c[2]};
-        int id = -1;
-        for (size_t q = 0; q < planes.size(); ++q)
-            if (planes[q] == key) { id = (int)q; break; }
-        if (id < 0) {
-            if (planes.size() >= 4096) { id = -1; }
-            else { planes.push_back(key); id = (int)planes.size() - 1; }
+    std::map<std::array<double, 4>, int> ids;
+    std::vector<int> nid(g.N);
+    for (int64_t i = 0; i < g.N; ++i) {
+        const double *n = &g.normal[3 * (i / npp)];
+        const std::array<double, 4> key{n[0], n[1], n[2], n[0] * g.x[i] + n[1] * g.y[i] + n[2] * g.z[i]};
+        auto it = ids.find(key);
+        if (it == ids.end()) {
+            const int id = ids.size() < 4096 ? (int)ids.size() : -1;
+            it = ids.emplace(key, id).first;
         }
-        pid[p] = id;
+        nid[i] = it->second;
     }
     const int64_t nt = (g.N + 255) / 256;
     std::vector<int> tp(nt);
     for (int64_t t = 0; t < nt; ++t) {
         const int64_t n0 = t * 256, n1 = std::min<int64_t>(g.N, n0 + 256);
-        int id = pid[n0 / npp];
+        int id = nid[n0];
         for (int64_t n = n0; n < n1 && id >= 0; ++n)
-            if (pid[n / npp] != id) id = -1;
+            if (nid[n] != id) id = -1;
         tp[t] = id;
     }
     return tp;
@@ -614,7 +640,35 @@ void apply_graphed(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
     ++op->graph_replays;
 }
 
+void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec);
+
+// Apply of a fused operator: each member's columns through the member's own apply (tuned shape, HIP
+// graph replay) on the member's stream; the rows past a member's N are set to zero.  The member
+// streams start after the work queued on the fused stream and the fused stream continues after all
+// of them, so the members' applies overlap on the device.
+void fused_apply(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
+    int total = 0;
+    for (const auto &m : op->members) total = std::max(total, m.c0 + m.K);
+    if (K != total) fail(DBIE_DIM_MISMATCH, "fused apply: K differs from the members' columns");
+    const int64_t N = op->N();
+    DBIE_HIP(hipEventRecord(op->fork_ev, op->stream));
+    for (auto &m : op->members) {
+        DBIE_HIP(hipStreamWaitEvent(m.op->stream, op->fork_ev, 0));
+        apply_dev(m.op, m.K, X + (size_t)m.c0 * ldx, ldx, Y + (size_t)m.c0 * ldy, ldy, prec);
+        const int64_t Nm = m.op->N();
+        if (Nm < N)
+            DBIE_HIP(hipMemset2DAsync(Y + (size_t)m.c0 * ldy + Nm, sizeof(double) * ldy, 0, sizeof(double) * (N - Nm),
+                                      m.K, m.op->stream));
+        DBIE_HIP(hipEventRecord(m.done, m.op->stream));
+        DBIE_HIP(hipStreamWaitEvent(op->stream, m.done, 0));
+    }
+}
+
 void apply_dev(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec) {
+    if (!op->members.empty()) {
+        fused_apply(op, K, X, ldx, Y, ldy, prec);
+        return;
+    }
     if (!autotune_on() || K < 1) {
         apply_graphed(op, K, X, ldx, Y, ldy, prec, FarTune());
         return;
@@ -1080,6 +1134,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         col_scale(st, Wp, nbuf.as<double>(), true, Vp, N, K);
         std::fill(gv.begin(), gv.end(), 0.0);
         for (int k = 0; k < K; ++k) gv[(size_t)k * (m + 1)] = active[k] ? beta[k] : 0.0;
+        std::vector<double> cur(beta);  // per column: residual estimate of the cycle so far
         int j = 0;
         for (; j < m && res.iters < itmax; ++j) {
             Vp = basis(j + 2);
@@ -1139,9 +1194,11 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                 g[j + 1] = -s[j] * g[j];
                 g[j] = c[j] * g[j];
                 nconv[k] = j + 1;
+                cur[k] = std::fabs(g[j + 1]);
                 if (std::fabs(g[j + 1]) <= tol[k] || hn[k] == 0.0) conv[k] = 1;
                 else all = false;
             }
+            op->note(0, cur);
             if (all) { ++j; break; }
         }
         // y_k = H_k^{-1} g_k (upper triangular, size nconv[k])
@@ -1422,6 +1479,7 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
             std::vector<double> resn;
             col_norms(op, Res, nr, K, ws.partial, ws.nbuf, resn);
             res.resid = resn;
+            op->note(0, resn);
             conv = true;
             for (int k = 0; k < K && conv; ++k) conv = resn[k] <= tol[k];
             if (conv) { ++j; break; }
@@ -1479,12 +1537,21 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
         upload(dinv, inv.data(), N, st);
         dinvp = dinv.as<double>();
     }
+    dbie_solve_stats s{};
+    std::vector<double> bnorm, rn;
+    // residual history of this solve, relative to ||F_k||
+    col_norms(op, dF, N, K, partial, hb, bnorm);
+    op->hist.clear();
+    op->hist_scale.resize(K);
+    for (int k = 0; k < K; ++k) op->hist_scale[k] = bnorm[k] > 0 ? 1.0 / bnorm[k] : 0.0;
+    struct ScaleReset {
+        dbie_op *op;
+        ~ScaleReset() { op->hist_scale.clear(); }
+    } scale_reset{op};
     hipEvent_t e0, e1;
     DBIE_HIP(hipEventCreate(&e0));
     DBIE_HIP(hipEventCreate(&e1));
     DBIE_HIP(hipEventRecord(e0, st));
-    dbie_solve_stats s{};
-    std::vector<double> bnorm, rn;
     if (o.precision == DBIE_FP64) {
         const GmresOut r = o.krylov == 1 ? block_gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp)
                                          : gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp);
@@ -1528,6 +1595,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 exact = true;
                 ++n_exact;
                 col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+                op->note(1, rn);
                 continue;
             }
             if (it >= o.max_refine || s.niter >= o.itmax) break;
@@ -1567,6 +1635,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 for (int k = 0; k < K; ++k)
                     if (mask[k] != 0.0) rn[k] = r.resid[k];
                 exact = false;
+                op->note(2, rn);
                 continue;
             }
             if (small) {
@@ -1582,6 +1651,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 ++n_exact;
             }
             col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+            op->note(exact ? 1 : 2, rn);
         }
     }
     DBIE_HIP(hipEventRecord(e1, st));
@@ -1597,6 +1667,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
         apply_dev(op, K, dX, N, dR.as<double>(), N, DBIE_FP64);
         sub_from(st, dF, dR.as<double>(), N, K);
         col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
+        op->note(1, rn);
     }
     double mx = 0.0;
     for (size_t k = 0; k < rn.size(); ++k) mx = std::max(mx, bnorm[k] > 0 ? rn[k] / bnorm[k] : rn[k]);
@@ -1607,6 +1678,63 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
 }
 
 }  // namespace
+
+dbie_status solve_fused_host(int B, dbie_op *const *ops, const int *Ks, const double *const *F, const int64_t *ldf,
+                             double *const *S, const int64_t *lds, const dbie_solve_opts *opts,
+                             dbie_solve_stats *stats) {
+    if (B < 1 || !ops || !Ks || !F || !ldf || !S || !lds) fail(DBIE_BAD_ARG, "fused solve: B >= 1 and arrays required");
+    // block Arnoldi mixes columns (block orthogonalisation), and the columns of different members belong
+    // to different operators: the fused solve runs GMRES per column
+    if (opts && opts->krylov != 0)
+        fail(DBIE_BAD_ARG, "fused solve: krylov must be 0 (GMRES per column; block GMRES would mix the columns "
+                           "of different operators)");
+    ensure_device();
+    int64_t Nmax = 0;
+    int Ktot = 0;
+    for (int b = 0; b < B; ++b) {
+        check_op(ops[b]);
+        if (!ops[b]->members.empty()) fail(DBIE_BAD_ARG, "fused solve: a member is itself a fused operator");
+        for (int c = 0; c < b; ++c)
+            if (ops[c] == ops[b]) fail(DBIE_BAD_ARG, "fused solve: an operator appears twice (merge its columns)");
+        if (Ks[b] < 1 || !F[b] || !S[b]) fail(DBIE_BAD_ARG, "fused solve: K >= 1 and buffers per member");
+        if (ldf[b] < ops[b]->N() || lds[b] < ops[b]->N()) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
+        Nmax = std::max(Nmax, ops[b]->N());
+        Ktot += Ks[b];
+    }
+    std::unique_ptr<dbie_op> f(new dbie_op);
+    f->g.N = Nmax;
+    f->graphs = false;  // the members replay their own graphs
+    f->diag_h.assign(Nmax, 1.0);  // right preconditioner: each member is a single contrast (a scalar
+                                  // multiple of the identity, which GMRES is invariant to)
+    DBIE_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+    DBIE_HIP(hipEventCreateWithFlags(&f->fork_ev, hipEventDisableTiming));
+    int c0 = 0;
+    for (int b = 0; b < B; ++b) {
+        hipEvent_t e = nullptr;
+        DBIE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        f->members.push_back(dbie_op::Member{ops[b], c0, Ks[b], e});
+        c0 += Ks[b];
+    }
+    hipStream_t st = f->stream;
+    const size_t NK = (size_t)Nmax * Ktot;
+    DBuf dF, dX;
+    dF.ensure(sizeof(double) * NK);
+    dX.ensure(sizeof(double) * NK);
+    DBIE_HIP(hipMemsetAsync(dF.p, 0, sizeof(double) * NK, st));
+    for (const auto &m : f->members) {
+        const int b = (int)(&m - f->members.data());
+        DBIE_HIP(hipMemcpy2DAsync(dF.as<double>() + (size_t)m.c0 * Nmax, sizeof(double) * Nmax, F[b],
+                                  sizeof(double) * ldf[b], sizeof(double) * m.op->N(), m.K, hipMemcpyHostToDevice, st));
+    }
+    const dbie_status rc = solve_dev(f.get(), Ktot, dF.as<double>(), dX.as<double>(), opts, stats);
+    for (const auto &m : f->members) {
+        const int b = (int)(&m - f->members.data());
+        DBIE_HIP(hipMemcpy2DAsync(S[b], sizeof(double) * lds[b], dX.as<double>() + (size_t)m.c0 * Nmax,
+                                  sizeof(double) * Nmax, sizeof(double) * m.op->N(), m.K, hipMemcpyDeviceToHost, st));
+    }
+    DBIE_HIP(hipStreamSynchronize(st));
+    return rc;
+}
 
 size_t apply_noalloc_bytes(const dbie_op *op, int K, dbie_precision prec) {
     const int64_t N = op->N();
@@ -2067,11 +2195,26 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
                               &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_d64, &op->nmask, &op->part,
                               &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32})
             b += (int64_t)d->bytes;
+        // the Krylov workspaces the operator keeps for later solves
+        const GmresWork &w = op->gw;
+        for (const DBuf *d : {&w.V, &w.W, &w.Z, &w.U, &w.partial, &w.hbuf, &w.nbuf, &w.V1, &w.abuf, &w.cbuf, &w.om,
+                              &w.n1, &w.H, &w.G, &w.tau, &w.Y, &w.gpart, &w.R, &w.info, &w.PV, &w.PT})
+            b += (int64_t)d->bytes;
         *bytes = b;
     });
 }
 
 void *dbie_op_stream(dbie_op *op) { return op ? (void *)op->stream : nullptr; }
+
+dbie_status dbie_op_residual_history(const dbie_op *op, int cap, int *n, double *kind_value) {
+    return guard([&] {
+        check_op(op);
+        if (!n) fail(DBIE_BAD_ARG, "NULL output");
+        const int total = (int)(op->hist.size() / 2);
+        *n = total;
+        if (kind_value && cap > 0) std::copy(op->hist.begin(), op->hist.begin() + 2 * std::min(cap, total), kind_value);
+    });
+}
 
 dbie_status dbie_op_synchronize(dbie_op *op) {
     return guard([&] {
@@ -2412,6 +2555,16 @@ dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, d
                                   hipMemcpyDeviceToHost, st));
         DBIE_HIP(hipStreamSynchronize(st));
     });
+    if (g != DBIE_OK) return g;
+    if (rc == DBIE_NOT_CONVERGED) set_error("solve did not reach the tolerance within itmax / max_refine");
+    return rc;
+}
+
+dbie_status dbie_solve_fused(int B, dbie_op *const *ops, const int *K, const double *const *F, const int64_t *ldf,
+                             double *const *Sigma, const int64_t *lds, const dbie_solve_opts *opts,
+                             dbie_solve_stats *stats) {
+    dbie_status rc = DBIE_OK;
+    const dbie_status g = guard([&] { rc = solve_fused_host(B, ops, K, F, ldf, Sigma, lds, opts, stats); });
     if (g != DBIE_OK) return g;
     if (rc == DBIE_NOT_CONVERGED) set_error("solve did not reach the tolerance within itmax / max_refine");
     return rc;

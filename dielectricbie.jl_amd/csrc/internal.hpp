@@ -204,6 +204,11 @@ size_t apply_noalloc_bytes(const dbie_op *op, int K, dbie_precision prec);
 void apply_noalloc(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
                    void *scratch, size_t bytes);
 
+// dbie_solve_fused: one Krylov process for the block systems of B operators (host blocks).
+dbie_status solve_fused_host(int B, dbie_op *const *ops, const int *Ks, const double *const *F, const int64_t *ldf,
+                             double *const *S, const int64_t *lds, const dbie_solve_opts *opts,
+                             dbie_solve_stats *stats);
+
 }  // namespace dbie
 
 // Opaque handles of the C ABI.

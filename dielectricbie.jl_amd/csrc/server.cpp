@@ -15,12 +15,16 @@
 //   idle cached operators are evicted (least recently used) to make room.  A request that does
 //   not fit even alone fails with DBIE_OOM; the rest keep running.  Admission is first in,
 //   first out: the head of the queue waits for memory rather than being overtaken.
-// * Operator cache: keyed by geometry and permittivities; a hit skips the host build.
-// * Continuous batching: a free slot takes the oldest ready request and every other ready request
-//   on the same operator with the same solve options, up to max_batch_rhs columns, and solves
-//   them as one block (each column converges on its own).  Requests that arrive while others
-//   are on the GPU are set up on the host meanwhile and start as slots free up; the slots run
-//   on separate streams, so their device work overlaps.
+// * Operator cache: keyed by geometry (box sides, n_quad, l_ec); the permittivities only set the
+//   operator's diagonal, which is updated before a solve with other permittivities.  A hit skips the
+//   build.
+// * Continuous batching: a free slot takes the oldest ready request and every other ready request with
+//   the same solve options, up to max_batch_rhs columns: those on the same operator (and permittivities)
+//   become one column block, and small requests on other free operators join the same solve as
+//   further members of a fused Krylov process (dbie_solve_fused: every iteration applies all members
+//   at once, each on its own stream, with one host synchronisation) as long as padding to the largest
+//   member wastes at most half the Krylov work.  Requests that arrive while others are on the GPU are set up on the host meanwhile and start
+//   as slots free up; the slots run on separate streams, so their device work overlaps.
 // * Failures (allocation, non-convergence, bad arguments) are per request and reported in its
 //   info; a cancelled request is dropped from the queue or its result discarded.
 #include <hip/hip_runtime.h>
@@ -52,12 +56,13 @@ std::string last_error_text() {
     return buf;
 }
 
-using GeoKey = std::tuple<double, double, double, int, double, double, double>;
+using GeoKey = std::tuple<double, double, double, int, double>;
 
 struct OpEntry {
     GeoKey key;
     dbie_op *op = nullptr;
-    int64_t N = 0;
+    int64_t N = 0, P = 0;
+    double eps_in = 0, eps_out = 0;  // permittivities the operator's diagonal holds
     size_t bytes = 0;         // device bytes (estimate until built, then measured)
     bool building = true;
     bool failed = false;
@@ -105,8 +110,18 @@ size_t estimate_op_bytes(int64_t N, int nq) {
     return (size_t)(N * 8 * 24 + blocks * npp * npp * 16.0 + blocks * 8 + 64.0 * 1024 * 1024);
 }
 // Krylov basis and the recycled basis of mixed-precision refinements (up to about 2 x 25 blocks for
-// inner solves of up to 24 iterations), work vectors and the far-field partial sums for K columns.
-size_t estimate_ws_bytes(int64_t N, int K) { return (size_t)N * K * 8 * 56 + (size_t)N * K * 8 * 6; }
+// inner solves of up to 24 iterations), work vectors and the far-field partial sums for K columns; block
+// GMRES (krylov = 1) also keeps its block Hessenberg ((m + 1) K x m K), the Cholesky factor of its
+// normal equations ((m K)^2) and three (m + 1) K x K blocks, m the restart length (40 by default).
+size_t estimate_ws_bytes(int64_t N, int K, const dbie_solve_opts &o) {
+    size_t b = (size_t)N * K * 8 * 56 + (size_t)N * K * 8 * 6;
+    if (o.krylov == 1) {
+        const double m = o.restart > 0 ? o.restart : 40;
+        const double mk = m * K;
+        b += (size_t)(8.0 * ((m + 1) * K * mk + mk * mk + 3.0 * (m + 1) * K * K));
+    }
+    return b;
+}
 
 }  // namespace
 
@@ -123,7 +138,7 @@ struct dbie_server {
     std::deque<std::shared_ptr<Request>> ready;        // set up, waiting for a slot
     std::map<GeoKey, std::shared_ptr<OpEntry>> cache;
     size_t budget = 0, inflight = 0, peak = 0;
-    int64_t builds = 0, hits = 0, done = 0, failed = 0, batches = 0, fused = 0;
+    int64_t builds = 0, hits = 0, done = 0, failed = 0, batches = 0, fused = 0, multi = 0;
     std::vector<std::thread> threads;
 
     size_t cached_bytes() const {
@@ -175,7 +190,7 @@ struct dbie_server {
             const std::shared_ptr<Request> r = queue.front();
             queue.pop_front();
             // admission: bind or create the operator entry and reserve memory
-            const GeoKey key{r->L[0], r->L[1], r->L[2], r->nq, r->l_ec, r->eps_in, r->eps_out};
+            const GeoKey key{r->L[0], r->L[1], r->L[2], r->nq, r->l_ec};
             auto it = cache.find(key);
             std::shared_ptr<OpEntry> ent;
             bool build = false;
@@ -250,7 +265,7 @@ struct dbie_server {
                 finish(r, St::Cancelled, DBIE_OK, "cancelled");
                 continue;
             }
-            const GeoKey key{r->L[0], r->L[1], r->L[2], r->nq, r->l_ec, r->eps_in, r->eps_out};
+            const GeoKey key{r->L[0], r->L[1], r->L[2], r->nq, r->l_ec};
             const size_t need = r->pending_ws + (cache.count(key) ? 0 : r->pending_op);
             if (need > budget) {
                 queue.pop_front();
@@ -290,6 +305,9 @@ struct dbie_server {
         std::lock_guard<std::mutex> lk(mu);
         ent.op = op;
         ent.N = N;
+        ent.P = P;
+        ent.eps_in = r.eps_in;
+        ent.eps_out = r.eps_out;
         ent.bytes = (size_t)b;
         ++builds;
         return DBIE_OK;
@@ -318,10 +336,27 @@ struct dbie_server {
                a.max_refine == b.max_refine && a.krylov == b.krylov &&
                a.recycle == b.recycle;
     }
+    // One member of a batch: requests on one operator with the same permittivities, one column block.
+    struct Group {
+        std::shared_ptr<OpEntry> ent;
+        double eps_in = 0, eps_out = 0;
+        std::vector<std::shared_ptr<Request>> reqs;
+        int K = 0;
+    };
+    static constexpr int kMaxMembers = 8;  // operators fused into one Krylov process
+    // Requests join a fused Krylov process of several operators only when their own apply is small
+    // (N <= 16,384 and N^2 K <= 4e9 pair-column products): there the per-iteration launch and
+    // synchronisation costs dominate and one fused process of 8 such systems takes half the time of
+    // solving them one after another (N = 4,992, 1-14 columns, split-FP16 and FP64).  Larger applies
+    // fill the GPU on their own; fusing them saves nothing and runs every member for the slowest
+    // member's iteration count.
+    static bool fusable(const Request &r) {
+        return r.N <= 16384 && (double)r.N * (double)r.N * r.K <= 4.0e9;
+    }
+
     void slot_loop() {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            std::vector<std::shared_ptr<Request>> batch;
             cv.wait(lk, [&] {
                 if (stop) return true;
                 if (paused) return false;
@@ -330,64 +365,157 @@ struct dbie_server {
                 return false;
             });
             if (stop) return;
-            // oldest ready request whose operator is free, plus its companions
-            auto head = ready.end();
-            for (auto it = ready.begin(); it != ready.end(); ++it)
-                if (!(*it)->ent->busy) {
-                    head = it;
+            // the oldest ready request whose operator is free, its companions on that operator, and
+            // ready requests on other free operators (fused members)
+            std::shared_ptr<Request> h;
+            for (auto &r : ready)
+                if (!r->ent->busy) {
+                    h = r;
                     break;
                 }
-            const std::shared_ptr<Request> h = *head;
+            std::vector<Group> groups;
             int Ktot = 0;
+            int64_t Nmax = 0;
+            double useful = 0.0;  // sum of N x K over the batch
+            size_t reserved = 0;  // workspace the batch's requests hold
+            size_t extra = 0;     // fused solve: padded workspace beyond those reservations
             for (auto it = ready.begin(); it != ready.end();) {
-                const auto &r = *it;
-                if (r->ent == h->ent && same_opts(r->sopts, h->sopts) &&
-                    (batch.empty() || Ktot + r->K <= o.max_batch_rhs)) {
-                    batch.push_back(r);
-                    Ktot += r->K;
-                    it = ready.erase(it);
-                } else {
-                    ++it;
+                const std::shared_ptr<Request> r = *it;
+                bool take = false;
+                if (!r->ent->busy && same_opts(r->sopts, h->sopts) &&
+                    (groups.empty() || Ktot + r->K <= o.max_batch_rhs)) {
+                    Group *g = nullptr;
+                    for (auto &gg : groups)
+                        if (gg.ent == r->ent) g = &gg;
+                    const int64_t nN = std::max(Nmax, r->N);
+                    const int nK = Ktot + r->K;
+                    const double nu = useful + (double)r->N * r->K;
+                    // padding to the largest member may waste at most half the Krylov work
+                    const bool pad_ok = groups.empty() || (double)nN * nK <= 2.0 * nu;
+                    const size_t nres = reserved + r->reserved;
+                    const bool multi_next = groups.size() + (g ? 0 : 1) > 1;
+                    size_t nextra = 0;
+                    if (multi_next) {
+                        const size_t need = estimate_ws_bytes(nN, nK, h->sopts) + 16 * (size_t)nN * nK;
+                        nextra = need > nres ? need - nres : 0;
+                    }
+                    const bool mem_ok = !multi_next || in_use() + nextra <= budget;
+                    if (g) {
+                        take = g->eps_in == r->eps_in && g->eps_out == r->eps_out && pad_ok && mem_ok;
+                    } else {
+                        // other operators join only the per-column Krylov solve (block GMRES would mix
+                        // the columns of different operators)
+                        take = groups.empty() || (h->sopts.krylov == 0 && (int)groups.size() < kMaxMembers &&
+                                                  fusable(*h) && fusable(*r) && pad_ok && mem_ok);
+                    }
+                    if (take) {
+                        if (!g) {
+                            groups.push_back(Group{r->ent, r->eps_in, r->eps_out, {}, 0});
+                            g = &groups.back();
+                        }
+                        g->reqs.push_back(r);
+                        g->K += r->K;
+                        Ktot = nK;
+                        Nmax = nN;
+                        useful = nu;
+                        reserved = nres;
+                        extra = nextra;
+                    }
                 }
+                if (take) it = ready.erase(it);
+                else ++it;
             }
-            std::shared_ptr<OpEntry> ent = h->ent;
-            ent->busy = true;
-            ent->last_use = ++tick;
+            int nreq = 0;
+            for (auto &g : groups) {
+                g.ent->busy = true;
+                g.ent->last_use = ++tick;
+                nreq += (int)g.reqs.size();
+            }
             ++batches;
-            fused += (int64_t)batch.size() - 1;
+            fused += nreq - 1;
+            if (groups.size() > 1) ++multi;
+            inflight += extra;
+            note_peak();
             const auto t0 = Clock::now();
-            for (auto &r : batch) {
-                r->st = St::Solving;
-                r->t_solve0 = t0;
-                r->batch_K = Ktot;
-            }
+            for (auto &g : groups)
+                for (auto &r : g.reqs) {
+                    r->st = St::Solving;
+                    r->t_solve0 = t0;
+                    r->batch_K = Ktot;
+                }
             lk.unlock();
-            const int64_t N = ent->N;
-            std::vector<double> F((size_t)N * Ktot), S((size_t)N * Ktot);
-            int64_t col = 0;
-            for (auto &r : batch) {
-                std::memcpy(&F[(size_t)col * N], r->F.data(), sizeof(double) * N * r->K);
-                col += r->K;
+            dbie_status s = DBIE_OK;
+            std::string msg;
+            // the operators' diagonals for the batch's permittivities
+            for (auto &g : groups) {
+                if (g.ent->eps_in == g.eps_in && g.ent->eps_out == g.eps_out) continue;
+                std::vector<double> ei(g.ent->P, g.eps_in), eo(g.ent->P, g.eps_out);
+                s = dbie_op_set_eps(g.ent->op, ei.data(), eo.data());
+                if (s != DBIE_OK) {
+                    msg = last_error_text();
+                    break;
+                }
+                g.ent->eps_in = g.eps_in;
+                g.ent->eps_out = g.eps_out;
+            }
+            std::vector<std::vector<double>> F(groups.size()), S(groups.size());
+            for (size_t b = 0; b < groups.size(); ++b) {
+                const int64_t N = groups[b].ent->N;
+                F[b].resize((size_t)N * groups[b].K);
+                S[b].resize((size_t)N * groups[b].K);
+                int64_t col = 0;
+                for (auto &r : groups[b].reqs) {
+                    std::memcpy(&F[b][(size_t)col * N], r->F.data(), sizeof(double) * N * r->K);
+                    col += r->K;
+                }
             }
             dbie_solve_stats st{};
-            dbie_status s = dbie_solve_block(ent->op, Ktot, F.data(), N, S.data(), N, &h->sopts, &st);
-            std::string msg = s == DBIE_OK ? "" : last_error_text();
-            lk.lock();
-            col = 0;
-            for (auto &r : batch) {
-                r->stats = st;
-                if (s == DBIE_OK || s == DBIE_NOT_CONVERGED) {
-                    r->sigma.assign(S.begin() + (size_t)col * N, S.begin() + (size_t)(col + r->K) * N);
+            if (s == DBIE_OK) {
+                if (groups.size() == 1) {
+                    const int64_t N = groups[0].ent->N;
+                    s = dbie_solve_block(groups[0].ent->op, Ktot, F[0].data(), N, S[0].data(), N, &h->sopts, &st);
+                } else {
+                    std::vector<dbie_op *> ops;
+                    std::vector<int> Ks;
+                    std::vector<const double *> Fp;
+                    std::vector<double *> Sp;
+                    std::vector<int64_t> ld;
+                    for (size_t b = 0; b < groups.size(); ++b) {
+                        ops.push_back(groups[b].ent->op);
+                        Ks.push_back(groups[b].K);
+                        Fp.push_back(F[b].data());
+                        Sp.push_back(S[b].data());
+                        ld.push_back(groups[b].ent->N);
+                    }
+                    dbie_status rc = DBIE_OK;
+                    s = dbie::guard([&] {
+                        rc = dbie::solve_fused_host((int)groups.size(), ops.data(), Ks.data(), Fp.data(), ld.data(),
+                                                    Sp.data(), ld.data(), &h->sopts, &st);
+                    });
+                    if (s == DBIE_OK) s = rc;
                 }
-                col += r->K;
-                if (r->cancel) finish(r, St::Cancelled, DBIE_OK, "cancelled");
-                else if (s == DBIE_OK) finish(r, St::Done, DBIE_OK, "");
-                else finish(r, St::Failed, s, msg);
+                if (s != DBIE_OK) msg = last_error_text();
             }
-            ent->busy = false;
-            int64_t b = 0;
-            if (ent->op && dbie_op_device_bytes(ent->op, &b) == DBIE_OK) ent->bytes = (size_t)b;
-            if (!o.cache_ops && ent->users == 0) cache.erase(ent->key);
+            lk.lock();
+            inflight -= extra;
+            for (size_t b = 0; b < groups.size(); ++b) {
+                const int64_t N = groups[b].ent->N;
+                int64_t col = 0;
+                for (auto &r : groups[b].reqs) {
+                    r->stats = st;
+                    if (s == DBIE_OK || s == DBIE_NOT_CONVERGED)
+                        r->sigma.assign(S[b].begin() + (size_t)col * N, S[b].begin() + (size_t)(col + r->K) * N);
+                    col += r->K;
+                    if (r->cancel) finish(r, St::Cancelled, DBIE_OK, "cancelled");
+                    else if (s == DBIE_OK) finish(r, St::Done, DBIE_OK, "");
+                    else finish(r, St::Failed, s, msg);
+                }
+                auto &ent = groups[b].ent;
+                ent->busy = false;
+                int64_t bytes = 0;
+                if (ent->op && dbie_op_device_bytes(ent->op, &bytes) == DBIE_OK) ent->bytes = (size_t)bytes;
+                if (!o.cache_ops && ent->users == 0) cache.erase(ent->key);
+            }
             note_peak();
             cv.notify_all();
         }
@@ -481,7 +609,7 @@ dbie_status dbie_server_submit_box(dbie_server *s, double Lx, double Ly, double 
         dbie_geom_sizes(g, &P, &N, &nq);
         dbie_geom_destroy(g);
         r->pending_op = estimate_op_bytes(N, n_quad);
-        r->pending_ws = estimate_ws_bytes(N, std::min(K, s->o.max_batch_rhs));
+        r->pending_ws = estimate_ws_bytes(N, K, r->sopts);  // a request is solved with all its columns
         r->t_arrive = Clock::now();
         std::lock_guard<std::mutex> lk(s->mu);
         r->id = s->next_id++;
@@ -593,6 +721,7 @@ dbie_status dbie_server_stats(dbie_server *s, dbie_server_summary *out) {
         out->cache_hits = s->hits;
         out->batches = s->batches;
         out->fused_requests = s->fused;
+        out->multi_geometry_batches = s->multi;
         out->cached_operators = (int64_t)s->cache.size();
         out->bytes_in_use = (int64_t)s->in_use();
         out->peak_bytes = (int64_t)s->peak;

@@ -46,6 +46,7 @@ from .api import (  # noqa: F401
     serving_requests,
     single_dielectric_box3d,
     solve_dielectric_box3d_block,
+    solve_fused,
     solve_gmres,
     solve_lu,
 )

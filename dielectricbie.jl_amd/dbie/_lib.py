@@ -76,7 +76,7 @@ class RequestInfo(C.Structure):
 class ServerSummary(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("completed", "failed", "operator_builds", "cache_hits", "batches",
                                          "fused_requests", "cached_operators", "bytes_in_use", "peak_bytes",
-                                         "budget_bytes")]
+                                         "budget_bytes", "multi_geometry_batches")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -130,6 +130,9 @@ SIGNATURES = {
     "dbie_pottrg": [_vp, C.c_int64, _dp, C.c_int, _dp, C.c_int64, _dp, C.c_int64],
     "dbie_op_stream": [_vp],
     "dbie_op_synchronize": [_vp],
+    "dbie_solve_fused": [C.c_int, C.POINTER(_vp), _ip, C.POINTER(_dp), _i64p, C.POINTER(_dp), _i64p,
+                         C.POINTER(SolveOpts), C.POINTER(SolveStats)],
+    "dbie_op_residual_history": [_vp, C.c_int, C.POINTER(C.c_int), _dp],
     "dbie_autotune": [C.c_int, C.c_int],
     "dbie_autotune_choice": [_vp, C.c_int, C.c_int, _ip, _ip],
     "dbie_autotune_set": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],

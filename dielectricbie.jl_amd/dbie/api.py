@@ -453,6 +453,7 @@ class BatchedDielectricOperator:
         status = L.lib().dbie_solve_block(self._h, K, L.dptr(Ff), self.n, L.dptr(S), self.n, C.byref(o),
                                           C.byref(st))
         stats = st.as_dict()
+        stats["residual_history"] = self.residual_history()
         if status == L.NOT_CONVERGED and not raise_on_fail:
             pass
         else:
@@ -489,7 +490,17 @@ class BatchedDielectricOperator:
                                                  C.byref(o), C.byref(st))
         if not (status == L.NOT_CONVERGED and not raise_on_fail):
             L.check(status)
-        return st.as_dict()
+        return dict(st.as_dict(), residual_history=self.residual_history())
+
+    def residual_history(self):
+        """[(kind, max_k ||r_k|| / ||F_k||)] of the last solve: kind "krylov" (an iteration's residual
+        estimate), "fp64" (a true FP64 residual), "updated" (low-precision update / inner estimate)."""
+        n = C.c_int()
+        L.check(L.lib().dbie_op_residual_history(self._h, 0, C.byref(n), None))
+        buf = np.zeros(2 * max(n.value, 1))
+        L.check(L.lib().dbie_op_residual_history(self._h, n.value, C.byref(n), L.dptr(buf)))
+        kinds = ("krylov", "fp64", "updated")
+        return [(kinds[int(buf[2 * i])], float(buf[2 * i + 1])) for i in range(n.value)]
 
     def pottrg(self, targets, Sigma):
         """Induced potential u[σ](t) = Σ_i w_i σ_i /(4π|t - x_i|) at targets (3 x M or M x 3)."""
@@ -505,6 +516,35 @@ class BatchedDielectricOperator:
         U = np.zeros((M, K), order="F")
         L.check(L.lib().dbie_pottrg(self._h, M, L.dptr(t.ravel()), K, L.dptr(Sf), self.n, L.dptr(U), M))
         return U[:, 0] if vec else U
+
+
+def solve_fused(ops, Fs, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precision="fp64", inner_rtol=1e-5,
+                max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns", recycle=True):
+    """Solve the block systems A_b Σ_b = F_b of several operators (different geometries) as one Krylov
+    process (dbie_solve_fused): every iteration applies all operators at once on the GPU.  Returns
+    ([Σ_b], stats)."""
+    B = len(ops)
+    if B != len(Fs) or B < 1:
+        raise ValueError("one right-hand side block per operator")
+    Ff = [L.fortran(F, op.n) for op, F in zip(ops, Fs)]
+    Ss = [np.zeros(F.shape, order="F") for F in Ff]
+    ha = (C.c_void_p * B)(*[op._h.value for op in ops])
+    Ka = np.array([F.shape[1] for F in Ff], np.int32)
+    Fa = (L._dp * B)(*[L.dptr(F) for F in Ff])
+    Sa = (L._dp * B)(*[L.dptr(S) for S in Ss])
+    ld = np.array([op.n for op in ops], np.int64)
+    o = L.SolveOpts()
+    L.lib().dbie_solve_opts_default(C.byref(o))
+    o.rtol, o.atol, o.itmax, o.restart = float(rtol), float(atol), int(itmax), int(restart)
+    o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
+    o.krylov = _krylov_code(krylov)
+    o.recycle = int(bool(recycle))
+    st = L.SolveStats()
+    status = L.lib().dbie_solve_fused(B, ha, L.iptr(Ka), Fa, L.i64ptr(ld), Sa, L.i64ptr(ld), C.byref(o), C.byref(st))
+    if not (status == L.NOT_CONVERGED and not raise_on_fail):
+        L.check(status)
+    return Ss, st.as_dict()
 
 
 def _krylov_code(krylov) -> int:

@@ -247,6 +247,29 @@ dbie_status dbie_screen_interface(dbie_op *op, int64_t S, const double *xyz, con
 dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, double *Sigma, int64_t lds,
                              const dbie_solve_opts *opts, dbie_solve_stats *stats);
 
+/* Residual history of the last solve on this operator (the Krylov stats the reference returns,
+   src/solver/multi_rhs.jl:285): *n entries of (kind, max_k ||r_k|| / ||F_k||) into kind_value (2 doubles
+   each, at most cap entries written); kind 0 = a Krylov iteration's residual estimate (outer GMRES, or the
+   low-precision inner GMRES of a mixed solve), 1 = a true FP64 residual, 2 = a residual updated with the
+   low-precision operator or taken from the inner solve's estimate. */
+dbie_status dbie_op_residual_history(const dbie_op *op, int cap, int *n, double *kind_value);
+
+/* No counterpart in the reference (its callers run solve_dielectric_box3d_block once per interface,
+   src/solver/multi_rhs.jl:270-286, src/solver/lattice_batch.jl:285-304): the block systems of B operators
+   of different geometries solved as ONE Krylov process.  Member b brings K[b] columns F[b] (N_b x K[b],
+   leading dimension ldf[b]) and receives Sigma[b]; the columns are held in N_max x sum(K) blocks (rows
+   past a member's N are zero), and every iteration applies all members at once, each on its own stream
+   with its own tuned launch shape and HIP graphs, so that small systems fill the GPU together and share
+   one host synchronisation per iteration.  Each operator may appear once (merge its columns).  The
+   right preconditioner is a scalar per member for single-contrast interfaces and is left out (GMRES is
+   invariant to it); multi-contrast members are solved unpreconditioned.  opts and stats as for
+   dbie_solve_block (one stats record for the whole solve), except that opts->krylov must be 0: block
+   GMRES orthogonalises the columns against each other, which would mix columns of different operators
+   (DBIE_BAD_ARG otherwise). */
+dbie_status dbie_solve_fused(int B, dbie_op *const *ops, const int *K, const double *const *F, const int64_t *ldf,
+                             double *const *Sigma, const int64_t *lds, const dbie_solve_opts *opts,
+                             dbie_solve_stats *stats);
+
 /* ---- induced potential off the surface --------------------------------- */
 /* laplace3d_pottrg / laplace3d_pottrg_fmm3d (src/kernel/laplace3d.jl:69-82,169-191):
    U[t,k] = sum_i w_i Sigma[i,k] / (4 pi |trg_t - x_i|) for M targets (3 x M). */
@@ -305,9 +328,10 @@ dbie_status dbie_gauss_hermite(int n, double *x, double *w);
 /* No counterpart in the reference, whose callers loop over solve_dielectric_box3d_block
    (src/solver/multi_rhs.jl:270-286).  Requests (a box and its point charges) are admitted by
    device-memory footprint, set up on host threads (mesh, near field, RHS) while other requests
-   run, and solved by a few concurrent slots; ready requests on one cached operator with the same
-   options are fused into one block solve.  Results are the same alone and in a mix, within the
-   solve tolerance. */
+   run, and solved by a few concurrent slots; ready requests with the same options are fused into one
+   solve: those on one cached operator (keyed by geometry; the permittivities set its diagonal) as one
+   column block, those on other operators as members of a fused Krylov process (dbie_solve_fused,
+   per-column Krylov only).  Results are the same alone and in a mix, within the solve tolerance. */
 typedef struct dbie_server dbie_server;
 typedef struct {
     double hbm_budget;     /* device bytes for operators + workspaces (0 = 80% of free memory) */
@@ -333,6 +357,7 @@ typedef struct {
 typedef struct {
     int64_t completed, failed, operator_builds, cache_hits, batches, fused_requests, cached_operators;
     int64_t bytes_in_use, peak_bytes, budget_bytes;
+    int64_t multi_geometry_batches;  /* batches that fused requests of several operators (dbie_solve_fused) */
 } dbie_server_summary;
 
 void dbie_server_opts_default(dbie_server_opts *o);

@@ -343,3 +343,25 @@ def test_operator_from_caller_nodes(dbie, O, cfg1):
     nrm = nd[3:6].T
     DT = np.einsum("ik,ijk->ij", nrm, d) / (4 * np.pi * r ** 3) * nd[6][None, :]
     assert rel(op3.apply_kernel(X, "DT"), DT @ X) < 1e-12
+
+
+@pytest.mark.parametrize("precision,krylov", [("fp64", "columns"), ("fp64", "block"), ("fp16x3", "columns"),
+                                              ("fp16x3", "block")])
+def test_residual_history(dbie, cfg1, precision, krylov):
+    """Per-solve residual history (the Krylov stats of src/solver/multi_rhs.jl:285): one Krylov entry per
+    (inner) iteration, one FP64 entry per true residual (nrefine of them for a mixed solve, the final
+    check of an FP64 solve), ending at the reported residual below the tolerance."""
+    g, op, ref_if, ref = cfg1
+    say(f"residual history ({precision}, {krylov})")
+    rng = np.random.default_rng(17)
+    F = dbie.rhs_point_charges(g, rng.uniform(-0.25, 0.25, (20, 3)), rng.uniform(-1, 1, 20), 4.0)
+    S, st = op.solve(F, rtol=1e-10, precision=precision, inner_rtol=1e-5, krylov=krylov)
+    h = st["residual_history"]
+    kinds = [k for k, _ in h]
+    assert kinds.count("krylov") == st["niter"]
+    assert kinds.count("fp64") == (st["nrefine"] if precision != "fp64" else 1)
+    assert h[-1][0] == "fp64" and abs(h[-1][1] - st["max_rel_residual"]) <= 1e-3 * st["max_rel_residual"]
+    assert h[-1][1] <= 1e-10
+    if precision == "fp64":  # GMRES residuals never increase
+        kr = [v for k, v in h if k == "krylov"]
+        assert all(b <= a * (1 + 1e-12) for a, b in zip(kr, kr[1:]))
