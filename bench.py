@@ -194,11 +194,11 @@ def single_workload(a, dbie, log):
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}), flush=True)
 
 
-def serve_stream(dbie, reqs, precision, inner_rtol, slots, log):
+def serve_stream(dbie, reqs, precision, inner_rtol, slots, log, setup_workers=2):
     """Submit the requests at their arrival times to a fresh request server; wait for all.
     Returns (infos, results, wall seconds, server stats)."""
     infos, results = [], []
-    with dbie.Server(solve_slots=slots) as srv:
+    with dbie.Server(solve_slots=slots, setup_workers=setup_workers) as srv:
         t0 = time.perf_counter()
         ids = []
         for r in reqs:
@@ -231,13 +231,13 @@ def serve_both(a, dbie, log, requests, rate):
     """The serving stream in mixed precision and in FP64 (same requests, fresh servers); the error of
     each served mixed result against the served FP64 result of the same request."""
     reqs = dbie.serving_requests(requests, seed=a.seed, rate=rate)
-    im, rm, wm, sm = serve_stream(dbie, reqs, a.precision, a.inner_rtol, a.slots, log)
-    i64, r64, w64, s64 = serve_stream(dbie, reqs, "fp64", a.inner_rtol, a.slots, log)
+    im, rm, wm, sm = serve_stream(dbie, reqs, a.precision, a.inner_rtol, a.slots, log, a.setup_workers)
+    i64, r64, w64, s64 = serve_stream(dbie, reqs, "fp64", a.inner_rtol, a.slots, log, a.setup_workers)
     errs = [float(np.max(np.linalg.norm(S - R, axis=0) / np.linalg.norm(R, axis=0)))
             for S, R in zip(rm, r64) if S is not None and R is not None]
     mixed, fp64 = serve_summary(im, wm), serve_summary(i64, w64)
     ok = [i for i in im if i["status"] == 0]
-    return {"requests": len(reqs), "arrival_rate": rate, "slots": a.slots,
+    return {"requests": len(reqs), "arrival_rate": rate, "slots": a.slots, "setup_workers": a.setup_workers,
             "N_range": [int(min(i["N"] for i in ok)), int(max(i["N"] for i in ok))] if ok else None,
             "mixed": dict(mixed, server=sm), "fp64": dict(fp64, server=s64),
             "speedup_vs_fp64": mixed["value"] / fp64["value"] if fp64["value"] > 0 else None,
@@ -285,6 +285,7 @@ def main():
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=2, help="serve: concurrent solve slots")
+    ap.add_argument("--setup-workers", type=int, default=2, help="serve: concurrent request setups")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; the path shards by RHS "

@@ -35,6 +35,66 @@ void hip_check(hipError_t e, const char *what) {
 }
 
 namespace {
+// size class of a pooled block: 256-byte granules below 1 MiB, then eight classes per octave
+size_t pool_class(size_t b) {
+    if (b <= ((size_t)1 << 20)) return (b + 255) & ~(size_t)255;
+    int e = 63 - __builtin_clzll((unsigned long long)b);  // 2^e <= b < 2^(e+1)
+    const size_t step = (size_t)1 << (e - 3);
+    return (b + step - 1) / step * step;
+}
+constexpr size_t kPoolMaxBlock = (size_t)8 << 30;
+constexpr size_t kPoolMaxCached = (size_t)16 << 30;
+struct ThreadPool {
+    std::multimap<size_t, void *> free_blocks;
+    size_t cached = 0;
+    void trim() {
+        for (auto &kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+        cached = 0;
+    }
+    ~ThreadPool() { trim(); }
+};
+ThreadPool &thread_pool() {
+    static thread_local ThreadPool tp;
+    return tp;
+}
+}  // namespace
+
+void *pool_take(size_t bytes, size_t *got) {
+    const size_t c = pool_class(std::max<size_t>(bytes, 1));
+    ThreadPool &tp = thread_pool();
+    auto it = tp.free_blocks.find(c);
+    if (it != tp.free_blocks.end()) {
+        void *p = it->second;
+        tp.free_blocks.erase(it);
+        tp.cached -= c;
+        *got = c;
+        return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, c) != hipSuccess) {
+        (void)hipGetLastError();
+        tp.trim();  // give the cached blocks back and try once more
+        if (hipMalloc(&p, c) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    *got = c;
+    return p;
+}
+
+void pool_give(void *p, size_t bytes) {
+    ThreadPool &tp = thread_pool();
+    if (bytes >= kPoolMaxBlock || tp.cached + bytes > kPoolMaxCached) {
+        (void)hipFree(p);
+        return;
+    }
+    tp.free_blocks.emplace(bytes, p);
+    tp.cached += bytes;
+}
+
+namespace {
 std::once_flag g_init_flag;
 dbie_status g_init_status = DBIE_OK;
 std::string g_init_msg;
@@ -88,6 +148,12 @@ struct GmresWork {
     DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
     DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
     DBuf PV, PT;          // block GMRES: Householder vectors (2K x K) and T factors (K x K) per Hessenberg step
+    // take the workspaces from (and return them to) the calling thread's block cache
+    void set_pooled() {
+        for (DBuf *d : {&V, &W, &Z, &U, &partial, &hbuf, &nbuf, &V1, &abuf, &cbuf, &om, &n1, &H, &G, &tau, &Y, &gpart,
+                        &R, &info, &PV, &PT})
+            d->pooled = true;
+    }
 };
 
 struct dbie_op {
@@ -186,7 +252,7 @@ PairList device_pair_list(const Geometry &g, const DevGeom &dg, int max_order, d
     const int64_t P = g.P;
     if (P > INT32_MAX) fail(DBIE_BAD_ARG, "too many panels");
     const PairPrep pp = pair_prep(g, max_order, atol, correct_edges);
-    DBuf pd, co, ie, cnt, dptr, dtrg, drho, dtouch;
+    DBuf pd(true), co(true), ie(true), cnt(true), dptr(true), dtrg(true), drho(true), dtouch(true);
     upload(pd, pp.pdata.data(), pp.pdata.size(), st);
     upload(co, g.corners.data(), g.corners.size(), st);
     upload(ie, g.is_edge.data(), g.is_edge.size(), st);
@@ -327,7 +393,7 @@ bool build_correction_blocks(dbie_op *op, const dbie_near_opts &o, int mode, Blo
     v64->ensure(sizeof(double) * std::max<size_t>(nv, 1));
     if (v32) v32->ensure(sizeof(float) * std::max<size_t>(nv, 1));
     if (vup32) vup32->ensure(sizeof(float) * std::max<size_t>(nv, 1));
-    DBuf us, ut, ul, co, cn, ca, as, at, al, ak;
+    DBuf us(true), ut(true), ul(true), co(true), cn(true), ca(true), as(true), at(true), al(true), ak(true);
     upload(us, cp.up_src.data(), cp.up_src.size(), st);
     upload(ut, cp.up_trg.data(), cp.up_trg.size(), st);
     upload(ul, cp.up_slot.data(), cp.up_slot.size(), st);
@@ -345,7 +411,7 @@ bool build_correction_blocks(dbie_op *op, const dbie_near_opts &o, int mode, Blo
     dgb.nx = op->nx.as<double>(); dgb.ny = op->ny.as<double>(); dgb.nz = op->nz.as<double>();
     dgb.w = op->w.as<double>();
     const int n_adp = (int)cp.adp_src.size();
-    DBuf ar, am, ru, ah;
+    DBuf ar(true), am(true), ru(true), ah(true);
     std::vector<int> hits;
     if (dev_quad) {
         // touching pairs: quadtrees on the device, K_up blocks into ak
@@ -894,7 +960,7 @@ ScreenBoxes upload_boxes(const HostBoxes &hb, DBuf &dbuf, hipStream_t st) {
 void screen_host(hipStream_t st, int64_t S, const double *xyz, const double *rho, const HostBoxes &hb,
                  double *rho_out) {
     if (S == 0) return;
-    DBuf dpos, drho, dout, dbox;
+    DBuf dpos(true), drho(true), dout(true), dbox(true);
     upload(dpos, xyz, 3 * (size_t)S, st);
     upload(drho, rho, (size_t)S, st);
     dout.ensure(sizeof(double) * S);
@@ -918,7 +984,7 @@ void rhs_volume_host(dbie_op *op, int K, const int64_t *col_ptr, const double *x
         if (col_ptr[k + 1] < col_ptr[k]) fail(DBIE_BAD_ARG, "col_ptr must be non-decreasing");
     if (S > 0 && (!xyz || !w || !rho)) fail(DBIE_BAD_ARG, "source arrays required");
     hipStream_t st = op->stream;
-    DBuf dcp, dpos, dw, drho, dwr, dbox;
+    DBuf dcp(true), dpos(true), dw(true), drho(true), dwr(true), dbox(true);
     upload(dcp, col_ptr, (size_t)K + 1, st);
     upload(dpos, xyz, 3 * (size_t)S, st);
     upload(dw, w, (size_t)S, st);
@@ -1011,7 +1077,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         blocks = std::min(blocks, m + 1);
         if (ws.V.bytes >= sizeof(double) * NK * blocks) return ws.V.as<double>();
         const int cap = std::min(m + 1, std::max(blocks, std::max(16, 2 * (int)(ws.V.bytes / (sizeof(double) * NK)))));
-        DBuf nv;
+        DBuf nv(ws.V.pooled);
         nv.ensure(sizeof(double) * NK * cap);
         if (ws.V.p) {
             DBIE_HIP(hipMemcpyAsync(nv.p, ws.V.p, ws.V.bytes, hipMemcpyDeviceToDevice, st));
@@ -1529,7 +1595,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     if (o.krylov != 0 && o.krylov != 1) fail(DBIE_BAD_ARG, "krylov must be 0 (per column) or 1 (block)");
     hipStream_t st = op->stream;
     const size_t NK = (size_t)N * K;
-    DBuf dR, dD, dT, dinv, partial, hb;
+    DBuf dR(true), dD(true), dT(true), dinv(true), partial(true), hb(true);
     const double *dinvp = nullptr;
     if (o.precondition) {
         std::vector<double> inv(N);
@@ -1702,6 +1768,7 @@ dbie_status solve_fused_host(int B, dbie_op *const *ops, const int *Ks, const do
         Ktot += Ks[b];
     }
     std::unique_ptr<dbie_op> f(new dbie_op);
+    f->gw.set_pooled();  // the next fused solve of this thread reuses the Krylov workspaces
     f->g.N = Nmax;
     f->graphs = false;  // the members replay their own graphs
     f->diag_h.assign(Nmax, 1.0);  // right preconditioner: each member is a single contrast (a scalar
@@ -1717,7 +1784,7 @@ dbie_status solve_fused_host(int B, dbie_op *const *ops, const int *Ks, const do
     }
     hipStream_t st = f->stream;
     const size_t NK = (size_t)Nmax * Ktot;
-    DBuf dF, dX;
+    DBuf dF(true), dX(true);
     dF.ensure(sizeof(double) * NK);
     dX.ensure(sizeof(double) * NK);
     DBIE_HIP(hipMemsetAsync(dF.p, 0, sizeof(double) * NK, st));
@@ -2359,7 +2426,7 @@ static void apply_kernel_host(dbie_op *op, dbie_kernel kernel, bool corrected, i
     if (corrected && kernel == DBIE_OP_S) fail(DBIE_BAD_ARG, "near corrections exist for DBIE_OP_DT and DBIE_OP_D");
     hipStream_t st = op->stream;
     DevBSR C = op->bsr;
-    DBuf zrow;
+    DBuf zrow(true);
     if (corrected) {
         if (!op->nopts.build_corrections) fail(DBIE_BAD_ARG, "the operator was created without near corrections");
         if (kernel == DBIE_OP_D) {
@@ -2456,7 +2523,7 @@ dbie_status dbie_rhs_points(dbie_op *op, int K, const double *src_xyz, const dou
         if (ldf < N) fail(DBIE_DIM_MISMATCH, "ldf < N");
         if (eps_src == 0.0) fail(DBIE_BAD_ARG, "eps_src must be nonzero");
         hipStream_t st = op->stream;
-        DBuf ds, dq;
+        DBuf ds(true), dq(true);
         upload(ds, src_xyz, 3 * (size_t)K, st);
         upload(dq, q, K, st);
         op->yout.ensure(sizeof(double) * N * K);
@@ -2545,7 +2612,7 @@ dbie_status dbie_solve_block(dbie_op *op, int K, const double *F, int64_t ldf, d
         if (K < 1 || !F || !Sigma) fail(DBIE_BAD_ARG, "K >= 1 and non-NULL buffers required");
         if (ldf < N || lds < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
         hipStream_t st = op->stream;
-        DBuf dF, dX;
+        DBuf dF(true), dX(true);
         dF.ensure(sizeof(double) * N * K);
         dX.ensure(sizeof(double) * N * K);
         DBIE_HIP(hipMemcpy2DAsync(dF.p, sizeof(double) * N, F, sizeof(double) * ldf, sizeof(double) * N, K,
@@ -2579,7 +2646,7 @@ dbie_status dbie_solve_block_device(dbie_op *op, int K, const double *dF, int64_
         if (K < 1 || !dF || !dSigma) fail(DBIE_BAD_ARG, "K >= 1 and non-NULL buffers required");
         if (ldf < N || lds < N) fail(DBIE_DIM_MISMATCH, "leading dimension smaller than N");
         hipStream_t st = op->stream;
-        DBuf cF, cX;
+        DBuf cF(true), cX(true);
         const double *F = dF;
         double *X = dSigma;
         if (ldf != N) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <exception>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -14,22 +15,46 @@ namespace dbie {
 // Checks once per process that a gfx950 device is visible; throws DBIE_NO_DEVICE otherwise.
 void ensure_device();
 
-// Device buffer with RAII.
+// Per-thread cache of device blocks for temporaries.  hipFree synchronises the whole device, so a
+// temporary freed by one host thread (an operator build, a solve) would stall the solves other threads
+// are running; a pooled buffer instead returns its block to its thread's cache, and the thread's next
+// temporary of the same size class takes it.  A block is reused only by the thread that released it,
+// and every function with pooled temporaries synchronises its stream before returning, so a reused
+// block is no longer in use on the device.  Blocks of 8 GiB and more, and blocks beyond 16 GiB cached
+// per thread, are freed.  pool_take returns nullptr when hipMalloc fails (after freeing the cache).
+void *pool_take(size_t bytes, size_t *got);
+void pool_give(void *p, size_t bytes);
+
+// Device buffer with RAII.  pooled = true: blocks come from and return to the thread's cache.
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool pooled = false;
     DBuf() = default;
+    explicit DBuf(bool pool) : pooled(pool) {}
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            // during stack unwinding the buffer's stream may still have work queued: free it (hipFree
+            // waits for the device) rather than hand it to a later temporary
+            if (pooled && std::uncaught_exceptions() == 0) pool_give(p, bytes);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
     }
     void ensure(size_t b) {
         if (b <= bytes) return;
         release();
+        if (pooled) {
+            size_t got = 0;
+            p = pool_take(b, &got);
+            if (!p) fail(DBIE_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed");
+            bytes = got;
+            return;
+        }
         if (hipMalloc(&p, b) != hipSuccess) {
             (void)hipGetLastError();
             p = nullptr;
