@@ -12,6 +12,7 @@
 #include <rocblas/rocblas.h>
 
 #include <array>
+#include <chrono>
 #include <cstring>
 #include <memory>
 
@@ -33,6 +34,7 @@ struct dbie_tmap {
     bool par_blocks = false;                   // the prolongation is block-structured by coarse panel
     DBuf c_ptr, c_col, c_val;                  // corrections (M rows)
     DBuf xr, xin, uout;                        // workspaces
+    bool resident = false;                     // everything above is on the device
     DevGeom dg;
     ~dbie_tmap() {
         if (blas) (void)rocblas_destroy_handle(blas);
@@ -52,24 +54,41 @@ void blas_check(rocblas_status s, const char *what) {
 
 // Uploads the refined nodes, targets and sparse matrices on first device use.
 void make_resident(dbie_tmap *tm) {
-    if (tm->stream) return;
+    if (tm->resident) return;
     ensure_device();
     const TargetMap &h = *tm->h;
-    DBIE_HIP(hipStreamCreateWithFlags(&tm->stream, hipStreamNonBlocking));
+    if (!tm->stream) DBIE_HIP(hipStreamCreateWithFlags(&tm->stream, hipStreamNonBlocking));
     hipStream_t st = tm->stream;
     const Geometry &R = h.ref;
-    upload(tm->x, R.x.data(), R.N, st);
-    upload(tm->y, R.y.data(), R.N, st);
-    upload(tm->z, R.z.data(), R.N, st);
-    upload(tm->w, R.w.data(), R.N, st);
+    if (!h.prolong_device) {  // the device builder left the refined nodes and the prolongation there
+        upload(tm->x, R.x.data(), R.N, st);
+        upload(tm->y, R.y.data(), R.N, st);
+        upload(tm->z, R.z.data(), R.N, st);
+        upload(tm->w, R.w.data(), R.N, st);
+        upload(tm->p_ptr, h.p_ptr.data(), h.p_ptr.size(), st);
+        upload(tm->p_col, h.p_col.data(), h.p_col.size(), st);
+        upload(tm->p_val, h.p_val.data(), h.p_val.size(), st);
+    }
     upload(tm->trg, h.trg.data(), h.trg.size(), st);
-    upload(tm->p_ptr, h.p_ptr.data(), h.p_ptr.size(), st);
-    upload(tm->p_col, h.p_col.data(), h.p_col.size(), st);
-    upload(tm->p_val, h.p_val.data(), h.p_val.size(), st);
     upload(tm->c_ptr, h.c_ptr.data(), h.c_ptr.size(), st);
     upload(tm->c_col, h.c_col.data(), h.c_col.size(), st);
     upload(tm->c_val, h.c_val.data(), h.c_val.size(), st);
-    {  // transpose of the prolongation: rows = coarse nodes
+    if (h.prolong_device) {
+        // by construction the refined nodes of coarse panel c are consecutive (its subtree is walked
+        // before the next panel's) and interpolate from c's nodes only
+        const int npp = h.ref.nq * h.ref.nq;
+        const int64_t P = tm->N / std::max(1, npp);
+        std::vector<int64_t> first(P, -1);
+        std::vector<int> count(P, 0);
+        for (size_t qp = 0; qp < h.parent.size(); ++qp) {
+            const int c = h.parent[qp];
+            if (first[c] < 0) first[c] = (int64_t)qp * npp;
+            count[c] += npp;
+        }
+        upload(tm->par_first, first.data(), first.size(), st);
+        upload(tm->par_count, count.data(), count.size(), st);
+        tm->par_blocks = true;
+    } else {  // transpose of the prolongation: rows = coarse nodes
         const int64_t N = tm->N, Nr = R.N;
         std::vector<int64_t> tp(N + 1, 0);
         for (int c : h.p_col) ++tp[c + 1];
@@ -87,7 +106,7 @@ void make_resident(dbie_tmap *tm) {
         upload(tm->pt_col, tc.data(), tc.size(), st);
         upload(tm->pt_val, tv.data(), tv.size(), st);
     }
-    {  // refined nodes per coarse panel: consecutive (the refinement finishes a panel's subtree
+    if (!h.prolong_device) {  // refined nodes per coarse panel: consecutive (the refinement finishes a panel's subtree
        // before the next panel), interpolating from that panel's nodes only
         const int64_t P = tm->N / std::max(1, h.ref.nq * h.ref.nq);
         const int npp = h.ref.nq * h.ref.nq;
@@ -122,6 +141,7 @@ void make_resident(dbie_tmap *tm) {
     d.z = tm->z.as<double>();
     d.w = tm->w.as<double>();
     DBIE_HIP(hipStreamSynchronize(st));
+    tm->resident = true;
 }
 
 // U (M x K) = base(P Sigma) + C (P Sigma), everything on the device.
@@ -265,15 +285,130 @@ void device_cubatures(const TargetCubatureJob &job, const Geometry &g, double *I
 }
 }  // namespace
 
+namespace {
+// The refinement around the targets and the prolongation on the device (targets.hip): the leaves
+// counted per coarse panel, placed in the host's order (panels from the last to the first, each
+// subtree depth first), their nodes computed, the prolongation filled; the refined nodes and the
+// prolongation stay in the map's device buffers.  Declines for n_quad > 5 (the reaction fold's
+// block kernel takes at most 32 nodes per panel) and when a subtree is deeper than the device stack.
+bool device_refine(dbie_tmap *tm, const RefineInput &in, RefineOutput &out) {
+    using Clk = std::chrono::steady_clock;
+    const auto t0 = Clk::now();
+    const Geometry &g = *in.g;
+    const int nq = g.nq, npp = nq * nq;
+    if (npp > 32 || g.P > INT32_MAX) return false;
+    ensure_device();
+    if (!tm->stream) DBIE_HIP(hipStreamCreateWithFlags(&tm->stream, hipStreamNonBlocking));
+    hipStream_t st = tm->stream;
+    const int P = (int)g.P;
+    DBuf dco(true), dtrg(true), dstart(true), didx(true), dcnt(true), doff(true), drc(true), dpar(true), dspl(true),
+        drule(true), dgl(true);
+    upload(dco, g.corners.data(), g.corners.size(), st);
+    upload(dtrg, in.trg, 3 * (size_t)in.M, st);
+    upload(dstart, in.start->data(), in.start->size(), st);
+    upload(didx, in.idx->data(), in.idx->size(), st);
+    RefineGrid G;
+    G.t = dtrg.as<double>();
+    for (int k = 0; k < 3; ++k) {
+        G.lo[k] = in.lo[k];
+        G.dim[k] = in.dim[k];
+    }
+    G.h = in.h;
+    G.start = dstart.as<int64_t>();
+    G.idx = didx.as<int>();
+    dcnt.ensure(sizeof(int64_t) * std::max(P, 1));
+    refine_count(st, P, dco.as<double>(), G, in.M, in.limit, in.range_factor, nq, dcnt.as<int64_t>());
+    DBIE_HIP(hipGetLastError());
+    std::vector<int64_t> cnt(P), off(P);
+    if (P > 0) DBIE_HIP(hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int64_t) * P, hipMemcpyDeviceToHost, st));
+    DBIE_HIP(hipStreamSynchronize(st));
+    int64_t Pr = 0;
+    for (int p = P - 1; p >= 0; --p) {  // the host pops the coarse panels from the last one
+        if (cnt[p] < 0) return false;
+        off[p] = Pr;
+        Pr += cnt[p];
+    }
+    if (Pr * npp > INT32_MAX) return false;
+    upload(doff, off.data(), off.size(), st);
+    drc.ensure(sizeof(double) * 12 * std::max<int64_t>(Pr, 1));
+    dpar.ensure(sizeof(int) * std::max<int64_t>(Pr, 1));
+    dspl.ensure(sizeof(int) * std::max<int64_t>(Pr, 1));
+    refine_emit(st, P, dco.as<double>(), G, in.M, in.limit, in.range_factor, nq, doff.as<int64_t>(), drc.as<double>(),
+                dpar.as<int>(), dspl.as<int>());
+    DBIE_HIP(hipGetLastError());
+    // refined nodes with the rule make_from_panels gives a new interface
+    double gx[16], gw[16];
+    gauss_legendre(nq, gx, gw);
+    std::vector<double> gl(gx, gx + nq);
+    gl.insert(gl.end(), gw, gw + nq);
+    upload(dgl, gl.data(), gl.size(), st);
+    const int64_t Nr = Pr * npp;
+    tm->x.ensure(sizeof(double) * std::max<int64_t>(Nr, 1));
+    tm->y.ensure(sizeof(double) * std::max<int64_t>(Nr, 1));
+    tm->z.ensure(sizeof(double) * std::max<int64_t>(Nr, 1));
+    tm->w.ensure(sizeof(double) * std::max<int64_t>(Nr, 1));
+    refined_nodes(st, Pr, nq, drc.as<double>(), dgl.as<double>(), dgl.as<double>() + nq, tm->x.as<double>(),
+                  tm->y.as<double>(), tm->z.as<double>(), tm->w.as<double>());
+    DBIE_HIP(hipGetLastError());
+    out.parent.resize(Pr);
+    out.from_split.resize(Pr);
+    out.corners.resize((size_t)Pr * 12);
+    out.x.resize(Nr);
+    out.y.resize(Nr);
+    out.z.resize(Nr);
+    out.w.resize(Nr);
+    if (Pr > 0) {
+        DBIE_HIP(hipMemcpyAsync(out.parent.data(), dpar.p, sizeof(int) * Pr, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(out.from_split.data(), dspl.p, sizeof(int) * Pr, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(out.corners.data(), drc.p, sizeof(double) * 12 * Pr, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(out.x.data(), tm->x.p, sizeof(double) * Nr, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(out.y.data(), tm->y.p, sizeof(double) * Nr, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(out.z.data(), tm->z.p, sizeof(double) * Nr, hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(out.w.data(), tm->w.p, sizeof(double) * Nr, hipMemcpyDeviceToHost, st));
+    }
+    DBIE_HIP(hipStreamSynchronize(st));
+    const auto t1 = Clk::now();
+    out.ms_refine = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    // prolongation: 1 entry per node of an unsplit panel, n_quad^2 per node of a split one
+    out.p_ptr.assign(Nr + 1, 0);
+    int64_t e = 0;
+    for (int64_t q = 0; q < Pr; ++q) {
+        const int64_t per = out.from_split[q] ? npp : 1;
+        for (int k = 0; k < npp; ++k) {
+            out.p_ptr[q * npp + k] = e;
+            e += per;
+        }
+    }
+    out.p_ptr[Nr] = e;
+    upload(tm->p_ptr, out.p_ptr.data(), out.p_ptr.size(), st);
+    tm->p_col.ensure(sizeof(int) * std::max<int64_t>(e, 1));
+    tm->p_val.ensure(sizeof(double) * std::max<int64_t>(e, 1));
+    std::vector<double> rules(g.glx, g.glx + nq);
+    rules.insert(rules.end(), g.bary, g.bary + nq);
+    upload(drule, rules.data(), rules.size(), st);
+    prolongation(st, Pr, nq, dco.as<double>(), drule.as<double>(), dpar.as<int>(), dspl.as<int>(), tm->x.as<double>(),
+                 tm->y.as<double>(), tm->z.as<double>(), tm->p_ptr.as<int64_t>(), tm->p_col.as<int>(),
+                 tm->p_val.as<double>());
+    DBIE_HIP(hipGetLastError());
+    DBIE_HIP(hipStreamSynchronize(st));
+    out.ms_prolong = std::chrono::duration<double, std::milli>(Clk::now() - t1).count();
+    return true;
+}
+}  // namespace
+
 dbie_status dbie_tmap_create_ex(const dbie_geom *geom, int64_t M, const double *trg, double hcub_atol,
                                 double range_factor, int n_threads, int device_cubature, dbie_tmap **out) {
     return guard([&] {
         if (!geom || !out) fail(DBIE_BAD_ARG, "NULL geometry or output");
+        if (device_cubature < 0 || device_cubature > 2) fail(DBIE_BAD_ARG, "device must be 0, 1 or 2");
         *out = nullptr;
         std::unique_ptr<dbie_tmap> tm(new dbie_tmap);
         TargetCubatureFn dev;
-        if (device_cubature) dev = device_cubatures;
-        tm->h.reset(build_target_map(geom->g, M, trg, hcub_atol, range_factor, n_threads, dev));
+        if (device_cubature >= 1) dev = device_cubatures;
+        TargetRefineFn ref;
+        dbie_tmap *tp = tm.get();
+        if (device_cubature == 2) ref = [tp](const RefineInput &in, RefineOutput &o) { return device_refine(tp, in, o); };
+        tm->h.reset(build_target_map(geom->g, M, trg, hcub_atol, range_factor, n_threads, dev, ref));
         tm->N = geom->g.N;
         *out = tm.release();
     });
@@ -325,8 +460,14 @@ dbie_status dbie_tmap_matrices(const dbie_tmap *tm, int64_t *p_ptr, int *p_col, 
         check_tm(tm);
         const TargetMap &h = *tm->h;
         if (p_ptr) std::copy(h.p_ptr.begin(), h.p_ptr.end(), p_ptr);
-        if (p_col) std::copy(h.p_col.begin(), h.p_col.end(), p_col);
-        if (p_val) std::copy(h.p_val.begin(), h.p_val.end(), p_val);
+        if (h.prolong_device) {  // the entries live on the device
+            const int64_t nnz = h.p_ptr.empty() ? 0 : h.p_ptr.back();
+            if (p_col && nnz) DBIE_HIP(hipMemcpy(p_col, tm->p_col.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+            if (p_val && nnz) DBIE_HIP(hipMemcpy(p_val, tm->p_val.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+        } else {
+            if (p_col) std::copy(h.p_col.begin(), h.p_col.end(), p_col);
+            if (p_val) std::copy(h.p_val.begin(), h.p_val.end(), p_val);
+        }
         if (c_ptr) std::copy(h.c_ptr.begin(), h.c_ptr.end(), c_ptr);
         if (c_col) std::copy(h.c_col.begin(), h.c_col.end(), c_col);
         if (c_val) std::copy(h.c_val.begin(), h.c_val.end(), c_val);

@@ -174,6 +174,8 @@ struct TargetMap {
     int64_t n_pairs = 0;              // (panel, target) cubatures
     int64_t n_capped = 0;             // cubatures that hit the evaluation cap
     int64_t n_device = 0;             // cubatures integrated on the device
+    bool prolong_device = false;      // refinement and prolongation built on the device: p_col / p_val
+                                      // live there only (p_ptr is kept on the host too)
     double setup_ms[4] = {0, 0, 0, 0};  // refinement, prolongation, pairs + cubatures, assembly
 };
 // The near-target cubatures of a target map: pair k = (refined panel panel[k], target target[k]);
@@ -190,8 +192,32 @@ struct TargetCubatureJob {
 // Genz-Malik two-dimensional rule: l2, l3, l4, l5, w[5], wp[4] (the host cubature's constants)
 std::array<double, 13> genz_malik_rule();
 using TargetCubatureFn = std::function<void(const TargetCubatureJob &, const Geometry &g, double *I, int *status)>;
+// Optional device builder of the refinement and the prolongation (the first two phases of the target
+// map).  In: the coarse interface, the targets and the host's uniform grid over them (cells of size
+// h from lo, dim cells per axis, target lists start / idx), the panel size limit and range factor.
+// Out: the refined panels (corners, parent, from_split) in the host's order, their nodes, and the
+// prolongation's row pointers; the prolongation's entries stay on the device (the hook's owner keeps
+// them).  Returns false (nothing written) when it declines; the host then builds both phases.
+struct RefineInput {
+    const Geometry *g = nullptr;
+    const double *trg = nullptr;
+    int64_t M = 0;
+    double lo[3] = {0, 0, 0}, h = 1;
+    int64_t dim[3] = {1, 1, 1};
+    const std::vector<int64_t> *start = nullptr;
+    const std::vector<int> *idx = nullptr;
+    double limit = 0, range_factor = 0;
+};
+struct RefineOutput {
+    std::vector<double> corners, x, y, z, w;
+    std::vector<int> parent, from_split;
+    std::vector<int64_t> p_ptr;
+    double ms_refine = 0, ms_prolong = 0;
+};
+using TargetRefineFn = std::function<bool(const RefineInput &, RefineOutput &)>;
 TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, double atol, double range_factor,
-                            int n_threads, const TargetCubatureFn &device = nullptr);
+                            int n_threads, const TargetCubatureFn &device = nullptr,
+                            const TargetRefineFn &refine = nullptr);
 
 // ---- operator apply without allocation (noalloc.hip) ----------------------------
 // Scratch bytes dbie_apply_device needs when every workspace comes from one caller buffer: the

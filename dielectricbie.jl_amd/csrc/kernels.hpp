@@ -159,6 +159,27 @@ bool prolong_t(hipStream_t st, int P, int npp, const int64_t *first, const int *
                const int *p_col, const double *p_val, int G, const double *Qr, int64_t Nr, double *Qc, int64_t N);
 void reaction_q(hipStream_t st, const DevGeom &g, int G, const int64_t *grp_ptr, const double *trg, const double *q,
                 double *Qt);
+// Refinement of an interface around targets and the prolongation onto it (targets.hip), the host
+// builder's arithmetic restated.  RefineGrid: the host TargetGrid on the device (targets t, 3 x M;
+// cell lists start / idx).  refine_count: leaves per coarse panel (-1: subtree deeper than the
+// device stack); refine_emit: the leaves of panel p from offset[p] on.  rules: glx[nq], bary[nq] of
+// the coarse interface; p_ptr: rows of the prolongation (1 or nq^2 entries per refined node).
+struct RefineGrid {
+    const double *t = nullptr;
+    double lo[3] = {0, 0, 0}, h = 1;
+    int64_t dim[3] = {1, 1, 1};
+    const int64_t *start = nullptr;
+    const int *idx = nullptr;
+};
+void refine_count(hipStream_t st, int P, const double *corners, const RefineGrid &G, int64_t M, double limit,
+                  double range_factor, int nq, int64_t *count);
+void refine_emit(hipStream_t st, int P, const double *corners, const RefineGrid &G, int64_t M, double limit,
+                 double range_factor, int nq, const int64_t *offset, double *out_corners, int *parent, int *from_split);
+void refined_nodes(hipStream_t st, int64_t Pr, int nq, const double *corners, const double *glx, const double *glw,
+                   double *x, double *y, double *z, double *w);
+void prolongation(hipStream_t st, int64_t Pr, int nq, const double *coarse_corners, const double *rules,
+                  const int *parent, const int *from_split, const double *x, const double *y, const double *z,
+                  const int64_t *p_ptr, int *p_col, double *p_val);
 // Y[idx[e]] += val[e] for n entries with distinct idx.
 void scatter_add(hipStream_t st, int64_t n, const int64_t *idx, const double *val, double *Y);
 

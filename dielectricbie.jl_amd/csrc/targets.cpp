@@ -279,7 +279,7 @@ std::array<double, 13> genz_malik_rule() {
 }
 
 TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, double atol, double range_factor,
-                            int n_threads, const TargetCubatureFn &device) {
+                            int n_threads, const TargetCubatureFn &device, const TargetRefineFn &refine) {
     if (M < 0 || (M > 0 && !trg)) fail(DBIE_BAD_ARG, "targets: need M >= 0 and a 3 x M array");
     if (!(atol > 0)) fail(DBIE_BAD_ARG, "hcubature atol must be positive");
     if (!(range_factor > 0)) fail(DBIE_BAD_ARG, "range_factor must be positive");
@@ -298,6 +298,49 @@ TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, dou
     auto tp = Clk::now();
     const TargetGrid grid(tm->trg.data(), M, range_factor * limit / nq);
 
+    if (refine) {  // both phases on the device
+        RefineInput in;
+        in.g = &g;
+        in.trg = tm->trg.data();
+        in.M = M;
+        for (int k = 0; k < 3; ++k) {
+            in.lo[k] = grid.lo[k];
+            in.dim[k] = grid.dim[k];
+        }
+        in.h = grid.h;
+        in.start = &grid.start;
+        in.idx = &grid.idx;
+        in.limit = limit;
+        in.range_factor = range_factor;
+        RefineOutput out;
+        if (refine(in, out)) {
+            const int64_t Pr = (int64_t)out.parent.size();
+            std::vector<double> rn((size_t)Pr * 3), pts((size_t)Pr * npp * 3);
+            std::vector<int> re(Pr);
+            for (int64_t q = 0; q < Pr; ++q) {
+                for (int k = 0; k < 3; ++k) rn[3 * q + k] = g.normal[3 * (int64_t)out.parent[q] + k];
+                re[q] = g.is_edge[out.parent[q]];
+            }
+            for (int64_t n = 0; n < Pr * npp; ++n) {
+                pts[3 * n] = out.x[n];
+                pts[3 * n + 1] = out.y[n];
+                pts[3 * n + 2] = out.z[n];
+            }
+            double gx[16], gw[16];
+            gauss_legendre(nq, gx, gw);  // the rule make_from_panels gives the refined interface
+            std::unique_ptr<Geometry> gr(make_from_panel_nodes(Pr, nq, out.corners.data(), rn.data(), re.data(),
+                                                               pts.data(), out.w.data(), gx, gw, nullptr));
+            tm->ref = std::move(*gr);
+            tm->parent = std::move(out.parent);
+            tm->from_split = std::move(out.from_split);
+            tm->p_ptr = std::move(out.p_ptr);
+            tm->prolong_device = true;
+            tm->setup_ms[0] = out.ms_refine;
+            tm->setup_ms[1] = ms_since(tp) - out.ms_refine;
+            tp = Clk::now();
+        }
+    }
+    if (!tm->prolong_device) {
     // ---- refinement around the targets (stack, last in first out) ----
     struct Item {
         int parent;
@@ -408,6 +451,9 @@ TargetMap *build_target_map(const Geometry &g, int64_t M, const double *trg, dou
     }
     tm->setup_ms[1] = ms_since(tp);
     tp = Clk::now();
+    }  // host refinement and prolongation
+    const Geometry &R = tm->ref;
+    const int64_t Pr = (int64_t)tm->parent.size();
     // ---- near-target corrections on the refined interface ----
     // pairs (refined panel, target) in panel order, targets ascending per panel
     const int nt = n_threads > 0 ? n_threads : (int)std::max(1u, std::thread::hardware_concurrency());

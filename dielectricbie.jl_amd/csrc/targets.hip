@@ -372,7 +372,218 @@ __global__ __launch_bounds__(64) void target_cubature_kernel(int npairs, int nq,
         __syncthreads();
     }
 }
+// ---- refinement of the interface around the targets (_refine_interface_for_targets,
+// src/kernel/laplace3d_near.jl:99-150) and the prolongation onto it (_refined_interface_prolongation,
+// :152-225), restating the host builder (targets.cpp) operation for operation without contraction, so
+// that every decision (split or keep) and every value are those of the host.
+
+__device__ inline double len3_k(double a, double b, double c) {
+#pragma clang fp contract(off)
+    return sqrt(a * a + b * b + c * c);
+}
+__device__ inline double panel_max_length_k(const double *c) {
+#pragma clang fp contract(off)
+    const double l1 = len3_k(c[0] - c[3], c[1] - c[4], c[2] - c[5]);
+    const double l2 = len3_k(c[3] - c[6], c[4] - c[7], c[5] - c[8]);
+    return l1 < l2 ? l2 : l1;
+}
+// clamped cell coordinate of the host TargetGrid
+__device__ inline int64_t grid_coord(const RefineGrid &G, int k, double v) {
+#pragma clang fp contract(off)
+    int64_t c = (int64_t)floor((v - G.lo[k]) / G.h);
+    c = c < 0 ? 0 : c;
+    return c > G.dim[k] - 1 ? G.dim[k] - 1 : c;
+}
+// some target with |t - c|^2 <= r^2 (TargetGrid::ball)
+__device__ bool any_in_ball(const RefineGrid &G, const double *c, double r) {
+#pragma clang fp contract(off)
+    int64_t a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        a[k] = grid_coord(G, k, c[k] - r);
+        b[k] = grid_coord(G, k, c[k] + r);
+    }
+    const double r2 = r * r;
+    for (int64_t i = a[0]; i <= b[0]; ++i)
+        for (int64_t j = a[1]; j <= b[1]; ++j)
+            for (int64_t k = a[2]; k <= b[2]; ++k) {
+                const int64_t cc = (i * G.dim[1] + j) * G.dim[2] + k;
+                for (int64_t q = G.start[cc]; q < G.start[cc + 1]; ++q) {
+                    const double *p = G.t + 3 * (int64_t)G.idx[q];
+                    const double dx = p[0] - c[0], dy = p[1] - c[1], dz = p[2] - c[2];
+                    if (dx * dx + dy * dy + dz * dz <= r2) return true;
+                }
+            }
+    return false;
+}
+
+constexpr int kRefineStack = 64;  // items per thread; deeper refinements fall back to the host
+
+// One thread per coarse panel, its subtree walked as the host stack walks it (children pushed in
+// order 0..3, so visited 3..0).  EMIT = false: count[p] = leaves of panel p (-1: stack overflow);
+// EMIT = true: the leaves from offset[p] on (corners, parent, from_split).
+template <bool EMIT>
+__global__ __launch_bounds__(64) void refine_kernel(int P, const double *__restrict__ corners, RefineGrid G, int64_t M,
+                                                    double limit, double range_factor, int nq, int64_t *__restrict__ count,
+                                                    const int64_t *__restrict__ offset, double *__restrict__ out_c,
+                                                    int *__restrict__ parent, int *__restrict__ from_split) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double stk[kRefineStack][12];
+    unsigned char spl[kRefineStack];
+    int sp = 1;
+    for (int k = 0; k < 12; ++k) stk[0][k] = corners[12 * (int64_t)p + k];
+    spl[0] = 0;
+    int64_t n = 0;
+    while (sp > 0) {
+        --sp;
+        double c[12];
+        for (int k = 0; k < 12; ++k) c[k] = stk[sp][k];
+        const unsigned char was_split = spl[sp];
+        const double l = panel_max_length_k(c);
+        double cc[3];
+        for (int k = 0; k < 3; ++k) cc[k] = (c[k] + c[3 + k] + c[6 + k] + c[9 + k]) / 4;
+        const double r = range_factor * l / nq;
+        const bool near = M > 0 && any_in_ball(G, cc, r);
+        if (near && l > limit) {
+            if (sp + 4 > kRefineStack) {
+                if (!EMIT) count[p] = -1;
+                return;
+            }
+            // _split_panel4 (src/kernel/laplace3d_near.jl:78-97): (a,ab,m,da), (ab,b,bc,m), (m,bc,c,cd), (da,m,cd,d)
+            double ab[3], bc[3], cd[3], da[3], m[3];
+            for (int k = 0; k < 3; ++k) {
+                ab[k] = (c[k] + c[3 + k]) / 2;
+                bc[k] = (c[3 + k] + c[6 + k]) / 2;
+                cd[k] = (c[6 + k] + c[9 + k]) / 2;
+                da[k] = (c[9 + k] + c[k]) / 2;
+                m[k] = (c[k] + c[3 + k] + c[6 + k] + c[9 + k]) / 4;
+            }
+            const double *quads[4][4] = {{c, ab, m, da}, {ab, c + 3, bc, m}, {m, bc, c + 6, cd}, {da, m, cd, c + 9}};
+            for (int q = 0; q < 4; ++q) {
+                for (int v = 0; v < 4; ++v)
+                    for (int k = 0; k < 3; ++k) stk[sp][3 * v + k] = quads[q][v][k];
+                spl[sp] = 1;
+                ++sp;
+            }
+        } else {
+            if (EMIT) {
+                const int64_t o = offset[p] + n;
+                for (int k = 0; k < 12; ++k) out_c[12 * o + k] = c[k];
+                parent[o] = p;
+                from_split[o] = was_split;
+            }
+            ++n;
+        }
+    }
+    if (!EMIT) count[p] = n;
+}
+
+// rect_panel3d_discretize nodes and weights of the refined panels (fill_nodes, geometry.cpp)
+__global__ __launch_bounds__(256) void refined_nodes_kernel(int64_t Pr, int nq, const double *__restrict__ corners,
+                                                            const double *__restrict__ glx,
+                                                            const double *__restrict__ glw, double *__restrict__ x,
+                                                            double *__restrict__ y, double *__restrict__ z,
+                                                            double *__restrict__ w) {
+#pragma clang fp contract(off)
+    const int npp = nq * nq;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Pr * npp) return;
+    const int64_t p = n / npp;
+    const int ij = (int)(n - p * npp), i = ij / nq, j = ij - i * nq;
+    const double *A = corners + 12 * p, *B = A + 3, *C = A + 6, *D = A + 9;
+    const double Lab = len3_k(B[0] - A[0], B[1] - A[1], B[2] - A[2]);
+    const double Lbc = len3_k(C[0] - B[0], C[1] - B[1], C[2] - B[2]);
+    double v[3];
+    for (int k = 0; k < 3; ++k) {
+        const double cc = (A[k] + B[k] + C[k] + D[k]) / 4;
+        v[k] = cc + (B[k] - A[k]) * (glx[i] / 2) + (D[k] - A[k]) * (glx[j] / 2);
+    }
+    x[n] = v[0];
+    y[n] = v[1];
+    z[n] = v[2];
+    w[n] = glw[i] * glw[j] * Lab * Lbc / 4;
+}
+
+// Prolongation rows: a node of an unsplit panel copies its coarse node; one of a split panel takes
+// the tensor Lagrange interpolant of its parent (coarse rule glx, bary), at the node's local
+// coordinates from the 2 x 2 normal equations of the parent's edge vectors.
+__global__ __launch_bounds__(256) void prolong_kernel(int64_t Pr, int nq, const double *__restrict__ coarse_corners,
+                                                      const double *__restrict__ rules, const int *__restrict__ parent,
+                                                      const int *__restrict__ from_split, const double *__restrict__ x,
+                                                      const double *__restrict__ y, const double *__restrict__ z,
+                                                      const int64_t *__restrict__ p_ptr, int *__restrict__ p_col,
+                                                      double *__restrict__ p_val) {
+#pragma clang fp contract(off)
+    const int npp = nq * nq;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Pr * npp) return;
+    const int64_t q = n / npp;
+    const int k = (int)(n - q * npp);
+    const int par = parent[q];
+    const int64_t col0 = (int64_t)par * npp;
+    int64_t e = p_ptr[n];
+    if (!from_split[q]) {
+        p_col[e] = (int)(col0 + k);
+        p_val[e] = 1.0;
+        return;
+    }
+    const double *glx = rules, *bary = rules + nq;
+    const double *c = coarse_corners + 12 * (int64_t)par;
+    const double *a = c, *b = c + 3, *d = c + 9;
+    double cc[3], bma[3], dma[3];
+    for (int t = 0; t < 3; ++t) {
+        cc[t] = (c[t] + c[3 + t] + c[6 + t] + c[9 + t]) / 4;
+        bma[t] = b[t] - a[t];
+        dma[t] = d[t] - a[t];
+    }
+    const double bb = bma[0] * bma[0] + bma[1] * bma[1] + bma[2] * bma[2];
+    const double bd = bma[0] * dma[0] + bma[1] * dma[1] + bma[2] * dma[2];
+    const double dd = dma[0] * dma[0] + dma[1] * dma[1] + dma[2] * dma[2];
+    const double det = bb * dd - bd * bd;
+    const double inv11 = dd / det, inv12 = -bd / det, inv22 = bb / det;
+    const double xv[3] = {x[n] - cc[0], y[n] - cc[1], z[n] - cc[2]};
+    const double rhs1 = bma[0] * xv[0] + bma[1] * xv[1] + bma[2] * xv[2];
+    const double rhs2 = dma[0] * xv[0] + dma[1] * xv[1] + dma[2] * xv[2];
+    const double s = inv11 * rhs1 + inv12 * rhs2;
+    const double t = inv12 * rhs1 + inv22 * rhs2;
+    double rx[16], ry[16];
+    bary_row(nq, glx, bary, 2 * s, rx);
+    bary_row(nq, glx, bary, 2 * t, ry);
+    for (int ii = 0; ii < nq; ++ii)
+        for (int jj = 0; jj < nq; ++jj, ++e) {
+            p_col[e] = (int)(col0 + ii * nq + jj);
+            p_val[e] = rx[ii] * ry[jj];
+        }
+}
 }  // namespace
+
+void refine_count(hipStream_t st, int P, const double *corners, const RefineGrid &G, int64_t M, double limit,
+                  double range_factor, int nq, int64_t *count) {
+    if (P > 0)
+        refine_kernel<false><<<(P + 63) / 64, 64, 0, st>>>(P, corners, G, M, limit, range_factor, nq, count, nullptr,
+                                                            nullptr, nullptr, nullptr);
+}
+void refine_emit(hipStream_t st, int P, const double *corners, const RefineGrid &G, int64_t M, double limit,
+                 double range_factor, int nq, const int64_t *offset, double *out_corners, int *parent, int *from_split) {
+    if (P > 0)
+        refine_kernel<true><<<(P + 63) / 64, 64, 0, st>>>(P, corners, G, M, limit, range_factor, nq, nullptr, offset,
+                                                           out_corners, parent, from_split);
+}
+void refined_nodes(hipStream_t st, int64_t Pr, int nq, const double *corners, const double *glx, const double *glw,
+                   double *x, double *y, double *z, double *w) {
+    const int64_t n = Pr * nq * nq;
+    if (n > 0)
+        refined_nodes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Pr, nq, corners, glx, glw, x, y, z, w);
+}
+void prolongation(hipStream_t st, int64_t Pr, int nq, const double *coarse_corners, const double *rules,
+                  const int *parent, const int *from_split, const double *x, const double *y, const double *z,
+                  const int64_t *p_ptr, int *p_col, double *p_val) {
+    const int64_t n = Pr * nq * nq;
+    if (n > 0)
+        prolong_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Pr, nq, coarse_corners, rules, parent, from_split,
+                                                                     x, y, z, p_ptr, p_col, p_val);
+}
 
 void csr_spmm(hipStream_t st, int64_t R, const int64_t *ptr, const int *col, const double *val, int K, const double *X,
               int64_t ldx, double *Y, int64_t ldy, bool accumulate) {

@@ -927,13 +927,13 @@ def blob_lattice_centers(Lx, Ly, Lz, nx=16, ny=8, nz=8):
 class TargetPotentialMap:
     """The M x N map sigma -> u[sigma](targets) of laplace3d_pottrg_fmm3d_corrected_hcubature
     (src/kernel/laplace3d_near_hcubature.jl:337-375): direct sum over the interface refined around
-    the targets plus the adaptive-cubature near-target corrections.  The host builds the
-    refinement and the prolongation; the near-target cubatures run on the GPU
-    (device_cubature=True, bit-identical to the host cubature) or on n_threads host threads;
-    evaluation runs on the GPU."""
+    the targets plus the adaptive-cubature near-target corrections.  device_cubature=True integrates
+    the near-target cubatures on the GPU (bit-identical to the host cubature) instead of on n_threads
+    host threads; device_refine=True (with it) also builds the refinement and the prolongation on the
+    GPU (bit-identical to the host build), keeping the prolongation there.  Evaluation runs on the GPU."""
 
     def __init__(self, interface: DielectricInterface, targets, hcubature_atol: float, range_factor: float = 5.0,
-                 n_threads: int = 0, device_cubature: bool = True):
+                 n_threads: int = 0, device_cubature: bool = True, device_refine: bool = True):
         t = np.asarray(targets, np.float64)
         if t.ndim == 2 and t.shape[0] == 3 and t.shape[1] != 3:
             t = t.T
@@ -941,7 +941,8 @@ class TargetPotentialMap:
         self.M, self.n = t.shape[0], interface.n_points
         h = C.c_void_p()
         L.check(L.lib().dbie_tmap_create_ex(interface._h, self.M, L.dptr(t.ravel()), float(hcubature_atol),
-                                            float(range_factor), int(n_threads), int(bool(device_cubature)),
+                                            float(range_factor), int(n_threads),
+                                            (2 if device_refine else 1) if device_cubature else 0,
                                             C.byref(h)))
         self._h = h
         self._iface = interface

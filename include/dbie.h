@@ -290,7 +290,10 @@ dbie_status dbie_tmap_create(const dbie_geom *g, int64_t M, const double *trg, d
                              double range_factor, int n_threads, dbie_tmap **out);
 /* As dbie_tmap_create (same reference function); device_cubature = 1 integrates the near-target
    cubatures on the GPU (one wave per panel-target pair, the host cubature's operation order, so
-   the map is bit-identical); the refinement and the prolongation stay on the host. */
+   the map is bit-identical); device_cubature = 2 also builds the refinement around the targets
+   (one thread per coarse panel walking its subtree in the host's order) and the prolongation on
+   the GPU, bit-identical to the host build, and keeps the prolongation there (for n_quad <= 5;
+   otherwise, or when a subtree is deeper than 21 levels, the host builds both). */
 dbie_status dbie_tmap_create_ex(const dbie_geom *g, int64_t M, const double *trg, double hcub_atol,
                                 double range_factor, int n_threads, int device_cubature, dbie_tmap **out);
 void dbie_tmap_destroy(dbie_tmap *tm);

@@ -113,3 +113,38 @@ def test_device_cubatures_match_host(dbie, nq, l_ec, atol):
     (pp, pc, pv), (cp, cc, cv) = th.matrices()
     assert np.array_equal(Cp, cp) and np.array_equal(Cc, cc)
     assert np.array_equal(Cv, cv), np.abs(Cv - cv).max()
+
+
+@pytest.mark.parametrize("nq,l_ec", [(4, 0.2), (5, 0.2), (4, 0.1)])
+def test_device_refinement_matches_oracle(dbie, O, nq, l_ec):
+    """The refinement around the targets and the prolongation built on the device
+    (_refine_interface_for_targets / _refined_interface_prolongation, src/kernel/laplace3d_near.jl:99-225)
+    equal the oracle's TargetMap and the host build bit for bit: refined nodes and weights, parent and
+    split flags in the same order, prolongation rows, columns and values; corrections and evaluation
+    follow."""
+    say(f"target map: device refinement + prolongation vs oracle and host (n_quad={nq}, l_ec={l_ec})")
+    centers = [(0.2, 0.1, 0.05), (0.55, -0.25, 0.27), (-0.58, 0.0, 0.0), (0.0, 0.39, -0.29)]
+    srcs, pos, q, grp = blob_cloud(dbie, centers, 0.02)
+    itf = dbie.single_dielectric_box3d(*BOX1, nq, l_ec, 4.0, 1.0)
+    td = dbie.TargetPotentialMap(itf, pos, 1e-9, device_refine=True)
+    th = dbie.TargetPotentialMap(itf, pos, 1e-9, device_refine=False)
+    to = O.TargetMap(O.Interface.box(*BOX1, nq, l_ec, 4.0, 1.0), pos, 1e-9)
+    oxyzw, opar, ospl, (opp, opc, opv), (ocp, occ, ocv) = to.parts()
+    xyzw, par, spl = td.refined()
+    say(f"  refined nodes {td.n_refined} (oracle {oxyzw.shape[1]}), pairs {td.n_pairs}")
+    assert np.array_equal(xyzw, oxyzw)
+    assert np.array_equal(par, opar) and np.array_equal(spl, ospl.astype(bool))
+    (pp, pc, pv), (cp, cc, cv) = td.matrices()
+    assert np.array_equal(pp, opp.astype(np.int64)) and np.array_equal(pc, opc) and np.array_equal(pv, opv)
+    (hpp, hpc, hpv), (hcp, hcc, hcv) = th.matrices()
+    hxyzw, hpar, hspl = th.refined()
+    assert np.array_equal(xyzw, hxyzw) and np.array_equal(pp, hpp) and np.array_equal(pc, hpc)
+    assert np.array_equal(pv, hpv)
+    assert np.array_equal(cp, hcp) and np.array_equal(cc, hcc) and np.array_equal(cv, hcv)
+    S = np.random.default_rng(7).standard_normal((itf.n_points, 3))
+    U, Uh = td.apply(S), th.apply(S)
+    assert np.abs(U - Uh).max() <= 1e-13 * np.abs(Uh).max()
+    R, Rh = td.reaction(grp, q, S), th.reaction(grp, q, S)
+    assert np.abs(R - Rh).max() <= 1e-13 * np.abs(Rh).max()
+    ms_d, ms_h = td.setup_ms(), th.setup_ms()
+    say(f"  setup ms device {ms_d} host {ms_h}")
