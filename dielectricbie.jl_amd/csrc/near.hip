@@ -92,69 +92,71 @@ __device__ inline int xcd_row(int w, int P) {
     return r;
 }
 
-template <typename VT>
+// NQ column blocks of 16 per wave: a workgroup covers 64 NQ columns (NQ = 1, 2, 4 by K).
+template <typename VT, int NQ>
 __global__ __launch_bounds__(256) void near_mfma_kernel(int P, const int *__restrict__ row_ptr,
                                                         const int *__restrict__ col, const VT *__restrict__ val,
                                                         int S, const double *__restrict__ part, int64_t N, int K,
                                                         const double *__restrict__ X, int64_t ldx,
                                                         const double *__restrict__ diag, double *__restrict__ Y,
                                                         int64_t ldy) {
-    __shared__ double s_out[256][17];  // [column within the 256-column group][row], padded
+    constexpr int CW = 64 * NQ;         // columns per workgroup
+    __shared__ double s_out[CW][17];  // [column within the column group][row], padded
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int p = xcd_row(blockIdx.x, P);
-    const int k0 = blockIdx.y * 256;
+    const int k0 = blockIdx.y * CW;
     if (p >= P) return;  // uniform per workgroup
-    v4d acc[4];
+    v4d acc[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
-    // column of this lane for each of its 4 column blocks (clamped; masked by a zero multiplier)
-    int64_t coff[4];
-    double cmask[4];
+    for (int q = 0; q < NQ; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    // column of this lane for each of its NQ column blocks (clamped; masked by a zero multiplier)
+    int64_t coff[NQ];
+    double cmask[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const int c = k0 + (wave + 4 * q) * 16 + (lane & 15);
         cmask[q] = c < K ? 1.0 : 0.0;
         coff[q] = (int64_t)(c < K ? c : K - 1) * ldx;
     }
     const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
     // operands of one block: A (4 k-slices) and X (4 k-slices x 4 column blocks), prefetched one block ahead
-    double an[4], xn[4][4];
-    auto load = [&](int b, double (&a)[4], double (&xv)[4][4]) {
+    double an[4], xn[4][NQ];
+    auto load = [&](int b, double (&a)[4], double (&xv)[4][NQ]) {
         const VT *v = val + (int64_t)b * 256;
         const int64_t src_row = (int64_t)col[b] * 16 + (lane >> 4);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             a[kk] = (double)v[kk * 64 + lane];  // C_b[t = lane%16][c = kk*4 + lane/16]
 #pragma unroll
-            for (int q = 0; q < 4; ++q) xv[kk][q] = X[src_row + kk * 4 + coff[q]];
+            for (int q = 0; q < NQ; ++q) xv[kk][q] = X[src_row + kk * 4 + coff[q]];
         }
     };
     if (b0 < b1) load(b0, an, xn);
     for (int b = b0; b < b1; ++b) {
-        double a[4], xv[4][4];
+        double a[4], xv[4][NQ];
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             a[kk] = an[kk];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) xv[kk][q] = xn[kk][q];
+            for (int q = 0; q < NQ; ++q) xv[kk][q] = xn[kk][q];
         }
         if (b + 1 < b1) load(b + 1, an, xn);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], xv[kk][q], acc[q], 0, 0, 0);
+            for (int q = 0; q < NQ; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], xv[kk][q], acc[q], 0, 0, 0);
     }
     // D(row = 4r + lane/16, col = lane%16) -> LDS [col][row]
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int r = 0; r < 4; ++r) s_out[(wave + 4 * q) * 16 + (lane & 15)][4 * r + (lane >> 4)] = acc[q][r] * cmask[q];
     __syncthreads();
-    // each thread finishes 16 outputs: row = tid % 16, columns tid/16 + 16*m
+    // each thread finishes CW / 16 outputs: row = tid % 16, columns tid/16 + 16*m
     const int row = tid & 15;
     const int64_t i = (int64_t)p * 16 + row;
     const double t = diag[i];
-    for (int m = 0; m < 16; ++m) {
+    for (int m = 0; m < CW / 16; ++m) {
         const int cl = (tid >> 4) + 16 * m;
         const int c = k0 + cl;
         if (c >= K) continue;
@@ -253,6 +255,7 @@ __global__ void x_tiles_f32_kernel(const double *__restrict__ X, int64_t ldx, in
 // Full near blocks (FP32, lane-major) times the tile-major FP32 copy of X on the FP64 MFMA: per
 // block one 16-byte load of A and one of B per column block and lane.  MFMA kk contracts source
 // columns c = 4 (lane / 16) + kk, a permutation of the block's 16 columns.
+template <int NQ>
 __global__ __launch_bounds__(256) void near_up32_kernel(int P, const int *__restrict__ row_ptr,
                                                         const int *__restrict__ col, const float *__restrict__ val,
                                                         int S, const double *__restrict__ part, int64_t N, int K,
@@ -260,48 +263,49 @@ __global__ __launch_bounds__(256) void near_up32_kernel(int P, const int *__rest
                                                         const double *__restrict__ X, int64_t ldx,
                                                         const double *__restrict__ diag, double *__restrict__ Y,
                                                         int64_t ldy) {
-    __shared__ double s_out[256][17];
+    constexpr int CW = 64 * NQ;  // columns per workgroup
+    __shared__ double s_out[CW][17];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int p = xcd_row(blockIdx.x, P);
-    const int k0 = blockIdx.y * 256;
+    const int k0 = blockIdx.y * CW;
     if (p >= P) return;  // uniform per workgroup
-    v4d acc[4];
+    v4d acc[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < NQ; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
     const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
-    float4 an, xn[4];
-    auto load = [&](int b, float4 &a, float4 (&xv)[4]) {
+    float4 an, xn[NQ];
+    auto load = [&](int b, float4 &a, float4 (&xv)[NQ]) {
         a = *reinterpret_cast<const float4 *>(val + (int64_t)b * 256 + lane * 4);
         const float *xs = Xt + (int64_t)col[b] * Kpad * 16 + 4 * (lane >> 4);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < NQ; ++q)
             xv[q] = *reinterpret_cast<const float4 *>(xs + (int64_t)(k0 + (wave + 4 * q) * 16 + (lane & 15)) * 16);
     };
     if (b0 < b1) load(b0, an, xn);
     for (int b = b0; b < b1; ++b) {
         const float4 a = an;
-        float4 xv[4];
+        float4 xv[NQ];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) xv[q] = xn[q];
+        for (int q = 0; q < NQ; ++q) xv[q] = xn[q];
         if (b + 1 < b1) load(b + 1, an, xn);
         const double av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 const double bv = kk == 0 ? xv[q].x : kk == 1 ? xv[q].y : kk == 2 ? xv[q].z : xv[q].w;
                 acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv, acc[q], 0, 0, 0);
             }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int r = 0; r < 4; ++r) s_out[(wave + 4 * q) * 16 + (lane & 15)][4 * r + (lane >> 4)] = acc[q][r];
     __syncthreads();
     const int row = tid & 15;
     const int64_t i = (int64_t)p * 16 + row;
     const double t = diag[i];
-    for (int m = 0; m < 16; ++m) {
+    for (int m = 0; m < CW / 16; ++m) {
         const int cl = (tid >> 4) + 16 * m;
         const int c = k0 + cl;
         if (c >= K) continue;
@@ -312,26 +316,41 @@ __global__ __launch_bounds__(256) void near_up32_kernel(int P, const int *__rest
 }
 }  // namespace
 
-size_t near_xt_bytes(int64_t N, int K) { return sizeof(float) * (size_t)N * (size_t)((K + 255) / 256 * 256); }
+// column blocks of 16 per wave of the near kernels: a workgroup covers 64, 128 or 256 columns
+static int near_nq(int K) { return K <= 64 ? 1 : (K <= 128 ? 2 : 4); }
+static int near_kpad(int K) {
+    const int cw = 64 * near_nq(K);
+    return (K + cw - 1) / cw * cw;
+}
+
+size_t near_xt_bytes(int64_t N, int K) { return sizeof(float) * (size_t)N * (size_t)near_kpad(K); }
 
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
                   const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt) {
     const int bd = C.bd;
     const float *v32 = vals == NEAR_UP32 ? C.valup32 : (vals == NEAR_C32 ? C.val32 : nullptr);
+    const int nq = near_nq(K), Kpad = near_kpad(K), cw = 64 * nq;
     if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 16) {  // xt: near_xt_bytes(N, K)
-        const int Kpad = (K + 255) / 256 * 256;
         x_tiles_f32_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)Kpad), 256, 0, st>>>(X, ldx, N, K, Kpad, xt);
         const int per = (C.P + 7) / 8;
-        dim3 grid((unsigned)(8 * per), (unsigned)(Kpad / 256));
-        near_up32_kernel<<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.valup32, S, part, N, K, xt, Kpad, X, ldx, diag,
-                                               Y, ldy);
+        dim3 grid((unsigned)(8 * per), (unsigned)(Kpad / cw));
+#define DBIE_UP32(Q) near_up32_kernel<Q><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.valup32, S, part, N, K, xt, \
+                                                               Kpad, X, ldx, diag, Y, ldy)
+        if (nq == 1) DBIE_UP32(1);
+        else if (nq == 2) DBIE_UP32(2);
+        else DBIE_UP32(4);
+#undef DBIE_UP32
         return;
     }
     if (C.mfma_order && bd == 16 && K >= 16) {
         const int per = (C.P + 7) / 8;
-        dim3 grid((unsigned)(8 * per), (unsigned)((K + 255) / 256));
-        near_mfma_kernel<double><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.val64, S, part, N, K, X, ldx, diag, Y,
-                                                       ldy);
+        dim3 grid((unsigned)(8 * per), (unsigned)((K + cw - 1) / cw));
+#define DBIE_NM(Q) near_mfma_kernel<double, Q><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, C.val64, S, part, N, K, X, \
+                                                                      ldx, diag, Y, ldy)
+        if (nq == 1) DBIE_NM(1);
+        else if (nq == 2) DBIE_NM(2);
+        else DBIE_NM(4);
+#undef DBIE_NM
         return;
     }
     if (!C.mfma_order && K >= 16 && (bd == 4 || bd == 9 || bd == 25)) {
