@@ -101,7 +101,9 @@ bool best_grid(double W, double H, double alpha, int *m_out, int *n_out) {
     };
     int64_t pmm = 0, pm = 1, qmm = 1, qm = 0;
     double x = r;
+    if (!(x < 0x1p62)) return false;   // the first partial quotient would not fit an int64
     for (int64_t it = 0; it < kMaxIter; ++it) {
+        // x < 2^62 here: r is checked above, and later x = 1/frac with frac > 1e-12
         const int64_t a = (int64_t)std::floor(x);
         const int64_t p = a * pm + pmm, q = a * qm + qmm;
         consider(p, q);
@@ -111,6 +113,10 @@ bool best_grid(double W, double H, double alpha, int *m_out, int *n_out) {
             if (qm > 0) tmax = std::min(tmax, (kMaxDen - qmm) / qm);
             for (int64_t t = 1; t <= tmax; ++t) consider(t * pm + pmm, t * qm + qmm);
         }
+        // p and q grow with every term (r >= 1), and so do the next semiconvergents: once either
+        // passes kMaxDen no later candidate is admissible.  Stopping here also keeps a*pm and
+        // a*qm (a < 1e12, pm, qm <= kMaxDen) inside int64.
+        if (p > kMaxDen || q > kMaxDen) break;
         const double frac = x - (double)a;
         if (std::fabs(frac) <= 1e-12) break;
         pmm = pm; pm = p; qmm = qm; qm = q;

@@ -11,7 +11,8 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdbie.so")
+# DBIE_LIB selects another build of the library (e.g. a sanitizer build of the host code)
+LIB_PATH = os.environ.get("DBIE_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdbie.so")
 
 OK, BAD_ARG, DIM_MISMATCH, OOM, HIP_ERROR, NOT_CONVERGED, NO_DEVICE, PENDING = range(8)
 FP64, FP32, BF16, BF16X3, FP16X3 = 0, 1, 2, 3, 4

@@ -68,6 +68,20 @@ def test_gauss_legendre_and_best_grid_match_oracle(dbie, O):
         assert np.array_equal(x, xo) and np.array_equal(w, wo)
     assert dbie.best_grid_mn(20.0, 1.0, math.sqrt(2)) == (15, 1)
     assert dbie.best_grid_mn(3.0, 2.0, 1.0) == (3, 2)
+    # extreme aspect ratios, whose continued fractions run past the denominator bound of 10^6
+    rng = np.random.default_rng(3)
+    cases = [(1.0, 1e-9, 1.1), (1.0, 3e-7, 1.05), (math.pi, 1.0, 1.0001), (1e6, 1.0, 1.5), (1.0, 1e-15, 2.0)]
+    cases += [tuple(rng.uniform(0.01, 100, 2)) + (float(rng.uniform(1, 1.01)),) for _ in range(200)]
+    for W, H, a in cases:
+        try:
+            got = dbie.best_grid_mn(W, H, a)
+        except ValueError:
+            got = None
+        try:
+            want = O.best_grid(W, H, a)
+        except ValueError:
+            want = None
+        assert got == want, (W, H, a)
     with pytest.raises(ValueError):
         dbie.best_grid_mn(0.0, 1.0, 1.0)
     with pytest.raises(ValueError):
