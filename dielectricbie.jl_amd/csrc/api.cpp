@@ -164,6 +164,7 @@ struct dbie_op {
     // resident device data
     DBuf x, y, z, nx, ny, nz, w, xf, yf, zf, nxf, nyf, nzf, wf, plane, diag;
     DBuf bsr_row, bsr_col, bsr_v64, bsr_v32, bsr_vup32, nmask;
+    DBuf bsr_vh, bsr_rexp;  // split-FP16 full near blocks (2^rexp C / w, hi and lo halves) and row exponents
     dbie_near_opts nopts{};       // near options the operator was built with
     DBuf bsr_d64;                 // correction blocks of the double layer D (built on first use)
     bool d_built = false;
@@ -556,6 +557,10 @@ bool excludes_near(const dbie_op *op, int K, dbie_precision prec) {
     return prec == DBIE_FP16X3 && op->dg.near_mask && op->bsr.valup32 && far_field_mfma_supported((int)prec, K);
 }
 
+// whether an apply that leaves the near panel pairs out of its far field takes their full interaction
+// from the split-FP16 blocks (no FP32 copy of X needed)
+bool near_uses_h3(const dbie_op *op, int K) { return op->bsr.valh && op->bsr.mfma_order && op->bsr.bd == 16 && K >= 16; }
+
 TuneKey tune_key(const dbie_op *op, int K, dbie_precision prec, bool exclude) {
     return TuneKey{(int)prec, exclude ? 1 : 0, op->g.npp(), (K + 31) / 32,
                    (int)std::lround(4.0 * std::log2((double)std::max<int64_t>(op->N(), 1)))};
@@ -845,8 +850,13 @@ void apply_core(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int
     {
         ProfScope ps(op, 2);
         const NearVals nv = exclude ? NEAR_UP32 : (prec != DBIE_FP64 ? NEAR_C32 : NEAR_C64);
+        // the full near blocks multiply the far field's split-FP16 operands of this X where they
+        // were packed for it, else a tile-major FP32 copy of X
+        NearH3 h3;
+        const bool use_h3 = exclude && near_uses_h3(op, K);
+        if (use_h3) far_f16_operands(ws.mscratch, N, K, &h3);
         near_combine(op->stream, op->bsr, nv, S, ws.part, N, K, X, ldx, op->diag.as<double>(), Y, ldy,
-                     nv == NEAR_UP32 ? ws.xt32 : nullptr);
+                     nv == NEAR_UP32 && !use_h3 ? ws.xt32 : nullptr, use_h3 ? &h3 : nullptr);
     }
     DBIE_HIP(hipGetLastError());
 }
@@ -867,7 +877,7 @@ void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
         }
         op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)prec, K));
         ws.mscratch = op->mscratch.p;
-        if (excludes_near(op, K, prec)) {
+        if (excludes_near(op, K, prec) && !near_uses_h3(op, K)) {
             op->xt32.ensure(near_xt_bytes(N, K));
             ws.xt32 = op->xt32.as<float>();
         }
@@ -1809,7 +1819,7 @@ size_t apply_noalloc_bytes(const dbie_op *op, int K, dbie_precision prec) {
     size_t b = 256;  // fp16 scale reduction
     if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
         b += up(far_field_mfma_scratch_bytes(N, (int)prec, K));
-        if (excludes_near(op, K, prec)) b += up(near_xt_bytes(N, K));
+        if (excludes_near(op, K, prec) && !near_uses_h3(op, K)) b += up(near_xt_bytes(N, K));
     }
     return b + sizeof(double) * (size_t)std::max(K, 1) * (size_t)N;  // one split of partial sums
 }
@@ -1839,7 +1849,7 @@ void apply_noalloc(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
     if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
         ws.mscratch = base + off;
         off += up(far_field_mfma_scratch_bytes(N, (int)prec, K));
-        if (excludes_near(op, K, prec)) {
+        if (excludes_near(op, K, prec) && !near_uses_h3(op, K)) {
             ws.xt32 = reinterpret_cast<float *>(base + off);
             off += up(near_xt_bytes(N, K));
         }
@@ -2182,6 +2192,14 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         c.val32 = op->bsr_v32.as<float>();
         c.valup32 = bs.nblocks() > 0 ? op->bsr_vup32.as<float>() : nullptr;
         c.mfma_order = mfma_order;
+        if (mfma_order && c.valup32) {
+            op->bsr_vh.ensure(sizeof(uint16_t) * 512 * (size_t)c.nb);
+            op->bsr_rexp.ensure(sizeof(int) * (size_t)c.P);
+            near_h3_pack(st, c, op->w.as<double>(), op->bsr_vh.p, op->bsr_rexp.as<int>());
+            DBIE_HIP(hipGetLastError());
+            c.valh = op->bsr_vh.p;
+            c.rexp = op->bsr_rexp.as<int>();
+        }
         DBIE_HIP(hipStreamSynchronize(st));
         op->setup_ms[3] += ms(t3, Clk::now());
         *out = op.release();
@@ -2259,7 +2277,8 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
         int64_t b = 0;
         for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
                               &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
-                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_d64, &op->nmask, &op->part,
+                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_vh, &op->bsr_rexp, &op->bsr_d64,
+                              &op->nmask, &op->part,
                               &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32})
             b += (int64_t)d->bytes;
         // the Krylov workspaces the operator keeps for later solves

@@ -27,4 +27,15 @@ __device__ inline void bary_row(int n, const double *x, const double *lam, doubl
     for (int i = 0; i < n; ++i) r[i] *= inv;
 }
 
+// Exponent shift of a split-FP16 operand column: with m = max |w_j X_jk| (float bits, as the
+// column-max kernel leaves them), column k is scaled by 2^col_exponent so that its entries stay
+// below 2^14 (0 for an all-zero column).
+__device__ inline int col_exponent(unsigned bits) {
+    const float m = __uint_as_float(bits);
+    if (!(m > 0.0f)) return 0;
+    int e;
+    (void)frexpf(m, &e);  // m < 2^e
+    return 14 - e;
+}
+
 }  // namespace dbie

@@ -28,6 +28,8 @@
 
 #include <algorithm>
 
+#include "devmath.hpp"
+
 #include "kernels.hpp"
 
 namespace dbie {
@@ -400,28 +402,24 @@ __global__ void zero_u32_kernel(unsigned *__restrict__ p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-// per-column max |w_j X_jk| as float bits (non-negative floats order like unsigned ints);
-// one workgroup per column
+// per-column max |w_j X_jk| as float bits (non-negative floats order like unsigned ints): a
+// 2-D grid of 4096-row chunks x columns, each workgroup folding its chunk into cmax[k] with one
+// atomic max (cmax cleared beforehand)
+constexpr int kColmaxRows = 4096;
 __global__ __launch_bounds__(256) void colmax_kernel(const double *__restrict__ X, int64_t ldx,
                                                      const double *__restrict__ w, int64_t N,
                                                      unsigned *__restrict__ cmax) {
     __shared__ float s_m[4];
-    const int k = blockIdx.x;
+    const int k = blockIdx.y;
     const double *xk = X + (int64_t)k * ldx;
+    const int64_t j0 = (int64_t)blockIdx.x * kColmaxRows;
+    const int64_t j1 = min(N, j0 + kColmaxRows);
     float m = 0.0f;
-    for (int64_t j = threadIdx.x; j < N; j += 256) m = fmaxf(m, fabsf((float)(w[j] * xk[j])));
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) m = fmaxf(m, fabsf((float)(w[j] * xk[j])));
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) cmax[k] = __float_as_uint(fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3])));
-}
-
-__device__ inline int col_exponent(unsigned bits) {
-    const float m = __uint_as_float(bits);
-    if (!(m > 0.0f)) return 0;
-    int e;
-    (void)frexpf(m, &e);  // m < 2^e
-    return 14 - e;
+    if (threadIdx.x == 0) atomicMax(cmax + k, __float_as_uint(fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]))));
 }
 
 __global__ void split_tiles_f16(const double *__restrict__ X, int64_t ldx, const double *__restrict__ w, int64_t N,
@@ -1074,6 +1072,18 @@ size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K) {
     return sizeof(__bf16) * zelems * (prec == 3 ? 2 : 1);
 }
 
+void far_f16_operands(void *scratch, int64_t N, int K, NearH3 *out) {
+    const int ncb = ncb_for(K);
+    const int Kp = ((K + ncb * 16 - 1) / (ncb * 16)) * ncb * 16;
+    const size_t zelems = (size_t)((N + kSb - 1) / kSb) * Kp * kSb;
+    const _Float16 *zh = static_cast<const _Float16 *>(scratch);
+    out->zh = zh;
+    out->zl = zh + zelems;
+    out->cmax = reinterpret_cast<const unsigned *>(zh + 2 * zelems);
+    out->Kp = Kp;
+    out->tile = kSb;
+}
+
 int far_field_max_splits(int64_t N) { return (int)std::max<int64_t>(1, N / 256); }
 
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
@@ -1111,7 +1121,9 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         _Float16 *zh = static_cast<_Float16 *>(scratch);
         _Float16 *zl = zh + zelems;
         unsigned *cmax = reinterpret_cast<unsigned *>(zl + zelems);
-        colmax_kernel<<<(unsigned)K, 256, 0, st>>>(X, ldx, g.w, N, cmax);
+        zero_u32_kernel<<<(unsigned)((K + 255) / 256), 256, 0, st>>>(cmax, K);
+        colmax_kernel<<<dim3((unsigned)((N + kColmaxRows - 1) / kColmaxRows), (unsigned)K), 256, 0, st>>>(X, ldx, g.w,
+                                                                                                         N, cmax);
         split_tiles_f16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
                                                                                                     cmax, zh, zl);
         NearMask nm;

@@ -32,6 +32,10 @@ struct DevBSR {
     const double *val64 = nullptr;  // nb x bd x bd blocks, element (t, c) at bsr_storage_index(bd, mfma_order, t, c)
     const float *val32 = nullptr;
     const float *valup32 = nullptr; // nb blocks of the full near interaction (correction + direct), FP32
+    // the full near blocks for the split-FP16 operator (bd = 16): per block the hi and then the lo
+    // halves of 2^rexp[p] C[t, c] / w_c, row-major (t, c), as _Float16; rexp per block row p
+    const void *valh = nullptr;
+    const int *rexp = nullptr;
     bool mfma_order = false;        // bd = 16: blocks stored in MFMA operand order
 };
 // Storage position of block element (t, c): MFMA operand order for bd = 16
@@ -74,12 +78,26 @@ float far_field_f16_scale(hipStream_t st, const DevGeom &g, void *tmp4);
 bool far_field_mfma_supported(int prec, int K);
 int64_t far_field_part_elems(int64_t N, int K);  // workspace needed for any far-field launch
 
+// The split-FP16 far field's operands of an apply (in its scratch): hi and lo halves of
+// 2^col_exponent(cmax[k]) w_j X[j, k], tile-major ((j / tile) Kp + k) tile + j % tile.
+struct NearH3 {
+    const void *zh = nullptr, *zl = nullptr;
+    const unsigned *cmax = nullptr;
+    int Kp = 0, tile = 0;
+};
+void far_f16_operands(void *scratch, int64_t N, int K, NearH3 *out);
+
 // Y[:,k] = sum_s part[s][k] + dscale*diag.*X[:,k] + C X[:,k] (C in FP64 or FP32 values).
 enum NearVals { NEAR_C64 = 0, NEAR_C32 = 1, NEAR_UP32 = 2 };
-// NEAR_UP32 with 16 x 16 blocks and K >= 16 uses xt, near_xt_bytes(N, K) bytes of scratch for a
-// tile-major FP32 copy of X; the full near blocks are then stored lane-major (bsr_lane_major_index).
+// NEAR_UP32 with 16 x 16 blocks and K >= 16: with h3 (the far field's split-FP16 operands of the
+// same X) and C.valh, the full near blocks multiply them on the FP16 MFMA (three products of the
+// hi/lo splits); otherwise xt, near_xt_bytes(N, K) bytes of scratch, takes a tile-major FP32 copy
+// of X for the FP32 blocks, stored lane-major (bsr_lane_major_index).
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
-                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt = nullptr);
+                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt = nullptr,
+                  const NearH3 *h3 = nullptr);
+// C.valh and C.rexp (device arrays of nb * 512 halves and P ints) from C.valup32 and the node weights w
+void near_h3_pack(hipStream_t st, const DevBSR &C, const double *w, void *valh, int *rexp);
 size_t near_xt_bytes(int64_t N, int K);
 int64_t bsr_lane_major_index(int t, int c);
 

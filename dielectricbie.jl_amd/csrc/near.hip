@@ -19,7 +19,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 
+#include "devmath.hpp"
 #include "kernels.hpp"
 
 namespace dbie {
@@ -38,6 +40,8 @@ int64_t bsr_lane_major_index(int t, int c) { return lane_major_idx(t, c); }
 
 namespace {
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 // One output (row, column) per CS threads: thread cs of the group takes the block columns
 // c = cs, cs + CS, ... of every block of the row, and the group's partial sums meet in LDS.  Thread
@@ -314,6 +318,129 @@ __global__ __launch_bounds__(256) void near_up32_kernel(int P, const int *__rest
         Y[i + (int64_t)c * ldy] = y;
     }
 }
+// One workgroup per block row p: the row's full near blocks (FP32, lane-major) as 2^rexp[p] C[t, c] / w_c
+// split into hi and lo halves (row-major per block: hi at b * 512 + 16 t + c, lo 256 further), with
+// rexp[p] chosen so that the row's largest |C[t, c] / w_c| scales below 2^14.  Dividing by the source
+// weights lets the blocks multiply the far field's split operands w .* X directly.
+__global__ __launch_bounds__(256) void near_h3_pack_kernel(const int *__restrict__ row_ptr, const int *__restrict__ col,
+                                                           const float *__restrict__ vup, const double *__restrict__ w,
+                                                           _Float16 *__restrict__ vh, int *__restrict__ rexp) {
+    __shared__ double s_m[4];
+    const int p = blockIdx.x;
+    const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
+    const int tid = threadIdx.x, t = tid >> 4, c = tid & 15;
+    double m = 0.0;
+    for (int b = b0; b < b1; ++b)
+        m = fmax(m, fabs((double)vup[(int64_t)b * 256 + lane_major_idx(t, c)] / w[(int64_t)col[b] * 16 + c]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((tid & 63) == 0) s_m[tid >> 6] = m;
+    __syncthreads();
+    m = fmax(fmax(s_m[0], s_m[1]), fmax(s_m[2], s_m[3]));
+    int e = 0;
+    if (m > 0.0) (void)frexp(m, &e);  // m < 2^e
+    const int r = 14 - e;
+    if (tid == 0) rexp[p] = r;
+    for (int b = b0; b < b1; ++b) {
+        const double v = ldexp((double)vup[(int64_t)b * 256 + lane_major_idx(t, c)] / w[(int64_t)col[b] * 16 + c], r);
+        const _Float16 hi = (_Float16)(float)v;
+        vh[(int64_t)b * 512 + 16 * t + c] = hi;
+        vh[(int64_t)b * 512 + 256 + 16 * t + c] = (_Float16)(float)(v - (double)(float)hi);
+    }
+}
+
+// Height of the far field's split-FP16 operand tiles (sources per tile, tile-major
+// ((j / kZTile) Kp + k) kZTile + j % kZTile), checked against the far field's at run time.
+constexpr int kZTile = 64;
+
+// Full near blocks of the split-FP16 operator on the FP16 MFMA (v_mfma_f32_16x16x32_f16): the 32-deep
+// contraction takes two blocks of the row at a time, lanes of k-groups 0-1 the first (sources 0-7,
+// 8-15), 2-3 the second, and each block pair costs the three products hi.hi + hi.lo + lo.hi of the
+// blocks' and the far field's splits (the operands the far field of the same apply left in its
+// scratch: no conversion of X here).  One workgroup per block row and 64 NQ columns, NQ column
+// blocks of 16 per wave; the next pair's operands are loaded while the current one multiplies.
+// The epilogue undoes the row and column scalings and adds the partial sums and the diagonal term.
+template <int NQ>
+__global__ __launch_bounds__(256) void near_h3_kernel(int P, const int *__restrict__ row_ptr, const int *__restrict__ col,
+                                                      const _Float16 *__restrict__ vh, const int *__restrict__ rexp,
+                                                      int S, const double *__restrict__ part, int64_t N, int K,
+                                                      const _Float16 *__restrict__ zh, const _Float16 *__restrict__ zl,
+                                                      const unsigned *__restrict__ cmax, int Kp,
+                                                      const double *__restrict__ X, int64_t ldx,
+                                                      const double *__restrict__ diag, double *__restrict__ Y,
+                                                      int64_t ldy) {
+    constexpr int CW = 64 * NQ;  // columns per workgroup
+    __shared__ double s_out[CW][17];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int p = xcd_row(blockIdx.x, P);
+    const int k0 = blockIdx.y * CW;
+    if (p >= P) return;  // uniform per workgroup
+    // lane: row / column t, sources 8 h .. 8 h + 7 of the pair's block `second`
+    const int t = lane & 15, h = (lane >> 4) & 1, second = lane >> 5;
+    int64_t zoff[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int c = k0 + (wave + 4 * q) * 16 + t;
+        zoff[q] = (int64_t)(c < K ? c : K - 1) * kZTile + 8 * h;  // columns past K: clamped, never stored
+    }
+    v4f acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int b0 = row_ptr[p], b1 = row_ptr[p + 1];
+    auto load = [&](int b, uint4 &ah, uint4 &al, uint4 (&bh)[NQ], uint4 (&bl)[NQ]) {
+        const int bb = b + second;
+        const int bv = bb < b1 ? bb : b;  // an odd row's last pair: the second block's A is zero
+        const _Float16 *a = vh + (int64_t)bv * 512 + 16 * t + 8 * h;
+        ah = *reinterpret_cast<const uint4 *>(a);
+        al = *reinterpret_cast<const uint4 *>(a + 256);
+        if (bb >= b1) ah = al = uint4{0u, 0u, 0u, 0u};
+        const int64_t s0 = (int64_t)col[bv] * 16;  // the source panel's first node: 16-aligned in its tile
+        const int64_t tb = (s0 / kZTile) * (int64_t)Kp * kZTile + (s0 % kZTile);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            bh[q] = *reinterpret_cast<const uint4 *>(zh + tb + zoff[q]);
+            bl[q] = *reinterpret_cast<const uint4 *>(zl + tb + zoff[q]);
+        }
+    };
+    uint4 nah, nal, nbh[NQ], nbl[NQ];
+    if (b0 < b1) load(b0, nah, nal, nbh, nbl);
+    for (int b = b0; b < b1; b += 2) {
+        const v8h ah = __builtin_bit_cast(v8h, nah), al = __builtin_bit_cast(v8h, nal);
+        v8h bh[NQ], bl[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            bh[q] = __builtin_bit_cast(v8h, nbh[q]);
+            bl[q] = __builtin_bit_cast(v8h, nbl[q]);
+        }
+        if (b + 2 < b1) load(b + 2, nah, nal, nbh, nbl);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[q], acc[q], 0, 0, 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[q], acc[q], 0, 0, 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[q], acc[q], 0, 0, 0);
+        }
+    }
+    // D(row = 4 (lane / 16) + r, col = lane % 16) -> LDS [col][row], scaled back
+    const int rp = rexp[p];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int c = k0 + (wave + 4 * q) * 16 + (lane & 15);
+        const double sc = ldexp(1.0, -(rp + col_exponent(cmax[c < K ? c : K - 1])));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s_out[(wave + 4 * q) * 16 + (lane & 15)][4 * (lane >> 4) + r] = (double)acc[q][r] * sc;
+    }
+    __syncthreads();
+    const int row = tid & 15;
+    const int64_t i = (int64_t)p * 16 + row;
+    const double dg = diag[i];
+    for (int m = 0; m < CW / 16; ++m) {
+        const int cl = (tid >> 4) + 16 * m;
+        const int c = k0 + cl;
+        if (c >= K) continue;
+        double y = s_out[cl][row] + dg * X[i + (int64_t)c * ldx];
+        for (int s2 = 0; s2 < S; ++s2) y += part[((int64_t)s2 * K + c) * N + i];
+        Y[i + (int64_t)c * ldy] = y;
+    }
+}
 }  // namespace
 
 // column blocks of 16 per wave of the near kernels: a workgroup covers 64, 128 or 256 columns
@@ -325,11 +452,32 @@ static int near_kpad(int K) {
 
 size_t near_xt_bytes(int64_t N, int K) { return sizeof(float) * (size_t)N * (size_t)near_kpad(K); }
 
+void near_h3_pack(hipStream_t st, const DevBSR &C, const double *w, void *valh, int *rexp) {
+    if (C.P < 1) return;
+    near_h3_pack_kernel<<<(unsigned)C.P, 256, 0, st>>>(C.row_ptr, C.col, C.valup32, w, static_cast<_Float16 *>(valh),
+                                                        rexp);
+}
+
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
-                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt) {
+                  const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt,
+                  const NearH3 *h3) {
     const int bd = C.bd;
     const float *v32 = vals == NEAR_UP32 ? C.valup32 : (vals == NEAR_C32 ? C.val32 : nullptr);
     const int nq = near_nq(K), Kpad = near_kpad(K), cw = 64 * nq;
+    if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 16 && h3 && C.valh) {
+        if (h3->tile != kZTile) throw std::logic_error("near_combine: split-FP16 operand tiles of an unexpected height");
+        const int per = (C.P + 7) / 8;
+        dim3 grid((unsigned)(8 * per), (unsigned)(Kpad / cw));
+        const _Float16 *vh = static_cast<const _Float16 *>(C.valh);
+        const _Float16 *zh = static_cast<const _Float16 *>(h3->zh), *zl = static_cast<const _Float16 *>(h3->zl);
+#define DBIE_H3(Q) near_h3_kernel<Q><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, vh, C.rexp, S, part, N, K, zh, zl, \
+                                                           h3->cmax, h3->Kp, X, ldx, diag, Y, ldy)
+        if (nq == 1) DBIE_H3(1);
+        else if (nq == 2) DBIE_H3(2);
+        else DBIE_H3(4);
+#undef DBIE_H3
+        return;
+    }
     if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 16) {  // xt: near_xt_bytes(N, K)
         x_tiles_f32_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)Kpad), 256, 0, st>>>(X, ldx, N, K, Kpad, xt);
         const int per = (C.P + 7) / 8;
