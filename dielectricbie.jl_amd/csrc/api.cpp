@@ -18,6 +18,7 @@
 #include <memory>
 #include <atomic>
 #include <tuple>
+#include <type_traits>
 #include <map>
 #include <mutex>
 
@@ -136,6 +137,7 @@ using namespace dbie;
 struct KrylovRecycle {
     bool valid = false;
     int nblk = 0, m = 0;
+    int elem = 8;  // bytes per element of the stored basis (V1): 8 (FP64) or 4 (FP32)
     std::vector<int> k1;
     std::vector<double> cs, sn, R, omega;
 };
@@ -144,6 +146,7 @@ struct GmresWork {
     DBuf V, W, Z, U, partial, hbuf, nbuf;
     DBuf V1, abuf, cbuf;  // recycled Krylov basis of an earlier inner solve, its projection coefficients
     DBuf om, n1;          // the recycled space's unit normals omega and block counts on the device
+    PinnedBuf pin;        // host staging of the per-iteration coefficients, norms and projections
     KrylovRecycle rec;
     DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
     DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
@@ -1045,8 +1048,9 @@ struct GmresOut {
 // r0 = (I - C C^T) r with c = C^T r, each new Arnoldi vector is made orthogonal to C as well
 // (b_j = C^T A M v_j), and the cycle's correction is M (U (c - B y) + V y), which minimises the
 // residual over span(U) + the new Krylov space: the refinement continues the earlier space.
-GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol, int itmax,
-               int restart, const double *dinv, KrylovRecycle *keep = nullptr, const KrylovRecycle *use = nullptr) {
+template <typename TV>
+GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol,
+                  int itmax, int restart, const double *dinv, KrylovRecycle *keep, const KrylovRecycle *use) {
     const int64_t N = op->N();
     const size_t NK = (size_t)N * K;
     const int m = std::max(1, std::min(restart > 0 ? restart : 100, std::max(itmax, 1)));
@@ -1055,15 +1059,21 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     GmresWork &ws = op->gw;
     DBuf &W = ws.W, &Z = ws.Z, &U = ws.U, &partial = ws.partial, &hbuf = ws.hbuf, &nbuf = ws.nbuf;
     const bool defl = use && use->valid && use->nblk > 0;
+    if (defl && use->elem != (int)sizeof(TV)) fail(DBIE_BAD_ARG, "recycled Krylov space stored in another precision");
     const int nb1 = defl ? use->nblk : 0, um = defl ? use->m : 0;
     const int cgs_passes = prec == DBIE_FP64 ? 2 : 1;
     W.ensure(sizeof(double) * NK);
     U.ensure(sizeof(double) * NK);
-    if (dinv) Z.ensure(sizeof(double) * NK);
+    constexpr bool kF64 = std::is_same<TV, double>::value;
+    if (dinv || !kF64) Z.ensure(sizeof(double) * NK);
     partial.ensure(sizeof(double) * std::max(dots_partial_elems(N, K, std::max(m + 1, nb1)), dots_partial_elems(N, K, 1)));
-    hbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
+    hbuf.ensure(sizeof(double) * (size_t)(cgs_passes == 2 ? 2 * (m + 1) : m + 1) * K);
     nbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
-    std::vector<double> a_h;  // V1^T W of the last deflation, copied back with the iteration's sync
+    // pinned host staging: V1^T W of the last deflation (nb1 x K), then an Arnoldi step's CGS
+    // coefficients of both passes ((m + 1) x K each) and the squared norms (K)
+    ws.pin.ensure(sizeof(double) * ((size_t)nb1 + 2 * (size_t)(m + 1) + 1) * K);
+    double *const a_h = ws.pin.as<double>();
+    double *const ph1 = a_h + (size_t)nb1 * K, *const ph2 = ph1 + (size_t)(m + 1) * K, *const phn = ph2 + (size_t)(m + 1) * K;
     if (defl) {
         ws.abuf.ensure(sizeof(double) * (size_t)nb1 * K);
         ws.cbuf.ensure(sizeof(double) * (size_t)nb1 * K);
@@ -1077,27 +1087,26 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         }
         upload(ws.om, om_h.data(), om_h.size(), op->stream);
         upload(ws.n1, n1_h.data(), n1_h.size(), op->stream);
-        a_h.resize((size_t)nb1 * K);
     }
     hipStream_t st = op->stream;
     double *Wp = W.as<double>();
-    const double *V1p = defl ? ws.V1.as<double>() : nullptr;
+    const TV *V1p = defl ? ws.V1.as<TV>() : nullptr;
     // basis capacity in N x K blocks; growing copies the blocks already built
-    auto basis = [&](int blocks) -> double * {
+    auto basis = [&](int blocks) -> TV * {
         blocks = std::min(blocks, m + 1);
-        if (ws.V.bytes >= sizeof(double) * NK * blocks) return ws.V.as<double>();
-        const int cap = std::min(m + 1, std::max(blocks, std::max(16, 2 * (int)(ws.V.bytes / (sizeof(double) * NK)))));
+        if (ws.V.bytes >= sizeof(TV) * NK * blocks) return ws.V.as<TV>();
+        const int cap = std::min(m + 1, std::max(blocks, std::max(16, 2 * (int)(ws.V.bytes / (sizeof(TV) * NK)))));
         DBuf nv(ws.V.pooled);
-        nv.ensure(sizeof(double) * NK * cap);
+        nv.ensure(sizeof(TV) * NK * cap);
         if (ws.V.p) {
             DBIE_HIP(hipMemcpyAsync(nv.p, ws.V.p, ws.V.bytes, hipMemcpyDeviceToDevice, st));
             DBIE_HIP(hipStreamSynchronize(st));
         }
         std::swap(ws.V.p, nv.p);
         std::swap(ws.V.bytes, nv.bytes);
-        return ws.V.as<double>();
+        return ws.V.as<TV>();
     };
-    double *Vp = basis(2);
+    TV *Vp = basis(2);
 
     // Wd <- (I - C C^T) Wd with C C^T = V1 (I - omega omega^T) V1^T on the span of V1_(k1+1), on the
     // device; V1^T Wd is copied back for the projections, which project() forms after the next sync:
@@ -1107,7 +1116,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         block_dots(st, V1p, nb1, Wd, N, K, partial.as<double>(), ws.abuf.as<double>());
         deflate_coef(st, ws.abuf.as<double>(), ws.om.as<double>(), ws.n1.as<int>(), nb1, K, ws.cbuf.as<double>());
         block_subtract(st, V1p, nb1, ws.cbuf.as<double>(), Wd, N, K);
-        DBIE_HIP(hipMemcpyAsync(a_h.data(), ws.abuf.p, sizeof(double) * a_h.size(), hipMemcpyDeviceToHost, st));
+        DBIE_HIP(hipMemcpyAsync(a_h, ws.abuf.p, sizeof(double) * (size_t)nb1 * K, hipMemcpyDeviceToHost, st));
     };
     auto project = [&]() {
         tproj.assign((size_t)(um + 1) * K, 0.0);
@@ -1128,9 +1137,14 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     DBIE_HIP(hipMemsetAsync(X, 0, sizeof(double) * NK, st));
     // r0 = B (x0 = 0) lives in W
     DBIE_HIP(hipMemcpyAsync(Wp, B, sizeof(double) * NK, hipMemcpyDeviceToDevice, st));
-    std::vector<double> beta, tol(K), H((size_t)(m + 1) * m * K), cs((size_t)m * K), sn((size_t)m * K),
-        gv((size_t)(m + 1) * K), hh((size_t)(m + 1) * K), yv((size_t)m * K);
-    std::vector<double> cvec, Bm;  // deflation: C^T r of the cycle's start (um per column), B (um x m per column)
+    std::vector<double> beta, tol(K), cs((size_t)m * K), sn((size_t)m * K), gv((size_t)(m + 1) * K),
+        hh((size_t)(m + 1) * K), yv((size_t)m * K);
+    // The Hessenberg columns of the cycle, grown one Arnoldi step at a time (the restart length can be
+    // far above the steps a solve takes; storage for all of it would be K (m + 1) m doubles per call):
+    // Hc[j][k (j + 2) + i] = H_k(i, j), i <= j + 1.  Deflation: Bc[j][k um + i] = B_k(i, j).
+    std::vector<std::vector<double>> Hc, Bc;
+    auto Hat = [&](int k, int i, int l) -> double & { return Hc[l][(size_t)k * (l + 2) + i]; };
+    std::vector<double> cvec;  // deflation: C^T r of the cycle's start (um per column)
     std::vector<int> nconv(K);
     std::vector<char> active(K), conv(K);
     GmresOut res;
@@ -1157,7 +1171,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                 for (int i = 0; i < k1; ++i) {
                     double v = cvec[(size_t)k * um + i];
                     for (int j = 0; j < nconv[k]; ++j)
-                        v -= Bm[((size_t)k * um + i) * m + j] * yv[(size_t)j * K + k];
+                        v -= Bc[j][(size_t)k * um + i] * yv[(size_t)j * K + k];
                     rhs[i] = v;
                 }
                 const double *Rk = &use->R[(size_t)k * (um + 1) * um];
@@ -1182,7 +1196,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
             cvec.assign((size_t)um * K, 0.0);
             for (int k = 0; k < K; ++k)
                 for (int i = 0; i < use->k1[k]; ++i) cvec[(size_t)k * um + i] = tproj[(size_t)k * (um + 1) + i];
-            Bm.assign((size_t)K * um * m, 0.0);
+            Bc.clear();
         }
         bool any = false;
         for (int k = 0; k < K; ++k) {
@@ -1206,6 +1220,7 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
             break;
         }
         // V0 = r / beta
+        Hc.clear();
         h2d(nbuf.p, beta.data(), sizeof(double) * K, st);
         col_scale(st, Wp, nbuf.as<double>(), true, Vp, N, K);
         std::fill(gv.begin(), gv.end(), 0.0);
@@ -1214,11 +1229,13 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         int j = 0;
         for (; j < m && res.iters < itmax; ++j) {
             Vp = basis(j + 2);
-            const double *Vj = Vp + (size_t)j * NK;
-            const double *Zin = Vj;
-            if (dinv) {
+            const TV *Vj = Vp + (size_t)j * NK;
+            const double *Zin;
+            if (dinv || !kF64) {  // M v_j in FP64 (the apply's input)
                 row_scale(st, dinv, Vj, Z.as<double>(), N, K);
                 Zin = Z.as<double>();
+            } else {
+                Zin = reinterpret_cast<const double *>(Vj);
             }
             apply_dev(op, K, Zin, N, Wp, N, prec);
             if (defl) deflate(Wp);
@@ -1227,33 +1244,38 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
             // ~1e-5: the orthogonality CGS1 keeps is far below both
             block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), hbuf.as<double>());
             block_subtract(st, Vp, j + 1, hbuf.as<double>(), Wp, N, K);
-            std::vector<double> h1((size_t)(j + 1) * K), h2((size_t)(j + 1) * K, 0.0);
+            const double *h1 = ph1, *h2 = ph2;
             if (cgs_passes == 2) {
-                block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), nbuf.as<double>());
-                block_subtract(st, Vp, j + 1, nbuf.as<double>(), Wp, N, K);
-                DBIE_HIP(hipMemcpyAsync(h2.data(), nbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
+                // the second pass's coefficients in a buffer of their own (nbuf is reused below)
+                block_dots(st, Vp, j + 1, Wp, N, K, partial.as<double>(), hbuf.as<double>() + (size_t)(j + 1) * K);
+                block_subtract(st, Vp, j + 1, hbuf.as<double>() + (size_t)(j + 1) * K, Wp, N, K);
+                DBIE_HIP(hipMemcpyAsync(ph2, hbuf.as<double>() + (size_t)(j + 1) * K, sizeof(double) * (j + 1) * K,
+                                        hipMemcpyDeviceToHost, st));
+            } else {
+                std::fill(ph2, ph2 + (size_t)(j + 1) * K, 0.0);
             }
-            DBIE_HIP(hipMemcpyAsync(h1.data(), hbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
+            DBIE_HIP(hipMemcpyAsync(ph1, hbuf.p, sizeof(double) * (j + 1) * K, hipMemcpyDeviceToHost, st));
             // the next basis vector from the squared norms on the device; one sync brings the
             // coefficients, the norms and the deflation projections to the host
             block_dots(st, Wp, 1, Wp, N, K, partial.as<double>(), nbuf.as<double>());
             col_normalize(st, Wp, nbuf.as<double>(), Vp + (size_t)(j + 1) * NK, N, K);
-            std::vector<double> hn(K);
-            DBIE_HIP(hipMemcpyAsync(hn.data(), nbuf.p, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+            DBIE_HIP(hipMemcpyAsync(phn, nbuf.p, sizeof(double) * K, hipMemcpyDeviceToHost, st));
             DBIE_HIP(hipStreamSynchronize(st));
+            std::vector<double> hn(phn, phn + K);
             for (auto &v : hn) v = std::sqrt(std::max(v, 0.0));
+            Hc.emplace_back((size_t)K * (j + 2), 0.0);
+            if (defl) Bc.emplace_back((size_t)K * um, 0.0);
             if (defl) {
                 project();
                 for (int k = 0; k < K; ++k)
                     for (int i = 0; i < use->k1[k]; ++i)
-                        Bm[((size_t)k * um + i) * m + j] = tproj[(size_t)k * (um + 1) + i];
+                        Bc[j][(size_t)k * um + i] = tproj[(size_t)k * (um + 1) + i];
             }
             ++res.iters;
             bool all = true;
             for (int k = 0; k < K; ++k) {
                 if (conv[k]) continue;
-                double *Hk = &H[(size_t)k * (m + 1) * m];
-                double *col = Hk + (size_t)j * (m + 1);
+                double *col = &Hat(k, 0, j);
                 for (int i = 0; i <= j; ++i) col[i] = h1[(size_t)i * K + k] + h2[(size_t)i * K + k];
                 col[j + 1] = hn[k];
                 double *c = &cs[(size_t)k * m], *s = &sn[(size_t)k * m], *g = &gv[(size_t)k * (m + 1)];
@@ -1282,13 +1304,12 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
         for (int k = 0; k < K; ++k) {
             if (!active[k]) continue;
             const int nk = nconv[k];
-            const double *Hk = &H[(size_t)k * (m + 1) * m];
             const double *g = &gv[(size_t)k * (m + 1)];
             std::vector<double> y(nk);
             for (int i = nk - 1; i >= 0; --i) {
                 double s = g[i];
-                for (int l = i + 1; l < nk; ++l) s -= Hk[(size_t)l * (m + 1) + i] * y[l];
-                y[i] = s / Hk[(size_t)i * (m + 1) + i];
+                for (int l = i + 1; l < nk; ++l) s -= Hat(k, i, l) * y[l];
+                y[i] = s / Hat(k, i, i);
             }
             for (int i = 0; i < nk; ++i) yv[(size_t)i * K + k] = y[i];
         }
@@ -1316,12 +1337,18 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
             std::swap(ws.V.p, ws.V1.p);
             std::swap(ws.V.bytes, ws.V1.bytes);
             keep->nblk = nmax + 1;
-            keep->m = m;
+            keep->elem = (int)sizeof(TV);
+            // stored for the steps the cycle took (mk), not the restart length
+            const int mk = nmax;
+            keep->m = mk;
             keep->k1 = nconv;
-            keep->omega.assign((size_t)K * (m + 1), 0.0);
+            keep->omega.assign((size_t)K * (mk + 1), 0.0);
+            keep->cs.assign((size_t)K * mk, 0.0);
+            keep->sn.assign((size_t)K * mk, 0.0);
+            keep->R.assign((size_t)K * (mk + 1) * mk, 0.0);
             for (int k = 0; k < K; ++k) {
                 const int k1 = nconv[k];
-                double *om = &keep->omega[(size_t)k * (m + 1)];
+                double *om = &keep->omega[(size_t)k * (mk + 1)];
                 om[k1] = 1.0;
                 const double *c = &cs[(size_t)k * m], *s = &sn[(size_t)k * m];
                 for (int i = k1 - 1; i >= 0; --i) {  // omega = G_0^T ... G_(k1-1)^T e_k1
@@ -1329,14 +1356,31 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                     om[i] = c[i] * x0 - s[i] * x1;
                     om[i + 1] = s[i] * x0 + c[i] * x1;
                 }
+                for (int i = 0; i < mk; ++i) {
+                    keep->cs[(size_t)k * mk + i] = c[i];
+                    keep->sn[(size_t)k * mk + i] = s[i];
+                }
+                double *Rk = &keep->R[(size_t)k * (mk + 1) * mk];
+                for (int l = 0; l < k1; ++l)
+                    for (int i = 0; i <= l + 1; ++i) Rk[(size_t)l * (mk + 1) + i] = Hat(k, i, l);
             }
-            keep->cs = std::move(cs);
-            keep->sn = std::move(sn);
-            keep->R = std::move(H);
             keep->valid = true;
         }
     }
     return res;
+}
+
+GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol, int itmax,
+               int restart, const double *dinv, KrylovRecycle *keep = nullptr, const KrylovRecycle *use = nullptr) {
+    // The low-precision inner solves of the refinement keep their Krylov basis in FP32: their operator
+    // is accurate to ~1e-7 and their tolerance ~1e-5, and the orthogonalisation's passes over the
+    // basis (the solve's largest HBM traffic after the operator) halve.  Not when the space is kept
+    // for, or deflated in, a later refinement: that one works on a residual ~1e-5 of the first, and
+    // a basis rounded to FP32 deflates it so poorly that it takes about twice the iterations
+    // (measured at N = 64,384, K = 256: 39 inner iterations instead of 19).
+    const bool f32 = prec != DBIE_FP64 && !keep && !use;
+    return f32 ? gmres_t<float>(op, K, B, X, prec, rtol, atol, itmax, restart, dinv, keep, use)
+               : gmres_t<double>(op, K, B, X, prec, rtol, atol, itmax, restart, dinv, keep, use);
 }
 
 void blas_ok(rocblas_status st, const char *what) {

@@ -132,24 +132,33 @@ void pottrg(hipStream_t st, const DevGeom &g, int64_t M, const double *trg, int 
 void dense_lhs(hipStream_t st, const DevGeom &g, const double *diag, double *A, int64_t lda);
 
 // ---- Krylov primitives on column blocks (N x K, column-major, ld = N) ----
+// A Krylov basis V is stored in FP64 or FP32 (the low-precision inner solves); the vectors it
+// multiplies and the coefficients are FP64, and every sum is accumulated in FP64.
 // h[i*K + k] = <V_i[:,k], W[:,k]> for i < nv   (V_i = V + i*N*K); partial buffer of
 // dots_partial_elems(N, K, nv) doubles.
 int64_t dots_partial_elems(int64_t N, int K, int nv);
 void block_dots(hipStream_t st, const double *V, int nv, const double *W, int64_t N, int K, double *partial,
                 double *h);
+void block_dots(hipStream_t st, const float *V, int nv, const double *W, int64_t N, int K, double *partial,
+                double *h);
 // W[:,k] -= sum_i V_i[:,k] h[i*K+k]
 void block_subtract(hipStream_t st, const double *V, int nv, const double *h, double *W, int64_t N, int K);
+void block_subtract(hipStream_t st, const float *V, int nv, const double *h, double *W, int64_t N, int K);
 // out[:,k] = in[:,k] * s[k] (s may be inverted: out = in / s when invert)
 void col_scale(hipStream_t st, const double *in, const double *s, bool invert, double *out, int64_t N, int K);
+void col_scale(hipStream_t st, const double *in, const double *s, bool invert, float *out, int64_t N, int K);
 // out[:, k] = in[:, k] / sqrt(sq[k]) (zero where sq[k] <= 0)
 void col_normalize(hipStream_t st, const double *in, const double *sq, double *out, int64_t N, int K);
+void col_normalize(hipStream_t st, const double *in, const double *sq, float *out, int64_t N, int K);
 // coef[i, k] = a[i, k] - om[i, k] sum_{l < n1[k]} om[l, k] a[l, k] for i < n1[k], zero for n1[k] <= i < nb
 // (a, om, coef: nb x K, element (i, k) at i K + k)
 void deflate_coef(hipStream_t st, const double *a, const double *om, const int *n1, int nb, int K, double *coef);
-// out[:,k] = d .* in[:,k]  (d per row)
+// out[:,k] = d .* in[:,k]  (d per row; d null: out = in)
 void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K);
+void row_scale(hipStream_t st, const double *d, const float *in, double *out, int64_t N, int K);
 // X[:,k] += sum_i V_i[:,k] y[i*K+k]
 void block_combine(hipStream_t st, const double *V, int nv, const double *y, double *X, int64_t N, int K);
+void block_combine(hipStream_t st, const float *V, int nv, const double *y, double *X, int64_t N, int K);
 // R = F - R (residual from AX stored in R), column-wise
 void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K);
 // X += D

@@ -187,7 +187,8 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void dots_kernel(const double *__restrict__ V, int nv, const double *__restrict__ W,
+template <typename TV>
+__global__ __launch_bounds__(256) void dots_kernel(const TV *__restrict__ V, int nv, const double *__restrict__ W,
                                                    int64_t N, int K, double *__restrict__ partial) {
     __shared__ double red[4];
     const int64_t c = blockIdx.x;
@@ -203,12 +204,12 @@ __global__ __launch_bounds__(256) void dots_kernel(const double *__restrict__ V,
     }
     const int64_t nchunks = gridDim.x;
     for (int i = 0; i < nv; ++i) {
-        const double *vk = V + ((int64_t)i * K + k) * N;
+        const TV *vk = V + ((int64_t)i * K + k) * N;
         double s = 0.0;
 #pragma unroll
         for (int e = 0; e < kChunk / 256; ++e) {
             const int64_t n = n0 + e * 256 + tid;
-            if (n < N) s += vk[n] * wv[e];
+            if (n < N) s += (double)vk[n] * wv[e];
         }
         s = wave_sum(s);
         if ((tid & 63) == 0) red[tid >> 6] = s;
@@ -227,20 +228,21 @@ __global__ void dots_reduce_kernel(const double *__restrict__ partial, int64_t n
     h[e] = s;
 }
 
-template <int SIGN>
-__global__ __launch_bounds__(256) void combine_kernel(const double *__restrict__ V, int nv, const double *__restrict__ y,
+template <int SIGN, typename TV>
+__global__ __launch_bounds__(256) void combine_kernel(const TV *__restrict__ V, int nv, const double *__restrict__ y,
                                                       double *__restrict__ W, int64_t N, int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (n >= N) return;
     double s = 0.0;
-    for (int i = 0; i < nv; ++i) s += V[((int64_t)i * K + k) * N + n] * y[(int64_t)i * K + k];
+    for (int i = 0; i < nv; ++i) s += (double)V[((int64_t)i * K + k) * N + n] * y[(int64_t)i * K + k];
     if (SIGN < 0) W[(int64_t)k * N + n] -= s;
     else W[(int64_t)k * N + n] += s;
 }
 
+template <typename TO>
 __global__ void col_scale_kernel(const double *__restrict__ in, const double *__restrict__ s, int invert,
-                                 double *__restrict__ out, int64_t N, int K) {
+                                 TO *__restrict__ out, int64_t N, int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (n >= N) return;
@@ -248,18 +250,19 @@ __global__ void col_scale_kernel(const double *__restrict__ in, const double *__
     double r;
     if (invert) r = s[k] != 0.0 ? v / s[k] : 0.0;
     else r = v * s[k];
-    out[(int64_t)k * N + n] = r;
+    out[(int64_t)k * N + n] = (TO)r;
 }
 
 // out[:, k] = in[:, k] / sqrt(sq[k]) (zero where sq[k] <= 0): a new Arnoldi vector from its squared norm
 // without a host round trip
+template <typename TO>
 __global__ void col_normalize_kernel(const double *__restrict__ in, const double *__restrict__ sq,
-                                     double *__restrict__ out, int64_t N, int K) {
+                                     TO *__restrict__ out, int64_t N, int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (n >= N) return;
     const double s = sq[k];
-    out[(int64_t)k * N + n] = s > 0.0 ? in[(int64_t)k * N + n] / sqrt(s) : 0.0;
+    out[(int64_t)k * N + n] = (TO)(s > 0.0 ? in[(int64_t)k * N + n] / sqrt(s) : 0.0);
 }
 
 // coef[i, k] = a[i, k] - om[i, k] sum_l om[l, k] a[l, k] for i < n1[k], zero otherwise (the projection
@@ -275,11 +278,13 @@ __global__ void deflate_coef_kernel(const double *__restrict__ a, const double *
         coef[(int64_t)i * K + k] = i < n ? a[(int64_t)i * K + k] - om[(int64_t)i * K + k] * wa : 0.0;
 }
 
-__global__ void row_scale_kernel(const double *__restrict__ d, const double *__restrict__ in, double *__restrict__ out,
+// d may be null: out = in (a conversion to FP64)
+template <typename TI>
+__global__ void row_scale_kernel(const double *__restrict__ d, const TI *__restrict__ in, double *__restrict__ out,
                                  int64_t N, int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    if (n < N) out[(int64_t)k * N + n] = d[n] * in[(int64_t)k * N + n];
+    if (n < N) out[(int64_t)k * N + n] = (d ? d[n] : 1.0) * (double)in[(int64_t)k * N + n];
 }
 
 __global__ void sub_from_kernel(const double *__restrict__ F, double *__restrict__ R, int64_t N, int K) {
@@ -378,28 +383,48 @@ void dense_lhs(hipStream_t st, const DevGeom &g, const double *diag, double *A, 
 
 int64_t dots_partial_elems(int64_t N, int K, int nv) { return ((N + kChunk - 1) / kChunk) * (int64_t)K * nv; }
 
-void block_dots(hipStream_t st, const double *V, int nv, const double *W, int64_t N, int K, double *partial,
-                double *h) {
+template <typename TV>
+void block_dots_t(hipStream_t st, const TV *V, int nv, const double *W, int64_t N, int K, double *partial, double *h) {
     const int64_t nchunks = (N + kChunk - 1) / kChunk;
-    dots_kernel<<<dim3((unsigned)nchunks, (unsigned)K), 256, 0, st>>>(V, nv, W, N, K, partial);
+    dots_kernel<TV><<<dim3((unsigned)nchunks, (unsigned)K), 256, 0, st>>>(V, nv, W, N, K, partial);
     const int64_t count = (int64_t)nv * K;
     dots_reduce_kernel<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(partial, nchunks, count, h);
 }
+void block_dots(hipStream_t st, const double *V, int nv, const double *W, int64_t N, int K, double *partial,
+                double *h) {
+    block_dots_t(st, V, nv, W, N, K, partial, h);
+}
+void block_dots(hipStream_t st, const float *V, int nv, const double *W, int64_t N, int K, double *partial,
+                double *h) {
+    block_dots_t(st, V, nv, W, N, K, partial, h);
+}
 
 void block_subtract(hipStream_t st, const double *V, int nv, const double *h, double *W, int64_t N, int K) {
-    combine_kernel<-1><<<grid2(N, K), 256, 0, st>>>(V, nv, h, W, N, K);
+    combine_kernel<-1, double><<<grid2(N, K), 256, 0, st>>>(V, nv, h, W, N, K);
+}
+void block_subtract(hipStream_t st, const float *V, int nv, const double *h, double *W, int64_t N, int K) {
+    combine_kernel<-1, float><<<grid2(N, K), 256, 0, st>>>(V, nv, h, W, N, K);
 }
 
 void block_combine(hipStream_t st, const double *V, int nv, const double *y, double *X, int64_t N, int K) {
-    combine_kernel<1><<<grid2(N, K), 256, 0, st>>>(V, nv, y, X, N, K);
+    combine_kernel<1, double><<<grid2(N, K), 256, 0, st>>>(V, nv, y, X, N, K);
+}
+void block_combine(hipStream_t st, const float *V, int nv, const double *y, double *X, int64_t N, int K) {
+    combine_kernel<1, float><<<grid2(N, K), 256, 0, st>>>(V, nv, y, X, N, K);
 }
 
 void col_scale(hipStream_t st, const double *in, const double *s, bool invert, double *out, int64_t N, int K) {
-    col_scale_kernel<<<grid2(N, K), 256, 0, st>>>(in, s, invert ? 1 : 0, out, N, K);
+    col_scale_kernel<double><<<grid2(N, K), 256, 0, st>>>(in, s, invert ? 1 : 0, out, N, K);
+}
+void col_scale(hipStream_t st, const double *in, const double *s, bool invert, float *out, int64_t N, int K) {
+    col_scale_kernel<float><<<grid2(N, K), 256, 0, st>>>(in, s, invert ? 1 : 0, out, N, K);
 }
 
 void col_normalize(hipStream_t st, const double *in, const double *sq, double *out, int64_t N, int K) {
-    col_normalize_kernel<<<grid2(N, K), 256, 0, st>>>(in, sq, out, N, K);
+    col_normalize_kernel<double><<<grid2(N, K), 256, 0, st>>>(in, sq, out, N, K);
+}
+void col_normalize(hipStream_t st, const double *in, const double *sq, float *out, int64_t N, int K) {
+    col_normalize_kernel<float><<<grid2(N, K), 256, 0, st>>>(in, sq, out, N, K);
 }
 
 void deflate_coef(hipStream_t st, const double *a, const double *om, const int *n1, int nb, int K, double *coef) {
@@ -407,7 +432,10 @@ void deflate_coef(hipStream_t st, const double *a, const double *om, const int *
 }
 
 void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K) {
-    row_scale_kernel<<<grid2(N, K), 256, 0, st>>>(d, in, out, N, K);
+    row_scale_kernel<double><<<grid2(N, K), 256, 0, st>>>(d, in, out, N, K);
+}
+void row_scale(hipStream_t st, const double *d, const float *in, double *out, int64_t N, int K) {
+    row_scale_kernel<float><<<grid2(N, K), 256, 0, st>>>(d, in, out, N, K);
 }
 
 void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K) {

@@ -65,6 +65,33 @@ struct DBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Pinned host memory (hipHostMalloc) for the small per-iteration device-to-host copies of a Krylov
+// solve: copies into it are asynchronous, where copies into pageable memory block the host until the
+// device has reached them.  Grows on demand; the caller re-sizes only with no copy in flight.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipHostMalloc(&p, b, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            fail(DBIE_OOM, "hipHostMalloc of " + std::to_string(b) + " bytes failed");
+        }
+        bytes = b;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
 // Host-to-device copy that completes before returning (host sources may be temporaries).
 template <class T>
 void upload(DBuf &b, const T *src, size_t n, hipStream_t st) {
