@@ -168,6 +168,7 @@ struct dbie_op {
     DBuf x, y, z, nx, ny, nz, w, xf, yf, zf, nxf, nyf, nzf, wf, plane, diag;
     DBuf bsr_row, bsr_col, bsr_v64, bsr_v32, bsr_vup32, nmask;
     DBuf bsr_vh, bsr_rexp;  // split-FP16 full near blocks (2^rexp C / w, hi and lo halves) and row exponents
+    DBuf sr_ptr, sr_col, sr_blk;  // super-rows of the split-FP16 near kernel
     dbie_near_opts nopts{};       // near options the operator was built with
     DBuf bsr_d64;                 // correction blocks of the double layer D (built on first use)
     bool d_built = false;
@@ -1059,7 +1060,7 @@ GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision 
     GmresWork &ws = op->gw;
     DBuf &W = ws.W, &Z = ws.Z, &U = ws.U, &partial = ws.partial, &hbuf = ws.hbuf, &nbuf = ws.nbuf;
     const bool defl = use && use->valid && use->nblk > 0;
-    if (defl && use->elem != (int)sizeof(TV)) fail(DBIE_BAD_ARG, "recycled Krylov space stored in another precision");
+    if (defl && use->elem != (int)sizeof(double)) fail(DBIE_BAD_ARG, "recycled Krylov space not stored in FP64");
     const int nb1 = defl ? use->nblk : 0, um = defl ? use->m : 0;
     const int cgs_passes = prec == DBIE_FP64 ? 2 : 1;
     W.ensure(sizeof(double) * NK);
@@ -1090,7 +1091,7 @@ GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision 
     }
     hipStream_t st = op->stream;
     double *Wp = W.as<double>();
-    const TV *V1p = defl ? ws.V1.as<TV>() : nullptr;
+    const double *V1p = defl ? ws.V1.as<double>() : nullptr;  // kept by an FP64-basis solve
     // basis capacity in N x K blocks; growing copies the blocks already built
     auto basis = [&](int blocks) -> TV * {
         blocks = std::min(blocks, m + 1);
@@ -1374,11 +1375,11 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
                int restart, const double *dinv, KrylovRecycle *keep = nullptr, const KrylovRecycle *use = nullptr) {
     // The low-precision inner solves of the refinement keep their Krylov basis in FP32: their operator
     // is accurate to ~1e-7 and their tolerance ~1e-5, and the orthogonalisation's passes over the
-    // basis (the solve's largest HBM traffic after the operator) halve.  Not when the space is kept
-    // for, or deflated in, a later refinement: that one works on a residual ~1e-5 of the first, and
-    // a basis rounded to FP32 deflates it so poorly that it takes about twice the iterations
-    // (measured at N = 64,384, K = 256: 39 inner iterations instead of 19).
-    const bool f32 = prec != DBIE_FP64 && !keep && !use;
+    // basis (the solve's largest HBM traffic after the operator) halve.  Not a space kept for a later
+    // refinement: that one works on a residual ~1e-5 of the first, and a kept space rounded to FP32
+    // deflates it so poorly that it takes about twice the iterations (measured at N = 64,384,
+    // K = 256: 39 inner iterations instead of 19).  The deflating solve's own basis may be FP32.
+    const bool f32 = prec != DBIE_FP64 && !keep;
     return f32 ? gmres_t<float>(op, K, B, X, prec, rtol, atol, itmax, restart, dinv, keep, use)
                : gmres_t<double>(op, K, B, X, prec, rtol, atol, itmax, restart, dinv, keep, use);
 }
@@ -2243,6 +2244,39 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
             DBIE_HIP(hipGetLastError());
             c.valh = op->bsr_vh.p;
             c.rexp = op->bsr_rexp.as<int>();
+            // super-rows: kNearSuperRows consecutive block rows (neighbouring panels of a face) with the
+            // sorted union of their block columns
+            constexpr int RT = kNearSuperRows;
+            const int Q = (int)((g.P + RT - 1) / RT);
+            std::vector<int> sp(Q + 1, 0), scol, sblk;
+            std::vector<std::pair<int, int>> ent;  // (block column, block index)
+            for (int q = 0; q < Q; ++q) {
+                ent.clear();
+                for (int r = 0; r < RT && q * RT + r < g.P; ++r) {
+                    const int p = q * RT + r;
+                    for (int b = bs.row_ptr[p]; b < bs.row_ptr[p + 1]; ++b) ent.push_back({bs.col[b], b});
+                }
+                std::sort(ent.begin(), ent.end());
+                for (size_t e = 0; e < ent.size();) {
+                    const int cl = ent[e].first;
+                    scol.push_back(cl);
+                    const size_t base = sblk.size();
+                    sblk.resize(base + RT, -1);
+                    for (; e < ent.size() && ent[e].first == cl; ++e) {
+                        const int b = ent[e].second;
+                        const int r = (int)(std::upper_bound(bs.row_ptr.begin(), bs.row_ptr.end(), b) - bs.row_ptr.begin()) - 1 - q * RT;
+                        sblk[base + r] = b;
+                    }
+                }
+                sp[q + 1] = (int)scol.size();
+            }
+            upload(op->sr_ptr, sp.data(), sp.size(), st);
+            upload(op->sr_col, scol.data(), scol.size(), st);
+            upload(op->sr_blk, sblk.data(), sblk.size(), st);
+            c.sr_ptr = op->sr_ptr.as<int>();
+            c.sr_col = op->sr_col.as<int>();
+            c.sr_blk = op->sr_blk.as<int>();
+            c.sr_q = Q;
         }
         DBIE_HIP(hipStreamSynchronize(st));
         op->setup_ms[3] += ms(t3, Clk::now());
@@ -2321,7 +2355,8 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
         int64_t b = 0;
         for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
                               &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
-                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_vh, &op->bsr_rexp, &op->bsr_d64,
+                              &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_vh, &op->bsr_rexp, &op->sr_ptr,
+                              &op->sr_col, &op->sr_blk, &op->bsr_d64,
                               &op->nmask, &op->part,
                               &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32})
             b += (int64_t)d->bytes;

@@ -36,6 +36,11 @@ struct DevBSR {
     // halves of 2^rexp[p] C[t, c] / w_c, row-major (t, c), as _Float16; rexp per block row p
     const void *valh = nullptr;
     const int *rexp = nullptr;
+    // super-rows of kNearSuperRows consecutive block rows (the split-FP16 near kernel): the sorted union
+    // of their block columns, sr_col[sr_ptr[q] .. sr_ptr[q + 1]), and per union entry u the block of each
+    // row, sr_blk[u * kNearSuperRows + r] (-1: none)
+    const int *sr_ptr = nullptr, *sr_col = nullptr, *sr_blk = nullptr;
+    int sr_q = 0;
     bool mfma_order = false;        // bd = 16: blocks stored in MFMA operand order
 };
 // Storage position of block element (t, c): MFMA operand order for bd = 16
@@ -96,6 +101,7 @@ enum NearVals { NEAR_C64 = 0, NEAR_C32 = 1, NEAR_UP32 = 2 };
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
                   const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt = nullptr,
                   const NearH3 *h3 = nullptr);
+constexpr int kNearSuperRows = 4;
 // C.valh and C.rexp (device arrays of nb * 512 halves and P ints) from C.valup32 and the node weights w
 void near_h3_pack(hipStream_t st, const DevBSR &C, const double *w, void *valh, int *rexp);
 size_t near_xt_bytes(int64_t N, int K);

@@ -441,6 +441,102 @@ __global__ __launch_bounds__(256) void near_h3_kernel(int P, const int *__restri
         Y[i + (int64_t)c * ldy] = y;
     }
 }
+// The full near blocks of RT consecutive block rows against the far field's split operands: one
+// workgroup per super-row and 64 NQ columns.  Adjacent panels share most of their near panels, so the
+// workgroup walks the union of the rows' block columns and loads each source panel's split operands
+// once for all RT rows (a row without a block at that column multiplies zeros); the 32-deep contraction
+// takes two union entries at a time, as near_h3_kernel takes two blocks.  The epilogue writes from the
+// registers (no LDS): lane (row 4 (lane / 16) + r, column lane % 16) of each 16 x 16 tile.
+template <int NQ, int RT>
+__global__ __launch_bounds__(256) void near_h3_sr_kernel(int P, int Q, const int *__restrict__ sr_ptr,
+                                                         const int *__restrict__ sr_col, const int *__restrict__ sr_blk,
+                                                         const _Float16 *__restrict__ vh, const int *__restrict__ rexp,
+                                                         int S, const double *__restrict__ part, int64_t N, int K,
+                                                         const _Float16 *__restrict__ zh, const _Float16 *__restrict__ zl,
+                                                         const unsigned *__restrict__ cmax, int Kp,
+                                                         const double *__restrict__ X, int64_t ldx,
+                                                         const double *__restrict__ diag, double *__restrict__ Y,
+                                                         int64_t ldy) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = xcd_row(blockIdx.x, Q);
+    const int k0 = blockIdx.y * 64 * NQ;
+    if (q >= Q) return;  // uniform per workgroup
+    const int t = lane & 15, h = (lane >> 4) & 1, second = lane >> 5;
+    int64_t zoff[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        const int col = k0 + (wave + 4 * c) * 16 + t;
+        zoff[c] = (int64_t)(col < K ? col : K - 1) * kZTile + 8 * h;  // columns past K: clamped, never stored
+    }
+    v4f acc[RT][NQ];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc[r][c] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int u0 = sr_ptr[q], u1 = sr_ptr[q + 1];
+    auto load = [&](int u, uint4 (&ah)[RT], uint4 (&al)[RT], uint4 (&bh)[NQ], uint4 (&bl)[NQ]) {
+        const int uu = u + second;
+        const int uv = uu < u1 ? uu : u;  // an odd count's last pair: the second entry's A is zero
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int b = uu < u1 ? sr_blk[(int64_t)uv * RT + r] : -1;
+            const _Float16 *a = vh + (int64_t)(b >= 0 ? b : 0) * 512 + 16 * t + 8 * h;
+            ah[r] = *reinterpret_cast<const uint4 *>(a);
+            al[r] = *reinterpret_cast<const uint4 *>(a + 256);
+            if (b < 0) ah[r] = al[r] = uint4{0u, 0u, 0u, 0u};
+        }
+        const int64_t s0 = (int64_t)sr_col[uv] * 16;
+        const int64_t tb = (s0 / kZTile) * (int64_t)Kp * kZTile + (s0 % kZTile);
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            bh[c] = *reinterpret_cast<const uint4 *>(zh + tb + zoff[c]);
+            bl[c] = *reinterpret_cast<const uint4 *>(zl + tb + zoff[c]);
+        }
+    };
+    uint4 nah[RT], nal[RT], nbh[NQ], nbl[NQ];
+    if (u0 < u1) load(u0, nah, nal, nbh, nbl);
+    for (int u = u0; u < u1; u += 2) {
+        v8h ah[RT], al[RT], bh[NQ], bl[NQ];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            ah[r] = __builtin_bit_cast(v8h, nah[r]);
+            al[r] = __builtin_bit_cast(v8h, nal[r]);
+        }
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            bh[c] = __builtin_bit_cast(v8h, nbh[c]);
+            bl[c] = __builtin_bit_cast(v8h, nbl[c]);
+        }
+        if (u + 2 < u1) load(u + 2, nah, nal, nbh, nbl);
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bh[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bh[c], acc[r][c], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        const int col = k0 + (wave + 4 * c) * 16 + (lane & 15);
+        if (col >= K) continue;
+        const int ce = col_exponent(cmax[col]);
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int p = q * RT + r;
+            if (p >= P) continue;
+            const double sc = ldexp(1.0, -(rexp[p] + ce));
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = (int64_t)p * 16 + 4 * (lane >> 4) + e;
+                double y = (double)acc[r][c][e] * sc + diag[i] * X[i + (int64_t)col * ldx];
+                for (int s2 = 0; s2 < S; ++s2) y += part[((int64_t)s2 * K + col) * N + i];
+                Y[i + (int64_t)col * ldy] = y;
+            }
+        }
+    }
+}
 }  // namespace
 
 // column blocks of 16 per wave of the near kernels: a workgroup covers 64, 128 or 256 columns
@@ -470,6 +566,18 @@ void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const d
         dim3 grid((unsigned)(8 * per), (unsigned)(Kpad / cw));
         const _Float16 *vh = static_cast<const _Float16 *>(C.valh);
         const _Float16 *zh = static_cast<const _Float16 *>(h3->zh), *zl = static_cast<const _Float16 *>(h3->zl);
+        if (C.sr_ptr) {
+            const int perq = (C.sr_q + 7) / 8;
+            dim3 gq((unsigned)(8 * perq), (unsigned)(Kpad / cw));
+#define DBIE_SR(Q) near_h3_sr_kernel<Q, kNearSuperRows><<<gq, 256, 0, st>>>(C.P, C.sr_q, C.sr_ptr, C.sr_col, C.sr_blk, \
+                                                                              vh, C.rexp, S, part, N, K, zh, zl, h3->cmax, \
+                                                                              h3->Kp, X, ldx, diag, Y, ldy)
+            if (nq == 1) DBIE_SR(1);
+            else if (nq == 2) DBIE_SR(2);
+            else DBIE_SR(4);
+#undef DBIE_SR
+            return;
+        }
 #define DBIE_H3(Q) near_h3_kernel<Q><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, vh, C.rexp, S, part, N, K, zh, zl, \
                                                            h3->cmax, h3->Kp, X, ldx, diag, Y, ldy)
         if (nq == 1) DBIE_H3(1);
