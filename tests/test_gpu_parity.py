@@ -365,3 +365,30 @@ def test_residual_history(dbie, cfg1, precision, krylov):
     if precision == "fp64":  # GMRES residuals never increase
         kr = [v for k, v in h if k == "krylov"]
         assert all(b <= a * (1 + 1e-12) for a, b in zip(kr, kr[1:]))
+
+
+@pytest.mark.parametrize("drop", [1, 2, 3])
+def test_split_fp16_near_super_rows_ragged(dbie, O, drop):
+    """The split-FP16 near field walks super-rows of four block rows; an interface whose panel count is
+    not a multiple of four (a box with its last `drop` panels removed) leaves a short last super-row.
+    FP64 apply against the oracle (1e-12), split-FP16 against FP64 per column (2e-6) at widths that take
+    one, two and four column blocks per wave, with an all-zero column exactly zero."""
+    g = dbie.single_dielectric_box3d(*CFG1, 4.0, 1.0)
+    corners, normals, is_edge = g.panels()
+    P = corners.shape[0] - drop
+    say(f"ragged super-rows: P={P} ({P % 4} rows in the last)")
+    g2 = dbie.interface_from_panels(corners[:P], normals[:P], 4, 4.0, 1.0, is_edge[:P])
+    op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g2, 1e-9, 1e-9, 8)
+    ref = O.CorrectedOperator(O.Interface.from_panels(corners[:P], normals[:P], 4, 4.0, 1.0, is_edge[:P]))
+    rng = np.random.default_rng(40 + drop)
+    X = rng.standard_normal((g2.n_points, 16))
+    assert rel(op @ X, ref @ X) < 1e-12
+    for K in (16, 24, 100, 256):
+        X = rng.standard_normal((g2.n_points, K))
+        X[:, 5] = 0.0
+        Y64 = op @ X
+        Yp = op.apply(X, "fp16x3")
+        assert np.all(Yp[:, 5] == 0.0), K
+        errs = [rel(Yp[:, k], Y64[:, k]) for k in range(K) if k != 5]
+        say(f"  K={K}: max column error {max(errs):.2e}")
+        assert max(errs) < 2e-6, K
