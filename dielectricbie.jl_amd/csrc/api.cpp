@@ -191,6 +191,16 @@ struct dbie_op {
     GmresWork gw;                 // GMRES workspaces, reused across solves
     double setup_ms[4] = {0, 0, 0, 0};  // near pairs, correction blocks (host part), derived data, uploads + device block build
     float f16_gscale = 0.0f;  // power-of-two scale of the fp16 kernel entries (0 = not yet computed)
+    // the split-FP16 far field's kernel matrix kept in HBM (launch variants 2 and 3, K > 128): built on
+    // the first apply whose launch shape uses it.  stored_mode 1: when device memory allows (default;
+    // DBIE_STORED_FAR=0 starts operators with 0), 0: never.  gst_state 0: not built, 1: built, -1:
+    // declined for this operator (not enough free device memory)
+    DBuf gst, gst_off;
+    int gst_ntile = 0;
+    int stored_mode = 1;
+    int gst_state = 0;
+    bool gst_excl = false;        // built with the near panel pairs left out
+    std::vector<int> tile_plane_h;  // plane ids of the 256-node tiles (host copy)
     // optional per-kernel timing: event pairs recorded around launches
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[3];
@@ -204,6 +214,7 @@ struct dbie_op {
         int K, prec, splits, variant;
         hipGraphExec_t exec;    // null: seen once, not captured yet
         void *part, *scratch, *xt;  // workspaces the graph writes; a reallocation retires it
+        void *gst;              // the stored kernel matrix the graph reads (or null); a change retires it
         uint64_t last_use;
     };
     rocblas_handle blas = nullptr;  // on the operator's stream (block GMRES), created on first use
@@ -571,8 +582,10 @@ TuneKey tune_key(const dbie_op *op, int K, dbie_precision prec, bool exclude) {
 }
 
 // Candidate launch shapes for one apply: split counts the launch accepts whose partial sums fit
-// in 2 GB, each with the default kernel and (split-FP16, K > 128) the general kernel and the
-// warp-specialised kernel with one consumer wave per SIMD.
+// in 2 GB, each with the default kernel and (split-FP16, K > 128) the general kernel, the
+// warp-specialised kernel with one consumer wave per SIMD and, at one or two splits, the GEMM over
+// the stored kernel matrix with four or eight waves per workgroup (an operator that keeps no
+// stored matrix runs those as the warp-specialised kernel).
 std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
     std::vector<FarTune> cand;
     const int smax = far_field_max_splits(N);
@@ -583,6 +596,11 @@ std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
         if (ws) {
             cand.push_back(FarTune{sp, 0});
             cand.push_back(FarTune{sp, 1});
+            // the stored kernel matrix: 256-target workgroups fill the CUs at one or two splits
+            if (sp <= 2) {
+                cand.push_back(FarTune{sp, 2});
+                cand.push_back(FarTune{sp, 3});
+            }
         }
     }
     return cand;
@@ -649,7 +667,9 @@ void apply_graphed(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
         return;
     }
     using AG = dbie_op::ApplyGraph;
-    auto live = [&](const AG &a) { return a.part == op->part.p && a.scratch == op->mscratch.p && a.xt == op->xt32.p; };
+    auto live = [&](const AG &a) {
+        return a.part == op->part.p && a.scratch == op->mscratch.p && a.xt == op->xt32.p && a.gst == op->gst.p;
+    };
     AG *hit = nullptr;
     for (size_t i = 0; i < op->agraphs.size();) {
         AG &a = op->agraphs[i];
@@ -680,7 +700,7 @@ void apply_graphed(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
             op->agraphs.pop_back();
         }
         op->agraphs.push_back(AG{X, ldx, Y, ldy, K, (int)prec, tune.splits, tune.variant, nullptr, nullptr, nullptr,
-                                 nullptr, ++op->graph_tick});
+                                 nullptr, nullptr, ++op->graph_tick});
         return;
     }
     // second sighting: capture, instantiate, launch
@@ -709,6 +729,7 @@ void apply_graphed(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
     hit->part = op->part.p;
     hit->scratch = op->mscratch.p;
     hit->xt = op->xt32.p;
+    hit->gst = op->gst.p;
     hit->last_use = ++op->graph_tick;
     ++op->graph_captures;
     DBIE_HIP(hipGraphLaunch(exec, op->stream));
@@ -841,8 +862,14 @@ void apply_core(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int
             if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
         }
         exclude = excludes_near(op, K, prec);
+        FarStored fs;
+        if (op->gst_state == 1 && op->gst_excl == exclude) {
+            fs.g = op->gst.p;
+            fs.tile_off = op->gst_off.as<int>();
+            fs.ntile = op->gst_ntile;
+        }
         S = far_field_mfma(op->stream, op->dg, (int)prec, K, X, ldx, ws.part, ws.part_cap, ws.mscratch,
-                           op->f16_gscale, exclude, e0, e1, tune);
+                           op->f16_gscale, exclude, e0, e1, tune, fs.g ? &fs : nullptr);
     } else {
         ProfScope ps(op, prec == DBIE_FP64 ? 0 : 1);
         if (prec == DBIE_FP64) {
@@ -865,10 +892,61 @@ void apply_core(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int
     DBIE_HIP(hipGetLastError());
 }
 
+// Builds the operator's stored kernel matrix (split-FP16 far field) when free device memory holds it
+// with a reserve of max(8 GiB, 1/16 of the device) to spare; otherwise the operator declines it for
+// good.  Never under stream capture (the first, eager apply of a launch shape builds it).
+void build_stored(dbie_op *op, bool exclude) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(op->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return;
+    }
+    std::vector<int> off;
+    const int64_t nblk = far_stored_layout(op->tile_plane_h, op->N(), off);
+    const size_t bytes = (size_t)nblk * far_stored_block_bytes();
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        op->gst_state = -1;
+        return;
+    }
+    const size_t reserve = std::max<size_t>((size_t)8 << 30, total_b / 16);
+    if (nblk <= 0 || bytes + reserve > free_b) {
+        op->gst_state = -1;
+        return;
+    }
+    try {
+        op->gst.ensure(bytes);
+        upload(op->gst_off, off.data(), off.size(), op->stream);
+    } catch (const Error &) {
+        op->gst.release();
+        op->gst_off.release();
+        op->gst_state = -1;
+        return;
+    }
+    op->gst_ntile = (int)((op->N() + 255) / 256);
+    FarStored fs;
+    fs.g = op->gst.p;
+    fs.tile_off = op->gst_off.as<int>();
+    fs.ntile = op->gst_ntile;
+    far_stored_build(op->stream, op->dg, op->f16_gscale, exclude, fs);
+    DBIE_HIP(hipGetLastError());
+    op->gst_excl = exclude;
+    op->gst_state = 1;
+}
+
 void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int64_t ldy, dbie_precision prec,
                    const FarTune &tune_in) {
     const int64_t N = op->N();
     const FarTune tune = clamp_tune(N, K, tune_in);
+    if (prec == DBIE_FP16X3 && tune.variant >= 2 && K > 128 && op->stored_mode == 1 && op->gst_state == 0) {
+        if (op->f16_gscale == 0.0f) {
+            op->gmax.ensure(sizeof(unsigned));
+            op->f16_gscale = far_field_f16_scale(op->stream, op->dg, op->gmax.p);
+            if (op->f16_gscale == 0.0f) DBIE_HIP(hipGetLastError());
+        }
+        build_stored(op, excludes_near(op, K, prec));
+    }
     const int64_t need = apply_part_need(N, K, tune);
     op->part.ensure(need * sizeof(double));
     ApplyWs ws;
@@ -2104,6 +2182,8 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         {
             const char *e = std::getenv("DBIE_GRAPHS");
             op->graphs = !(e && e[0] == '0');
+            const char *f = std::getenv("DBIE_STORED_FAR");
+            op->stored_mode = (f && f[0] == '0') ? 0 : 1;
         }
         hipStream_t st = op->stream;
         const int64_t N = g.N;
@@ -2135,6 +2215,7 @@ dbie_status dbie_operator_create(const dbie_geom *gh, const double *eps_in, cons
         upload(op->wf, fw.data(), N, st);
         const std::vector<int> tp = tile_planes(g);
         upload(op->plane, tp.data(), tp.size(), st);
+        op->tile_plane_h = tp;
         // near-field corrections
         BlockSparse bs;
         bs.bd = g.npp();
@@ -2358,7 +2439,7 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
                               &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_vh, &op->bsr_rexp, &op->sr_ptr,
                               &op->sr_col, &op->sr_blk, &op->bsr_d64,
                               &op->nmask, &op->part,
-                              &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32})
+                              &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32, &op->gst, &op->gst_off})
             b += (int64_t)d->bytes;
         // the Krylov workspaces the operator keeps for later solves
         const GmresWork &w = op->gw;
@@ -2429,7 +2510,7 @@ dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision
     return guard([&] {
         check_op(op);
         if (K < 1 || splits < 1) fail(DBIE_BAD_ARG, "K and splits must be >= 1");
-        if (variant < -1 || variant > 1) fail(DBIE_BAD_ARG, "variant must be -1, 0 or 1");
+        if (variant < -1 || variant > 3) fail(DBIE_BAD_ARG, "variant must be -1 ... 3");
         check_precision(precision);
         TuneState ts;
         ts.cand = {FarTune{splits, variant}};
@@ -2452,6 +2533,23 @@ dbie_status dbie_op_graphs(dbie_op *op, int enable, int64_t *captures, int64_t *
         }
         if (captures) *captures = op->graph_captures;
         if (replays) *replays = op->graph_replays;
+    });
+}
+
+dbie_status dbie_op_stored_far(dbie_op *op, int mode, int64_t *bytes) {
+    return guard([&] {
+        check_op(op);
+        if (mode < -1 || mode > 1) fail(DBIE_BAD_ARG, "mode must be -1, 0 or 1");
+        if (mode >= 0 && mode != op->stored_mode) {
+            DBIE_HIP(hipStreamSynchronize(op->stream));
+            op->stored_mode = mode;
+            if (mode == 0) {  // graphs that read the matrix retire when it goes (their liveness check)
+                op->gst.release();
+                op->gst_off.release();
+            }
+            op->gst_state = 0;
+        }
+        if (bytes) *bytes = op->gst_state == 1 ? (int64_t)op->gst.bytes : 0;
     });
 }
 

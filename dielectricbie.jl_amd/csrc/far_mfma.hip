@@ -395,6 +395,7 @@ __global__ __launch_bounds__(kThreads) void far_mfma_bf16_kernel(DevGeom g, int 
 // taken relative to the tile's first target in FP64 and then rounded to FP32.  Range: entries are scaled by 2^-eG (|G| <= 1/r_min^2 <= 2^14 after scaling)
 // and each column of w.*X by 2^e_k (max |.| in [2^13, 2^14)); both are undone on output.
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef _Float16 v2h __attribute__((ext_vector_type(2)));
 
 __global__ void zero_u32_kernel(unsigned *__restrict__ p, int n) {
@@ -715,8 +716,8 @@ constexpr int kWsMaskWords = 512;  // near-mask words per target panel staged in
 // __syncthreads would wait for every VM operation in flight.
 template <int VM>  // VM = 63: no wait on VM operations
 __device__ inline void ws_producer_barrier() {
-    static_assert((VM >= 0 && VM < 16) || VM == 63, "vmcnt field");
-    __builtin_amdgcn_s_waitcnt(VM == 63 ? 0xC07F : 0x0070 | VM);
+    static_assert(VM >= 0 && VM <= 63, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt(0x0070 | (VM & 15) | ((VM >> 4) << 14));
     asm volatile("s_barrier" ::: "memory");
 }
 
@@ -990,6 +991,417 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
     }
 }
 
+// ---------------------------------------------------------------- stored split-FP16 far field
+// The split-FP16 far field of a 256-column block moves five times more MFMA operand bytes per stage
+// through the LDS than it generates entries, and the generation (about 18 VALU instructions per
+// entry) competes with the MFMAs for every SIMD.  With 288 GB of HBM the kernel matrix itself fits:
+// at N = 64,384 its cross-plane entries are 3.4e9 hi/lo fp16 pairs, 13.6 GB.  far_g_build_kernel
+// writes them once per operator, exactly as the generating kernels compute them, and
+// far_gs_f16x3_kernel streams them as the A operand of an MFMA GEMM: no VALU work, the HBM stream
+// of the stored entries (4 bytes per pair and apply) against the three products of the split.
+//
+// Storage: per 256-target super-tile T (two 128-target halves, each generated relative to its own
+// first target as the 128-target kernels do) and per 32-source stage of each source tile of 256
+// nodes that the stage walk does not skip for T (same-plane tiles hold exact zeros), one block of
+// 16 target slices x hi/lo x 64 lanes x 8 halves (32 KiB) in MFMA A-operand order.  tile_off[T *
+// (ntile + 1) + st] counts the stored source tiles of T before source tile st; block b of tile
+// st's stages is at tile_off[...] * 8 + (stage within the tile).
+constexpr int kGsT = 256;                // targets per super-tile
+constexpr int kGsChunks = 32;            // 1 KiB chunks per block: [slice][hi/lo]
+constexpr int kGsBlockU4 = kGsChunks * 64;
+constexpr int kGsBlockBytes = kGsBlockU4 * 16;
+
+// grid (super-tiles, source tiles), 8 waves: wave w generates target slices 2 w, 2 w + 1 of the
+// super-tile for the 8 stages of source tile blockIdx.y.  EXCL as in far_ws_f16x3_kernel (1: npp =
+// 16, a zero normal per 8-source group; 2: per source), so that every stored entry equals the one the
+// generating kernel computes for the same pair, bit for bit.
+template <int EXCL>
+__global__ __launch_bounds__(512) void far_g_build_kernel(DevGeom g, float gscale, NearMask nm, uint4 *__restrict__ gst,
+                                                          const int *__restrict__ tile_off, int ntile) {
+    const int T = blockIdx.x, st = blockIdx.y;
+    const int64_t N = g.N;
+    const int tpid = g.tile_plane ? g.tile_plane[T] : -1;
+    if (tpid >= 0 && g.tile_plane[st] == tpid) return;
+    const int64_t blk0 = (int64_t)tile_off[(int64_t)T * (ntile + 1) + st] * 8;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int h = w >> 2, pw = w & 3;  // 128-target half, producer slot within it (slices 2 pw, 2 pw + 1)
+    const int64_t i0 = (int64_t)T * kGsT + 128 * h;
+    const int64_t io = i0 < N ? i0 : N - 1;
+    const double ox = g.x[io], oy = g.y[io], oz = g.z[io];
+    float tx[2], ty[2], tz[2], tn0[2], tn1[2], tn2[2];
+    int64_t tpan[2] = {0, 0};  // target panel of the mask row
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        int64_t tg = i0 + (2 * pw + u) * 16 + (lane & 15);
+        tg = tg < N ? tg : N - 1;
+        tx[u] = (float)(g.x[tg] - ox);
+        ty[u] = (float)(g.y[tg] - oy);
+        tz[u] = (float)(g.z[tg] - oz);
+        tn0[u] = g.nxf[tg] * gscale;
+        tn1[u] = g.nyf[tg] * gscale;
+        tn2[u] = g.nzf[tg] * gscale;
+        if (EXCL == 1) {
+            const int64_t ts = i0 + (2 * pw + u) * 16;
+            tpan[u] = (ts < N ? ts : N - 1) / nm.npp;
+        } else if (EXCL == 2) {
+            tpan[u] = tg / nm.npp;
+        }
+    }
+    auto bit = [&](int64_t tp, int64_t sp) -> bool {
+        return (nm.bits[tp * (int64_t)nm.mwords + (sp >> 5)] >> (sp & 31)) & 1u;
+    };
+    for (int q = 0; q < 8; ++q) {
+        const int64_t js = (int64_t)st * 256 + 32 * q;
+        if (js >= N) break;
+        const int64_t s0 = js + 8 * (lane >> 4);  // the lane's 8 sources s0 .. s0 + 7
+        float n0[2], n1[2], n2[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            n0[u] = tn0[u];
+            n1[u] = tn1[u];
+            n2[u] = tn2[u];
+            if (EXCL == 1) {
+                const bool ex = bit(tpan[u], js / nm.npp + ((lane >> 4) >> 1));
+                n0[u] = ex ? 0.f : n0[u];
+                n1[u] = ex ? 0.f : n1[u];
+                n2[u] = ex ? 0.f : n2[u];
+            }
+        }
+        unsigned hw[2][4], lw[2][4];
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+            float v[2][2];
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const int64_t j = s0 + e + d;
+                const bool in = j < N;
+                const float sx = in ? (float)(g.x[j] - ox) : 1.0e18f;
+                const float sy = in ? (float)(g.y[j] - oy) : 1.0e18f;
+                const float sz = in ? (float)(g.z[j] - oz) : 1.0e18f;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const float dx = tx[u] - sx, dy = ty[u] - sy, dz = tz[u] - sz;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-20f)));
+                    const float rinv = __builtin_amdgcn_rsqf(r2);
+                    v[u][d] = (n0[u] * dx + n1[u] * dy + n2[u] * dz) * (rinv * rinv * rinv);
+                    if (EXCL == 2) v[u][d] = (in && bit(tpan[u], j / nm.npp)) ? 0.f : v[u][d];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const v2h hh = __builtin_convertvector(v2f{v[u][0], v[u][1]}, v2h);
+                const v2f hf = __builtin_convertvector(hh, v2f);
+                const v2h ll = __builtin_convertvector(v2f{v[u][0] - hf[0], v[u][1] - hf[1]}, v2h);
+                hw[u][e / 2] = __builtin_bit_cast(unsigned, hh);
+                lw[u][e / 2] = __builtin_bit_cast(unsigned, ll);
+            }
+        }
+        uint4 *blk = gst + (blk0 + q) * kGsBlockU4;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int slice = 8 * h + 2 * pw + u;
+            blk[(slice * 2 + 0) * 64 + lane] = uint4{hw[u][0], hw[u][1], hw[u][2], hw[u][3]};
+            blk[(slice * 2 + 1) * 64 + lane] = uint4{lw[u][0], lw[u][1], lw[u][2], lw[u][3]};
+        }
+    }
+}
+
+// Streamed GEMM over the stored entries: 256 targets x 256 columns per workgroup, 32 sources per
+// stage.  CW = 4 waves (one per SIMD, 2 x 2: 128 targets x 128 columns each, 256 accumulator
+// registers) or CW = 8 (two per SIMD, 2 x 4: 128 targets x 64 columns).  Every wave copies its share
+// of the stage's stored block (32 KiB, global -> LDS, three-deep ring: issued two stages ahead) and
+// of the B operands (the far field's fp16 hi/lo split of w.*X, 32 KiB, two-deep: one stage ahead),
+// then multiplies from LDS.  Each ring slot is its own LDS variable and the stage loop is unrolled
+// over the six slot combinations, so that the compiler sees that a copy in flight never writes
+// what the wave reads.  One barrier per stage waits for the copies issued a stage earlier.  The
+// accumulation order per output (stage by stage: hi*hi, hi*lo, lo*hi) is that of
+// far_ws_f16x3_kernel, so with the same source split the partial sums are bit-identical.
+template <int CW>
+__global__ __launch_bounds__(CW * 64) void far_gs_f16x3_kernel(const int *__restrict__ tplane, int64_t N, int K, int Kp,
+                                                               const _Float16 *__restrict__ zh,
+                                                               const _Float16 *__restrict__ zl,
+                                                               const unsigned *__restrict__ cmax, float gscale,
+                                                               double *__restrict__ part, int src_per_split,
+                                                               const uint4 *__restrict__ gst,
+                                                               const int *__restrict__ tile_off, int ntile) {
+    static_assert(CW == 4 || CW == 8, "4 or 8 waves");
+    constexpr int NCB = CW == 4 ? 8 : 4;  // 16-column blocks per wave
+    constexpr int NC = kGsChunks / CW;    // chunks of A and of B each wave copies per stage
+    __shared__ uint4 s_a0[kGsChunks][64], s_a1[kGsChunks][64], s_a2[kGsChunks][64];
+    __shared__ uint4 s_b0[kGsChunks][64], s_b1[kGsChunks][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int th = CW == 4 ? wave >> 1 : wave >> 2;  // target half: slices 8 th .. 8 th + 7
+    const int cg = CW == 4 ? wave & 1 : wave & 3;    // column blocks NCB cg .. NCB cg + NCB - 1
+    const int T = blockIdx.x;
+    const int k0 = blockIdx.y * 256;
+    const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
+    const StageWalk walk{min(N, jb0 + src_per_split), tplane ? tplane[T] : -1, tplane};
+    auto after = [&](int64_t j) { return j < walk.jend ? walk.next(j + kWsSb) : walk.jend; };
+    const int *toff = tile_off + (int64_t)T * (ntile + 1);
+    const int t0 = toff[0];
+    typedef __attribute__((address_space(3))) void *lptr_t;
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(gst + (int64_t)t0 * 8 * kGsBlockU4), (short)0, (toff[ntile] - t0) * 8 * kGsBlockBytes, 0x00020000);
+    const int zbytes = (int)(((N + kSb - 1) / kSb) * Kp * kSb * 2);  // the host keeps this below 2^31
+    const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc((void *)zh, (short)0, zbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void *)zl, (short)0, zbytes, 0x00020000);
+    auto copy_a = [&](int64_t js, uint4 (&sa)[kGsChunks][64]) {
+        const int soff = ((toff[js / 256] - t0) * 8 + (int)((js % 256) / 32)) * kGsBlockBytes;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int ch = NC * wave + c;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lptr_t)&sa[ch][0], 16, ch * 1024 + lane * 16, soff, 0, 0);
+        }
+    };
+    auto copy_b = [&](int64_t js, uint4 (&sb)[kGsChunks][64]) {
+        const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int ch = NC * wave + c;  // column block ch / 2, hi (even) or lo (odd)
+            const int loff = (((ch >> 1) * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds((c & 1) ? rl : rh, (lptr_t)&sb[ch][0], 16, loff, soff, 0, 0);
+        }
+    };
+    v4f acc[8][NCB];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int q = 0; q < NCB; ++q) acc[r][q] = v4f{0.f, 0.f, 0.f, 0.f};
+    // The A operands of target slice r + 1 are read into the other register pair before the MFMAs of
+    // slice r are issued (scheduling barriers keep that order), so no slice waits on its LDS read.
+    auto compute = [&](const uint4 (&sa)[kGsChunks][64], const uint4 (&sb)[kGsChunks][64]) {
+        uint4 bh[NCB], bl[NCB], ah[2], al[2];
+        ah[0] = sa[(8 * th) * 2 + 0][lane];
+        al[0] = sa[(8 * th) * 2 + 1][lane];
+#pragma unroll
+        for (int q = 0; q < NCB; ++q) {
+            bh[q] = sb[(NCB * cg + q) * 2 + 0][lane];
+            bl[q] = sb[(NCB * cg + q) * 2 + 1][lane];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if (r < 7) {
+                ah[(r + 1) & 1] = sa[(8 * th + r + 1) * 2 + 0][lane];
+                al[(r + 1) & 1] = sa[(8 * th + r + 1) * 2 + 1][lane];
+            }
+            const v8h xh = __builtin_bit_cast(v8h, ah[r & 1]);
+            const v8h xl = __builtin_bit_cast(v8h, al[r & 1]);
+#pragma unroll
+            for (int q = 0; q < NCB; ++q) {
+                v4f t = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, __builtin_bit_cast(v8h, bh[q]), acc[r][q], 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, __builtin_bit_cast(v8h, bl[q]), t, 0, 0, 0);
+                acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, __builtin_bit_cast(v8h, bh[q]), t, 0, 0, 0);
+            }
+            // this region: the LDS reads (slice 0: A(0), B, A(1); later slices: A(r + 1)) ahead of
+            // the slice's MFMAs
+            if (r == 0) __builtin_amdgcn_sched_group_barrier(0x100, 4 + 2 * NCB, 0);
+            else if (r < 7) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 3 * NCB, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    int64_t jc = walk.next(jb0);
+    int64_t j1 = after(jc);
+    int64_t j2 = after(j1);
+    // The copies are issued without branches (a stage past the walk copies the current stage again
+    // into a slot no later stage reads): the compiler's wait analysis then keeps the copies in
+    // flight apart from the LDS reads of other slots across every join of the control flow.
+    if (jc < walk.jend) {
+        copy_a(jc, s_a0);
+        copy_b(jc, s_b0);
+        copy_a(j1 < walk.jend ? j1 : jc, s_a1);
+        ws_producer_barrier<NC>();  // stage jc in LDS; the next stage's block may fly
+    }
+    // one stage: A in slot a, B in slot b; copies of B(j1) into the other B slot and of A(j2) into
+    // slot (a + 2) % 3, both read last in the previous stage (every wave has passed its barrier)
+    auto stage = [&](uint4 (&sa)[kGsChunks][64], uint4 (&sb)[kGsChunks][64], uint4 (&na)[kGsChunks][64],
+                     uint4 (&nb)[kGsChunks][64]) {
+        copy_b(j1 < walk.jend ? j1 : jc, nb);
+        copy_a(j2 < walk.jend ? j2 : jc, na);
+        // the copies go out first: the barrier at the end of the stage waits for them
+        __builtin_amdgcn_sched_barrier(0);
+        compute(sa, sb);
+        // keep this stage's LDS reads on this side of the barrier
+        __builtin_amdgcn_sched_barrier(0);
+        ws_producer_barrier<NC>();  // A(j1), B(j1) landed; A(j2) may fly
+        jc = j1;
+        j1 = j2;
+        j2 = after(j2);
+        return jc < walk.jend;
+    };
+    while (jc < walk.jend) {
+        if (!stage(s_a0, s_b0, s_a2, s_b1)) break;
+        if (!stage(s_a1, s_b1, s_a0, s_b0)) break;
+        if (!stage(s_a2, s_b0, s_a1, s_b1)) break;
+        if (!stage(s_a0, s_b1, s_a2, s_b0)) break;
+        if (!stage(s_a1, s_b0, s_a0, s_b1)) break;
+        if (!stage(s_a2, s_b1, s_a1, s_b0)) break;
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F & ~0xC00F);  // vmcnt(0): no copy outlives the loop
+    const double out_scale = kInv4Pi / (double)gscale;
+#pragma unroll
+    for (int q = 0; q < NCB; ++q) {
+        const int col = k0 + (NCB * cg + q) * 16 + (lane & 15);
+        if (col >= K) continue;
+        const double cs = ldexp(out_scale, -col_exponent(cmax[col]));
+        double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = (int64_t)T * kGsT + (8 * th + r) * 16 + 4 * (lane >> 4) + e;
+                if (i < N) dst[i] = (double)acc[r][q][e] * cs;
+            }
+    }
+}
+
+// Streamed GEMM with the stored entries loaded straight into registers: 256 targets x 256 columns
+// per workgroup, four waves (one per SIMD); wave w owns target slices 4 w .. 4 w + 3 (64 targets) and
+// all 16 column blocks (256 accumulator registers).  A wave's part of a stage's block (its 4 slices,
+// hi and lo: 8 KiB, contiguous) is used by that wave alone, so it goes from HBM into registers with
+// non-temporal loads three stages ahead (four register sets), without passing through the LDS; only
+// the B operands, which all four waves read, are copied global -> LDS, into a four-deep ring, also
+// three stages ahead.  The copies of the LDS path then carry 32 KiB per stage instead of 64, and each
+// CU keeps about 190 KiB of loads in flight.  Accumulation order per output as in
+// far_ws_f16x3_kernel (bit-identical partial sums at the same source split).
+constexpr int kGrWaves = 4;
+__global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
+    const int *__restrict__ tplane, int64_t N, int K, int Kp, const _Float16 *__restrict__ zh,
+    const _Float16 *__restrict__ zl, const unsigned *__restrict__ cmax, float gscale, double *__restrict__ part,
+    int src_per_split, const uint4 *__restrict__ gst, const int *__restrict__ tile_off, int ntile) {
+    constexpr int NC = kGsChunks / kGrWaves;  // B chunks each wave copies per stage (8)
+    __shared__ uint4 s_b0[kGsChunks][64], s_b1[kGsChunks][64], s_b2[kGsChunks][64], s_b3[kGsChunks][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int T = blockIdx.x;
+    const int k0 = blockIdx.y * 256;
+    const int64_t jb0 = (int64_t)blockIdx.z * src_per_split;
+    const StageWalk walk{min(N, jb0 + src_per_split), tplane ? tplane[T] : -1, tplane};
+    auto after = [&](int64_t j) { return j < walk.jend ? walk.next(j + kWsSb) : walk.jend; };
+    const int *toff = tile_off + (int64_t)T * (ntile + 1);
+    const int t0 = toff[0];
+    // A through a buffer resource over this super-tile's blocks: the per-stage offset is a scalar,
+    // the per-lane offset fixed, so no vector address is computed (or waited for) per stage
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(gst + (int64_t)t0 * 8 * kGsBlockU4), (short)0, (toff[ntile] - t0) * 8 * kGsBlockBytes, 0x00020000);
+    const int aoff = ((NC * wave) * 64 + lane) * 16;
+    typedef __attribute__((address_space(3))) void *lptr_t;
+    const int zbytes = (int)(((N + kSb - 1) / kSb) * Kp * kSb * 2);  // the host keeps this below 2^31
+    const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc((void *)zh, (short)0, zbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void *)zl, (short)0, zbytes, 0x00020000);
+    auto copy_b = [&](int64_t js, uint4 (&sb)[kGsChunks][64]) {
+        const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int ch = NC * wave + c;  // column block ch / 2, hi (even) or lo (odd)
+            const int loff = (((ch >> 1) * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds((c & 1) ? rl : rh, (lptr_t)&sb[ch][0], 16, loff, soff, 0, 0);
+        }
+    };
+    // chunk c of this wave: target slice 4 wave + c / 2, hi (c even) or lo (c odd)
+    auto load_a = [&](int64_t js, uint4 (&a)[8]) {
+        const int soff = ((toff[js / 256] - t0) * 8 + (int)((js % 256) / 32)) * kGsBlockBytes;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)  // non-temporal (aux 2): the stream is read once
+            a[c] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, aoff + c * 1024, soff, 2));
+    };
+    v4f acc[4][16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[r][q] = v4f{0.f, 0.f, 0.f, 0.f};
+    // column blocks in pairs: the B operands of the next pair are read before this pair's 24 MFMAs
+    auto compute = [&](const uint4 (&sb)[kGsChunks][64], const uint4 (&a)[8]) {
+        uint4 bb[2][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bb[0][u] = sb[u][lane];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            if (p < 7) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) bb[(p + 1) & 1][u] = sb[4 * (p + 1) + u][lane];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = 2 * p + h;
+                const v8h bh = __builtin_bit_cast(v8h, bb[p & 1][2 * h]);
+                const v8h bl = __builtin_bit_cast(v8h, bb[p & 1][2 * h + 1]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const v8h xh = __builtin_bit_cast(v8h, a[2 * r]);
+                    const v8h xl = __builtin_bit_cast(v8h, a[2 * r + 1]);
+                    v4f t = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh, acc[r][q], 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl, t, 0, 0, 0);
+                    acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh, t, 0, 0, 0);
+                }
+            }
+            if (p == 0) __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+            else if (p < 7) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    int64_t jc = walk.next(jb0);
+    int64_t j1 = after(jc);
+    int64_t j2 = after(j1);
+    int64_t j3 = after(j2);
+    uint4 a0[8], a1[8], a2[8], a3[8];
+    // Loads are issued without branches (a stage past the walk loads the current stage again into a
+    // slot no later stage reads), so that the compiler's wait analysis stays exact across the loop.
+    if (jc < walk.jend) {
+        copy_b(jc, s_b0);
+        load_a(jc, a0);
+        copy_b(j1 < walk.jend ? j1 : jc, s_b1);
+        load_a(j1 < walk.jend ? j1 : jc, a1);
+        copy_b(j2 < walk.jend ? j2 : jc, s_b2);
+        load_a(j2 < walk.jend ? j2 : jc, a2);
+        // a copy into the fourth slot too (overwritten, in issue order, by the first stage's copy):
+        // the loop is then entered with copies pending into the four slots in the order a loop
+        // iteration leaves them, and the compiler's wait analysis keeps them apart from the reads
+        copy_b(jc, s_b3);
+        ws_producer_barrier<5 * NC>();  // B(jc), A(jc) in LDS / registers; later loads may fly
+    }
+    // one stage: B in ring slot b, A in register set a; B(j3) into slot (b + 3) % 4 (read last in the
+    // previous stage, before its barrier) and A(j3) into register set (a + 3) % 4 (consumed last in
+    // the previous stage)
+    auto stage = [&](const uint4 (&sb)[kGsChunks][64], const uint4 (&a)[8], uint4 (&nb)[kGsChunks][64],
+                     uint4 (&na)[8]) {
+        const int64_t jn = j3 < walk.jend ? j3 : jc;
+        copy_b(jn, nb);
+        load_a(jn, na);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(sb, a);
+        __builtin_amdgcn_sched_barrier(0);
+        ws_producer_barrier<5 * NC>();  // B(j1) landed; A(j1), B(j2), A(j2), B(j3), A(j3) may fly
+        jc = j1;
+        j1 = j2;
+        j2 = j3;
+        j3 = after(j3);
+        return jc < walk.jend;
+    };
+    while (jc < walk.jend) {
+        if (!stage(s_b0, a0, s_b3, a3)) break;
+        if (!stage(s_b1, a1, s_b0, a0)) break;
+        if (!stage(s_b2, a2, s_b1, a1)) break;
+        if (!stage(s_b3, a3, s_b2, a2)) break;
+    }
+    __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0): no load outlives the loop
+    const double out_scale = kInv4Pi / (double)gscale;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int col = k0 + q * 16 + (lane & 15);
+        if (col >= K) continue;
+        const double cs = ldexp(out_scale, -col_exponent(cmax[col]));
+        double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = (int64_t)T * kGsT + (4 * wave + r) * 16 + 4 * (lane >> 4) + e;
+                if (i < N) dst[i] = (double)acc[r][q][e] * cs;
+            }
+    }
+}
+
 // 16-column blocks per workgroup for K columns (power of two, at most 16)
 int ncb_for(int K) {
     const int nb = (K + 15) / 16;
@@ -1058,6 +1470,37 @@ float far_field_f16_scale(hipStream_t st, const DevGeom &g, void *tmp4) {
     return std::ldexp(1.0f, 14 - e);  // scaled entries stay below 2^14
 }
 
+int64_t far_stored_layout(const std::vector<int> &tile_plane, int64_t N, std::vector<int> &tile_off) {
+    const int nt = (int)((N + 255) / 256);
+    tile_off.assign((size_t)nt * (nt + 1), 0);
+    int64_t c = 0;
+    for (int T = 0; T < nt; ++T) {
+        const int tp = (size_t)T < tile_plane.size() ? tile_plane[T] : -1;
+        int *row = &tile_off[(size_t)T * (nt + 1)];
+        for (int st = 0; st < nt; ++st) {
+            row[st] = (int)c;
+            if (!(tp >= 0 && (size_t)st < tile_plane.size() && tile_plane[st] == tp)) ++c;
+        }
+        row[nt] = (int)c;
+    }
+    return c * 8;
+}
+
+size_t far_stored_block_bytes() { return (size_t)kGsBlockBytes; }
+
+void far_stored_build(hipStream_t st, const DevGeom &g, float gscale, bool exclude_near, const FarStored &fs) {
+    NearMask nm;
+    nm.bits = g.near_mask;
+    nm.npp = g.npp;
+    nm.mwords = g.mask_words;
+    const int ex = (!exclude_near || !g.near_mask) ? 0 : (g.npp == 16 ? 1 : 2);
+    const dim3 grid((unsigned)fs.ntile, (unsigned)fs.ntile);
+    uint4 *gst = static_cast<uint4 *>(const_cast<void *>(fs.g));
+    if (ex == 1) far_g_build_kernel<1><<<grid, 512, 0, st>>>(g, gscale, nm, gst, fs.tile_off, fs.ntile);
+    else if (ex == 2) far_g_build_kernel<2><<<grid, 512, 0, st>>>(g, gscale, nm, gst, fs.tile_off, fs.ntile);
+    else far_g_build_kernel<0><<<grid, 512, 0, st>>>(g, gscale, nm, gst, fs.tile_off, fs.ntile);
+}
+
 bool far_field_mfma_supported(int prec, int K) {
     if (K < 16) return false;
     return prec == 0 || prec == 2 || prec == 3 || prec == 4;
@@ -1088,7 +1531,7 @@ int far_field_max_splits(int64_t N) { return (int)std::max<int64_t>(1, N / 256);
 
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
                    int64_t part_cap, void *scratch, float gscale, bool exclude_near, hipEvent_t ev0, hipEvent_t ev1,
-                   const FarTune &tune) {
+                   const FarTune &tune, const FarStored *fs) {
     auto mark = [&](hipEvent_t e) {
         if (e) (void)hipEventRecord(e, st);
     };
@@ -1133,6 +1576,20 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         const int ex = (!exclude_near || !g.near_mask) ? 0 : (g.npp == 16 ? 1 : 2);
         mark(ev0);
         const int64_t mask_words_split = (per / std::max(g.npp, 1)) / 32 + 2;
+        // the stored kernel matrix (variants 2 and 3), where the operator keeps one
+        if (tune.variant >= 2 && fs && fs->g && ncb == 16 && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
+            fs->ntile == (int)((N + kGsT - 1) / kGsT)) {
+            const dim3 gg((unsigned)fs->ntile, (unsigned)by, (unsigned)S);
+            const uint4 *gst = static_cast<const uint4 *>(fs->g);
+            if (tune.variant == 2)
+                far_gr_f16x3_kernel<<<gg, kGrWaves * 64, 0, st>>>(g.tile_plane, N, K, Kp, zh, zl, cmax, gscale, part,
+                                                                  (int)per, gst, fs->tile_off, fs->ntile);
+            else
+                far_gs_f16x3_kernel<8><<<gg, 512, 0, st>>>(g.tile_plane, N, K, Kp, zh, zl, cmax, gscale, part, (int)per,
+                                                           gst, fs->tile_off, fs->ntile);
+            mark(ev1);
+            return S;
+        }
         // the warp-specialised kernel: exclusion per lane (npp = 16) or per source for panels of at
         // least 19 nodes (the 128 targets of a tile then touch at most 8 target panels)
         if (tune.variant != 0 && ncb == 16 && (ex != 2 || g.npp >= 19) && zelems * sizeof(_Float16) < ((size_t)1 << 31) &&
@@ -1140,7 +1597,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
 #define DBIE_WS(E, C)                                                                                         \
     far_ws_f16x3_kernel<E, C><<<grid, ws_threads<C>(), 0, st>>>(g, K, Kp, zh, zl, cmax, gscale, part, (int)per, nm, \
                                                                g.tile_plane)
-            if (tune.variant == 1) {
+            if (tune.variant >= 1) {  // 2, 3 without a stored kernel matrix: one consumer wave per SIMD
                 if (ex == 2) DBIE_WS(2, 4);
                 else if (ex == 1) DBIE_WS(1, 4);
                 else DBIE_WS(0, 4);

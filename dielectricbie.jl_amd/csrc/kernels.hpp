@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace dbie {
 
@@ -52,11 +53,28 @@ enum FarMode { FAR_DT = 0, FAR_S = 1, FAR_D = 2 };
 // Launch shape of a far-field contraction, as picked by the autotuner: splits = number of
 // source splits (0: the built-in heuristic), variant = -1: the warp-specialised split-FP16
 // kernel where it applies (two consumer waves per SIMD), 1: the same with one consumer wave per
-// SIMD, 0: never.
+// SIMD, 0: never, 2 / 3: the streamed GEMM over the stored kernel matrix (four / eight waves per
+// workgroup) where the operator keeps one, else as 1.
 struct FarTune {
     int splits = 0;
     int variant = -1;
 };
+
+// The split-FP16 far field's kernel matrix kept in HBM (launch variants 2: four waves, 3: eight
+// waves per workgroup; K > 128): per 256-target super-tile and per 32-source stage of each source
+// tile of 256 nodes that is not on the super-tile's plane, a 32 KiB block of hi/lo fp16 entries in
+// MFMA operand order, as the generating kernels compute them.  tile_off (ntile x (ntile + 1) ints,
+// device): stored source tiles before (super-tile T, source tile st) in row-major order.
+struct FarStored {
+    const void *g = nullptr;
+    const int *tile_off = nullptr;
+    int ntile = 0;
+};
+// host: fills tile_off from the plane ids of the 256-node tiles; returns the number of blocks
+int64_t far_stored_layout(const std::vector<int> &tile_plane, int64_t N, std::vector<int> &tile_off);
+size_t far_stored_block_bytes();
+// writes every stored block (gscale: the operator's fp16 scale; exclude_near as in far_field_mfma)
+void far_stored_build(hipStream_t st, const DevGeom &g, float gscale, bool exclude_near, const FarStored &fs);
 
 // Far-field partial sums: part[s][k][i] = sum over sources in split s of G(i,j) w_j X[j,k]
 // (G of the chosen mode, self pair excluded, 1/(4 pi) applied).  Returns the split count S.
@@ -73,7 +91,7 @@ int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int6
 // ev0/ev1 (optional) are recorded immediately before and after the contraction kernel.
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
                    int64_t part_cap, void *scratch, float gscale, bool exclude_near, hipEvent_t ev0 = nullptr,
-                   hipEvent_t ev1 = nullptr, const FarTune &tune = FarTune());
+                   hipEvent_t ev1 = nullptr, const FarTune &tune = FarTune(), const FarStored *fs = nullptr);
 // largest useful split count of a far-field launch (splits hold >= 256 sources)
 int far_field_max_splits(int64_t N);
 size_t far_field_mfma_scratch_bytes(int64_t N, int prec, int K);

@@ -300,6 +300,9 @@ struct dbie_server {
         if (s != DBIE_OK) msg = last_error_text();
         dbie_geom_destroy(g);
         if (s != DBIE_OK) return s;
+        // admission counts the operator's resident bytes and the Krylov workspaces only: no stored
+        // far-field matrix for served operators
+        dbie_op_stored_far(op, 0, nullptr);
         int64_t b = 0;
         dbie_op_device_bytes(op, &b);
         std::lock_guard<std::mutex> lk(mu);

@@ -138,6 +138,7 @@ SIGNATURES = {
     "dbie_autotune_choice": [_vp, C.c_int, C.c_int, _ip, _ip],
     "dbie_autotune_set": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],
     "dbie_op_graphs": [_vp, C.c_int, _i64p, _i64p],
+    "dbie_op_stored_far": [_vp, C.c_int, _i64p],
     "dbie_op_profile": [_vp, C.c_int],
     "dbie_op_profile_read": [_vp, C.c_int, _i64p, _dp],
     "dbie_apply_device": [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, C.c_int],

@@ -374,6 +374,13 @@ class BatchedDielectricOperator:
         L.check(L.lib().dbie_op_graphs(self._h, -1 if enable is None else int(bool(enable)), C.byref(c), C.byref(r)))
         return c.value, r.value
 
+    def stored_far(self, mode=None):
+        """Allow (True) or forbid (False) the stored split-FP16 kernel matrix for applies of more than
+        128 columns (None: unchanged); returns the device bytes it holds now (0 until built)."""
+        b = C.c_int64()
+        L.check(L.lib().dbie_op_stored_far(self._h, -1 if mode is None else int(bool(mode)), C.byref(b)))
+        return b.value
+
     def profile(self, enable=True):
         """Record HIP events around the far-field and combine launches (kernel timing)."""
         L.check(L.lib().dbie_op_profile(self._h, int(bool(enable))))

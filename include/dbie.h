@@ -404,7 +404,9 @@ dbie_status dbie_autotune(int enable, int clear);
 dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precision, int *splits, int *variant);
 /* pins the choice for this operator's class at K columns (e.g. one saved from an earlier run):
    splits >= 1 (clamped to what the launch accepts), variant -1 = default kernel, 0 = the general
-   split-FP16 kernel, 1 = the warp-specialised split-FP16 kernel with one consumer wave per SIMD */
+   split-FP16 kernel, 1 = the warp-specialised split-FP16 kernel with one consumer wave per SIMD,
+   2 / 3 = the GEMM over the stored split-FP16 kernel matrix with four / eight waves per workgroup
+   (see dbie_op_stored_far; as 1 where the operator keeps none) */
 dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision, int splits, int variant);
 
 /* ---- HIP graphs ------------------------------------------------------------ */
@@ -414,6 +416,16 @@ dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision
    off).  enable: 1 on, 0 off (eager launches; captured graphs stay cached), -1 unchanged.
    captures / replays (may be NULL) receive the operator's counts so far. */
 dbie_status dbie_op_graphs(dbie_op *op, int enable, int64_t *captures, int64_t *replays);
+
+/* No counterpart in the reference (its far field is FMM3D).  The split-FP16 far field of applies
+   with more than 128 columns can stream a stored copy of its kernel matrix (hi/lo fp16 entries of
+   every cross-plane node pair, 4 bytes per pair: 13.6 GB at N = 64,384) instead of generating the
+   entries on the fly; the results are bit-identical at the same launch shape.  The operator builds
+   it on the first apply whose tuned launch shape uses it, when free device memory holds it with
+   max(8 GiB, 1/16 of the device) to spare.  mode: 1 allowed (the default; environment
+   DBIE_STORED_FAR=0 starts operators with 0), 0 never (frees a stored matrix), -1 unchanged.
+   bytes (may be NULL) receives the device bytes the stored matrix holds (0 if none). */
+dbie_status dbie_op_stored_far(dbie_op *op, int mode, int64_t *bytes);
 
 /* ---- device-pointer variants (stream-ordered on the operator's stream) ---- */
 void *dbie_op_stream(dbie_op *op);
