@@ -397,15 +397,27 @@ def main():
 
     # the mixed solve in its Krylov variants (per column with and without recycling the first inner
     # solve's Krylov space in the later refinements, and block GMRES); the fastest is the headline
+    # ("unverified": the solve may end on its refined residual estimate, verify=False, as both FP64
+    # variants end on their Krylov estimates; "verified": it ends only on an FP64 residual.)  Every
+    # variant's true FP64 residual F - A S is evaluated after its timed solves (untimed), and only a
+    # variant whose every column meets the tolerance can be the headline.
+    def true_residual(S):
+        R = F - op.apply(S, "fp64")
+        return float((np.linalg.norm(R, axis=0) / np.linalg.norm(F, axis=0)).max())
+
     mixed = {}
-    for name, kr, rec in (("columns", "columns", True), ("columns_restarted", "columns", False),
-                          ("block", "block", True)):
-        kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol, krylov=kr, recycle=rec)
+    for name, kr, rec, ver in (("columns", "columns", True, False), ("columns_verified", "columns", True, True),
+                               ("columns_restarted", "columns", False, False), ("block", "block", True, False),
+                               ("block_verified", "block", True, True)):
+        kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol, krylov=kr, recycle=rec, verify=ver,
+                  check_residual=False)
         lat, wall, st = timed(kw, a.steps, a.warmup)
-        mixed[name] = {"lat": lat, "wall": wall, "st": st, "err": col_err(dS.to_host()), "kw": kw}
+        Sm = dS.to_host()
+        mixed[name] = {"lat": lat, "wall": wall, "st": st, "err": col_err(Sm), "kw": kw, "res": true_residual(Sm)}
         log(f"mixed {a.precision} ({name}): iters={st['niter']} refine={st['nrefine']} "
-            f"{np.median(lat):.1f} ms/step")
-    mixed_kr = min(mixed, key=lambda k: mixed[k]["wall"])
+            f"{np.median(lat):.1f} ms/step, true residual {mixed[name]['res']:.2e}")
+    ok = [k for k in mixed if mixed[k]["res"] <= 1e-10] or list(mixed)
+    mixed_kr = min(ok, key=lambda k: mixed[k]["wall"])
     lat, wall, st, err = (mixed[mixed_kr][k] for k in ("lat", "wall", "st", "err"))
     value = a.K * a.steps / wall
     fp64_value = a.K / (fp64_ms * 1e-3)
@@ -421,12 +433,15 @@ def main():
     # the other low precisions at the same size (per-column Krylov with the recycled space)
     variants = {}
     for prec in others:
-        kw = dict(rtol=1e-10, precision=prec, inner_rtol=a.inner_rtol, krylov="columns", recycle=True)
+        kw = dict(rtol=1e-10, precision=prec, inner_rtol=a.inner_rtol, krylov="columns", recycle=True, verify=False,
+                  check_residual=False)
         vlat, vwall, vst = timed(kw, max(a.variant_steps, 1), 1)
-        verr = col_err(dS.to_host())
+        Sv = dS.to_host()
+        verr = col_err(Sv)
         variants[prec] = {"value": a.K * len(vlat) / vwall, "ms_per_step": float(np.median(vlat)),
                           "inner_iters": vst["niter"], "refinements": vst["nrefine"],
-                          "max_rel_err_vs_fp64": float(verr.max()), "speedup_vs_fp64": a.K * len(vlat) / vwall / fp64_value}
+                          "max_rel_err_vs_fp64": float(verr.max()), "true_max_rel_residual": true_residual(Sv),
+                          "speedup_vs_fp64": a.K * len(vlat) / vwall / fp64_value}
         log(f"mixed {prec}: {variants[prec]}")
 
     # roofline of the dominant kernel (largest device time in the profiled pass)
@@ -484,14 +499,18 @@ def main():
         "fp64_value": fp64_value, "fp64_ms_per_step": fp64_ms, "fp64_iters": fp64[fp64_kr]["iters"],
         "fp64_krylov": fp64_kr, "fp64_variants": fp64, "mixed_krylov": mixed_kr,
         "mixed_variants": {k: {"ms_per_step": float(np.mean(v["lat"])), "inner_iters": v["st"]["niter"],
-                               "value": a.K * a.steps / v["wall"], "max_rel_err_vs_fp64": float(v["err"].max())}
+                               "refinements": v["st"]["nrefine"], "value": a.K * a.steps / v["wall"],
+                               "max_rel_err_vs_fp64": float(v["err"].max()), "true_max_rel_residual": v["res"],
+                               "verify": bool(v["kw"]["verify"])}
                            for k, v in mixed.items()},
         "other_precisions": variants,
         "speedup_vs_fp64": value / fp64_value,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_rel_err_vs_fp64": float(err.max()), "median_rel_err_vs_fp64": float(np.median(err)),
         "mixed_inner_iters": st["niter"], "mixed_refinements": st["nrefine"],
-        "mixed_max_rel_residual": st["max_rel_residual"], "setup_s": t_setup, "setup_ms": op.setup_ms(),
+        "mixed_max_rel_residual": mixed[mixed_kr]["res"], "mixed_residual_note": "true FP64 residual max_k "
+        "||F_k - A S_k|| / ||F_k|| of the headline variant's result, evaluated after the timed solves",
+        "setup_s": t_setup, "setup_ms": op.setup_ms(),
         "roofline": roof, "cpu_baseline": cpu, "serve": serve,
         "tuned_shapes": {p: op.tuned_shape(a.K, p) for p in ["fp64", a.precision] + others},
     }

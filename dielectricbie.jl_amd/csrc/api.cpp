@@ -1798,6 +1798,15 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 continue;
             }
             if (it >= o.max_refine || s.niter >= o.itmax) break;
+            // without the FP64 verification the solve may end on this inner solve's residual norms:
+            // once a true residual is known, run it to half of what the tolerance still allows
+            double irtol = o.inner_rtol;
+            if (!o.verify && n_exact >= 1) {
+                double r_need = o.inner_rtol;
+                for (int k = 0; k < K; ++k)
+                    if (mask[k] != 0.0 && rn[k] > 0.0) r_need = std::min(r_need, 0.5 * (o.atol + o.rtol * bnorm[k]) / rn[k]);
+                irtol = std::max(r_need, 1e-7);
+            }
             hb.ensure(sizeof(double) * K);
             h2d(hb.p, mask.data(), sizeof(double) * K, st);
             col_scale(st, dR.as<double>(), hb.as<double>(), false, dR.as<double>(), N, K);
@@ -1806,9 +1815,9 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
             if (it == 0) rec.valid = false;
             const bool recyc = o.recycle != 0 && o.krylov == 0;
             const GmresOut r =
-                o.krylov == 1 ? block_gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
+                o.krylov == 1 ? block_gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, irtol, 0.0,
                                             o.itmax - s.niter, o.restart, dinvp)
-                              : gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, o.inner_rtol, 0.0,
+                              : gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, irtol, 0.0,
                                       o.itmax - s.niter, o.restart, dinvp, recyc && it == 0 ? &rec : nullptr,
                                       recyc && it > 0 ? &rec : nullptr);
             s.niter += r.iters;
@@ -1827,6 +1836,30 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
             // orthogonality of the Krylov basis.  When those norms already meet the tolerance ten
             // times over, the update would only confirm it: go to the FP64 verification at once (a
             // failed verification costs an FP64 apply and another inner solve, hence the margin)
+            // Without verification (verify = 0) the solve ends here when every column's inner
+            // residual norm meets half the tolerance and the low-precision operator's part of the
+            // true residual, at most delta ||A|| ||D_k|| (delta: twice the operator's measured
+            // relative error, ||A|| taken as 10), stays below a quarter of it; the rest is the
+            // Krylov estimate's own error.  For split FP16 (delta 1e-8, also on the edge-localised
+            // vectors corrections are: its far field leaves the near pairs out) that allows
+            // corrections up to 2.5e-4 ||F_k|| at rtol 1e-10.  The other forms keep the near pairs'
+            // direct entries in their far field, whose rounding cancellation against the corrections
+            // amplifies about a hundredfold on such vectors (delta 1e-4): they end on an FP64 residual.
+            if (!o.verify && small && (int)r.resid.size() == K) {
+                const double delta = o.precision == DBIE_FP16X3 ? 1e-8 : o.precision == DBIE_BF16 ? 1e-1 : 1e-4;
+                bool end = true;
+                for (int k = 0; k < K && end; ++k) {
+                    const double tk = o.atol + o.rtol * bnorm[k];
+                    if (mask[k] != 0.0 && !(r.resid[k] <= 0.5 * tk && delta * 10.0 * dn[k] <= 0.25 * tk)) end = false;
+                }
+                if (end) {
+                    for (int k = 0; k < K; ++k)
+                        if (mask[k] != 0.0) rn[k] = r.resid[k];
+                    op->note(2, rn);
+                    s.solved = 1;
+                    break;
+                }
+            }
             bool est_ok = small && (int)r.resid.size() == K;
             for (int k = 0; k < K && est_ok; ++k)
                 if (mask[k] != 0.0 && !(r.resid[k] <= 0.1 * (o.atol + o.rtol * bnorm[k]))) est_ok = false;
@@ -1860,7 +1893,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     s.time_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (o.precision == DBIE_FP64 && o.check_residual) {
+    if ((o.precision == DBIE_FP64 || !o.verify) && o.check_residual) {
         dR.ensure(sizeof(double) * NK);
         col_norms(op, dF, N, K, partial, hb, bnorm);
         apply_dev(op, K, dX, N, dR.as<double>(), N, DBIE_FP64);
@@ -2037,6 +2070,7 @@ void dbie_solve_opts_default(dbie_solve_opts *o) {
     o->check_residual = 1;
     o->krylov = 0;
     o->recycle = 1;
+    o->verify = 1;
 }
 
 dbie_status dbie_gauss_legendre(int n, double *x, double *w) {

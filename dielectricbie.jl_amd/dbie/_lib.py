@@ -41,7 +41,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("itmax", C.c_int), ("restart", C.c_int),
                 ("precondition", C.c_int), ("precision", C.c_int), ("inner_rtol", C.c_double),
                 ("max_refine", C.c_int), ("check_residual", C.c_int), ("krylov", C.c_int),
-                ("recycle", C.c_int)]
+                ("recycle", C.c_int), ("verify", C.c_int)]
 
 
 class SolveStats(C.Structure):

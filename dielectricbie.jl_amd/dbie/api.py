@@ -437,12 +437,14 @@ class BatchedDielectricOperator:
 
     def solve(self, F, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precondition=True, precision="fp64",
               inner_rtol=1e-4, max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns",
-              recycle=True):
+              recycle=True, verify=True):
         """GMRES solve of A Σ = F for all columns of F on the GPU; returns (Σ, stats dict).
         ``krylov``: "columns" (one Krylov space per column, shared block matvecs) or "block"
         (one block Krylov space for all columns, as Krylov.block_gmres).  ``recycle`` (mixed
         precision, per-column Krylov): refinements after the first deflate the first inner solve's
-        Krylov space instead of starting a new one."""
+        Krylov space instead of starting a new one.  ``verify`` (mixed precision): end only on an
+        FP64 residual (True) or also on the refined residual estimate, as the FP64 GMRES ends on its
+        Krylov estimate (False; ``check_residual`` then reports the true residual)."""
         F = np.asarray(F, np.float64)
         vec = F.ndim == 1
         Ff = L.fortran(F, self.n)
@@ -456,6 +458,7 @@ class BatchedDielectricOperator:
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
         o.krylov = _krylov_code(krylov)
         o.recycle = int(bool(recycle))
+        o.verify = int(bool(verify))
         st = L.SolveStats()
         status = L.lib().dbie_solve_block(self._h, K, L.dptr(Ff), self.n, L.dptr(S), self.n, C.byref(o),
                                           C.byref(st))
@@ -480,7 +483,7 @@ class BatchedDielectricOperator:
 
     def solve_device(self, dF: "DeviceMatrix", dSigma: "DeviceMatrix", *, rtol=1e-10, atol=0.0, itmax=500,
                      restart=0, precondition=True, precision="fp64", inner_rtol=1e-4, max_refine=10,
-                     check_residual=True, raise_on_fail=True, krylov="columns", recycle=True):
+                     check_residual=True, raise_on_fail=True, krylov="columns", recycle=True, verify=True):
         """Solve with F and Σ resident in device memory (dbie_solve_block_device); returns stats."""
         if dF.shape[0] != self.n or dSigma.shape != dF.shape:
             raise ValueError("DimensionMismatch: device blocks must be N x K and equal in shape")
@@ -492,6 +495,7 @@ class BatchedDielectricOperator:
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
         o.krylov = _krylov_code(krylov)
         o.recycle = int(bool(recycle))
+        o.verify = int(bool(verify))
         st = L.SolveStats()
         status = L.lib().dbie_solve_block_device(self._h, dF.shape[1], dF.ptr, self.n, dSigma.ptr, self.n,
                                                  C.byref(o), C.byref(st))
@@ -526,7 +530,8 @@ class BatchedDielectricOperator:
 
 
 def solve_fused(ops, Fs, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precision="fp64", inner_rtol=1e-5,
-                max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns", recycle=True):
+                max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns", recycle=True,
+                verify=True):
     """Solve the block systems A_b Σ_b = F_b of several operators (different geometries) as one Krylov
     process (dbie_solve_fused): every iteration applies all operators at once on the GPU.  Returns
     ([Σ_b], stats)."""
@@ -547,6 +552,7 @@ def solve_fused(ops, Fs, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precisio
     o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
     o.krylov = _krylov_code(krylov)
     o.recycle = int(bool(recycle))
+    o.verify = int(bool(verify))
     st = L.SolveStats()
     status = L.lib().dbie_solve_fused(B, ha, L.iptr(Ka), Fa, L.i64ptr(ld), Sa, L.i64ptr(ld), C.byref(o), C.byref(st))
     if not (status == L.NOT_CONVERGED and not raise_on_fail):

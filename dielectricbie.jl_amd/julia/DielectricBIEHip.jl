@@ -41,6 +41,7 @@ struct SolveOpts
     check_residual::Cint
     krylov::Cint
     recycle::Cint
+    verify::Cint
 end
 
 mutable struct SolveStats
@@ -164,11 +165,12 @@ end
 function solve_block(op::HipDielectricOperator, F::Matrix{Float64}; rtol = 1e-10, atol = 0.0,
                      itmax = 500, precondition = true, precision = :fp64, inner_rtol = 1e-5, max_refine = 10,
                      krylov = 1,   # 1 = block GMRES as Krylov.block_gmres, 0 = per column
-                     recycle = 1)  # mixed precision, per column: later refinements deflate the first space
+                     recycle = 1,  # mixed precision, per column: later refinements deflate the first space
+                     verify = 1)   # mixed precision: 1 = end on an FP64 residual, 0 = on the Krylov estimate
     size(F, 1) == op.n || throw(DimensionMismatch("F has $(size(F, 1)) rows, operator has $(op.n)"))
     Σ = similar(F)
     o = Ref(SolveOpts(rtol, atol, itmax, 0, precondition, PRECISIONS[precision], inner_rtol, max_refine, 1,
-                      krylov, recycle))
+                      krylov, recycle, verify))
     s = Ref(SolveStats(0, 0, 0, 0.0, 0.0))
     st = ccall((:dbie_solve_block, lib), Cint,
                (Ptr{Cvoid}, Cint, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ref{SolveOpts}, Ref{SolveStats}),

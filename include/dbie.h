@@ -87,8 +87,9 @@ typedef struct {
     dbie_precision precision;/* DBIE_FP64 = plain GMRES; otherwise mixed-precision IR */
     double inner_rtol;       /* mixed: inner-solve relative tolerance (default 1e-4) */
     int max_refine;          /* mixed: maximum FP64 refinement steps (default 10) */
-    int check_residual;      /* FP64 path: evaluate the true FP64 residual after the solve
-                                (one extra matvec; fills max_rel_residual; default 1) */
+    int check_residual;      /* FP64 path (and mixed with verify = 0): evaluate the true FP64
+                                residual after the solve (one extra matvec, outside time_ms; fills
+                                max_rel_residual; default 1) */
     int krylov;              /* 0 = GMRES per column on a shared block matvec (default),
                                 1 = block GMRES: one block Krylov space for all K columns
                                 (Krylov.block_gmres, src/solver/multi_rhs.jl:144-148) */
@@ -96,6 +97,17 @@ typedef struct {
                                 deflate the first one's Krylov space (GCRO: minimise over it and
                                 keep new directions orthogonal to its image), so the refinement
                                 continues that space instead of restarting (default 1) */
+    int verify;              /* mixed precision: 1 = the solve ends only on an FP64 residual that
+                                meets the tolerance (default); 0 = it may also end, as the FP64
+                                GMRES does, on its residual estimate: after the first FP64
+                                residual the inner solve runs to half the remaining tolerance, and
+                                when every column's inner residual norm meets half the tolerance
+                                and the low-precision operator's bound on its part of the residual
+                                (2 x its relative error x 10 x ||correction||) a quarter of it, the
+                                solve ends (split FP16: corrections up to 2.5e-4 ||F|| at rtol
+                                1e-10; the forms without near-pair exclusion in practice always
+                                end on an FP64 residual); check_residual then
+                                evaluates the true FP64 residual after the solve, as for FP64 */
 } dbie_solve_opts;
 
 typedef struct {
