@@ -1771,6 +1771,9 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
         bool exact = true;
         int n_exact = 0;
         std::vector<double> dn, xn;
+        // per column: bound on |true residual - rn| from the low-precision residual updates since the
+        // last FP64 residual
+        std::vector<double> ubound(K, 0.0);
         const bool trace = std::getenv("DBIE_TRACE") != nullptr;
         auto worst = [&](const std::vector<double> &v) {
             double m = 0.0;
@@ -1793,6 +1796,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 ++s.nrefine;
                 exact = true;
                 ++n_exact;
+                std::fill(ubound.begin(), ubound.end(), 0.0);
                 col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
                 op->note(1, rn);
                 continue;
@@ -1831,26 +1835,29 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 for (int k = 0; k < K && small; ++k)
                     if (mask[k] != 0.0 && dn[k] > 1e-3 * xn[k]) small = false;
             }
-            // with small corrections the updated residual R - A_low D differs from the true one by
-            // ~delta ||A|| ||D||; its norm is the one the inner GMRES ended with, up to the loss of
-            // orthogonality of the Krylov basis.  When those norms already meet the tolerance ten
-            // times over, the update would only confirm it: go to the FP64 verification at once (a
-            // failed verification costs an FP64 apply and another inner solve, hence the margin)
-            // Without verification (verify = 0) the solve ends here when every column's inner
-            // residual norm meets half the tolerance and the low-precision operator's part of the
-            // true residual, at most delta ||A|| ||D_k|| (delta: twice the operator's measured
-            // relative error, ||A|| taken as 10), stays below a quarter of it; the rest is the
-            // Krylov estimate's own error.  For split FP16 (delta 1e-8, also on the edge-localised
-            // vectors corrections are: its far field leaves the near pairs out) that allows
-            // corrections up to 2.5e-4 ||F_k|| at rtol 1e-10.  The other forms keep the near pairs'
-            // direct entries in their far field, whose rounding cancellation against the corrections
-            // amplifies about a hundredfold on such vectors (delta 1e-4): they end on an FP64 residual.
-            if (!o.verify && small && (int)r.resid.size() == K) {
+            // With small corrections the updated residual R - A_low D differs from the true one by at
+            // most delta ||A|| ||D|| (delta: twice the low-precision operator's measured relative
+            // error, ||A|| taken as 10), and these bounds add up over the updates since the last FP64
+            // residual.  Split FP16 has delta 1e-8 also on the edge-localised vectors corrections are
+            // (its far field leaves the near pairs out); the other forms keep the near pairs' direct
+            // entries in their far field, whose rounding cancellation against the corrections
+            // amplifies about a hundredfold on such vectors (delta 1e-4).
+            std::vector<double> ub_next;
+            if (small) {
                 const double delta = o.precision == DBIE_FP16X3 ? 1e-8 : o.precision == DBIE_BF16 ? 1e-1 : 1e-4;
+                ub_next = ubound;
+                for (int k = 0; k < K; ++k)
+                    if (mask[k] != 0.0) ub_next[k] += delta * 10.0 * dn[k];
+            }
+            // Without verification (verify = 0) the solve ends here when every column's inner residual
+            // norm meets half the tolerance and the accumulated bound a quarter of it (split FP16 at
+            // rtol 1e-10: corrections up to 2.5e-4 ||F_k|| after an FP64 residual; the other forms in
+            // practice end on an FP64 residual).
+            if (!o.verify && small && (int)r.resid.size() == K) {
                 bool end = true;
                 for (int k = 0; k < K && end; ++k) {
                     const double tk = o.atol + o.rtol * bnorm[k];
-                    if (mask[k] != 0.0 && !(r.resid[k] <= 0.5 * tk && delta * 10.0 * dn[k] <= 0.25 * tk)) end = false;
+                    if (mask[k] != 0.0 && !(r.resid[k] <= 0.5 * tk && ub_next[k] <= 0.25 * tk)) end = false;
                 }
                 if (end) {
                     for (int k = 0; k < K; ++k)
@@ -1860,6 +1867,10 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                     break;
                 }
             }
+            // The updated residual's norm is the one the inner GMRES ended with, up to the loss of
+            // orthogonality of the Krylov basis.  When those norms already meet the tolerance ten
+            // times over, the update would only confirm it: go to the FP64 verification at once (a
+            // failed verification costs an FP64 apply and another inner solve, hence the margin).
             bool est_ok = small && (int)r.resid.size() == K;
             for (int k = 0; k < K && est_ok; ++k)
                 if (mask[k] != 0.0 && !(r.resid[k] <= 0.1 * (o.atol + o.rtol * bnorm[k]))) est_ok = false;
@@ -1867,6 +1878,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 for (int k = 0; k < K; ++k)
                     if (mask[k] != 0.0) rn[k] = r.resid[k];
                 exact = false;
+                ubound = ub_next;
                 op->note(2, rn);
                 continue;
             }
@@ -1875,12 +1887,14 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
                 apply_dev(op, K, dD.as<double>(), N, dT.as<double>(), N, o.precision);
                 sub_to(st, dT.as<double>(), dR.as<double>(), N, K);
                 exact = false;
+                ubound = ub_next;
             } else {
                 apply_dev(op, K, dX, N, dR.as<double>(), N, DBIE_FP64);
                 sub_from(st, dF, dR.as<double>(), N, K);
                 ++s.nrefine;
                 exact = true;
                 ++n_exact;
+                std::fill(ubound.begin(), ubound.end(), 0.0);
             }
             col_norms(op, dR.as<double>(), N, K, partial, hb, rn);
             op->note(exact ? 1 : 2, rn);

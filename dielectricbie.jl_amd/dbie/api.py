@@ -1106,7 +1106,7 @@ class Server:
         self.close()
 
     def submit_box(self, Lx, Ly, Lz, n_quad, l_ec, eps_in, eps_out, points, charges, *, precision="fp16x3",
-                   rtol=1e-10, itmax=500, inner_rtol=1e-5, max_refine=10) -> int:
+                   rtol=1e-10, itmax=500, inner_rtol=1e-5, max_refine=10, verify=True) -> int:
         pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
         q = np.ascontiguousarray(charges, np.float64).ravel()
         if len(q) != pts.shape[0]:
@@ -1116,6 +1116,7 @@ class Server:
         o.rtol, o.itmax = float(rtol), int(itmax)
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine = float(inner_rtol), int(max_refine)
+        o.verify = int(bool(verify))
         rid = C.c_int64()
         L.check(L.lib().dbie_server_submit_box(self._h, Lx, Ly, Lz, int(n_quad), l_ec, eps_in, eps_out, len(q),
                                                L.dptr(pts.ravel()), L.dptr(q), C.byref(o), C.byref(rid)))
