@@ -486,6 +486,16 @@ def main():
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": None, "kernel": ("far_mfma_f64" if name == "far_fp64" else f"far_mfma_{a.precision}"),
                 "avg_launch_ms": avg_ms, "launches": launches, "algorithmic_flops_per_launch": flops}
+        shp = op.tuned_shape(a.K, a.precision)
+        if name == "far_lowprec" and a.precision == "fp16x3" and shp and shp[1] >= 2 and op.stored_far() > 0:
+            # the GEMM over the stored kernel matrix streams 4 bytes per cross-plane pair from HBM: its
+            # algorithmic bytes are the stored matrix, the split operands (hi/lo fp16 of w.*X) and the
+            # FP64 partial sums of its source splits
+            nbytes = op.stored_far() + 2 * 2 * N * a.K + 8 * shp[0] * N * a.K
+            roof.update({"bound": "hbm", "achieved": nbytes / (avg_ms * 1e-3) / 1e9, "peak": PEAK["hbm"],
+                         "unit": "GB/s", "frac": nbytes / (avg_ms * 1e-3) / 1e9 / PEAK["hbm"],
+                         "algorithmic_bytes_per_launch": nbytes, "mfma_tflops": achieved,
+                         "mfma_frac": achieved / peak, "stored_matrix_bytes": op.stored_far()})
     # HBM bytes per launch of the dominant kernel, from the committed FETCH_SIZE/WRITE_SIZE passes
     # (split-FP16 far field: one entry per launch shape, s<splits>v<variant>; the entry of the shape
     # the tuner chose in this run, if that shape was profiled)
