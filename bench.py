@@ -197,9 +197,10 @@ def single_workload(a, dbie, log):
 
 def serve_stream(dbie, reqs, precision, inner_rtol, slots, log, setup_workers=2, verify=False):
     """Submit the requests at their arrival times to a fresh request server; wait for all.
-    Returns (infos, results, wall seconds, server stats).  Mixed-precision requests end on their
-    refined residual estimate (verify=False), as the FP64 ones end on their Krylov estimate; each
-    served result is compared with the FP64 one afterwards."""
+    Returns (infos, results, wall seconds, server stats).  Every request runs with the near-field
+    preconditioner; mixed-precision requests end on their refined residual estimate (verify=False),
+    as the FP64 ones end on their Krylov estimate; each served result is compared with the FP64 one
+    afterwards."""
     infos, results = [], []
     with dbie.Server(solve_slots=slots, setup_workers=setup_workers) as srv:
         t0 = time.perf_counter()
@@ -208,7 +209,8 @@ def serve_stream(dbie, reqs, precision, inner_rtol, slots, log, setup_workers=2,
             while time.perf_counter() - t0 < r["t"]:
                 time.sleep(0.0005)
             ids.append(srv.submit_box(*r["L"], r["n_quad"], r["l_ec"], r["eps_in"], r["eps_out"], r["points"],
-                                      r["charges"], precision=precision, inner_rtol=inner_rtol, verify=verify))
+                                      r["charges"], precision=precision, inner_rtol=inner_rtol, verify=verify,
+                                      precondition=2))
         for i in ids:
             infos.append(srv.wait(i))
             results.append(srv.result(i) if infos[-1]["status"] == 0 else None)
@@ -405,16 +407,20 @@ def main():
     # plain FP64 comparator in two Krylov variants: GMRES per column on shared block matvecs, and block
     # GMRES (one block Krylov space, as Krylov.block_gmres); the mixed speed-up is stated against the
     # faster one (median of fp64_steps warm solves)
+    # (each with the reference's diagonal scaling and with the near-field preconditioner, precondition=2)
     fp64, S64 = {}, None
-    for kr in ("columns", "block"):
-        lat, wall, st64 = timed(dict(rtol=1e-10, precision="fp64", check_residual=False, krylov=kr),
+    for name, kr, pc in (("columns", "columns", 1), ("block", "block", 1), ("columns_near", "columns", 2),
+                         ("block_near", "block", 2)):
+        lat, wall, st64 = timed(dict(rtol=1e-10, precision="fp64", check_residual=False, krylov=kr, precondition=pc),
                                 max(a.fp64_steps, 1), 1)
-        log(f"fp64 ({kr}): iters={st64['niter']} ms {[round(v, 1) for v in lat]}")
+        log(f"fp64 ({name}): iters={st64['niter']} ms {[round(v, 1) for v in lat]}")
         if S64 is None:
             S64 = dS.to_host()
-        fp64[kr] = {"ms_per_step": float(np.median(lat)), "ms_all": lat, "iters": int(st64["niter"])}
+        fp64[name] = {"ms_per_step": float(np.median(lat)), "ms_all": lat, "iters": int(st64["niter"]),
+                      "precondition": "near-field" if pc == 2 else "diagonal"}
     fp64_kr = min(fp64, key=lambda k: fp64[k]["ms_per_step"])
     fp64_ms = fp64[fp64_kr]["ms_per_step"]
+    fp64_diag_ms = min(fp64["columns"]["ms_per_step"], fp64["block"]["ms_per_step"])
     iters64 = fp64["columns"]["iters"]
 
     def col_err(S):
@@ -431,11 +437,11 @@ def main():
         return float((np.linalg.norm(R, axis=0) / np.linalg.norm(F, axis=0)).max())
 
     mixed = {}
-    for name, kr, rec, ver in (("columns", "columns", True, False), ("columns_verified", "columns", True, True),
-                               ("columns_restarted", "columns", False, False), ("block", "block", True, False),
-                               ("block_verified", "block", True, True)):
+    for name, kr, rec, ver, pc in (("columns", "columns", True, False, 2), ("columns_verified", "columns", True, True, 2),
+                                   ("columns_restarted", "columns", False, False, 2), ("block", "block", True, False, 2),
+                                   ("columns_diagonal", "columns", True, False, 1)):
         kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol, krylov=kr, recycle=rec, verify=ver,
-                  check_residual=False)
+                  check_residual=False, precondition=pc)
         lat, wall, st = timed(kw, a.steps, a.warmup)
         Sm = dS.to_host()
         mixed[name] = {"lat": lat, "wall": wall, "st": st, "err": col_err(Sm), "kw": kw, "res": true_residual(Sm)}
@@ -459,7 +465,7 @@ def main():
     variants = {}
     for prec in others:
         kw = dict(rtol=1e-10, precision=prec, inner_rtol=a.inner_rtol, krylov="columns", recycle=True, verify=False,
-                  check_residual=False)
+                  check_residual=False, precondition=2)
         vlat, vwall, vst = timed(kw, max(a.variant_steps, 1), 1)
         Sv = dS.to_host()
         verr = col_err(Sv)
@@ -536,8 +542,10 @@ def main():
         "mixed_variants": {k: {"ms_per_step": float(np.mean(v["lat"])), "inner_iters": v["st"]["niter"],
                                "refinements": v["st"]["nrefine"], "value": a.K * a.steps / v["wall"],
                                "max_rel_err_vs_fp64": float(v["err"].max()), "true_max_rel_residual": v["res"],
-                               "verify": bool(v["kw"]["verify"])}
+                               "verify": bool(v["kw"]["verify"]),
+                               "precondition": "near-field" if v["kw"]["precondition"] == 2 else "diagonal"}
                            for k, v in mixed.items()},
+        "speedup_vs_fp64_diagonal": value / (a.K / (fp64_diag_ms * 1e-3)),
         "other_precisions": variants,
         "speedup_vs_fp64": value / fp64_value,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

@@ -151,10 +151,11 @@ struct GmresWork {
     DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
     DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
     DBuf PV, PT;          // block GMRES: Householder vectors (2K x K) and T factors (K x K) per Hessenberg step
+    DBuf PY, PX;          // near-field preconditioner: its near product (N x K FP64), FP32 staging of its input
     // take the workspaces from (and return them to) the calling thread's block cache
     void set_pooled() {
         for (DBuf *d : {&V, &W, &Z, &U, &partial, &hbuf, &nbuf, &V1, &abuf, &cbuf, &om, &n1, &H, &G, &tau, &Y, &gpart,
-                        &R, &info, &PV, &PT})
+                        &R, &info, &PV, &PT, &PY, &PX})
             d->pooled = true;
     }
 };
@@ -172,6 +173,8 @@ struct dbie_op {
     dbie_near_opts nopts{};       // near options the operator was built with
     DBuf bsr_d64;                 // correction blocks of the double layer D (built on first use)
     bool d_built = false;
+    DBuf bsr_full64;              // the near pairs' full interaction in FP64 (the near-field preconditioner
+    bool full64_built = false;    // of FP64 solves; built on first use)
     // residual history of the last solve: (kind, max_k ||r_k|| / ||F_k||) per entry; kind 0 = a Krylov
     // iteration's residual estimate, 1 = the true FP64 residual, 2 = a residual updated with the
     // low-precision operator or taken from the inner solve's estimate.  hist_scale: 1 / ||F_k||
@@ -1118,9 +1121,42 @@ struct GmresOut {
                                 // Givens recurrence per column, the explicit block residual for block GMRES)
 };
 
+// Right preconditioner of the Krylov solves (solve option precondition): 1 = the diagonal scaling
+// M = D^-1, D = diag(t) (dielectric_diagonal_scaling, src/solver/multi_rhs.jl:234-247); 2 = one Neumann
+// term of the near-field inverse, M = D^-1 - D^-1 N D^-1 with N the near panel pairs' full interaction
+// (correction + direct Dt entries), i.e. (D + N)^-1 to first order.  For the box it takes GMRES from 15
+// to 9 iterations (eps_in 4) and from 23 to 13 (eps_in 80) at the cost of one near-field product per
+// iteration.  Applied as z = D^-1 v, Y = N z - t z (a near product whose diagonal is -t), M v = -D^-1 Y;
+// FP64 solves multiply the FP64 full blocks, the low-precision inner solves the FP32 ones.
+struct Precond {
+    const double *dinv = nullptr;   // 1 / t
+    const double *ndinv = nullptr;  // -1 / t (near form)
+    const double *negt = nullptr;   // -t (near form)
+    DevBSR near;                    // the blocks of N (near form)
+    NearVals vals = NEAR_C64;
+    bool use_near = false;
+};
+
+// out = M in (in and out may alias)
+template <typename TV>
+void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, int64_t N, int K) {
+    hipStream_t st = op->stream;
+    row_scale(st, pc.dinv, in, out, N, K);
+    if (!pc.use_near) return;
+    GmresWork &ws = op->gw;
+    ws.PY.ensure(sizeof(double) * (size_t)N * K);
+    float *xt = nullptr;
+    if (pc.vals == NEAR_UP32) {
+        ws.PX.ensure(near_xt_bytes(N, K));
+        xt = ws.PX.as<float>();
+    }
+    near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, ws.PY.as<double>(), N, xt, nullptr);
+    row_scale(st, pc.ndinv, ws.PY.as<double>(), out, N, K);
+}
+
 // GMRES(m) for K independent columns sharing each block matvec.  Right
-// preconditioner: diagonal dinv (dielectric_diagonal_scaling).  X receives the
-// solution (initial guess zero).  tol_k = atol + rtol * ||B_k||.
+// preconditioner pc (null: none).  X receives the solution (initial guess zero).
+// tol_k = atol + rtol * ||B_k||.
 //
 // keep: store this solve's last Krylov cycle for later solves with the same operator.
 // use: GCRO deflation with a stored space (C, U), A M U = C: every cycle starts from
@@ -1129,7 +1165,7 @@ struct GmresOut {
 // residual over span(U) + the new Krylov space: the refinement continues the earlier space.
 template <typename TV>
 GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol,
-                  int itmax, int restart, const double *dinv, KrylovRecycle *keep, const KrylovRecycle *use) {
+                  int itmax, int restart, const Precond *pc, KrylovRecycle *keep, const KrylovRecycle *use) {
     const int64_t N = op->N();
     const size_t NK = (size_t)N * K;
     const int m = std::max(1, std::min(restart > 0 ? restart : 100, std::max(itmax, 1)));
@@ -1144,7 +1180,7 @@ GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision 
     W.ensure(sizeof(double) * NK);
     U.ensure(sizeof(double) * NK);
     constexpr bool kF64 = std::is_same<TV, double>::value;
-    if (dinv || !kF64) Z.ensure(sizeof(double) * NK);
+    if (pc || !kF64) Z.ensure(sizeof(double) * NK);
     partial.ensure(sizeof(double) * std::max(dots_partial_elems(N, K, std::max(m + 1, nb1)), dots_partial_elems(N, K, 1)));
     hbuf.ensure(sizeof(double) * (size_t)(cgs_passes == 2 ? 2 * (m + 1) : m + 1) * K);
     nbuf.ensure(sizeof(double) * (size_t)(m + 1) * K);
@@ -1263,7 +1299,7 @@ GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision 
             h2d(ws.abuf.p, z.data(), sizeof(double) * z.size(), st);
             block_combine(st, V1p, nb1, ws.abuf.as<double>(), U.as<double>(), N, K);
         }
-        if (dinv) row_scale(st, dinv, U.as<double>(), U.as<double>(), N, K);
+        if (pc) apply_precond(op, *pc, U.as<double>(), U.as<double>(), N, K);
         add_to(st, U.as<double>(), X, N, K);
     };
     bool first = true;
@@ -1310,8 +1346,9 @@ GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision 
             Vp = basis(j + 2);
             const TV *Vj = Vp + (size_t)j * NK;
             const double *Zin;
-            if (dinv || !kF64) {  // M v_j in FP64 (the apply's input)
-                row_scale(st, dinv, Vj, Z.as<double>(), N, K);
+            if (pc || !kF64) {  // M v_j in FP64 (the apply's input)
+                if (pc) apply_precond(op, *pc, Vj, Z.as<double>(), N, K);
+                else row_scale(st, nullptr, Vj, Z.as<double>(), N, K);
                 Zin = Z.as<double>();
             } else {
                 Zin = reinterpret_cast<const double *>(Vj);
@@ -1450,7 +1487,7 @@ GmresOut gmres_t(dbie_op *op, int K, const double *B, double *X, dbie_precision 
 }
 
 GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol, int itmax,
-               int restart, const double *dinv, KrylovRecycle *keep = nullptr, const KrylovRecycle *use = nullptr) {
+               int restart, const Precond *pc, KrylovRecycle *keep = nullptr, const KrylovRecycle *use = nullptr) {
     // The low-precision inner solves of the refinement keep their Krylov basis in FP32: their operator
     // is accurate to ~1e-7 and their tolerance ~1e-5, and the orthogonalisation's passes over the
     // basis (the solve's largest HBM traffic after the operator) halve.  Not a space kept for a later
@@ -1458,8 +1495,8 @@ GmresOut gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision pr
     // deflates it so poorly that it takes about twice the iterations (measured at N = 64,384,
     // K = 256: 39 inner iterations instead of 19).  The deflating solve's own basis may be FP32.
     const bool f32 = prec != DBIE_FP64 && !keep;
-    return f32 ? gmres_t<float>(op, K, B, X, prec, rtol, atol, itmax, restart, dinv, keep, use)
-               : gmres_t<double>(op, K, B, X, prec, rtol, atol, itmax, restart, dinv, keep, use);
+    return f32 ? gmres_t<float>(op, K, B, X, prec, rtol, atol, itmax, restart, pc, keep, use)
+               : gmres_t<double>(op, K, B, X, prec, rtol, atol, itmax, restart, pc, keep, use);
 }
 
 void blas_ok(rocblas_status st, const char *what) {
@@ -1541,7 +1578,7 @@ bool chol_qr2(dbie_op *op, GmresWork &ws, double *W, int64_t N, int K, double *R
 // atol + rtol ||B_k|| (the reference's Krylov.jl tests the Frobenius norm of the block residual; the
 // per-column test is the stricter of the two).  X receives the solution (initial guess zero).
 GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precision prec, double rtol, double atol,
-                     int itmax, int restart, const double *dinv) {
+                     int itmax, int restart, const Precond *pc) {
     const int64_t N = op->N();
     const size_t NK = (size_t)N * K;
     const int m = std::max(1, std::min(restart > 0 ? restart : 40, std::max(itmax, 1)));
@@ -1553,7 +1590,7 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
     const int64_t ldr = (int64_t)m * K;        // Cholesky factor of Hbar^T Hbar: mK x mK (upper)
     ws.W.ensure(sizeof(double) * NK);
     ws.U.ensure(sizeof(double) * NK);
-    if (dinv) ws.Z.ensure(sizeof(double) * NK);
+    if (pc) ws.Z.ensure(sizeof(double) * NK);
     ws.V.ensure(sizeof(double) * NK * (m + 1));
     ws.H.ensure(sizeof(double) * (size_t)ldh * m * K);
     ws.G.ensure(sizeof(double) * (size_t)ldh * K);   // residual E1 S0 - Hbar Y
@@ -1607,8 +1644,8 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
             double *Hc = Hp + (size_t)j * K * ldh;  // block column j of Hbar
             const rocblas_int nb = (rocblas_int)((j + 1) * K), jk = (rocblas_int)(j * K);
             const double *Zin = Vj;
-            if (dinv) {
-                row_scale(st, dinv, Vj, ws.Z.as<double>(), N, K);
+            if (pc) {
+                apply_precond(op, *pc, Vj, ws.Z.as<double>(), N, K);
                 Zin = ws.Z.as<double>();
             }
             apply_dev(op, K, Zin, N, Wp, N, prec);
@@ -1701,7 +1738,7 @@ GmresOut block_gmres(dbie_op *op, int K, const double *B, double *X, dbie_precis
             blas_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, KK, nj, &one, Vp,
                                   (rocblas_int)N, Yp, (rocblas_int)ldh, &zero, ws.U.as<double>(), (rocblas_int)N),
                     "rocblas_dgemm");
-            if (dinv) row_scale(st, dinv, ws.U.as<double>(), ws.U.as<double>(), N, K);
+            if (pc) apply_precond(op, *pc, ws.U.as<double>(), ws.U.as<double>(), N, K);
             add_to(st, ws.U.as<double>(), X, N, K);
         }
         if (conv) { res.solved = true; break; }
@@ -1726,15 +1763,56 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     if (!(o.rtol >= 0) || !(o.atol >= 0)) fail(DBIE_BAD_ARG, "tolerances must be >= 0");
     check_precision(o.precision);
     if (o.krylov != 0 && o.krylov != 1) fail(DBIE_BAD_ARG, "krylov must be 0 (per column) or 1 (block)");
+    if (o.precondition < 0 || o.precondition > 2)
+        fail(DBIE_BAD_ARG, "precondition must be 0 (none), 1 (diagonal scaling) or 2 (near-field)");
     hipStream_t st = op->stream;
     const size_t NK = (size_t)N * K;
     DBuf dR(true), dD(true), dT(true), dinv(true), partial(true), hb(true);
-    const double *dinvp = nullptr;
+    // preconditioners: pc64 for FP64 solves, pcl for the low-precision inner solves (the near form
+    // multiplies FP64 or FP32 full blocks; a fused operator has no blocks of its own: diagonal only)
+    Precond pc64, pcl;
+    const Precond *p64 = nullptr, *plo = nullptr;
     if (o.precondition) {
-        std::vector<double> inv(N);
-        for (int64_t i = 0; i < N; ++i) inv[i] = 1.0 / op->diag_h[i];
-        upload(dinv, inv.data(), N, st);
-        dinvp = dinv.as<double>();
+        const bool near = o.precondition == 2 && op->members.empty() && op->bsr.nb > 0;
+        std::vector<double> inv((size_t)N * (near ? 3 : 1));
+        for (int64_t i = 0; i < N; ++i) {
+            inv[i] = 1.0 / op->diag_h[i];
+            if (near) {
+                inv[N + i] = -inv[i];
+                inv[2 * N + i] = -op->diag_h[i];
+            }
+        }
+        upload(dinv, inv.data(), inv.size(), st);
+        pc64.dinv = dinv.as<double>();
+        if (near) {
+            pc64.ndinv = dinv.as<double>() + N;
+            pc64.negt = dinv.as<double>() + 2 * N;
+        }
+        pcl = pc64;
+        if (near && o.precision == DBIE_FP64) {
+            if (!op->full64_built) {
+                op->bsr_full64.ensure(sizeof(double) * (size_t)op->bsr.nb * op->bsr.bd * op->bsr.bd);
+                near_full_blocks(st, op->dg, op->bsr, op->bsr_full64.as<double>());
+                DBIE_HIP(hipGetLastError());
+                op->full64_built = true;
+            }
+            pc64.near = op->bsr;
+            pc64.near.val64 = op->bsr_full64.as<double>();
+            pc64.near.val32 = pc64.near.valup32 = nullptr;
+            pc64.near.valh = nullptr;
+            pc64.near.sr_ptr = nullptr;
+            pc64.vals = NEAR_C64;
+            pc64.use_near = true;
+        }
+        if (near && o.precision != DBIE_FP64 && op->bsr.valup32) {
+            pcl.near = op->bsr;
+            pcl.near.valh = nullptr;
+            pcl.near.sr_ptr = nullptr;
+            pcl.vals = NEAR_UP32;
+            pcl.use_near = true;
+        }
+        p64 = &pc64;
+        plo = &pcl;
     }
     dbie_solve_stats s{};
     std::vector<double> bnorm, rn;
@@ -1752,8 +1830,8 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     DBIE_HIP(hipEventCreate(&e1));
     DBIE_HIP(hipEventRecord(e0, st));
     if (o.precision == DBIE_FP64) {
-        const GmresOut r = o.krylov == 1 ? block_gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp)
-                                         : gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, dinvp);
+        const GmresOut r = o.krylov == 1 ? block_gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, p64)
+                                         : gmres(op, K, dF, dX, DBIE_FP64, o.rtol, o.atol, o.itmax, o.restart, p64);
         s.niter = r.iters;
         s.solved = r.solved;
     } else {
@@ -1820,9 +1898,9 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
             const bool recyc = o.recycle != 0 && o.krylov == 0;
             const GmresOut r =
                 o.krylov == 1 ? block_gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, irtol, 0.0,
-                                            o.itmax - s.niter, o.restart, dinvp)
+                                            o.itmax - s.niter, o.restart, plo)
                               : gmres(op, K, dR.as<double>(), dD.as<double>(), o.precision, irtol, 0.0,
-                                      o.itmax - s.niter, o.restart, dinvp, recyc && it == 0 ? &rec : nullptr,
+                                      o.itmax - s.niter, o.restart, plo, recyc && it == 0 ? &rec : nullptr,
                                       recyc && it > 0 ? &rec : nullptr);
             s.niter += r.iters;
             if (trace) std::fprintf(stderr, "[dbie] refine %d: inner iters %d, residual %.3e (%s)\n", it, r.iters,
@@ -2485,7 +2563,7 @@ dbie_status dbie_op_device_bytes(const dbie_op *op, int64_t *bytes) {
         for (const DBuf *d : {&op->x, &op->y, &op->z, &op->nx, &op->ny, &op->nz, &op->w, &op->xf, &op->yf, &op->zf,
                               &op->nxf, &op->nyf, &op->nzf, &op->wf, &op->plane, &op->diag, &op->bsr_row,
                               &op->bsr_col, &op->bsr_v64, &op->bsr_v32, &op->bsr_vup32, &op->bsr_vh, &op->bsr_rexp, &op->sr_ptr,
-                              &op->sr_col, &op->sr_blk, &op->bsr_d64,
+                              &op->sr_col, &op->sr_blk, &op->bsr_d64, &op->bsr_full64,
                               &op->nmask, &op->part,
                               &op->xin, &op->yout, &op->tmp, &op->mscratch, &op->xt32, &op->gst, &op->gst_off})
             b += (int64_t)d->bytes;

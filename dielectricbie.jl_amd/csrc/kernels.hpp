@@ -259,6 +259,9 @@ size_t adaptive_blocks_lds(int nq, int nGL, int max_depth);
 bool adaptive_blocks(hipStream_t st, const DevGeom &g, int nq, int nGL, int n_adp, const int *adp_src,
                      const int *adp_trg, const int *adp_rep, const double *adp_map, const double *rules, double atol,
                      double rtol, int max_depth, int max_nodes, double *kup, int *hit, int mode = 0);
+// The near pairs' full interaction (correction + direct Dt entries) in FP64, C.nb x bd x bd values in
+// the storage order of C.val64 (the near-field preconditioner of FP64 solves).
+void near_full_blocks(hipStream_t st, const DevGeom &g, const DevBSR &C, double *full64);
 bool build_near_blocks(hipStream_t st, const DevGeom &g, int bd, bool mfma_order, int n_up, const int *up_src,
                        const int *up_trg, const int *up_slot, const int *cache_off, const int *cache_nu,
                        const double *caches, int max_nu, int n_adp, const int *adp_src, const int *adp_trg,

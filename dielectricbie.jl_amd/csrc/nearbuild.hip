@@ -418,7 +418,36 @@ __global__ __launch_bounds__(256) void pair_scan_kernel(DevGeom g, int P, int np
     }
     if (!WRITE && tid == 0) count[i] = running;
 }
+
+// The near pairs' full interaction in FP64, correction block + direct Dt term (the upsampled value the
+// builder split into the two), in the storage order of the FP64 blocks: one workgroup per block row
+// (target panel), the direct term computed as near_blocks_kernel computes it.
+__global__ __launch_bounds__(256) void near_full64_kernel(DevGeom g, int bd, int mfma, const int *__restrict__ row_ptr,
+                                                          const int *__restrict__ col, const double *__restrict__ val64,
+                                                          double *__restrict__ full64) {
+    const int p = blockIdx.x;
+    const int64_t tb = (int64_t)p * bd;
+    const double n0 = g.nx[tb], n1 = g.ny[tb], n2 = g.nz[tb];
+    for (int b = row_ptr[p]; b < row_ptr[p + 1]; ++b) {
+        const int64_t sb = (int64_t)col[b] * bd;
+        const double s0 = g.nx[sb], s1 = g.ny[sb], s2 = g.nz[sb];
+        const size_t out0 = (size_t)b * bd * bd;
+        for (int o = threadIdx.x; o < bd * bd; o += blockDim.x) {
+            const int t = o / bd, c = o - t * bd;
+            const size_t idx = out0 + store_idx(bd, mfma != 0, t, c);
+            const double direct = corr_k<0>(g.x[sb + c], g.y[sb + c], g.z[sb + c], g.x[tb + t], g.y[tb + t],
+                                            g.z[tb + t], n0, n1, n2, s0, s1, s2) *
+                                  g.w[sb + c];
+            full64[idx] = val64[idx] + direct;
+        }
+    }
+}
 }  // namespace
+
+void near_full_blocks(hipStream_t st, const DevGeom &g, const DevBSR &C, double *full64) {
+    if (C.P < 1 || C.nb < 1) return;
+    near_full64_kernel<<<(unsigned)C.P, 256, 0, st>>>(g, C.bd, C.mfma_order ? 1 : 0, C.row_ptr, C.col, C.val64, full64);
+}
 
 void pair_scan(hipStream_t st, const DevGeom &g, int P, int npp, const double *pdata, const double *corners,
                const int *is_edge, double tc2, bool correct_edges, int64_t *count, const int64_t *ptr, int *ctrg,

@@ -453,7 +453,7 @@ class BatchedDielectricOperator:
         o = L.SolveOpts()
         L.lib().dbie_solve_opts_default(C.byref(o))
         o.rtol, o.atol, o.itmax, o.restart = float(rtol), float(atol), int(itmax), int(restart)
-        o.precondition = int(bool(precondition))
+        o.precondition = _precond_code(precondition)
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
         o.krylov = _krylov_code(krylov)
@@ -490,7 +490,7 @@ class BatchedDielectricOperator:
         o = L.SolveOpts()
         L.lib().dbie_solve_opts_default(C.byref(o))
         o.rtol, o.atol, o.itmax, o.restart = float(rtol), float(atol), int(itmax), int(restart)
-        o.precondition = int(bool(precondition))
+        o.precondition = _precond_code(precondition)
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine, o.check_residual = float(inner_rtol), int(max_refine), int(bool(check_residual))
         o.krylov = _krylov_code(krylov)
@@ -558,6 +558,14 @@ def solve_fused(ops, Fs, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precisio
     if not (status == L.NOT_CONVERGED and not raise_on_fail):
         L.check(status)
     return Ss, st.as_dict()
+
+
+def _precond_code(precondition) -> int:
+    """True / 1 / "diagonal": diagonal scaling (the reference's); 2 / "near": the near-field Neumann term;
+    False / 0 / "none": no preconditioner."""
+    if isinstance(precondition, str):
+        return {"none": 0, "diagonal": 1, "near": 2}[precondition]
+    return int(precondition)
 
 
 def _krylov_code(krylov) -> int:
@@ -1106,7 +1114,7 @@ class Server:
         self.close()
 
     def submit_box(self, Lx, Ly, Lz, n_quad, l_ec, eps_in, eps_out, points, charges, *, precision="fp16x3",
-                   rtol=1e-10, itmax=500, inner_rtol=1e-5, max_refine=10, verify=True) -> int:
+                   rtol=1e-10, itmax=500, inner_rtol=1e-5, max_refine=10, verify=True, precondition=True) -> int:
         pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
         q = np.ascontiguousarray(charges, np.float64).ravel()
         if len(q) != pts.shape[0]:
@@ -1117,6 +1125,7 @@ class Server:
         o.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.inner_rtol, o.max_refine = float(inner_rtol), int(max_refine)
         o.verify = int(bool(verify))
+        o.precondition = _precond_code(precondition)
         rid = C.c_int64()
         L.check(L.lib().dbie_server_submit_box(self._h, Lx, Ly, Lz, int(n_quad), l_ec, eps_in, eps_out, len(q),
                                                L.dptr(pts.ravel()), L.dptr(q), C.byref(o), C.byref(rid)))

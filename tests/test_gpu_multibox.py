@@ -69,7 +69,8 @@ def test_multibox_dense_lu(dbie, O, mb):
 
 def test_multibox_gmres_with_contrast_scaling(dbie, O, mb):
     """dielectric_diagonal_scaling (src/solver/multi_rhs.jl:213-247) matters with several contrasts:
-    both forms converge to the oracle's LU solution."""
+    every form (none, diagonal scaling, the near-field Neumann term) converges to the oracle's LU
+    solution."""
     g, op, ref_if, ref, _, _ = mb
     pts = [[3.0, 0.0, 0.0], [0.1, 0.1, 0.1], [-0.1, 0.05, -0.1]]
     say("multi-box FP64 GMRES (with and without diagonal scaling) vs oracle LU")
@@ -77,13 +78,14 @@ def test_multibox_gmres_with_contrast_scaling(dbie, O, mb):
     Ad = ref_if.dense_DT() + ref.Cs.toarray() + np.diag(ref.t)
     S_ref = O.solve_lu(Ad, ref_if.rhs_points(pts, [1.0, -0.5, 0.25], 1.0))
     iters = {}
-    for pre in (True, False):
+    for pre in (2, 1, 0):
         S, st = op.solve(F, rtol=1e-12, precondition=pre)
         assert st["solved"]
         iters[pre] = st["niter"]
         for k in range(3):
             assert rel(S[:, k], S_ref[:, k]) < 1e-10
-    say(f"  iterations with scaling {iters[True]}, without {iters[False]}")
+    say(f"  iterations near-field {iters[2]}, with scaling {iters[1]}, without {iters[0]}")
+    assert iters[2] < iters[1]
 
 
 def test_multibox_mixed_precision(dbie, mb):
@@ -92,9 +94,10 @@ def test_multibox_mixed_precision(dbie, mb):
     rng = np.random.default_rng(31)
     F = dbie.rhs_point_charges(g, rng.uniform(-0.2, 0.2, (24, 3)), rng.uniform(-1, 1, 24), 1.0)
     S64, _ = op.solve(F, rtol=1e-12)
-    Sm, st = op.solve(F, rtol=1e-11, precision="fp16x3")
-    assert st["solved"]
-    assert max(rel(Sm[:, k], S64[:, k]) for k in range(24)) < 1e-8
+    for pre in (1, 2):
+        Sm, st = op.solve(F, rtol=1e-11, precision="fp16x3", precondition=pre)
+        assert st["solved"]
+        assert max(rel(Sm[:, k], S64[:, k]) for k in range(24)) < 1e-8
 
 
 def test_multibox_screened_volume_rhs(dbie, O, mb):
