@@ -38,4 +38,14 @@ __device__ inline int col_exponent(unsigned bits) {
     return 14 - e;
 }
 
+// Split-FP16 operands (hi and lo halves of the scaled w .* X, Kp columns): per tile of kZt = 32
+// sources and block of 16 columns one 1 KiB chunk in MFMA B-operand order -- group of 8 sources,
+// column, source within the group -- so that lane l of a wave reading 16 bytes at chunk + 16 l holds
+// column l % 16, sources 8 (l / 16) .. 8 (l / 16) + 7, and a stage of 32 sources reads whole cache
+// lines.  zsplit_idx is the element index of source j, column k.
+constexpr int kZt = 32;
+__device__ inline int64_t zsplit_idx(int64_t j, int k, int Kp) {
+    return ((j >> 5) * Kp + (k & ~15)) * kZt + ((j >> 3) & 3) * 128 + (k & 15) * 8 + (j & 7);
+}
+
 }  // namespace dbie

@@ -428,15 +428,14 @@ __global__ void split_tiles_f16(const double *__restrict__ X, int64_t ldx, const
                                 _Float16 *__restrict__ zl) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    const int64_t ntile = (N + kSb - 1) / kSb;
-    if (j >= ntile * kSb) return;
+    if (j >= (N + kSb - 1) / kSb * kSb) return;  // the operands cover whole 64-source stages
     _Float16 h = (_Float16)0.0f, l = (_Float16)0.0f;
     if (j < N && k < K) {
         const double zd = ldexp(w[j] * X[j + (int64_t)k * ldx], col_exponent(cmax[k]));
         h = (_Float16)(float)zd;
         l = (_Float16)(float)(zd - (double)(float)h);
     }
-    const int64_t idx = ((j / kSb) * Kp + k) * kSb + (j % kSb);
+    const int64_t idx = zsplit_idx(j, k, Kp);
     zh[idx] = h;
     zl[idx] = l;
 }
@@ -529,12 +528,12 @@ __global__ __launch_bounds__(kThreads) void far_mfma_f16x3_kernel(DevGeom g, int
     // halves swap roles without register copies (a copy would wait for the loads in flight)
     uint4 bh[2][I::kCbPerWave][KS], bl[2][I::kCbPerWave][KS];
     auto load_b = [&](int64_t jb, uint4 (&bh)[I::kCbPerWave][KS], uint4 (&bl)[I::kCbPerWave][KS]) {
-        const int64_t base = ((jb / kSb) * Kp + k0) * kSb;
 #pragma unroll
         for (int q = 0; q < I::kCbPerWave; ++q)
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                const int64_t off = base + (int64_t)(I::cb(wave, q) * 16 + (lane & 15)) * kSb + ks * 32 + 8 * (lane >> 4);
+                // k-slice ks of the stage is one operand tile: its chunk of column block cb, lane l at 16 l
+                const int64_t off = ((jb / kZt + ks) * Kp + k0 + I::cb(wave, q) * 16) * kZt + lane * 8;
                 bh[q][ks] = *reinterpret_cast<const uint4 *>(zh + off);
                 bl[q][ks] = *reinterpret_cast<const uint4 *>(zl + off);
             }
@@ -803,12 +802,12 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
         const double *coord = pw == 0 ? g.x : (pw == 1 ? g.y : g.z);
         const __amdgpu_buffer_rsrc_t rc =
             __builtin_amdgcn_make_buffer_rsrc((void *)coord, (short)0, (int)(N * (int64_t)sizeof(double)), 0x00020000);
-        const int loff = ((4 * pw * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+        const int loff = 4 * pw * 1024 + lane * 16;  // column block 4 pw: lane l at 16 l of its chunk
         // The coordinate copy goes first: a barrier preceded by vmcnt(8) then has every wave's
         // coordinates of the stage landed (the other producer waves read them in generate) while
         // the 8 B-operand copies, read by the consumers a stage later, may still fly.
         auto copy_stage = [&](int64_t js, int pb) {
-            const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+            const int soff = (int)(((js / kZt) * Kp + k0) * kZt * 2);
             // 64 lanes x 4 bytes = the stage's 32 doubles (reads past the end return zero)
             if (pw < 3)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(rc, (lptr_t)&s_pos[pb][pw][0], 4, lane * 4,
@@ -816,10 +815,8 @@ __global__ __launch_bounds__(ws_threads<CW>()) void far_ws_f16x3_kernel(DevGeom 
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int cb = 4 * pw + q;
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rh, (lptr_t)&s_b[pb][cb][0][0], 16, loff + q * 16 * kSb * 2,
-                                                         soff, 0, 0);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rl, (lptr_t)&s_b[pb][cb][1][0], 16, loff + q * 16 * kSb * 2,
-                                                         soff, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rh, (lptr_t)&s_b[pb][cb][0][0], 16, loff + q * 1024, soff, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rl, (lptr_t)&s_b[pb][cb][1][0], 16, loff + q * 1024, soff, 0, 0);
             }
         };
         // A tiles of stage js (coordinates in s_pos[cb]) into s_a[pb].  Lane 16 r + e (and
@@ -1154,11 +1151,11 @@ __global__ __launch_bounds__(CW * 64) void far_gs_f16x3_kernel(const int *__rest
         }
     };
     auto copy_b = [&](int64_t js, uint4 (&sb)[kGsChunks][64]) {
-        const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+        const int soff = (int)(((js / kZt) * Kp + k0) * kZt * 2);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int ch = NC * wave + c;  // column block ch / 2, hi (even) or lo (odd)
-            const int loff = (((ch >> 1) * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+            const int loff = (ch >> 1) * 1024 + lane * 16;
             __builtin_amdgcn_raw_ptr_buffer_load_lds((c & 1) ? rl : rh, (lptr_t)&sb[ch][0], 16, loff, soff, 0, 0);
         }
     };
@@ -1289,11 +1286,11 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
     const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc((void *)zh, (short)0, zbytes, 0x00020000);
     const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void *)zl, (short)0, zbytes, 0x00020000);
     auto copy_b = [&](int64_t js, uint4 (&sb)[kGsChunks][64]) {
-        const int soff = (int)((((js / kSb) * Kp + k0) * kSb + (js % kSb)) * 2);
+        const int soff = (int)(((js / kZt) * Kp + k0) * kZt * 2);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int ch = NC * wave + c;  // column block ch / 2, hi (even) or lo (odd)
-            const int loff = (((ch >> 1) * 16 + (lane & 15)) * kSb + 8 * (lane >> 4)) * 2;
+            const int loff = (ch >> 1) * 1024 + lane * 16;
             __builtin_amdgcn_raw_ptr_buffer_load_lds((c & 1) ? rl : rh, (lptr_t)&sb[ch][0], 16, loff, soff, 0, 0);
         }
     };
@@ -1524,7 +1521,7 @@ void far_f16_operands(void *scratch, int64_t N, int K, NearH3 *out) {
     out->zl = zh + zelems;
     out->cmax = reinterpret_cast<const unsigned *>(zh + 2 * zelems);
     out->Kp = Kp;
-    out->tile = kSb;
+    out->tile = kZt;
 }
 
 int far_field_max_splits(int64_t N) { return (int)std::max<int64_t>(1, N / 256); }

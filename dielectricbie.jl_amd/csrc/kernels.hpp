@@ -102,7 +102,8 @@ bool far_field_mfma_supported(int prec, int K);
 int64_t far_field_part_elems(int64_t N, int K);  // workspace needed for any far-field launch
 
 // The split-FP16 far field's operands of an apply (in its scratch): hi and lo halves of
-// 2^col_exponent(cmax[k]) w_j X[j, k], tile-major ((j / tile) Kp + k) tile + j % tile.
+// 2^col_exponent(cmax[k]) w_j X[j, k], per tile of `tile` sources and 16-column block one chunk
+// in MFMA B-operand order (zsplit_idx in devmath.hpp).
 struct NearH3 {
     const void *zh = nullptr, *zl = nullptr;
     const unsigned *cmax = nullptr;

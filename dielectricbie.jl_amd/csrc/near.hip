@@ -348,9 +348,11 @@ __global__ __launch_bounds__(256) void near_h3_pack_kernel(const int *__restrict
     }
 }
 
-// Height of the far field's split-FP16 operand tiles (sources per tile, tile-major
-// ((j / kZTile) Kp + k) kZTile + j % kZTile), checked against the far field's at run time.
-constexpr int kZTile = 64;
+// The far field's split-FP16 operands are in the layout of zsplit_idx (tiles of kZt sources); the
+// host checks the far field's tile height against it at run time.  Column part of the index of a
+// 16-byte unit (8 sources of one column): zsplit_idx = zsplit_row(j) + zsplit_col(k) for 8-aligned j.
+__device__ inline int64_t zsplit_row(int64_t j, int Kp) { return (j >> 5) * (int64_t)Kp * kZt + ((j >> 3) & 3) * 128; }
+__device__ inline int zsplit_col(int k) { return (k & ~15) * kZt + (k & 15) * 8; }
 
 // Full near blocks of the split-FP16 operator on the FP16 MFMA (v_mfma_f32_16x16x32_f16): the 32-deep
 // contraction takes two blocks of the row at a time, lanes of k-groups 0-1 the first (sources 0-7,
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(256) void near_h3_kernel(int P, const int *__restri
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int c = k0 + (wave + 4 * q) * 16 + t;
-        zoff[q] = (int64_t)(c < K ? c : K - 1) * kZTile + 8 * h;  // columns past K: clamped, never stored
+        zoff[q] = zsplit_col(c < K ? c : K - 1);  // columns past K: clamped, never stored
     }
     v4f acc[NQ];
 #pragma unroll
@@ -393,8 +395,7 @@ __global__ __launch_bounds__(256) void near_h3_kernel(int P, const int *__restri
         ah = *reinterpret_cast<const uint4 *>(a);
         al = *reinterpret_cast<const uint4 *>(a + 256);
         if (bb >= b1) ah = al = uint4{0u, 0u, 0u, 0u};
-        const int64_t s0 = (int64_t)col[bv] * 16;  // the source panel's first node: 16-aligned in its tile
-        const int64_t tb = (s0 / kZTile) * (int64_t)Kp * kZTile + (s0 % kZTile);
+        const int64_t tb = zsplit_row((int64_t)col[bv] * 16 + 8 * h, Kp);  // the panel's sources 8 h .. 8 h + 7
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             bh[q] = *reinterpret_cast<const uint4 *>(zh + tb + zoff[q]);
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(256) void near_h3_sr_kernel(int P, int Q, const int
 #pragma unroll
     for (int c = 0; c < NQ; ++c) {
         const int col = k0 + (wave + 4 * c) * 16 + t;
-        zoff[c] = (int64_t)(col < K ? col : K - 1) * kZTile + 8 * h;  // columns past K: clamped, never stored
+        zoff[c] = zsplit_col(col < K ? col : K - 1);  // columns past K: clamped, never stored
     }
     v4f acc[RT][NQ];
 #pragma unroll
@@ -485,8 +486,7 @@ __global__ __launch_bounds__(256) void near_h3_sr_kernel(int P, int Q, const int
             al[r] = *reinterpret_cast<const uint4 *>(a + 256);
             if (b < 0) ah[r] = al[r] = uint4{0u, 0u, 0u, 0u};
         }
-        const int64_t s0 = (int64_t)sr_col[uv] * 16;
-        const int64_t tb = (s0 / kZTile) * (int64_t)Kp * kZTile + (s0 % kZTile);
+        const int64_t tb = zsplit_row((int64_t)sr_col[uv] * 16 + 8 * h, Kp);
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
             bh[c] = *reinterpret_cast<const uint4 *>(zh + tb + zoff[c]);
@@ -561,7 +561,7 @@ void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const d
     const float *v32 = vals == NEAR_UP32 ? C.valup32 : (vals == NEAR_C32 ? C.val32 : nullptr);
     const int nq = near_nq(K), Kpad = near_kpad(K), cw = 64 * nq;
     if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 16 && h3 && C.valh) {
-        if (h3->tile != kZTile) throw std::logic_error("near_combine: split-FP16 operand tiles of an unexpected height");
+        if (h3->tile != kZt) throw std::logic_error("near_combine: split-FP16 operand tiles of an unexpected height");
         const int per = (C.P + 7) / 8;
         dim3 grid((unsigned)(8 * per), (unsigned)(Kpad / cw));
         const _Float16 *vh = static_cast<const _Float16 *>(C.valh);
