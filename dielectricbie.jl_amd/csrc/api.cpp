@@ -151,7 +151,8 @@ struct GmresWork {
     DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
     DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
     DBuf PV, PT;          // block GMRES: Householder vectors (2K x K) and T factors (K x K) per Hessenberg step
-    DBuf PY, PX;          // near-field preconditioner: its near product (N x K FP64), FP32 staging of its input
+    DBuf PY, PX;          // near-field preconditioner: its near product (N x K FP64), staging of its input (FP32
+                          // copy, or the split-FP16 operands)
     // take the workspaces from (and return them to) the calling thread's block cache
     void set_pooled() {
         for (DBuf *d : {&V, &W, &Z, &U, &partial, &hbuf, &nbuf, &V1, &abuf, &cbuf, &om, &n1, &H, &G, &tau, &Y, &gpart,
@@ -1135,6 +1136,8 @@ struct Precond {
     DevBSR near;                    // the blocks of N (near form)
     NearVals vals = NEAR_C64;
     bool use_near = false;
+    const double *w = nullptr;      // quadrature weights: the split-FP16 form multiplies w .* x
+    bool h3 = false;                // near form on the split-FP16 blocks of the low-precision operator
 };
 
 // out = M in (in and out may alias)
@@ -1144,6 +1147,16 @@ void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, in
     row_scale(st, pc.dinv, in, out, N, K);
     if (!pc.use_near) return;
     GmresWork &ws = op->gw;
+    if (pc.h3 && K >= 16) {
+        // out <- -(1/t) .* (-t .* out + N out), in place: N out from out's split-FP16 operands (the
+        // three products of the operator's near field), the scaling in the near kernel's epilogue
+        ws.PX.ensure(far_field_mfma_scratch_bytes(N, (int)DBIE_FP16X3, K));
+        f16_split_operands(st, N, K, out, N, pc.w, ws.PX.p);
+        NearH3 h3;
+        far_f16_operands(ws.PX.p, N, K, &h3);
+        near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, out, N, nullptr, &h3, pc.ndinv);
+        return;
+    }
     ws.PY.ensure(sizeof(double) * (size_t)N * K);
     float *xt = nullptr;
     if (pc.vals == NEAR_UP32) {
@@ -1806,8 +1819,12 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
         }
         if (near && o.precision != DBIE_FP64 && op->bsr.valup32) {
             pcl.near = op->bsr;
-            pcl.near.valh = nullptr;
-            pcl.near.sr_ptr = nullptr;
+            pcl.h3 = op->bsr.valh && op->bsr.mfma_order && op->bsr.bd == 16;
+            pcl.w = op->w.as<double>();
+            if (!pcl.h3) {
+                pcl.near.valh = nullptr;
+                pcl.near.sr_ptr = nullptr;
+            }
             pcl.vals = NEAR_UP32;
             pcl.use_near = true;
         }

@@ -1524,6 +1524,21 @@ void far_f16_operands(void *scratch, int64_t N, int K, NearH3 *out) {
     out->tile = kZt;
 }
 
+void f16_split_operands(hipStream_t st, int64_t N, int K, const double *X, int64_t ldx, const double *w, void *scratch) {
+    const int ncb = ncb_for(K);
+    const int Kp = ((K + ncb * 16 - 1) / (ncb * 16)) * ncb * 16;
+    const int64_t ntile = (N + kSb - 1) / kSb;
+    const size_t zelems = (size_t)ntile * Kp * kSb;
+    _Float16 *zh = static_cast<_Float16 *>(scratch);
+    _Float16 *zl = zh + zelems;
+    unsigned *cmax = reinterpret_cast<unsigned *>(zl + zelems);
+    zero_u32_kernel<<<(unsigned)((K + 255) / 256), 256, 0, st>>>(cmax, K);
+    colmax_kernel<<<dim3((unsigned)((N + kColmaxRows - 1) / kColmaxRows), (unsigned)K), 256, 0, st>>>(X, ldx, w, N,
+                                                                                                     cmax);
+    split_tiles_f16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, w, N, K, Kp, cmax,
+                                                                                                zh, zl);
+}
+
 int far_field_max_splits(int64_t N) { return (int)std::max<int64_t>(1, N / 256); }
 
 int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const double *X, int64_t ldx, double *part,
@@ -1561,11 +1576,7 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
         _Float16 *zh = static_cast<_Float16 *>(scratch);
         _Float16 *zl = zh + zelems;
         unsigned *cmax = reinterpret_cast<unsigned *>(zl + zelems);
-        zero_u32_kernel<<<(unsigned)((K + 255) / 256), 256, 0, st>>>(cmax, K);
-        colmax_kernel<<<dim3((unsigned)((N + kColmaxRows - 1) / kColmaxRows), (unsigned)K), 256, 0, st>>>(X, ldx, g.w,
-                                                                                                         N, cmax);
-        split_tiles_f16<<<dim3((unsigned)((ntile * kSb + 255) / 256), (unsigned)Kp), 256, 0, st>>>(X, ldx, g.w, N, K, Kp,
-                                                                                                    cmax, zh, zl);
+        f16_split_operands(st, N, K, X, ldx, g.w, scratch);
         NearMask nm;
         nm.bits = g.near_mask;
         nm.npp = g.npp;

@@ -117,9 +117,13 @@ enum NearVals { NEAR_C64 = 0, NEAR_C32 = 1, NEAR_UP32 = 2 };
 // same X) and C.valh, the full near blocks multiply them on the FP16 MFMA (three products of the
 // hi/lo splits); otherwise xt, near_xt_bytes(N, K) bytes of scratch, takes a tile-major FP32 copy
 // of X for the FP32 blocks, stored lane-major (bsr_lane_major_index).
+// yscale (split-FP16 path only): Y[i, k] is multiplied by yscale[i]; then Y may be X itself
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
                   const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt = nullptr,
-                  const NearH3 *h3 = nullptr);
+                  const NearH3 *h3 = nullptr, const double *yscale = nullptr);
+// The split-FP16 operands of X (zh, zl, cmax as far_f16_operands lays them out in `scratch`, which
+// holds far_field_mfma_scratch_bytes(N, DBIE_FP16X3, K) bytes): what the far field computes first.
+void f16_split_operands(hipStream_t st, int64_t N, int K, const double *X, int64_t ldx, const double *w, void *scratch);
 constexpr int kNearSuperRows = 4;
 // C.valh and C.rexp (device arrays of nb * 512 halves and P ints) from C.valup32 and the node weights w
 void near_h3_pack(hipStream_t st, const DevBSR &C, const double *w, void *valh, int *rexp);

@@ -367,9 +367,8 @@ __global__ __launch_bounds__(256) void near_h3_kernel(int P, const int *__restri
                                                       int S, const double *__restrict__ part, int64_t N, int K,
                                                       const _Float16 *__restrict__ zh, const _Float16 *__restrict__ zl,
                                                       const unsigned *__restrict__ cmax, int Kp,
-                                                      const double *__restrict__ X, int64_t ldx,
-                                                      const double *__restrict__ diag, double *__restrict__ Y,
-                                                      int64_t ldy) {
+                                                      const double *X, int64_t ldx, const double *__restrict__ diag,
+                                                      const double *__restrict__ yscale, double *Y, int64_t ldy) {
     constexpr int CW = 64 * NQ;  // columns per workgroup
     __shared__ double s_out[CW][17];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -433,13 +432,14 @@ __global__ __launch_bounds__(256) void near_h3_kernel(int P, const int *__restri
     const int row = tid & 15;
     const int64_t i = (int64_t)p * 16 + row;
     const double dg = diag[i];
+    const double ys = yscale ? yscale[i] : 1.0;
     for (int m = 0; m < CW / 16; ++m) {
         const int cl = (tid >> 4) + 16 * m;
         const int c = k0 + cl;
         if (c >= K) continue;
         double y = s_out[cl][row] + dg * X[i + (int64_t)c * ldx];
         for (int s2 = 0; s2 < S; ++s2) y += part[((int64_t)s2 * K + c) * N + i];
-        Y[i + (int64_t)c * ldy] = y;
+        Y[i + (int64_t)c * ldy] = yscale ? ys * y : y;
     }
 }
 // The full near blocks of RT consecutive block rows against the far field's split operands: one
@@ -455,9 +455,9 @@ __global__ __launch_bounds__(256) void near_h3_sr_kernel(int P, int Q, const int
                                                          int S, const double *__restrict__ part, int64_t N, int K,
                                                          const _Float16 *__restrict__ zh, const _Float16 *__restrict__ zl,
                                                          const unsigned *__restrict__ cmax, int Kp,
-                                                         const double *__restrict__ X, int64_t ldx,
-                                                         const double *__restrict__ diag, double *__restrict__ Y,
-                                                         int64_t ldy) {
+                                                         const double *X, int64_t ldx,
+                                                         const double *__restrict__ diag,
+                                                         const double *__restrict__ yscale, double *Y, int64_t ldy) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = xcd_row(blockIdx.x, Q);
     const int k0 = blockIdx.y * 64 * NQ;
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(256) void near_h3_sr_kernel(int P, int Q, const int
                 const int64_t i = (int64_t)p * 16 + 4 * (lane >> 4) + e;
                 double y = (double)acc[r][c][e] * sc + diag[i] * X[i + (int64_t)col * ldx];
                 for (int s2 = 0; s2 < S; ++s2) y += part[((int64_t)s2 * K + col) * N + i];
-                Y[i + (int64_t)col * ldy] = y;
+                Y[i + (int64_t)col * ldy] = yscale ? yscale[i] * y : y;
             }
         }
     }
@@ -556,7 +556,7 @@ void near_h3_pack(hipStream_t st, const DevBSR &C, const double *w, void *valh, 
 
 void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const double *part, int64_t N, int K,
                   const double *X, int64_t ldx, const double *diag, double *Y, int64_t ldy, float *xt,
-                  const NearH3 *h3) {
+                  const NearH3 *h3, const double *yscale) {
     const int bd = C.bd;
     const float *v32 = vals == NEAR_UP32 ? C.valup32 : (vals == NEAR_C32 ? C.val32 : nullptr);
     const int nq = near_nq(K), Kpad = near_kpad(K), cw = 64 * nq;
@@ -571,7 +571,7 @@ void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const d
             dim3 gq((unsigned)(8 * perq), (unsigned)(Kpad / cw));
 #define DBIE_SR(Q) near_h3_sr_kernel<Q, kNearSuperRows><<<gq, 256, 0, st>>>(C.P, C.sr_q, C.sr_ptr, C.sr_col, C.sr_blk, \
                                                                               vh, C.rexp, S, part, N, K, zh, zl, h3->cmax, \
-                                                                              h3->Kp, X, ldx, diag, Y, ldy)
+                                                                              h3->Kp, X, ldx, diag, yscale, Y, ldy)
             if (nq == 1) DBIE_SR(1);
             else if (nq == 2) DBIE_SR(2);
             else DBIE_SR(4);
@@ -579,13 +579,14 @@ void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const d
             return;
         }
 #define DBIE_H3(Q) near_h3_kernel<Q><<<grid, 256, 0, st>>>(C.P, C.row_ptr, C.col, vh, C.rexp, S, part, N, K, zh, zl, \
-                                                           h3->cmax, h3->Kp, X, ldx, diag, Y, ldy)
+                                                           h3->cmax, h3->Kp, X, ldx, diag, yscale, Y, ldy)
         if (nq == 1) DBIE_H3(1);
         else if (nq == 2) DBIE_H3(2);
         else DBIE_H3(4);
 #undef DBIE_H3
         return;
     }
+    if (yscale) throw std::logic_error("near_combine: an output row scale needs the split-FP16 near blocks");
     if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 16) {  // xt: near_xt_bytes(N, K)
         x_tiles_f32_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)Kpad), 256, 0, st>>>(X, ldx, N, K, Kpad, xt);
         const int per = (C.P + 7) / 8;
