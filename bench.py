@@ -38,6 +38,7 @@ METRIC = "RHS-solves/s at ≤1e-8 rel err vs FP64, mixed vs plain FP64 GPU; p50/
 BOX = (2.4, 1.6, 1.2)
 L_EC, NQ, EPS_IN, EPS_OUT = 0.05, 4, 4.0, 1.0
 PEAK = {"fp64": 78.6, "bf16": 2500.0, "hbm": 8000.0}  # TFLOP/s, TFLOP/s, GB/s
+PRECONDITIONERS = {1: "diagonal", 2: "near-field", 3: "near-field, two sweeps"}
 
 
 def charges(K: int, seed: int = 0, margin: float = 0.05):
@@ -315,6 +316,8 @@ def main():
     ap.add_argument("--mode", default="solve", choices=["solve", "apply", "headline"],
                     help="apply: time only operator applies (one per precision) for kernel profiling")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--precondition", type=int, default=2, choices=[1, 2, 3],
+                    help="headline mode: the preconditioner (1 diagonal, 2 near field, 3 near field, two sweeps)")
     ap.add_argument("--shape", default="", help="apply mode: pin the low-precision far-field launch shape "
                                                 "'splits,variant' instead of tuning it")
     ap.add_argument("--blobs", type=int, default=1024, help="reaction workload: first B lattice blobs")
@@ -399,7 +402,7 @@ def main():
                 if op.tuned_shape(a.K, prec) is not None:
                     break
         kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol, krylov="columns", recycle=True,
-                  verify=False, check_residual=False, precondition=2)
+                  verify=False, check_residual=False, precondition=a.precondition)
         for _ in range(a.warmup):
             op.solve_device(dF, dS, **kw)
         op.synchronize()
@@ -439,17 +442,18 @@ def main():
     # plain FP64 comparator in two Krylov variants: GMRES per column on shared block matvecs, and block
     # GMRES (one block Krylov space, as Krylov.block_gmres); the mixed speed-up is stated against the
     # faster one (median of fp64_steps warm solves)
-    # (each with the reference's diagonal scaling and with the near-field preconditioner, precondition=2)
+    # (each with the reference's diagonal scaling and with the near-field preconditioner in one and two
+    # sweeps, precondition=2 and 3)
     fp64, S64 = {}, None
     for name, kr, pc in (("columns", "columns", 1), ("block", "block", 1), ("columns_near", "columns", 2),
-                         ("block_near", "block", 2)):
+                         ("block_near", "block", 2), ("columns_near2", "columns", 3), ("block_near2", "block", 3)):
         lat, wall, st64 = timed(dict(rtol=1e-10, precision="fp64", check_residual=False, krylov=kr, precondition=pc),
                                 max(a.fp64_steps, 1), 1)
         log(f"fp64 ({name}): iters={st64['niter']} ms {[round(v, 1) for v in lat]}")
         if S64 is None:
             S64 = dS.to_host()
         fp64[name] = {"ms_per_step": float(np.median(lat)), "ms_all": lat, "iters": int(st64["niter"]),
-                      "precondition": "near-field" if pc == 2 else "diagonal"}
+                      "precondition": PRECONDITIONERS[pc]}
     fp64_kr = min(fp64, key=lambda k: fp64[k]["ms_per_step"])
     fp64_ms = fp64[fp64_kr]["ms_per_step"]
     fp64_diag_ms = min(fp64["columns"]["ms_per_step"], fp64["block"]["ms_per_step"])
@@ -471,7 +475,8 @@ def main():
     mixed = {}
     for name, kr, rec, ver, pc in (("columns", "columns", True, False, 2), ("columns_verified", "columns", True, True, 2),
                                    ("columns_restarted", "columns", False, False, 2), ("block", "block", True, False, 2),
-                                   ("columns_diagonal", "columns", True, False, 1)):
+                                   ("columns_diagonal", "columns", True, False, 1),
+                                   ("columns_near2", "columns", True, False, 3), ("block_near2", "block", True, False, 3)):
         kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol, krylov=kr, recycle=rec, verify=ver,
                   check_residual=False, precondition=pc)
         lat, wall, st = timed(kw, a.steps, a.warmup)
@@ -575,7 +580,7 @@ def main():
                                "refinements": v["st"]["nrefine"], "value": a.K * a.steps / v["wall"],
                                "max_rel_err_vs_fp64": float(v["err"].max()), "true_max_rel_residual": v["res"],
                                "verify": bool(v["kw"]["verify"]),
-                               "precondition": "near-field" if v["kw"]["precondition"] == 2 else "diagonal"}
+                               "precondition": PRECONDITIONERS[v["kw"]["precondition"]]}
                            for k, v in mixed.items()},
         "speedup_vs_fp64_diagonal": value / (a.K / (fp64_diag_ms * 1e-3)),
         "other_precisions": variants,

@@ -151,12 +151,12 @@ struct GmresWork {
     DBuf H, G, tau, Y;  // block GMRES: block Hessenberg, projected right-hand side, reflectors, coefficients
     DBuf gpart, R, info;  // block GMRES: split-K partial products, K x K triangle, rocSOLVER info
     DBuf PV, PT;          // block GMRES: Householder vectors (2K x K) and T factors (K x K) per Hessenberg step
-    DBuf PY, PX;          // near-field preconditioner: its near product (N x K FP64), staging of its input (FP32
-                          // copy, or the split-FP16 operands)
+    DBuf PY, PX, PZ;      // near-field preconditioner: its near product (N x K FP64), staging of its input (FP32
+                          // copy, or the split-FP16 operands), the first sweep's result (two sweeps)
     // take the workspaces from (and return them to) the calling thread's block cache
     void set_pooled() {
         for (DBuf *d : {&V, &W, &Z, &U, &partial, &hbuf, &nbuf, &V1, &abuf, &cbuf, &om, &n1, &H, &G, &tau, &Y, &gpart,
-                        &R, &info, &PV, &PT, &PY, &PX})
+                        &R, &info, &PV, &PT, &PY, &PX, &PZ})
             d->pooled = true;
     }
 };
@@ -1138,9 +1138,11 @@ struct Precond {
     bool use_near = false;
     const double *w = nullptr;      // quadrature weights: the split-FP16 form multiplies w .* x
     bool h3 = false;                // near form on the split-FP16 blocks of the low-precision operator
+    int sweeps = 1;                 // near form: Jacobi sweeps z <- z0 - D^-1 N z from z = z0 = D^-1 in
 };
 
-// out = M in (in and out may alias)
+// out = M in (in and out may alias).  Near form: z0 = D^-1 in, then each sweep z <- z0 - D^-1 N z
+// (one sweep: the first Neumann term of (D + N)^-1; two: the second as well).
 template <typename TV>
 void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, int64_t N, int K) {
     hipStream_t st = op->stream;
@@ -1148,13 +1150,20 @@ void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, in
     if (!pc.use_near) return;
     GmresWork &ws = op->gw;
     if (pc.h3 && K >= 16) {
-        // out <- -(1/t) .* (-t .* out + N out), in place: N out from out's split-FP16 operands (the
-        // three products of the operator's near field), the scaling in the near kernel's epilogue
+        // z <- -(1/t) .* (N z - t .* z0) with z0 = out: N z from z's split-FP16 operands (the three
+        // products of the operator's near field), the scaling in the near kernel's epilogue; the last
+        // sweep writes out in place (each element read and written by one thread)
         ws.PX.ensure(far_field_mfma_scratch_bytes(N, (int)DBIE_FP16X3, K));
-        f16_split_operands(st, N, K, out, N, pc.w, ws.PX.p);
+        if (pc.sweeps > 1) ws.PY.ensure(sizeof(double) * (size_t)N * K);
         NearH3 h3;
         far_f16_operands(ws.PX.p, N, K, &h3);
-        near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, out, N, nullptr, &h3, pc.ndinv);
+        const double *z = out;
+        for (int s = 0; s < pc.sweeps; ++s) {
+            f16_split_operands(st, N, K, z, N, pc.w, ws.PX.p);
+            double *y = s + 1 < pc.sweeps ? ws.PY.as<double>() : out;
+            near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, y, N, nullptr, &h3, pc.ndinv);
+            z = y;
+        }
         return;
     }
     ws.PY.ensure(sizeof(double) * (size_t)N * K);
@@ -1163,8 +1172,18 @@ void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, in
         ws.PX.ensure(near_xt_bytes(N, K));
         xt = ws.PX.as<float>();
     }
-    near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, ws.PY.as<double>(), N, xt, nullptr);
-    row_scale(st, pc.ndinv, ws.PY.as<double>(), out, N, K);
+    if (pc.sweeps == 1) {
+        near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, ws.PY.as<double>(), N, xt, nullptr);
+        row_scale(st, pc.ndinv, ws.PY.as<double>(), out, N, K);
+        return;
+    }
+    // two sweeps: z1 = z0 - D^-1 N z0 into PZ, then out = z0 - D^-1 N z1 = z0 - z1 - D^-1 (N z1 - t .* z1)
+    ws.PZ.ensure(sizeof(double) * (size_t)N * K);
+    double *z1 = ws.PZ.as<double>(), *py = ws.PY.as<double>();
+    near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, py, N, xt, nullptr);
+    row_scale(st, pc.ndinv, py, z1, N, K);
+    near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, z1, N, pc.negt, py, N, xt, nullptr);
+    jacobi_update(st, pc.ndinv, py, z1, out, N, K);
 }
 
 // GMRES(m) for K independent columns sharing each block matvec.  Right
@@ -1776,8 +1795,9 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     if (!(o.rtol >= 0) || !(o.atol >= 0)) fail(DBIE_BAD_ARG, "tolerances must be >= 0");
     check_precision(o.precision);
     if (o.krylov != 0 && o.krylov != 1) fail(DBIE_BAD_ARG, "krylov must be 0 (per column) or 1 (block)");
-    if (o.precondition < 0 || o.precondition > 2)
-        fail(DBIE_BAD_ARG, "precondition must be 0 (none), 1 (diagonal scaling) or 2 (near-field)");
+    if (o.precondition < 0 || o.precondition > 3)
+        fail(DBIE_BAD_ARG, "precondition must be 0 (none), 1 (diagonal scaling), 2 (near-field) or 3 (near-field, "
+                           "two sweeps)");
     hipStream_t st = op->stream;
     const size_t NK = (size_t)N * K;
     DBuf dR(true), dD(true), dT(true), dinv(true), partial(true), hb(true);
@@ -1786,7 +1806,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     Precond pc64, pcl;
     const Precond *p64 = nullptr, *plo = nullptr;
     if (o.precondition) {
-        const bool near = o.precondition == 2 && op->members.empty() && op->bsr.nb > 0;
+        const bool near = o.precondition >= 2 && op->members.empty() && op->bsr.nb > 0;
         std::vector<double> inv((size_t)N * (near ? 3 : 1));
         for (int64_t i = 0; i < N; ++i) {
             inv[i] = 1.0 / op->diag_h[i];
@@ -1801,6 +1821,7 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
             pc64.ndinv = dinv.as<double>() + N;
             pc64.negt = dinv.as<double>() + 2 * N;
         }
+        pc64.sweeps = o.precondition == 3 ? 2 : 1;
         pcl = pc64;
         if (near && o.precision == DBIE_FP64) {
             if (!op->full64_built) {

@@ -189,6 +189,8 @@ void row_scale(hipStream_t st, const double *d, const float *in, double *out, in
 void block_combine(hipStream_t st, const double *V, int nv, const double *y, double *X, int64_t N, int K);
 void block_combine(hipStream_t st, const float *V, int nv, const double *y, double *X, int64_t N, int K);
 // R = F - R (residual from AX stored in R), column-wise
+// out = out - z + s .* py (row scale s)
+void jacobi_update(hipStream_t st, const double *s, const double *py, const double *z, double *out, int64_t N, int K);
 void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K);
 // X += D
 void add_to(hipStream_t st, const double *D, double *X, int64_t N, int K);

@@ -287,6 +287,14 @@ __global__ void row_scale_kernel(const double *__restrict__ d, const TI *__restr
     if (n < N) out[(int64_t)k * N + n] = (d ? d[n] : 1.0) * (double)in[(int64_t)k * N + n];
 }
 
+// out = out - z + s .* py (the second Jacobi sweep of the near-field preconditioner)
+__global__ void jacobi_update_kernel(const double *__restrict__ s, const double *__restrict__ py,
+                                     const double *__restrict__ z, double *__restrict__ out, int64_t N, int K) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = (int64_t)blockIdx.y * N + n;
+    if (n < N) out[idx] = out[idx] - z[idx] + s[n] * py[idx];
+}
+
 __global__ void sub_from_kernel(const double *__restrict__ F, double *__restrict__ R, int64_t N, int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -436,6 +444,10 @@ void row_scale(hipStream_t st, const double *d, const double *in, double *out, i
 }
 void row_scale(hipStream_t st, const double *d, const float *in, double *out, int64_t N, int K) {
     row_scale_kernel<float><<<grid2(N, K), 256, 0, st>>>(d, in, out, N, K);
+}
+
+void jacobi_update(hipStream_t st, const double *s, const double *py, const double *z, double *out, int64_t N, int K) {
+    jacobi_update_kernel<<<grid2(N, K), 256, 0, st>>>(s, py, z, out, N, K);
 }
 
 void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K) {

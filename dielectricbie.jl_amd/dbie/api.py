@@ -562,9 +562,10 @@ def solve_fused(ops, Fs, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precisio
 
 def _precond_code(precondition) -> int:
     """True / 1 / "diagonal": diagonal scaling (the reference's); 2 / "near": the near-field Neumann term;
-    False / 0 / "none": no preconditioner."""
+    3 / "near2": two Jacobi sweeps of the near field (two Neumann terms); False / 0 / "none": no
+    preconditioner."""
     if isinstance(precondition, str):
-        return {"none": 0, "diagonal": 1, "near": 2}[precondition]
+        return {"none": 0, "diagonal": 1, "near": 2, "near2": 3}[precondition]
     return int(precondition)
 
 

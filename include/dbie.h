@@ -87,7 +87,9 @@ typedef struct {
                                 (dielectric_diagonal_scaling, default), 2 = near-field: one Neumann
                                 term of (diag(t) + N)^-1, N the near panel pairs' full interaction (one
                                 near-field product per iteration; GMRES needs ~40% fewer iterations;
-                                no counterpart in the reference), 0 = none */
+                                no counterpart in the reference), 3 = near-field with two Jacobi
+                                sweeps z <- D^-1 (x - N z) (both Neumann terms, two near-field
+                                products), 0 = none */
     dbie_precision precision;/* DBIE_FP64 = plain GMRES; otherwise mixed-precision IR */
     double inner_rtol;       /* mixed: inner-solve relative tolerance (default 1e-4) */
     int max_refine;          /* mixed: maximum FP64 refinement steps (default 10) */

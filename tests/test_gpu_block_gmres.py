@@ -31,7 +31,8 @@ def cfg(dbie, O):
     return g, op, ref_if, Ad
 
 
-@pytest.mark.parametrize("K,pre", [(1, True), (5, True), (16, False), (40, True), (1, 2), (5, 2), (40, 2)])
+@pytest.mark.parametrize("K,pre", [(1, True), (5, True), (16, False), (40, True), (1, 2), (5, 2), (40, 2), (5, 3),
+                                   (40, 3)])
 def test_block_gmres_matches_oracle(dbie, O, cfg, K, pre):
     g, op, ref_if, Ad = cfg
     rng = np.random.default_rng(60 + K)

@@ -112,8 +112,9 @@ def test_unverified_stop_meets_the_tolerance_in_every_form(small, precision):
 @pytest.mark.parametrize("precision,krylov", [("fp64", "columns"), ("fp64", "block"), ("fp16x3", "columns"),
                                               ("fp16x3", "block")])
 def test_near_field_preconditioner_k256_vs_oracle_lu(small, precision, krylov):
-    """precondition=2 (one Neumann term of the near-field inverse): the same solutions (FP64 1e-10, mixed
-    1e-8 per column against the oracle's LU) in fewer iterations than the diagonal scaling."""
+    """precondition=2 (one Neumann term of the near-field inverse) and 3 (two Jacobi sweeps, both terms):
+    the same solutions (FP64 1e-10, mixed 1e-8 per column against the oracle's LU) in fewer iterations
+    than the diagonal scaling."""
     g, op, F, S_ref = small
     say(f"{precision} ({krylov}) with the near-field preconditioner, N=2560 K=256 vs oracle LU")
     kw = dict(rtol=1e-10, precision=precision, krylov=krylov, check_residual=True)
@@ -127,6 +128,14 @@ def test_near_field_preconditioner_k256_vs_oracle_lu(small, precision, krylov):
     assert st2["solved"] and st2["max_rel_residual"] <= 1e-10
     assert e2.max() < (1e-10 if precision == "fp64" else 1e-8)
     assert st2["niter"] < st1["niter"]
+    # two Jacobi sweeps (precondition=3): the same bars, no more iterations than one sweep
+    S3, st3 = op.solve(F, precondition=3, **kw)
+    e3 = col_rel(S3, S_ref)
+    say(f"  near-field, two sweeps: {st3['niter']} iterations, true residual {st3['max_rel_residual']:.2e}, "
+        f"max column error {e3.max():.2e}")
+    assert st3["solved"] and st3["max_rel_residual"] <= 1e-10
+    assert e3.max() < (1e-10 if precision == "fp64" else 1e-8)
+    assert st3["niter"] <= st2["niter"]
 
 
 def test_mixed_k256_device_resident(dbie, small):

@@ -119,7 +119,8 @@ def test_failures_stay_with_their_request(dbie):
         srv.cancel(rs[-1])
         last = srv.wait(rs[-1])
         assert last["state"] == "cancelled"
-        assert all(srv.wait(r)["state"] == "done" for r in rs[:-1])
+        done = [srv.wait(r) for r in rs[:-1]]
+        assert all(i["state"] == "done" for i in done), done
 
 
 def test_serving_stream_small(dbie):
