@@ -195,7 +195,7 @@ struct dbie_op {
     GmresWork gw;                 // GMRES workspaces, reused across solves
     double setup_ms[4] = {0, 0, 0, 0};  // near pairs, correction blocks (host part), derived data, uploads + device block build
     float f16_gscale = 0.0f;  // power-of-two scale of the fp16 kernel entries (0 = not yet computed)
-    // the split-FP16 far field's kernel matrix kept in HBM (launch variants 2 and 3, K > 128): built on
+    // the split-FP16 far field's kernel matrix kept in HBM (launch variants 2 - 4, K > 128): built on
     // the first apply whose launch shape uses it.  stored_mode 1: when device memory allows (default;
     // DBIE_STORED_FAR=0 starts operators with 0), 0: never.  gst_state 0: not built, 1: built, -1:
     // declined for this operator (not enough free device memory)
@@ -588,7 +588,7 @@ TuneKey tune_key(const dbie_op *op, int K, dbie_precision prec, bool exclude) {
 // Candidate launch shapes for one apply: split counts the launch accepts whose partial sums fit
 // in 2 GB, each with the default kernel and (split-FP16, K > 128) the general kernel, the
 // warp-specialised kernel with one consumer wave per SIMD and, at one or two splits, the GEMM over
-// the stored kernel matrix with four or eight waves per workgroup (an operator that keeps no
+// the stored kernel matrix in its three forms (an operator that keeps no
 // stored matrix runs those as the warp-specialised kernel).
 std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
     std::vector<FarTune> cand;
@@ -604,6 +604,7 @@ std::vector<FarTune> tune_candidates(int64_t N, int K, dbie_precision prec) {
             if (sp <= 2) {
                 cand.push_back(FarTune{sp, 2});
                 cand.push_back(FarTune{sp, 3});
+                cand.push_back(FarTune{sp, 4});
             }
         }
     }
@@ -2674,7 +2675,7 @@ dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision
     return guard([&] {
         check_op(op);
         if (K < 1 || splits < 1) fail(DBIE_BAD_ARG, "K and splits must be >= 1");
-        if (variant < -1 || variant > 3) fail(DBIE_BAD_ARG, "variant must be -1 ... 3");
+        if (variant < -1 || variant > 4) fail(DBIE_BAD_ARG, "variant must be -1 ... 4");
         check_precision(precision);
         TuneState ts;
         ts.cand = {FarTune{splits, variant}};

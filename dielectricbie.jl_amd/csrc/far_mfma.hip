@@ -1253,20 +1253,25 @@ __global__ __launch_bounds__(CW * 64) void far_gs_f16x3_kernel(const int *__rest
 }
 
 // Streamed GEMM with the stored entries loaded straight into registers: 256 targets x 256 columns
-// per workgroup, four waves (one per SIMD); wave w owns target slices 4 w .. 4 w + 3 (64 targets) and
-// all 16 column blocks (256 accumulator registers).  A wave's part of a stage's block (its 4 slices,
-// hi and lo: 8 KiB, contiguous) is used by that wave alone, so it goes from HBM into registers with
-// non-temporal loads three stages ahead (four register sets), without passing through the LDS; only
-// the B operands, which all four waves read, are copied global -> LDS, into a four-deep ring, also
-// three stages ahead.  The copies of the LDS path then carry 32 KiB per stage instead of 64, and each
-// CU keeps about 190 KiB of loads in flight.  Accumulation order per output as in
-// far_ws_f16x3_kernel (bit-identical partial sums at the same source split).
-constexpr int kGrWaves = 4;
-__global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
+// per workgroup, W = 4 waves (one per SIMD) or 8 (two per SIMD); wave w owns target slices RS w ..
+// RS w + RS - 1 (RS = 16 / W: 64 or 32 targets) and all 16 column blocks (16 RS accumulator
+// registers per lane).  A wave's part of a stage's block (its RS slices, hi and lo: 2 RS KiB,
+// contiguous) is used by that wave alone, so it goes from HBM into registers with non-temporal loads
+// three stages ahead (four register sets), without passing through the LDS; only the B operands,
+// which all waves read, are copied global -> LDS, into a four-deep ring, also three stages ahead.
+// The copies of the LDS path then carry 32 KiB per stage instead of 64, and each CU keeps about
+// 190 KiB of loads in flight.  With W = 8 a SIMD has a second wave to issue MFMAs while one waits.
+// Accumulation order per output as in far_ws_f16x3_kernel (bit-identical partial sums at the same
+// source split).
+template <int W>
+__global__ __launch_bounds__(W * 64) void far_gr_f16x3_kernel(
     const int *__restrict__ tplane, int64_t N, int K, int Kp, const _Float16 *__restrict__ zh,
     const _Float16 *__restrict__ zl, const unsigned *__restrict__ cmax, float gscale, double *__restrict__ part,
     int src_per_split, const uint4 *__restrict__ gst, const int *__restrict__ tile_off, int ntile) {
-    constexpr int NC = kGsChunks / kGrWaves;  // B chunks each wave copies per stage (8)
+    static_assert(W == 4 || W == 8, "4 or 8 waves");
+    constexpr int NC = kGsChunks / W;  // B chunks each wave copies per stage (8 or 4)
+    constexpr int RS = 16 / W;         // target slices per wave
+    constexpr int NA = 2 * RS;         // A chunks per wave and stage (hi and lo per slice)
     __shared__ uint4 s_b0[kGsChunks][64], s_b1[kGsChunks][64], s_b2[kGsChunks][64], s_b3[kGsChunks][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int T = blockIdx.x;
@@ -1280,7 +1285,7 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
     // the per-lane offset fixed, so no vector address is computed (or waited for) per stage
     const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(gst + (int64_t)t0 * 8 * kGsBlockU4), (short)0, (toff[ntile] - t0) * 8 * kGsBlockBytes, 0x00020000);
-    const int aoff = ((NC * wave) * 64 + lane) * 16;
+    const int aoff = ((NA * wave) * 64 + lane) * 16;
     typedef __attribute__((address_space(3))) void *lptr_t;
     const int zbytes = (int)(((N + kSb - 1) / kSb) * Kp * kSb * 2);  // the host keeps this below 2^31
     const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc((void *)zh, (short)0, zbytes, 0x00020000);
@@ -1294,20 +1299,20 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
             __builtin_amdgcn_raw_ptr_buffer_load_lds((c & 1) ? rl : rh, (lptr_t)&sb[ch][0], 16, loff, soff, 0, 0);
         }
     };
-    // chunk c of this wave: target slice 4 wave + c / 2, hi (c even) or lo (c odd)
-    auto load_a = [&](int64_t js, uint4 (&a)[8]) {
+    // chunk c of this wave: target slice RS wave + c / 2, hi (c even) or lo (c odd)
+    auto load_a = [&](int64_t js, uint4 (&a)[NA]) {
         const int soff = ((toff[js / 256] - t0) * 8 + (int)((js % 256) / 32)) * kGsBlockBytes;
 #pragma unroll
-        for (int c = 0; c < 8; ++c)  // non-temporal (aux 2): the stream is read once
+        for (int c = 0; c < NA; ++c)  // non-temporal (aux 2): the stream is read once
             a[c] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, aoff + c * 1024, soff, 2));
     };
-    v4f acc[4][16];
+    v4f acc[RS][16];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < RS; ++r)
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[r][q] = v4f{0.f, 0.f, 0.f, 0.f};
-    // column blocks in pairs: the B operands of the next pair are read before this pair's 24 MFMAs
-    auto compute = [&](const uint4 (&sb)[kGsChunks][64], const uint4 (&a)[8]) {
+    // column blocks in pairs: the B operands of the next pair are read before this pair's 6 RS MFMAs
+    auto compute = [&](const uint4 (&sb)[kGsChunks][64], const uint4 (&a)[NA]) {
         uint4 bb[2][4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) bb[0][u] = sb[u][lane];
@@ -1323,7 +1328,7 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
                 const v8h bh = __builtin_bit_cast(v8h, bb[p & 1][2 * h]);
                 const v8h bl = __builtin_bit_cast(v8h, bb[p & 1][2 * h + 1]);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
+                for (int r = 0; r < RS; ++r) {
                     const v8h xh = __builtin_bit_cast(v8h, a[2 * r]);
                     const v8h xl = __builtin_bit_cast(v8h, a[2 * r + 1]);
                     v4f t = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh, acc[r][q], 0, 0, 0);
@@ -1333,7 +1338,7 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
             }
             if (p == 0) __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
             else if (p < 7) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 6 * RS, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
     };
@@ -1341,9 +1346,10 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
     int64_t j1 = after(jc);
     int64_t j2 = after(j1);
     int64_t j3 = after(j2);
-    uint4 a0[8], a1[8], a2[8], a3[8];
+    uint4 a0[NA], a1[NA], a2[NA], a3[NA];
     // Loads are issued without branches (a stage past the walk loads the current stage again into a
     // slot no later stage reads), so that the compiler's wait analysis stays exact across the loop.
+    // Per stage a wave issues NC copies and NA = NC loads.
     if (jc < walk.jend) {
         copy_b(jc, s_b0);
         load_a(jc, a0);
@@ -1360,8 +1366,8 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
     // one stage: B in ring slot b, A in register set a; B(j3) into slot (b + 3) % 4 (read last in the
     // previous stage, before its barrier) and A(j3) into register set (a + 3) % 4 (consumed last in
     // the previous stage)
-    auto stage = [&](const uint4 (&sb)[kGsChunks][64], const uint4 (&a)[8], uint4 (&nb)[kGsChunks][64],
-                     uint4 (&na)[8]) {
+    auto stage = [&](const uint4 (&sb)[kGsChunks][64], const uint4 (&a)[NA], uint4 (&nb)[kGsChunks][64],
+                     uint4 (&na)[NA]) {
         const int64_t jn = j3 < walk.jend ? j3 : jc;
         copy_b(jn, nb);
         load_a(jn, na);
@@ -1390,10 +1396,10 @@ __global__ __launch_bounds__(kGrWaves * 64) void far_gr_f16x3_kernel(
         const double cs = ldexp(out_scale, -col_exponent(cmax[col]));
         double *dst = part + ((int64_t)blockIdx.z * K + col) * N;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < RS; ++r)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t i = (int64_t)T * kGsT + (4 * wave + r) * 16 + 4 * (lane >> 4) + e;
+                const int64_t i = (int64_t)T * kGsT + (RS * wave + r) * 16 + 4 * (lane >> 4) + e;
                 if (i < N) dst[i] = (double)acc[r][q][e] * cs;
             }
     }
@@ -1590,8 +1596,11 @@ int far_field_mfma(hipStream_t st, const DevGeom &g, int prec, int K, const doub
             const dim3 gg((unsigned)fs->ntile, (unsigned)by, (unsigned)S);
             const uint4 *gst = static_cast<const uint4 *>(fs->g);
             if (tune.variant == 2)
-                far_gr_f16x3_kernel<<<gg, kGrWaves * 64, 0, st>>>(g.tile_plane, N, K, Kp, zh, zl, cmax, gscale, part,
-                                                                  (int)per, gst, fs->tile_off, fs->ntile);
+                far_gr_f16x3_kernel<4><<<gg, 4 * 64, 0, st>>>(g.tile_plane, N, K, Kp, zh, zl, cmax, gscale, part,
+                                                              (int)per, gst, fs->tile_off, fs->ntile);
+            else if (tune.variant == 4)
+                far_gr_f16x3_kernel<8><<<gg, 8 * 64, 0, st>>>(g.tile_plane, N, K, Kp, zh, zl, cmax, gscale, part,
+                                                              (int)per, gst, fs->tile_off, fs->ntile);
             else
                 far_gs_f16x3_kernel<8><<<gg, 512, 0, st>>>(g.tile_plane, N, K, Kp, zh, zl, cmax, gscale, part, (int)per,
                                                            gst, fs->tile_off, fs->ntile);

@@ -53,15 +53,15 @@ enum FarMode { FAR_DT = 0, FAR_S = 1, FAR_D = 2 };
 // Launch shape of a far-field contraction, as picked by the autotuner: splits = number of
 // source splits (0: the built-in heuristic), variant = -1: the warp-specialised split-FP16
 // kernel where it applies (two consumer waves per SIMD), 1: the same with one consumer wave per
-// SIMD, 0: never, 2 / 3: the streamed GEMM over the stored kernel matrix (four / eight waves per
-// workgroup) where the operator keeps one, else as 1.
+// SIMD, 0: never, 2 / 3 / 4: the streamed GEMM over the stored kernel matrix (2: four waves per
+// workgroup, the stored entries straight into registers; 3: eight waves, through the LDS; 4: eight
+// waves, into registers) where the operator keeps one, else as 1.
 struct FarTune {
     int splits = 0;
     int variant = -1;
 };
 
-// The split-FP16 far field's kernel matrix kept in HBM (launch variants 2: four waves, 3: eight
-// waves per workgroup; K > 128): per 256-target super-tile and per 32-source stage of each source
+// The split-FP16 far field's kernel matrix kept in HBM (launch variants 2, 3 and 4; K > 128): per 256-target super-tile and per 32-source stage of each source
 // tile of 256 nodes that is not on the super-tile's plane, a 32 KiB block of hi/lo fp16 entries in
 // MFMA operand order, as the generating kernels compute them.  tile_off (ntile x (ntile + 1) ints,
 // device): stored source tiles before (super-tile T, source tile st) in row-major order.

@@ -423,8 +423,9 @@ dbie_status dbie_autotune_choice(const dbie_op *op, int K, dbie_precision precis
 /* pins the choice for this operator's class at K columns (e.g. one saved from an earlier run):
    splits >= 1 (clamped to what the launch accepts), variant -1 = default kernel, 0 = the general
    split-FP16 kernel, 1 = the warp-specialised split-FP16 kernel with one consumer wave per SIMD,
-   2 / 3 = the GEMM over the stored split-FP16 kernel matrix with four / eight waves per workgroup
-   (see dbie_op_stored_far; as 1 where the operator keeps none) */
+   2 / 3 / 4 = the GEMM over the stored split-FP16 kernel matrix (2: four waves per workgroup, the
+   stored entries loaded into registers; 3: eight waves, through the LDS; 4: eight waves, into
+   registers; see dbie_op_stored_far; as 1 where the operator keeps none) */
 dbie_status dbie_autotune_set(const dbie_op *op, int K, dbie_precision precision, int splits, int variant);
 
 /* ---- HIP graphs ------------------------------------------------------------ */

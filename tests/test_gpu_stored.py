@@ -1,4 +1,4 @@
-"""GPU checks of the stored split-FP16 kernel matrix (launch variants 2 and 3): the streamed GEMM
+"""GPU checks of the stored split-FP16 kernel matrix (launch variants 2, 3 and 4): the streamed GEMM
 over the stored hi/lo fp16 entries gives the same partial sums, bit for bit, as the warp-specialised
 kernel that generates the entries on the fly (panels of 16 or more nodes), at the same source split,
 and agrees to rounding with the general generating kernel (9-node panels); the operator meets the
@@ -53,7 +53,7 @@ def test_stored_matrix_equals_generated(dbie, O, geom, K):
             Yg3 = Yg.copy()
             Yg3[:, 3] = 1.0
             assert col_err(Yg3, Yref3) < BAR
-            for var in (2, 3):
+            for var in (2, 3, 4):
                 say(f"  splits={splits} variant={var}")
                 op.set_tuned_shape(K, "fp16x3", splits, var)
                 Ys = op.apply(X, "fp16x3")
@@ -91,7 +91,7 @@ def test_stored_matrix_config3_size(dbie):
         for graphs in (False, True):
             op.graphs(graphs)
             for splits in (1, 2):
-                for var in (2, 3):
+                for var in (2, 3, 4):
                     say(f"  graphs={graphs} splits={splits} variant={var}")
                     op.set_tuned_shape(256, "fp16x3", splits, var)
                     for it in range(3):
