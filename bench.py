@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--mode", default="solve", choices=["solve", "apply", "headline"],
                     help="apply: time only operator applies (one per precision) for kernel profiling")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--krylov", default="columns", choices=["columns", "block"],
+                    help="headline mode: GMRES per column (the headline form) or block GMRES")
     ap.add_argument("--precondition", type=int, default=2, choices=[1, 2, 3],
                     help="headline mode: the preconditioner (1 diagonal, 2 near field, 3 near field, two sweeps)")
     ap.add_argument("--shape", default="", help="apply mode: pin the low-precision far-field launch shape "
@@ -401,7 +403,7 @@ def main():
                 op.apply(F, prec)
                 if op.tuned_shape(a.K, prec) is not None:
                     break
-        kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol, krylov="columns", recycle=True,
+        kw = dict(rtol=1e-10, precision=a.precision, inner_rtol=a.inner_rtol, krylov=a.krylov, recycle=True,
                   verify=False, check_residual=False, precondition=a.precondition)
         for _ in range(a.warmup):
             op.solve_device(dF, dS, **kw)

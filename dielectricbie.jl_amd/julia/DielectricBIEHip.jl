@@ -163,7 +163,9 @@ end
 
 # solve_dielectric_box3d_block on the device (src/solver/multi_rhs.jl:270-286)
 function solve_block(op::HipDielectricOperator, F::Matrix{Float64}; rtol = 1e-10, atol = 0.0,
-                     itmax = 500, precondition = true, precision = :fp64, inner_rtol = 1e-5, max_refine = 10,
+                     itmax = 500,
+                     precondition = true,  # true / 1 diagonal scaling, 2 near field, 3 near field in two sweeps, 0 none
+                     precision = :fp64, inner_rtol = 1e-5, max_refine = 10,
                      krylov = 1,   # 1 = block GMRES as Krylov.block_gmres, 0 = per column
                      recycle = 1,  # mixed precision, per column: later refinements deflate the first space
                      verify = 1)   # mixed precision: 1 = end on an FP64 residual, 0 = on the Krylov estimate
