@@ -328,7 +328,7 @@ def main():
                          "serve: a seeded Poisson stream of box requests through the request server; "
                          "single: one point charge, eps_in 10, FP64 GMRES to 1e-10 (solve time)")
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=4.0, help="serve: arrival rate (requests/s)")
+    ap.add_argument("--rate", type=float, default=64.0, help="serve: arrival rate (requests/s; 64 saturates one MI355X)")
     # 4 solve slots and 3 setup workers: served throughput of the config-4 stream levels off there
     # (32 requests at 64/s, mixed: 2/2 666-769, 3/2 795, 4/3 835, 6/3 851, 8/4 843 RHS-solves/s)
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent solve slots")

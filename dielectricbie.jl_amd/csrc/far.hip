@@ -89,12 +89,15 @@ __global__ __launch_bounds__(kTile) void far_f64_kernel(DevGeom g, int K, const 
         if (k0 + c < K) part[((int64_t)blockIdx.z * K + k0 + c) * N + i] = acc[c] * kInv4Pi;
 }
 
-// FP32 tile generation and accumulation (mixed-precision inner operator, DT only).
+// FP32 tile generation and accumulation (mixed-precision inner operator, DT only).  A source's KC
+// charges are contiguous in LDS (KC / 4 vector reads per pair), and 1/r is the hardware reciprocal
+// square root (about 1 ulp) of r^2 clamped at 1e-20: the self pair has n.(x - y) = 0 and stays 0.
 template <int KC>
 __global__ __launch_bounds__(kTile) void far_f32_kernel(DevGeom g, int K, const double *__restrict__ X, int64_t ldx,
                                                         double *__restrict__ part, int src_per_split) {
+    static_assert(KC == 1 || KC == 2 || KC % 4 == 0, "charges per source: 1, 2 or a multiple of 4");
     __shared__ float4 s_pos[kTile];
-    __shared__ float s_q[KC][kTile];
+    __shared__ __align__(16) float s_q[kTile][KC];
     const int64_t N = g.N;
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kTile + tid;
@@ -117,18 +120,29 @@ __global__ __launch_bounds__(kTile) void far_f32_kernel(DevGeom g, int K, const 
             s_pos[tid] = make_float4(g.xf[j], g.yf[j], g.zf[j], 0.0f);
             const double wj = g.w[j];
 #pragma unroll
-            for (int c = 0; c < KC; ++c) s_q[c][tid] = (k0 + c < K) ? (float)(wj * X[j + (int64_t)(k0 + c) * ldx]) : 0.0f;
+            for (int c = 0; c < KC; ++c) s_q[tid][c] = (k0 + c < K) ? (float)(wj * X[j + (int64_t)(k0 + c) * ldx]) : 0.0f;
         }
         __syncthreads();
         const int nt = (int)min<int64_t>(kTile, jend - jb);
         for (int jj = 0; jj < nt; ++jj) {
             const float4 sp = s_pos[jj];
             const float dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const float rinv = r2 > 0.0f ? __frsqrt_rn(r2) : 0.0f;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-20f)));
+            const float rinv = __builtin_amdgcn_rsqf(r2);
             const float gk = (tnx * dx + tny * dy + tnz * dz) * (rinv * rinv * rinv);
+            if constexpr (KC % 4 == 0) {
 #pragma unroll
-            for (int c = 0; c < KC; ++c) acc[c] += gk * s_q[c][jj];
+                for (int c = 0; c < KC; c += 4) {
+                    const float4 q = *reinterpret_cast<const float4 *>(&s_q[jj][c]);
+                    acc[c] += gk * q.x;
+                    acc[c + 1] += gk * q.y;
+                    acc[c + 2] += gk * q.z;
+                    acc[c + 3] += gk * q.w;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < KC; ++c) acc[c] += gk * s_q[jj][c];
+            }
         }
     }
     if (!valid) return;
@@ -188,13 +202,15 @@ int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const d
 
 int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int64_t ldx, double *part,
                   int64_t part_cap, const FarTune &tune) {
-    const int KC = K >= 8 ? 8 : (K >= 4 ? 4 : (K >= 2 ? 2 : 1));
+    // charges per source and workgroup: up to 16, so that K <= 16 computes each pair's kernel once
+    const int KC = K > 8 ? 16 : (K > 4 ? 8 : (K >= 3 ? 4 : (K == 2 ? 2 : 1)));
     return far_launch(g.N, K, KC, part_cap, tune.splits, [&](dim3 grid, int per) {
         switch (KC) {
         case 1: far_f32_kernel<1><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
         case 2: far_f32_kernel<2><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
         case 4: far_f32_kernel<4><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
-        default: far_f32_kernel<8><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
+        case 8: far_f32_kernel<8><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
+        default: far_f32_kernel<16><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
         }
     });
 }
