@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 
 #include "kernels.hpp"
 
@@ -92,7 +93,10 @@ __global__ __launch_bounds__(kTile) void far_f64_kernel(DevGeom g, int K, const 
 // FP32 tile generation and accumulation (mixed-precision inner operator, DT only).  A source's KC
 // charges are contiguous in LDS (KC / 4 vector reads per pair), and 1/r is the hardware reciprocal
 // square root (about 1 ulp) of r^2 clamped at 1e-20: the self pair has n.(x - y) = 0 and stays 0.
-template <int KC>
+// EXCL (16-node panels): the near panel pairs of g.near_mask are left out, as the split-FP16 far
+// field leaves them out, and the caller adds their full interaction; a 256-source tile covers 16
+// source panels, 16 bits of the target panel's mask row.
+template <int KC, bool EXCL>
 __global__ __launch_bounds__(kTile) void far_f32_kernel(DevGeom g, int K, const double *__restrict__ X, int64_t ldx,
                                                         double *__restrict__ part, int src_per_split) {
     static_assert(KC == 1 || KC == 2 || KC % 4 == 0, "charges per source: 1, 2 or a multiple of 4");
@@ -124,12 +128,16 @@ __global__ __launch_bounds__(kTile) void far_f32_kernel(DevGeom g, int K, const 
         }
         __syncthreads();
         const int nt = (int)min<int64_t>(kTile, jend - jb);
+        // source panels jb / 16 .. jb / 16 + 15 (jb is a multiple of 256): bits 0-15 or 16-31 of one word
+        const uint32_t mbits =
+            EXCL ? g.near_mask[(ii >> 4) * (int64_t)g.mask_words + (jb >> 9)] >> ((jb >> 4) & 31) : 0u;
         for (int jj = 0; jj < nt; ++jj) {
             const float4 sp = s_pos[jj];
             const float dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-20f)));
             const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float gk = (tnx * dx + tny * dy + tnz * dz) * (rinv * rinv * rinv);
+            float gk = (tnx * dx + tny * dy + tnz * dz) * (rinv * rinv * rinv);
+            if (EXCL) gk = ((mbits >> (jj >> 4)) & 1u) ? 0.0f : gk;
             if constexpr (KC % 4 == 0) {
 #pragma unroll
                 for (int c = 0; c < KC; c += 4) {
@@ -201,17 +209,21 @@ int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const d
 }
 
 int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int64_t ldx, double *part,
-                  int64_t part_cap, const FarTune &tune) {
+                  int64_t part_cap, const FarTune &tune, bool exclude_near) {
+    if (exclude_near && !(g.near_mask && g.npp == 16))
+        throw std::logic_error("far_field_f32: near-pair exclusion needs 16-node panels and the near mask");
     // charges per source and workgroup: up to 16, so that K <= 16 computes each pair's kernel once
     const int KC = K > 8 ? 16 : (K > 4 ? 8 : (K >= 3 ? 4 : (K == 2 ? 2 : 1)));
     return far_launch(g.N, K, KC, part_cap, tune.splits, [&](dim3 grid, int per) {
+#define DBIE_F32(C)                                                                          \
+    case C:                                                                                  \
+        if (exclude_near) far_f32_kernel<C, true><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per);  \
+        else far_f32_kernel<C, false><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per);              \
+        break;
         switch (KC) {
-        case 1: far_f32_kernel<1><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
-        case 2: far_f32_kernel<2><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
-        case 4: far_f32_kernel<4><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
-        case 8: far_f32_kernel<8><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
-        default: far_f32_kernel<16><<<grid, kTile, 0, st>>>(g, K, X, ldx, part, per); break;
+            DBIE_F32(1) DBIE_F32(2) DBIE_F32(4) DBIE_F32(8) DBIE_F32(16)
         }
+#undef DBIE_F32
     });
 }
 

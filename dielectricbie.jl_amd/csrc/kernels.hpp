@@ -80,8 +80,10 @@ void far_stored_build(hipStream_t st, const DevGeom &g, float gscale, bool exclu
 // (G of the chosen mode, self pair excluded, 1/(4 pi) applied).  Returns the split count S.
 int far_field_f64(hipStream_t st, const DevGeom &g, FarMode mode, int K, const double *X, int64_t ldx,
                   double *part, int64_t part_cap, const FarTune &tune = FarTune());
+// exclude_near (16-node panels with a near mask): leave the near panel pairs out, as the split-FP16
+// far field does
 int far_field_f32(hipStream_t st, const DevGeom &g, int K, const double *X, int64_t ldx, double *part,
-                  int64_t part_cap, const FarTune &tune = FarTune());
+                  int64_t part_cap, const FarTune &tune = FarTune(), bool exclude_near = false);
 // MFMA contraction of the DT far field: tiles generated on the fly and contracted with the
 // RHS block.  prec: 0 = FP64 MFMA, 2 = BF16 MFMA, 3 = split BF16 (3 products), 4 = split FP16.
 // scratch: far_field_mfma_scratch_bytes(N, prec, K) bytes of device memory (the
