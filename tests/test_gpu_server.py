@@ -112,11 +112,15 @@ def test_failures_stay_with_their_request(dbie):
         info = srv.wait(r)
         assert info["status"] == 3 and "budget" in info["message"]
     with dbie.Server(setup_workers=1, solve_slots=1) as srv:
+        # the solve slots paused while the requests arrive and one is cancelled (queued, in setup or
+        # ready, never inside a batch): the cancellation alone decides its fate, the others complete
+        srv.pause(True)
         rs = []
         for j in range(4):
             p, q = charges(70 + j, 2, boxes[j][:3])
             rs.append(srv.submit_box(*boxes[j], p, q))
         srv.cancel(rs[-1])
+        srv.pause(False)
         last = srv.wait(rs[-1])
         assert last["state"] == "cancelled"
         done = [srv.wait(r) for r in rs[:-1]]
