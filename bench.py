@@ -285,6 +285,46 @@ def serve_child(a, log, timeout_s=240):
     return out
 
 
+def requests_workload(a, dbie, log):
+    """Single serving-size requests solved alone on the device (no server, no fusion): per shape the
+    FP64 and the split-FP16 solve times (best of 3 after 3 warm-ups), iterations, and the far- and
+    near-field time per apply; the operator keeps no stored kernel matrix, as served operators do."""
+    rng = np.random.default_rng(a.seed)
+    shapes = [((1.0, 0.8, 0.6), 0.1, 4), ((1.0, 0.8, 0.6), 0.1, 32), ((1.5, 1.2, 1.0), 0.07, 8),
+              ((1.5, 1.2, 1.0), 0.07, 48), ((1.8, 1.6, 1.4), 0.05, 4), ((1.8, 1.6, 1.4), 0.05, 64)]
+    rows = []
+    for L, lec, K in shapes:
+        g = dbie.single_dielectric_box3d(*L, 4, lec, 20.0, 1.0)
+        op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
+        op.stored_far(False)
+        half = np.asarray(L) / 2 - 0.05
+        F = dbie.rhs_point_charges(g, rng.uniform(-half, half, (K, 3)), rng.uniform(-1, 1, K), 20.0)
+        dF, dS = dbie.DeviceMatrix.from_host(F), dbie.DeviceMatrix(g.n_points, K)
+        row = {"sides": list(L), "l_ec": lec, "N": g.n_points, "K": K}
+        for name, kw in (("fp64", dict(precision="fp64")),
+                         ("mixed", dict(precision=a.precision, inner_rtol=a.inner_rtol, verify=False))):
+            for _ in range(3):
+                op.solve_device(dF, dS, rtol=1e-10, precondition=2, check_residual=False, **kw)
+            op.synchronize()
+            ts, st = [], None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                st = op.solve_device(dF, dS, rtol=1e-10, precondition=2, check_residual=False, **kw)
+                op.synchronize()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            op.profile(True)
+            op.profile_read()
+            op.apply(F, kw["precision"])
+            prof = op.profile_read()
+            op.profile(False)
+            row[name] = {"ms": min(ts), "iters": st["niter"], "refinements": st["nrefine"],
+                         "apply_ms": {k: v[1] / max(v[0], 1) for k, v in prof.items() if v[0]}}
+        row["speedup"] = row["fp64"]["ms"] / row["mixed"]["ms"]
+        log(f"requests: {row}")
+        rows.append(row)
+    print(json.dumps({"workload": "requests", "precision": a.precision, "rows": rows}), flush=True)
+
+
 def serve_workload(a, dbie, log):
     """Seeded Poisson stream of single-box requests (sides ~U(0.5, 2), l_ec in {0.1, 0.07, 0.05},
     n_quad 4, eps_in ~U(2, 80), 1-64 charges) through the request server, in mixed precision and in
@@ -323,9 +363,10 @@ def main():
     ap.add_argument("--shape", default="", help="apply mode: pin the low-precision far-field launch shape "
                                                 "'splits,variant' instead of tuning it")
     ap.add_argument("--blobs", type=int, default=1024, help="reaction workload: first B lattice blobs")
-    ap.add_argument("--workload", default="box", choices=["box", "reaction", "serve", "single"],
+    ap.add_argument("--workload", default="box", choices=["box", "reaction", "serve", "single", "requests"],
                     help="reaction: the volume-source reaction-matrix workload (1024 Gauss-Hermite blobs); "
                          "serve: a seeded Poisson stream of box requests through the request server; "
+                         "requests: serving-size requests solved alone, FP64 and mixed; "
                          "single: one point charge, eps_in 10, FP64 GMRES to 1e-10 (solve time)")
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=64.0, help="serve: arrival rate (requests/s; 64 saturates one MI355X)")
@@ -352,6 +393,9 @@ def main():
         return
     if a.workload == "single":
         single_workload(a, dbie, log)
+        return
+    if a.workload == "requests":
+        requests_workload(a, dbie, log)
         return
 
     t0 = time.perf_counter()
