@@ -585,6 +585,12 @@ bool excludes_near(const dbie_op *op, int K, dbie_precision prec) {
 // from the split-FP16 blocks (no FP32 copy of X needed)
 bool near_uses_h3(const dbie_op *op, int K) { return op->bsr.valh && op->bsr.mfma_order && op->bsr.bd == 16 && K >= 16; }
 
+// split-FP16 applies at K < 16 (FP32 far field without the near pairs): the near pairs' full blocks
+// multiply split-FP16 operands made for them (padded to 16 columns) where the apply has the scratch
+bool small_k_h3(const dbie_op *op, int K, dbie_precision prec) {
+    return K < 16 && excludes_near(op, K, prec) && op->bsr.valh && op->bsr.mfma_order && op->bsr.bd == 16;
+}
+
 TuneKey tune_key(const dbie_op *op, int K, dbie_precision prec, bool exclude) {
     return TuneKey{(int)prec, exclude ? 1 : 0, op->g.npp(), (K + 31) / 32,
                    (int)std::lround(4.0 * std::log2((double)std::max<int64_t>(op->N(), 1)))};
@@ -895,7 +901,9 @@ void apply_core(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, int
         // the full near blocks multiply the far field's split-FP16 operands of this X where they
         // were packed for it, else a tile-major FP32 copy of X
         NearH3 h3;
-        const bool use_h3 = exclude && near_uses_h3(op, K);
+        const bool small_h3 = ws.mscratch && small_k_h3(op, K, prec);
+        const bool use_h3 = exclude && (near_uses_h3(op, K) || small_h3);
+        if (small_h3) f16_split_operands(op->stream, N, K, X, ldx, op->dg.w, ws.mscratch);
         if (use_h3) far_f16_operands(ws.mscratch, N, K, &h3);
         near_combine(op->stream, op->bsr, nv, S, ws.part, N, K, X, ldx, op->diag.as<double>(), Y, ldy,
                      nv == NEAR_UP32 && !use_h3 ? ws.xt32 : nullptr, use_h3 ? &h3 : nullptr);
@@ -974,6 +982,9 @@ void apply_dev_cfg(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
             op->xt32.ensure(near_xt_bytes(N, K));
             ws.xt32 = op->xt32.as<float>();
         }
+    } else if (small_k_h3(op, K, prec)) {
+        op->mscratch.ensure(far_field_mfma_scratch_bytes(N, (int)DBIE_FP16X3, K));
+        ws.mscratch = op->mscratch.p;
     }
     apply_core(op, K, X, ldx, Y, ldy, prec, tune, ws);
 }
@@ -1156,7 +1167,7 @@ void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, in
     row_scale(st, pc.dinv, in, out, N, K);
     if (!pc.use_near) return;
     GmresWork &ws = op->gw;
-    if (pc.h3 && K >= 16) {
+    if (pc.h3) {  // any K: the split pads the operands to 16 columns
         // z <- -(1/t) .* (N z - t .* z0) with z0 = out: N z from z's split-FP16 operands (the three
         // products of the operator's near field), the scaling in the near kernel's epilogue; the last
         // sweep writes out in place (each element read and written by one thread)
@@ -2113,6 +2124,8 @@ size_t apply_noalloc_bytes(const dbie_op *op, int K, dbie_precision prec) {
     if (prec != DBIE_FP32 && far_field_mfma_supported((int)prec, K)) {
         b += up(far_field_mfma_scratch_bytes(N, (int)prec, K));
         if (excludes_near(op, K, prec) && !near_uses_h3(op, K)) b += up(near_xt_bytes(N, K));
+    } else if (small_k_h3(op, K, prec)) {
+        b += up(far_field_mfma_scratch_bytes(N, (int)DBIE_FP16X3, K));
     }
     return b + sizeof(double) * (size_t)std::max(K, 1) * (size_t)N;  // one split of partial sums
 }
@@ -2146,6 +2159,9 @@ void apply_noalloc(dbie_op *op, int K, const double *X, int64_t ldx, double *Y, 
             ws.xt32 = reinterpret_cast<float *>(base + off);
             off += up(near_xt_bytes(N, K));
         }
+    } else if (small_k_h3(op, K, prec)) {  // as dbie_apply_device: the near blocks' split operands
+        ws.mscratch = base + off;
+        off += up(far_field_mfma_scratch_bytes(N, (int)DBIE_FP16X3, K));
     }
     ws.part = reinterpret_cast<double *>(base + off);
     ws.part_cap = (int64_t)((bytes - off) / sizeof(double));  // >= K x N: at least one split fits

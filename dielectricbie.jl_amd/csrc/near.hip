@@ -560,7 +560,8 @@ void near_combine(hipStream_t st, const DevBSR &C, NearVals vals, int S, const d
     const int bd = C.bd;
     const float *v32 = vals == NEAR_UP32 ? C.valup32 : (vals == NEAR_C32 ? C.val32 : nullptr);
     const int nq = near_nq(K), Kpad = near_kpad(K), cw = 64 * nq;
-    if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 16 && h3 && C.valh) {
+    // (the split-FP16 operands cover Kp >= 16 columns, so any K >= 1 runs here when they are given)
+    if (vals == NEAR_UP32 && C.mfma_order && bd == 16 && K >= 1 && h3 && C.valh) {
         if (h3->tile != kZt) throw std::logic_error("near_combine: split-FP16 operand tiles of an unexpected height");
         const int per = (C.P + 7) / 8;
         dim3 grid((unsigned)(8 * per), (unsigned)(Kpad / cw));

@@ -43,7 +43,7 @@ def run_pair(dbie, op, X, prec, shape):
 
 
 @pytest.mark.parametrize("prec,Ks,shape", [("fp64", (24, 8), (2, -1)), ("fp16x3", (256, 40), (1, 1)),
-                                           ("fp32", (20, 3), (2, -1))])
+                                           ("fp16x3", (12, 3), (1, -1)), ("fp32", (20, 3), (2, -1))])
 def test_noalloc_apply_equals_apply_device(dbie, O, prec, Ks, shape):
     g = dbie.single_dielectric_box3d(1.2, 0.8, 0.6, 4, 0.2, 4.0, 1.0)
     op = dbie.batched_lhs_dielectric_box3d_fmm3d_corrected(g, 1e-9, 1e-9, 8)
