@@ -1157,14 +1157,46 @@ struct Precond {
     const double *w = nullptr;      // quadrature weights: the split-FP16 form multiplies w .* x
     bool h3 = false;                // near form on the split-FP16 blocks of the low-precision operator
     int sweeps = 1;                 // near form: Jacobi sweeps z <- z0 - D^-1 N z from z = z0 = D^-1 in
+    // fused operator: member b's own preconditioner on its rows and columns (the fused dinv unused)
+    std::vector<Precond> mem;
 };
 
-// out = M in (in and out may alias).  Near form: z0 = D^-1 in, then each sweep z <- z0 - D^-1 N z
-// (one sweep: the first Neumann term of (D + N)^-1; two: the second as well).
 template <typename TV>
-void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, int64_t N, int K) {
+void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, int64_t N, int K, int64_t ldi = 0,
+                   int64_t ldo = 0);
+
+// Fused operator: each member's preconditioner on its columns and first N_m rows (leading dimension the
+// fused N) on the member's stream, the rows past N_m set to zero; forked from and joined to the fused
+// stream as the fused apply is.
+template <typename TV>
+void apply_precond_fused(dbie_op *op, const Precond &pc, const TV *in, double *out, int64_t N) {
+    DBIE_HIP(hipEventRecord(op->fork_ev, op->stream));
+    for (size_t b = 0; b < op->members.size(); ++b) {
+        const dbie_op::Member &m = op->members[b];
+        DBIE_HIP(hipStreamWaitEvent(m.op->stream, op->fork_ev, 0));
+        const int64_t Nm = m.op->N();
+        apply_precond(m.op, pc.mem[b], in + (size_t)m.c0 * N, out + (size_t)m.c0 * N, Nm, m.K, N, N);
+        if (Nm < N)
+            DBIE_HIP(hipMemset2DAsync(out + (size_t)m.c0 * N + Nm, sizeof(double) * N, 0, sizeof(double) * (N - Nm),
+                                      m.K, m.op->stream));
+        DBIE_HIP(hipEventRecord(m.done, m.op->stream));
+        DBIE_HIP(hipStreamWaitEvent(op->stream, m.done, 0));
+    }
+}
+
+// out = M in (in and out may alias; leading dimensions ldi, ldo, 0: N).  Near form: z0 = D^-1 in, then
+// each sweep z <- z0 - D^-1 N z (one sweep: the first Neumann term of (D + N)^-1; two: the second as well).
+template <typename TV>
+void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, int64_t N, int K, int64_t ldi,
+                   int64_t ldo) {
+    if (!pc.mem.empty()) {
+        apply_precond_fused(op, pc, in, out, N);
+        return;
+    }
+    if (ldi <= 0) ldi = N;
+    if (ldo <= 0) ldo = N;
     hipStream_t st = op->stream;
-    row_scale(st, pc.dinv, in, out, N, K);
+    row_scale(st, pc.dinv, in, out, N, K, ldi, ldo);
     if (!pc.use_near) return;
     GmresWork &ws = op->gw;
     if (pc.h3) {  // any K: the split pads the operands to 16 columns
@@ -1176,11 +1208,15 @@ void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, in
         NearH3 h3;
         far_f16_operands(ws.PX.p, N, K, &h3);
         const double *z = out;
+        int64_t ldz = ldo;
         for (int s = 0; s < pc.sweeps; ++s) {
-            f16_split_operands(st, N, K, z, N, pc.w, ws.PX.p);
-            double *y = s + 1 < pc.sweeps ? ws.PY.as<double>() : out;
-            near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, y, N, nullptr, &h3, pc.ndinv);
+            f16_split_operands(st, N, K, z, ldz, pc.w, ws.PX.p);
+            const bool last = s + 1 == pc.sweeps;
+            double *y = last ? out : ws.PY.as<double>();
+            const int64_t ldy = last ? ldo : N;
+            near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, ldo, pc.negt, y, ldy, nullptr, &h3, pc.ndinv);
             z = y;
+            ldz = ldy;
         }
         return;
     }
@@ -1191,17 +1227,17 @@ void apply_precond(dbie_op *op, const Precond &pc, const TV *in, double *out, in
         xt = ws.PX.as<float>();
     }
     if (pc.sweeps == 1) {
-        near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, ws.PY.as<double>(), N, xt, nullptr);
-        row_scale(st, pc.ndinv, ws.PY.as<double>(), out, N, K);
+        near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, ldo, pc.negt, ws.PY.as<double>(), N, xt, nullptr);
+        row_scale(st, pc.ndinv, ws.PY.as<double>(), out, N, K, N, ldo);
         return;
     }
     // two sweeps: z1 = z0 - D^-1 N z0 into PZ, then out = z0 - D^-1 N z1 = z0 - z1 - D^-1 (N z1 - t .* z1)
     ws.PZ.ensure(sizeof(double) * (size_t)N * K);
     double *z1 = ws.PZ.as<double>(), *py = ws.PY.as<double>();
-    near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, N, pc.negt, py, N, xt, nullptr);
+    near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, out, ldo, pc.negt, py, N, xt, nullptr);
     row_scale(st, pc.ndinv, py, z1, N, K);
     near_combine(st, pc.near, pc.vals, 0, nullptr, N, K, z1, N, pc.negt, py, N, xt, nullptr);
-    jacobi_update(st, pc.ndinv, py, z1, out, N, K);
+    jacobi_update(st, pc.ndinv, py, z1, out, N, K, ldo);
 }
 
 // GMRES(m) for K independent columns sharing each block matvec.  Right
@@ -1820,52 +1856,65 @@ dbie_status solve_dev(dbie_op *op, int K, const double *dF, double *dX, const db
     const size_t NK = (size_t)N * K;
     DBuf dR(true), dD(true), dT(true), dinv(true), partial(true), hb(true);
     // preconditioners: pc64 for FP64 solves, pcl for the low-precision inner solves (the near form
-    // multiplies FP64 or FP32 full blocks; a fused operator has no blocks of its own: diagonal only)
+    // multiplies FP64 or FP32 / split-FP16 full blocks).  A fused operator takes each member's own
+    // near-field preconditioner on the member's rows and columns, or (diagonal scaling) its unit diagonal
     Precond pc64, pcl;
     const Precond *p64 = nullptr, *plo = nullptr;
-    if (o.precondition) {
-        const bool near = o.precondition >= 2 && op->members.empty() && op->bsr.nb > 0;
-        std::vector<double> inv((size_t)N * (near ? 3 : 1));
-        for (int64_t i = 0; i < N; ++i) {
-            inv[i] = 1.0 / op->diag_h[i];
+    std::vector<DBuf> mdinv(op->members.size());
+    // the preconditioners of one (non-fused) operator o2, its inverse diagonals in buf
+    auto build = [&](dbie_op *o2, DBuf &buf, Precond &q64, Precond &ql) {
+        const int64_t n2 = o2->N();
+        const bool near = o.precondition >= 2 && o2->members.empty() && o2->bsr.nb > 0;
+        std::vector<double> inv((size_t)n2 * (near ? 3 : 1));
+        for (int64_t i = 0; i < n2; ++i) {
+            inv[i] = 1.0 / o2->diag_h[i];
             if (near) {
-                inv[N + i] = -inv[i];
-                inv[2 * N + i] = -op->diag_h[i];
+                inv[n2 + i] = -inv[i];
+                inv[2 * n2 + i] = -o2->diag_h[i];
             }
         }
-        upload(dinv, inv.data(), inv.size(), st);
-        pc64.dinv = dinv.as<double>();
+        upload(buf, inv.data(), inv.size(), st);
+        q64.dinv = buf.as<double>();
         if (near) {
-            pc64.ndinv = dinv.as<double>() + N;
-            pc64.negt = dinv.as<double>() + 2 * N;
+            q64.ndinv = buf.as<double>() + n2;
+            q64.negt = buf.as<double>() + 2 * n2;
         }
-        pc64.sweeps = o.precondition == 3 ? 2 : 1;
-        pcl = pc64;
+        q64.sweeps = o.precondition == 3 ? 2 : 1;
+        ql = q64;
         if (near && o.precision == DBIE_FP64) {
-            if (!op->full64_built) {
-                op->bsr_full64.ensure(sizeof(double) * (size_t)op->bsr.nb * op->bsr.bd * op->bsr.bd);
-                near_full_blocks(st, op->dg, op->bsr, op->bsr_full64.as<double>());
+            if (!o2->full64_built) {
+                o2->bsr_full64.ensure(sizeof(double) * (size_t)o2->bsr.nb * o2->bsr.bd * o2->bsr.bd);
+                near_full_blocks(st, o2->dg, o2->bsr, o2->bsr_full64.as<double>());
                 DBIE_HIP(hipGetLastError());
-                op->full64_built = true;
+                o2->full64_built = true;
             }
-            pc64.near = op->bsr;
-            pc64.near.val64 = op->bsr_full64.as<double>();
-            pc64.near.val32 = pc64.near.valup32 = nullptr;
-            pc64.near.valh = nullptr;
-            pc64.near.sr_ptr = nullptr;
-            pc64.vals = NEAR_C64;
-            pc64.use_near = true;
+            q64.near = o2->bsr;
+            q64.near.val64 = o2->bsr_full64.as<double>();
+            q64.near.val32 = q64.near.valup32 = nullptr;
+            q64.near.valh = nullptr;
+            q64.near.sr_ptr = nullptr;
+            q64.vals = NEAR_C64;
+            q64.use_near = true;
         }
-        if (near && o.precision != DBIE_FP64 && op->bsr.valup32) {
-            pcl.near = op->bsr;
-            pcl.h3 = op->bsr.valh && op->bsr.mfma_order && op->bsr.bd == 16;
-            pcl.w = op->w.as<double>();
-            if (!pcl.h3) {
-                pcl.near.valh = nullptr;
-                pcl.near.sr_ptr = nullptr;
+        if (near && o.precision != DBIE_FP64 && o2->bsr.valup32) {
+            ql.near = o2->bsr;
+            ql.h3 = o2->bsr.valh && o2->bsr.mfma_order && o2->bsr.bd == 16;
+            ql.w = o2->w.as<double>();
+            if (!ql.h3) {
+                ql.near.valh = nullptr;
+                ql.near.sr_ptr = nullptr;
             }
-            pcl.vals = NEAR_UP32;
-            pcl.use_near = true;
+            ql.vals = NEAR_UP32;
+            ql.use_near = true;
+        }
+    };
+    if (o.precondition) {
+        build(op, dinv, pc64, pcl);
+        if (o.precondition >= 2 && !op->members.empty()) {
+            pc64.mem.resize(op->members.size());
+            pcl.mem.resize(op->members.size());
+            for (size_t b = 0; b < op->members.size(); ++b)
+                build(op->members[b].op, mdinv[b], pc64.mem[b], pcl.mem[b]);
         }
         p64 = &pc64;
         plo = &pcl;

@@ -185,14 +185,18 @@ void col_normalize(hipStream_t st, const double *in, const double *sq, float *ou
 // (a, om, coef: nb x K, element (i, k) at i K + k)
 void deflate_coef(hipStream_t st, const double *a, const double *om, const int *n1, int nb, int K, double *coef);
 // out[:,k] = d .* in[:,k]  (d per row; d null: out = in)
-void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K);
-void row_scale(hipStream_t st, const double *d, const float *in, double *out, int64_t N, int K);
+// leading dimensions ldi / ldo (0: N)
+void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K, int64_t ldi = 0,
+               int64_t ldo = 0);
+void row_scale(hipStream_t st, const double *d, const float *in, double *out, int64_t N, int K, int64_t ldi = 0,
+               int64_t ldo = 0);
 // X[:,k] += sum_i V_i[:,k] y[i*K+k]
 void block_combine(hipStream_t st, const double *V, int nv, const double *y, double *X, int64_t N, int K);
 void block_combine(hipStream_t st, const float *V, int nv, const double *y, double *X, int64_t N, int K);
 // R = F - R (residual from AX stored in R), column-wise
-// out = out - z + s .* py (row scale s)
-void jacobi_update(hipStream_t st, const double *s, const double *py, const double *z, double *out, int64_t N, int K);
+// out = out - z + s .* py (row scale s; py and z N x K contiguous, out with leading dimension ldo, 0: N)
+void jacobi_update(hipStream_t st, const double *s, const double *py, const double *z, double *out, int64_t N, int K,
+                   int64_t ldo = 0);
 void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K);
 // X += D
 void add_to(hipStream_t st, const double *D, double *X, int64_t N, int K);

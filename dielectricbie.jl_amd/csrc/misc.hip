@@ -280,19 +280,20 @@ __global__ void deflate_coef_kernel(const double *__restrict__ a, const double *
 
 // d may be null: out = in (a conversion to FP64)
 template <typename TI>
-__global__ void row_scale_kernel(const double *__restrict__ d, const TI *__restrict__ in, double *__restrict__ out,
-                                 int64_t N, int K) {
+__global__ void row_scale_kernel(const double *__restrict__ d, const TI *__restrict__ in, int64_t ldi,
+                                 double *__restrict__ out, int64_t ldo, int64_t N, int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    if (n < N) out[(int64_t)k * N + n] = (d ? d[n] : 1.0) * (double)in[(int64_t)k * N + n];
+    if (n < N) out[(int64_t)k * ldo + n] = (d ? d[n] : 1.0) * (double)in[(int64_t)k * ldi + n];
 }
 
 // out = out - z + s .* py (the second Jacobi sweep of the near-field preconditioner)
 __global__ void jacobi_update_kernel(const double *__restrict__ s, const double *__restrict__ py,
-                                     const double *__restrict__ z, double *__restrict__ out, int64_t N, int K) {
+                                     const double *__restrict__ z, double *__restrict__ out, int64_t ldo, int64_t N,
+                                     int K) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t idx = (int64_t)blockIdx.y * N + n;
-    if (n < N) out[idx] = out[idx] - z[idx] + s[n] * py[idx];
+    const int64_t idx = (int64_t)blockIdx.y * N + n, io = (int64_t)blockIdx.y * ldo + n;
+    if (n < N) out[io] = out[io] - z[idx] + s[n] * py[idx];
 }
 
 __global__ void sub_from_kernel(const double *__restrict__ F, double *__restrict__ R, int64_t N, int K) {
@@ -439,15 +440,18 @@ void deflate_coef(hipStream_t st, const double *a, const double *om, const int *
     deflate_coef_kernel<<<(unsigned)((K + 63) / 64), 64, 0, st>>>(a, om, n1, nb, K, coef);
 }
 
-void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K) {
-    row_scale_kernel<double><<<grid2(N, K), 256, 0, st>>>(d, in, out, N, K);
+void row_scale(hipStream_t st, const double *d, const double *in, double *out, int64_t N, int K, int64_t ldi,
+               int64_t ldo) {
+    row_scale_kernel<double><<<grid2(N, K), 256, 0, st>>>(d, in, ldi > 0 ? ldi : N, out, ldo > 0 ? ldo : N, N, K);
 }
-void row_scale(hipStream_t st, const double *d, const float *in, double *out, int64_t N, int K) {
-    row_scale_kernel<float><<<grid2(N, K), 256, 0, st>>>(d, in, out, N, K);
+void row_scale(hipStream_t st, const double *d, const float *in, double *out, int64_t N, int K, int64_t ldi,
+               int64_t ldo) {
+    row_scale_kernel<float><<<grid2(N, K), 256, 0, st>>>(d, in, ldi > 0 ? ldi : N, out, ldo > 0 ? ldo : N, N, K);
 }
 
-void jacobi_update(hipStream_t st, const double *s, const double *py, const double *z, double *out, int64_t N, int K) {
-    jacobi_update_kernel<<<grid2(N, K), 256, 0, st>>>(s, py, z, out, N, K);
+void jacobi_update(hipStream_t st, const double *s, const double *py, const double *z, double *out, int64_t N, int K,
+                   int64_t ldo) {
+    jacobi_update_kernel<<<grid2(N, K), 256, 0, st>>>(s, py, z, out, ldo > 0 ? ldo : N, N, K);
 }
 
 void sub_from(hipStream_t st, const double *F, double *R, int64_t N, int K) {

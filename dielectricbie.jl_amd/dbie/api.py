@@ -531,7 +531,7 @@ class BatchedDielectricOperator:
 
 def solve_fused(ops, Fs, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precision="fp64", inner_rtol=1e-5,
                 max_refine=10, check_residual=True, raise_on_fail=True, krylov="columns", recycle=True,
-                verify=True):
+                verify=True, precondition=True):
     """Solve the block systems A_b Σ_b = F_b of several operators (different geometries) as one Krylov
     process (dbie_solve_fused): every iteration applies all operators at once on the GPU.  Returns
     ([Σ_b], stats)."""
@@ -553,6 +553,7 @@ def solve_fused(ops, Fs, *, rtol=1e-10, atol=0.0, itmax=500, restart=0, precisio
     o.krylov = _krylov_code(krylov)
     o.recycle = int(bool(recycle))
     o.verify = int(bool(verify))
+    o.precondition = _precond_code(precondition)  # 2 / 3: each member's near-field preconditioner
     st = L.SolveStats()
     status = L.lib().dbie_solve_fused(B, ha, L.iptr(Ka), Fa, L.i64ptr(ld), Sa, L.i64ptr(ld), C.byref(o), C.byref(st))
     if not (status == L.NOT_CONVERGED and not raise_on_fail):

@@ -186,11 +186,13 @@ def test_admission_uses_full_width(dbie):
         assert srv.result(rid).shape == (N, 40)
 
 
-@pytest.mark.parametrize("precision,krylov", [("fp64", "columns"), ("fp16x3", "columns"), ("fp16x3", "block")])
-def test_fused_solve_equals_alone(dbie, precision, krylov):
+@pytest.mark.parametrize("precision,krylov,pre", [("fp64", "columns", 1), ("fp16x3", "columns", 1), ("fp16x3", "block", 1),
+                                                 ("fp64", "columns", 2), ("fp16x3", "columns", 2), ("fp16x3", "columns", 3)])
+def test_fused_solve_equals_alone(dbie, precision, krylov, pre):
     """dbie_solve_fused: three boxes of different geometry (N 5k-13k, 1 to 12 columns) in one Krylov
     process give each box's own solution (FP64 within 1e-10, mixed within 1e-8 per column); block GMRES
-    is refused (its block orthogonalisation would mix columns of different operators)."""
+    is refused (its block orthogonalisation would mix columns of different operators).  pre = 2 / 3: each
+    member's near-field preconditioner on its own rows and columns, in fewer iterations than pre = 1."""
     boxes = [((0.9, 0.7, 0.6, 4, 0.1, 10.0, 1.0), 5), ((0.6, 0.6, 0.6, 4, 0.07, 40.0, 1.0), 1),
              ((1.3, 0.7, 0.9, 4, 0.1, 2.5, 1.0), 12)]
     say(f"fused solve of 3 geometries ({precision}, {krylov})")
@@ -209,9 +211,13 @@ def test_fused_solve_equals_alone(dbie, precision, krylov):
         with pytest.raises(ValueError, match="krylov"):
             dbie.solve_fused(ops, Fs, rtol=rtol, precision=precision, krylov=krylov)
         return
-    Ss, st = dbie.solve_fused(ops, Fs, rtol=rtol, precision=precision, krylov=krylov)
+    Ss, st = dbie.solve_fused(ops, Fs, rtol=rtol, precision=precision, krylov=krylov, precondition=pre)
     say(f"  stats {st}")
     assert st["solved"]
+    if pre >= 2:
+        _, st1 = dbie.solve_fused(ops, Fs, rtol=rtol, precision=precision, krylov=krylov, precondition=1)
+        say(f"  iterations: diagonal scaling {st1['niter']}, near field {st['niter']}")
+        assert st["niter"] < st1["niter"]
     bound = 1e-10 if precision == "fp64" else 1e-8
     for S, R in zip(Ss, alone):
         for k in range(S.shape[1]):
