@@ -64,12 +64,12 @@ __global__ __launch_bounds__(kTile) void far_f64_kernel(DevGeom g, int K, const 
             const double4 sp = s_pos[jj];
             const double dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
-            // 1/sqrt(r2): the FP32 hardware estimate (relative error below 2e-7 for r2 in the normal
-            // FP32 range, which node distances are) and two Newton steps y (3 - r2 y^2) / 2 in FP64,
-            // each squaring the relative error
+            // 1/sqrt(r2): the FP32 hardware estimate y (relative error below 2e-7 for r2 in the normal
+            // FP32 range, which node distances are) and one third-order (Halley) step in FP64,
+            // y (1 + e / 2 + 3 e^2 / 8) with e = 1 - r2 y^2, which cubes the relative error
             double rinv = (double)__builtin_amdgcn_rsqf((float)r2);
-            rinv = rinv * fma(-0.5 * r2, rinv * rinv, 1.5);
-            rinv = rinv * fma(-0.5 * r2, rinv * rinv, 1.5);
+            const double e = fma(-r2, rinv * rinv, 1.0);
+            rinv = fma(rinv, e * fma(0.375, e, 0.5), rinv);
             rinv = r2 > 0.0 ? rinv : 0.0;
             double gk;
             if (MODE == FAR_DT) {
