@@ -278,9 +278,11 @@ dbie_status dbie_op_residual_history(const dbie_op *op, int cap, int *n, double 
    leading dimension ldf[b]) and receives Sigma[b]; the columns are held in N_max x sum(K) blocks (rows
    past a member's N are zero), and every iteration applies all members at once, each on its own stream
    with its own tuned launch shape and HIP graphs, so that small systems fill the GPU together and share
-   one host synchronisation per iteration.  Each operator may appear once (merge its columns).  The
-   right preconditioner is a scalar per member for single-contrast interfaces and is left out (GMRES is
-   invariant to it); multi-contrast members are solved unpreconditioned.  opts and stats as for
+   one host synchronisation per iteration.  Each operator may appear once (merge its columns).  With
+   opts->precondition = 1 the right preconditioner (a scalar per member for single-contrast interfaces)
+   is left out, GMRES being invariant to it, and multi-contrast members are solved unpreconditioned;
+   with 2 or 3 each member's own near-field preconditioner acts on the member's rows and columns, on the
+   member's stream.  opts and stats as for
    dbie_solve_block (one stats record for the whole solve), except that opts->krylov must be 0: block
    GMRES orthogonalises the columns against each other, which would mix columns of different operators
    (DBIE_BAD_ARG otherwise). */
